@@ -1,0 +1,11 @@
+"""eqtl_amd — MI355X (gfx950) association engine for eqtl (trans LM/LMM scans,
+REML, GRM, cis permutations) behind a C ABI (include/eqtl_gpu.h).
+
+The compute path is libeqtl_gpu.so (hand-written HIP kernels); this package
+holds its ctypes binding, the in-tree build, and the synthetic-input generator.
+"""
+from ._lib import (Context, EqtlGpuError, HIT_DTYPE, GENE_STATE_DTYPE, MODEL_LM, MODEL_LMM, PREC_FP32,
+                   PREC_BF16X3, DELTA_REF, DELTA_NEWTON, DELTA_GIVEN, LIB_PATH)
+
+__all__ = ["Context", "EqtlGpuError", "HIT_DTYPE", "GENE_STATE_DTYPE", "MODEL_LM", "MODEL_LMM", "PREC_FP32",
+           "PREC_BF16X3", "DELTA_REF", "DELTA_NEWTON", "DELTA_GIVEN", "LIB_PATH"]

@@ -1,0 +1,856 @@
+// C-ABI implementation of include/eqtl_gpu.h: context, device buffers,
+// launches. Built for gfx950 only (one translation unit).
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/eqtl_gpu.h"
+#include "numerics.hpp"
+#include "reml_kernels.hpp"
+#include "scan_kernels.hpp"
+#include "grm_kernels.hpp"
+
+using namespace eqtl_amd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPC(expr)                                                                         \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(EQTL_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));          \
+  } while (0)
+
+#define CHK(expr)            \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != EQTL_OK) return r_; \
+  } while (0)
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+int ensure(DBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return EQTL_OK;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+  if (bytes == 0) bytes = 256;
+  if (hipMalloc(&b.p, bytes) != hipSuccess) {
+    b.p = nullptr;
+    (void)hipGetLastError();
+    return fail(EQTL_E_NOMEM, "hipMalloc(" + std::to_string(bytes) + ") failed");
+  }
+  b.bytes = bytes;
+  return EQTL_OK;
+}
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+struct TimingSlot {
+  double ms = 0;
+  int64_t launches = 0;
+};
+
+}  // namespace
+
+struct eqtl_gpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  int n_cu = 0;
+  std::string name;
+  int64_t hbm = 0;
+  // samples / policy
+  int n_file = 0, n = 0, n_pad = 0;
+  DBuf sample_col;
+  int impute = 1;
+  double max_miss = 0.8, maf_min = 0.0;
+  // snp annotation
+  int64_t m_annot = 0;
+  DBuf snp_chrom, snp_pos;
+  // basis
+  bool have_basis = false;
+  DBuf Q64, lam64, Q32, Qhi, Qlo;
+  // genes
+  int G = 0, G_pad = 0, p = 0, prepped_model = -1, nq = 0;
+  DBuf Yt64, Xt64, X64, Y64, delta, gscal, gvalid, B32, Bhi, Blo, Wx, zeros;
+  bool have_windows = false;
+  DBuf gene_chrom, gene_cs, gene_ce;
+  // scan
+  bool scanning = false;
+  int scan_model = 0, scan_prec = 0;
+  double pthr = 1e-5;
+  DBuf state, partial, hits, hit_count, tail;
+  double tail_df = -1;
+  int64_t hit_cap_dev = 0;
+  // per-block buffers
+  int64_t m_cap = 0;
+  DBuf rows, A32, A64, Gt, keep, af, gtg, dosage;
+  // GRM
+  DBuf grm_acc, grm_miss, grm_cnt, grm_z;
+  int64_t grm_m = 0;
+  // synthetic bed buffers
+  std::vector<DBuf*> synth;
+  // timing
+  bool timing = false;
+  std::map<std::string, TimingSlot> tslots;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  ~eqtl_gpu_ctx() {
+    for (DBuf* b : synth) delete b;
+    if (blas) rocblas_destroy_handle(blas);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Bracket one kernel-family launch with events when timing is on.
+struct Timed {
+  eqtl_gpu_ctx* c;
+  const char* fam;
+  Timed(eqtl_gpu_ctx* ctx, const char* f) : c(ctx), fam(f) {
+    if (c->timing) (void)hipEventRecord(c->ev0, c->stream);
+  }
+  int done() {
+    if (!c->timing) return EQTL_OK;
+    HIPC(hipEventRecord(c->ev1, c->stream));
+    HIPC(hipEventSynchronize(c->ev1));
+    float ms = 0;
+    HIPC(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    auto& s = c->tslots[fam];
+    s.ms += ms;
+    s.launches += 1;
+    return EQTL_OK;
+  }
+};
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(EQTL_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return EQTL_OK;
+}
+
+int set_device(eqtl_gpu_ctx* c) {
+  HIPC(hipSetDevice(c->device));
+  return EQTL_OK;
+}
+
+// Block buffers for up to m SNPs.
+int ensure_block(eqtl_gpu_ctx* c, int64_t m, int32_t bps, bool want64) {
+  const int64_t m_pad = round_up(std::max<int64_t>(m, 1), 64);
+  CHK(ensure(c->rows, static_cast<size_t>(m_pad) * bps));
+  CHK(ensure(c->A32, static_cast<size_t>(m_pad) * c->n_pad * sizeof(float)));
+  CHK(ensure(c->Gt, static_cast<size_t>(m_pad) * c->n_pad * sizeof(float)));
+  CHK(ensure(c->keep, static_cast<size_t>(m_pad)));
+  CHK(ensure(c->af, static_cast<size_t>(m_pad) * sizeof(double)));
+  CHK(ensure(c->gtg, static_cast<size_t>(m_pad) * sizeof(double)));
+  if (want64) CHK(ensure(c->A64, static_cast<size_t>(m_pad) * c->n_pad * sizeof(double)));
+  return EQTL_OK;
+}
+
+int run_decode(eqtl_gpu_ctx* c, const uint8_t* rows_dev, int64_t m, int32_t bps, bool want64,
+               double* dosage_dev) {
+  const int64_t m_pad = round_up(std::max<int64_t>(m, 1), 64);
+  DecodeArgs a;
+  a.rows = rows_dev;
+  a.m = m;
+  a.m_pad = m_pad;
+  a.bps = bps;
+  a.n = c->n;
+  a.n_file = c->n_file;
+  a.n_pad = c->n_pad;
+  a.sample_col = c->sample_col.as<int32_t>();
+  a.impute = c->impute;
+  a.max_miss = c->max_miss;
+  a.maf_min = c->maf_min;
+  a.A = c->A32.as<float>();
+  a.A64 = want64 ? c->A64.as<double>() : nullptr;
+  a.dosage = dosage_dev;
+  a.keep = c->keep.as<uint8_t>();
+  a.af = c->af.as<double>();
+  Timed t(c, "decode");
+  hipLaunchKernelGGL(k_decode, dim3(static_cast<unsigned>((m_pad + 3) / 4)), dim3(256), 0, c->stream, a);
+  CHK(check_launch("k_decode"));
+  return t.done();
+}
+
+const uint8_t* stage_rows(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, int64_t m, int32_t bps,
+                          int* rc) {
+  *rc = EQTL_OK;
+  if (on_dev) return rows;
+  if (hipMemcpyAsync(c->rows.p, rows, static_cast<size_t>(m) * bps, hipMemcpyHostToDevice, c->stream) !=
+      hipSuccess) {
+    *rc = fail(EQTL_E_HIP, "copy of bed rows failed");
+    return nullptr;
+  }
+  return c->rows.as<uint8_t>();
+}
+
+template <int PREC>
+int launch_wald_lmm(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
+  const int nq = c->nq;
+#define EQ_WALD(NQV)                                                                                 \
+  hipLaunchKernelGGL((k_wald<EQTL_MODEL_LMM, PREC, NQV>), grid, dim3(256), 0, c->stream, a);         \
+  return check_launch("k_wald");
+  if (nq <= 4) { EQ_WALD(4) }
+  if (nq <= 8) { EQ_WALD(8) }
+  if (nq <= 12) { EQ_WALD(12) }
+  if (nq <= 16) { EQ_WALD(16) }
+  if (nq <= 20) { EQ_WALD(20) }
+  if (nq <= 24) { EQ_WALD(24) }
+  if (nq <= 28) { EQ_WALD(28) }
+  if (nq <= 34) { EQ_WALD(34) }
+#undef EQ_WALD
+  return fail(EQTL_E_ARG, "too many covariates for the Wald kernel");
+}
+
+int launch_wald(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
+  if (c->scan_model == EQTL_MODEL_LMM) {
+    if (c->scan_prec == EQTL_PREC_FP32) return launch_wald_lmm<EQTL_PREC_FP32>(c, a, grid);
+    return launch_wald_lmm<EQTL_PREC_BF16X3>(c, a, grid);
+  }
+  if (c->scan_prec == EQTL_PREC_FP32)
+    hipLaunchKernelGGL((k_wald<EQTL_MODEL_LM, EQTL_PREC_FP32, 1>), grid, dim3(256), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL((k_wald<EQTL_MODEL_LM, EQTL_PREC_BF16X3, 1>), grid, dim3(256), 0, c->stream, a);
+  return check_launch("k_wald");
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char* eqtl_gpu_last_error(void) { return g_err.c_str(); }
+
+int eqtl_gpu_open(int32_t device, eqtl_gpu_ctx** out) {
+  if (!out) return fail(EQTL_E_ARG, "null out");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(EQTL_E_HIP, "no HIP device");
+  if (device < 0 || device >= ndev) return fail(EQTL_E_ARG, "bad device index");
+  auto* c = new eqtl_gpu_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return fail(EQTL_E_HIP, "stream/event creation failed");
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    c->n_cu = prop.multiProcessorCount;
+    c->name = prop.gcnArchName;
+    c->hbm = static_cast<int64_t>(prop.totalGlobalMem);
+  }
+  *out = c;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_close(eqtl_gpu_ctx* c) {
+  if (!c) return EQTL_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  delete c;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_device_info(eqtl_gpu_ctx* c, char* name, int32_t name_len, int32_t* n_cu, int64_t* hbm) {
+  if (!c) return fail(EQTL_E_ARG, "null ctx");
+  if (name && name_len > 0) {
+    std::snprintf(name, static_cast<size_t>(name_len), "%s", c->name.c_str());
+  }
+  if (n_cu) *n_cu = c->n_cu;
+  if (hbm) *hbm = c->hbm;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_synchronize(eqtl_gpu_ctx* c) {
+  if (!c) return fail(EQTL_E_ARG, "null ctx");
+  HIPC(hipStreamSynchronize(c->stream));
+  return EQTL_OK;
+}
+
+int eqtl_gpu_set_samples(eqtl_gpu_ctx* c, int32_t n_file, const int32_t* sample_col, int32_t n) {
+  if (!c || !sample_col || n <= 0 || n_file <= 0) return fail(EQTL_E_ARG, "bad samples");
+  for (int i = 0; i < n; ++i)
+    if (sample_col[i] < 0 || sample_col[i] >= n_file) return fail(EQTL_E_ARG, "sample column out of range");
+  CHK(set_device(c));
+  if (c->n != n) {
+    c->have_basis = false;
+    c->prepped_model = -1;
+  }
+  c->n_file = n_file;
+  c->n = n;
+  c->n_pad = static_cast<int>(round_up(n, 64));
+  CHK(ensure(c->sample_col, sizeof(int32_t) * n));
+  HIPC(hipMemcpyAsync(c->sample_col.p, sample_col, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return EQTL_OK;
+}
+
+int eqtl_gpu_set_policy(eqtl_gpu_ctx* c, int32_t impute, double max_miss, double maf_min) {
+  if (!c) return fail(EQTL_E_ARG, "null ctx");
+  c->impute = impute ? 1 : 0;
+  c->max_miss = max_miss;
+  c->maf_min = maf_min;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_set_snp_annot(eqtl_gpu_ctx* c, const int32_t* chrom, const int64_t* pos, int64_t m) {
+  if (!c || !chrom || !pos || m <= 0) return fail(EQTL_E_ARG, "bad snp annotation");
+  CHK(set_device(c));
+  CHK(ensure(c->snp_chrom, sizeof(int32_t) * m));
+  CHK(ensure(c->snp_pos, sizeof(int64_t) * m));
+  HIPC(hipMemcpyAsync(c->snp_chrom.p, chrom, sizeof(int32_t) * m, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->snp_pos.p, pos, sizeof(int64_t) * m, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  c->m_annot = m;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_decode_bed(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, int64_t m, int32_t bps,
+                        double* dosage, uint8_t* keep, double* af) {
+  if (!c || !rows || m <= 0) return fail(EQTL_E_ARG, "bad decode arguments");
+  if (c->n <= 0) return fail(EQTL_E_STATE, "set_samples first");
+  if (bps < (c->n_file + 3) / 4) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
+  CHK(set_device(c));
+  CHK(ensure_block(c, m, bps, false));
+  if (dosage) CHK(ensure(c->dosage, sizeof(double) * static_cast<size_t>(m) * c->n));
+  int rc;
+  const uint8_t* rd = stage_rows(c, rows, on_dev, m, bps, &rc);
+  if (rc) return rc;
+  CHK(run_decode(c, rd, m, bps, false, dosage ? c->dosage.as<double>() : nullptr));
+  if (dosage)
+    HIPC(hipMemcpyAsync(dosage, c->dosage.p, sizeof(double) * static_cast<size_t>(m) * c->n,
+                        hipMemcpyDeviceToHost, c->stream));
+  if (keep) HIPC(hipMemcpyAsync(keep, c->keep.p, static_cast<size_t>(m), hipMemcpyDeviceToHost, c->stream));
+  if (af) HIPC(hipMemcpyAsync(af, c->af.p, sizeof(double) * m, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return EQTL_OK;
+}
+
+// ---- basis ----------------------------------------------------------------
+static int upload_basis(eqtl_gpu_ctx* c, int n) {
+  // Q32/Qhi/Qlo coordinate-major padded copies for the rotation kernel
+  const int np = c->n_pad;
+  std::vector<double> Q(static_cast<size_t>(n) * n);
+  HIPC(hipMemcpyAsync(Q.data(), c->Q64.p, sizeof(double) * Q.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  std::vector<float> q32(static_cast<size_t>(np) * np, 0.f);
+  std::vector<__bf16> qh(static_cast<size_t>(np) * np), ql(static_cast<size_t>(np) * np);
+  for (int cc = 0; cc < n; ++cc)
+    for (int i = 0; i < n; ++i) q32[static_cast<size_t>(cc) * np + i] = static_cast<float>(Q[static_cast<size_t>(cc) * n + i]);
+  for (size_t t = 0; t < q32.size(); ++t) {
+    const __bf16 h = static_cast<__bf16>(q32[t]);
+    qh[t] = h;
+    ql[t] = static_cast<__bf16>(q32[t] - static_cast<float>(h));
+  }
+  CHK(ensure(c->Q32, sizeof(float) * q32.size()));
+  CHK(ensure(c->Qhi, sizeof(__bf16) * qh.size()));
+  CHK(ensure(c->Qlo, sizeof(__bf16) * ql.size()));
+  HIPC(hipMemcpyAsync(c->Q32.p, q32.data(), sizeof(float) * q32.size(), hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->Qhi.p, qh.data(), sizeof(__bf16) * qh.size(), hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->Qlo.p, ql.data(), sizeof(__bf16) * ql.size(), hipMemcpyHostToDevice, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  c->have_basis = true;
+  c->prepped_model = -1;
+  return EQTL_OK;
+}
+
+static int ensure_blas(eqtl_gpu_ctx* c) {
+  if (c->blas) return EQTL_OK;
+  if (rocblas_create_handle(&c->blas) != rocblas_status_success) return fail(EQTL_E_HIP, "rocblas_create_handle failed");
+  if (rocblas_set_stream(c->blas, c->stream) != rocblas_status_success) return fail(EQTL_E_HIP, "rocblas_set_stream failed");
+  return EQTL_OK;
+}
+
+int eqtl_gpu_eigh(eqtl_gpu_ctx* c, const double* K, int32_t n, double* Q_out, double* lam_out) {
+  if (!c || !K || n <= 0) return fail(EQTL_E_ARG, "bad eigh arguments");
+  if (c->n != n) return fail(EQTL_E_ARG, "GRM size differs from the analysis sample count");
+  CHK(set_device(c));
+  CHK(ensure_blas(c));
+  CHK(ensure(c->Q64, sizeof(double) * n * n));
+  CHK(ensure(c->lam64, sizeof(double) * n));
+  DBuf E, info;
+  CHK(ensure(E, sizeof(double) * n));
+  CHK(ensure(info, sizeof(int)));
+  HIPC(hipMemcpyAsync(c->Q64.p, K, sizeof(double) * n * n, hipMemcpyHostToDevice, c->stream));
+  if (rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_lower, n, c->Q64.as<double>(), n,
+                       c->lam64.as<double>(), E.as<double>(), info.as<int>()) != rocblas_status_success)
+    return fail(EQTL_E_SOLVER, "rocsolver_dsyevd failed");
+  int hinfo = 0;
+  HIPC(hipMemcpyAsync(&hinfo, info.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> lam(n);
+  HIPC(hipMemcpyAsync(lam.data(), c->lam64.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  if (hinfo != 0) return fail(EQTL_E_SOLVER, "GRM eigen decomposition failed");
+  for (auto& v : lam) v = std::max(v, 0.0);  // cwiseMax(0), model_lmm.cpp:111
+  HIPC(hipMemcpyAsync(c->lam64.p, lam.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if (Q_out) HIPC(hipMemcpyAsync(Q_out, c->Q64.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->stream));
+  if (lam_out) std::memcpy(lam_out, lam.data(), sizeof(double) * n);
+  HIPC(hipStreamSynchronize(c->stream));
+  return upload_basis(c, n);
+}
+
+int eqtl_gpu_set_basis(eqtl_gpu_ctx* c, const double* Q, const double* lam, int32_t n) {
+  if (!c || !Q || !lam || n <= 0) return fail(EQTL_E_ARG, "bad basis");
+  if (c->n != n) return fail(EQTL_E_ARG, "basis size differs from the analysis sample count");
+  CHK(set_device(c));
+  CHK(ensure(c->Q64, sizeof(double) * n * n));
+  CHK(ensure(c->lam64, sizeof(double) * n));
+  std::vector<double> l(lam, lam + n);
+  for (auto& v : l) v = std::max(v, 0.0);
+  HIPC(hipMemcpyAsync(c->Q64.p, Q, sizeof(double) * n * n, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->lam64.p, l.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return upload_basis(c, n);
+}
+
+// ---- gene preparation -------------------------------------------------------
+static int alloc_gene_side(eqtl_gpu_ctx* c, int G, int nq) {
+  c->G = G;
+  c->G_pad = static_cast<int>(round_up(G, 128));  // multiple of 8 groups of 16
+  c->nq = nq;
+  const size_t cols = static_cast<size_t>(c->G_pad / 16) * nq * 16 * c->n_pad;
+  CHK(ensure(c->B32, cols * sizeof(float)));
+  CHK(ensure(c->Bhi, cols * sizeof(__bf16)));
+  CHK(ensure(c->Blo, cols * sizeof(__bf16)));
+  HIPC(hipMemsetAsync(c->B32.p, 0, cols * sizeof(float), c->stream));
+  HIPC(hipMemsetAsync(c->Bhi.p, 0, cols * sizeof(__bf16), c->stream));
+  HIPC(hipMemsetAsync(c->Blo.p, 0, cols * sizeof(__bf16), c->stream));
+  CHK(ensure(c->gscal, sizeof(double) * c->G_pad));
+  CHK(ensure(c->gvalid, sizeof(int) * c->G_pad));
+  HIPC(hipMemsetAsync(c->gvalid.p, 0, sizeof(int) * c->G_pad, c->stream));
+  CHK(ensure(c->delta, sizeof(double) * c->G_pad));
+  return EQTL_OK;
+}
+
+int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double* X, int32_t p,
+                      int32_t mode, const double* delta_in, double* delta_out, double* rss_out) {
+  if (!c || !Y || !X || G <= 0 || p <= 0) return fail(EQTL_E_ARG, "bad lmm_prep arguments");
+  if (p > kPMax) return fail(EQTL_E_ARG, "more than 32 covariate columns are not supported");
+  if (!c->have_basis) return fail(EQTL_E_STATE, "no eigenbasis: call eqtl_gpu_eigh or eqtl_gpu_set_basis");
+  if (mode == EQTL_DELTA_GIVEN && !delta_in) return fail(EQTL_E_ARG, "delta_in required");
+  const int n = c->n;
+  CHK(set_device(c));
+  CHK(ensure_blas(c));
+  c->p = p;
+  CHK(alloc_gene_side(c, G, p + 2));
+  CHK(ensure(c->Y64, sizeof(double) * static_cast<size_t>(n) * G));
+  CHK(ensure(c->X64, sizeof(double) * static_cast<size_t>(n) * p));
+  CHK(ensure(c->Yt64, sizeof(double) * static_cast<size_t>(n) * G));
+  CHK(ensure(c->Xt64, sizeof(double) * static_cast<size_t>(n) * p));
+  HIPC(hipMemcpyAsync(c->Y64.p, Y, sizeof(double) * static_cast<size_t>(n) * G, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->X64.p, X, sizeof(double) * static_cast<size_t>(n) * p, hipMemcpyHostToDevice, c->stream));
+  // K4: Y~ = Q^T Y, X~ = Q^T X once for all genes (FP64 library GEMM)
+  const double one = 1.0, zero = 0.0;
+  {
+    Timed t(c, "rotate_yx");
+    if (rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, n, G, n, &one,
+                      c->Q64.as<double>(), n, c->Y64.as<double>(), n, &zero, c->Yt64.as<double>(), n) !=
+            rocblas_status_success ||
+        rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, n, p, n, &one,
+                      c->Q64.as<double>(), n, c->X64.as<double>(), n, &zero, c->Xt64.as<double>(), n) !=
+            rocblas_status_success)
+      return fail(EQTL_E_HIP, "rocblas_dgemm failed");
+    CHK(t.done());
+  }
+  if (mode == EQTL_DELTA_GIVEN) {
+    HIPC(hipMemcpyAsync(c->delta.p, delta_in, sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
+  } else {
+    Timed t(c, "reml");
+    hipLaunchKernelGGL(k_reml, dim3(G), dim3(256), 0, c->stream, c->Xt64.as<double>(), c->Yt64.as<double>(),
+                       c->lam64.as<double>(), n, p, G, mode == EQTL_DELTA_NEWTON ? 1 : 0, c->delta.as<double>());
+    CHK(check_launch("k_reml"));
+    CHK(t.done());
+  }
+  {
+    Timed t(c, "basis");
+    hipLaunchKernelGGL(k_lmm_basis, dim3(G), dim3(256), 0, c->stream, c->Xt64.as<double>(), c->Yt64.as<double>(),
+                       c->lam64.as<double>(), n, c->n_pad, p, G, c->delta.as<double>(), c->B32.as<float>(),
+                       c->Bhi.as<__bf16>(), c->Blo.as<__bf16>(), c->gscal.as<double>(), c->gvalid.as<int>());
+    CHK(check_launch("k_lmm_basis"));
+    CHK(t.done());
+  }
+  std::vector<int> valid(G);
+  HIPC(hipMemcpyAsync(valid.data(), c->gvalid.p, sizeof(int) * G, hipMemcpyDeviceToHost, c->stream));
+  if (delta_out) HIPC(hipMemcpyAsync(delta_out, c->delta.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  if (rss_out) HIPC(hipMemcpyAsync(rss_out, c->gscal.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < G; ++k)
+    if (!valid[k]) return fail(EQTL_E_ARG, "whitened covariate matrix is not positive definite (gene " + std::to_string(k) + ")");
+  c->prepped_model = EQTL_MODEL_LMM;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_reml_grad_hess(eqtl_gpu_ctx* c, const double* log_delta, int32_t G, double* f, double* g,
+                            double* h) {
+  if (!c || !log_delta || !f || !g || !h) return fail(EQTL_E_ARG, "bad reml_grad_hess arguments");
+  if (c->prepped_model != EQTL_MODEL_LMM || G != c->G) return fail(EQTL_E_STATE, "call eqtl_gpu_lmm_prep first");
+  CHK(set_device(c));
+  DBuf ld, F, Gd, H;
+  CHK(ensure(ld, sizeof(double) * G));
+  CHK(ensure(F, sizeof(double) * G));
+  CHK(ensure(Gd, sizeof(double) * G));
+  CHK(ensure(H, sizeof(double) * G));
+  HIPC(hipMemcpyAsync(ld.p, log_delta, sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_reml_grad_hess, dim3(G), dim3(256), 0, c->stream, c->Xt64.as<double>(), c->Yt64.as<double>(),
+                     c->lam64.as<double>(), c->n, c->p, G, ld.as<double>(), F.as<double>(), Gd.as<double>(),
+                     H.as<double>());
+  CHK(check_launch("k_reml_grad_hess"));
+  HIPC(hipMemcpyAsync(f, F.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipMemcpyAsync(g, Gd.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipMemcpyAsync(h, H.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return EQTL_OK;
+}
+
+int eqtl_gpu_lm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double* X, int32_t p, double* yty_out) {
+  if (!c || !Y || !X || G <= 0 || p <= 0) return fail(EQTL_E_ARG, "bad lm_prep arguments");
+  if (p > kPMax) return fail(EQTL_E_ARG, "more than 32 covariate columns are not supported");
+  if (c->n <= 0) return fail(EQTL_E_STATE, "set_samples first");
+  const int n = c->n;
+  CHK(set_device(c));
+  c->p = p;
+  CHK(alloc_gene_side(c, G, 1));
+  CHK(ensure(c->Y64, sizeof(double) * static_cast<size_t>(n) * G));
+  CHK(ensure(c->X64, sizeof(double) * static_cast<size_t>(n) * p));
+  CHK(ensure(c->Wx, sizeof(double) * static_cast<size_t>(n) * p));
+  CHK(ensure(c->zeros, sizeof(double) * n));
+  HIPC(hipMemsetAsync(c->zeros.p, 0, sizeof(double) * n, c->stream));
+  HIPC(hipMemcpyAsync(c->Y64.p, Y, sizeof(double) * static_cast<size_t>(n) * G, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->X64.p, X, sizeof(double) * static_cast<size_t>(n) * p, hipMemcpyHostToDevice, c->stream));
+  DBuf okb;
+  CHK(ensure(okb, sizeof(int)));
+  hipLaunchKernelGGL(k_lm_basis, dim3(1), dim3(256), 0, c->stream, c->X64.as<double>(), c->zeros.as<double>(), n, p,
+                     c->Wx.as<double>(), okb.as<int>());
+  CHK(check_launch("k_lm_basis"));
+  int ok = 0;
+  HIPC(hipMemcpyAsync(&ok, okb.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  if (!ok) return fail(EQTL_E_ARG, "covariate matrix X^T X is not positive definite");
+  hipLaunchKernelGGL(k_lm_gene, dim3(G), dim3(256), 0, c->stream, c->Y64.as<double>(), c->Wx.as<double>(), n,
+                     c->n_pad, p, G, c->B32.as<float>(), c->Bhi.as<__bf16>(), c->Blo.as<__bf16>(),
+                     c->gscal.as<double>(), c->gvalid.as<int>());
+  CHK(check_launch("k_lm_gene"));
+  if (yty_out) HIPC(hipMemcpyAsync(yty_out, c->gscal.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  c->prepped_model = EQTL_MODEL_LM;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_set_gene_windows(eqtl_gpu_ctx* c, const int32_t* chrom, const int64_t* cs, const int64_t* ce,
+                              int32_t G) {
+  if (!c || !chrom || !cs || !ce || G <= 0) return fail(EQTL_E_ARG, "bad gene windows");
+  CHK(set_device(c));
+  CHK(ensure(c->gene_chrom, sizeof(int32_t) * G));
+  CHK(ensure(c->gene_cs, sizeof(int64_t) * G));
+  CHK(ensure(c->gene_ce, sizeof(int64_t) * G));
+  HIPC(hipMemcpyAsync(c->gene_chrom.p, chrom, sizeof(int32_t) * G, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->gene_cs.p, cs, sizeof(int64_t) * G, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->gene_ce.p, ce, sizeof(int64_t) * G, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  c->have_windows = true;
+  return EQTL_OK;
+}
+
+// ---- scans ------------------------------------------------------------------
+int eqtl_gpu_scan_begin(eqtl_gpu_ctx* c, int32_t model, int32_t prec, double pthr) {
+  if (!c) return fail(EQTL_E_ARG, "null ctx");
+  if (model != EQTL_MODEL_LM && model != EQTL_MODEL_LMM) return fail(EQTL_E_ARG, "bad model");
+  if (prec != EQTL_PREC_FP32 && prec != EQTL_PREC_BF16X3) return fail(EQTL_E_ARG, "bad precision");
+  if (c->prepped_model != model) return fail(EQTL_E_STATE, "gene side not prepared for this model");
+  if (c->have_windows && c->m_annot <= 0) return fail(EQTL_E_STATE, "gene windows need the SNP annotation");
+  CHK(set_device(c));
+  c->scan_model = model;
+  c->scan_prec = prec;
+  c->pthr = pthr;
+  CHK(ensure(c->state, sizeof(GeneAcc) * c->G_pad));
+  hipLaunchKernelGGL(k_state_init, dim3((c->G_pad + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
+                     c->G_pad);
+  CHK(check_launch("k_state_init"));
+  CHK(ensure(c->hit_count, sizeof(unsigned long long)));
+  // t-tail table for this scan's residual df (shared by every test)
+  const double df = static_cast<double>(c->n - c->p - 1);
+  if (df > 0 && c->tail_df != df) {
+    std::vector<TTail> T(1);
+    const double lbeta = std::lgamma(0.5 * df) + std::lgamma(0.5) - std::lgamma(0.5 * df + 0.5);
+    tail_build(T[0], df, lbeta);
+    CHK(ensure(c->tail, sizeof(TTail)));
+    HIPC(hipMemcpyAsync(c->tail.p, T.data(), sizeof(TTail), hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    c->tail_df = df;
+  } else if (df <= 0) {
+    CHK(ensure(c->tail, sizeof(TTail)));
+    HIPC(hipMemsetAsync(c->tail.p, 0, sizeof(TTail), c->stream));
+    c->tail_df = df;
+  }
+  c->scanning = true;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, int64_t snp0, int64_t m,
+                        int32_t bps, eqtl_hit* hits, int64_t hit_cap, int64_t* n_hits) {
+  if (!c || !rows || m <= 0 || snp0 < 0) return fail(EQTL_E_ARG, "bad scan_block arguments");
+  if (!c->scanning) return fail(EQTL_E_STATE, "scan_begin first");
+  if (bps < (c->n_file + 3) / 4) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
+  if (c->have_windows && snp0 + m > c->m_annot) return fail(EQTL_E_ARG, "SNP block beyond the annotation");
+  if (hit_cap > 0 && !hits) return fail(EQTL_E_ARG, "hits buffer is null");
+  CHK(set_device(c));
+  const bool lm = c->scan_model == EQTL_MODEL_LM;
+  CHK(ensure_block(c, m, bps, lm));
+  int rc;
+  const uint8_t* rd = stage_rows(c, rows, on_dev, m, bps, &rc);
+  if (rc) return rc;
+  CHK(run_decode(c, rd, m, bps, lm, nullptr));
+  const int64_t m_pad = round_up(m, 64);
+  const int np = c->n_pad;
+  const float* Aop = c->A32.as<float>();
+  if (!lm) {
+    Timed t(c, "rotate");
+    dim3 grid(static_cast<unsigned>(np / 64), static_cast<unsigned>(m_pad / 64));
+    if (c->scan_prec == EQTL_PREC_FP32)
+      hipLaunchKernelGGL(k_rotate<EQTL_PREC_FP32>, grid, dim3(256), 0, c->stream, c->A32.as<float>(),
+                         c->Q32.as<float>(), c->Qhi.as<__bf16>(), c->Qlo.as<__bf16>(), c->Gt.as<float>(), np);
+    else
+      hipLaunchKernelGGL(k_rotate<EQTL_PREC_BF16X3>, grid, dim3(256), 0, c->stream, c->A32.as<float>(),
+                         c->Q32.as<float>(), c->Qhi.as<__bf16>(), c->Qlo.as<__bf16>(), c->Gt.as<float>(), np);
+    CHK(check_launch("k_rotate"));
+    CHK(t.done());
+    Aop = c->Gt.as<float>();
+  } else {
+    hipLaunchKernelGGL(k_lm_gtg, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream,
+                       c->A64.as<double>(), m_pad, c->n, np, c->Wx.as<double>(), c->p, c->gtg.as<double>());
+    CHK(check_launch("k_lm_gtg"));
+  }
+  // work split: enough workgroups to fill the chip, SNP tiles strided per split
+  const int n_groups = c->G_pad / 16;
+  const int n_groups_pad = static_cast<int>(round_up(n_groups, 8));
+  const int n_tiles = static_cast<int>(m_pad / 64);
+  int n_splits = std::max(1, std::min(n_tiles, (4 * std::max(c->n_cu, 1) * 2 + n_groups - 1) / n_groups));
+  CHK(ensure(c->partial, sizeof(GeneAcc) * static_cast<size_t>(n_splits) * c->G_pad));
+  const int64_t cap_dev = std::max<int64_t>(hit_cap, 1);
+  if (c->hit_cap_dev < cap_dev) {
+    CHK(ensure(c->hits, sizeof(eqtl_hit) * static_cast<size_t>(cap_dev)));
+    c->hit_cap_dev = cap_dev;
+  }
+  HIPC(hipMemsetAsync(c->hit_count.p, 0, sizeof(unsigned long long), c->stream));
+  WaldArgs a;
+  a.A = Aop;
+  a.B32 = c->B32.as<float>();
+  a.Bhi = c->Bhi.as<__bf16>();
+  a.Blo = c->Blo.as<__bf16>();
+  a.n_pad = np;
+  a.nq = c->nq;
+  a.G = c->G;
+  a.G_pad = c->G_pad;
+  a.n_groups = n_groups;
+  a.n_groups_pad = n_groups_pad;
+  a.n_splits = n_splits;
+  a.n_tiles = n_tiles;
+  a.m = m;
+  a.snp0 = snp0;
+  a.keep = c->keep.as<uint8_t>();
+  a.af = c->af.as<double>();
+  a.gtg = c->gtg.as<double>();
+  a.gene_scal = c->gscal.as<double>();
+  a.gene_valid = c->gvalid.as<int32_t>();
+  a.snp_chrom = c->m_annot > 0 ? c->snp_chrom.as<int32_t>() : nullptr;
+  a.snp_pos = c->m_annot > 0 ? c->snp_pos.as<int64_t>() : nullptr;
+  a.gene_chrom = c->have_windows ? c->gene_chrom.as<int32_t>() : nullptr;
+  a.gene_cs = c->have_windows ? c->gene_cs.as<int64_t>() : nullptr;
+  a.gene_ce = c->have_windows ? c->gene_ce.as<int64_t>() : nullptr;
+  a.df = static_cast<double>(c->n - c->p - 1);
+  a.tail = c->tail.as<TTail>();
+  a.pthr = c->pthr;
+  a.partial = c->partial.as<GeneAcc>();
+  a.hits = c->hits.as<eqtl_hit>();
+  a.hit_count = c->hit_count.as<unsigned long long>();
+  a.hit_cap = cap_dev;
+  {
+    Timed t(c, "wald");
+    CHK(launch_wald(c, a, dim3(static_cast<unsigned>(n_groups_pad * n_splits))));
+    CHK(t.done());
+  }
+  hipLaunchKernelGGL(k_merge, dim3((c->G + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
+                     c->partial.as<GeneAcc>(), c->G, c->G_pad, n_splits);
+  CHK(check_launch("k_merge"));
+  unsigned long long cnt = 0;
+  HIPC(hipMemcpyAsync(&cnt, c->hit_count.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  const int64_t got = static_cast<int64_t>(cnt);
+  if (n_hits) *n_hits = got;
+  const int64_t take = std::min<int64_t>(got, hit_cap);
+  if (take > 0) {
+    HIPC(hipMemcpyAsync(hits, c->hits.p, sizeof(eqtl_hit) * take, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    std::sort(hits, hits + take, [](const eqtl_hit& x, const eqtl_hit& y) {
+      return x.snp != y.snp ? x.snp < y.snp : x.gene < y.gene;
+    });
+  }
+  if (got > hit_cap) return fail(EQTL_E_CAPACITY, "hit buffer too small for this block");
+  return EQTL_OK;
+}
+
+int eqtl_gpu_scan_end(eqtl_gpu_ctx* c, eqtl_gene_state* out) {
+  if (!c || !out) return fail(EQTL_E_ARG, "bad scan_end arguments");
+  if (!c->scanning) return fail(EQTL_E_STATE, "no scan in progress");
+  CHK(set_device(c));
+  DBuf o;
+  CHK(ensure(o, sizeof(eqtl_gene_state) * c->G));
+  hipLaunchKernelGGL(k_finalize, dim3((c->G + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
+                     o.as<eqtl_gene_state>(), c->G);
+  CHK(check_launch("k_finalize"));
+  HIPC(hipMemcpyAsync(out, o.p, sizeof(eqtl_gene_state) * c->G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  c->scanning = false;
+  return EQTL_OK;
+}
+
+// ---- GRM ----------------------------------------------------------------------
+int eqtl_gpu_grm_begin(eqtl_gpu_ctx* c) {
+  if (!c) return fail(EQTL_E_ARG, "null ctx");
+  if (c->n <= 0) return fail(EQTL_E_STATE, "set_samples first");
+  CHK(set_device(c));
+  const size_t nn = static_cast<size_t>(c->n_pad) * c->n_pad;
+  CHK(ensure(c->grm_acc, nn * sizeof(double)));
+  CHK(ensure(c->grm_miss, nn * sizeof(int32_t)));
+  CHK(ensure(c->grm_cnt, static_cast<size_t>(c->n_pad) * sizeof(int32_t)));
+  HIPC(hipMemsetAsync(c->grm_acc.p, 0, nn * sizeof(double), c->stream));
+  HIPC(hipMemsetAsync(c->grm_miss.p, 0, nn * sizeof(int32_t), c->stream));
+  HIPC(hipMemsetAsync(c->grm_cnt.p, 0, static_cast<size_t>(c->n_pad) * sizeof(int32_t), c->stream));
+  c->grm_m = 0;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, int64_t m, int32_t bps) {
+  if (!c || !rows || m <= 0) return fail(EQTL_E_ARG, "bad grm_accumulate arguments");
+  if (!c->grm_acc.p) return fail(EQTL_E_STATE, "grm_begin first");
+  if (bps < (c->n_file + 3) / 4) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
+  CHK(set_device(c));
+  CHK(ensure_block(c, m, bps, false));
+  const int64_t m_pad = round_up(m, 64);
+  const int np = c->n_pad;
+  CHK(ensure(c->grm_z, static_cast<size_t>(m_pad) * np * 3 * sizeof(__bf16)));
+  int rc;
+  const uint8_t* rd = stage_rows(c, rows, on_dev, m, bps, &rc);
+  if (rc) return rc;
+  __bf16* zhi = c->grm_z.as<__bf16>();
+  __bf16* zlo = zhi + static_cast<size_t>(m_pad) * np;
+  __bf16* mis = zlo + static_cast<size_t>(m_pad) * np;
+  DBuf used;
+  CHK(ensure(used, sizeof(unsigned long long)));
+  HIPC(hipMemsetAsync(used.p, 0, sizeof(unsigned long long), c->stream));
+  hipLaunchKernelGGL(k_grm_standardize, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream, rd, m, m_pad,
+                     bps, c->n, np, c->n_file, c->sample_col.as<int32_t>(), c->keep.as<uint8_t>(), zhi, zlo, mis,
+                     c->grm_cnt.as<int32_t>(), used.as<unsigned long long>(), c->impute, c->max_miss, c->maf_min);
+  CHK(check_launch("k_grm_standardize"));
+  {
+    Timed t(c, "grm");
+    dim3 grid(static_cast<unsigned>(np / 64), static_cast<unsigned>(np / 64));
+    hipLaunchKernelGGL(k_grm_gram, grid, dim3(256), 0, c->stream, zhi, zlo, mis, static_cast<int>(m_pad), np,
+                       c->grm_acc.as<double>(), c->grm_miss.as<int32_t>());
+    CHK(check_launch("k_grm_gram"));
+    CHK(t.done());
+  }
+  unsigned long long u = 0;
+  HIPC(hipMemcpyAsync(&u, used.p, sizeof(u), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  c->grm_m += static_cast<int64_t>(u);
+  return EQTL_OK;
+}
+
+int eqtl_gpu_grm_finish(eqtl_gpu_ctx* c, double* K, int32_t* N_packed, int64_t* m_used) {
+  if (!c) return fail(EQTL_E_ARG, "null ctx");
+  if (!c->grm_acc.p) return fail(EQTL_E_STATE, "grm_begin first");
+  if (c->grm_m == 0) return fail(EQTL_E_ARG, "no SNPs for GRM");
+  CHK(set_device(c));
+  const int n = c->n, np = c->n_pad;
+  DBuf Kd, Nd;
+  CHK(ensure(Kd, sizeof(double) * static_cast<size_t>(n) * n));
+  CHK(ensure(Nd, sizeof(int32_t) * static_cast<size_t>(n) * (n + 1) / 2));
+  hipLaunchKernelGGL(k_grm_finish, dim3((n + 15) / 16, (n + 15) / 16), dim3(16, 16), 0, c->stream,
+                     c->grm_acc.as<double>(), c->grm_miss.as<int32_t>(), c->grm_cnt.as<int32_t>(), n, np,
+                     static_cast<int>(c->grm_m), Kd.as<double>(), Nd.as<int32_t>());
+  CHK(check_launch("k_grm_finish"));
+  if (K) HIPC(hipMemcpyAsync(K, Kd.p, sizeof(double) * static_cast<size_t>(n) * n, hipMemcpyDeviceToHost, c->stream));
+  if (N_packed)
+    HIPC(hipMemcpyAsync(N_packed, Nd.p, sizeof(int32_t) * static_cast<size_t>(n) * (n + 1) / 2, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  if (m_used) *m_used = c->grm_m;
+  return EQTL_OK;
+}
+
+// ---- instrumentation / synthetic inputs ----------------------------------------
+int eqtl_gpu_set_timing(eqtl_gpu_ctx* c, int32_t enable) {
+  if (!c) return fail(EQTL_E_ARG, "null ctx");
+  c->timing = enable != 0;
+  if (!c->timing) c->tslots.clear();
+  return EQTL_OK;
+}
+
+int eqtl_gpu_get_timing(eqtl_gpu_ctx* c, const char* fam, double* ms, int64_t* launches) {
+  if (!c || !fam) return fail(EQTL_E_ARG, "bad get_timing arguments");
+  auto it = c->tslots.find(fam);
+  if (ms) *ms = it == c->tslots.end() ? 0.0 : it->second.ms;
+  if (launches) *launches = it == c->tslots.end() ? 0 : it->second.launches;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_synth_bed(eqtl_gpu_ctx* c, uint64_t seed, int64_t m, int32_t n, double maf_lo, double maf_hi,
+                       double miss, uint8_t** rows_dev) {
+  if (!c || !rows_dev || m <= 0 || n <= 0) return fail(EQTL_E_ARG, "bad synth_bed arguments");
+  CHK(set_device(c));
+  const int bps = (n + 3) / 4;
+  auto* b = new DBuf();
+  const int r = ensure(*b, static_cast<size_t>(m) * bps);
+  if (r) {
+    delete b;
+    return r;
+  }
+  c->synth.push_back(b);
+  const int64_t total = m * bps;
+  hipLaunchKernelGGL(k_synth_bed, dim3(static_cast<unsigned>((total + 255) / 256)), dim3(256), 0, c->stream,
+                     b->as<uint8_t>(), m, n, bps, seed, maf_lo, maf_hi, miss);
+  CHK(check_launch("k_synth_bed"));
+  HIPC(hipStreamSynchronize(c->stream));
+  *rows_dev = b->as<uint8_t>();
+  return EQTL_OK;
+}
+
+int eqtl_gpu_copy_rows(eqtl_gpu_ctx* c, const uint8_t* rows_dev, int64_t j0, int64_t m, int32_t bps, uint8_t* out) {
+  if (!c || !rows_dev || !out || m <= 0 || j0 < 0) return fail(EQTL_E_ARG, "bad copy_rows arguments");
+  CHK(set_device(c));
+  HIPC(hipMemcpyAsync(out, rows_dev + j0 * bps, static_cast<size_t>(m) * bps, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return EQTL_OK;
+}
+
+}  // extern "C"

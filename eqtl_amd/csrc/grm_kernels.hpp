@@ -1,0 +1,150 @@
+// K2: GRM (compute_grm_from, grm.cpp:112-224).
+//   k_grm_standardize — decode each SNP row, apply the GRM's SNP rules
+//       (decode_row filters, then p = mean/2 in [1e-8, 1-1e-8], sd >= 1e-12),
+//       write z = (g - mean)/sd (missing → 0) split into bf16 hi/lo and the
+//       0/1 missing indicator, sample-major (SNP-contiguous) for the MFMA
+//       operands; count per-sample missing calls of used SNPs.
+//   k_grm_gram — lower block-triangle of Z^T Z (3 bf16 MFMA products, fp32
+//       within 256-SNP chunks, fp64 across chunks) and of M^T M (joint
+//       missing counts, exact), accumulated into FP64 / int32 tiles.
+//   k_grm_finish — per-pair GCTA divisor m - miss_i - miss_j + joint_ij
+//       (diagonal m - miss_i), symmetric K and packed N.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mfma.hpp"
+
+namespace eqtl_amd {
+
+__global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, int64_t m, int64_t m_pad, int bps,
+                                                         int n, int n_pad, int n_file, const int32_t* sample_col,
+                                                         const uint8_t* /*unused*/, __bf16* zhi, __bf16* zlo,
+                                                         __bf16* mis, int32_t* miss_cnt,
+                                                         unsigned long long* used, int impute, double max_miss,
+                                                         double maf_min) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (j >= m_pad) return;
+  bool use = j < m;
+  int n_miss = 0, sum = 0;
+  const uint8_t* row = use ? rows + j * bps : nullptr;
+  if (use) {
+    for (int i = lane; i < n; i += 64) {
+      const int col = sample_col[i];
+      const int code = (row[col >> 2] >> ((col & 3) * 2)) & 3;
+      if (code == 1) { ++n_miss; continue; }
+      sum += code == 0 ? 2 : (code == 2 ? 1 : 0);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      n_miss += __shfl_xor(n_miss, o);
+      sum += __shfl_xor(sum, o);
+    }
+  }
+  double mu = 0, mean = 0, sd = 1;
+  if (use) {
+    const int n_ok = n - n_miss;
+    if (n_ok == 0) use = false;
+    if (use && static_cast<double>(n_miss) / n > max_miss + 1e-15) use = false;
+    if (use && !impute && n_miss > 0) use = false;
+    if (use) {
+      mu = static_cast<double>(sum) / n_ok;
+      double af = fmin(1.0, fmax(0.0, mu / 2.0));
+      if (af < 1e-12 || af > 1.0 - 1e-12) use = false;
+      if (maf_min > 0.0 && (af < maf_min || af > 1.0 - maf_min)) use = false;
+    }
+    if (use) {
+      mean = (sum + n_miss * mu) / n;
+      const double pf = mean / 2.0;
+      if (pf < 1e-8 || pf > 1.0 - 1e-8) use = false;
+      sd = sqrt(2.0 * pf * (1.0 - pf));
+      if (sd < 1e-12) use = false;
+    }
+  }
+  for (int i = lane; i < n_pad; i += 64) {
+    float z = 0.f, mi = 0.f;
+    if (use && i < n) {
+      const int col = sample_col[i];
+      const int code = (row[col >> 2] >> ((col & 3) * 2)) & 3;
+      if (code == 1) {
+        mi = 1.f;
+        atomicAdd(&miss_cnt[i], 1);
+      } else {
+        const double g = code == 0 ? 2.0 : (code == 2 ? 1.0 : 0.0);
+        z = static_cast<float>((g - mean) / sd);
+      }
+    }
+    const size_t o = static_cast<size_t>(i) * m_pad + j;
+    const __bf16 h = static_cast<__bf16>(z);
+    zhi[o] = h;
+    zlo[o] = static_cast<__bf16>(z - static_cast<float>(h));
+    mis[o] = static_cast<__bf16>(mi);
+  }
+  if (use && lane == 0) atomicAdd(used, 1ull);
+}
+
+// One workgroup per 64x64 tile (by >= bx, lower block triangle).
+__global__ __launch_bounds__(256) void k_grm_gram(const __bf16* zhi, const __bf16* zlo, const __bf16* mis,
+                                                  int m_pad, int n_pad, double* acc, int32_t* joint) {
+  const int bx = blockIdx.x, by = blockIdx.y;
+  if (bx > by) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int r0 = by * 64 + wave * 16;  // sample rows i
+  const int c0 = bx * 64;              // sample cols k
+  const int kk = 8 * (lane >> 4);
+  double a64[4][4];
+  float cm[4][4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { a64[cb][r] = 0; cm[cb][r] = 0; }
+  const size_t arow = static_cast<size_t>(r0 + (lane & 15)) * m_pad + kk;
+  for (int c0k = 0; c0k < m_pad; c0k += 256) {
+    f32x4 acc32[4], accm[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) { acc32[cb] = f32x4{0, 0, 0, 0}; accm[cb] = f32x4{0, 0, 0, 0}; }
+    const int kend = min(m_pad, c0k + 256);
+    for (int k0 = c0k; k0 < kend; k0 += 32) {
+      const bf16x8 ah = load8_bf16(zhi + arow + k0), al = load8_bf16(zlo + arow + k0);
+      const bf16x8 am = load8_bf16(mis + arow + k0);
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const size_t bo = static_cast<size_t>(c0 + cb * 16 + (lane & 15)) * m_pad + kk + k0;
+        const bf16x8 bh = load8_bf16(zhi + bo), bl = load8_bf16(zlo + bo), bm = load8_bf16(mis + bo);
+        acc32[cb] = mfma_bf16(ah, bh, acc32[cb]);
+        acc32[cb] = mfma_bf16(ah, bl, acc32[cb]);
+        acc32[cb] = mfma_bf16(al, bh, acc32[cb]);
+        accm[cb] = mfma_bf16(am, bm, accm[cb]);
+      }
+    }
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) { a64[cb][r] += acc32[cb][r]; cm[cb][r] += accm[cb][r]; }
+  }
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const size_t o = static_cast<size_t>(r0 + 4 * (lane >> 4) + r) * n_pad + c0 + cb * 16 + (lane & 15);
+      acc[o] += a64[cb][r];
+      joint[o] += static_cast<int32_t>(cm[cb][r]);
+    }
+}
+
+__global__ void k_grm_finish(const double* acc, const int32_t* joint, const int32_t* miss_cnt, int n, int n_pad,
+                             int m, double* K, int32_t* Npk) {
+  const int i = blockIdx.y * 16 + threadIdx.y;
+  const int j = blockIdx.x * 16 + threadIdx.x;
+  if (i >= n || j >= n || j > i) return;
+  const size_t o = static_cast<size_t>(i) * n_pad + j;  // i >= j: stored in the lower block triangle
+  int subN;
+  if (i == j) subN = m - miss_cnt[i];
+  else subN = m - miss_cnt[i] - miss_cnt[j] + joint[o];
+  const double v = subN > 0 ? acc[o] / static_cast<double>(subN) : 0.0;
+  K[static_cast<size_t>(j) * n + i] = v;
+  K[static_cast<size_t>(i) * n + j] = v;
+  Npk[static_cast<size_t>(i) * (i + 1) / 2 + j] = subN;
+}
+
+}  // namespace eqtl_amd

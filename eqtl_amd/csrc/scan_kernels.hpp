@@ -1,0 +1,484 @@
+// Genotype-side kernels of the trans scans:
+//   K1 k_decode   — 2-bit PLINK rows → centred, mean-imputed dosages + per-SNP
+//                   filters/stats (plink_bed.cpp:263-320, scan_common.cpp:133-151)
+//   K6 k_rotate   — G~ = (G - mu)^T Q on MFMA (scan_trans_lmm.cpp:168)
+//   K7/K8 k_wald  — fused contraction against the per-gene column blocks +
+//                   FP64 Wald epilogue + per-gene reductions + hit emission
+//                   (scan_trans_lmm.cpp:11-40,170-188; scan_trans_lm.cpp:21-36,
+//                   161-177; scan_trans.hpp:53-110)
+//   k_merge / k_finalize — fold per-split gene partials, finish ACAT.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mfma.hpp"
+#include "numerics.hpp"
+#include "../../include/eqtl_gpu.h"
+
+namespace eqtl_amd {
+
+constexpr int kWave = 64;
+
+// Per-gene running summary (device side of eqtl_gene_state).
+struct GeneAcc {
+  double acat_hi, acat_lo;
+  double best_p;
+  int64_t best_snp;
+  double best_af, best_beta, best_se, best_stat, best_r2;
+  int32_t used, n_tested, n_sig, pad;
+};
+
+__host__ __device__ inline void acc_init(GeneAcc& a) {
+  a.acat_hi = 0; a.acat_lo = 0; a.best_p = 2.0; a.best_snp = -1;
+  a.best_af = a.best_beta = a.best_se = a.best_stat = a.best_r2 = 0;
+  a.used = a.n_tested = a.n_sig = 0; a.pad = 0;
+}
+
+// (p, snp) lexicographic minimum reproduces "strict < in SNP order" for best.
+__host__ __device__ inline void acc_merge(GeneAcc& a, const GeneAcc& b) {
+  const dd s = dd_add(dd{a.acat_hi, a.acat_lo}, dd{b.acat_hi, b.acat_lo});
+  a.acat_hi = s.hi; a.acat_lo = s.lo;
+  a.used += b.used; a.n_tested += b.n_tested; a.n_sig += b.n_sig;
+  if (b.best_snp >= 0 && (b.best_p < a.best_p || (b.best_p == a.best_p && (a.best_snp < 0 || b.best_snp < a.best_snp)))) {
+    a.best_p = b.best_p; a.best_snp = b.best_snp; a.best_af = b.best_af; a.best_beta = b.best_beta;
+    a.best_se = b.best_se; a.best_stat = b.best_stat; a.best_r2 = b.best_r2;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K1: decode. One wave per SNP row. Rows >= m (padding) are written as zeros.
+// ---------------------------------------------------------------------------
+struct DecodeArgs {
+  const uint8_t* rows;
+  int64_t m, m_pad;
+  int32_t bps, n, n_file, n_pad;
+  const int32_t* sample_col;
+  int32_t impute;
+  double max_miss, maf_min;
+  float* A;            // m_pad x n_pad, centred imputed dosage (fp32)
+  double* A64;         // nullable, same centred values in fp64
+  double* dosage;      // nullable, m x n imputed dosage (uncentred)
+  uint8_t* keep;       // m_pad
+  double* af;          // m_pad
+};
+
+__global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (j >= a.m_pad) return;
+  float* Arow = a.A + j * a.n_pad;
+  double* A64row = a.A64 ? a.A64 + j * a.n_pad : nullptr;
+  if (j >= a.m) {
+    for (int i = lane; i < a.n_pad; i += kWave) {
+      Arow[i] = 0.f;
+      if (A64row) A64row[i] = 0.0;
+    }
+    if (lane == 0) { a.keep[j] = 0; a.af[j] = __builtin_nan(""); }
+    return;
+  }
+  const uint8_t* row = a.rows + j * a.bps;
+  int n_miss = 0, sum = 0, sum2 = 0, bad = 0;
+  for (int i = lane; i < a.n; i += kWave) {
+    const int col = a.sample_col[i];
+    if (col < 0 || col >= a.n_file) { bad = 1; continue; }
+    const int code = (row[col >> 2] >> ((col & 3) * 2)) & 3;
+    // PLINK-1 LUT {00→2, 01→missing, 10→1, 11→0} (plink_bed.cpp:221-224)
+    if (code == 1) { ++n_miss; continue; }
+    const int g = code == 0 ? 2 : (code == 2 ? 1 : 0);
+    sum += g;
+    sum2 += g * g;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    n_miss += __shfl_xor(n_miss, o);
+    sum += __shfl_xor(sum, o);
+    sum2 += __shfl_xor(sum2, o);
+    bad |= __shfl_xor(bad, o);
+  }
+  const int n = a.n;
+  const int n_ok = n - n_miss;
+  bool keep = !bad && n_ok > 0;
+  double mu = 0, muc = 0, afs = __builtin_nan("");
+  if (keep) {
+    const double miss_frac = static_cast<double>(n_miss) / static_cast<double>(n);
+    if (miss_frac > a.max_miss + 1e-15) keep = false;
+    if (!a.impute && n_miss > 0) keep = false;
+  }
+  if (keep) {
+    mu = static_cast<double>(sum) / n_ok;
+    double afd = (static_cast<double>(sum) / n_ok) / 2.0;
+    afd = fmin(1.0, fmax(0.0, afd));
+    if (afd < 1e-12 || afd > 1.0 - 1e-12) keep = false;
+    if (a.maf_min > 0.0 && (afd < a.maf_min || afd > 1.0 - a.maf_min)) keep = false;  // pass_maf
+  }
+  if (keep) {
+    // subset_maf_or_nan on the imputed vector
+    const double sI = sum + n_miss * mu, s2I = sum2 + n_miss * mu * mu;
+    double af = (sI / n) / 2.0;
+    af = fmin(1.0, fmax(0.0, af));
+    const double maf = af > 0.5 ? 1.0 - af : af;
+    const double var = s2I / n - (sI / n) * (sI / n);
+    if (maf < 1e-12 || var < 1e-12) keep = false;
+    afs = af;
+    muc = sI / n;
+  }
+  for (int i = lane; i < a.n_pad; i += kWave) {
+    double v = 0.0, dose = __builtin_nan("");
+    if (i < n) {
+      const int col = a.sample_col[i];
+      if (col >= 0 && col < a.n_file) {
+        const int code = (row[col >> 2] >> ((col & 3) * 2)) & 3;
+        if (code == 1) {
+          dose = a.impute ? mu : __builtin_nan("");
+        } else {
+          dose = code == 0 ? 2.0 : (code == 2 ? 1.0 : 0.0);
+        }
+        if (keep) v = (code == 1 ? mu : dose) - muc;
+      }
+      if (a.dosage) a.dosage[j * n + i] = dose;
+    }
+    Arow[i] = static_cast<float>(v);
+    if (A64row) A64row[i] = v;
+  }
+  if (lane == 0) {
+    a.keep[j] = keep ? 1 : 0;
+    a.af[j] = keep ? afs : __builtin_nan("");
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K6: rotation Gt[j, c] = sum_i A[j, i] Q[i, c]. WG tile 64 rows x 64 coords,
+// 4 waves x (16 rows x 4 blocks of 16 coords). Q stored coordinate-major
+// (Q[c * n_pad + i] = Q(i, c)), i.e. K-contiguous for the B operand.
+// ---------------------------------------------------------------------------
+template <int PREC>
+__global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ A, const float* __restrict__ Q32,
+                                                const __bf16* __restrict__ Qhi, const __bf16* __restrict__ Qlo,
+                                                float* __restrict__ Gt, int n_pad) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t r0 = static_cast<int64_t>(blockIdx.y) * 64 + wave * 16;
+  const int c0 = blockIdx.x * 64;
+  const int kk = 8 * (lane >> 4);
+  f32x4 acc[4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb) acc[cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const float* arow = A + (r0 + (lane & 15)) * n_pad + kk;
+  for (int k0 = 0; k0 < n_pad; k0 += 32) {
+    const f32x8 av = load8(arow + k0);
+    if constexpr (PREC == EQTL_PREC_FP32) {
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const size_t bo = static_cast<size_t>(c0 + cb * 16 + (lane & 15)) * n_pad + k0 + kk;
+        acc[cb] = mfma_f32x8(av, load8(Q32 + bo), acc[cb]);
+      }
+    } else {
+      bf16x8 ah, al;
+      split_bf16(av, ah, al);
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        const size_t bo = static_cast<size_t>(c0 + cb * 16 + (lane & 15)) * n_pad + k0 + kk;
+        const bf16x8 bh = load8_bf16(Qhi + bo), bl = load8_bf16(Qlo + bo);
+        acc[cb] = mfma_bf16(ah, bh, acc[cb]);
+        acc[cb] = mfma_bf16(ah, bl, acc[cb]);
+        acc[cb] = mfma_bf16(al, bh, acc[cb]);
+      }
+    }
+  }
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t row = r0 + 4 * (lane >> 4) + r;
+      Gt[row * n_pad + c0 + cb * 16 + (lane & 15)] = acc[cb][r];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K7/K8: fused Wald. One WG = one group of 16 genes x a strided set of
+// 64-SNP tiles; wave w owns rows 16w..16w+15 of a tile. Column block b of the
+// group holds quantity b for genes 0..15 (lane & 15), so each lane ends with
+// all NQ quantities of one (SNP row, gene) pair per output register r:
+//   LMM: b=0 d (against A^2 → gg), b=1..p whitened orthonormal covariates
+//        (→ W^T h), b=p+1 whitened null residual (→ r^T h)
+//   LM : b=0 residualised phenotype y_s (→ g^T y_s)
+// ---------------------------------------------------------------------------
+struct WaldArgs {
+  const float* A;          // m_pad x n_pad
+  const float* B32;        // gene blocks, fp32
+  const __bf16* Bhi;       // gene blocks, bf16 hi
+  const __bf16* Blo;       // gene blocks, bf16 lo
+  int32_t n_pad, nq, G, G_pad, n_groups, n_groups_pad, n_splits, n_tiles;
+  int64_t m, snp0;
+  const uint8_t* keep;     // block-local
+  const double* af;        // block-local
+  const double* gtg;       // LM: per SNP ||M_X g||^2 (block-local)
+  const double* gene_scal; // LMM rss_null, LM yty
+  const int32_t* gene_valid;
+  const int32_t* snp_chrom;   // global SNP annotation (nullable)
+  const int64_t* snp_pos;
+  const int32_t* gene_chrom;  // nullable: no cis exclusion
+  const int64_t* gene_cs;
+  const int64_t* gene_ce;
+  double df, pthr;
+  const TTail* tail;          // t tail table for df
+  GeneAcc* partial;           // [n_splits][G_pad]
+  eqtl_hit* hits;
+  unsigned long long* hit_count;
+  int64_t hit_cap;
+};
+
+template <int MODEL, int PREC, int NQ>
+__global__ __launch_bounds__(256) void k_wald(WaldArgs a) {
+  // XCD-aware mapping: workgroups are dealt round-robin to the 8 XCDs; keep
+  // consecutive workgroups of one XCD on the same gene group so that the
+  // group's column blocks stay resident in that XCD's L2.
+  const int wg = blockIdx.x;
+  const int xcd = wg & 7;
+  const int jx = wg >> 3;
+  const int group = (jx / a.n_splits) * 8 + xcd;
+  const int split = jx % a.n_splits;
+  if (group >= a.n_groups) return;
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gl = lane & 15;
+  const int gene = group * 16 + gl;
+  const bool gene_ok = gene < a.G && a.gene_valid[gene];
+  const int kk = 8 * (lane >> 4);
+  const int nq = a.nq;
+  const double gscal = gene_ok ? a.gene_scal[gene] : 0.0;
+  const bool excl = gene_ok && a.gene_chrom && a.gene_chrom[gene] >= 0;
+  const int32_t gch = excl ? a.gene_chrom[gene] : -1;
+  const int64_t gcs = excl ? a.gene_cs[gene] : 0, gce = excl ? a.gene_ce[gene] : 0;
+
+  GeneAcc st;
+  acc_init(st);
+  dd asum{0.0, 0.0};
+
+  const size_t gstride = static_cast<size_t>(16) * a.n_pad;  // one block of 16 columns
+  const size_t gbase = static_cast<size_t>(group) * nq * gstride + static_cast<size_t>(gl) * a.n_pad + kk;
+
+  for (int tile = split; tile < a.n_tiles; tile += a.n_splits) {
+    const int64_t r0 = static_cast<int64_t>(tile) * 64 + wave * 16;
+    f32x4 acc[NQ];
+#pragma unroll
+    for (int b = 0; b < NQ; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const float* arow = a.A + (r0 + gl) * a.n_pad + kk;
+    for (int k0 = 0; k0 < a.n_pad; k0 += 32) {
+      const f32x8 av = load8(arow + k0);
+      if constexpr (PREC == EQTL_PREC_FP32) {
+        f32x8 sv;
+        if constexpr (MODEL == EQTL_MODEL_LMM) {
+#pragma unroll
+          for (int t = 0; t < 8; ++t) sv.v[t] = av.v[t] * av.v[t];
+        }
+#pragma unroll
+        for (int b = 0; b < NQ; ++b) {
+          if (b < nq) {
+            const f32x8 bv = load8(a.B32 + gbase + b * gstride + k0);
+            if (MODEL == EQTL_MODEL_LMM && b == 0) acc[b] = mfma_f32x8(sv, bv, acc[b]);
+            else acc[b] = mfma_f32x8(av, bv, acc[b]);
+          }
+        }
+      } else {
+        bf16x8 ah, al, sh, sl;
+        split_bf16(av, ah, al);
+        if constexpr (MODEL == EQTL_MODEL_LMM) {
+          f32x8 sv;
+#pragma unroll
+          for (int t = 0; t < 8; ++t) sv.v[t] = av.v[t] * av.v[t];
+          split_bf16(sv, sh, sl);
+        }
+#pragma unroll
+        for (int b = 0; b < NQ; ++b) {
+          if (b < nq) {
+            const size_t bo = gbase + b * gstride + k0;
+            const bf16x8 bh = load8_bf16(a.Bhi + bo), bl = load8_bf16(a.Blo + bo);
+            if (MODEL == EQTL_MODEL_LMM && b == 0) {
+              acc[b] = mfma_bf16(sh, bh, acc[b]);
+              acc[b] = mfma_bf16(sh, bl, acc[b]);
+              acc[b] = mfma_bf16(sl, bh, acc[b]);
+            } else {
+              acc[b] = mfma_bf16(ah, bh, acc[b]);
+              acc[b] = mfma_bf16(ah, bl, acc[b]);
+              acc[b] = mfma_bf16(al, bh, acc[b]);
+            }
+          }
+        }
+      }
+    }
+    // ---- FP64 epilogue: 4 (SNP, gene) tests per lane ----
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t j = r0 + 4 * (lane >> 4) + r;  // block-local SNP row
+      if (!gene_ok || j >= a.m || !a.keep[j]) continue;
+      const int64_t snp = a.snp0 + j;
+      if (excl && a.snp_chrom && a.snp_chrom[snp] == gch && a.snp_pos[snp] >= gcs && a.snp_pos[snp] <= gce)
+        continue;
+      double beta = 0, se = 0, stat = 0, p = 1.0, r2 = 0;
+      const double df = a.df;
+      if constexpr (MODEL == EQTL_MODEL_LMM) {
+        const double gg = acc[0][r];
+        double ss = 0;
+#pragma unroll
+        for (int b = 1; b < NQ; ++b)
+          if (b < nq - 1) ss += static_cast<double>(acc[b][r]) * acc[b][r];
+        double num = 0;
+#pragma unroll
+        for (int b = 1; b < NQ; ++b)
+          if (b == nq - 1) num = acc[b][r];
+        const double S = gg - ss;
+        if (!(gg < 1e-12) && !(S <= 1e-15)) {
+          beta = num / S;
+          double q = gscal - S * beta * beta;
+          if (q < 0) q = 0;
+          const double sigma2 = df > 0 ? q / df : 1.0;
+          se = sqrt(fmax(sigma2 / S, 0.0));
+          stat = se > 0 ? beta / se : 0.0;
+          p = p_from_t_tab(*a.tail, stat);
+          r2 = gscal > 1e-15 ? fmax(0.0, 1.0 - q / gscal) : 0.0;
+        }
+      } else {
+        const double gtg = a.gtg[j];
+        const double gty = acc[0][r];
+        if (!(gtg < 1e-12) && df > 0) {
+          beta = gty / gtg;
+          const double rss = gscal - beta * gty;
+          const double s2 = fmax(rss / df, 0.0);
+          se = sqrt(s2 / gtg);
+          stat = se > 0 ? beta / se : 0.0;
+          p = p_from_t_tab(*a.tail, stat);
+          r2 = gscal > 0 ? 1.0 - rss / gscal : 0.0;
+        } else if (!(gtg < 1e-12)) {
+          beta = gty / gtg;  // df <= 0: p = 1 with beta set (model_lm.cpp:44-48)
+        }
+      }
+      if (!isfinite(p)) continue;
+      ++st.n_tested;
+      double term;
+      if (acat_accept(p, &term)) {
+        asum = dd_add_d(asum, term);
+        ++st.used;
+      }
+      const double afj = a.af[j];
+      if (p <= a.pthr) {
+        ++st.n_sig;
+        const unsigned long long idx = atomicAdd(a.hit_count, 1ull);
+        if (static_cast<int64_t>(idx) < a.hit_cap) {
+          eqtl_hit h;
+          h.snp = snp; h.gene = gene; h.pad = 0;
+          h.af = afj; h.beta = beta; h.se = se; h.stat = stat; h.p = p; h.r2 = r2;
+          a.hits[idx] = h;
+        }
+      }
+      if (p < st.best_p || (p == st.best_p && snp < st.best_snp)) {
+        st.best_p = p; st.best_snp = snp; st.best_af = afj; st.best_beta = beta;
+        st.best_se = se; st.best_stat = stat; st.best_r2 = r2;
+      }
+    }
+  }
+  st.acat_hi = asum.hi;
+  st.acat_lo = asum.lo;
+  // ---- reduce the 16 partial states of each gene (4 lane groups x 4 waves) ----
+  __shared__ GeneAcc red[16][16];
+  red[wave * 4 + (lane >> 4)][gl] = st;
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    GeneAcc t = red[0][threadIdx.x];
+    for (int s = 1; s < 16; ++s) acc_merge(t, red[s][threadIdx.x]);
+    const int g = group * 16 + threadIdx.x;
+    a.partial[static_cast<size_t>(split) * a.G_pad + g] = t;
+  }
+}
+
+// Fold one block's partials into the running per-gene state.
+__global__ void k_merge(GeneAcc* state, const GeneAcc* partial, int G, int G_pad, int n_splits) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  GeneAcc t = state[g];
+  for (int s = 0; s < n_splits; ++s) acc_merge(t, partial[static_cast<size_t>(s) * G_pad + g]);
+  state[g] = t;
+}
+
+__global__ void k_state_init(GeneAcc* state, int G) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g < G) acc_init(state[g]);
+}
+
+__global__ void k_finalize(const GeneAcc* state, eqtl_gene_state* out, int G) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const GeneAcc a = state[g];
+  eqtl_gene_state o;
+  o.acat_p = acat_p_final(dd{a.acat_hi, a.acat_lo}, a.used);
+  o.best_p = a.best_p;
+  o.best_snp = a.best_snp;
+  o.best_af = a.best_af; o.best_beta = a.best_beta; o.best_se = a.best_se;
+  o.best_stat = a.best_stat; o.best_r2 = a.best_r2;
+  o.n_tested = a.n_tested;
+  o.n_sig = a.n_sig;
+  out[g] = o;
+}
+
+// LM: per-SNP gtg = ||g_c||^2 - ||Wx^T g_c||^2 in FP64 (residualize_g,
+// scan_trans_lm.cpp:11-19, with an orthonormal basis Wx of the covariates).
+__global__ __launch_bounds__(256) void k_lm_gtg(const double* A64, int64_t m_pad, int n, int n_pad,
+                                                const double* Wx, int p, double* gtg) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  if (j >= m_pad) return;
+  const double* g = A64 + j * n_pad;
+  double ss = 0;
+  for (int i = lane; i < n; i += kWave) ss += g[i] * g[i];
+  for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+  double proj = 0;
+  for (int c = 0; c < p; ++c) {
+    double s = 0;
+    const double* w = Wx + static_cast<size_t>(c) * n;
+    for (int i = lane; i < n; i += kWave) s += w[i] * g[i];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    proj += s * s;
+  }
+  if (lane == 0) gtg[j] = ss - proj;
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic PLINK rows (benchmark input generated in HBM). Counter-based hash
+// so SNP j depends only on (seed, j).
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint64_t mix64(uint64_t x) {
+  x ^= x >> 30; x *= 0xbf58476d1ce4e5b9ull;
+  x ^= x >> 27; x *= 0x94d049bb133111ebull;
+  x ^= x >> 31;
+  return x;
+}
+__host__ __device__ inline double u01(uint64_t h) { return (h >> 11) * (1.0 / 9007199254740992.0); }
+
+__global__ void k_synth_bed(uint8_t* rows, int64_t m, int n, int bps, uint64_t seed, double maf_lo,
+                            double maf_hi, double miss) {
+  const int64_t idx = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= m * bps) return;
+  const int64_t j = idx / bps;
+  const int byte = static_cast<int>(idx % bps);
+  const uint64_t hs = mix64(seed * 0x9E3779B97F4A7C15ull + static_cast<uint64_t>(j));
+  const double maf = maf_lo + (maf_hi - maf_lo) * u01(hs);
+  const double f = (mix64(hs ^ 0x5555) & 1) ? 1.0 - maf : maf;
+  uint8_t out = 0;
+  for (int q = 0; q < 4; ++q) {
+    const int i = byte * 4 + q;
+    int code = 1;  // padding / missing
+    if (i < n) {
+      const uint64_t h = mix64(hs + 0x632BE59BD9B4E019ull * static_cast<uint64_t>(i + 1));
+      const double um = u01(h), u1 = u01(mix64(h ^ 0xA5A5A5A5ull)), u2 = u01(mix64(h ^ 0x3C3C3C3Cull));
+      if (um < miss) {
+        code = 1;
+      } else {
+        const int g = (u1 < f) + (u2 < f);
+        code = g == 2 ? 0 : (g == 1 ? 2 : 3);
+      }
+    }
+    out |= static_cast<uint8_t>(code << (2 * q));
+  }
+  rows[idx] = out;
+}
+
+}  // namespace eqtl_amd

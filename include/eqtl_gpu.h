@@ -1,0 +1,163 @@
+/* eqtl_gpu.h — C ABI of the MI355X (gfx950) eqtl association engine.
+ *
+ * Plain C types only; every call returns 0 (EQTL_OK) or a negative EQTL_E_*
+ * code, and eqtl_gpu_last_error() returns the calling thread's last message.
+ * One host thread drives one context; all device work of a context runs in
+ * order on that context's single HIP stream (default priority).
+ *
+ * Layouts: Q n x n column-major (column c = eigenvector c), lambda ascending;
+ * Y n x G column-major (gene k contiguous); X n x p column-major with the
+ * intercept included by the caller; bed rows SNP-major, bytes_per_snp each
+ * (PLINK-1 2-bit codes, no magic header).
+ *
+ * The reference (WWz33/eqtl, C++17) has no FFI; its seams are C++ functions.
+ * Each entry point below names the reference function(s) whose work it takes
+ * over; the C++ host mirror in eqtl_amd/host/ calls these in place of them.
+ */
+#ifndef EQTL_GPU_H
+#define EQTL_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct eqtl_gpu_ctx eqtl_gpu_ctx;
+
+enum {
+  EQTL_OK = 0,
+  EQTL_E_ARG = -1,      /* bad argument / shape */
+  EQTL_E_HIP = -2,      /* HIP runtime error */
+  EQTL_E_NOMEM = -3,    /* device allocation failed */
+  EQTL_E_STATE = -4,    /* call order (e.g. scan before prep) */
+  EQTL_E_SOLVER = -5,   /* eigensolver failure (make_lmm_basis dies) */
+  EQTL_E_CAPACITY = -6  /* hit buffer too small: *n_hits holds the count */
+};
+
+enum { EQTL_MODEL_LM = 0, EQTL_MODEL_LMM = 1 };
+/* Contraction precision of the fused Wald kernel (and the genotype rotation).
+ * FP32: fp32 MFMA on fp32 operands. BF16X3: operands split into bf16 hi+lo,
+ * three bf16 MFMA products (hi*hi + hi*lo + lo*hi) with fp32 accumulation. */
+enum { EQTL_PREC_FP32 = 0, EQTL_PREC_BF16X3 = 1 };
+/* REML delta: REF = restated grid + golden + numeric-Newton search
+ * (model_lmm.cpp:39-86); NEWTON = grid start + safeguarded Newton on log delta
+ * with the analytic adjoint gradient/Hessian; GIVEN = caller's delta_in. */
+enum { EQTL_DELTA_REF = 0, EQTL_DELTA_NEWTON = 1, EQTL_DELTA_GIVEN = 2 };
+
+/* One significant (p <= pthr) pair — the fields of write_pair_line
+ * (output.cpp:27-76) that the engine computes; text columns come from the bim. */
+typedef struct {
+  int64_t snp;   /* global SNP index (bim order) */
+  int32_t gene;  /* gene index (order of Y's columns) */
+  int32_t pad;
+  double af, beta, se, stat, p, r2;
+} eqtl_hit;
+
+/* Per-gene summary of a trans scan (scan_trans.hpp:87-110 + AcatAcc). */
+typedef struct {
+  double acat_p;    /* AcatAcc::p() */
+  double best_p;    /* 2.0 when nothing tested */
+  int64_t best_snp; /* first SNP (bim order) reaching best_p; -1 if none */
+  double best_af, best_beta, best_se, best_stat, best_r2;
+  int32_t n_tested;
+  int32_t n_sig;
+} eqtl_gene_state;
+
+/* ---- context ------------------------------------------------------------- */
+int eqtl_gpu_open(int32_t device, eqtl_gpu_ctx** out);
+int eqtl_gpu_close(eqtl_gpu_ctx* ctx);
+const char* eqtl_gpu_last_error(void);
+/* Device properties for callers that size their own work. */
+int eqtl_gpu_device_info(eqtl_gpu_ctx* ctx, char* name, int32_t name_len, int32_t* n_cu,
+                         int64_t* hbm_bytes);
+
+/* ---- genotype source (PlinkBed, plink_bed.cpp:245-357) -------------------- */
+/* Analysis sample order: sample_col[i] = fam column of analysis sample i
+ * (PlinkBed::set_sample_order, plink_bed.cpp:245-255). */
+int eqtl_gpu_set_samples(eqtl_gpu_ctx* ctx, int32_t n_file, const int32_t* sample_col, int32_t n);
+/* Missing policy and MAF gate (MissPolicy + maf_min). impute: 1 = mean-impute,
+ * 0 = drop any SNP with a missing call. */
+int eqtl_gpu_set_policy(eqtl_gpu_ctx* ctx, int32_t impute, double max_miss, double maf_min);
+/* Per-SNP chromosome id (any int, equal = same chromosome) and position for
+ * the trans cis-exclusion (in_cis_window, scan_common.hpp:65-69). */
+int eqtl_gpu_set_snp_annot(eqtl_gpu_ctx* ctx, const int32_t* chrom, const int64_t* pos, int64_t m);
+/* Decode m bed rows (K1; decode_row plink_bed.cpp:263-320 followed by
+ * subset_maf_or_nan scan_common.cpp:133-151). rows_on_device: rows is a device
+ * pointer. Outputs (host, nullable): dosage m x n imputed (NaN for missing when
+ * not imputing), keep[m] (1 = SNP tested), af[m] (subset allele frequency). */
+int eqtl_gpu_decode_bed(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_device, int64_t m,
+                        int32_t bytes_per_snp, double* dosage, uint8_t* keep, double* af);
+
+/* ---- GRM (compute_grm_from, grm.cpp:112-224) ----------------------------- */
+/* Accumulate m bed rows into the GRM; call repeatedly, then eqtl_gpu_grm_finish. */
+int eqtl_gpu_grm_begin(eqtl_gpu_ctx* ctx);
+int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_device,
+                            int64_t m, int32_t bytes_per_snp);
+/* K n x n column-major FP64, N_packed n(n+1)/2 (GCTA .grm.N.bin order), m_used. */
+int eqtl_gpu_grm_finish(eqtl_gpu_ctx* ctx, double* K, int32_t* N_packed, int64_t* m_used);
+
+/* ---- eigenbasis (make_lmm_basis, model_lmm.cpp:106-113) ------------------ */
+/* K (n x n, col-major) -> Q, lambda (ascending, clamped >= 0), FP64 on the GPU
+ * (rocSOLVER dsyevd). The basis stays resident in the context. Q_out/lambda_out
+ * nullable. */
+int eqtl_gpu_eigh(eqtl_gpu_ctx* ctx, const double* K, int32_t n, double* Q_out, double* lambda_out);
+/* Inject a basis (test hook: lets LMM permutation streams match an external Q). */
+int eqtl_gpu_set_basis(eqtl_gpu_ctx* ctx, const double* Q, const double* lambda, int32_t n);
+
+/* ---- gene preparation ------------------------------------------------------ */
+/* prep_lmm for all genes at once (model_lmm.cpp:124-153; scan_trans_lmm.cpp:110-135):
+ * Y~ = Q^T Y, X~ = Q^T X (once, FP64), per-gene REML delta (delta_mode), then
+ * the per-gene whitened orthonormal covariate basis and null residual that the
+ * fused Wald kernel contracts against. delta_in used when mode is GIVEN.
+ * delta_out / rss_null_out (G each) nullable. */
+int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* ctx, const double* Y, int32_t G, const double* X, int32_t p,
+                      int32_t delta_mode, const double* delta_in, double* delta_out,
+                      double* rss_null_out);
+/* REML adjoint: f(theta), df/dtheta, d2f/dtheta2 at theta = log(delta) for all
+ * G genes in one launch (reml_negll, model_lmm.cpp:10-37, differentiated).
+ * Needs a prior eqtl_gpu_lmm_prep (uses its Y~, X~). */
+int eqtl_gpu_reml_grad_hess(eqtl_gpu_ctx* ctx, const double* log_delta, int32_t G, double* f,
+                            double* g, double* h);
+/* prep_lm for all genes (model_lm.cpp:19-30): residualised y_s, yty. yty_out nullable. */
+int eqtl_gpu_lm_prep(eqtl_gpu_ctx* ctx, const double* Y, int32_t G, const double* X, int32_t p,
+                     double* yty_out);
+
+/* Cis windows for the trans exclusion, per gene: chrom id (< 0: gene has no
+ * location, never excluded) and [cstart, cend] (cis_window_bounds). Optional:
+ * without it nothing is excluded (scope "gw"). */
+int eqtl_gpu_set_gene_windows(eqtl_gpu_ctx* ctx, const int32_t* chrom, const int64_t* cstart,
+                              const int64_t* cend, int32_t G);
+
+/* ---- trans scans (scan_lm_snp_outer / scan_lmm_snp_outer) ------------------ */
+int eqtl_gpu_scan_begin(eqtl_gpu_ctx* ctx, int32_t model, int32_t precision, double pthr);
+/* One block of m SNPs with global indices snp0 .. snp0+m-1 (bim order). Hits
+ * for this block are appended to hits (capacity hit_cap) in (snp, gene)
+ * order; *n_hits is the block's count (EQTL_E_CAPACITY if it exceeds hit_cap;
+ * the gene states are still updated). */
+int eqtl_gpu_scan_block(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_device,
+                        int64_t snp0, int64_t m, int32_t bytes_per_snp, eqtl_hit* hits,
+                        int64_t hit_cap, int64_t* n_hits);
+/* Final per-gene summaries (G entries). */
+int eqtl_gpu_scan_end(eqtl_gpu_ctx* ctx, eqtl_gene_state* state);
+
+/* ---- instrumentation ------------------------------------------------------- */
+/* When enabled, the library brackets each launch of the named kernel family
+ * ("wald", "rotate", "decode", "reml") with HIP events on its stream. */
+int eqtl_gpu_set_timing(eqtl_gpu_ctx* ctx, int32_t enable);
+int eqtl_gpu_get_timing(eqtl_gpu_ctx* ctx, const char* family, double* total_ms, int64_t* launches);
+/* Synthetic PLINK rows generated on the device (benchmark inputs resident in
+ * HBM): m SNPs x n samples, MAF ~ U(maf_lo, maf_hi), A1 flipped by a fair
+ * coin, HWE genotypes, miss rate; counter-based hash, so row j depends only
+ * on (seed, j). Returns a device pointer owned by the context. */
+int eqtl_gpu_synth_bed(eqtl_gpu_ctx* ctx, uint64_t seed, int64_t m, int32_t n, double maf_lo,
+                       double maf_hi, double miss, uint8_t** rows_dev);
+/* Copy m rows starting at row j0 of a device bed buffer back to the host. */
+int eqtl_gpu_copy_rows(eqtl_gpu_ctx* ctx, const uint8_t* rows_dev, int64_t j0, int64_t m,
+                       int32_t bytes_per_snp, uint8_t* out);
+int eqtl_gpu_synchronize(eqtl_gpu_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EQTL_GPU_H */

@@ -1,0 +1,324 @@
+"""CPU: pin the FP64 oracle against golden vectors, closed forms, and the
+reference's own compiled translation units (oracle/_ref, when built)."""
+import ctypes
+import json
+import os
+import tempfile
+
+import numpy as np
+import pytest
+from scipy import optimize, special
+
+import oracle as O
+from eqtl_amd import synth
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---------------------------------------------------------------- t tail
+def test_p_from_t_mpmath_vectors():
+    rows = json.load(open(os.path.join(GOLD, "t_tail.json")))
+    for r in rows:
+        p = O.p_from_t([r["t"]], r["df"])[0]
+        exact = float(r["p"])
+        assert abs(p - exact) <= 1e-9 * exact + 1e-300, r
+
+
+def test_p_from_t_scipy_and_symmetry():
+    rng = np.random.default_rng(1)
+    for df in (1.0, 4.0, 57.0, 1978.0):
+        t = rng.normal(0, 4, 500)
+        p = O.p_from_t(t, df)
+        ps = 2 * special.stdtr(df, -np.abs(t))
+        np.testing.assert_allclose(p, ps, rtol=1e-9, atol=1e-300)
+        np.testing.assert_array_equal(p, O.p_from_t(-t, df))
+    assert O.p_from_t([np.inf], 10)[0] == 1.0
+    assert O.p_from_t([np.nan], 10)[0] == 1.0
+    assert O.p_from_t([1.0], 0)[0] == 1.0
+    assert O.p_from_t([0.0], 10)[0] == 1.0
+
+
+def test_p_from_t_matches_reference_util():
+    R = O.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(2)
+    for df in (5.0, 200.0, 1978.0):
+        for t in rng.normal(0, 10, 200):
+            assert O.p_from_t([t], df)[0] == R.ref_p_from_t(ctypes.c_double(t), ctypes.c_double(df))
+
+
+# ---------------------------------------------------------------- ACAT, p_emp, BH
+def test_acat_closed_form():
+    import mpmath as mp
+
+    mp.mp.dps = 40
+    rng = np.random.default_rng(3)
+    p = rng.uniform(1e-8, 1 - 1e-9, 300)
+    T = sum(mp.tan((mp.mpf("0.5") - mp.mpf(float(v))) * mp.pi) for v in p) / len(p)
+    exact = float(mp.mpf("0.5") - mp.atan(T) / mp.pi)
+    assert abs(O.acat(p) - exact) <= 1e-12 * exact
+
+
+def test_acat_skip_rules_and_saturation():
+    assert O.acat([]) == 1.0
+    assert O.acat([1.0, 1 - 1e-13, np.nan, -0.1]) == 1.0          # all skipped
+    assert O.acat([0.3, 1.0]) == O.acat([0.3])
+    # x87 quirk of the reference: tanl((0.5L - p)*kPi) for p < 2^-66 is a
+    # large negative number (tan of the rounded pi/2), so ACAT → 1
+    assert O.acat([1e-300]) == 1.0
+    assert O.acat([0.0]) == 1.0                                   # p <= 0 → 1e-300
+    assert O.acat([1e-19]) < 1e-18
+
+
+def test_p_emp_and_bh():
+    assert O.p_emp(2.0, [1.0, 2.0, 3.0, 0.5]) == (1 + 2) / 5
+    assert O.p_emp(9.0, [1.0, 2.0]) == 1 / 3
+    a = np.array([0.01, 0.04, 0.03, 0.5, 0.02])
+    q = O.bh(a)
+    # by hand: sorted 0.01,0.02,0.03,0.04,0.5 → q = min(p*m/rank) cumulatively from the top
+    exp_sorted = np.minimum.accumulate((np.array([0.01, 0.02, 0.03, 0.04, 0.5]) * 5 / np.arange(1, 6))[::-1])[::-1]
+    order = np.argsort(a)
+    np.testing.assert_allclose(q[order], np.minimum(exp_sorted, 1.0))
+
+
+# ---------------------------------------------------------------- decode
+def test_decode_known_answer_table():
+    # one byte: samples 0..3 carry codes 00, 01, 10, 11 → A1 counts 2, missing, 1, 0
+    row = np.array([[0b11100100]], dtype=np.uint8)
+    dos, miss, keep, af = O.decode_bed(row, 4, impute=True, max_miss=0.8)
+    assert keep[0]
+    np.testing.assert_array_equal(miss[0], [0, 1, 0, 0])
+    np.testing.assert_allclose(dos[0], [2, 1, 1, 0])   # missing imputed with mean(2,1,0)=1
+    assert af[0] == 0.5
+    dos, miss, keep, af = O.decode_bed(row, 4, impute=False)
+    assert not keep[0]                                  # filter drops any missing
+
+
+def test_decode_bruteforce():
+    rng = np.random.default_rng(4)
+    for n in (1, 3, 4, 5, 31, 64, 65):
+        g, _ = synth.draw_genotypes(rng, 80, n, 0.0, 0.5, 0.1)
+        g[0] = -1
+        rows = synth.pack_bed(g)
+        assert np.array_equal(synth.unpack_bed(rows, n), g)
+        for max_miss in (0.8, 0.05):
+            dos, miss, keep, af = O.decode_bed(rows, n, impute=True, max_miss=max_miss)
+            for j in range(80):
+                obs = g[j] >= 0
+                nm = (~obs).sum()
+                if obs.sum() == 0 or nm / n > max_miss + 1e-15:
+                    assert not keep[j]
+                    continue
+                mu = g[j][obs].mean()
+                a = mu / 2
+                if a < 1e-12 or a > 1 - 1e-12:
+                    assert not keep[j]
+                    continue
+                assert keep[j]
+                np.testing.assert_allclose(dos[j], np.where(obs, g[j], mu))
+                assert abs(af[j] - a) < 1e-15
+
+
+def test_decode_matches_reference_plinkbed(tmp_path):
+    R = O.ref_lib()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    ds = synth.make_trans(8, n=37, m=300, G=2, n_chrom=2, ncov=1, miss=0.05)
+    ds.geno[5] = 0
+    ds.geno[6] = -1
+    prefix = str(tmp_path / "d")
+    ds.write(prefix)
+    cap = ds.m
+    idx = np.empty(cap, dtype=np.int64)
+    af = np.empty(cap)
+    dos = np.empty((cap, ds.n))
+    for impute in (1, 0):
+        k = R.ref_decode_bed(prefix.encode(), ctypes.c_int(impute), ctypes.c_double(0.8), ctypes.c_double(0.0),
+                             ctypes.c_long(cap), idx.ctypes.data_as(ctypes.c_void_p),
+                             af.ctypes.data_as(ctypes.c_void_p), dos.ctypes.data_as(ctypes.c_void_p),
+                             ctypes.c_int(ds.n))
+        od, om, ok, oaf = O.decode_bed(ds.bed_rows(), ds.n, impute=bool(impute))
+        assert k == ok.sum()
+        np.testing.assert_array_equal(idx[:k], np.nonzero(ok)[0])
+        np.testing.assert_array_equal(af[:k], oaf[ok])
+        np.testing.assert_array_equal(dos[:k], od[ok])
+
+
+def test_subset_maf_or_nan():
+    r, maf = O.subset_maf_or_nan(np.array([0.0, 1.0, 2.0, 1.0]))
+    assert r == 0.5 and maf == 0.5
+    assert np.isnan(O.subset_maf_or_nan(np.zeros(5))[0])          # MAF < 1e-12
+    assert np.isnan(O.subset_maf_or_nan(np.array([1.0, 1.0]))[0])  # var < 1e-12
+    assert np.isnan(O.subset_maf_or_nan(np.array([1.0, np.nan]))[0])
+
+
+# ---------------------------------------------------------------- GRM, eigh
+def test_grm_closed_form_and_reference(tmp_path):
+    ds = synth.make_trans(9, n=50, m=700, G=2, n_chrom=2, ncov=1, miss=0.03, maf=(0.01, 0.5))
+    dos, miss, keep, af = O.decode_bed(ds.bed_rows(), ds.n)
+    K, N, used = O.grm(dos, miss, keep)
+    Kc, Nc = synth.grm_closed_form(ds.geno[keep])
+    np.testing.assert_allclose(K, Kc, atol=1e-13)
+    il = np.tril_indices(ds.n)
+    np.testing.assert_array_equal(N, Nc[il[0], il[1]].astype(np.int32)[np.argsort(il[0] * (ds.n + 1) + il[1], kind="stable")])
+    assert np.allclose(K, K.T)
+    R = O.ref_lib()
+    if R is None:
+        return
+    prefix = str(tmp_path / "g")
+    ds.write(prefix)
+    Kr = np.empty((ds.n, ds.n), order="F")
+    Nr = np.empty(ds.n * (ds.n + 1) // 2, dtype=np.int32)
+    assert R.ref_compute_grm(prefix.encode(), ctypes.c_int(1), ctypes.c_double(0.8), ctypes.c_double(0.0),
+                             ctypes.c_int(ds.n), Kr.ctypes.data_as(ctypes.c_void_p),
+                             Nr.ctypes.data_as(ctypes.c_void_p)) == 0
+    np.testing.assert_allclose(K, Kr, atol=1e-12)
+    np.testing.assert_array_equal(N, Nr)
+
+
+def test_jacobi_eigh():
+    rng = np.random.default_rng(10)
+    A = rng.normal(size=(40, 40))
+    K = A @ A.T / 40
+    Q, lam = O.eigh(K)
+    w = np.linalg.eigvalsh(K)
+    np.testing.assert_allclose(lam, np.maximum(w, 0), atol=1e-12)
+    np.testing.assert_allclose(Q.T @ Q, np.eye(40), atol=1e-12)
+    np.testing.assert_allclose(Q @ np.diag(lam) @ Q.T, K, atol=1e-12)
+
+
+# ---------------------------------------------------------------- LMM pieces
+def _small_lmm(seed=12, n=70, p=4):
+    rng = np.random.default_rng(seed)
+    A = rng.normal(size=(n, 3 * n))
+    K = A @ A.T / (3 * n)
+    Q, lam = O.eigh(K)
+    X = np.hstack([np.ones((n, 1)), rng.normal(size=(n, p - 1))])
+    y = rng.normal(size=n) + (np.linalg.cholesky(K + 1e-9 * np.eye(n)) @ rng.normal(size=n))
+    return K, Q, lam, X, y, rng
+
+
+def test_reml_negll_closed_form():
+    K, Q, lam, X, y, rng = _small_lmm()
+    n, p = X.shape
+    yt, Xt = Q.T @ y, Q.T @ X
+    for delta in (1e-3, 0.5, 3.0, 200.0):
+        V = delta * (Q @ np.diag(lam) @ Q.T) + np.eye(n)
+        Vi = np.linalg.inv(V)
+        A = X.T @ Vi @ X
+        b = np.linalg.solve(A, X.T @ Vi @ y)
+        q = (y - X @ b) @ Vi @ (y - X @ b)
+        df = n - p
+        f = 0.5 * (df * np.log(q / df) + np.linalg.slogdet(V)[1] + np.linalg.slogdet(A)[1])
+        assert abs(O.reml_negll(delta, yt, Xt, lam) - f) < 1e-9 * abs(f)
+
+
+def test_optimize_delta_near_argmin():
+    K, Q, lam, X, y, rng = _small_lmm(13)
+    yt, Xt = Q.T @ y, Q.T @ X
+    d = O.optimize_delta(yt[:, None], Xt, lam)[0]
+    res = optimize.minimize_scalar(lambda t: O.reml_negll(np.exp(t), yt, Xt, lam), bounds=(np.log(1e-8), np.log(1e6)),
+                                   method="bounded", options={"xatol": 1e-10})
+    fo = O.reml_negll(d, yt, Xt, lam)
+    assert fo <= res.fun + 1e-9 * abs(res.fun)
+
+
+def test_lmm_wald_vs_bruteforce_gls():
+    K, Q, lam, X, y, rng = _small_lmm(14)
+    n, p = X.shape
+    yt, Xt = Q.T @ y, Q.T @ X
+    Kp = Q @ np.diag(lam) @ Q.T
+    for delta in (0.05, 1.7):
+        V = delta * Kp + np.eye(n)
+        Vi = np.linalg.inv(V)
+        for s in range(5):
+            g = rng.binomial(2, 0.3, n).astype(float)
+            Xg = np.hstack([X, g[:, None]])
+            A = Xg.T @ Vi @ Xg
+            beta = np.linalg.solve(A, Xg.T @ Vi @ y)
+            r = y - Xg @ beta
+            df = n - p - 1
+            sigma2 = (r @ Vi @ r) / df
+            se = np.sqrt(sigma2 * np.linalg.inv(A)[-1, -1])
+            out, _ = O.lmm_test(yt, Xt, lam, delta, Q.T @ g)
+            np.testing.assert_allclose(out[0], beta[-1], rtol=1e-9)
+            np.testing.assert_allclose(out[1], se, rtol=1e-9)
+            np.testing.assert_allclose(out[3], 2 * special.stdtr(df, -abs(beta[-1] / se)), rtol=1e-8)
+
+
+def test_lmm_and_lm_match_reference_fixture():
+    z = np.load(os.path.join(GOLD, "ref_cases.npz"))
+    n = int(z["n"])
+    rows, Y, X, sel, Q, lam = z["rows"], z["Y"], z["X"], z["sel"], z["Q"], z["lam"]
+    dos, miss, keep, af = O.decode_bed(rows, n)
+    Yt, Xt = O.rotate(Q, Y), O.rotate(Q, X)
+    for k in range(Y.shape[1]):
+        d = O.optimize_delta(Yt[:, k:k + 1], Xt, lam)[0]
+        dref = z["lmm_prep"][k, 0]
+        assert abs(d - dref) <= 1e-6 * dref, (k, d, dref)
+        for s, j in enumerate(sel):
+            out, _ = O.lmm_test(Yt[:, k], Xt, lam, d, O.rotate(Q, dos[j]))
+            ref = z["lmm_out"][k, s]
+            assert abs(out[2] - ref[2]) <= 1e-5 * (1 + abs(ref[2])), (k, s)
+            assert abs(out[3] - ref[3]) <= 1e-4 * ref[3]
+        # LM via the trans scan restatement on the same SNPs
+        res = O.trans_scan("lm", dos[sel], np.ones(len(sel), bool), np.ones(len(sel)), np.arange(len(sel)),
+                           Y[:, k:k + 1], X, pthr=1.0, dense=True)
+        np.testing.assert_allclose(res["stat"][:, 0], z["lm_out"][k, :, 2], rtol=1e-10, atol=1e-12)
+        np.testing.assert_allclose(res["p"][:, 0], z["lm_out"][k, :, 3], rtol=1e-9)
+
+
+def test_lm_vs_ols():
+    rng = np.random.default_rng(15)
+    n, p = 80, 4
+    X = np.hstack([np.ones((n, 1)), rng.normal(size=(n, p - 1))])
+    Y = rng.normal(size=(n, 3))
+    G = rng.binomial(2, 0.4, (30, n)).astype(float)
+    res = O.trans_scan("lm", G, np.ones(30, bool), np.ones(30), np.arange(30), Y, X, pthr=1.0, dense=True)
+    for j in range(30):
+        for k in range(3):
+            Xg = np.hstack([X, G[j][:, None]])
+            b, *_ = np.linalg.lstsq(Xg, Y[:, k], rcond=None)
+            r = Y[:, k] - Xg @ b
+            df = n - p - 1
+            se = np.sqrt(r @ r / df * np.linalg.inv(Xg.T @ Xg)[-1, -1])
+            np.testing.assert_allclose(res["beta"][j, k], b[-1], rtol=1e-9)
+            np.testing.assert_allclose(res["se"][j, k], se, rtol=1e-9)
+
+
+# ---------------------------------------------------------------- scans: semantics
+def test_trans_scan_semantics_cis_exclusion_and_best_ties():
+    rng = np.random.default_rng(16)
+    n, m = 60, 20
+    g = rng.binomial(2, 0.3, (m, n)).astype(float)
+    g[7] = g[3]                    # identical SNPs → identical p: first (3) must win
+    X = np.ones((n, 1))
+    y = g[3] * 0.8 + rng.normal(size=n)
+    chrom = np.array([1] * 10 + [2] * 10)
+    pos = np.arange(m) * 100 + 1
+    res = O.trans_scan("lm", g, np.ones(m, bool), chrom, pos, y[:, None], X, gene_chrom=[2], gene_cstart=[1],
+                       gene_cend=[10**9], pthr=1.0, dense=True)
+    st = res["state"][0]
+    assert st["n_tested"] == 10                # chromosome 2 SNPs are cis → excluded
+    assert st["best_snp"] == 3
+    assert np.all(np.isnan(res["p"][10:, 0]))
+
+
+# ---------------------------------------------------------------- permutations
+def test_perm_index_fixture():
+    for c in json.load(open(os.path.join(GOLD, "perm_idx.json"))):
+        assert O.fnv1a(c["gene"]) == c["fnv1a"]
+        np.testing.assert_array_equal(O.perm_index(c["seed"], c["b"], c["gene"], c["n"]), c["idx"])
+
+
+def test_fnv1a_known_values():
+    # FNV-1a 64-bit offset basis and prime, low 32 bits (util.hpp:51-58)
+    def f(s):
+        h = 14695981039346656037
+        for ch in s.encode():
+            h ^= ch
+            h = (h * 1099511628211) % (1 << 64)
+        return h & 0xFFFFFFFF
+    for s in ("", "a", "g0", "ENSG00000123456"):
+        assert O.fnv1a(s) == f(s)

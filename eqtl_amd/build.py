@@ -15,6 +15,21 @@ HIPCC_FLAGS = [
 ]
 
 
+HOST_SRC = [os.path.join(HERE, "host", "eqtl_host.cpp"), os.path.join(HERE, "host", "main.cpp")]
+CLI = os.path.join(HERE, "eqtl")
+
+
+def build_cli() -> str:
+    """The eqtl command-line driver (C++17 host code over libeqtl_gpu.so)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", *HOST_SRC, "-L" + HERE, "-leqtl_gpu",
+           "-Wl,-rpath,$ORIGIN", "-o", CLI + ".tmp"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + res.stderr[-4000:])
+    os.replace(CLI + ".tmp", CLI)
+    return CLI
+
+
 def build_lib(verbose: bool = False) -> str:
     cmd = ["hipcc", *HIPCC_FLAGS, SRC, "-lrocblas", "-lrocsolver", "-o", OUT + ".tmp"]
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -30,3 +45,4 @@ def build_lib(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build_lib(verbose=True))
+    print(build_cli())

@@ -757,32 +757,32 @@ int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev
   CHK(ensure_block(c, m, bps, false));
   const int64_t m_pad = round_up(m, 64);
   const int np = c->n_pad;
-  CHK(ensure(c->grm_z, static_cast<size_t>(m_pad) * np * 3 * sizeof(__bf16)));
+  CHK(ensure(c->grm_z, static_cast<size_t>(m_pad) * np * (sizeof(float) + sizeof(__bf16))));
   int rc;
   const uint8_t* rd = stage_rows(c, rows, on_dev, m, bps, &rc);
   if (rc) return rc;
-  __bf16* zhi = c->grm_z.as<__bf16>();
-  __bf16* zlo = zhi + static_cast<size_t>(m_pad) * np;
-  __bf16* mis = zlo + static_cast<size_t>(m_pad) * np;
+  float* z32 = c->grm_z.as<float>();
+  __bf16* mis = reinterpret_cast<__bf16*>(z32 + static_cast<size_t>(m_pad) * np);
   DBuf used;
-  CHK(ensure(used, sizeof(unsigned long long)));
-  HIPC(hipMemsetAsync(used.p, 0, sizeof(unsigned long long), c->stream));
+  CHK(ensure(used, 2 * sizeof(unsigned long long)));
+  HIPC(hipMemsetAsync(used.p, 0, 2 * sizeof(unsigned long long), c->stream));
   hipLaunchKernelGGL(k_grm_standardize, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream, rd, m, m_pad,
-                     bps, c->n, np, c->n_file, c->sample_col.as<int32_t>(), c->keep.as<uint8_t>(), zhi, zlo, mis,
-                     c->grm_cnt.as<int32_t>(), used.as<unsigned long long>(), c->impute, c->max_miss, c->maf_min);
+                     bps, c->n, np, c->n_file, c->sample_col.as<int32_t>(), z32, mis, c->grm_cnt.as<int32_t>(),
+                     used.as<unsigned long long>(), c->impute, c->max_miss, c->maf_min);
   CHK(check_launch("k_grm_standardize"));
+  unsigned long long uu[2] = {0, 0};
+  HIPC(hipMemcpyAsync(uu, used.p, sizeof(uu), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
   {
     Timed t(c, "grm");
     dim3 grid(static_cast<unsigned>(np / 64), static_cast<unsigned>(np / 64));
-    hipLaunchKernelGGL(k_grm_gram, grid, dim3(256), 0, c->stream, zhi, zlo, mis, static_cast<int>(m_pad), np,
-                       c->grm_acc.as<double>(), c->grm_miss.as<int32_t>());
+    hipLaunchKernelGGL(k_grm_gram, grid, dim3(256), 0, c->stream, z32, mis, static_cast<int>(m_pad), np,
+                       c->grm_acc.as<double>(), c->grm_miss.as<int32_t>(), uu[1] ? 1 : 0);
     CHK(check_launch("k_grm_gram"));
     CHK(t.done());
   }
-  unsigned long long u = 0;
-  HIPC(hipMemcpyAsync(&u, used.p, sizeof(u), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
-  c->grm_m += static_cast<int64_t>(u);
+  c->grm_m += static_cast<int64_t>(uu[0]);
   return EQTL_OK;
 }
 

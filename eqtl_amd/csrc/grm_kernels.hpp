@@ -1,12 +1,12 @@
 // K2: GRM (compute_grm_from, grm.cpp:112-224).
 //   k_grm_standardize — decode each SNP row, apply the GRM's SNP rules
 //       (decode_row filters, then p = mean/2 in [1e-8, 1-1e-8], sd >= 1e-12),
-//       write z = (g - mean)/sd (missing → 0) split into bf16 hi/lo and the
-//       0/1 missing indicator, sample-major (SNP-contiguous) for the MFMA
+//       write z = (g - mean)/sd (missing → 0) in fp32 and the 0/1 missing
+//       indicator in bf16, sample-major (SNP-contiguous) for the MFMA
 //       operands; count per-sample missing calls of used SNPs.
-//   k_grm_gram — lower block-triangle of Z^T Z (3 bf16 MFMA products, fp32
-//       within 256-SNP chunks, fp64 across chunks) and of M^T M (joint
-//       missing counts, exact), accumulated into FP64 / int32 tiles.
+//   k_grm_gram — lower block-triangle of Z^T Z (fp32 MFMA, fp32 within
+//       256-SNP chunks, fp64 across chunks) and of M^T M (joint missing
+//       counts, bf16 MFMA on 0/1, exact), accumulated into FP64 / int32 tiles.
 //   k_grm_finish — per-pair GCTA divisor m - miss_i - miss_j + joint_ij
 //       (diagonal m - miss_i), symmetric K and packed N.
 #pragma once
@@ -19,8 +19,7 @@ namespace eqtl_amd {
 
 __global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, int64_t m, int64_t m_pad, int bps,
                                                          int n, int n_pad, int n_file, const int32_t* sample_col,
-                                                         const uint8_t* /*unused*/, __bf16* zhi, __bf16* zlo,
-                                                         __bf16* mis, int32_t* miss_cnt,
+                                                         float* z32, __bf16* mis, int32_t* miss_cnt,
                                                          unsigned long long* used, int impute, double max_miss,
                                                          double maf_min) {
   const int lane = threadIdx.x & 63;
@@ -75,17 +74,18 @@ __global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, in
       }
     }
     const size_t o = static_cast<size_t>(i) * m_pad + j;
-    const __bf16 h = static_cast<__bf16>(z);
-    zhi[o] = h;
-    zlo[o] = static_cast<__bf16>(z - static_cast<float>(h));
+    z32[o] = z;
     mis[o] = static_cast<__bf16>(mi);
   }
-  if (use && lane == 0) atomicAdd(used, 1ull);
+  if (use && lane == 0) {
+    atomicAdd(used, 1ull);
+    if (n_miss > 0) atomicAdd(used + 1, 1ull);
+  }
 }
 
 // One workgroup per 64x64 tile (by >= bx, lower block triangle).
-__global__ __launch_bounds__(256) void k_grm_gram(const __bf16* zhi, const __bf16* zlo, const __bf16* mis,
-                                                  int m_pad, int n_pad, double* acc, int32_t* joint) {
+__global__ __launch_bounds__(256) void k_grm_gram(const float* z32, const __bf16* mis, int m_pad, int n_pad,
+                                                  double* acc, int32_t* joint, int any_missing) {
   const int bx = blockIdx.x, by = blockIdx.y;
   if (bx > by) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -105,16 +105,19 @@ __global__ __launch_bounds__(256) void k_grm_gram(const __bf16* zhi, const __bf1
     for (int cb = 0; cb < 4; ++cb) { acc32[cb] = f32x4{0, 0, 0, 0}; accm[cb] = f32x4{0, 0, 0, 0}; }
     const int kend = min(m_pad, c0k + 256);
     for (int k0 = c0k; k0 < kend; k0 += 32) {
-      const bf16x8 ah = load8_bf16(zhi + arow + k0), al = load8_bf16(zlo + arow + k0);
-      const bf16x8 am = load8_bf16(mis + arow + k0);
+      const f32x8 az = load8(z32 + arow + k0);
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
         const size_t bo = static_cast<size_t>(c0 + cb * 16 + (lane & 15)) * m_pad + kk + k0;
-        const bf16x8 bh = load8_bf16(zhi + bo), bl = load8_bf16(zlo + bo), bm = load8_bf16(mis + bo);
-        acc32[cb] = mfma_bf16(ah, bh, acc32[cb]);
-        acc32[cb] = mfma_bf16(ah, bl, acc32[cb]);
-        acc32[cb] = mfma_bf16(al, bh, acc32[cb]);
-        accm[cb] = mfma_bf16(am, bm, accm[cb]);
+        acc32[cb] = mfma_f32x8(az, load8(z32 + bo), acc32[cb]);
+      }
+      if (any_missing) {
+        const bf16x8 am = load8_bf16(mis + arow + k0);
+#pragma unroll
+        for (int cb = 0; cb < 4; ++cb) {
+          const size_t bo = static_cast<size_t>(c0 + cb * 16 + (lane & 15)) * m_pad + kk + k0;
+          accm[cb] = mfma_bf16(am, load8_bf16(mis + bo), accm[cb]);
+        }
       }
     }
 #pragma unroll

@@ -1,0 +1,923 @@
+// Host side of the engine (see eqtl_host.hpp). Each function names the
+// reference code whose behaviour it mirrors.
+#include "eqtl_host.hpp"
+
+#include <getopt.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <limits>
+#include <numeric>
+#include <sstream>
+#include <unordered_set>
+
+namespace eqtl {
+
+// ============================================================ util
+void die(const std::string& msg) {
+  std::cerr << "[E] " << msg << "\n";
+  throw std::runtime_error(msg);
+}
+void warn(const std::string& msg) { std::cerr << "[W] " << msg << "\n"; }
+void info(const std::string& msg) { std::cerr << "[I] " << msg << "\n"; }
+
+std::string trim(const std::string& s) {
+  size_t a = 0, b = s.size();
+  while (a < b && std::isspace(static_cast<unsigned char>(s[a]))) ++a;
+  while (b > a && std::isspace(static_cast<unsigned char>(s[b - 1]))) --b;
+  return s.substr(a, b - a);
+}
+
+std::vector<std::string> split_ws(const std::string& s) {
+  std::vector<std::string> out;
+  std::istringstream iss(s);
+  std::string t;
+  while (iss >> t) out.push_back(t);
+  return out;
+}
+
+std::vector<std::string> split_char(const std::string& s, char c) {
+  std::vector<std::string> out;
+  std::string cur;
+  for (char ch : s) {
+    if (ch == c) {
+      out.push_back(cur);
+      cur.clear();
+    } else {
+      cur.push_back(ch);
+    }
+  }
+  out.push_back(cur);
+  return out;
+}
+
+static std::vector<std::string> split_line(const std::string& line) {  // pheno.cpp:11-15
+  if (line.find('\t') != std::string::npos) return split_char(line, '\t');
+  return split_ws(line);
+}
+
+std::string chrom_key(const std::string& chrom) {  // util.cpp:68-83
+  std::string s = chrom;
+  if (s.size() >= 3 && std::tolower(static_cast<unsigned char>(s[0])) == 'c' &&
+      std::tolower(static_cast<unsigned char>(s[1])) == 'h' && std::tolower(static_cast<unsigned char>(s[2])) == 'r')
+    s = s.substr(3);
+  size_t start = 0;
+  while (start + 1 < s.size() && s[start] == '0') ++start;
+  return s.substr(start);
+}
+
+bool chrom_equal(const std::string& a, const std::string& b) { return a == b || chrom_key(a) == chrom_key(b); }
+
+std::vector<std::string> intersect_order(const std::vector<std::string>& prefer,
+                                         const std::vector<std::string>& other) {
+  std::unordered_set<std::string> m(other.begin(), other.end());
+  std::vector<std::string> out;
+  for (const auto& id : prefer)
+    if (m.count(id)) out.push_back(id);
+  return out;
+}
+
+static void assert_unique(const std::vector<std::string>& ids, const std::string& what) {
+  std::unordered_set<std::string> seen;
+  for (const auto& id : ids)
+    if (!seen.insert(id).second) die(what + ": duplicate id: " + id);
+}
+
+std::string model_str(Model m) {
+  switch (m) {
+    case Model::Lm: return "lm";
+    case Model::Glm: return "glm";
+    case Model::Lmm: return "lmm";
+    case Model::Glmm: return "glmm";
+  }
+  return "?";
+}
+
+// ============================================================ options (options.cpp:132-302)
+static std::vector<Model> parse_models(const std::string& s) {
+  std::vector<Model> out;
+  for (auto& p : split_char(s, ',')) {
+    const auto t = trim(p);
+    if (t.empty()) continue;
+    if (t == "lm") out.push_back(Model::Lm);
+    else if (t == "lmm") out.push_back(Model::Lmm);
+    else if (t == "glm" || t == "nb_glm" || t == "glmm" || t == "p_glmm")
+      die("--model " + t + " is not provided by this engine (lm, lmm)");
+    else die("invalid --model: " + t);
+  }
+  if (out.empty()) die("empty --model");
+  return out;
+}
+
+static Mode parse_mode(const std::string& s) {
+  if (s == "cis") return Mode::Cis;
+  if (s == "trans") return Mode::Trans;
+  if (s == "all") return Mode::All;
+  if (s == "gw" || s == "gwas") return Mode::Gw;
+  die("invalid --mode: " + s + " (cis|trans|all|gw)");
+}
+
+int parse_options(int argc, char** argv, Options& opt) {
+  static struct option long_opts[] = {
+      {"vcf", required_argument, 0, 'v'},       {"bfile", required_argument, 0, 'b'},
+      {"pheno", required_argument, 0, 'e'},     {"out", required_argument, 0, 'o'},
+      {"gff", required_argument, 0, 'g'},       {"covar", required_argument, 0, 'c'},
+      {"grm", required_argument, 0, 'k'},       {"gff-id-key", required_argument, 0, 1001},
+      {"make-grm", no_argument, 0, 1002},       {"mode", required_argument, 0, 'm'},
+      {"model", required_argument, 0, 1003},    {"window", required_argument, 0, 'w'},
+      {"pval-cis", required_argument, 0, 1004}, {"pval-trans", required_argument, 0, 1005},
+      {"miss-hand", required_argument, 0, 1006}, {"max-miss", required_argument, 0, 1012},
+      {"maf", required_argument, 0, 1013},      {"thread", required_argument, 0, 't'},
+      {"threads", required_argument, 0, 't'},   {"perm", required_argument, 0, 1008},
+      {"permutations", required_argument, 0, 1008}, {"perm-trans-thr", required_argument, 0, 1020},
+      {"perm-trans-top", required_argument, 0, 1021}, {"seed", required_argument, 0, 1009},
+      {"disable-beta-approx", no_argument, 0, 1010}, {"help", no_argument, 0, 'h'},
+      {"precision", required_argument, 0, 2001}, {"delta", required_argument, 0, 2002},
+      {"device", required_argument, 0, 2003},   {0, 0, 0, 0}};
+  if (argc <= 1) return 2;
+  optind = 1;
+  int c, idx;
+  while ((c = getopt_long(argc, argv, "v:b:e:o:g:c:k:m:w:t:h", long_opts, &idx)) != -1) {
+    switch (c) {
+      case 'v': opt.vcf = optarg; break;
+      case 'b': opt.bfile = optarg; break;
+      case 'e': opt.pheno = optarg; break;
+      case 'o': opt.out = optarg; break;
+      case 'g': opt.gff = optarg; break;
+      case 'c': opt.covar = optarg; break;
+      case 'k': opt.grm = optarg; break;
+      case 'm': opt.mode = parse_mode(optarg); break;
+      case 'w': opt.window = std::atoi(optarg); break;
+      case 't': opt.threads = std::max(1, std::atoi(optarg)); break;
+      case 'h': return 2;
+      case 1001: opt.gff_id_key = optarg; break;
+      case 1002: opt.make_grm = true; break;
+      case 1003: opt.models = parse_models(optarg); break;
+      case 1004: opt.pval_cis = std::atof(optarg); break;
+      case 1005: opt.pval_trans = std::atof(optarg); break;
+      case 1006:
+        if (std::string(optarg) == "filter") opt.miss = MissHand::Filter;
+        else if (std::string(optarg) == "impute") opt.miss = MissHand::Impute;
+        else die("invalid --miss-hand (filter|impute)");
+        break;
+      case 1008: opt.perm = std::atoi(optarg); break;
+      case 1009: opt.seed = std::atoi(optarg); break;
+      case 1010: opt.disable_beta_approx = true; break;
+      case 1012:
+        opt.max_miss = std::atof(optarg);
+        if (opt.max_miss < 0.0 || opt.max_miss > 1.0) die("--max-miss must be in [0,1]");
+        break;
+      case 1013:
+        opt.maf = std::atof(optarg);
+        if (opt.maf < 0.0 || opt.maf > 0.5) die("--maf must be in [0,0.5]");
+        break;
+      case 1020:
+        opt.perm_trans_thr = std::atof(optarg);
+        if (opt.perm_trans_thr <= 0 || opt.perm_trans_thr > 1) die("--perm-trans-thr must be in (0,1]");
+        break;
+      case 1021:
+        opt.perm_trans_top = std::atoi(optarg);
+        if (opt.perm_trans_top < 1) die("--perm-trans-top must be >= 1");
+        break;
+      case 2001:
+        if (std::string(optarg) == "fp32") opt.precision = EQTL_PREC_FP32;
+        else if (std::string(optarg) == "bf16x3") opt.precision = EQTL_PREC_BF16X3;
+        else die("invalid --precision (fp32|bf16x3)");
+        break;
+      case 2002:
+        if (std::string(optarg) == "ref") opt.delta_mode = EQTL_DELTA_REF;
+        else if (std::string(optarg) == "newton") opt.delta_mode = EQTL_DELTA_NEWTON;
+        else die("invalid --delta (ref|newton)");
+        break;
+      case 2003: opt.device = std::atoi(optarg); break;
+      default: return 1;
+    }
+  }
+  if (!opt.vcf.empty()) die("VCF/BCF input is not provided by this engine; use -b/--bfile");
+  if (opt.bfile.empty()) {
+    std::cerr << "[E] need -b/--bfile\n";
+    return 1;
+  }
+  if (!opt.make_grm && opt.pheno.empty()) {
+    std::cerr << "[E] missing -e/--pheno (or use --make-grm)\n";
+    return 1;
+  }
+  if (opt.window < 0) die("--window must be >= 0");
+  return 0;
+}
+
+// ============================================================ PLINK bed
+static std::vector<std::string> read_fam(const std::string& path) {  // plink_bed.cpp:26-40
+  std::ifstream in(path);
+  if (!in) die("cannot open " + path);
+  std::vector<std::string> s;
+  std::string line;
+  while (std::getline(in, line)) {
+    if (line.empty()) continue;
+    auto t = split_ws(line);
+    if (t.size() < 2) die("malformed fam line (need FID IID ...): " + path);
+    s.push_back(t[1]);
+  }
+  if (s.empty()) die("empty fam: " + path);
+  assert_unique(s, "plink fam IID");
+  return s;
+}
+
+static std::vector<BimSite> read_bim(const std::string& path) {  // plink_bed.cpp:42-160
+  std::ifstream in(path);
+  if (!in) die("cannot open " + path);
+  std::vector<BimSite> v;
+  std::string line;
+  while (std::getline(in, line)) {
+    auto t = split_ws(line);
+    if (t.empty()) continue;
+    if (t.size() < 6) die("malformed bim line in: " + path);
+    BimSite s;
+    s.chrom = t[0];
+    s.id = t[1];
+    char* end = nullptr;
+    s.pos = std::strtoll(t[3].c_str(), &end, 10);
+    if (!end || *end) die("malformed bim pos in: " + path);
+    s.a1 = t[4];
+    s.a2 = t[5];
+    v.push_back(std::move(s));
+  }
+  if (v.empty()) die("empty bim: " + path);
+  // one contiguous, position-sorted block per chromosome (plink_bed.cpp:143-159)
+  std::unordered_set<std::string> seen;
+  for (size_t i = 0; i < v.size(); ++i) {
+    const bool first = i == 0 || !chrom_equal(v[i - 1].chrom, v[i].chrom);
+    if (first) {
+      if (!seen.insert(chrom_key(v[i].chrom)).second)
+        die("plink bim: chromosome " + v[i].chrom + " appears in more than one block");
+    } else if (v[i - 1].pos > v[i].pos) {
+      die("plink bim: positions not sorted within chromosome " + v[i].chrom);
+    }
+  }
+  return v;
+}
+
+void PlinkBed::open(const std::string& prefix) {  // plink_bed.cpp:187-230
+  prefix_ = prefix;
+  samples_ = read_fam(prefix + ".fam");
+  sites_ = read_bim(prefix + ".bim");
+  bed_ = std::make_unique<std::ifstream>(prefix + ".bed", std::ios::binary);
+  if (!*bed_) die("cannot open " + prefix + ".bed");
+  unsigned char magic[3];
+  bed_->read(reinterpret_cast<char*>(magic), 3);
+  if (bed_->gcount() != 3) die("cannot read bed magic: " + prefix + ".bed");
+  if (magic[0] != 0x6c || magic[1] != 0x1b) die("not a PLINK1 .bed (bad magic): " + prefix + ".bed");
+  if (magic[2] != 0x01) die("only SNP-major PLINK .bed is supported: " + prefix + ".bed");
+  bps_ = static_cast<int32_t>((samples_.size() + 3) / 4);
+  bed_->seekg(0, std::ios::end);
+  const uint64_t fsz = static_cast<uint64_t>(bed_->tellg());
+  if (fsz < 3ull + static_cast<uint64_t>(bps_) * sites_.size()) die("bed size too small for bim/fam: " + prefix + ".bed");
+  info("bfile: " + std::to_string(samples_.size()) + " samples, " + std::to_string(sites_.size()) + " SNPs [" +
+       prefix + "]");
+}
+
+std::vector<std::string> PlinkBed::chromosomes() const {
+  std::vector<std::string> out;
+  for (size_t i = 0; i < sites_.size(); ++i)
+    if (i == 0 || !chrom_equal(sites_[i].chrom, out.back())) out.push_back(sites_[i].chrom);
+  return out;
+}
+
+void PlinkBed::read_rows(size_t j0, size_t m, std::vector<uint8_t>& buf) {
+  buf.resize(m * static_cast<size_t>(bps_));
+  bed_->clear();
+  bed_->seekg(static_cast<std::streamoff>(3 + j0 * static_cast<size_t>(bps_)), std::ios::beg);
+  bed_->read(reinterpret_cast<char*>(buf.data()), static_cast<std::streamsize>(buf.size()));
+  if (static_cast<size_t>(bed_->gcount()) != buf.size()) die("bed fread short at SNP " + std::to_string(j0));
+}
+
+// ============================================================ pheno / covar
+PhenoData load_pheno(const std::string& path) {  // pheno.cpp:17-75
+  std::ifstream in(path);
+  if (!in) die("cannot open pheno: " + path);
+  std::string line;
+  if (!std::getline(in, line)) die("empty pheno: " + path);
+  line = trim(line);
+  if (!line.empty() && line[0] == '#') line = line.substr(1);
+  auto header = split_line(line);
+  if (header.size() < 2) die("pheno header needs sample + >=1 gene");
+  PhenoData P;
+  for (size_t i = 1; i < header.size(); ++i) P.gene_ids.push_back(trim(header[i]));
+  assert_unique(P.gene_ids, "pheno genes");
+  std::vector<std::string> ids;
+  std::vector<std::vector<double>> rows;
+  std::unordered_map<std::string, size_t> first;
+  while (std::getline(in, line)) {
+    line = trim(line);
+    if (line.empty() || line[0] == '#') continue;
+    auto tok = split_line(line);
+    if (tok.size() != header.size()) die("pheno row column mismatch for sample " + (tok.empty() ? "?" : tok[0]));
+    std::vector<double> r(P.gene_ids.size());
+    for (size_t j = 0; j < r.size(); ++j) {
+      const std::string& v = tok[j + 1];
+      r[j] = (v == "NA" || v == "NaN" || v == ".") ? std::numeric_limits<double>::quiet_NaN() : std::stod(v);
+    }
+    const std::string id = trim(tok[0]);
+    auto it = first.find(id);
+    if (it != first.end()) {  // duplicate id: identical payload → keep first (util.cpp dedupe_ids)
+      const auto& a = rows[it->second];
+      for (size_t j = 0; j < a.size(); ++j)
+        if (!((std::isnan(a[j]) && std::isnan(r[j])) || a[j] == r[j]))
+          die("pheno samples: duplicate id with conflicting values: " + id);
+      warn("pheno samples: duplicate id row with identical values (kept first): " + id);
+      continue;
+    }
+    first.emplace(id, rows.size());
+    ids.push_back(id);
+    rows.push_back(std::move(r));
+  }
+  if (ids.empty()) die("no samples in pheno");
+  P.sample_ids = ids;
+  P.Y.resize(ids.size() * P.gene_ids.size());
+  for (size_t r = 0; r < ids.size(); ++r)
+    for (size_t j = 0; j < P.gene_ids.size(); ++j) P.Y[j * ids.size() + r] = rows[r][j];
+  info("pheno: " + std::to_string(P.sample_ids.size()) + " samples, " + std::to_string(P.gene_ids.size()) + " genes");
+  return P;
+}
+
+void slice_pheno(PhenoData& p, const std::vector<std::string>& sample_ids) {  // pheno.cpp:77-87
+  std::unordered_map<std::string, size_t> m;
+  for (size_t i = 0; i < p.sample_ids.size(); ++i) m[p.sample_ids[i]] = i;
+  std::vector<double> Y(sample_ids.size() * p.G());
+  for (size_t i = 0; i < sample_ids.size(); ++i) {
+    auto it = m.find(sample_ids[i]);
+    if (it == m.end()) die("internal: sample missing in pheno slice");
+    for (size_t j = 0; j < p.G(); ++j) Y[j * sample_ids.size() + i] = p.Y[j * p.n() + it->second];
+  }
+  p.sample_ids = sample_ids;
+  p.Y = std::move(Y);
+}
+
+CovData load_covar(const std::string& path, const std::vector<std::string>& sample_ids) {  // pheno.cpp:89-196
+  CovData C;
+  C.sample_ids = sample_ids;
+  const size_t n = sample_ids.size();
+  if (path.empty()) {
+    C.cov_names = {"intercept"};
+    C.p = 1;
+    C.X.assign(n, 1.0);
+    return C;
+  }
+  std::ifstream in(path);
+  if (!in) die("cannot open covar: " + path);
+  std::string line;
+  std::vector<std::string> file_ids, names;
+  std::vector<std::vector<double>> rows;
+  bool first = true;
+  auto is_num = [](const std::string& s) {
+    if (s.empty()) return false;
+    char* end = nullptr;
+    std::strtod(s.c_str(), &end);
+    return end && *end == '\0';
+  };
+  while (std::getline(in, line)) {
+    line = trim(line);
+    if (line.empty() || line[0] == '#') continue;
+    auto tok = split_line(line);
+    if (tok.size() < 2) continue;
+    if (first) {
+      first = false;
+      if (!is_num(tok[1])) {
+        names.assign(tok.begin() + 1, tok.end());
+        continue;
+      }
+    }
+    file_ids.push_back(tok[0]);
+    std::vector<double> r;
+    for (size_t j = 1; j < tok.size(); ++j) r.push_back(std::stod(tok[j]));
+    rows.push_back(std::move(r));
+  }
+  if (rows.empty()) die("no covariates in " + path);
+  const int nc = static_cast<int>(rows[0].size());
+  if (names.empty())
+    for (int j = 0; j < nc; ++j) names.push_back("cov" + std::to_string(j + 1));
+  std::unordered_map<std::string, size_t> fmap;
+  for (size_t i = 0; i < file_ids.size(); ++i) fmap.emplace(file_ids[i], i);
+  std::vector<double> raw(n * nc);
+  for (size_t i = 0; i < n; ++i) {
+    auto it = fmap.find(sample_ids[i]);
+    if (it == fmap.end()) die("sample " + sample_ids[i] + " missing in covar");
+    if (static_cast<int>(rows[it->second].size()) != nc) die("covar row width mismatch");
+    for (int j = 0; j < nc; ++j) raw[j * n + i] = rows[it->second][j];
+  }
+  bool has_int = false;
+  for (int j = 0; j < nc && !has_int; ++j) {
+    bool ok = true;
+    for (size_t i = 0; i < n; ++i)
+      if (std::fabs(raw[j * n + i] - 1.0) > 1e-12) { ok = false; break; }
+    has_int = ok;
+  }
+  if (!has_int) {
+    C.p = nc + 1;
+    C.X.assign(n, 1.0);
+    C.X.insert(C.X.end(), raw.begin(), raw.end());
+    C.cov_names.push_back("intercept");
+    for (auto& s : names) C.cov_names.push_back(s);
+  } else {
+    C.p = nc;
+    C.X = raw;
+    C.cov_names = names;
+  }
+  info("covar: " + std::to_string(C.p) + " columns (incl intercept if added)");
+  return C;
+}
+
+// ============================================================ GFF3 genes
+static std::string attr_value(const std::string& attrs, const std::string& key) {
+  for (const auto& kv : split_char(attrs, ';')) {
+    const auto eq = kv.find('=');
+    if (eq == std::string::npos) continue;
+    if (trim(kv.substr(0, eq)) == key) {
+      std::string v = trim(kv.substr(eq + 1));
+      const auto comma = v.find(',');
+      return comma == std::string::npos ? v : v.substr(0, comma);
+    }
+  }
+  return {};
+}
+
+std::unordered_map<std::string, GeneLoc> load_gff_tss(const std::string& path,
+                                                      const std::string& id_key) {  // annot.cpp:26-71
+  std::ifstream in(path);
+  if (!in) die("cannot parse gff: " + path);
+  std::unordered_map<std::string, GeneLoc> out;
+  std::unordered_set<std::string> multi;
+  std::string line;
+  int n_gene = 0;
+  while (std::getline(in, line)) {
+    if (line.empty() || line[0] == '#') continue;
+    auto f = split_char(line, '\t');
+    if (f.size() < 9 || f[2] != "gene") continue;
+    ++n_gene;
+    std::string id;
+    if (!id_key.empty()) id = attr_value(f[8], id_key);
+    else {
+      id = attr_value(f[8], "ID");
+      if (id.empty()) id = attr_value(f[8], "Name");
+      if (id.empty()) id = attr_value(f[8], "gene_id");
+    }
+    if (id.empty()) continue;
+    if (id.rfind("gene:", 0) == 0) id = id.substr(5);
+    const char strand = (f[6] == "-" || f[6] == "+") ? f[6][0] : '+';
+    const int64_t start = std::atoll(f[3].c_str()), end = std::atoll(f[4].c_str());
+    if (out.count(id) || multi.count(id)) {
+      if (out.count(id)) {
+        warn("GFF multi-hit gene skipped: " + id);
+        out.erase(id);
+        multi.insert(id);
+      }
+      continue;
+    }
+    GeneLoc g;
+    g.id = id;
+    g.chrom = f[0];
+    g.start = start;
+    g.end = end;
+    g.tss = strand == '-' ? end : start;
+    g.strand = strand;
+    g.ok = true;
+    out.emplace(id, std::move(g));
+  }
+  info("gff: " + std::to_string(n_gene) + " gene features, " + std::to_string(out.size()) + " unique ids with TSS");
+  return out;
+}
+
+// ============================================================ outputs (output.cpp:8-110)
+void write_pairs_header(std::ostream& os, Model) {
+  os << "gene\tsnp\tchrom\tpos\tref\talt\taf\tbeta\tse\tstat\tp\tr2\tn\ttss_dist\tscope\n";
+}
+
+void write_pair_line(std::ostream& os, const AssocHit& h, Model, const std::string& scope) {
+  char buf[2048];
+  int n;
+  if (h.has_tss_dist)
+    n = std::snprintf(buf, sizeof(buf), "%s\t%s\t%s\t%lld\t%s\t%s\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%d\t%.10g\t%s\n",
+                      h.gene.c_str(), h.snp.c_str(), h.chrom.c_str(), static_cast<long long>(h.pos), h.ref.c_str(),
+                      h.alt.c_str(), h.maf, h.beta, h.se, h.stat, h.p, h.r2, h.n, h.tss_dist, scope.c_str());
+  else
+    n = std::snprintf(buf, sizeof(buf), "%s\t%s\t%s\t%lld\t%s\t%s\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%d\tNA\t%s\n",
+                      h.gene.c_str(), h.snp.c_str(), h.chrom.c_str(), static_cast<long long>(h.pos), h.ref.c_str(),
+                      h.alt.c_str(), h.maf, h.beta, h.se, h.stat, h.p, h.r2, h.n, scope.c_str());
+  if (n > 0 && static_cast<size_t>(n) < sizeof(buf)) os.write(buf, n);
+  else die("pair line too long");
+}
+
+void write_region_header(std::ostream& os) {
+  os << "gene\tchrom\ttss\tn_tested\tn_sig\tacat_p\tq_bh\tp_emp\tp_beta\tbeta_shape1\tbeta_shape2\n";
+}
+
+static void fmt_na_or(char* out, size_t n, double v) {
+  if (!std::isfinite(v)) std::snprintf(out, n, "NA");
+  else std::snprintf(out, n, "%.10g", v);
+}
+
+void write_region_line(std::ostream& os, const GeneSummary& g) {
+  char p_emp[32], p_beta[32], q_bh[32], buf[1024];
+  fmt_na_or(p_emp, sizeof(p_emp), g.p_emp);
+  fmt_na_or(p_beta, sizeof(p_beta), g.p_beta);
+  fmt_na_or(q_bh, sizeof(q_bh), g.q_bh);
+  const int n = std::snprintf(buf, sizeof(buf), "%s\t%s\t%lld\t%d\t%d\t%.10g\t%s\t%s\t%s\t%.10g\t%.10g\n", g.gene.c_str(),
+                              g.chrom.c_str(), static_cast<long long>(g.tss), g.n_tested, g.n_sig, g.acat_p, q_bh,
+                              p_emp, p_beta, g.beta_shape1, g.beta_shape2);
+  if (n > 0 && static_cast<size_t>(n) < sizeof(buf)) os.write(buf, n);
+}
+
+void bh_adjust(std::vector<GeneSummary>& gs) {  // scan_common.cpp:201-217
+  const int m = static_cast<int>(gs.size());
+  if (m == 0) return;
+  std::vector<int> idx(m);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::sort(idx.begin(), idx.end(), [&](int a, int b) { return gs[a].acat_p < gs[b].acat_p; });
+  double prev = 1.0;
+  for (int rank = m; rank >= 1; --rank) {
+    auto& g = gs[idx[rank - 1]];
+    double q = g.acat_p * static_cast<double>(m) / static_cast<double>(rank);
+    if (q > prev) q = prev;
+    if (q > 1.0) q = 1.0;
+    g.q_bh = q;
+    prev = q;
+  }
+}
+
+// ============================================================ GRM files (grm.cpp:10-108)
+Grm load_grm_gcta(const std::string& prefix) {
+  Grm g;
+  std::ifstream idf(prefix + ".grm.id");
+  if (!idf) die("cannot open " + prefix + ".grm.id");
+  std::string line;
+  while (std::getline(idf, line)) {
+    auto t = split_ws(line);
+    if (t.size() >= 2) g.ids.push_back(t[1]);
+    else if (t.size() == 1) g.ids.push_back(t[0]);
+  }
+  const int n = g.n();
+  if (n == 0) die("empty GRM id file");
+  const uint64_t ntri = static_cast<uint64_t>(n) * (n + 1) / 2;
+  std::ifstream bf(prefix + ".grm.bin", std::ios::binary);
+  if (!bf) die("cannot open " + prefix + ".grm.bin");
+  std::vector<float> buf(ntri);
+  bf.read(reinterpret_cast<char*>(buf.data()), static_cast<std::streamsize>(ntri * sizeof(float)));
+  if (static_cast<uint64_t>(bf.gcount()) != ntri * sizeof(float)) die("GRM bin size mismatch");
+  g.K.assign(static_cast<size_t>(n) * n, 0.0);
+  size_t k = 0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) {
+      const double v = buf[k++];
+      g.K[static_cast<size_t>(j) * n + i] = v;
+      g.K[static_cast<size_t>(i) * n + j] = v;
+    }
+  info("grm: " + std::to_string(n) + " samples from " + prefix);
+  return g;
+}
+
+void write_grm_gcta(const std::string& prefix, const Grm& g) {
+  const int n = g.n();
+  std::ofstream idf(prefix + ".grm.id");
+  if (!idf) die("cannot write " + prefix + ".grm.id");
+  for (const auto& id : g.ids) idf << id << '\t' << id << '\n';
+  const uint64_t ntri = static_cast<uint64_t>(n) * (n + 1) / 2;
+  std::vector<float> buf(ntri), nb(ntri);
+  size_t k = 0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j <= i; ++j) {
+      buf[k] = static_cast<float>(g.K[static_cast<size_t>(j) * n + i]);
+      nb[k] = static_cast<float>(g.N_packed.empty() ? 0 : g.N_packed[k]);
+      ++k;
+    }
+  std::ofstream bf(prefix + ".grm.bin", std::ios::binary);
+  if (!bf) die("cannot write " + prefix + ".grm.bin");
+  bf.write(reinterpret_cast<const char*>(buf.data()), static_cast<std::streamsize>(ntri * sizeof(float)));
+  std::ofstream nf(prefix + ".grm.N.bin", std::ios::binary);
+  if (!nf) die("cannot write " + prefix + ".grm.N.bin");
+  nf.write(reinterpret_cast<const char*>(nb.data()), static_cast<std::streamsize>(ntri * sizeof(float)));
+  info("GRM written: " + prefix + ".grm.{id,bin,N.bin}");
+}
+
+Grm slice_grm(const Grm& g, const std::vector<std::string>& sample_ids) {
+  std::unordered_map<std::string, int> m;
+  for (int i = 0; i < g.n(); ++i) m[g.ids[i]] = i;
+  Grm o;
+  o.ids = sample_ids;
+  const int n = static_cast<int>(sample_ids.size());
+  std::vector<int> ix(n);
+  for (int i = 0; i < n; ++i) {
+    auto it = m.find(sample_ids[i]);
+    if (it == m.end()) die("GRM missing sample: " + sample_ids[i]);
+    ix[i] = it->second;
+  }
+  o.K.resize(static_cast<size_t>(n) * n);
+  for (int c = 0; c < n; ++c)
+    for (int r = 0; r < n; ++r) o.K[static_cast<size_t>(c) * n + r] = g.K[static_cast<size_t>(ix[c]) * g.n() + ix[r]];
+  return o;
+}
+
+// ============================================================ engine
+Engine::Engine(int device) { check(eqtl_gpu_open(device, &ctx_), "eqtl_gpu_open"); }
+Engine::~Engine() {
+  if (ctx_) eqtl_gpu_close(ctx_);
+}
+void Engine::check(int rc, const char* what) const {
+  if (rc != EQTL_OK) die(std::string(what) + ": " + eqtl_gpu_last_error());
+}
+
+static std::vector<int32_t> sample_cols(const PlinkBed& bed, const std::vector<std::string>& ids) {
+  std::unordered_map<std::string, int> m;
+  for (size_t i = 0; i < bed.samples().size(); ++i) m[bed.samples()[i]] = static_cast<int>(i);
+  std::vector<int32_t> col;
+  for (const auto& id : ids) {
+    auto it = m.find(id);
+    if (it == m.end()) die("sample not in fam: " + id);
+    col.push_back(it->second);
+  }
+  return col;
+}
+
+static constexpr size_t kBlockSnps = 16384;
+
+Grm compute_grm(Engine& eng, PlinkBed& bed, const std::vector<std::string>& sample_ids, MissHand miss,
+                double max_miss, double maf) {
+  const auto col = sample_cols(bed, sample_ids);
+  eng.check(eqtl_gpu_set_samples(eng.ctx(), static_cast<int32_t>(bed.n_file()), col.data(),
+                                 static_cast<int32_t>(col.size())), "set_samples");
+  eng.check(eqtl_gpu_set_policy(eng.ctx(), miss == MissHand::Impute, max_miss, maf), "set_policy");
+  eng.check(eqtl_gpu_grm_begin(eng.ctx()), "grm_begin");
+  std::vector<uint8_t> buf;
+  for (size_t j0 = 0; j0 < bed.sites().size(); j0 += kBlockSnps) {
+    const size_t m = std::min(kBlockSnps, bed.sites().size() - j0);
+    bed.read_rows(j0, m, buf);
+    eng.check(eqtl_gpu_grm_accumulate(eng.ctx(), buf.data(), 0, static_cast<int64_t>(m), bed.bytes_per_snp()),
+              "grm_accumulate");
+  }
+  Grm g;
+  g.ids = sample_ids;
+  const int n = static_cast<int>(sample_ids.size());
+  g.K.resize(static_cast<size_t>(n) * n);
+  g.N_packed.resize(static_cast<size_t>(n) * (n + 1) / 2);
+  int64_t used = 0;
+  const int rc = eqtl_gpu_grm_finish(eng.ctx(), g.K.data(), g.N_packed.data(), &used);
+  if (rc == EQTL_E_ARG) die("no SNPs for GRM");
+  eng.check(rc, "grm_finish");
+  info("computed GRM from " + std::to_string(used) + " SNPs");
+  return g;
+}
+
+void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model, const std::string& scope,
+                    ScopeOut& out, double pthr, std::vector<TransJob>& jobs, const PhenoData& ph,
+                    const CovData& cov, const std::vector<double>* Q, const std::vector<double>* lambda) {
+  if (jobs.empty()) return;
+  const int n = static_cast<int>(ph.n());
+  const int G = static_cast<int>(jobs.size());
+  std::vector<double> Y(static_cast<size_t>(n) * G);
+  for (int k = 0; k < G; ++k)
+    std::copy(ph.Y.begin() + static_cast<size_t>(jobs[k].col) * n, ph.Y.begin() + static_cast<size_t>(jobs[k].col + 1) * n,
+              Y.begin() + static_cast<size_t>(k) * n);
+  // gene windows for the trans exclusion (in_cis_window, scan_common.hpp:65-69)
+  std::vector<std::string> chroms = bed.chromosomes();
+  auto chrom_id = [&](const std::string& c) -> int32_t {
+    for (size_t i = 0; i < chroms.size(); ++i)
+      if (chrom_equal(chroms[i], c)) return static_cast<int32_t>(i);
+    return -2;  // chromosome not in the bed: never matches
+  };
+  std::vector<int32_t> snp_chrom(bed.sites().size());
+  std::vector<int64_t> snp_pos(bed.sites().size());
+  for (size_t j = 0, ci = 0; j < bed.sites().size(); ++j) {
+    while (!chrom_equal(bed.sites()[j].chrom, chroms[ci])) ++ci;
+    snp_chrom[j] = static_cast<int32_t>(ci);
+    snp_pos[j] = bed.sites()[j].pos;
+  }
+  std::vector<int32_t> gch(G, -1);
+  std::vector<int64_t> gcs(G, 0), gce(G, 0);
+  for (int k = 0; k < G; ++k) {
+    if (scope == "trans" && jobs[k].has_loc) {
+      gch[k] = chrom_id(jobs[k].loc.chrom);
+      gcs[k] = std::max<int64_t>(1, jobs[k].loc.start - opt.window);
+      gce[k] = jobs[k].loc.end + opt.window;
+    }
+  }
+  eqtl_gpu_ctx* c = eng.ctx();
+  eng.check(eqtl_gpu_set_snp_annot(c, snp_chrom.data(), snp_pos.data(), static_cast<int64_t>(snp_pos.size())),
+            "set_snp_annot");
+  if (model == Model::Lmm) {
+    std::vector<double> delta(G);
+    eng.check(eqtl_gpu_lmm_prep(c, Y.data(), G, cov.X.data(), cov.p, opt.delta_mode, nullptr, delta.data(), nullptr),
+              "lmm_prep");
+    (void)Q;
+    (void)lambda;
+  } else {
+    eng.check(eqtl_gpu_lm_prep(c, Y.data(), G, cov.X.data(), cov.p, nullptr), "lm_prep");
+  }
+  eng.check(eqtl_gpu_set_gene_windows(c, gch.data(), gcs.data(), gce.data(), G), "set_gene_windows");
+  eng.check(eqtl_gpu_scan_begin(c, model == Model::Lmm ? EQTL_MODEL_LMM : EQTL_MODEL_LM, opt.precision, pthr),
+            "scan_begin");
+  std::vector<uint8_t> buf;
+  std::vector<eqtl_hit> hits(1 << 20);
+  const auto& sites = bed.sites();
+  auto fill = [&](AssocHit& h, const eqtl_hit& e) {
+    const TransJob& job = jobs[e.gene];
+    const BimSite& s = sites[e.snp];
+    h.gene = job.gene;
+    h.snp = s.id;
+    h.chrom = s.chrom;
+    h.pos = s.pos;
+    h.ref = s.a2;
+    h.alt = s.a1;
+    h.maf = e.af;
+    h.beta = e.beta;
+    h.se = e.se;
+    h.stat = e.stat;
+    h.p = e.p;
+    h.r2 = e.r2;
+    h.n = n;
+    if (job.has_loc && job.loc.ok) {
+      h.has_tss_dist = true;
+      h.tss_dist = static_cast<double>(s.pos - job.loc.tss);
+    }
+  };
+  size_t snp_cnt = 0;
+  for (size_t j0 = 0; j0 < sites.size(); j0 += kBlockSnps) {
+    const size_t m = std::min(kBlockSnps, sites.size() - j0);
+    bed.read_rows(j0, m, buf);
+    int64_t nh = 0;
+    int rc = eqtl_gpu_scan_block(c, buf.data(), 0, static_cast<int64_t>(j0), static_cast<int64_t>(m),
+                                 bed.bytes_per_snp(), hits.data(), static_cast<int64_t>(hits.size()), &nh);
+    if (rc == EQTL_E_CAPACITY)
+      die("more than " + std::to_string(hits.size()) + " significant pairs in one SNP block; lower --pval-trans");
+    eng.check(rc, "scan_block");
+    for (int64_t i = 0; i < nh; ++i) {  // SNP-major, gene order (scan_trans_lmm.cpp:185-188)
+      AssocHit h;
+      fill(h, hits[i]);
+      write_pair_line(out.pairs, h, model, out.tag);
+    }
+    const size_t before = snp_cnt;
+    snp_cnt += m;
+    if (before / 100000 != snp_cnt / 100000)
+      info(std::string("trans/gw ") + (model == Model::Lmm ? "LMM" : "LM") + ": SNP " + std::to_string(snp_cnt));
+  }
+  std::vector<eqtl_gene_state> st(G);
+  eng.check(eqtl_gpu_scan_end(c, st.data()), "scan_end");
+  for (int k = 0; k < G; ++k) {
+    auto& job = jobs[k];
+    job.summary.gene = job.gene;
+    if (job.has_loc && job.loc.ok) {
+      job.summary.chrom = job.loc.chrom;
+      job.summary.tss = job.loc.tss;
+    }
+    job.summary.n_tested = st[k].n_tested;
+    job.summary.n_sig = st[k].n_sig;
+    job.summary.acat_p = st[k].acat_p;
+    if (st[k].best_snp >= 0 && st[k].best_p <= pthr && st[k].best_p <= 1.0) {
+      eqtl_hit e;
+      e.snp = st[k].best_snp;
+      e.gene = k;
+      e.af = st[k].best_af;
+      e.beta = st[k].best_beta;
+      e.se = st[k].best_se;
+      e.stat = st[k].best_stat;
+      e.p = st[k].best_p;
+      e.r2 = st[k].best_r2;
+      AssocHit h;
+      fill(h, e);
+      write_pair_line(out.top, h, model, scope);
+    }
+    if (opt.perm > 0) warn("trans stage-2 permutations are not provided by this engine; p_emp = NA");
+  }
+}
+
+// ============================================================ orchestrator (scan.cpp:18-340)
+int run_make_grm(const Options& opt) {
+  PlinkBed bed;
+  bed.open(opt.bfile);
+  std::vector<std::string> samples = bed.samples();
+  if (!opt.pheno.empty()) {
+    PhenoData ph = load_pheno(opt.pheno);
+    samples = intersect_order(ph.sample_ids, bed.samples());
+    if (samples.empty()) die("no overlapping samples for GRM");
+  }
+  Engine eng(opt.device);
+  Grm g = compute_grm(eng, bed, samples, opt.miss, opt.max_miss, opt.maf);
+  write_grm_gcta(opt.out, g);
+  return 0;
+}
+
+static bool all_finite(const PhenoData& ph, int col, const CovData& cov) {
+  const size_t n = ph.n();
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(ph.Y[static_cast<size_t>(col) * n + i])) return false;
+  for (double v : cov.X)
+    if (!std::isfinite(v)) return false;
+  return true;
+}
+
+int run_eqtl(const Options& opt) {
+  PhenoData ph = load_pheno(opt.pheno);
+  PlinkBed bed;
+  bed.open(opt.bfile);
+  std::vector<std::string> order = intersect_order(ph.sample_ids, bed.samples());
+  if (order.empty()) die("no overlapping samples between pheno and fam");
+  info("overlap samples: " + std::to_string(order.size()));
+  slice_pheno(ph, order);
+  CovData cov = load_covar(opt.covar, order);
+  std::unordered_map<std::string, GeneLoc> annot;
+  const bool have_gff = !opt.gff.empty();
+  if (have_gff) annot = load_gff_tss(opt.gff, opt.gff_id_key);
+  else info("no GFF: mode ignored; genome-wide all-pairs (gw)");
+  std::vector<std::string> scopes;
+  if (!have_gff) scopes = {"gw"};
+  else if (opt.mode == Mode::Cis) scopes = {"cis"};
+  else if (opt.mode == Mode::Trans) scopes = {"trans"};
+  else if (opt.mode == Mode::Gw) scopes = {"gw"};
+  else scopes = {"cis", "trans"};
+  {
+    std::ofstream probe(opt.out + ".probe");
+    if (!probe) die("cannot write to output prefix: " + opt.out);
+    probe.close();
+    std::remove((opt.out + ".probe").c_str());
+  }
+  if (have_gff) {
+    size_t matched = 0;
+    for (const auto& g : ph.gene_ids) matched += annot.count(g);
+    info("pheno genes matched to GFF: " + std::to_string(matched) + "/" + std::to_string(ph.gene_ids.size()));
+    if (matched == 0) die("no phenotype gene IDs match the GFF annotation (check --gff-id-key and ID format)");
+  }
+  Engine eng(opt.device);
+  const auto col = sample_cols(bed, order);
+  bool any_lmm = false;
+  for (Model m : opt.models) any_lmm |= (m == Model::Lmm);
+  if (any_lmm) {
+    Grm grm;
+    if (!opt.grm.empty()) grm = slice_grm(load_grm_gcta(opt.grm), order);
+    else {
+      info("computing GRM from genotypes (overlap samples)");
+      grm = compute_grm(eng, bed, order, opt.miss, opt.max_miss, opt.maf);
+    }
+    eng.check(eqtl_gpu_set_samples(eng.ctx(), static_cast<int32_t>(bed.n_file()), col.data(),
+                                   static_cast<int32_t>(col.size())), "set_samples");
+    const int rc = eqtl_gpu_eigh(eng.ctx(), grm.K.data(), grm.n(), nullptr, nullptr);
+    if (rc == EQTL_E_SOLVER) die("GRM eigen decomposition failed");
+    eng.check(rc, "eigh");
+  } else {
+    eng.check(eqtl_gpu_set_samples(eng.ctx(), static_cast<int32_t>(bed.n_file()), col.data(),
+                                   static_cast<int32_t>(col.size())), "set_samples");
+  }
+  eng.check(eqtl_gpu_set_policy(eng.ctx(), opt.miss == MissHand::Impute, opt.max_miss, opt.maf), "set_policy");
+  for (Model model : opt.models) {
+    for (const std::string& scope : scopes) {
+      if (scope == "cis") die("cis scope is not provided by this engine build; use --mode trans or gw");
+      const std::string prefix = opt.out + "." + model_str(model) + "." + scope;
+      ScopeOut so;
+      so.tag = scope;
+      so.pairs.open(prefix + ".pairs.tsv");
+      so.top.open(prefix + ".top.tsv");
+      so.region.open(prefix + ".region.tsv");
+      if (!so.pairs || !so.top || !so.region) die("cannot open output for " + prefix);
+      write_pairs_header(so.pairs, model);
+      write_pairs_header(so.top, model);
+      write_region_header(so.region);
+      const double pthr = opt.pval_trans;
+      std::vector<TransJob> jobs;
+      for (size_t gi = 0; gi < ph.gene_ids.size(); ++gi) {
+        TransJob job;
+        job.gene = ph.gene_ids[gi];
+        job.col = static_cast<int>(gi);
+        if (have_gff) {
+          auto it = annot.find(job.gene);
+          if (it != annot.end()) {
+            job.loc = it->second;
+            job.has_loc = true;
+          } else if (scope != "gw") {
+            continue;
+          }
+        }
+        if (scope == "trans" && !job.has_loc) continue;
+        // build_gene_ready (scan_common.cpp:11-35): need >= 3 complete samples and a non-zero y
+        if (!all_finite(ph, job.col, cov))
+          die("gene " + job.gene + " has missing phenotype/covariate values: per-gene sample subsets are not "
+              "provided by this engine");
+        double ymax = 0;
+        for (size_t i = 0; i < ph.n(); ++i) ymax = std::max(ymax, std::fabs(ph.Y[gi * ph.n() + i]));
+        if (ph.n() < 3 || ymax < 1e-15) continue;
+        jobs.push_back(std::move(job));
+      }
+      info(std::string("trans/gw ") + (model == Model::Lmm ? "LMM" : "LM") + ": SNP-outer (" +
+           std::to_string(jobs.size()) + " genes)");
+      scan_snp_outer(opt, eng, bed, model, scope, so, pthr, jobs, ph, cov, nullptr, nullptr);
+      std::vector<GeneSummary> summaries;
+      for (auto& j : jobs) summaries.push_back(j.summary);
+      bh_adjust(summaries);
+      for (const auto& s : summaries) write_region_line(so.region, s);
+      info("finished " + prefix);
+    }
+  }
+  return 0;
+}
+
+}  // namespace eqtl

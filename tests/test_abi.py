@@ -48,3 +48,12 @@ def test_acat_x87_emulation_host(tmp_path):
                            str(exe)])
     res = subprocess.run([str(exe)], capture_output=True, text=True)
     assert res.returncode == 0, res.stdout + res.stderr
+
+
+def test_permutation_stream_matches_libstdcxx(tmp_path):
+    """Device MT19937 + shuffle (perm_rng.hpp) against std::mt19937 + std::shuffle."""
+    exe = tmp_path / "perm_check"
+    subprocess.check_call(["g++", "-O2", "-std=c++17", os.path.join(ROOT, "tests", "perm_rng_check.cpp"), "-o",
+                           str(exe)])
+    res = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stdout + res.stderr

@@ -1,12 +1,19 @@
 """GPU parity: the HIP engine (through the C ABI) against the FP64 oracle.
 
 Tolerances (stated per quantity):
-  decode / keep / af / n_tested / n_sig / hit sets   exact
-  LM, LMM Wald at a shared delta (both precisions)   |dt| <= 2e-5 (1 + |t|),
-      beta and se relative 1e-4 where |t| > 0.1, p relative 1e-3 (p >= 1e-300)
-  REML delta (reference search restated)             relative 1e-5
-  ACAT per gene                                      relative 1e-4
+  decode / keep / af / n_tested / best SNP / tested sets     exact
+  Wald at a shared delta, FP32 form     |dt| <= 5e-6 (1+|t|), beta/se rel 1e-4
+  Wald at a shared delta, BF16X3 form   |dt| <= 5e-5 (1+|t|), beta/se rel 5e-4
+      (beta where |t| > 0.1); p relative 1e-3 for p >= 1e-300 (both forms)
+  ACAT machinery                        acat_p equals the reference formula
+      applied to the engine's own p values (rel 1e-12, or one 2^-54 step)
+  ACAT vs oracle                        relative 1e-2 (terms -1/(pi(1-p)) for p
+      near 1 amplify the contraction's absolute t error)
+  REML delta (reference search)         f(delta_gpu) - f(delta_oracle) <= 1e-9|f|
+      and delta relative 1e-4 (the search is only defined to ~1e-8 by its own
+      floating-point noise; flat likelihoods widen that)
 """
+
 import numpy as np
 import pytest
 
@@ -17,6 +24,19 @@ from eqtl_amd import synth
 from _cases import dense_from_hits, lmm_case, rel_err, trans_lm_case
 
 pytestmark = pytest.mark.gpu
+
+TOL = {eqtl_amd.PREC_FP32: (5e-6, 1e-4), eqtl_amd.PREC_BF16X3: (5e-5, 5e-4)}
+
+
+def _acat_consistent(st, hits, G):
+    """The engine's acat_p equals AcatAcc applied to its own p values, in SNP order."""
+    order = np.lexsort((hits["snp"], hits["gene"]))
+    h = hits[order]
+    for k in range(G):
+        pk = h["p"][h["gene"] == k]
+        ref = O.acat(pk)
+        got = st["acat_p"][k]
+        assert abs(got - ref) <= 1.2e-16 + 1e-12 * ref, (k, got, ref)
 
 
 def _p_rel(pg, pr):
@@ -81,16 +101,18 @@ def test_trans_lm_parity(gpu_ctx, prec):
     t_r, t_g = ref["stat"][tested_ref], gd["stat"][tested_ref]
     dt = np.abs(t_g - t_r) / (1 + np.abs(t_r))
     print(f"LM prec={prec}: max |dt|/(1+|t|) = {dt.max():.3e}, max p rel = {_p_rel(gd['p'][tested_ref], ref['p'][tested_ref]).max():.3e}")
-    assert dt.max() < 2e-5
+    tol_t, tol_rel = TOL[prec]
+    assert dt.max() < tol_t
     assert _p_rel(gd["p"][tested_ref], ref["p"][tested_ref]).max() < 1e-3
     big = np.abs(t_r) > 0.1
-    assert rel_err(gd["beta"][tested_ref][big], ref["beta"][tested_ref][big]).max() < 1e-4
-    assert rel_err(gd["se"][tested_ref], ref["se"][tested_ref]).max() < 1e-4
+    assert rel_err(gd["beta"][tested_ref][big], ref["beta"][tested_ref][big]).max() < tol_rel
+    assert rel_err(gd["se"][tested_ref], ref["se"][tested_ref]).max() < tol_rel
     rs = ref["state"]
     assert np.array_equal(st["n_tested"], rs["n_tested"])
     assert np.array_equal(st["n_sig"], rs["n_sig"])
     assert np.array_equal(st["best_snp"], rs["best_snp"])
-    assert rel_err(st["acat_p"], rs["acat_p"]).max() < 1e-4
+    _acat_consistent(st, hits, G)
+    assert rel_err(st["acat_p"], rs["acat_p"]).max() < 1e-2
 
 
 def _lmm_run(ctx, ds, rows, Q, lam, win, prec, delta_mode, delta_in=None, pthr=1.0):
@@ -111,7 +133,13 @@ def test_reml_delta_reference_search(gpu_ctx):
     delta, rss = gpu_ctx.lmm_prep(ds.Y, ds.X(), delta_mode=eqtl_amd.DELTA_REF)
     r = rel_err(delta, ref["state"]["delta"])
     print("REML ref-search delta max rel err", r.max())
-    assert r.max() < 1e-5
+    assert r.max() < 1e-4
+    Yt = O.rotate(Q, ds.Y)
+    Xt = O.rotate(Q, ds.X())
+    for k in range(ds.Y.shape[1]):
+        fg = O.reml_negll(delta[k], Yt[:, k], Xt, lam)
+        fo = O.reml_negll(ref["state"]["delta"][k], Yt[:, k], Xt, lam)
+        assert fg - fo <= 1e-9 * abs(fo), (k, fg, fo)
 
 
 @pytest.mark.parametrize("prec", [eqtl_amd.PREC_FP32, eqtl_amd.PREC_BF16X3])
@@ -128,16 +156,18 @@ def test_trans_lmm_parity(gpu_ctx, prec):
     pr = _p_rel(gd["p"][tested_ref], ref["p"][tested_ref])
     print(f"LMM prec={prec}: max |dt|/(1+|t|) = {dt.max():.3e}, max p rel = {pr.max():.3e}, "
           f"min p = {np.nanmin(ref['p']):.3e}")
-    assert dt.max() < 2e-5
+    tol_t, tol_rel = TOL[prec]
+    assert dt.max() < tol_t
     assert pr.max() < 1e-3
     big = np.abs(t_r) > 0.1
-    assert rel_err(gd["beta"][tested_ref][big], ref["beta"][tested_ref][big]).max() < 1e-4
-    assert rel_err(gd["se"][tested_ref], ref["se"][tested_ref]).max() < 1e-4
+    assert rel_err(gd["beta"][tested_ref][big], ref["beta"][tested_ref][big]).max() < tol_rel
+    assert rel_err(gd["se"][tested_ref], ref["se"][tested_ref]).max() < tol_rel
     rs = ref["state"]
     assert np.array_equal(st["n_tested"], rs["n_tested"])
     assert np.array_equal(st["n_sig"], rs["n_sig"])
     assert np.array_equal(st["best_snp"], rs["best_snp"])
-    assert rel_err(st["acat_p"], rs["acat_p"]).max() < 1e-4
+    _acat_consistent(st, hits, G)
+    assert rel_err(st["acat_p"], rs["acat_p"]).max() < 1e-2
 
 
 def test_reml_grad_hess_fd(gpu_ctx):
@@ -158,9 +188,11 @@ def test_reml_grad_hess_fd(gpu_ctx):
             fm = O.reml_negll(np.exp(th[k] - hstep), Yt[:, k], Xt, lam)
             gfd = (fp - fm) / (2 * hstep)
             hfd = (fp - 2 * fo + fm) / hstep ** 2
+            # central differences carry rounding noise ~eps|f|/h (gradient), 4 eps|f|/h^2 (Hessian)
+            noise = np.finfo(float).eps * abs(fo)
             assert abs(f[k] - fo) <= 1e-9 * (1 + abs(fo))
-            assert abs(g[k] - gfd) <= 1e-5 * (1 + abs(gfd)), (k, shift, g[k], gfd)
-            assert abs(h[k] - hfd) <= 1e-3 * (1 + abs(hfd)), (k, shift, h[k], hfd)
+            assert abs(g[k] - gfd) <= 1e-5 * (1 + abs(gfd)) + 2 * noise / hstep, (k, shift, g[k], gfd)
+            assert abs(h[k] - hfd) <= 1e-3 * (1 + abs(hfd)) + 8 * noise / hstep ** 2, (k, shift, h[k], hfd)
 
 
 def test_newton_delta_is_argmin(gpu_ctx):
@@ -196,7 +228,7 @@ def test_grm_matches_closed_form(gpu_ctx):
         assert np.array_equal(N, Np)
         err = np.abs(K - Kr).max()
         print(f"GRM n={n} m={m} miss={miss}: max abs err {err:.3e}")
-        assert err < 2e-6
+        assert err < 1e-7
 
 
 def test_eigh_gpu(gpu_ctx):

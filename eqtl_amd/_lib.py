@@ -33,7 +33,7 @@ EXPORTS = [
     "eqtl_gpu_set_basis", "eqtl_gpu_lmm_prep", "eqtl_gpu_reml_grad_hess", "eqtl_gpu_lm_prep",
     "eqtl_gpu_set_gene_windows", "eqtl_gpu_scan_begin", "eqtl_gpu_scan_block", "eqtl_gpu_scan_end",
     "eqtl_gpu_set_timing", "eqtl_gpu_get_timing", "eqtl_gpu_synth_bed", "eqtl_gpu_copy_rows",
-    "eqtl_gpu_synchronize",
+    "eqtl_gpu_synchronize", "eqtl_gpu_selftest",
 ]
 
 _lib = None
@@ -101,6 +101,11 @@ class Context:
         hbm = ctypes.c_int64(0)
         _check(lib().eqtl_gpu_device_info(self._c, name, ctypes.c_int32(128), ctypes.byref(ncu), ctypes.byref(hbm)))
         return name.value.decode(), ncu.value, hbm.value
+
+    def selftest(self):
+        m = ctypes.c_int32(-1)
+        _check(lib().eqtl_gpu_selftest(self._c, ctypes.byref(m)))
+        return m.value
 
     def synchronize(self):
         _check(lib().eqtl_gpu_synchronize(self._c))

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -21,6 +22,14 @@
 using namespace eqtl_amd;
 
 namespace {
+
+// |t| below which a test's numerator is recomputed in FP64 (near-null
+// refinement in the Wald kernels). Chosen from each form's absolute t error
+// (FP32 ~1e-6, BF16X3 ~2e-5): above the threshold the ACAT term's relative
+// error stays below ~1e-2 of one term, and the refined share of tests is
+// about 0.8 tau (0.04% / 0.16%).
+constexpr double kRefineTauF32 = 5e-4;
+constexpr double kRefineTauBF16X3 = 2e-3;
 
 thread_local std::string g_err;
 
@@ -98,7 +107,8 @@ struct eqtl_gpu_ctx {
   DBuf Q64, lam64, Q32, Qhi, Qlo;
   // genes
   int G = 0, G_pad = 0, p = 0, prepped_model = -1, nq = 0;
-  DBuf Yt64, Xt64, X64, Y64, delta, gscal, gvalid, B32, Bhi, Blo, Wx, zeros;
+  DBuf Yt64, Xt64, X64, Y64, delta, gscal, gvalid, Bf32, Bbf, Wx, zeros;
+  DBuf U64, V64;  // per-gene FP64 residual columns (model space / sample space)
   bool have_windows = false;
   DBuf gene_chrom, gene_cs, gene_ce;
   // scan
@@ -108,6 +118,8 @@ struct eqtl_gpu_ctx {
   DBuf state, partial, hits, hit_count, tail;
   double tail_df = -1;
   int64_t hit_cap_dev = 0;
+  int wald_variant = 2;  // 2: LDS-staged k_wald2; 1: k_wald (EQTL_WALD_VARIANT=1)
+  int last_splits = 1;
   // per-block buffers
   int64_t m_cap = 0;
   DBuf rows, A32, A64, Gt, keep, af, gtg, dosage;
@@ -162,9 +174,18 @@ int set_device(eqtl_gpu_ctx* c) {
   return EQTL_OK;
 }
 
+GeneCols gene_cols(eqtl_gpu_ctx* c) {
+  GeneCols g;
+  g.f32 = c->Bf32.as<float>();
+  g.bf = c->Bbf.as<__bf16>();
+  g.nq = c->nq;
+  g.nks = c->n_pad / 32;
+  return g;
+}
+
 // Block buffers for up to m SNPs.
 int ensure_block(eqtl_gpu_ctx* c, int64_t m, int32_t bps, bool want64) {
-  const int64_t m_pad = round_up(std::max<int64_t>(m, 1), 64);
+  const int64_t m_pad = round_up(std::max<int64_t>(m, 1), 128);
   CHK(ensure(c->rows, static_cast<size_t>(m_pad) * bps));
   CHK(ensure(c->A32, static_cast<size_t>(m_pad) * c->n_pad * sizeof(float)));
   CHK(ensure(c->Gt, static_cast<size_t>(m_pad) * c->n_pad * sizeof(float)));
@@ -177,7 +198,7 @@ int ensure_block(eqtl_gpu_ctx* c, int64_t m, int32_t bps, bool want64) {
 
 int run_decode(eqtl_gpu_ctx* c, const uint8_t* rows_dev, int64_t m, int32_t bps, bool want64,
                double* dosage_dev) {
-  const int64_t m_pad = round_up(std::max<int64_t>(m, 1), 64);
+  const int64_t m_pad = round_up(std::max<int64_t>(m, 1), 128);
   DecodeArgs a;
   a.rows = rows_dev;
   a.m = m;
@@ -213,34 +234,53 @@ const uint8_t* stage_rows(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, 
   return c->rows.as<uint8_t>();
 }
 
+template <int MODEL, int PREC, int NQ>
+int launch_wald_one(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
+  if (c->wald_variant == 1) {
+    hipLaunchKernelGGL((k_wald<MODEL, PREC, NQ>), grid, dim3(256), 0, c->stream, a);
+    return check_launch("k_wald");
+  }
+  const size_t lds = std::max<size_t>(2 * static_cast<size_t>(c->nq) * 2048, 256 * sizeof(GeneAcc));
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald2<MODEL, PREC, NQ>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_set = true;
+  }
+  // 128-row tiles: split count recomputed for them
+  WaldArgs b = a;
+  const int n_tiles2 = (a.n_tiles + 1) / 2;
+  b.n_splits = std::max(1, std::min(n_tiles2, a.n_splits));
+  dim3 g2(static_cast<unsigned>(a.n_groups_pad * b.n_splits));
+  hipLaunchKernelGGL((k_wald2<MODEL, PREC, NQ>), g2, dim3(256), lds, c->stream, b);
+  c->last_splits = b.n_splits;
+  return check_launch("k_wald2");
+}
+
 template <int PREC>
 int launch_wald_lmm(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
   const int nq = c->nq;
-#define EQ_WALD(NQV)                                                                                 \
-  hipLaunchKernelGGL((k_wald<EQTL_MODEL_LMM, PREC, NQV>), grid, dim3(256), 0, c->stream, a);         \
-  return check_launch("k_wald");
-  if (nq <= 4) { EQ_WALD(4) }
-  if (nq <= 8) { EQ_WALD(8) }
-  if (nq <= 12) { EQ_WALD(12) }
-  if (nq <= 16) { EQ_WALD(16) }
-  if (nq <= 20) { EQ_WALD(20) }
-  if (nq <= 24) { EQ_WALD(24) }
-  if (nq <= 28) { EQ_WALD(28) }
-  if (nq <= 34) { EQ_WALD(34) }
-#undef EQ_WALD
+  if (nq <= 4) return launch_wald_one<EQTL_MODEL_LMM, PREC, 4>(c, a, grid);
+  if (nq <= 8) return launch_wald_one<EQTL_MODEL_LMM, PREC, 8>(c, a, grid);
+  if (nq <= 12) return launch_wald_one<EQTL_MODEL_LMM, PREC, 12>(c, a, grid);
+  if (nq <= 16) return launch_wald_one<EQTL_MODEL_LMM, PREC, 16>(c, a, grid);
+  if (nq <= 18) return launch_wald_one<EQTL_MODEL_LMM, PREC, 18>(c, a, grid);
+  if (nq <= 20) return launch_wald_one<EQTL_MODEL_LMM, PREC, 20>(c, a, grid);
+  if (nq <= 23) return launch_wald_one<EQTL_MODEL_LMM, PREC, 23>(c, a, grid);
+  if (nq <= 24) return launch_wald_one<EQTL_MODEL_LMM, PREC, 24>(c, a, grid);
+  if (nq <= 28) return launch_wald_one<EQTL_MODEL_LMM, PREC, 28>(c, a, grid);
+  if (nq <= 34) return launch_wald_one<EQTL_MODEL_LMM, PREC, 34>(c, a, grid);
   return fail(EQTL_E_ARG, "too many covariates for the Wald kernel");
 }
 
 int launch_wald(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
+  c->last_splits = a.n_splits;
   if (c->scan_model == EQTL_MODEL_LMM) {
     if (c->scan_prec == EQTL_PREC_FP32) return launch_wald_lmm<EQTL_PREC_FP32>(c, a, grid);
     return launch_wald_lmm<EQTL_PREC_BF16X3>(c, a, grid);
   }
-  if (c->scan_prec == EQTL_PREC_FP32)
-    hipLaunchKernelGGL((k_wald<EQTL_MODEL_LM, EQTL_PREC_FP32, 1>), grid, dim3(256), 0, c->stream, a);
-  else
-    hipLaunchKernelGGL((k_wald<EQTL_MODEL_LM, EQTL_PREC_BF16X3, 1>), grid, dim3(256), 0, c->stream, a);
-  return check_launch("k_wald");
+  if (c->scan_prec == EQTL_PREC_FP32) return launch_wald_one<EQTL_MODEL_LM, EQTL_PREC_FP32, 1>(c, a, grid);
+  return launch_wald_one<EQTL_MODEL_LM, EQTL_PREC_BF16X3, 1>(c, a, grid);
 }
 
 }  // namespace
@@ -262,6 +302,7 @@ int eqtl_gpu_open(int32_t device, eqtl_gpu_ctx** out) {
     delete c;
     return fail(EQTL_E_HIP, "stream/event creation failed");
   }
+  if (const char* v = std::getenv("EQTL_WALD_VARIANT")) c->wald_variant = std::atoi(v) == 1 ? 1 : 2;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     c->n_cu = prop.multiProcessorCount;
@@ -438,12 +479,10 @@ static int alloc_gene_side(eqtl_gpu_ctx* c, int G, int nq) {
   c->G_pad = static_cast<int>(round_up(G, 128));  // multiple of 8 groups of 16
   c->nq = nq;
   const size_t cols = static_cast<size_t>(c->G_pad / 16) * nq * 16 * c->n_pad;
-  CHK(ensure(c->B32, cols * sizeof(float)));
-  CHK(ensure(c->Bhi, cols * sizeof(__bf16)));
-  CHK(ensure(c->Blo, cols * sizeof(__bf16)));
-  HIPC(hipMemsetAsync(c->B32.p, 0, cols * sizeof(float), c->stream));
-  HIPC(hipMemsetAsync(c->Bhi.p, 0, cols * sizeof(__bf16), c->stream));
-  HIPC(hipMemsetAsync(c->Blo.p, 0, cols * sizeof(__bf16), c->stream));
+  CHK(ensure(c->Bf32, cols * sizeof(float)));
+  CHK(ensure(c->Bbf, 2 * cols * sizeof(__bf16)));
+  HIPC(hipMemsetAsync(c->Bf32.p, 0, cols * sizeof(float), c->stream));
+  HIPC(hipMemsetAsync(c->Bbf.p, 0, 2 * cols * sizeof(__bf16), c->stream));
   CHK(ensure(c->gscal, sizeof(double) * c->G_pad));
   CHK(ensure(c->gvalid, sizeof(int) * c->G_pad));
   HIPC(hipMemsetAsync(c->gvalid.p, 0, sizeof(int) * c->G_pad, c->stream));
@@ -466,6 +505,8 @@ int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double*
   CHK(ensure(c->X64, sizeof(double) * static_cast<size_t>(n) * p));
   CHK(ensure(c->Yt64, sizeof(double) * static_cast<size_t>(n) * G));
   CHK(ensure(c->Xt64, sizeof(double) * static_cast<size_t>(n) * p));
+  CHK(ensure(c->U64, sizeof(double) * static_cast<size_t>(n) * G));
+  CHK(ensure(c->V64, sizeof(double) * static_cast<size_t>(n) * G));
   HIPC(hipMemcpyAsync(c->Y64.p, Y, sizeof(double) * static_cast<size_t>(n) * G, hipMemcpyHostToDevice, c->stream));
   HIPC(hipMemcpyAsync(c->X64.p, X, sizeof(double) * static_cast<size_t>(n) * p, hipMemcpyHostToDevice, c->stream));
   // K4: Y~ = Q^T Y, X~ = Q^T X once for all genes (FP64 library GEMM)
@@ -493,9 +534,13 @@ int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double*
   {
     Timed t(c, "basis");
     hipLaunchKernelGGL(k_lmm_basis, dim3(G), dim3(256), 0, c->stream, c->Xt64.as<double>(), c->Yt64.as<double>(),
-                       c->lam64.as<double>(), n, c->n_pad, p, G, c->delta.as<double>(), c->B32.as<float>(),
-                       c->Bhi.as<__bf16>(), c->Blo.as<__bf16>(), c->gscal.as<double>(), c->gvalid.as<int>());
+                       c->lam64.as<double>(), n, c->n_pad, p, G, c->delta.as<double>(), gene_cols(c),
+                       c->gscal.as<double>(), c->gvalid.as<int>(), c->U64.as<double>());
     CHK(check_launch("k_lmm_basis"));
+    // sample-space residual vectors for the near-null FP64 refinement: V = Q U
+    if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, G, n, &one, c->Q64.as<double>(), n,
+                      c->U64.as<double>(), n, &zero, c->V64.as<double>(), n) != rocblas_status_success)
+      return fail(EQTL_E_HIP, "rocblas_dgemm failed");
     CHK(t.done());
   }
   std::vector<int> valid(G);
@@ -543,6 +588,7 @@ int eqtl_gpu_lm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double* 
   CHK(ensure(c->X64, sizeof(double) * static_cast<size_t>(n) * p));
   CHK(ensure(c->Wx, sizeof(double) * static_cast<size_t>(n) * p));
   CHK(ensure(c->zeros, sizeof(double) * n));
+  CHK(ensure(c->U64, sizeof(double) * static_cast<size_t>(n) * G));
   HIPC(hipMemsetAsync(c->zeros.p, 0, sizeof(double) * n, c->stream));
   HIPC(hipMemcpyAsync(c->Y64.p, Y, sizeof(double) * static_cast<size_t>(n) * G, hipMemcpyHostToDevice, c->stream));
   HIPC(hipMemcpyAsync(c->X64.p, X, sizeof(double) * static_cast<size_t>(n) * p, hipMemcpyHostToDevice, c->stream));
@@ -556,8 +602,8 @@ int eqtl_gpu_lm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double* 
   HIPC(hipStreamSynchronize(c->stream));
   if (!ok) return fail(EQTL_E_ARG, "covariate matrix X^T X is not positive definite");
   hipLaunchKernelGGL(k_lm_gene, dim3(G), dim3(256), 0, c->stream, c->Y64.as<double>(), c->Wx.as<double>(), n,
-                     c->n_pad, p, G, c->B32.as<float>(), c->Bhi.as<__bf16>(), c->Blo.as<__bf16>(),
-                     c->gscal.as<double>(), c->gvalid.as<int>());
+                     c->n_pad, p, G, gene_cols(c), c->gscal.as<double>(), c->gvalid.as<int>(),
+                     c->U64.as<double>());
   CHK(check_launch("k_lm_gene"));
   if (yty_out) HIPC(hipMemcpyAsync(yty_out, c->gscal.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
@@ -624,12 +670,12 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   if (hit_cap > 0 && !hits) return fail(EQTL_E_ARG, "hits buffer is null");
   CHK(set_device(c));
   const bool lm = c->scan_model == EQTL_MODEL_LM;
-  CHK(ensure_block(c, m, bps, lm));
+  CHK(ensure_block(c, m, bps, true));
   int rc;
   const uint8_t* rd = stage_rows(c, rows, on_dev, m, bps, &rc);
   if (rc) return rc;
-  CHK(run_decode(c, rd, m, bps, lm, nullptr));
-  const int64_t m_pad = round_up(m, 64);
+  CHK(run_decode(c, rd, m, bps, true, nullptr));
+  const int64_t m_pad = round_up(m, 128);
   const int np = c->n_pad;
   const float* Aop = c->A32.as<float>();
   if (!lm) {
@@ -663,9 +709,7 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   HIPC(hipMemsetAsync(c->hit_count.p, 0, sizeof(unsigned long long), c->stream));
   WaldArgs a;
   a.A = Aop;
-  a.B32 = c->B32.as<float>();
-  a.Bhi = c->Bhi.as<__bf16>();
-  a.Blo = c->Blo.as<__bf16>();
+  a.gc = gene_cols(c);
   a.n_pad = np;
   a.nq = c->nq;
   a.G = c->G;
@@ -693,13 +737,17 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   a.hits = c->hits.as<eqtl_hit>();
   a.hit_count = c->hit_count.as<unsigned long long>();
   a.hit_cap = cap_dev;
+  a.A64 = c->A64.as<double>();
+  a.vref = lm ? c->U64.as<double>() : c->V64.as<double>();
+  a.n = c->n;
+  a.tau = c->scan_prec == EQTL_PREC_FP32 ? kRefineTauF32 : kRefineTauBF16X3;
   {
     Timed t(c, "wald");
     CHK(launch_wald(c, a, dim3(static_cast<unsigned>(n_groups_pad * n_splits))));
     CHK(t.done());
   }
   hipLaunchKernelGGL(k_merge, dim3((c->G + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
-                     c->partial.as<GeneAcc>(), c->G, c->G_pad, n_splits);
+                     c->partial.as<GeneAcc>(), c->G, c->G_pad, c->last_splits);
   CHK(check_launch("k_merge"));
   unsigned long long cnt = 0;
   HIPC(hipMemcpyAsync(&cnt, c->hit_count.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
@@ -755,19 +803,19 @@ int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev
   if (bps < (c->n_file + 3) / 4) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
   CHK(set_device(c));
   CHK(ensure_block(c, m, bps, false));
-  const int64_t m_pad = round_up(m, 64);
+  const int64_t m_pad = round_up(m, 128);
   const int np = c->n_pad;
-  CHK(ensure(c->grm_z, static_cast<size_t>(m_pad) * np * (sizeof(float) + sizeof(__bf16))));
+  CHK(ensure(c->grm_z, static_cast<size_t>(m_pad) * np * (sizeof(double) + sizeof(__bf16))));
   int rc;
   const uint8_t* rd = stage_rows(c, rows, on_dev, m, bps, &rc);
   if (rc) return rc;
-  float* z32 = c->grm_z.as<float>();
-  __bf16* mis = reinterpret_cast<__bf16*>(z32 + static_cast<size_t>(m_pad) * np);
+  double* z64 = c->grm_z.as<double>();
+  __bf16* mis = reinterpret_cast<__bf16*>(z64 + static_cast<size_t>(m_pad) * np);
   DBuf used;
   CHK(ensure(used, 2 * sizeof(unsigned long long)));
   HIPC(hipMemsetAsync(used.p, 0, 2 * sizeof(unsigned long long), c->stream));
   hipLaunchKernelGGL(k_grm_standardize, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream, rd, m, m_pad,
-                     bps, c->n, np, c->n_file, c->sample_col.as<int32_t>(), z32, mis, c->grm_cnt.as<int32_t>(),
+                     bps, c->n, np, c->n_file, c->sample_col.as<int32_t>(), z64, mis, c->grm_cnt.as<int32_t>(),
                      used.as<unsigned long long>(), c->impute, c->max_miss, c->maf_min);
   CHK(check_launch("k_grm_standardize"));
   unsigned long long uu[2] = {0, 0};
@@ -776,7 +824,7 @@ int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev
   {
     Timed t(c, "grm");
     dim3 grid(static_cast<unsigned>(np / 64), static_cast<unsigned>(np / 64));
-    hipLaunchKernelGGL(k_grm_gram, grid, dim3(256), 0, c->stream, z32, mis, static_cast<int>(m_pad), np,
+    hipLaunchKernelGGL(k_grm_gram, grid, dim3(256), 0, c->stream, z64, mis, static_cast<int>(m_pad), np,
                        c->grm_acc.as<double>(), c->grm_miss.as<int32_t>(), uu[1] ? 1 : 0);
     CHK(check_launch("k_grm_gram"));
     CHK(t.done());
@@ -842,6 +890,22 @@ int eqtl_gpu_synth_bed(eqtl_gpu_ctx* c, uint64_t seed, int64_t m, int32_t n, dou
   CHK(check_launch("k_synth_bed"));
   HIPC(hipStreamSynchronize(c->stream));
   *rows_dev = b->as<uint8_t>();
+  return EQTL_OK;
+}
+
+int eqtl_gpu_selftest(eqtl_gpu_ctx* c, int32_t* failed_mask) {
+  if (!c || !failed_mask) return fail(EQTL_E_ARG, "bad selftest arguments");
+  CHK(set_device(c));
+  DBuf buf;
+  CHK(ensure(buf, sizeof(int) * (16 + 1024)));
+  std::vector<int> h(16 + 1024, 0);
+  for (int i = 0; i < 1024; ++i) h[16 + i] = i * 7 + 3;
+  HIPC(hipMemcpyAsync(buf.p, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_selftest, dim3(1), dim3(256), 0, c->stream, buf.as<int>());
+  CHK(check_launch("k_selftest"));
+  HIPC(hipMemcpyAsync(h.data(), buf.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  *failed_mask = h[0];
   return EQTL_OK;
 }
 

@@ -1,12 +1,12 @@
 // K2: GRM (compute_grm_from, grm.cpp:112-224).
 //   k_grm_standardize — decode each SNP row, apply the GRM's SNP rules
 //       (decode_row filters, then p = mean/2 in [1e-8, 1-1e-8], sd >= 1e-12),
-//       write z = (g - mean)/sd (missing → 0) in fp32 and the 0/1 missing
+//       write z = (g - mean)/sd (missing → 0) in fp64 and the 0/1 missing
 //       indicator in bf16, sample-major (SNP-contiguous) for the MFMA
 //       operands; count per-sample missing calls of used SNPs.
-//   k_grm_gram — lower block-triangle of Z^T Z (fp32 MFMA, fp32 within
-//       256-SNP chunks, fp64 across chunks) and of M^T M (joint missing
-//       counts, bf16 MFMA on 0/1, exact), accumulated into FP64 / int32 tiles.
+//   k_grm_gram — lower block-triangle of Z^T Z (fp64 MFMA) and of M^T M
+//       (joint missing counts, bf16 MFMA on 0/1, exact), accumulated into
+//       FP64 / int32 tiles.
 //   k_grm_finish — per-pair GCTA divisor m - miss_i - miss_j + joint_ij
 //       (diagonal m - miss_i), symmetric K and packed N.
 #pragma once
@@ -19,7 +19,7 @@ namespace eqtl_amd {
 
 __global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, int64_t m, int64_t m_pad, int bps,
                                                          int n, int n_pad, int n_file, const int32_t* sample_col,
-                                                         float* z32, __bf16* mis, int32_t* miss_cnt,
+                                                         double* z64, __bf16* mis, int32_t* miss_cnt,
                                                          unsigned long long* used, int impute, double max_miss,
                                                          double maf_min) {
   const int lane = threadIdx.x & 63;
@@ -61,7 +61,8 @@ __global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, in
     }
   }
   for (int i = lane; i < n_pad; i += 64) {
-    float z = 0.f, mi = 0.f;
+    double z = 0.0;
+    float mi = 0.f;
     if (use && i < n) {
       const int col = sample_col[i];
       const int code = (row[col >> 2] >> ((col & 3) * 2)) & 3;
@@ -70,11 +71,11 @@ __global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, in
         atomicAdd(&miss_cnt[i], 1);
       } else {
         const double g = code == 0 ? 2.0 : (code == 2 ? 1.0 : 0.0);
-        z = static_cast<float>((g - mean) / sd);
+        z = (g - mean) / sd;
       }
     }
     const size_t o = static_cast<size_t>(i) * m_pad + j;
-    z32[o] = z;
+    z64[o] = z;
     mis[o] = static_cast<__bf16>(mi);
   }
   if (use && lane == 0) {
@@ -83,8 +84,11 @@ __global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, in
   }
 }
 
-// One workgroup per 64x64 tile (by >= bx, lower block triangle).
-__global__ __launch_bounds__(256) void k_grm_gram(const float* z32, const __bf16* mis, int m_pad, int n_pad,
+// One workgroup per 64x64 tile (by >= bx, lower block triangle). Z^T Z on the
+// fp64 matrix cores (v_mfma_f64_16x16x4f64, same lane layout as the fp32
+// 16x16x4 form: per instruction t, lane group l>>4 supplies k = 8*(l>>4)+t of
+// the current 32-SNP step), so K carries no reduced-precision rounding.
+__global__ __launch_bounds__(256) void k_grm_gram(const double* z64, const __bf16* mis, int m_pad, int n_pad,
                                                   double* acc, int32_t* joint, int any_missing) {
   const int bx = blockIdx.x, by = blockIdx.y;
   if (bx > by) return;
@@ -92,38 +96,26 @@ __global__ __launch_bounds__(256) void k_grm_gram(const float* z32, const __bf16
   const int r0 = by * 64 + wave * 16;  // sample rows i
   const int c0 = bx * 64;              // sample cols k
   const int kk = 8 * (lane >> 4);
-  double a64[4][4];
-  float cm[4][4];
+  f64x4 a64[4];
+  f32x4 accm[4];
 #pragma unroll
-  for (int cb = 0; cb < 4; ++cb)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) { a64[cb][r] = 0; cm[cb][r] = 0; }
+  for (int cb = 0; cb < 4; ++cb) { a64[cb] = f64x4{0, 0, 0, 0}; accm[cb] = f32x4{0, 0, 0, 0}; }
   const size_t arow = static_cast<size_t>(r0 + (lane & 15)) * m_pad + kk;
-  for (int c0k = 0; c0k < m_pad; c0k += 256) {
-    f32x4 acc32[4], accm[4];
+  for (int k0 = 0; k0 < m_pad; k0 += 32) {
+    const f64x8 az = load8_f64(z64 + arow + k0);
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb) { acc32[cb] = f32x4{0, 0, 0, 0}; accm[cb] = f32x4{0, 0, 0, 0}; }
-    const int kend = min(m_pad, c0k + 256);
-    for (int k0 = c0k; k0 < kend; k0 += 32) {
-      const f32x8 az = load8(z32 + arow + k0);
+    for (int cb = 0; cb < 4; ++cb) {
+      const size_t bo = static_cast<size_t>(c0 + cb * 16 + (lane & 15)) * m_pad + kk + k0;
+      a64[cb] = mfma_f64x8(az, load8_f64(z64 + bo), a64[cb]);
+    }
+    if (any_missing) {
+      const bf16x8 am = load8_bf16(mis + arow + k0);
 #pragma unroll
       for (int cb = 0; cb < 4; ++cb) {
         const size_t bo = static_cast<size_t>(c0 + cb * 16 + (lane & 15)) * m_pad + kk + k0;
-        acc32[cb] = mfma_f32x8(az, load8(z32 + bo), acc32[cb]);
-      }
-      if (any_missing) {
-        const bf16x8 am = load8_bf16(mis + arow + k0);
-#pragma unroll
-        for (int cb = 0; cb < 4; ++cb) {
-          const size_t bo = static_cast<size_t>(c0 + cb * 16 + (lane & 15)) * m_pad + kk + k0;
-          accm[cb] = mfma_bf16(am, load8_bf16(mis + bo), accm[cb]);
-        }
+        accm[cb] = mfma_bf16(am, load8_bf16(mis + bo), accm[cb]);  // 0/1 products: exact below 2^24 SNPs
       }
     }
-#pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) { a64[cb][r] += acc32[cb][r]; cm[cb][r] += accm[cb][r]; }
   }
 #pragma unroll
   for (int cb = 0; cb < 4; ++cb)
@@ -131,7 +123,7 @@ __global__ __launch_bounds__(256) void k_grm_gram(const float* z32, const __bf16
     for (int r = 0; r < 4; ++r) {
       const size_t o = static_cast<size_t>(r0 + 4 * (lane >> 4) + r) * n_pad + c0 + cb * 16 + (lane & 15);
       acc[o] += a64[cb][r];
-      joint[o] += static_cast<int32_t>(cm[cb][r]);
+      joint[o] += static_cast<int32_t>(accm[cb][r]);
     }
 }
 

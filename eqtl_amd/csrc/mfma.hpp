@@ -21,8 +21,14 @@ namespace eqtl_amd {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
+typedef __attribute__((__vector_size__(4 * sizeof(double)))) double f64x4;
+
 struct f32x8 {
   float v[8];
+};
+
+struct f64x8 {
+  double v[8];
 };
 
 __device__ inline f32x4 mfma_bf16(const bf16x8& a, const bf16x8& b, const f32x4& c) {
@@ -32,6 +38,13 @@ __device__ inline f32x4 mfma_bf16(const bf16x8& a, const bf16x8& b, const f32x4&
 __device__ inline f32x4 mfma_f32x8(const f32x8& a, const f32x8& b, f32x4 c) {
 #pragma unroll
   for (int t = 0; t < 8; ++t) c = __builtin_amdgcn_mfma_f32_16x16x4f32(a.v[t], b.v[t], c, 0, 0, 0);
+  return c;
+}
+
+// fp64 16x16x4: same lane layout as the fp32 form, 8 instructions per 32 k.
+__device__ inline f64x4 mfma_f64x8(const f64x8& a, const f64x8& b, f64x4 c) {
+#pragma unroll
+  for (int t = 0; t < 8; ++t) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a.v[t], b.v[t], c, 0, 0, 0);
   return c;
 }
 
@@ -56,8 +69,43 @@ __device__ inline f32x8 load8(const float* p) {
   return r;
 }
 
+__device__ inline f64x8 load8_f64(const double* p) {
+  f64x8 r;
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const double2 v = reinterpret_cast<const double2*>(p)[h];
+    r.v[2 * h] = v.x;
+    r.v[2 * h + 1] = v.y;
+  }
+  return r;
+}
+
 __device__ inline bf16x8 load8_bf16(const __bf16* p) {
   return *reinterpret_cast<const bf16x8*>(p);
+}
+
+// Gene-side operand, fragment-major: for gene group g (16 genes) and k-step s
+// (32 samples) one contiguous tile holds, per quantity block b, the 64 lanes'
+// 8-value B fragments (lane = gene_in_group + 16*((k % 32) / 8), value k % 8).
+// bf16 tiles hold the hi plane then the lo plane (2 * nq KiB), fp32 tiles
+// nq * 2 KiB, so a whole tile is one contiguous copy into LDS.
+struct GeneCols {
+  float* f32;
+  __bf16* bf;
+  int nq, nks;
+  __host__ __device__ size_t tile_f32(int g, int s) const { return (static_cast<size_t>(g) * nks + s) * nq * 512; }
+  __host__ __device__ size_t tile_bf(int g, int s) const { return (static_cast<size_t>(g) * nks + s) * nq * 1024; }
+};
+
+// value v of quantity b for gene `gene` at sample i, in all precision copies
+__device__ inline void put_val(const GeneCols& gc, int gene, int b, int i, double v) {
+  const int g = gene >> 4, col = gene & 15, s = i >> 5, lane = col + 16 * ((i & 31) >> 3), e = i & 7;
+  const float f = static_cast<float>(v);
+  gc.f32[gc.tile_f32(g, s) + (static_cast<size_t>(b) * 64 + lane) * 8 + e] = f;
+  const __bf16 h = static_cast<__bf16>(f);
+  const size_t ob = gc.tile_bf(g, s) + (static_cast<size_t>(b) * 64 + lane) * 8 + e;
+  gc.bf[ob] = h;
+  gc.bf[ob + static_cast<size_t>(gc.nq) * 512] = static_cast<__bf16>(f - static_cast<float>(h));
 }
 
 }  // namespace eqtl_amd

@@ -414,16 +414,6 @@ __global__ __launch_bounds__(256) void k_reml_grad_hess(const double* Xt, const 
   if (threadIdx.x == 0) { f[k] = F; g[k] = Gd; h[k] = H; }
 }
 
-// Writes one value v for column (group, block b, lane gl) at sample i into the
-// three precision copies of the gene-side operand.
-__device__ inline void put_col(float* B32, __bf16* Bhi, __bf16* Blo, size_t off, double v) {
-  const float f = static_cast<float>(v);
-  B32[off] = f;
-  const __bf16 h = static_cast<__bf16>(f);
-  Bhi[off] = h;
-  Blo[off] = static_cast<__bf16>(f - static_cast<float>(h));
-}
-
 // Plain Cholesky (no pivoting) of sm.A into lower Lc (in sm.A1). Thread 0.
 __device__ inline bool chol_lower(RemlSmem& sm, int p) {
   for (int j = 0; j < p; ++j) {
@@ -446,8 +436,8 @@ __device__ inline bool chol_lower(RemlSmem& sm, int p) {
 // where W = (s X~) Lc^-T is orthonormal (Lc Lc^T = X~^T D X~) and
 // r = s y~ - W W^T (s y~) is the whitened null residual; rss_null = ||r||^2.
 __global__ __launch_bounds__(256) void k_lmm_basis(const double* Xt, const double* Yt, const double* lam, int n,
-                                                   int n_pad, int p, int G, const double* delta, float* B32,
-                                                   __bf16* Bhi, __bf16* Blo, double* rss_null, int* valid) {
+                                                   int n_pad, int p, int G, const double* delta, GeneCols gc,
+                                                   double* rss_null, int* valid, double* u64) {
   __shared__ RemlSmem sm;
   const int k = blockIdx.x;
   if (k >= G) return;
@@ -457,9 +447,6 @@ __global__ __launch_bounds__(256) void k_lmm_basis(const double* Xt, const doubl
   if (threadIdx.x == 0) sm.flag = chol_lower(sm, p) ? 1 : 0;
   __syncthreads();
   const int nq = p + 2;
-  const int group = k >> 4, gl = k & 15;
-  const size_t blk = static_cast<size_t>(16) * n_pad;
-  const size_t base = static_cast<size_t>(group) * nq * blk + static_cast<size_t>(gl) * n_pad;
   const bool ok = sm.flag != 0;
   // pass A: W rows, u = W^T (s y~)
   double ucol[kPMax];
@@ -467,7 +454,7 @@ __global__ __launch_bounds__(256) void k_lmm_basis(const double* Xt, const doubl
   for (int c = 0; c < kPMax; ++c) ucol[c] = 0;
   for (int i = threadIdx.x; i < n_pad; i += 256) {
     if (i >= n || !ok) {
-      for (int b = 0; b < nq; ++b) put_col(B32, Bhi, Blo, base + b * blk + i, 0.0);
+      for (int b = 0; b < nq; ++b) put_val(gc, k, b, i, 0.0);
       continue;
     }
     double v = dk * lam[i] + 1.0;
@@ -484,10 +471,10 @@ __global__ __launch_bounds__(256) void k_lmm_basis(const double* Xt, const doubl
         ucol[c] += w[c] * yw;
       }
     }
-    put_col(B32, Bhi, Blo, base + i, d);
+    put_val(gc, k, 0, i, d);
 #pragma unroll
     for (int c = 0; c < kPMax; ++c)
-      if (c < p) put_col(B32, Bhi, Blo, base + (1 + c) * blk + i, s * w[c]);
+      if (c < p) put_val(gc, k, 1 + c, i, s * w[c]);
   }
 #pragma unroll
   for (int c = 0; c < kPMax; ++c) {
@@ -515,7 +502,8 @@ __global__ __launch_bounds__(256) void k_lmm_basis(const double* Xt, const doubl
       }
     }
     rss += r * r;
-    put_col(B32, Bhi, Blo, base + (p + 1) * blk + i, s * r);
+    put_val(gc, k, p + 1, i, s * r);
+    u64[static_cast<size_t>(k) * n + i] = s * r;
   }
   rss = block_sum(rss, sm);
   if (threadIdx.x == 0) {
@@ -550,7 +538,7 @@ __global__ __launch_bounds__(256) void k_lm_basis(const double* X, const double*
 
 // LM gene: y_s = y - Wx Wx^T y (written as column block 0), yty = ||y_s||^2.
 __global__ __launch_bounds__(256) void k_lm_gene(const double* Y, const double* Wx, int n, int n_pad, int p, int G,
-                                                 float* B32, __bf16* Bhi, __bf16* Blo, double* yty, int* valid) {
+                                                 GeneCols gc, double* yty, int* valid, double* u64) {
   __shared__ RemlSmem sm;
   const int k = blockIdx.x;
   if (k >= G) return;
@@ -562,8 +550,6 @@ __global__ __launch_bounds__(256) void k_lm_gene(const double* Y, const double* 
     if (threadIdx.x == 0) sm.u[c] = s;
   }
   __syncthreads();
-  const int group = k >> 4, gl = k & 15;
-  const size_t base = static_cast<size_t>(group) * 16 * n_pad + static_cast<size_t>(gl) * n_pad;
   double ss = 0;
   for (int i = threadIdx.x; i < n_pad; i += 256) {
     double r = 0;
@@ -571,8 +557,9 @@ __global__ __launch_bounds__(256) void k_lm_gene(const double* Y, const double* 
       r = y[i];
       for (int c = 0; c < p; ++c) r -= Wx[static_cast<size_t>(c) * n + i] * sm.u[c];
       ss += r * r;
+      u64[static_cast<size_t>(k) * n + i] = r;
     }
-    put_col(B32, Bhi, Blo, base + i, r);
+    put_val(gc, k, 0, i, r);
   }
   ss = block_sum(ss, sm);
   if (threadIdx.x == 0) {

@@ -204,9 +204,7 @@ __global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ A, con
 // ---------------------------------------------------------------------------
 struct WaldArgs {
   const float* A;          // m_pad x n_pad
-  const float* B32;        // gene blocks, fp32
-  const __bf16* Bhi;       // gene blocks, bf16 hi
-  const __bf16* Blo;       // gene blocks, bf16 lo
+  GeneCols gc;             // gene-side operand (fragment-major tiles)
   int32_t n_pad, nq, G, G_pad, n_groups, n_groups_pad, n_splits, n_tiles;
   int64_t m, snp0;
   const uint8_t* keep;     // block-local
@@ -225,7 +223,143 @@ struct WaldArgs {
   eqtl_hit* hits;
   unsigned long long* hit_count;
   int64_t hit_cap;
+  // near-null refinement (tau = 0 disables it)
+  const double* A64;          // m_pad x n_pad centred dosages, FP64, analysis order
+  const double* vref;         // G x n FP64 residual vectors in sample space
+  int32_t n;
+  double tau;
 };
+
+// Near-null refinement. For p near 1 the ACAT term is about -1/(pi (1 - p)),
+// and 1 - p is proportional to |t| there, so the contraction's absolute error
+// in t turns into a large relative error of that term (and of the gene's ACAT
+// p). A test whose contracted |t| is below tau gets its numerator recomputed
+// in FP64: num = v_k^T g_c with g_c the centred FP64 dosage and v_k the gene's
+// residual vector mapped back to sample space (LMM: v_k = Q (s o r_k), so
+// v_k^T g_c = (s o r_k)^T Q^T g_c; LM: v_k = r_k). The whole wave computes
+// each such dot product together, so the refinement costs one 2n-double read
+// per refined test and no divergence.
+template <int MODEL, int NQ>
+__device__ inline bool near_null(const WaldArgs& a, const float* q, int64_t j, bool gene_ok, double gscal) {
+  if (a.tau <= 0 || !gene_ok || j >= a.m || !a.keep[j] || !(a.df > 0)) return false;
+  const double tau2 = a.tau * a.tau;
+  if constexpr (MODEL == EQTL_MODEL_LMM) {
+    const double gg = q[0];
+    double ss = 0, num = 0;
+#pragma unroll
+    for (int b = 1; b < NQ; ++b) {
+      if (b < a.nq - 1) ss += static_cast<double>(q[b]) * q[b];
+      if (b == a.nq - 1) num = q[b];
+    }
+    const double S = gg - ss;
+    if (gg < 1e-12 || S <= 1e-15) return false;
+    const double beta = num / S;
+    const double qq = fmax(gscal - S * beta * beta, 0.0);
+    return beta * beta * S * a.df < tau2 * qq;  // t^2 = beta^2 S df / qq
+  } else {
+    const double gtg = a.gtg[j];
+    if (gtg < 1e-12) return false;
+    const double gty = q[0];
+    const double beta = gty / gtg;
+    const double rss = fmax(gscal - beta * gty, 0.0);
+    return beta * beta * gtg * a.df < tau2 * rss;
+  }
+}
+
+// Wave-cooperative FP64 dot products for the lanes that flagged a test; every
+// lane of the wave must call this (wave-uniform control flow).
+__device__ inline double wave_refine(const WaldArgs& a, bool cand, int64_t j, int gene) {
+  unsigned long long mask = __ballot(cand);
+  const int lane = threadIdx.x & 63;
+  double mine = 0.0;
+  while (mask) {
+    const int src = __builtin_ctzll(mask);
+    mask &= mask - 1;
+    const int js = __shfl(static_cast<int>(j), src);
+    const int gs = __shfl(gene, src);
+    const double* x = a.A64 + static_cast<size_t>(js) * a.n_pad;
+    const double* v = a.vref + static_cast<size_t>(gs) * a.n;
+    double s = 0.0;
+    for (int i = lane; i < a.n; i += 64) s = fma(x[i], v[i], s);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == src) mine = s;
+  }
+  return mine;
+}
+
+
+// One (SNP row j, gene) test from its NQ accumulated quantities: FP64 Wald
+// (test_lmm_gtil, scan_trans_lmm.cpp:11-40 / hit_from_gty, scan_trans_lm.cpp:21-36),
+// then the per-gene statistics (apply_snp_hit_stats, scan_trans.hpp:87-102).
+template <int MODEL, int NQ>
+__device__ inline void wald_test(const WaldArgs& a, const float* q, int64_t j, int gene, bool gene_ok, double gscal,
+                                 bool excl, int32_t gch, int64_t gcs, int64_t gce, bool refined, double num64,
+                                 GeneAcc& st, dd& asum) {
+  if (!gene_ok || j >= a.m || !a.keep[j]) return;
+  const int64_t snp = a.snp0 + j;
+  if (excl && a.snp_chrom && a.snp_chrom[snp] == gch && a.snp_pos[snp] >= gcs && a.snp_pos[snp] <= gce) return;
+  const int nq = a.nq;
+  double beta = 0, se = 0, stat = 0, p = 1.0, r2 = 0;
+  const double df = a.df;
+  if constexpr (MODEL == EQTL_MODEL_LMM) {
+    const double gg = q[0];
+    double ss = 0, num = 0;
+#pragma unroll
+    for (int b = 1; b < NQ; ++b) {
+      if (b < nq - 1) ss += static_cast<double>(q[b]) * q[b];
+      if (b == nq - 1) num = q[b];
+    }
+    if (refined) num = num64;
+    const double S = gg - ss;
+    if (!(gg < 1e-12) && !(S <= 1e-15)) {
+      beta = num / S;
+      double qq = gscal - S * beta * beta;
+      if (qq < 0) qq = 0;
+      const double sigma2 = df > 0 ? qq / df : 1.0;
+      se = sqrt(fmax(sigma2 / S, 0.0));
+      stat = se > 0 ? beta / se : 0.0;
+      p = p_from_t_tab(*a.tail, stat);
+      r2 = gscal > 1e-15 ? fmax(0.0, 1.0 - qq / gscal) : 0.0;
+    }
+  } else {
+    const double gtg = a.gtg[j];
+    const double gty = refined ? num64 : static_cast<double>(q[0]);
+    if (!(gtg < 1e-12) && df > 0) {
+      beta = gty / gtg;
+      const double rss = gscal - beta * gty;
+      const double s2 = fmax(rss / df, 0.0);
+      se = sqrt(s2 / gtg);
+      stat = se > 0 ? beta / se : 0.0;
+      p = p_from_t_tab(*a.tail, stat);
+      r2 = gscal > 0 ? 1.0 - rss / gscal : 0.0;
+    } else if (!(gtg < 1e-12)) {
+      beta = gty / gtg;  // df <= 0: p = 1 with beta set (model_lm.cpp:44-48)
+    }
+  }
+  if (!isfinite(p)) return;
+  ++st.n_tested;
+  double term;
+  if (acat_accept(p, &term)) {
+    asum = dd_add_d(asum, term);
+    ++st.used;
+  }
+  const double afj = a.af[j];
+  if (p <= a.pthr) {
+    ++st.n_sig;
+    const unsigned long long idx = atomicAdd(a.hit_count, 1ull);
+    if (static_cast<int64_t>(idx) < a.hit_cap) {
+      eqtl_hit h;
+      h.snp = snp; h.gene = gene; h.pad = 0;
+      h.af = afj; h.beta = beta; h.se = se; h.stat = stat; h.p = p; h.r2 = r2;
+      a.hits[idx] = h;
+    }
+  }
+  if (p < st.best_p || (p == st.best_p && snp < st.best_snp)) {
+    st.best_p = p; st.best_snp = snp; st.best_af = afj; st.best_beta = beta;
+    st.best_se = se; st.best_stat = stat; st.best_r2 = r2;
+  }
+}
 
 template <int MODEL, int PREC, int NQ>
 __global__ __launch_bounds__(256) void k_wald(WaldArgs a) {
@@ -254,8 +388,6 @@ __global__ __launch_bounds__(256) void k_wald(WaldArgs a) {
   acc_init(st);
   dd asum{0.0, 0.0};
 
-  const size_t gstride = static_cast<size_t>(16) * a.n_pad;  // one block of 16 columns
-  const size_t gbase = static_cast<size_t>(group) * nq * gstride + static_cast<size_t>(gl) * a.n_pad + kk;
 
   for (int tile = split; tile < a.n_tiles; tile += a.n_splits) {
     const int64_t r0 = static_cast<int64_t>(tile) * 64 + wave * 16;
@@ -274,7 +406,7 @@ __global__ __launch_bounds__(256) void k_wald(WaldArgs a) {
 #pragma unroll
         for (int b = 0; b < NQ; ++b) {
           if (b < nq) {
-            const f32x8 bv = load8(a.B32 + gbase + b * gstride + k0);
+            const f32x8 bv = load8(a.gc.f32 + a.gc.tile_f32(group, k0 >> 5) + (b * 64 + lane) * 8);
             if (MODEL == EQTL_MODEL_LMM && b == 0) acc[b] = mfma_f32x8(sv, bv, acc[b]);
             else acc[b] = mfma_f32x8(av, bv, acc[b]);
           }
@@ -291,8 +423,8 @@ __global__ __launch_bounds__(256) void k_wald(WaldArgs a) {
 #pragma unroll
         for (int b = 0; b < NQ; ++b) {
           if (b < nq) {
-            const size_t bo = gbase + b * gstride + k0;
-            const bf16x8 bh = load8_bf16(a.Bhi + bo), bl = load8_bf16(a.Blo + bo);
+            const size_t bo = a.gc.tile_bf(group, k0 >> 5) + (b * 64 + lane) * 8;
+            const bf16x8 bh = load8_bf16(a.gc.bf + bo), bl = load8_bf16(a.gc.bf + bo + static_cast<size_t>(nq) * 512);
             if (MODEL == EQTL_MODEL_LMM && b == 0) {
               acc[b] = mfma_bf16(sh, bh, acc[b]);
               acc[b] = mfma_bf16(sh, bl, acc[b]);
@@ -309,71 +441,13 @@ __global__ __launch_bounds__(256) void k_wald(WaldArgs a) {
     // ---- FP64 epilogue: 4 (SNP, gene) tests per lane ----
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int64_t j = r0 + 4 * (lane >> 4) + r;  // block-local SNP row
-      if (!gene_ok || j >= a.m || !a.keep[j]) continue;
-      const int64_t snp = a.snp0 + j;
-      if (excl && a.snp_chrom && a.snp_chrom[snp] == gch && a.snp_pos[snp] >= gcs && a.snp_pos[snp] <= gce)
-        continue;
-      double beta = 0, se = 0, stat = 0, p = 1.0, r2 = 0;
-      const double df = a.df;
-      if constexpr (MODEL == EQTL_MODEL_LMM) {
-        const double gg = acc[0][r];
-        double ss = 0;
+      float q[NQ];
 #pragma unroll
-        for (int b = 1; b < NQ; ++b)
-          if (b < nq - 1) ss += static_cast<double>(acc[b][r]) * acc[b][r];
-        double num = 0;
-#pragma unroll
-        for (int b = 1; b < NQ; ++b)
-          if (b == nq - 1) num = acc[b][r];
-        const double S = gg - ss;
-        if (!(gg < 1e-12) && !(S <= 1e-15)) {
-          beta = num / S;
-          double q = gscal - S * beta * beta;
-          if (q < 0) q = 0;
-          const double sigma2 = df > 0 ? q / df : 1.0;
-          se = sqrt(fmax(sigma2 / S, 0.0));
-          stat = se > 0 ? beta / se : 0.0;
-          p = p_from_t_tab(*a.tail, stat);
-          r2 = gscal > 1e-15 ? fmax(0.0, 1.0 - q / gscal) : 0.0;
-        }
-      } else {
-        const double gtg = a.gtg[j];
-        const double gty = acc[0][r];
-        if (!(gtg < 1e-12) && df > 0) {
-          beta = gty / gtg;
-          const double rss = gscal - beta * gty;
-          const double s2 = fmax(rss / df, 0.0);
-          se = sqrt(s2 / gtg);
-          stat = se > 0 ? beta / se : 0.0;
-          p = p_from_t_tab(*a.tail, stat);
-          r2 = gscal > 0 ? 1.0 - rss / gscal : 0.0;
-        } else if (!(gtg < 1e-12)) {
-          beta = gty / gtg;  // df <= 0: p = 1 with beta set (model_lm.cpp:44-48)
-        }
-      }
-      if (!isfinite(p)) continue;
-      ++st.n_tested;
-      double term;
-      if (acat_accept(p, &term)) {
-        asum = dd_add_d(asum, term);
-        ++st.used;
-      }
-      const double afj = a.af[j];
-      if (p <= a.pthr) {
-        ++st.n_sig;
-        const unsigned long long idx = atomicAdd(a.hit_count, 1ull);
-        if (static_cast<int64_t>(idx) < a.hit_cap) {
-          eqtl_hit h;
-          h.snp = snp; h.gene = gene; h.pad = 0;
-          h.af = afj; h.beta = beta; h.se = se; h.stat = stat; h.p = p; h.r2 = r2;
-          a.hits[idx] = h;
-        }
-      }
-      if (p < st.best_p || (p == st.best_p && snp < st.best_snp)) {
-        st.best_p = p; st.best_snp = snp; st.best_af = afj; st.best_beta = beta;
-        st.best_se = se; st.best_stat = stat; st.best_r2 = r2;
-      }
+      for (int b = 0; b < NQ; ++b) q[b] = acc[b][r];
+      const int64_t j = r0 + 4 * (lane >> 4) + r;
+      const bool cand = near_null<MODEL, NQ>(a, q, j, gene_ok, gscal);
+      const double n64 = wave_refine(a, cand, j, gene);
+      wald_test<MODEL, NQ>(a, q, j, gene, gene_ok, gscal, excl, gch, gcs, gce, cand, n64, st, asum);
     }
   }
   st.acat_hi = asum.hi;
@@ -388,6 +462,221 @@ __global__ __launch_bounds__(256) void k_wald(WaldArgs a) {
     const int g = group * 16 + threadIdx.x;
     a.partial[static_cast<size_t>(split) * a.G_pad + g] = t;
   }
+}
+
+
+// ---------------------------------------------------------------------------
+// K7/K8 v2: the same contraction with the gene group's k-step tile staged in
+// LDS (double-buffered, filled by direct global→LDS 16-byte loads) and shared
+// by the 4 waves; each wave owns 2 row blocks (32 SNP rows), so every B
+// fragment read from LDS feeds two MFMA chains. Workgroup tile: 128 SNPs x 16
+// genes x NQ quantities. One workgroup per CU (accumulators use the AGPRs).
+// ---------------------------------------------------------------------------
+constexpr int kRB = 2;
+
+__device__ inline void lds_direct16(const void* gsrc, unsigned char* lds_wave_base) {
+  __builtin_amdgcn_global_load_lds(gsrc, (__attribute__((address_space(3))) void*)(lds_wave_base), 16, 0, 0);
+}
+
+template <int MODEL, int PREC, int NQ>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_wald2(WaldArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  const int wg = blockIdx.x;
+  const int xcd = wg & 7;
+  const int jx = wg >> 3;
+  const int group = (jx / a.n_splits) * 8 + xcd;
+  const int split = jx % a.n_splits;
+  if (group >= a.n_groups) return;
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int gl = lane & 15;
+  const int gene = group * 16 + gl;
+  const bool gene_ok = gene < a.G && a.gene_valid[gene];
+  const int kk = 8 * (lane >> 4);
+  const int nq = a.nq;
+  const int nks = a.n_pad >> 5;
+  const double gscal = gene_ok ? a.gene_scal[gene] : 0.0;
+  const bool excl = gene_ok && a.gene_chrom && a.gene_chrom[gene] >= 0;
+  const int32_t gch = excl ? a.gene_chrom[gene] : -1;
+  const int64_t gcs = excl ? a.gene_cs[gene] : 0, gce = excl ? a.gene_ce[gene] : 0;
+
+  const int tile_bytes = nq * 2048;  // one k-step tile of the group (either precision)
+  unsigned char* buf[2] = {smem, smem + tile_bytes};
+  const unsigned char* gsrc =
+      PREC == EQTL_PREC_FP32 ? reinterpret_cast<const unsigned char*>(a.gc.f32 + a.gc.tile_f32(group, 0))
+                             : reinterpret_cast<const unsigned char*>(a.gc.bf + a.gc.tile_bf(group, 0));
+  const int chunks = tile_bytes >> 10;  // 1 KiB per wave instruction (64 lanes x 16 B)
+
+  GeneAcc st;
+  acc_init(st);
+  dd asum{0.0, 0.0};
+  const int n_tiles2 = (a.n_tiles + 1) >> 1;  // 128-row tiles
+
+  for (int tile = split; tile < n_tiles2; tile += a.n_splits) {
+    const int64_t rowA = static_cast<int64_t>(tile) * 128 + wave * 32;
+    f32x4 acc[kRB][NQ];
+#pragma unroll
+    for (int rb = 0; rb < kRB; ++rb)
+#pragma unroll
+      for (int b = 0; b < NQ; ++b) acc[rb][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const float* arow[kRB];
+#pragma unroll
+    for (int rb = 0; rb < kRB; ++rb) arow[rb] = a.A + (rowA + rb * 16 + gl) * a.n_pad + kk;
+    // prologue: stage k-step 0
+    for (int c = wave; c < chunks; c += 4) lds_direct16(gsrc + c * 1024 + lane * 16, buf[0] + c * 1024);
+    f32x8 an[kRB];
+#pragma unroll
+    for (int rb = 0; rb < kRB; ++rb) an[rb] = load8(arow[rb]);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int s = 0; s < nks; ++s) {
+      const unsigned char* cur = buf[s & 1];
+      f32x8 ac[kRB];
+#pragma unroll
+      for (int rb = 0; rb < kRB; ++rb) ac[rb] = an[rb];
+      if (s + 1 < nks) {
+        const unsigned char* src = gsrc + static_cast<size_t>(s + 1) * tile_bytes;
+        unsigned char* dst = buf[(s + 1) & 1];
+        for (int c = wave; c < chunks; c += 4) lds_direct16(src + c * 1024 + lane * 16, dst + c * 1024);
+#pragma unroll
+        for (int rb = 0; rb < kRB; ++rb) an[rb] = load8(arow[rb] + (s + 1) * 32);
+      }
+      if constexpr (PREC == EQTL_PREC_FP32) {
+        f32x8 sq[kRB];
+        if constexpr (MODEL == EQTL_MODEL_LMM) {
+#pragma unroll
+          for (int rb = 0; rb < kRB; ++rb)
+#pragma unroll
+            for (int t = 0; t < 8; ++t) sq[rb].v[t] = ac[rb].v[t] * ac[rb].v[t];
+        }
+#pragma unroll
+        for (int b = 0; b < NQ; ++b) {
+          if (b < nq) {
+            const float* bp = reinterpret_cast<const float*>(cur + (b * 64 + lane) * 32);
+            const f32x8 bv = load8(bp);
+#pragma unroll
+            for (int rb = 0; rb < kRB; ++rb) {
+              if (MODEL == EQTL_MODEL_LMM && b == 0) acc[rb][b] = mfma_f32x8(sq[rb], bv, acc[rb][b]);
+              else acc[rb][b] = mfma_f32x8(ac[rb], bv, acc[rb][b]);
+            }
+          }
+        }
+      } else {
+        bf16x8 ah[kRB], al[kRB], sh[kRB], sl[kRB];
+#pragma unroll
+        for (int rb = 0; rb < kRB; ++rb) {
+          split_bf16(ac[rb], ah[rb], al[rb]);
+          if constexpr (MODEL == EQTL_MODEL_LMM) {
+            f32x8 sv;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) sv.v[t] = ac[rb].v[t] * ac[rb].v[t];
+            split_bf16(sv, sh[rb], sl[rb]);
+          }
+        }
+#pragma unroll
+        for (int b = 0; b < NQ; ++b) {
+          if (b < nq) {
+            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(cur + (b * 64 + lane) * 16);
+            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(cur + nq * 1024 + (b * 64 + lane) * 16);
+#pragma unroll
+            for (int rb = 0; rb < kRB; ++rb) {
+              if (MODEL == EQTL_MODEL_LMM && b == 0) {
+                acc[rb][b] = mfma_bf16(sh[rb], bh, acc[rb][b]);
+                acc[rb][b] = mfma_bf16(sh[rb], bl, acc[rb][b]);
+                acc[rb][b] = mfma_bf16(sl[rb], bh, acc[rb][b]);
+              } else {
+                acc[rb][b] = mfma_bf16(ah[rb], bh, acc[rb][b]);
+                acc[rb][b] = mfma_bf16(ah[rb], bl, acc[rb][b]);
+                acc[rb][b] = mfma_bf16(al[rb], bh, acc[rb][b]);
+              }
+            }
+          }
+        }
+      }
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    }
+#pragma unroll
+    for (int rb = 0; rb < kRB; ++rb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float q[NQ];
+#pragma unroll
+        for (int b = 0; b < NQ; ++b) q[b] = acc[rb][b][r];
+        const int64_t j = rowA + rb * 16 + 4 * (lane >> 4) + r;
+        const bool cand = near_null<MODEL, NQ>(a, q, j, gene_ok, gscal);
+        const double n64 = wave_refine(a, cand, j, gene);
+        wald_test<MODEL, NQ>(a, q, j, gene, gene_ok, gscal, excl, gch, gcs, gce, cand, n64, st, asum);
+      }
+  }
+  st.acat_hi = asum.hi;
+  st.acat_lo = asum.lo;
+  // reduce the 16 partial states of each gene; the staging buffers are free now
+  GeneAcc* red = reinterpret_cast<GeneAcc*>(smem);
+  __syncthreads();
+  red[(wave * 4 + (lane >> 4)) * 16 + gl] = st;
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    GeneAcc t = red[threadIdx.x];
+    for (int q = 1; q < 16; ++q) acc_merge(t, red[q * 16 + threadIdx.x]);
+    a.partial[static_cast<size_t>(split) * a.G_pad + group * 16 + threadIdx.x] = t;
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// Self-test of the hardware conventions the kernels rely on (one workgroup):
+//  bit 0: bf16 16x16x32 MFMA operand/result lane layout
+//  bit 1: fp32 16x16x4 MFMA layout with the 8-k-slot grouping of mfma_f32x8
+//  bit 2: direct global→LDS 16-byte loads land at wave base + 16 * lane
+//  bit 3: the fp64 16x16x4 layout with the grouping of mfma_f64x8
+// Small integers make every product and sum exact, so equality is exact.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_selftest(int* fail) {
+  __shared__ __attribute__((aligned(16))) unsigned char lds[4096];
+  __shared__ int src_pattern[1024];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // A[r][k] = (r + 2k) % 5 - 2, B[k][c] = (3c + k) % 7 - 3, C = A B (16 x 16, K = 32)
+  auto Av = [](int r, int k) { return static_cast<float>((r + 2 * k) % 5 - 2); };
+  auto Bv = [](int k, int c) { return static_cast<float>((3 * c + k) % 7 - 3); };
+  if (wave == 0) {
+    bf16x8 a8, b8;
+    f32x8 af, bfv;
+    for (int t = 0; t < 8; ++t) {
+      const int k = 8 * (lane >> 4) + t;
+      a8[t] = static_cast<__bf16>(Av(lane & 15, k));
+      b8[t] = static_cast<__bf16>(Bv(k, lane & 15));
+      af.v[t] = Av(lane & 15, k);
+      bfv.v[t] = Bv(k, lane & 15);
+    }
+    f64x8 ad, bd;
+    for (int t = 0; t < 8; ++t) { ad.v[t] = af.v[t]; bd.v[t] = bfv.v[t]; }
+    f32x4 c1 = mfma_bf16(a8, b8, f32x4{0, 0, 0, 0});
+    f32x4 c2 = mfma_f32x8(af, bfv, f32x4{0, 0, 0, 0});
+    f64x4 c3 = mfma_f64x8(ad, bd, f64x4{0, 0, 0, 0});
+    int bad1 = 0, bad2 = 0, bad3 = 0;
+    for (int r = 0; r < 4; ++r) {
+      const int row = 4 * (lane >> 4) + r, col = lane & 15;
+      float e = 0;
+      for (int k = 0; k < 32; ++k) e += Av(row, k) * Bv(k, col);
+      bad1 |= c1[r] != e;
+      bad2 |= c2[r] != e;
+      bad3 |= c3[r] != static_cast<double>(e);
+    }
+    if (bad1) atomicOr(fail, 1);
+    if (bad2) atomicOr(fail, 2);
+    if (bad3) atomicOr(fail, 8);
+  }
+  // LDS-direct layout: global int pattern → LDS via 16-byte per-lane loads
+  int* g = reinterpret_cast<int*>(fail + 16);  // caller provides 1024 ints after the flag area
+  for (int i = threadIdx.x; i < 1024; i += 256) src_pattern[i] = 0;
+  __syncthreads();
+  for (int c = wave; c < 4; c += 4) lds_direct16(g + c * 256 + lane * 4, lds + c * 1024);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  const int* li = reinterpret_cast<const int*>(lds);
+  for (int i = threadIdx.x; i < 1024; i += 256)
+    if (li[i] != g[i]) atomicOr(fail, 4);
 }
 
 // Fold one block's partials into the running per-gene state.

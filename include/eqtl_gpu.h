@@ -156,6 +156,9 @@ int eqtl_gpu_synth_bed(eqtl_gpu_ctx* ctx, uint64_t seed, int64_t m, int32_t n, d
 int eqtl_gpu_copy_rows(eqtl_gpu_ctx* ctx, const uint8_t* rows_dev, int64_t j0, int64_t m,
                        int32_t bytes_per_snp, uint8_t* out);
 int eqtl_gpu_synchronize(eqtl_gpu_ctx* ctx);
+/* Checks the device conventions the kernels rely on (matrix-core lane layouts,
+ * direct global-to-LDS loads). *failed_mask = 0 when all hold. */
+int eqtl_gpu_selftest(eqtl_gpu_ctx* ctx, int32_t* failed_mask);
 
 #ifdef __cplusplus
 }

@@ -147,11 +147,21 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--snps", type=int, default=M, help="SNPs per step (default: the full 500k)")
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="first run a 1 GiB device-to-device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
     import eqtl_amd
+    if args.probe_copy:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        log("copy probe: 1 GiB read + 1 GiB written")
+        del x, y
     m = min(args.snps, M)
     ctx = eqtl_amd.Context(0)
     name, ncu, hbm = ctx.device_info()

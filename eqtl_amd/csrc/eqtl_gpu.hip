@@ -18,6 +18,7 @@
 #include "reml_kernels.hpp"
 #include "scan_kernels.hpp"
 #include "grm_kernels.hpp"
+#include "reml_batch.hpp"
 
 using namespace eqtl_amd;
 
@@ -30,6 +31,12 @@ namespace {
 // about 0.8 tau (0.04% / 0.16%).
 constexpr double kRefineTauF32 = 5e-4;
 constexpr double kRefineTauBF16X3 = 2e-3;
+// |t| above which the numerator is refined as well (two-sided p < ~1e-15).
+constexpr double kRefineTauHi = 8.0;
+// Capacity of the per-block list of LMM tests polished in FP64 (|t| > 8), and
+// how many are polished per launch (g~ scratch: chunk x n doubles).
+constexpr int64_t kPolishCap = int64_t(1) << 20;
+constexpr int64_t kPolishChunk = 4096;
 
 thread_local std::string g_err;
 
@@ -87,6 +94,12 @@ struct TimingSlot {
 
 }  // namespace
 
+// Device work buffers of the batched REML search.
+struct RemlWork {
+  DBuf P, dgrid, Dg, logvg, XD, Y2, CAg, Ainv, logdet, okg, Bg, C0, F, best;
+  DBuf W, Z, CA, CB, S, ek, ed, Fo;
+};
+
 struct eqtl_gpu_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -107,8 +120,12 @@ struct eqtl_gpu_ctx {
   DBuf Q64, lam64, Q32, Qhi, Qlo;
   // genes
   int G = 0, G_pad = 0, p = 0, prepped_model = -1, nq = 0;
+  int nq_pad = 0;  // column blocks as stored (>= nq, zero-padded to a compiled kernel size)
   DBuf Yt64, Xt64, X64, Y64, delta, gscal, gvalid, Bf32, Bbf, Wx, zeros;
   DBuf U64, V64;  // per-gene FP64 residual columns (model space / sample space)
+  DBuf Lc;        // per-gene Cholesky factor of X~^T D X~ (LMM)
+  DBuf polish, polish_gt, polish_out;
+  RemlWork rw;
   bool have_windows = false;
   DBuf gene_chrom, gene_cs, gene_ce;
   // scan
@@ -118,8 +135,6 @@ struct eqtl_gpu_ctx {
   DBuf state, partial, hits, hit_count, tail;
   double tail_df = -1;
   int64_t hit_cap_dev = 0;
-  int wald_variant = 2;  // 2: LDS-staged k_wald2; 1: k_wald (EQTL_WALD_VARIANT=1)
-  int last_splits = 1;
   // per-block buffers
   int64_t m_cap = 0;
   DBuf rows, A32, A64, Gt, keep, af, gtg, dosage;
@@ -178,7 +193,7 @@ GeneCols gene_cols(eqtl_gpu_ctx* c) {
   GeneCols g;
   g.f32 = c->Bf32.as<float>();
   g.bf = c->Bbf.as<__bf16>();
-  g.nq = c->nq;
+  g.nq = c->nq_pad;
   g.nks = c->n_pad / 32;
   return g;
 }
@@ -236,45 +251,45 @@ const uint8_t* stage_rows(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, 
 
 template <int MODEL, int PREC, int NQ>
 int launch_wald_one(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
-  if (c->wald_variant == 1) {
-    hipLaunchKernelGGL((k_wald<MODEL, PREC, NQ>), grid, dim3(256), 0, c->stream, a);
-    return check_launch("k_wald");
-  }
-  const size_t lds = std::max<size_t>(2 * static_cast<size_t>(c->nq) * 2048, 256 * sizeof(GeneAcc));
+  const size_t lds = wald_lds_bytes(NQ);
   static bool attr_set = false;
   if (!attr_set) {
-    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald2<MODEL, PREC, NQ>),
+    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald<MODEL, PREC, NQ>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_set = true;
   }
-  // 128-row tiles: split count recomputed for them
-  WaldArgs b = a;
-  const int n_tiles2 = (a.n_tiles + 1) / 2;
-  b.n_splits = std::max(1, std::min(n_tiles2, a.n_splits));
-  dim3 g2(static_cast<unsigned>(a.n_groups_pad * b.n_splits));
-  hipLaunchKernelGGL((k_wald2<MODEL, PREC, NQ>), g2, dim3(256), lds, c->stream, b);
-  c->last_splits = b.n_splits;
-  return check_launch("k_wald2");
+  hipLaunchKernelGGL((k_wald<MODEL, PREC, NQ>), grid, dim3(256), lds, c->stream, a);
+  return check_launch("k_wald");
+}
+
+// Column-block counts the LMM Wald kernel is compiled for (nq = p + 2 is
+// zero-padded up to the next one; p = 16 and p = 21 are exact).
+constexpr int kLmmNq[] = {4, 8, 12, 16, 18, 20, 23, 26, 30, 34};
+
+int lmm_nq_pad(int nq) {
+  for (int v : kLmmNq)
+    if (nq <= v) return v;
+  return -1;
 }
 
 template <int PREC>
 int launch_wald_lmm(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
-  const int nq = c->nq;
-  if (nq <= 4) return launch_wald_one<EQTL_MODEL_LMM, PREC, 4>(c, a, grid);
-  if (nq <= 8) return launch_wald_one<EQTL_MODEL_LMM, PREC, 8>(c, a, grid);
-  if (nq <= 12) return launch_wald_one<EQTL_MODEL_LMM, PREC, 12>(c, a, grid);
-  if (nq <= 16) return launch_wald_one<EQTL_MODEL_LMM, PREC, 16>(c, a, grid);
-  if (nq <= 18) return launch_wald_one<EQTL_MODEL_LMM, PREC, 18>(c, a, grid);
-  if (nq <= 20) return launch_wald_one<EQTL_MODEL_LMM, PREC, 20>(c, a, grid);
-  if (nq <= 23) return launch_wald_one<EQTL_MODEL_LMM, PREC, 23>(c, a, grid);
-  if (nq <= 24) return launch_wald_one<EQTL_MODEL_LMM, PREC, 24>(c, a, grid);
-  if (nq <= 28) return launch_wald_one<EQTL_MODEL_LMM, PREC, 28>(c, a, grid);
-  if (nq <= 34) return launch_wald_one<EQTL_MODEL_LMM, PREC, 34>(c, a, grid);
-  return fail(EQTL_E_ARG, "too many covariates for the Wald kernel");
+  switch (c->nq_pad) {
+    case 4: return launch_wald_one<EQTL_MODEL_LMM, PREC, 4>(c, a, grid);
+    case 8: return launch_wald_one<EQTL_MODEL_LMM, PREC, 8>(c, a, grid);
+    case 12: return launch_wald_one<EQTL_MODEL_LMM, PREC, 12>(c, a, grid);
+    case 16: return launch_wald_one<EQTL_MODEL_LMM, PREC, 16>(c, a, grid);
+    case 18: return launch_wald_one<EQTL_MODEL_LMM, PREC, 18>(c, a, grid);
+    case 20: return launch_wald_one<EQTL_MODEL_LMM, PREC, 20>(c, a, grid);
+    case 23: return launch_wald_one<EQTL_MODEL_LMM, PREC, 23>(c, a, grid);
+    case 26: return launch_wald_one<EQTL_MODEL_LMM, PREC, 26>(c, a, grid);
+    case 30: return launch_wald_one<EQTL_MODEL_LMM, PREC, 30>(c, a, grid);
+    case 34: return launch_wald_one<EQTL_MODEL_LMM, PREC, 34>(c, a, grid);
+    default: return fail(EQTL_E_ARG, "too many covariates for the Wald kernel");
+  }
 }
 
 int launch_wald(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
-  c->last_splits = a.n_splits;
   if (c->scan_model == EQTL_MODEL_LMM) {
     if (c->scan_prec == EQTL_PREC_FP32) return launch_wald_lmm<EQTL_PREC_FP32>(c, a, grid);
     return launch_wald_lmm<EQTL_PREC_BF16X3>(c, a, grid);
@@ -302,7 +317,6 @@ int eqtl_gpu_open(int32_t device, eqtl_gpu_ctx** out) {
     delete c;
     return fail(EQTL_E_HIP, "stream/event creation failed");
   }
-  if (const char* v = std::getenv("EQTL_WALD_VARIANT")) c->wald_variant = std::atoi(v) == 1 ? 1 : 2;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
     c->n_cu = prop.multiProcessorCount;
@@ -431,6 +445,273 @@ static int ensure_blas(eqtl_gpu_ctx* c) {
   return EQTL_OK;
 }
 
+// ---- batched REML (reml_negll search, model_lmm.cpp:39-86) -------------------
+
+constexpr int kEvalChunk = 16384;  // evaluations per GEMM batch
+
+static int dgemm_tn(eqtl_gpu_ctx* c, int m, int nn, int k, const double* A, int lda, const double* B, int ldb, double* C,
+             int ldc) {
+  const double one = 1.0, zero = 0.0;
+  if (rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, m, nn, k, &one, A, lda, B, ldb,
+                    &zero, C, ldc) != rocblas_status_success)
+    return fail(EQTL_E_HIP, "rocblas_dgemm failed");
+  return EQTL_OK;
+}
+
+// Per-sample outer products of X~ (once per lmm_prep).
+static int reml_setup(eqtl_gpu_ctx* c) {
+  const int n = c->n, p = c->p, TA = p * (p + 1) / 2;
+  CHK(ensure(c->rw.P, sizeof(double) * static_cast<size_t>(n) * TA));
+  hipLaunchKernelGGL(k_reml_outer, dim3((n + 255) / 256, TA), dim3(256), 0, c->stream, c->Xt64.as<double>(), n, p,
+                     c->rw.P.as<double>());
+  return check_launch("k_reml_outer");
+}
+
+// f (and f', f'' in ln delta) at (gene ek[e], delta ed[e]) for all e.
+static int reml_eval(eqtl_gpu_ctx* c, const std::vector<int32_t>& ek, const std::vector<double>& ed, bool deriv, double* F,
+              double* Gd, double* H) {
+  const int n = c->n, p = c->p, TA = p * (p + 1) / 2, df = n - p;
+  const int E = static_cast<int>(ek.size());
+  const int nw = deriv ? 3 : 1;
+  auto& r = c->rw;
+  const size_t cols = static_cast<size_t>(nw) * std::min(E, kEvalChunk);
+  CHK(ensure(r.W, sizeof(double) * n * cols));
+  CHK(ensure(r.Z, sizeof(double) * n * cols));
+  CHK(ensure(r.CA, sizeof(double) * TA * cols));
+  CHK(ensure(r.CB, sizeof(double) * p * cols));
+  CHK(ensure(r.S, sizeof(double) * kNumScal * std::min(E, kEvalChunk)));
+  CHK(ensure(r.ek, sizeof(int32_t) * std::min(E, kEvalChunk)));
+  CHK(ensure(r.ed, sizeof(double) * std::min(E, kEvalChunk)));
+  CHK(ensure(r.Fo, sizeof(double) * 3 * std::min(E, kEvalChunk)));
+  for (int e0 = 0; e0 < E; e0 += kEvalChunk) {
+    const int Ec = std::min(kEvalChunk, E - e0);
+    double* Fo = r.Fo.as<double>();
+    double* Go = Fo + Ec;
+    double* Ho = Fo + 2 * Ec;
+    HIPC(hipMemcpyAsync(r.ek.p, ek.data() + e0, sizeof(int32_t) * Ec, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(r.ed.p, ed.data() + e0, sizeof(double) * Ec, hipMemcpyHostToDevice, c->stream));
+    if (deriv)
+      hipLaunchKernelGGL(k_reml_eval_weights<true>, dim3(Ec), dim3(256), 0, c->stream, c->lam64.as<double>(),
+                         c->Yt64.as<double>(), n, Ec, r.ek.as<int32_t>(), r.ed.as<double>(), r.W.as<double>(),
+                         r.Z.as<double>(), r.S.as<double>());
+    else
+      hipLaunchKernelGGL(k_reml_eval_weights<false>, dim3(Ec), dim3(256), 0, c->stream, c->lam64.as<double>(),
+                         c->Yt64.as<double>(), n, Ec, r.ek.as<int32_t>(), r.ed.as<double>(), r.W.as<double>(),
+                         r.Z.as<double>(), r.S.as<double>());
+    CHK(check_launch("k_reml_eval_weights"));
+    CHK(dgemm_tn(c, TA, nw * Ec, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA));
+    CHK(dgemm_tn(c, p, nw * Ec, n, c->Xt64.as<double>(), n, r.Z.as<double>(), n, r.CB.as<double>(), p));
+    if (deriv)
+      hipLaunchKernelGGL(k_reml_eval<true>, dim3(Ec), dim3(64), 0, c->stream, r.CA.as<double>(), r.CB.as<double>(),
+                         r.S.as<double>(), Ec, TA, p, df, Fo, Go, Ho);
+    else
+      hipLaunchKernelGGL(k_reml_eval<false>, dim3(Ec), dim3(64), 0, c->stream, r.CA.as<double>(), r.CB.as<double>(),
+                         r.S.as<double>(), Ec, TA, p, df, Fo, Go, Ho);
+    CHK(check_launch("k_reml_eval"));
+    HIPC(hipMemcpyAsync(F + e0, Fo, sizeof(double) * Ec, hipMemcpyDeviceToHost, c->stream));
+    if (deriv) {
+      HIPC(hipMemcpyAsync(Gd + e0, Go, sizeof(double) * Ec, hipMemcpyDeviceToHost, c->stream));
+      HIPC(hipMemcpyAsync(H + e0, Ho, sizeof(double) * Ec, hipMemcpyDeviceToHost, c->stream));
+    }
+  }
+  HIPC(hipStreamSynchronize(c->stream));
+  return EQTL_OK;
+}
+
+// Reference grid (delta = 1, then 1e-5 * 2^k <= 1e5) for all genes at once.
+static int reml_grid(eqtl_gpu_ctx* c, std::vector<double>& best_d) {
+  const int n = c->n, p = c->p, G = c->G, TA = p * (p + 1) / 2, df = n - p;
+  std::vector<double> dg{1.0};
+  for (double d = 1e-5; d <= 1e5; d *= 2.0) dg.push_back(d);
+  const int ng = static_cast<int>(dg.size());
+  auto& r = c->rw;
+  CHK(ensure(r.dgrid, sizeof(double) * ng));
+  CHK(ensure(r.Dg, sizeof(double) * static_cast<size_t>(n) * ng));
+  CHK(ensure(r.logvg, sizeof(double) * ng));
+  CHK(ensure(r.XD, sizeof(double) * static_cast<size_t>(n) * ng * p));
+  CHK(ensure(r.Y2, sizeof(double) * static_cast<size_t>(n) * G));
+  CHK(ensure(r.CAg, sizeof(double) * static_cast<size_t>(TA) * ng));
+  CHK(ensure(r.Ainv, sizeof(double) * static_cast<size_t>(p) * p * ng));
+  CHK(ensure(r.logdet, sizeof(double) * ng));
+  CHK(ensure(r.okg, sizeof(int32_t) * ng));
+  CHK(ensure(r.Bg, sizeof(double) * static_cast<size_t>(ng) * p * G));
+  CHK(ensure(r.C0, sizeof(double) * static_cast<size_t>(ng) * G));
+  CHK(ensure(r.F, sizeof(double) * static_cast<size_t>(ng) * G));
+  CHK(ensure(r.best, sizeof(double) * 2 * G));
+  HIPC(hipMemcpyAsync(r.dgrid.p, dg.data(), sizeof(double) * ng, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_reml_grid_weights, dim3(ng), dim3(256), 0, c->stream, c->lam64.as<double>(), n,
+                     r.dgrid.as<double>(), r.Dg.as<double>(), r.logvg.as<double>());
+  CHK(check_launch("k_reml_grid_weights"));
+  hipLaunchKernelGGL(k_reml_grid_xd, dim3((n + 255) / 256, ng * p), dim3(256), 0, c->stream, c->Xt64.as<double>(),
+                     r.Dg.as<double>(), n, p, r.XD.as<double>());
+  CHK(check_launch("k_reml_grid_xd"));
+  const size_t NG = static_cast<size_t>(n) * G;
+  hipLaunchKernelGGL(k_square, dim3(static_cast<unsigned>((NG + 255) / 256)), dim3(256), 0, c->stream,
+                     c->Yt64.as<double>(), NG, r.Y2.as<double>());
+  CHK(check_launch("k_square"));
+  CHK(dgemm_tn(c, TA, ng, n, r.P.as<double>(), n, r.Dg.as<double>(), n, r.CAg.as<double>(), TA));
+  CHK(dgemm_tn(c, ng * p, G, n, r.XD.as<double>(), n, c->Yt64.as<double>(), n, r.Bg.as<double>(), ng * p));
+  CHK(dgemm_tn(c, ng, G, n, r.Dg.as<double>(), n, r.Y2.as<double>(), n, r.C0.as<double>(), ng));
+  hipLaunchKernelGGL(k_reml_grid_factor, dim3(ng), dim3(64), 0, c->stream, r.CAg.as<double>(), TA, p,
+                     r.Ainv.as<double>(), r.logdet.as<double>(), r.okg.as<int32_t>());
+  CHK(check_launch("k_reml_grid_factor"));
+  hipLaunchKernelGGL(k_reml_grid_f, dim3((G + 255) / 256, ng), dim3(256), 0, c->stream, r.Ainv.as<double>(),
+                     r.CAg.as<double>(), TA, r.logdet.as<double>(), r.okg.as<int32_t>(), r.logvg.as<double>(), r.Bg.as<double>(),
+                     r.C0.as<double>(), ng, p, G, df, r.F.as<double>());
+  CHK(check_launch("k_reml_grid_f"));
+  hipLaunchKernelGGL(k_reml_grid_argmin, dim3((G + 255) / 256), dim3(256), 0, c->stream, r.F.as<double>(),
+                     r.dgrid.as<double>(), ng, G, r.best.as<double>(), r.best.as<double>() + G);
+  CHK(check_launch("k_reml_grid_argmin"));
+  best_d.resize(G);
+  HIPC(hipMemcpyAsync(best_d.data(), r.best.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return EQTL_OK;
+}
+
+// Per-gene delta for all genes. mode EQTL_DELTA_REF: the reference search
+// (grid, 40 golden-section steps on [best/4, best*4] clipped to [1e-8, 1e6],
+// then at most 8 central-difference Newton steps with the reference's
+// acceptance rules), each stage run for every gene in lock-step; mode
+// EQTL_DELTA_NEWTON: the same grid and bracket, then safeguarded Newton on
+// ln delta with the analytic derivatives and step halving.
+static int reml_search(eqtl_gpu_ctx* c, int mode, std::vector<double>& delta) {
+  const int G = c->G, n = c->n, p = c->p;
+  delta.assign(G, 1.0);
+  if (n - p <= 0) return EQTL_OK;
+  CHK(reml_setup(c));
+  std::vector<double> best;
+  CHK(reml_grid(c, best));
+  std::vector<double> lo(G), hi(G);
+  for (int k = 0; k < G; ++k) {
+    lo[k] = std::max(best[k] / 4.0, 1e-8);
+    hi[k] = std::min(best[k] * 4.0, 1e6);
+  }
+  std::vector<int32_t> ek;
+  std::vector<double> ed, F, Gd, H;
+  if (mode == EQTL_DELTA_REF) {
+    const double phi = (1.0 + std::sqrt(5.0)) / 2.0;
+    ek.resize(2 * G);
+    ed.resize(2 * G);
+    F.resize(2 * G);
+    for (int k = 0; k < G; ++k) ek[k] = ek[G + k] = k;
+    for (int it = 0; it < 40; ++it) {
+      for (int k = 0; k < G; ++k) {
+        ed[k] = hi[k] - (hi[k] - lo[k]) / phi;
+        ed[G + k] = lo[k] + (hi[k] - lo[k]) / phi;
+      }
+      CHK(reml_eval(c, ek, ed, false, F.data(), nullptr, nullptr));
+      for (int k = 0; k < G; ++k) {
+        if (F[k] < F[G + k]) hi[k] = ed[G + k];
+        else lo[k] = ed[k];
+      }
+    }
+    std::vector<double> d(G), cur(G);
+    std::vector<int32_t> all(G);
+    for (int k = 0; k < G; ++k) { d[k] = 0.5 * (lo[k] + hi[k]); all[k] = k; }
+    CHK(reml_eval(c, all, d, false, cur.data(), nullptr, nullptr));
+    std::vector<char> active(G, 1);
+    for (int it = 0; it < 8; ++it) {
+      std::vector<int32_t> ks;
+      for (int k = 0; k < G; ++k)
+        if (active[k]) ks.push_back(k);
+      if (ks.empty()) break;
+      const int A = static_cast<int>(ks.size());
+      ek.assign(2 * A, 0);
+      ed.assign(2 * A, 0.0);
+      std::vector<double> hstep(A);
+      for (int q = 0; q < A; ++q) {
+        const int k = ks[q];
+        hstep[q] = std::max(d[k] * 1e-4, 1e-9);
+        ek[q] = ek[A + q] = k;
+        ed[q] = d[k] - hstep[q];
+        ed[A + q] = d[k] + hstep[q];
+      }
+      F.assign(2 * A, 0.0);
+      CHK(reml_eval(c, ek, ed, false, F.data(), nullptr, nullptr));
+      std::vector<int32_t> k2;
+      std::vector<double> dn2, step2;
+      for (int q = 0; q < A; ++q) {
+        const int k = ks[q];
+        const double h = hstep[q], fm = F[q], fp = F[A + q];
+        const double g1 = (fp - fm) / (2.0 * h);
+        const double g2 = (fp - 2.0 * cur[k] + fm) / (h * h);
+        if (!(g2 > 1e-12)) { active[k] = 0; continue; }
+        const double step = g1 / g2;
+        const double dn = d[k] - step;
+        if (!(dn > 1e-8 && dn < 1e6)) { active[k] = 0; continue; }
+        k2.push_back(k);
+        dn2.push_back(dn);
+        step2.push_back(step);
+      }
+      if (k2.empty()) break;
+      std::vector<double> vn(k2.size());
+      CHK(reml_eval(c, k2, dn2, false, vn.data(), nullptr, nullptr));
+      for (size_t q = 0; q < k2.size(); ++q) {
+        const int k = k2[q];
+        if (!(vn[q] < cur[k])) { active[k] = 0; continue; }
+        d[k] = dn2[q];
+        cur[k] = vn[q];
+        if (std::fabs(step2[q]) < d[k] * 1e-8) active[k] = 0;
+      }
+    }
+    delta = d;
+    return EQTL_OK;
+  }
+  // safeguarded Newton on theta = ln delta inside the bracket
+  std::vector<double> th(G), f(G), g(G), h(G), tn(G), tlo(G), thi(G);
+  std::vector<int> back(G, 0), iters(G, 0);
+  std::vector<char> state(G, 1);  // 1: propose a step, 2: trial pending, 0: done
+  std::vector<int32_t> all(G);
+  for (int k = 0; k < G; ++k) {
+    all[k] = k;
+    th[k] = std::log(best[k]);
+    tlo[k] = std::log(lo[k]);
+    thi[k] = std::log(hi[k]);
+    ed.push_back(std::exp(th[k]));
+  }
+  CHK(reml_eval(c, all, ed, true, f.data(), g.data(), h.data()));
+  auto propose = [&](int k) {
+    if (iters[k] >= 60) { state[k] = 0; return; }
+    double step = (h[k] > 0) ? -g[k] / h[k] : (g[k] > 0 ? -0.5 : 0.5);
+    step = std::max(-1.0, std::min(1.0, step));
+    const double t = std::min(thi[k], std::max(tlo[k], th[k] + step));
+    if (std::fabs(t - th[k]) < 1e-12) { state[k] = 0; return; }
+    tn[k] = t;
+    back[k] = 0;
+    state[k] = 2;
+  };
+  for (int k = 0; k < G; ++k) propose(k);
+  for (int round = 0; round < 60 * 32; ++round) {
+    ek.clear();
+    ed.clear();
+    for (int k = 0; k < G; ++k)
+      if (state[k] == 2) { ek.push_back(k); ed.push_back(std::exp(tn[k])); }
+    if (ek.empty()) break;
+    const size_t A = ek.size();
+    F.assign(A, 0.0); Gd.assign(A, 0.0); H.assign(A, 0.0);
+    CHK(reml_eval(c, ek, ed, true, F.data(), Gd.data(), H.data()));
+    for (size_t q = 0; q < A; ++q) {
+      const int k = ek[q];
+      if (F[q] > f[k] + 1e-13 * std::fabs(f[k])) {
+        if (back[k] < 30) {
+          tn[k] = th[k] + 0.5 * (tn[k] - th[k]);
+          ++back[k];
+        } else {
+          state[k] = 0;
+        }
+        continue;
+      }
+      const double moved = std::fabs(tn[k] - th[k]);
+      th[k] = tn[k]; f[k] = F[q]; g[k] = Gd[q]; h[k] = H[q];
+      ++iters[k];
+      if (moved < 1e-10) { state[k] = 0; continue; }
+      propose(k);
+    }
+  }
+  for (int k = 0; k < G; ++k) delta[k] = std::exp(th[k]);
+  return EQTL_OK;
+}
+
 int eqtl_gpu_eigh(eqtl_gpu_ctx* c, const double* K, int32_t n, double* Q_out, double* lam_out) {
   if (!c || !K || n <= 0) return fail(EQTL_E_ARG, "bad eigh arguments");
   if (c->n != n) return fail(EQTL_E_ARG, "GRM size differs from the analysis sample count");
@@ -474,11 +755,13 @@ int eqtl_gpu_set_basis(eqtl_gpu_ctx* c, const double* Q, const double* lam, int3
 }
 
 // ---- gene preparation -------------------------------------------------------
-static int alloc_gene_side(eqtl_gpu_ctx* c, int G, int nq) {
+static int alloc_gene_side(eqtl_gpu_ctx* c, int G, int nq, int nq_pad) {
+  if (nq_pad < nq) return fail(EQTL_E_ARG, "too many covariates for the Wald kernel");
   c->G = G;
   c->G_pad = static_cast<int>(round_up(G, 128));  // multiple of 8 groups of 16
   c->nq = nq;
-  const size_t cols = static_cast<size_t>(c->G_pad / 16) * nq * 16 * c->n_pad;
+  c->nq_pad = nq_pad;
+  const size_t cols = static_cast<size_t>(c->G_pad / 16) * nq_pad * 16 * c->n_pad;
   CHK(ensure(c->Bf32, cols * sizeof(float)));
   CHK(ensure(c->Bbf, 2 * cols * sizeof(__bf16)));
   HIPC(hipMemsetAsync(c->Bf32.p, 0, cols * sizeof(float), c->stream));
@@ -500,13 +783,14 @@ int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double*
   CHK(set_device(c));
   CHK(ensure_blas(c));
   c->p = p;
-  CHK(alloc_gene_side(c, G, p + 2));
+  CHK(alloc_gene_side(c, G, p + 2, lmm_nq_pad(p + 2)));
   CHK(ensure(c->Y64, sizeof(double) * static_cast<size_t>(n) * G));
   CHK(ensure(c->X64, sizeof(double) * static_cast<size_t>(n) * p));
   CHK(ensure(c->Yt64, sizeof(double) * static_cast<size_t>(n) * G));
   CHK(ensure(c->Xt64, sizeof(double) * static_cast<size_t>(n) * p));
   CHK(ensure(c->U64, sizeof(double) * static_cast<size_t>(n) * G));
   CHK(ensure(c->V64, sizeof(double) * static_cast<size_t>(n) * G));
+  CHK(ensure(c->Lc, sizeof(double) * static_cast<size_t>(p) * p * G));
   HIPC(hipMemcpyAsync(c->Y64.p, Y, sizeof(double) * static_cast<size_t>(n) * G, hipMemcpyHostToDevice, c->stream));
   HIPC(hipMemcpyAsync(c->X64.p, X, sizeof(double) * static_cast<size_t>(n) * p, hipMemcpyHostToDevice, c->stream));
   // K4: Y~ = Q^T Y, X~ = Q^T X once for all genes (FP64 library GEMM)
@@ -526,16 +810,17 @@ int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double*
     HIPC(hipMemcpyAsync(c->delta.p, delta_in, sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
   } else {
     Timed t(c, "reml");
-    hipLaunchKernelGGL(k_reml, dim3(G), dim3(256), 0, c->stream, c->Xt64.as<double>(), c->Yt64.as<double>(),
-                       c->lam64.as<double>(), n, p, G, mode == EQTL_DELTA_NEWTON ? 1 : 0, c->delta.as<double>());
-    CHK(check_launch("k_reml"));
+    std::vector<double> dl;
+    CHK(reml_search(c, mode, dl));
+    HIPC(hipMemcpyAsync(c->delta.p, dl.data(), sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
     CHK(t.done());
   }
   {
     Timed t(c, "basis");
     hipLaunchKernelGGL(k_lmm_basis, dim3(G), dim3(256), 0, c->stream, c->Xt64.as<double>(), c->Yt64.as<double>(),
                        c->lam64.as<double>(), n, c->n_pad, p, G, c->delta.as<double>(), gene_cols(c),
-                       c->gscal.as<double>(), c->gvalid.as<int>(), c->U64.as<double>());
+                       c->gscal.as<double>(), c->gvalid.as<int>(), c->U64.as<double>(), c->Lc.as<double>());
     CHK(check_launch("k_lmm_basis"));
     // sample-space residual vectors for the near-null FP64 refinement: V = Q U
     if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, G, n, &one, c->Q64.as<double>(), n,
@@ -559,21 +844,15 @@ int eqtl_gpu_reml_grad_hess(eqtl_gpu_ctx* c, const double* log_delta, int32_t G,
   if (!c || !log_delta || !f || !g || !h) return fail(EQTL_E_ARG, "bad reml_grad_hess arguments");
   if (c->prepped_model != EQTL_MODEL_LMM || G != c->G) return fail(EQTL_E_STATE, "call eqtl_gpu_lmm_prep first");
   CHK(set_device(c));
-  DBuf ld, F, Gd, H;
-  CHK(ensure(ld, sizeof(double) * G));
-  CHK(ensure(F, sizeof(double) * G));
-  CHK(ensure(Gd, sizeof(double) * G));
-  CHK(ensure(H, sizeof(double) * G));
-  HIPC(hipMemcpyAsync(ld.p, log_delta, sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_reml_grad_hess, dim3(G), dim3(256), 0, c->stream, c->Xt64.as<double>(), c->Yt64.as<double>(),
-                     c->lam64.as<double>(), c->n, c->p, G, ld.as<double>(), F.as<double>(), Gd.as<double>(),
-                     H.as<double>());
-  CHK(check_launch("k_reml_grad_hess"));
-  HIPC(hipMemcpyAsync(f, F.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
-  HIPC(hipMemcpyAsync(g, Gd.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
-  HIPC(hipMemcpyAsync(h, H.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
-  HIPC(hipStreamSynchronize(c->stream));
-  return EQTL_OK;
+  CHK(ensure_blas(c));
+  CHK(reml_setup(c));
+  std::vector<int32_t> ek(G);
+  std::vector<double> ed(G);
+  for (int k = 0; k < G; ++k) {
+    ek[k] = k;
+    ed[k] = std::exp(log_delta[k]);
+  }
+  return reml_eval(c, ek, ed, true, f, g, h);
 }
 
 int eqtl_gpu_lm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double* X, int32_t p, double* yty_out) {
@@ -583,7 +862,7 @@ int eqtl_gpu_lm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double* 
   const int n = c->n;
   CHK(set_device(c));
   c->p = p;
-  CHK(alloc_gene_side(c, G, 1));
+  CHK(alloc_gene_side(c, G, 1, 1));
   CHK(ensure(c->Y64, sizeof(double) * static_cast<size_t>(n) * G));
   CHK(ensure(c->X64, sizeof(double) * static_cast<size_t>(n) * p));
   CHK(ensure(c->Wx, sizeof(double) * static_cast<size_t>(n) * p));
@@ -641,7 +920,7 @@ int eqtl_gpu_scan_begin(eqtl_gpu_ctx* c, int32_t model, int32_t prec, double pth
   hipLaunchKernelGGL(k_state_init, dim3((c->G_pad + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
                      c->G_pad);
   CHK(check_launch("k_state_init"));
-  CHK(ensure(c->hit_count, sizeof(unsigned long long)));
+  CHK(ensure(c->hit_count, 2 * sizeof(unsigned long long)));
   // t-tail table for this scan's residual df (shared by every test)
   const double df = static_cast<double>(c->n - c->p - 1);
   if (df > 0 && c->tail_df != df) {
@@ -699,14 +978,22 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   const int n_groups = c->G_pad / 16;
   const int n_groups_pad = static_cast<int>(round_up(n_groups, 8));
   const int n_tiles = static_cast<int>(m_pad / 64);
-  int n_splits = std::max(1, std::min(n_tiles, (4 * std::max(c->n_cu, 1) * 2 + n_groups - 1) / n_groups));
+  const int n_tiles128 = static_cast<int>(m_pad / kTileRows);
+  // Each gene group is split over as many workgroups as one XCD has CUs, and
+  // the kernels place a group's splits on one XCD: the XCD then sweeps its
+  // SNP tiles against one group at a time, so the group's column blocks are
+  // read from HBM once and served from that XCD's L2 to all of its CUs.
+  const int cus = std::max(c->n_cu, 8);
+  int n_splits = std::max(cus / 8, (2 * cus + n_groups - 1) / n_groups);
+  n_splits = std::max(1, std::min(n_tiles128, n_splits));
   CHK(ensure(c->partial, sizeof(GeneAcc) * static_cast<size_t>(n_splits) * c->G_pad));
   const int64_t cap_dev = std::max<int64_t>(hit_cap, 1);
   if (c->hit_cap_dev < cap_dev) {
     CHK(ensure(c->hits, sizeof(eqtl_hit) * static_cast<size_t>(cap_dev)));
     c->hit_cap_dev = cap_dev;
   }
-  HIPC(hipMemsetAsync(c->hit_count.p, 0, sizeof(unsigned long long), c->stream));
+  HIPC(hipMemsetAsync(c->hit_count.p, 0, 2 * sizeof(unsigned long long), c->stream));
+  if (!lm) CHK(ensure(c->polish, sizeof(PolishItem) * static_cast<size_t>(kPolishCap)));
   WaldArgs a;
   a.A = Aop;
   a.gc = gene_cols(c);
@@ -741,18 +1028,59 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   a.vref = lm ? c->U64.as<double>() : c->V64.as<double>();
   a.n = c->n;
   a.tau = c->scan_prec == EQTL_PREC_FP32 ? kRefineTauF32 : kRefineTauBF16X3;
+  a.tau_hi = kRefineTauHi;
+  a.polish = lm ? nullptr : c->polish.as<PolishItem>();
+  a.polish_cap = lm ? 0 : kPolishCap;
   {
     Timed t(c, "wald");
     CHK(launch_wald(c, a, dim3(static_cast<unsigned>(n_groups_pad * n_splits))));
     CHK(t.done());
   }
   hipLaunchKernelGGL(k_merge, dim3((c->G + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
-                     c->partial.as<GeneAcc>(), c->G, c->G_pad, c->last_splits);
+                     c->partial.as<GeneAcc>(), c->G, c->G_pad, n_splits);
   CHK(check_launch("k_merge"));
-  unsigned long long cnt = 0;
-  HIPC(hipMemcpyAsync(&cnt, c->hit_count.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+  unsigned long long cnt2[2] = {0, 0};
+  HIPC(hipMemcpyAsync(cnt2, c->hit_count.p, sizeof(cnt2), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
-  const int64_t got = static_cast<int64_t>(cnt);
+  const int64_t got = static_cast<int64_t>(cnt2[0]);
+  const int64_t npol = static_cast<int64_t>(cnt2[1]);
+  if (npol > kPolishCap) return fail(EQTL_E_CAPACITY, "more than 2^20 tests with |t| > 8 in one block");
+  if (npol > 0) {
+    Timed t(c, "polish");
+    const int64_t chunk = std::min<int64_t>(npol, kPolishChunk);
+    CHK(ensure(c->polish_gt, sizeof(double) * static_cast<size_t>(chunk) * c->n));
+    CHK(ensure(c->polish_out, sizeof(eqtl_hit) * static_cast<size_t>(chunk)));
+    PolishArgs pa;
+    pa.A64 = c->A64.as<double>();
+    pa.n_pad = np;
+    pa.n = c->n;
+    pa.p = c->p;
+    pa.Q = c->Q64.as<double>();
+    pa.lam = c->lam64.as<double>();
+    pa.delta = c->delta.as<double>();
+    pa.U = c->U64.as<double>();
+    pa.Xt = c->Xt64.as<double>();
+    pa.Lc = c->Lc.as<double>();
+    pa.gene_scal = c->gscal.as<double>();
+    pa.af = c->af.as<double>();
+    pa.df = a.df;
+    pa.lbeta = std::lgamma(0.5 * a.df) + std::lgamma(0.5) - std::lgamma(0.5 * a.df + 0.5);
+    pa.snp0 = snp0;
+    pa.gt = c->polish_gt.as<double>();
+    pa.hits = c->hits.as<eqtl_hit>();
+    pa.out = c->polish_out.as<eqtl_hit>();
+    pa.state = c->state.as<GeneAcc>();
+    for (int64_t i0 = 0; i0 < npol; i0 += chunk) {
+      pa.items = c->polish.as<PolishItem>() + i0;
+      pa.count = std::min<int64_t>(chunk, npol - i0);
+      hipLaunchKernelGGL(k_polish_lmm, dim3(static_cast<unsigned>(pa.count)), dim3(256), 0, c->stream, pa);
+      CHK(check_launch("k_polish_lmm"));
+      hipLaunchKernelGGL(k_polish_best, dim3(static_cast<unsigned>((pa.count + 255) / 256)), dim3(256), 0,
+                         c->stream, c->state.as<GeneAcc>(), pa.out, pa.count);
+      CHK(check_launch("k_polish_best"));
+    }
+    CHK(t.done());
+  }
   if (n_hits) *n_hits = got;
   const int64_t take = std::min<int64_t>(got, hit_cap);
   if (take > 0) {

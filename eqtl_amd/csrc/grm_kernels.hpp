@@ -85,9 +85,10 @@ __global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, in
 }
 
 // One workgroup per 64x64 tile (by >= bx, lower block triangle). Z^T Z on the
-// fp64 matrix cores (v_mfma_f64_16x16x4f64, same lane layout as the fp32
-// 16x16x4 form: per instruction t, lane group l>>4 supplies k = 8*(l>>4)+t of
-// the current 32-SNP step), so K carries no reduced-precision rounding.
+// fp64 matrix cores (v_mfma_f64_16x16x4f64: operands as in the fp32 16x16x4
+// form — per instruction t, lane group l>>4 supplies k = 8*(l>>4)+t of the
+// current 32-SNP step — while result element r of lane l is row (l>>4) + 4r),
+// so K carries no reduced-precision rounding.
 __global__ __launch_bounds__(256) void k_grm_gram(const double* z64, const __bf16* mis, int m_pad, int n_pad,
                                                   double* acc, int32_t* joint, int any_missing) {
   const int bx = blockIdx.x, by = blockIdx.y;
@@ -121,9 +122,11 @@ __global__ __launch_bounds__(256) void k_grm_gram(const double* z64, const __bf1
   for (int cb = 0; cb < 4; ++cb)
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const size_t o = static_cast<size_t>(r0 + 4 * (lane >> 4) + r) * n_pad + c0 + cb * 16 + (lane & 15);
+      // fp64 result rows are interleaved ((l >> 4) + 4 r); the bf16 count tile uses 4 (l >> 4) + r
+      const size_t o = static_cast<size_t>(r0 + (lane >> 4) + 4 * r) * n_pad + c0 + cb * 16 + (lane & 15);
       acc[o] += a64[cb][r];
-      joint[o] += static_cast<int32_t>(accm[cb][r]);
+      const size_t om = static_cast<size_t>(r0 + 4 * (lane >> 4) + r) * n_pad + c0 + cb * 16 + (lane & 15);
+      joint[om] += static_cast<int32_t>(accm[cb][r]);
     }
 }
 

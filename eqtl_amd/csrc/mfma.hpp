@@ -4,6 +4,7 @@
 //   A (16 rows x K): lane l holds row (l & 15), k-slots 8*(l >> 4) + [0..7]
 //   B (K x 16 cols): lane l holds col (l & 15), k-slots 8*(l >> 4) + [0..7]
 //   C (16 x 16)    : lane l holds col (l & 15), rows 4*(l >> 4) + [0..3]
+//                    (fp64 16x16x4: rows (l >> 4) + 4*[0..3])
 // Both operands are K-contiguous in memory, so each lane loads 8 consecutive
 // k values (16 B of bf16, 32 B of fp32) for its row/column.
 //
@@ -41,7 +42,8 @@ __device__ inline f32x4 mfma_f32x8(const f32x8& a, const f32x8& b, f32x4 c) {
   return c;
 }
 
-// fp64 16x16x4: same lane layout as the fp32 form, 8 instructions per 32 k.
+// fp64 16x16x4: operands laid out as in the fp32 form (8 instructions per 32 k);
+// the result differs: lane l holds col (l & 15), rows (l >> 4) + 4 * [0..3].
 __device__ inline f64x4 mfma_f64x8(const f64x8& a, const f64x8& b, f64x4 c) {
 #pragma unroll
   for (int t = 0; t < 8; ++t) c = __builtin_amdgcn_mfma_f64_16x16x4f64(a.v[t], b.v[t], c, 0, 0, 0);

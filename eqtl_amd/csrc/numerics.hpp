@@ -7,6 +7,7 @@
 //  * double-double accumulation for the per-gene tan sums.
 #pragma once
 #include <math.h>
+#include <stddef.h>
 #include <stdint.h>
 #if defined(__HIPCC__) || defined(__HIP__)
 #include <hip/hip_runtime.h>
@@ -171,12 +172,44 @@ EQ_HD double acat_term_ld(double p) {
   return neg ? -t : t;
 }
 
+// The same term in double precision for 1e-6 <= p <= 1 - 1e-6, from
+// sin/cos of pi times an exactly formed argument (tan((0.5 - p) pi) =
+// cot(pi p) = tan(pi (0.5 - p)) = -cot(pi (1 - p)) on the three ranges). There
+// the reference's own rounding of its long double argument moves the term by
+// at most ~5e-14 relative (|dy| <= 2^-64 next to pi/2, divided by pi min(p, 1-p)),
+// so this agrees with the x87 result to that level; outside the range (tiny p,
+// or p within 1e-6 of 1) the x87 emulation above is used.
+EQ_HD double acat_term_fast(double p) {
+#ifdef __HIP_DEVICE_COMPILE__
+  double s, c;
+  if (p < 0.25) {
+    sincospi(p, &s, &c);
+    return c / s;
+  }
+  if (p <= 0.75) {
+    sincospi(0.5 - p, &s, &c);
+    return s / c;
+  }
+  sincospi(1.0 - p, &s, &c);
+  return -c / s;
+#else
+  const long double pi = 3.14159265358979323846264338327950288L;
+  if (p < 0.25) return static_cast<double>(cosl(pi * p) / sinl(pi * p));
+  if (p <= 0.75) return static_cast<double>(tanl(pi * (0.5L - p)));
+  return static_cast<double>(-cosl(pi * (1.0L - p)) / sinl(pi * (1.0L - p)));
+#endif
+}
+
+EQ_HD double acat_term(double p) {
+  return (p >= 1e-6 && p <= 1.0 - 1e-6) ? acat_term_fast(p) : acat_term_ld(p);
+}
+
 // AcatAcc::add skip rules (stats_extra.cpp:17-19); returns false if skipped.
 EQ_HD bool acat_accept(double p, double* term) {
   if (!isfinite(p) || p < 0.0) return false;
   if (p >= 1.0 - 1e-12) return false;
   if (p <= 0.0) p = 1e-300;
-  *term = acat_term_ld(p);
+  *term = acat_term(p);
   return true;
 }
 
@@ -230,6 +263,7 @@ struct TTail {
   double coef_a[kTailA * kTailDeg];
   double coef_b[kTailB * kTailDeg];
 };
+static_assert(sizeof(double) * 8 == offsetof(TTail, coef_a), "TTail header layout");
 
 EQ_HD double tail_poly(const double* c, double z) {
   double r = c[kTailDeg - 1];
@@ -237,27 +271,36 @@ EQ_HD double tail_poly(const double* c, double z) {
   return r;
 }
 
-EQ_HD double p_from_t_tab(const TTail& T, double t) {
-  if (!isfinite(t) || T.df <= 0) return 1.0;
-  const double x = T.df / (T.df + t * t);
+// The scalar header (8 doubles) and piece A's coefficients lead the struct, so
+// a kernel can stage those 8 + kTailA * kTailDeg doubles in LDS and read only
+// piece B (larger |t|) from global memory.
+constexpr int kTailHead = 8 + kTailA * kTailDeg;
+
+// head: df, half_df, u_sw, inv_h_a, w_lo, w_hi, inv_hw_b, x_zero, coef_a[]
+EQ_HD double p_from_t_tab2(const double* head, const double* coef_b, double t) {
+  const double df = head[0];
+  if (!isfinite(t) || df <= 0) return 1.0;
+  const double x = df / (df + t * t);
   const double y = 1.0 - x;
   const double u = sqrt(y);
-  if (u <= T.u_sw) {
-    const double q = u * T.inv_h_a;
+  if (u <= head[2]) {
+    const double q = u * head[3];
     int i = static_cast<int>(q);
     if (i >= kTailA) i = kTailA - 1;
     const double z = 2.0 * (q - i) - 1.0;
-    return fmin(1.0, fmax(0.0, 1.0 - u * tail_poly(T.coef_a + i * kTailDeg, z)));
+    return fmin(1.0, fmax(0.0, 1.0 - u * tail_poly(head + 8 + i * kTailDeg, z)));
   }
-  if (x <= T.x_zero) return 0.0;
-  const double w = log(-T.half_df * log(x));
-  const double q = (w - T.w_lo) * T.inv_hw_b;
+  if (x <= head[7]) return 0.0;
+  const double w = log(-head[1] * log(x));
+  const double q = (w - head[4]) * head[6];
   int i = static_cast<int>(q);
   if (i < 0) i = 0;
   if (i >= kTailB) i = kTailB - 1;
   const double z = 2.0 * (q - i) - 1.0;
-  return fmin(1.0, fmax(0.0, exp(tail_poly(T.coef_b + i * kTailDeg, z))));
+  return fmin(1.0, fmax(0.0, exp(tail_poly(coef_b + i * kTailDeg, z))));
 }
+
+EQ_HD double p_from_t_tab(const TTail& T, double t) { return p_from_t_tab2(&T.df, T.coef_b, t); }
 
 // Host: fit one interval (Chebyshev interpolation, then monomial in z).
 inline void tail_fit_interval(double (*f)(double, const void*), const void* ctx, double lo, double hi,

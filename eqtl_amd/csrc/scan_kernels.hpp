@@ -22,6 +22,7 @@ constexpr int kWave = 64;
 // Per-gene running summary (device side of eqtl_gene_state).
 struct GeneAcc {
   double acat_hi, acat_lo;
+  double acat_pol;  // ACAT terms of tests polished in FP64 after the Wald kernel
   double best_p;
   int64_t best_snp;
   double best_af, best_beta, best_se, best_stat, best_r2;
@@ -29,7 +30,7 @@ struct GeneAcc {
 };
 
 __host__ __device__ inline void acc_init(GeneAcc& a) {
-  a.acat_hi = 0; a.acat_lo = 0; a.best_p = 2.0; a.best_snp = -1;
+  a.acat_hi = 0; a.acat_lo = 0; a.acat_pol = 0; a.best_p = 2.0; a.best_snp = -1;
   a.best_af = a.best_beta = a.best_se = a.best_stat = a.best_r2 = 0;
   a.used = a.n_tested = a.n_sig = 0; a.pad = 0;
 }
@@ -38,6 +39,7 @@ __host__ __device__ inline void acc_init(GeneAcc& a) {
 __host__ __device__ inline void acc_merge(GeneAcc& a, const GeneAcc& b) {
   const dd s = dd_add(dd{a.acat_hi, a.acat_lo}, dd{b.acat_hi, b.acat_lo});
   a.acat_hi = s.hi; a.acat_lo = s.lo;
+  a.acat_pol += b.acat_pol;
   a.used += b.used; a.n_tested += b.n_tested; a.n_sig += b.n_sig;
   if (b.best_snp >= 0 && (b.best_p < a.best_p || (b.best_p == a.best_p && (a.best_snp < 0 || b.best_snp < a.best_snp)))) {
     a.best_p = b.best_p; a.best_snp = b.best_snp; a.best_af = b.best_af; a.best_beta = b.best_beta;
@@ -194,14 +196,27 @@ __global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ A, con
 }
 
 // ---------------------------------------------------------------------------
-// K7/K8: fused Wald. One WG = one group of 16 genes x a strided set of
-// 64-SNP tiles; wave w owns rows 16w..16w+15 of a tile. Column block b of the
-// group holds quantity b for genes 0..15 (lane & 15), so each lane ends with
-// all NQ quantities of one (SNP row, gene) pair per output register r:
+// K7/K8: fused Wald. One workgroup = one group of 16 genes x a strided set of
+// 128-SNP tiles; wave w owns rows 32w..32w+31 of a tile as two 16-row blocks,
+// so every B fragment read from LDS feeds two MFMA chains. Column block b of
+// the group holds quantity b for genes 0..15 (lane & 15), so each lane ends
+// with all NQ quantities of one (SNP row, gene) pair per output register r:
 //   LMM: b=0 d (against A^2 → gg), b=1..p whitened orthonormal covariates
 //        (→ W^T h), b=p+1 whitened null residual (→ r^T h)
 //   LM : b=0 residualised phenotype y_s (→ g^T y_s)
+// The group's k-step tile (32 samples x 16 genes x NQ blocks) is staged in
+// LDS, double-buffered and filled by direct global→LDS 16-byte loads, and
+// shared by the 4 waves. Right after the contraction the accumulators are
+// folded to (gg, S = gg - |W^T h|^2, num) per test, so the FP64 epilogue runs
+// with the accumulator registers free; the tile's SNP data and the t tail's
+// head sit in LDS.
 // ---------------------------------------------------------------------------
+struct PolishItem {
+  int32_t j;     // block-local SNP row
+  int32_t gene;
+  int64_t hit;   // index in the block's hit list, -1 if not a hit
+};
+
 struct WaldArgs {
   const float* A;          // m_pad x n_pad
   GeneCols gc;             // gene-side operand (fragment-major tiles)
@@ -221,53 +236,79 @@ struct WaldArgs {
   const TTail* tail;          // t tail table for df
   GeneAcc* partial;           // [n_splits][G_pad]
   eqtl_hit* hits;
-  unsigned long long* hit_count;
+  unsigned long long* hit_count;  // [0] hits, [1] polish items
   int64_t hit_cap;
-  // near-null refinement (tau = 0 disables it)
+  // FP64 numerator refinement (tau = 0 and tau_hi = inf disable it)
   const double* A64;          // m_pad x n_pad centred dosages, FP64, analysis order
   const double* vref;         // G x n FP64 residual vectors in sample space
   int32_t n;
-  double tau;
+  double tau, tau_hi;
+  // LMM tests with |t| > tau_hi, recomputed entirely in FP64 afterwards
+  PolishItem* polish;
+  int64_t polish_cap;
+  // scope: 0 trans (a gene's cis window is excluded), 1 cis (only the window
+  // is tested). Cis launches run over work items {group, tile_lo, tile_hi}
+  // (128-row tiles of the block) instead of (group, split) pairs; partial
+  // states are then written per item.
+  int32_t cis;
+  const int32_t* work;
 };
 
-// Near-null refinement. For p near 1 the ACAT term is about -1/(pi (1 - p)),
-// and 1 - p is proportional to |t| there, so the contraction's absolute error
-// in t turns into a large relative error of that term (and of the gene's ACAT
-// p). A test whose contracted |t| is below tau gets its numerator recomputed
-// in FP64: num = v_k^T g_c with g_c the centred FP64 dosage and v_k the gene's
-// residual vector mapped back to sample space (LMM: v_k = Q (s o r_k), so
-// v_k^T g_c = (s o r_k)^T Q^T g_c; LM: v_k = r_k). The whole wave computes
-// each such dot product together, so the refinement costs one 2n-double read
-// per refined test and no divergence.
-template <int MODEL, int NQ>
-__device__ inline bool near_null(const WaldArgs& a, const float* q, int64_t j, bool gene_ok, double gscal) {
-  if (a.tau <= 0 || !gene_ok || j >= a.m || !a.keep[j] || !(a.df > 0)) return false;
-  const double tau2 = a.tau * a.tau;
+constexpr int kRB = 2;          // 16-row blocks per wave
+// s_waitcnt immediate (gfx9 encoding): vmcnt <= 2 * kRB (bits 3:0, 15:14),
+// expcnt and lgkmcnt at their maximum (no wait).
+constexpr int kWaitVm4 = (2 * kRB) | (0x7 << 4) | (0xF << 8);
+static_assert(2 * kRB < 16, "vmcnt field");
+constexpr int kTileRows = 128;  // SNP rows per workgroup tile
+
+// Per-tile SNP data staged in LDS.
+struct RowInfo {
+  double af[kTileRows];
+  double gtg[kTileRows];
+  int64_t pos[kTileRows];
+  int32_t chrom[kTileRows];
+  uint8_t keep[kTileRows];
+};
+
+struct TailView {
+  const double* head;    // LDS copy of the header + piece A
+  const double* coef_b;  // global
+};
+
+// FP64 numerator refinement at both ends of the t range.
+//  * near null: for p near 1 the ACAT term is about -1/(pi (1 - p)), and
+//    1 - p is proportional to |t| there, so the contraction's absolute error in
+//    t turns into a large relative error of that term (and of the gene's ACAT
+//    p); tests with |t| < tau are refined.
+//  * extreme t: the relative p error grows like |t| * dt, so tests with
+//    |t| > tau_hi are refined too (and, for LMM, polished after the kernel).
+// Refinement recomputes the numerator in FP64: num = v_k^T g_c with g_c the
+// centred FP64 dosage and v_k the gene's residual vector mapped back to sample
+// space (LMM: v_k = Q (s o r_k), so v_k^T g_c = (s o r_k)^T Q^T g_c; LM:
+// v_k = r_k, and with the FP64 g^T g the LM test is then entirely FP64).
+template <int MODEL>
+__device__ inline bool refine_wanted(const WaldArgs& a, bool ok, double gg, double S, double num, double gscal,
+                                     double gtg) {
+  if (!ok || !(a.df > 0)) return false;
+  const double tau2 = a.tau * a.tau, hi2 = a.tau_hi * a.tau_hi;
   if constexpr (MODEL == EQTL_MODEL_LMM) {
-    const double gg = q[0];
-    double ss = 0, num = 0;
-#pragma unroll
-    for (int b = 1; b < NQ; ++b) {
-      if (b < a.nq - 1) ss += static_cast<double>(q[b]) * q[b];
-      if (b == a.nq - 1) num = q[b];
-    }
-    const double S = gg - ss;
     if (gg < 1e-12 || S <= 1e-15) return false;
     const double beta = num / S;
     const double qq = fmax(gscal - S * beta * beta, 0.0);
-    return beta * beta * S * a.df < tau2 * qq;  // t^2 = beta^2 S df / qq
+    const double t2q = beta * beta * S * a.df;  // t^2 = t2q / qq
+    return t2q < tau2 * qq || t2q > hi2 * qq;
   } else {
-    const double gtg = a.gtg[j];
     if (gtg < 1e-12) return false;
-    const double gty = q[0];
-    const double beta = gty / gtg;
-    const double rss = fmax(gscal - beta * gty, 0.0);
-    return beta * beta * gtg * a.df < tau2 * rss;
+    const double beta = num / gtg;
+    const double rss = fmax(gscal - beta * num, 0.0);
+    const double t2r = beta * beta * gtg * a.df;  // t^2 = t2r / rss
+    return t2r < tau2 * rss || t2r > hi2 * rss;
   }
 }
 
-// Wave-cooperative FP64 dot products for the lanes that flagged a test; every
-// lane of the wave must call this (wave-uniform control flow).
+// The whole wave computes each flagged lane's FP64 dot product together: one
+// 2n-double read per refined test and no divergence. Every lane of the wave
+// must call this (wave-uniform control flow).
 __device__ inline double wave_refine(const WaldArgs& a, bool cand, int64_t j, int gene) {
   unsigned long long mask = __ballot(cand);
   const int lane = threadIdx.x & 63;
@@ -288,30 +329,17 @@ __device__ inline double wave_refine(const WaldArgs& a, bool cand, int64_t j, in
   return mine;
 }
 
-
-// One (SNP row j, gene) test from its NQ accumulated quantities: FP64 Wald
-// (test_lmm_gtil, scan_trans_lmm.cpp:11-40 / hit_from_gty, scan_trans_lm.cpp:21-36),
-// then the per-gene statistics (apply_snp_hit_stats, scan_trans.hpp:87-102).
-template <int MODEL, int NQ>
-__device__ inline void wald_test(const WaldArgs& a, const float* q, int64_t j, int gene, bool gene_ok, double gscal,
-                                 bool excl, int32_t gch, int64_t gcs, int64_t gce, bool refined, double num64,
-                                 GeneAcc& st, dd& asum) {
-  if (!gene_ok || j >= a.m || !a.keep[j]) return;
+// One (SNP row j, gene) test: FP64 Wald (test_lmm_gtil, scan_trans_lmm.cpp:11-40
+// / hit_from_gty, scan_trans_lm.cpp:21-36), then the per-gene statistics
+// (apply_snp_hit_stats, scan_trans.hpp:87-102). The caller has checked the
+// gene, the row bound, the SNP's keep flag and the window rule of the scope.
+template <int MODEL>
+__device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, int64_t j, int gene, double gscal,
+                                 double gg, double S, double num, const TailView& tv, GeneAcc& st, dd& asum) {
   const int64_t snp = a.snp0 + j;
-  if (excl && a.snp_chrom && a.snp_chrom[snp] == gch && a.snp_pos[snp] >= gcs && a.snp_pos[snp] <= gce) return;
-  const int nq = a.nq;
   double beta = 0, se = 0, stat = 0, p = 1.0, r2 = 0;
   const double df = a.df;
   if constexpr (MODEL == EQTL_MODEL_LMM) {
-    const double gg = q[0];
-    double ss = 0, num = 0;
-#pragma unroll
-    for (int b = 1; b < NQ; ++b) {
-      if (b < nq - 1) ss += static_cast<double>(q[b]) * q[b];
-      if (b == nq - 1) num = q[b];
-    }
-    if (refined) num = num64;
-    const double S = gg - ss;
     if (!(gg < 1e-12) && !(S <= 1e-15)) {
       beta = num / S;
       double qq = gscal - S * beta * beta;
@@ -319,19 +347,19 @@ __device__ inline void wald_test(const WaldArgs& a, const float* q, int64_t j, i
       const double sigma2 = df > 0 ? qq / df : 1.0;
       se = sqrt(fmax(sigma2 / S, 0.0));
       stat = se > 0 ? beta / se : 0.0;
-      p = p_from_t_tab(*a.tail, stat);
+      p = p_from_t_tab2(tv.head, tv.coef_b, stat);
       r2 = gscal > 1e-15 ? fmax(0.0, 1.0 - qq / gscal) : 0.0;
     }
   } else {
-    const double gtg = a.gtg[j];
-    const double gty = refined ? num64 : static_cast<double>(q[0]);
+    const double gtg = ri.gtg[jl];
+    const double gty = num;
     if (!(gtg < 1e-12) && df > 0) {
       beta = gty / gtg;
       const double rss = gscal - beta * gty;
       const double s2 = fmax(rss / df, 0.0);
       se = sqrt(s2 / gtg);
       stat = se > 0 ? beta / se : 0.0;
-      p = p_from_t_tab(*a.tail, stat);
+      p = p_from_t_tab2(tv.head, tv.coef_b, stat);
       r2 = gscal > 0 ? 1.0 - rss / gscal : 0.0;
     } else if (!(gtg < 1e-12)) {
       beta = gty / gtg;  // df <= 0: p = 1 with beta set (model_lm.cpp:44-48)
@@ -339,12 +367,16 @@ __device__ inline void wald_test(const WaldArgs& a, const float* q, int64_t j, i
   }
   if (!isfinite(p)) return;
   ++st.n_tested;
+  // LMM tests with |t| > tau_hi are polished in FP64 after the kernel, which
+  // adds their ACAT term (an accepted p stays accepted: both are < 1e-15)
+  const bool polish = MODEL == EQTL_MODEL_LMM && fabs(stat) > a.tau_hi;
   double term;
   if (acat_accept(p, &term)) {
-    asum = dd_add_d(asum, term);
+    if (!polish) asum = dd_add_d(asum, term);
     ++st.used;
   }
-  const double afj = a.af[j];
+  const double afj = ri.af[jl];
+  int64_t hidx = -1;
   if (p <= a.pthr) {
     ++st.n_sig;
     const unsigned long long idx = atomicAdd(a.hit_count, 1ull);
@@ -353,139 +385,74 @@ __device__ inline void wald_test(const WaldArgs& a, const float* q, int64_t j, i
       h.snp = snp; h.gene = gene; h.pad = 0;
       h.af = afj; h.beta = beta; h.se = se; h.stat = stat; h.p = p; h.r2 = r2;
       a.hits[idx] = h;
+      hidx = static_cast<int64_t>(idx);
     }
   }
   if (p < st.best_p || (p == st.best_p && snp < st.best_snp)) {
     st.best_p = p; st.best_snp = snp; st.best_af = afj; st.best_beta = beta;
     st.best_se = se; st.best_stat = stat; st.best_r2 = r2;
   }
-}
-
-template <int MODEL, int PREC, int NQ>
-__global__ __launch_bounds__(256) void k_wald(WaldArgs a) {
-  // XCD-aware mapping: workgroups are dealt round-robin to the 8 XCDs; keep
-  // consecutive workgroups of one XCD on the same gene group so that the
-  // group's column blocks stay resident in that XCD's L2.
-  const int wg = blockIdx.x;
-  const int xcd = wg & 7;
-  const int jx = wg >> 3;
-  const int group = (jx / a.n_splits) * 8 + xcd;
-  const int split = jx % a.n_splits;
-  if (group >= a.n_groups) return;
-
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int gl = lane & 15;
-  const int gene = group * 16 + gl;
-  const bool gene_ok = gene < a.G && a.gene_valid[gene];
-  const int kk = 8 * (lane >> 4);
-  const int nq = a.nq;
-  const double gscal = gene_ok ? a.gene_scal[gene] : 0.0;
-  const bool excl = gene_ok && a.gene_chrom && a.gene_chrom[gene] >= 0;
-  const int32_t gch = excl ? a.gene_chrom[gene] : -1;
-  const int64_t gcs = excl ? a.gene_cs[gene] : 0, gce = excl ? a.gene_ce[gene] : 0;
-
-  GeneAcc st;
-  acc_init(st);
-  dd asum{0.0, 0.0};
-
-
-  for (int tile = split; tile < a.n_tiles; tile += a.n_splits) {
-    const int64_t r0 = static_cast<int64_t>(tile) * 64 + wave * 16;
-    f32x4 acc[NQ];
-#pragma unroll
-    for (int b = 0; b < NQ; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
-    const float* arow = a.A + (r0 + gl) * a.n_pad + kk;
-    for (int k0 = 0; k0 < a.n_pad; k0 += 32) {
-      const f32x8 av = load8(arow + k0);
-      if constexpr (PREC == EQTL_PREC_FP32) {
-        f32x8 sv;
-        if constexpr (MODEL == EQTL_MODEL_LMM) {
-#pragma unroll
-          for (int t = 0; t < 8; ++t) sv.v[t] = av.v[t] * av.v[t];
-        }
-#pragma unroll
-        for (int b = 0; b < NQ; ++b) {
-          if (b < nq) {
-            const f32x8 bv = load8(a.gc.f32 + a.gc.tile_f32(group, k0 >> 5) + (b * 64 + lane) * 8);
-            if (MODEL == EQTL_MODEL_LMM && b == 0) acc[b] = mfma_f32x8(sv, bv, acc[b]);
-            else acc[b] = mfma_f32x8(av, bv, acc[b]);
-          }
-        }
-      } else {
-        bf16x8 ah, al, sh, sl;
-        split_bf16(av, ah, al);
-        if constexpr (MODEL == EQTL_MODEL_LMM) {
-          f32x8 sv;
-#pragma unroll
-          for (int t = 0; t < 8; ++t) sv.v[t] = av.v[t] * av.v[t];
-          split_bf16(sv, sh, sl);
-        }
-#pragma unroll
-        for (int b = 0; b < NQ; ++b) {
-          if (b < nq) {
-            const size_t bo = a.gc.tile_bf(group, k0 >> 5) + (b * 64 + lane) * 8;
-            const bf16x8 bh = load8_bf16(a.gc.bf + bo), bl = load8_bf16(a.gc.bf + bo + static_cast<size_t>(nq) * 512);
-            if (MODEL == EQTL_MODEL_LMM && b == 0) {
-              acc[b] = mfma_bf16(sh, bh, acc[b]);
-              acc[b] = mfma_bf16(sh, bl, acc[b]);
-              acc[b] = mfma_bf16(sl, bh, acc[b]);
-            } else {
-              acc[b] = mfma_bf16(ah, bh, acc[b]);
-              acc[b] = mfma_bf16(ah, bl, acc[b]);
-              acc[b] = mfma_bf16(al, bh, acc[b]);
-            }
-          }
-        }
-      }
-    }
-    // ---- FP64 epilogue: 4 (SNP, gene) tests per lane ----
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      float q[NQ];
-#pragma unroll
-      for (int b = 0; b < NQ; ++b) q[b] = acc[b][r];
-      const int64_t j = r0 + 4 * (lane >> 4) + r;
-      const bool cand = near_null<MODEL, NQ>(a, q, j, gene_ok, gscal);
-      const double n64 = wave_refine(a, cand, j, gene);
-      wald_test<MODEL, NQ>(a, q, j, gene, gene_ok, gscal, excl, gch, gcs, gce, cand, n64, st, asum);
-    }
-  }
-  st.acat_hi = asum.hi;
-  st.acat_lo = asum.lo;
-  // ---- reduce the 16 partial states of each gene (4 lane groups x 4 waves) ----
-  __shared__ GeneAcc red[16][16];
-  red[wave * 4 + (lane >> 4)][gl] = st;
-  __syncthreads();
-  if (threadIdx.x < 16) {
-    GeneAcc t = red[0][threadIdx.x];
-    for (int s = 1; s < 16; ++s) acc_merge(t, red[s][threadIdx.x]);
-    const int g = group * 16 + threadIdx.x;
-    a.partial[static_cast<size_t>(split) * a.G_pad + g] = t;
+  if (polish) {
+    const unsigned long long pi = atomicAdd(a.hit_count + 1, 1ull);
+    if (static_cast<int64_t>(pi) < a.polish_cap) a.polish[pi] = PolishItem{static_cast<int32_t>(j), gene, hidx};
   }
 }
-
-
-// ---------------------------------------------------------------------------
-// K7/K8 v2: the same contraction with the gene group's k-step tile staged in
-// LDS (double-buffered, filled by direct global→LDS 16-byte loads) and shared
-// by the 4 waves; each wave owns 2 row blocks (32 SNP rows), so every B
-// fragment read from LDS feeds two MFMA chains. Workgroup tile: 128 SNPs x 16
-// genes x NQ quantities. One workgroup per CU (accumulators use the AGPRs).
-// ---------------------------------------------------------------------------
-constexpr int kRB = 2;
 
 __device__ inline void lds_direct16(const void* gsrc, unsigned char* lds_wave_base) {
   __builtin_amdgcn_global_load_lds(gsrc, (__attribute__((address_space(3))) void*)(lds_wave_base), 16, 0, 0);
 }
 
+// LDS layout of the Wald kernel. Region 0 holds, during the contraction, the
+// two k-step buffers of the gene tile and a ring of SNP-row stages; then the
+// folded (gg, S, num) of the tile's tests during the epilogue; then the
+// per-gene partials of the final reduction. RowInfo and the t-tail head
+// follow it. The SNP rows (read from HBM once per gene group) are staged
+// three k-steps deep when that fits in 160 KiB, two otherwise; for the widest
+// gene tiles they go through registers one k-step ahead.
+constexpr int kFoldDoubles = kRB * 4 * 3 * 256;
+constexpr int kAStageBytes = kTileRows * 32 * 4;  // 128 rows x 32 samples fp32
+constexpr long kLdsCap = 160 * 1024;
+__host__ __device__ constexpr long wald_tail_bytes() {
+  return static_cast<long>(sizeof(RowInfo)) + static_cast<long>(sizeof(double)) * kTailHead;
+}
+__host__ __device__ constexpr int wald_a_stages(int nq) {
+  return (2L * nq * 2048 + 3L * kAStageBytes + wald_tail_bytes() <= kLdsCap)   ? 3
+         : (2L * nq * 2048 + 2L * kAStageBytes + wald_tail_bytes() <= kLdsCap) ? 2
+                                                                                : 0;
+}
+__host__ __device__ constexpr size_t wald_region0_bytes(int nq) {
+  const size_t b = 2 * static_cast<size_t>(nq) * 2048 + static_cast<size_t>(wald_a_stages(nq)) * kAStageBytes;
+  const size_t f = sizeof(double) * kFoldDoubles, r = 256 * sizeof(GeneAcc);
+  return b > f ? (b > r ? b : r) : (f > r ? f : r);
+}
+__host__ __device__ constexpr size_t wald_lds_bytes(int nq) {
+  return wald_region0_bytes(nq) + static_cast<size_t>(wald_tail_bytes());
+}
+static_assert(wald_lds_bytes(34) <= kLdsCap && wald_a_stages(23) == 3, "Wald LDS layout");
+
 template <int MODEL, int PREC, int NQ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_wald2(WaldArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_wald(WaldArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  const int wg = blockIdx.x;
-  const int xcd = wg & 7;
-  const int jx = wg >> 3;
-  const int group = (jx / a.n_splits) * 8 + xcd;
-  const int split = jx % a.n_splits;
+  // XCD-aware mapping: workgroups are dealt round-robin to the 8 XCDs; the
+  // n_splits consecutive workgroups of one XCD share a gene group, so the
+  // group's column blocks are served from that XCD's L2.
+  int group, split, t_lo, t_hi, t_step;
+  if (a.work) {  // cis: one work item per workgroup
+    group = a.work[3 * blockIdx.x];
+    t_lo = a.work[3 * blockIdx.x + 1];
+    t_hi = a.work[3 * blockIdx.x + 2];
+    split = blockIdx.x;
+    t_step = 1;
+  } else {
+    const int wg = blockIdx.x;
+    const int xcd = wg & 7;
+    const int jx = wg >> 3;
+    group = (jx / a.n_splits) * 8 + xcd;
+    split = jx % a.n_splits;
+    t_lo = split;
+    t_hi = (a.n_tiles + 1) >> 1;  // 128-row tiles
+    t_step = a.n_splits;
+  }
   if (group >= a.n_groups) return;
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -496,51 +463,111 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const int nq = a.nq;
   const int nks = a.n_pad >> 5;
   const double gscal = gene_ok ? a.gene_scal[gene] : 0.0;
+  // the gene's window (trans: excluded; cis: the only SNPs tested)
   const bool excl = gene_ok && a.gene_chrom && a.gene_chrom[gene] >= 0;
   const int32_t gch = excl ? a.gene_chrom[gene] : -1;
   const int64_t gcs = excl ? a.gene_cs[gene] : 0, gce = excl ? a.gene_ce[gene] : 0;
+  const bool win_known = excl && a.snp_chrom;
 
-  const int tile_bytes = nq * 2048;  // one k-step tile of the group (either precision)
-  unsigned char* buf[2] = {smem, smem + tile_bytes};
+  constexpr int tile_bytes = NQ * 2048;  // one k-step tile of the group (either precision)
+  unsigned char* const buf0 = smem;
+  unsigned char* const buf1 = smem + tile_bytes;
+  const size_t r0b = wald_region0_bytes(NQ);
+  RowInfo& ri = *reinterpret_cast<RowInfo*>(smem + r0b);
+  double* thead = reinterpret_cast<double*>(smem + r0b + sizeof(RowInfo));
+  double* fold = reinterpret_cast<double*>(smem);
+  for (int i = threadIdx.x; i < kTailHead; i += 256) thead[i] = (&a.tail->df)[i];
+  const TailView tv{thead, a.tail->coef_b};
   const unsigned char* gsrc =
       PREC == EQTL_PREC_FP32 ? reinterpret_cast<const unsigned char*>(a.gc.f32 + a.gc.tile_f32(group, 0))
                              : reinterpret_cast<const unsigned char*>(a.gc.bf + a.gc.tile_bf(group, 0));
-  const int chunks = tile_bytes >> 10;  // 1 KiB per wave instruction (64 lanes x 16 B)
+  constexpr int chunks = tile_bytes >> 10;  // 1 KiB per wave instruction (64 lanes x 16 B)
 
   GeneAcc st;
   acc_init(st);
   dd asum{0.0, 0.0};
-  const int n_tiles2 = (a.n_tiles + 1) >> 1;  // 128-row tiles
-
-  for (int tile = split; tile < n_tiles2; tile += a.n_splits) {
-    const int64_t rowA = static_cast<int64_t>(tile) * 128 + wave * 32;
+  for (int tile = t_lo; tile < t_hi; tile += t_step) {
+    const int64_t row0 = static_cast<int64_t>(tile) * kTileRows;
+    const int64_t rowA = row0 + wave * 32;
+    __syncthreads();  // the previous tile's epilogue is done with ri
+    if (threadIdx.x < kTileRows) {
+      const int t = threadIdx.x;
+      const int64_t j = row0 + t;
+      const bool in = j < a.m;
+      ri.keep[t] = in ? a.keep[j] : 0;
+      ri.af[t] = in ? a.af[j] : 0.0;
+      ri.gtg[t] = (MODEL == EQTL_MODEL_LM && in) ? a.gtg[j] : 0.0;
+      const bool ann = in && a.snp_chrom;
+      ri.chrom[t] = ann ? a.snp_chrom[a.snp0 + j] : -1;
+      ri.pos[t] = ann ? a.snp_pos[a.snp0 + j] : 0;
+    }
     f32x4 acc[kRB][NQ];
 #pragma unroll
     for (int rb = 0; rb < kRB; ++rb)
 #pragma unroll
       for (int b = 0; b < NQ; ++b) acc[rb][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // SNP-row operand: lane l of row block rb holds row rowA + 16 rb + (l & 15),
+    // samples 32 s + 8 (l >> 4) + [0..7] of k-step s
     const float* arow[kRB];
 #pragma unroll
     for (int rb = 0; rb < kRB; ++rb) arow[rb] = a.A + (rowA + rb * 16 + gl) * a.n_pad + kk;
-    // prologue: stage k-step 0
-    for (int c = wave; c < chunks; c += 4) lds_direct16(gsrc + c * 1024 + lane * 16, buf[0] + c * 1024);
-    f32x8 an[kRB];
+    constexpr int kAS = wald_a_stages(NQ);
+    unsigned char* const abuf = smem + 2 * tile_bytes;
+    // stage k-step s of this wave's rows: chunk (rb, h) = 64 lanes x 16 B, so
+    // every later ds_read_b128 of it is conflict-free
+    auto stage_rows = [&](int s) {
 #pragma unroll
-    for (int rb = 0; rb < kRB; ++rb) an[rb] = load8(arow[rb]);
+      for (int rb = 0; rb < kRB; ++rb)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          lds_direct16(arow[rb] + s * 32 + 4 * h,
+                       abuf + (s % (kAS > 0 ? kAS : 1)) * kAStageBytes + ((wave * kRB + rb) * 2 + h) * 1024);
+    };
+    f32x8 an[kRB];
+    // prologue: gene tile of k-step 0, rows of the first stages
+    for (int c = wave; c < chunks; c += 4) lds_direct16(gsrc + c * 1024 + lane * 16, buf0 + c * 1024);
+    if constexpr (kAS > 0) {
+      for (int s = 0; s < kAS - 1 && s < nks; ++s) stage_rows(s);
+    } else {
+#pragma unroll
+      for (int rb = 0; rb < kRB; ++rb) an[rb] = load8(arow[rb]);
+    }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int s = 0; s < nks; ++s) {
-      const unsigned char* cur = buf[s & 1];
+      const unsigned char* cur = (s & 1) ? buf1 : buf0;
       f32x8 ac[kRB];
+      if constexpr (kAS > 0) {
+        const unsigned char* as = abuf + (s % kAS) * kAStageBytes;
 #pragma unroll
-      for (int rb = 0; rb < kRB; ++rb) ac[rb] = an[rb];
+        for (int rb = 0; rb < kRB; ++rb) {
+          const float4 lo = *reinterpret_cast<const float4*>(as + ((wave * kRB + rb) * 2 + 0) * 1024 + lane * 16);
+          const float4 hi = *reinterpret_cast<const float4*>(as + ((wave * kRB + rb) * 2 + 1) * 1024 + lane * 16);
+          ac[rb].v[0] = lo.x; ac[rb].v[1] = lo.y; ac[rb].v[2] = lo.z; ac[rb].v[3] = lo.w;
+          ac[rb].v[4] = hi.x; ac[rb].v[5] = hi.y; ac[rb].v[6] = hi.z; ac[rb].v[7] = hi.w;
+        }
+      } else {
+#pragma unroll
+        for (int rb = 0; rb < kRB; ++rb) ac[rb] = an[rb];
+      }
       if (s + 1 < nks) {
         const unsigned char* src = gsrc + static_cast<size_t>(s + 1) * tile_bytes;
-        unsigned char* dst = buf[(s + 1) & 1];
+        unsigned char* dst = ((s + 1) & 1) ? buf1 : buf0;
         for (int c = wave; c < chunks; c += 4) lds_direct16(src + c * 1024 + lane * 16, dst + c * 1024);
-#pragma unroll
-        for (int rb = 0; rb < kRB; ++rb) an[rb] = load8(arow[rb] + (s + 1) * 32);
       }
+      // the row stage goes out after the tile loads, so the wait below can
+      // leave it (the youngest 2 kRB operations) in flight
+      __builtin_amdgcn_sched_barrier(0);
+      const bool ahead = kAS == 3 && s + 2 < nks;
+      if constexpr (kAS > 0) {
+        if (s + kAS - 1 < nks) stage_rows(s + kAS - 1);
+      } else {
+        if (s + 1 < nks) {
+#pragma unroll
+          for (int rb = 0; rb < kRB; ++rb) an[rb] = load8(arow[rb] + (s + 1) * 32);
+        }
+      }
+      __builtin_amdgcn_sched_barrier(0);
       if constexpr (PREC == EQTL_PREC_FP32) {
         f32x8 sq[kRB];
         if constexpr (MODEL == EQTL_MODEL_LMM) {
@@ -549,16 +576,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
 #pragma unroll
             for (int t = 0; t < 8; ++t) sq[rb].v[t] = ac[rb].v[t] * ac[rb].v[t];
         }
+        // gene fragments are read one quantity ahead of the MFMAs using them
+        f32x8 bn = load8(reinterpret_cast<const float*>(cur + lane * 32));
 #pragma unroll
         for (int b = 0; b < NQ; ++b) {
-          if (b < nq) {
-            const float* bp = reinterpret_cast<const float*>(cur + (b * 64 + lane) * 32);
-            const f32x8 bv = load8(bp);
+          const f32x8 bv = bn;
+          if (b + 1 < NQ) bn = load8(reinterpret_cast<const float*>(cur + ((b + 1) * 64 + lane) * 32));
 #pragma unroll
-            for (int rb = 0; rb < kRB; ++rb) {
-              if (MODEL == EQTL_MODEL_LMM && b == 0) acc[rb][b] = mfma_f32x8(sq[rb], bv, acc[rb][b]);
-              else acc[rb][b] = mfma_f32x8(ac[rb], bv, acc[rb][b]);
-            }
+          for (int rb = 0; rb < kRB; ++rb) {
+            if (MODEL == EQTL_MODEL_LMM && b == 0) acc[rb][b] = mfma_f32x8(sq[rb], bv, acc[rb][b]);
+            else acc[rb][b] = mfma_f32x8(ac[rb], bv, acc[rb][b]);
           }
         }
       } else {
@@ -573,41 +600,76 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
             split_bf16(sv, sh[rb], sl[rb]);
           }
         }
+        bf16x8 bhn = *reinterpret_cast<const bf16x8*>(cur + lane * 16);
+        bf16x8 bln = *reinterpret_cast<const bf16x8*>(cur + NQ * 1024 + lane * 16);
 #pragma unroll
         for (int b = 0; b < NQ; ++b) {
-          if (b < nq) {
-            const bf16x8 bh = *reinterpret_cast<const bf16x8*>(cur + (b * 64 + lane) * 16);
-            const bf16x8 bl = *reinterpret_cast<const bf16x8*>(cur + nq * 1024 + (b * 64 + lane) * 16);
+          const bf16x8 bh = bhn, bl = bln;
+          if (b + 1 < NQ) {
+            bhn = *reinterpret_cast<const bf16x8*>(cur + ((b + 1) * 64 + lane) * 16);
+            bln = *reinterpret_cast<const bf16x8*>(cur + NQ * 1024 + ((b + 1) * 64 + lane) * 16);
+          }
 #pragma unroll
-            for (int rb = 0; rb < kRB; ++rb) {
-              if (MODEL == EQTL_MODEL_LMM && b == 0) {
-                acc[rb][b] = mfma_bf16(sh[rb], bh, acc[rb][b]);
-                acc[rb][b] = mfma_bf16(sh[rb], bl, acc[rb][b]);
-                acc[rb][b] = mfma_bf16(sl[rb], bh, acc[rb][b]);
-              } else {
-                acc[rb][b] = mfma_bf16(ah[rb], bh, acc[rb][b]);
-                acc[rb][b] = mfma_bf16(ah[rb], bl, acc[rb][b]);
-                acc[rb][b] = mfma_bf16(al[rb], bh, acc[rb][b]);
-              }
+          for (int rb = 0; rb < kRB; ++rb) {
+            if (MODEL == EQTL_MODEL_LMM && b == 0) {
+              acc[rb][b] = mfma_bf16(sh[rb], bh, acc[rb][b]);
+              acc[rb][b] = mfma_bf16(sh[rb], bl, acc[rb][b]);
+              acc[rb][b] = mfma_bf16(sl[rb], bh, acc[rb][b]);
+            } else {
+              acc[rb][b] = mfma_bf16(ah[rb], bh, acc[rb][b]);
+              acc[rb][b] = mfma_bf16(ah[rb], bl, acc[rb][b]);
+              acc[rb][b] = mfma_bf16(al[rb], bh, acc[rb][b]);
             }
           }
         }
       }
-      __builtin_amdgcn_s_waitcnt(0);
+      // the next k-step's gene tile and rows must have landed; with three row
+      // stages the rows of k-step s+2 (issued last) stay in flight
+      if (ahead) __builtin_amdgcn_s_waitcnt(kWaitVm4);
+      else __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
     }
+    // fold the accumulators to (gg, S, num) per test in FP64, parked in LDS
+    // (the last k-step's barrier has released the staging buffers)
 #pragma unroll
     for (int rb = 0; rb < kRB; ++rb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        float q[NQ];
+        const int t = rb * 4 + r;
+        double gg = 0, S = 0, num;
+        if constexpr (MODEL == EQTL_MODEL_LMM) {
+          gg = acc[rb][0][r];
+          double ss = 0;
+          num = 0;
 #pragma unroll
-        for (int b = 0; b < NQ; ++b) q[b] = acc[rb][b][r];
-        const int64_t j = rowA + rb * 16 + 4 * (lane >> 4) + r;
-        const bool cand = near_null<MODEL, NQ>(a, q, j, gene_ok, gscal);
-        const double n64 = wave_refine(a, cand, j, gene);
-        wald_test<MODEL, NQ>(a, q, j, gene, gene_ok, gscal, excl, gch, gcs, gce, cand, n64, st, asum);
+          for (int b = 1; b < NQ; ++b) {
+            if (b < nq - 1) ss += static_cast<double>(acc[rb][b][r]) * acc[rb][b][r];
+            if (b == nq - 1) num = acc[rb][b][r];
+          }
+          S = gg - ss;
+        } else {
+          num = acc[rb][0][r];
+        }
+        fold[(t * 3 + 0) * 256 + threadIdx.x] = gg;
+        fold[(t * 3 + 1) * 256 + threadIdx.x] = S;
+        fold[(t * 3 + 2) * 256 + threadIdx.x] = num;
       }
+    // FP64 epilogue, one test per iteration (one copy of the FP64 code)
+#pragma unroll 1
+    for (int t = 0; t < kRB * 4; ++t) {
+      const int rb = t >> 2, r = t & 3;
+      const int jl = wave * 32 + rb * 16 + 4 * (lane >> 4) + r;
+      const int64_t j = row0 + jl;
+      const bool inwin = win_known && ri.chrom[jl] == gch && ri.pos[jl] >= gcs && ri.pos[jl] <= gce;
+      const bool ok = gene_ok && j < a.m && ri.keep[jl] && (a.cis ? inwin : !inwin);
+      const double gg = fold[(t * 3 + 0) * 256 + threadIdx.x];
+      const double S = fold[(t * 3 + 1) * 256 + threadIdx.x];
+      double num = fold[(t * 3 + 2) * 256 + threadIdx.x];
+      const bool cand = refine_wanted<MODEL>(a, ok, gg, S, num, gscal, ri.gtg[jl]);
+      const double n64 = wave_refine(a, cand, j, gene);
+      if (cand) num = n64;
+      if (ok) wald_test<MODEL>(a, ri, jl, j, gene, gscal, gg, S, num, tv, st, asum);
+    }
   }
   st.acat_hi = asum.hi;
   st.acat_lo = asum.lo;
@@ -619,17 +681,135 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   if (threadIdx.x < 16) {
     GeneAcc t = red[threadIdx.x];
     for (int q = 1; q < 16; ++q) acc_merge(t, red[q * 16 + threadIdx.x]);
-    a.partial[static_cast<size_t>(split) * a.G_pad + group * 16 + threadIdx.x] = t;
+    const size_t slot = a.work ? static_cast<size_t>(split) * 16 : static_cast<size_t>(split) * a.G_pad + group * 16;
+    a.partial[slot + threadIdx.x] = t;
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 polish of LMM tests with |t| > tau_hi (strong signals, p far below any
+// reporting threshold, where the contraction's relative error in S is
+// amplified by |t| and by 1/(1 - r^2)): test_lmm_gtil recomputed from the
+// FP64 centred dosage, one workgroup per test:
+//   g~ = Q^T g_c,  gg = sum d g~^2,  num = (s o r)^T g~,  a = X~^T (d o g~),
+//   W^T h = Lc^-1 a (Lc Lc^T = X~^T D X~),  S = gg - |Lc^-1 a|^2,
+// then beta/se/stat/p (continued-fraction tail) and r^2 as in the kernel. The
+// hit record (if any) is overwritten; k_polish_best then updates the gene's
+// best entry if it is this SNP.
+// ---------------------------------------------------------------------------
+struct PolishArgs {
+  const PolishItem* items;
+  int64_t count;
+  const double* A64;
+  int32_t n_pad, n, p;
+  const double* Q;      // n x n col-major
+  const double* lam;
+  const double* delta;
+  const double* U;      // G x n: s o r in the rotated space
+  const double* Xt;     // n x p col-major (rotated covariates)
+  const double* Lc;     // G x p x p, lower, col-major
+  const double* gene_scal;
+  const double* af;
+  double df, lbeta;
+  int64_t snp0;
+  double* gt;           // count x n scratch
+  eqtl_hit* hits;
+  eqtl_hit* out;        // count
+  GeneAcc* state;       // running per-gene state (ACAT term of the polished p)
+};
+
+__global__ __launch_bounds__(256) void k_polish_lmm(PolishArgs a) {
+  __shared__ double red[4];
+  __shared__ double va[34];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const PolishItem it = a.items[blockIdx.x];
+  const int n = a.n, p = a.p, k = it.gene;
+  const double* gc = a.A64 + static_cast<size_t>(it.j) * a.n_pad;
+  double* gt = a.gt + static_cast<size_t>(blockIdx.x) * n;
+  for (int c = wave; c < n; c += 4) {
+    const double* qc = a.Q + static_cast<size_t>(c) * n;
+    double s = 0;
+    for (int i = lane; i < n; i += 64) s = fma(qc[i], gc[i], s);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) gt[c] = s;
+  }
+  __syncthreads();
+  const double dk = a.delta[k];
+  const double* u = a.U + static_cast<size_t>(k) * n;
+  double part[34];
+#pragma unroll
+  for (int c = 0; c < 34; ++c) part[c] = 0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    double v = dk * a.lam[i] + 1.0;
+    if (v < 1e-12) v = 1e-12;
+    const double d = 1.0 / v, g = gt[i], dg = d * g;
+    part[0] += dg * g;
+    part[1] += u[i] * g;
+#pragma unroll
+    for (int c = 0; c < 32; ++c)
+      if (c < p) part[2 + c] += a.Xt[static_cast<size_t>(c) * n + i] * dg;
+  }
+#pragma unroll
+  for (int c = 0; c < 34; ++c) {
+    if (c < p + 2) {
+      double s = part[c];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      __syncthreads();
+      if (lane == 0) red[wave] = s;
+      __syncthreads();
+      if (threadIdx.x == 0) va[c] = red[0] + red[1] + red[2] + red[3];
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double* L = a.Lc + static_cast<size_t>(k) * p * p;
+  double w[32];
+  double ss = 0;
+  for (int i = 0; i < p; ++i) {
+    double t = va[2 + i];
+    for (int c = 0; c < i; ++c) t -= L[c * p + i] * w[c];
+    w[i] = t / L[i * p + i];
+    ss += w[i] * w[i];
+  }
+  const double gg = va[0], num = va[1], S = gg - ss, gscal = a.gene_scal[k], df = a.df;
+  double beta = 0, se = 0, stat = 0, pv = 1.0, r2 = 0;
+  if (!(gg < 1e-12) && !(S <= 1e-15)) {
+    beta = num / S;
+    double qq = gscal - S * beta * beta;
+    if (qq < 0) qq = 0;
+    const double sigma2 = df > 0 ? qq / df : 1.0;
+    se = sqrt(fmax(sigma2 / S, 0.0));
+    stat = se > 0 ? beta / se : 0.0;
+    pv = p_from_t_dev(stat, df, a.lbeta);
+    r2 = gscal > 1e-15 ? fmax(0.0, 1.0 - qq / gscal) : 0.0;
+  }
+  eqtl_hit h;
+  h.snp = a.snp0 + it.j; h.gene = k; h.pad = 0;
+  h.af = a.af[it.j]; h.beta = beta; h.se = se; h.stat = stat; h.p = pv; h.r2 = r2;
+  a.out[blockIdx.x] = h;
+  if (it.hit >= 0) a.hits[it.hit] = h;
+  double term;
+  if (acat_accept(pv, &term)) atomicAdd(&a.state[k].acat_pol, term);
+}
+
+__global__ void k_polish_best(GeneAcc* state, const eqtl_hit* out, int64_t count) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const eqtl_hit h = out[i];
+  GeneAcc& s = state[h.gene];
+  if (s.best_snp == h.snp) {
+    s.best_p = h.p; s.best_beta = h.beta; s.best_se = h.se; s.best_stat = h.stat; s.best_r2 = h.r2;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Self-test of the hardware conventions the kernels rely on (one workgroup):
 //  bit 0: bf16 16x16x32 MFMA operand/result lane layout
 //  bit 1: fp32 16x16x4 MFMA layout with the 8-k-slot grouping of mfma_f32x8
 //  bit 2: direct global→LDS 16-byte loads land at wave base + 16 * lane
-//  bit 3: the fp64 16x16x4 layout with the grouping of mfma_f64x8
+//  bit 3: fp64 16x16x4 with the grouping of mfma_f64x8 (result rows (l>>4)+4r)
 // Small integers make every product and sum exact, so equality is exact.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_selftest(int* fail) {
@@ -661,7 +841,9 @@ __global__ __launch_bounds__(256) void k_selftest(int* fail) {
       for (int k = 0; k < 32; ++k) e += Av(row, k) * Bv(k, col);
       bad1 |= c1[r] != e;
       bad2 |= c2[r] != e;
-      bad3 |= c3[r] != static_cast<double>(e);
+      double e3 = 0;
+      for (int k = 0; k < 32; ++k) e3 += Av((lane >> 4) + 4 * r, k) * Bv(k, col);
+      bad3 |= c3[r] != e3;
     }
     if (bad1) atomicOr(fail, 1);
     if (bad2) atomicOr(fail, 2);
@@ -688,6 +870,17 @@ __global__ void k_merge(GeneAcc* state, const GeneAcc* partial, int G, int G_pad
   state[g] = t;
 }
 
+// Cis: fold each gene's work-item partials (items sorted by group; item_first
+// has n_groups + 1 entries).
+__global__ void k_merge_items(GeneAcc* state, const GeneAcc* partial, const int32_t* item_first, int G) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const int grp = g >> 4;
+  GeneAcc t = state[g];
+  for (int it = item_first[grp]; it < item_first[grp + 1]; ++it) acc_merge(t, partial[static_cast<size_t>(it) * 16 + (g & 15)]);
+  state[g] = t;
+}
+
 __global__ void k_state_init(GeneAcc* state, int G) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g < G) acc_init(state[g]);
@@ -698,7 +891,7 @@ __global__ void k_finalize(const GeneAcc* state, eqtl_gene_state* out, int G) {
   if (g >= G) return;
   const GeneAcc a = state[g];
   eqtl_gene_state o;
-  o.acat_p = acat_p_final(dd{a.acat_hi, a.acat_lo}, a.used);
+  o.acat_p = acat_p_final(dd_add_d(dd{a.acat_hi, a.acat_lo}, a.acat_pol), a.used);
   o.best_p = a.best_p;
   o.best_snp = a.best_snp;
   o.best_af = a.best_af; o.best_beta = a.best_beta; o.best_se = a.best_se;

@@ -10,8 +10,9 @@ Tolerances (stated per quantity):
   ACAT vs oracle                        relative 1e-2 (terms -1/(pi(1-p)) for p
       near 1 amplify the contraction's absolute t error)
   REML delta (reference search)         f(delta_gpu) - f(delta_oracle) <= 1e-9|f|
-      and delta relative 1e-4 (the search is only defined to ~1e-8 by its own
-      floating-point noise; flat likelihoods widen that)
+      and, where f'' in ln(delta) exceeds 1e-3, delta relative 1e-4 (the search
+      is only defined to its own floating-point noise; on flat likelihoods that
+      noise decides the golden-section path)
 """
 
 import numpy as np
@@ -132,10 +133,15 @@ def test_reml_delta_reference_search(gpu_ctx):
     gpu_ctx.set_basis(Q, lam)
     delta, rss = gpu_ctx.lmm_prep(ds.Y, ds.X(), delta_mode=eqtl_amd.DELTA_REF)
     r = rel_err(delta, ref["state"]["delta"])
-    print("REML ref-search delta max rel err", r.max())
-    assert r.max() < 1e-4
     Yt = O.rotate(Q, ds.Y)
     Xt = O.rotate(Q, ds.X())
+    # curvature of f in ln(delta) at the oracle's delta; below 1e-3 the likelihood
+    # is flat to ~1e-9 |f| across the bracket and the search's path is set by
+    # rounding noise, so only the f criterion applies there
+    _, _, h = gpu_ctx.reml_grad_hess(np.log(ref["state"]["delta"]))
+    curved = h > 1e-3
+    print("REML ref-search delta max rel err (curved genes)", r[curved].max(), "flat genes", int((~curved).sum()))
+    assert r[curved].max() < 1e-4
     for k in range(ds.Y.shape[1]):
         fg = O.reml_negll(delta[k], Yt[:, k], Xt, lam)
         fo = O.reml_negll(ref["state"]["delta"][k], Yt[:, k], Xt, lam)

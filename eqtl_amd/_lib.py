@@ -18,6 +18,7 @@ EQTL_E_CAPACITY = -6
 MODEL_LM, MODEL_LMM = 0, 1
 PREC_FP32, PREC_BF16X3 = 0, 1
 DELTA_REF, DELTA_NEWTON, DELTA_GIVEN = 0, 1, 2
+SCOPE_TRANS, SCOPE_CIS = 0, 1
 
 HIT_DTYPE = np.dtype([("snp", "i8"), ("gene", "i4"), ("pad", "i4"), ("af", "f8"), ("beta", "f8"),
                       ("se", "f8"), ("stat", "f8"), ("p", "f8"), ("r2", "f8")])
@@ -33,7 +34,7 @@ EXPORTS = [
     "eqtl_gpu_set_basis", "eqtl_gpu_lmm_prep", "eqtl_gpu_reml_grad_hess", "eqtl_gpu_lm_prep",
     "eqtl_gpu_set_gene_windows", "eqtl_gpu_scan_begin", "eqtl_gpu_scan_block", "eqtl_gpu_scan_end",
     "eqtl_gpu_set_timing", "eqtl_gpu_get_timing", "eqtl_gpu_synth_bed", "eqtl_gpu_copy_rows",
-    "eqtl_gpu_synchronize", "eqtl_gpu_selftest",
+    "eqtl_gpu_synchronize", "eqtl_gpu_selftest", "eqtl_gpu_upload_bed", "eqtl_gpu_cis_perm",
 ]
 
 _lib = None
@@ -206,9 +207,9 @@ class Context:
         _check(lib().eqtl_gpu_set_gene_windows(self._c, _p(ch), _p(cs), _p(ce), ctypes.c_int32(ch.size)))
 
     # ---- scans ----
-    def scan_begin(self, model, precision, pthr):
+    def scan_begin(self, model, precision, pthr, scope=SCOPE_TRANS):
         _check(lib().eqtl_gpu_scan_begin(self._c, ctypes.c_int32(model), ctypes.c_int32(precision),
-                                         ctypes.c_double(pthr)))
+                                         ctypes.c_int32(scope), ctypes.c_double(pthr)))
 
     def scan_block(self, rows, snp0, hit_cap=1 << 16, rows_dev=None, m=None, bps=None):
         """rows: host (m, bps) uint8 array, or rows_dev (device pointer int) with m, bps."""
@@ -231,15 +232,37 @@ class Context:
         _check(lib().eqtl_gpu_scan_end(self._c, _p(st)))
         return st
 
-    def scan(self, model, precision, rows, pthr=1e-5, block=8192, hit_cap=1 << 18):
-        """Full trans scan over host bed rows (m, bps). Returns (hits, gene_state)."""
-        self.scan_begin(model, precision, pthr)
+    def scan(self, model, precision, rows, pthr=1e-5, block=8192, hit_cap=1 << 18, scope=SCOPE_TRANS):
+        """Full scan over host bed rows (m, bps). Returns (hits, gene_state)."""
+        self.scan_begin(model, precision, pthr, scope)
         out = []
         for j0 in range(0, rows.shape[0], block):
             out.append(self.scan_block(rows[j0:j0 + block], j0, hit_cap=hit_cap))
         st = self.scan_end()
         hits = np.concatenate(out) if out else np.empty(0, dtype=HIT_DTYPE)
         return hits, st
+
+    # ---- cis permutations ----
+    def upload_bed(self, rows=None, rows_dev=None, m=None, bps=None):
+        """Resident genotypes: host (m, bps) uint8 rows, or a device pointer with m, bps."""
+        if rows_dev is not None:
+            _check(lib().eqtl_gpu_upload_bed(self._c, ctypes.c_void_p(rows_dev), ctypes.c_int32(1),
+                                             ctypes.c_int64(m), ctypes.c_int32(bps)))
+        else:
+            rows = np.ascontiguousarray(rows, dtype=np.uint8)
+            _check(lib().eqtl_gpu_upload_bed(self._c, _p(rows), ctypes.c_int32(0), ctypes.c_int64(rows.shape[0]),
+                                             ctypes.c_int32(rows.shape[1])))
+
+    def cis_perm(self, precision, B, seed, gene_hash, best_p_obs, delta_mode=DELTA_REF):
+        """Gene-level permutations after an observed cis scan. Returns (perm_min_p G x B, p_emp G)."""
+        G = self.G
+        gh = np.ascontiguousarray(gene_hash, dtype=np.uint32)
+        bp = np.ascontiguousarray(best_p_obs, dtype=np.float64)
+        pm = np.empty((G, B), dtype=np.float64)
+        pe = np.empty(G, dtype=np.float64)
+        _check(lib().eqtl_gpu_cis_perm(self._c, ctypes.c_int32(precision), ctypes.c_int32(B), ctypes.c_int32(seed),
+                                       _p(gh), _p(bp), ctypes.c_int32(delta_mode), _p(pm), _p(pe)))
+        return pm, pe
 
     # ---- instrumentation ----
     def set_timing(self, on=True):

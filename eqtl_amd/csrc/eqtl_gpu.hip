@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <string>
 #include <vector>
@@ -19,6 +20,7 @@
 #include "scan_kernels.hpp"
 #include "grm_kernels.hpp"
 #include "reml_batch.hpp"
+#include "perm_rng.hpp"
 
 using namespace eqtl_amd;
 
@@ -37,6 +39,8 @@ constexpr double kRefineTauHi = 8.0;
 // how many are polished per launch (g~ scratch: chunk x n doubles).
 constexpr int64_t kPolishCap = int64_t(1) << 20;
 constexpr int64_t kPolishChunk = 4096;
+// cis work items span at most this many 128-SNP tiles
+constexpr int kCisItemTiles = 4;
 
 thread_local std::string g_err;
 
@@ -115,6 +119,8 @@ struct eqtl_gpu_ctx {
   // snp annotation
   int64_t m_annot = 0;
   DBuf snp_chrom, snp_pos;
+  std::vector<int32_t> h_snp_chrom;  // host copies (cis work items)
+  std::vector<int64_t> h_snp_pos;
   // basis
   bool have_basis = false;
   DBuf Q64, lam64, Q32, Qhi, Qlo;
@@ -128,9 +134,18 @@ struct eqtl_gpu_ctx {
   RemlWork rw;
   bool have_windows = false;
   DBuf gene_chrom, gene_cs, gene_ce;
+  std::vector<int32_t> h_gene_chrom;
+  std::vector<int64_t> h_gene_cs, h_gene_ce;
+  // cis scope: per-gene global SNP range [jlo, jhi) holding its window (empty: jlo == jhi)
+  std::vector<int64_t> win_lo, win_hi;
+  DBuf work, item_first;
+  // resident genotypes for cis scans
+  DBuf bed_res;
+  int64_t bed_m = 0;
+  int32_t bed_bps = 0;
   // scan
   bool scanning = false;
-  int scan_model = 0, scan_prec = 0;
+  int scan_model = 0, scan_prec = 0, scan_scope = EQTL_SCOPE_TRANS;
   double pthr = 1e-5;
   DBuf state, partial, hits, hit_count, tail;
   double tail_df = -1;
@@ -386,6 +401,8 @@ int eqtl_gpu_set_snp_annot(eqtl_gpu_ctx* c, const int32_t* chrom, const int64_t*
   HIPC(hipMemcpyAsync(c->snp_pos.p, pos, sizeof(int64_t) * m, hipMemcpyHostToDevice, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   c->m_annot = m;
+  c->h_snp_chrom.assign(chrom, chrom + m);
+  c->h_snp_pos.assign(pos, pos + m);
   return EQTL_OK;
 }
 
@@ -773,6 +790,46 @@ static int alloc_gene_side(eqtl_gpu_ctx* c, int G, int nq, int nq_pad) {
   return EQTL_OK;
 }
 
+// Per-gene REML delta, whitened bases, null residuals and column blocks for
+// the c->G genes whose rotated phenotypes are in c->Yt64 (X~ in c->Xt64, gene
+// side allocated). delta_in / delta_out / rss_out are host arrays (nullable).
+static int lmm_prep_core(eqtl_gpu_ctx* c, int G, int mode, const double* delta_in, double* delta_out,
+                         double* rss_out) {
+  const int n = c->n, p = c->p;
+  const double one = 1.0, zero = 0.0;
+  if (mode == EQTL_DELTA_GIVEN) {
+    HIPC(hipMemcpyAsync(c->delta.p, delta_in, sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
+  } else {
+    Timed t(c, "reml");
+    std::vector<double> dl;
+    CHK(reml_search(c, mode, dl));
+    HIPC(hipMemcpyAsync(c->delta.p, dl.data(), sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    CHK(t.done());
+  }
+  {
+    Timed t(c, "basis");
+    hipLaunchKernelGGL(k_lmm_basis, dim3(G), dim3(256), 0, c->stream, c->Xt64.as<double>(), c->Yt64.as<double>(),
+                       c->lam64.as<double>(), n, c->n_pad, p, G, c->delta.as<double>(), gene_cols(c),
+                       c->gscal.as<double>(), c->gvalid.as<int>(), c->U64.as<double>(), c->Lc.as<double>());
+    CHK(check_launch("k_lmm_basis"));
+    // sample-space residual vectors for the near-null FP64 refinement: V = Q U
+    if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, G, n, &one, c->Q64.as<double>(), n,
+                      c->U64.as<double>(), n, &zero, c->V64.as<double>(), n) != rocblas_status_success)
+      return fail(EQTL_E_HIP, "rocblas_dgemm failed");
+    CHK(t.done());
+  }
+  std::vector<int> valid(G);
+  HIPC(hipMemcpyAsync(valid.data(), c->gvalid.p, sizeof(int) * G, hipMemcpyDeviceToHost, c->stream));
+  if (delta_out) HIPC(hipMemcpyAsync(delta_out, c->delta.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  if (rss_out) HIPC(hipMemcpyAsync(rss_out, c->gscal.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  for (int k = 0; k < G; ++k)
+    if (!valid[k]) return fail(EQTL_E_ARG, "whitened covariate matrix is not positive definite (gene " + std::to_string(k) + ")");
+  c->prepped_model = EQTL_MODEL_LMM;
+  return EQTL_OK;
+}
+
 int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double* X, int32_t p,
                       int32_t mode, const double* delta_in, double* delta_out, double* rss_out) {
   if (!c || !Y || !X || G <= 0 || p <= 0) return fail(EQTL_E_ARG, "bad lmm_prep arguments");
@@ -806,37 +863,7 @@ int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double*
       return fail(EQTL_E_HIP, "rocblas_dgemm failed");
     CHK(t.done());
   }
-  if (mode == EQTL_DELTA_GIVEN) {
-    HIPC(hipMemcpyAsync(c->delta.p, delta_in, sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
-  } else {
-    Timed t(c, "reml");
-    std::vector<double> dl;
-    CHK(reml_search(c, mode, dl));
-    HIPC(hipMemcpyAsync(c->delta.p, dl.data(), sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
-    CHK(t.done());
-  }
-  {
-    Timed t(c, "basis");
-    hipLaunchKernelGGL(k_lmm_basis, dim3(G), dim3(256), 0, c->stream, c->Xt64.as<double>(), c->Yt64.as<double>(),
-                       c->lam64.as<double>(), n, c->n_pad, p, G, c->delta.as<double>(), gene_cols(c),
-                       c->gscal.as<double>(), c->gvalid.as<int>(), c->U64.as<double>(), c->Lc.as<double>());
-    CHK(check_launch("k_lmm_basis"));
-    // sample-space residual vectors for the near-null FP64 refinement: V = Q U
-    if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, G, n, &one, c->Q64.as<double>(), n,
-                      c->U64.as<double>(), n, &zero, c->V64.as<double>(), n) != rocblas_status_success)
-      return fail(EQTL_E_HIP, "rocblas_dgemm failed");
-    CHK(t.done());
-  }
-  std::vector<int> valid(G);
-  HIPC(hipMemcpyAsync(valid.data(), c->gvalid.p, sizeof(int) * G, hipMemcpyDeviceToHost, c->stream));
-  if (delta_out) HIPC(hipMemcpyAsync(delta_out, c->delta.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
-  if (rss_out) HIPC(hipMemcpyAsync(rss_out, c->gscal.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
-  HIPC(hipStreamSynchronize(c->stream));
-  for (int k = 0; k < G; ++k)
-    if (!valid[k]) return fail(EQTL_E_ARG, "whitened covariate matrix is not positive definite (gene " + std::to_string(k) + ")");
-  c->prepped_model = EQTL_MODEL_LMM;
-  return EQTL_OK;
+  return lmm_prep_core(c, G, mode, delta_in, delta_out, rss_out);
 }
 
 int eqtl_gpu_reml_grad_hess(eqtl_gpu_ctx* c, const double* log_delta, int32_t G, double* f, double* g,
@@ -902,20 +929,72 @@ int eqtl_gpu_set_gene_windows(eqtl_gpu_ctx* c, const int32_t* chrom, const int64
   HIPC(hipMemcpyAsync(c->gene_ce.p, ce, sizeof(int64_t) * G, hipMemcpyHostToDevice, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   c->have_windows = true;
+  c->h_gene_chrom.assign(chrom, chrom + G);
+  c->h_gene_cs.assign(cs, cs + G);
+  c->h_gene_ce.assign(ce, ce + G);
   return EQTL_OK;
 }
 
 // ---- scans ------------------------------------------------------------------
-int eqtl_gpu_scan_begin(eqtl_gpu_ctx* c, int32_t model, int32_t prec, double pthr) {
+// Per-gene global SNP index range holding the gene's window (in_cis_window,
+// scan_common.hpp:58-69): SNPs on the gene's chromosome with cstart <= pos <=
+// cend. The range is [min index, max index] of those SNPs, so a bim whose
+// chromosomes are not contiguous still gets a (wider) correct range.
+static int cis_windows(eqtl_gpu_ctx* c) {
+  const int64_t m = c->m_annot;
+  if (m <= 0 || static_cast<int64_t>(c->h_snp_pos.size()) != m)
+    return fail(EQTL_E_STATE, "cis scope needs the SNP annotation");
+  std::map<int32_t, std::vector<std::pair<int64_t, int64_t>>> by_chrom;  // chrom -> (pos, index)
+  for (int64_t j = 0; j < m; ++j) by_chrom[c->h_snp_chrom[j]].emplace_back(c->h_snp_pos[j], j);
+  for (auto& kv : by_chrom) std::sort(kv.second.begin(), kv.second.end());
+  const int G = c->G;
+  c->win_lo.assign(G, 0);
+  c->win_hi.assign(G, 0);
+  for (int k = 0; k < G; ++k) {
+    if (c->h_gene_chrom[k] < 0) continue;
+    auto it = by_chrom.find(c->h_gene_chrom[k]);
+    if (it == by_chrom.end()) continue;
+    const auto& v = it->second;
+    auto a = std::lower_bound(v.begin(), v.end(), std::make_pair(c->h_gene_cs[k], INT64_MIN));
+    auto b = std::upper_bound(v.begin(), v.end(), std::make_pair(c->h_gene_ce[k], INT64_MAX));
+    int64_t lo = INT64_MAX, hi = -1;
+    for (auto q = a; q < b; ++q) {
+      lo = std::min(lo, q->second);
+      hi = std::max(hi, q->second);
+    }
+    if (hi >= 0) {
+      c->win_lo[k] = lo;
+      c->win_hi[k] = hi + 1;
+    }
+  }
+  return EQTL_OK;
+}
+
+static int scan_begin_impl(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t scope, double pthr,
+                           bool windows_ready);
+
+int eqtl_gpu_scan_begin(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t scope, double pthr) {
+  return scan_begin_impl(c, model, prec, scope, pthr, false);
+}
+
+// windows_ready: c->win_lo / win_hi already hold the cis SNP ranges.
+static int scan_begin_impl(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t scope, double pthr,
+                           bool windows_ready) {
   if (!c) return fail(EQTL_E_ARG, "null ctx");
   if (model != EQTL_MODEL_LM && model != EQTL_MODEL_LMM) return fail(EQTL_E_ARG, "bad model");
   if (prec != EQTL_PREC_FP32 && prec != EQTL_PREC_BF16X3) return fail(EQTL_E_ARG, "bad precision");
+  if (scope != EQTL_SCOPE_TRANS && scope != EQTL_SCOPE_CIS) return fail(EQTL_E_ARG, "bad scope");
   if (c->prepped_model != model) return fail(EQTL_E_STATE, "gene side not prepared for this model");
   if (c->have_windows && c->m_annot <= 0) return fail(EQTL_E_STATE, "gene windows need the SNP annotation");
+  if (scope == EQTL_SCOPE_CIS && !c->have_windows) return fail(EQTL_E_STATE, "cis scope needs gene windows");
+  if (scope == EQTL_SCOPE_CIS && !windows_ready && static_cast<int>(c->h_gene_chrom.size()) != c->G)
+    return fail(EQTL_E_STATE, "gene windows were set for a different gene count");
   CHK(set_device(c));
   c->scan_model = model;
   c->scan_prec = prec;
+  c->scan_scope = scope;
   c->pthr = pthr;
+  if (scope == EQTL_SCOPE_CIS && !windows_ready) CHK(cis_windows(c));
   CHK(ensure(c->state, sizeof(GeneAcc) * c->G_pad));
   hipLaunchKernelGGL(k_state_init, dim3((c->G_pad + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
                      c->G_pad);
@@ -974,8 +1053,43 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
                        c->A64.as<double>(), m_pad, c->n, np, c->Wx.as<double>(), c->p, c->gtg.as<double>());
     CHK(check_launch("k_lm_gtg"));
   }
-  // work split: enough workgroups to fill the chip, SNP tiles strided per split
   const int n_groups = c->G_pad / 16;
+  // cis: work items {group, tile_lo, tile_hi} over the tiles of this block
+  // that hold some window SNP of the group's genes
+  const bool cis = c->scan_scope == EQTL_SCOPE_CIS;
+  int n_items = 0;
+  if (cis) {
+    std::vector<int32_t> items, first(n_groups + 1, 0);
+    for (int g = 0; g < n_groups; ++g) {
+      first[g] = static_cast<int32_t>(items.size() / 3);
+      int64_t lo = INT64_MAX, hi = -1;
+      for (int k = 16 * g; k < std::min(16 * g + 16, c->G); ++k)
+        if (c->win_lo[k] < c->win_hi[k]) {
+          lo = std::min(lo, c->win_lo[k]);
+          hi = std::max(hi, c->win_hi[k]);
+        }
+      const int64_t l = std::max(lo, snp0) - snp0, h = std::min(hi, snp0 + m) - snp0;
+      if (l >= h) continue;
+      const int t0 = static_cast<int>(l / kTileRows), t1 = static_cast<int>((h + kTileRows - 1) / kTileRows);
+      for (int t = t0; t < t1; t += kCisItemTiles) {
+        items.push_back(g);
+        items.push_back(t);
+        items.push_back(std::min(t + kCisItemTiles, t1));
+      }
+    }
+    first[n_groups] = static_cast<int32_t>(items.size() / 3);
+    n_items = first[n_groups];
+    if (n_items == 0) {
+      if (n_hits) *n_hits = 0;
+      return EQTL_OK;
+    }
+    CHK(ensure(c->work, sizeof(int32_t) * items.size()));
+    CHK(ensure(c->item_first, sizeof(int32_t) * first.size()));
+    HIPC(hipMemcpyAsync(c->work.p, items.data(), sizeof(int32_t) * items.size(), hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->item_first.p, first.data(), sizeof(int32_t) * first.size(), hipMemcpyHostToDevice,
+                        c->stream));
+  }
+  // work split: enough workgroups to fill the chip, SNP tiles strided per split
   const int n_groups_pad = static_cast<int>(round_up(n_groups, 8));
   const int n_tiles = static_cast<int>(m_pad / 64);
   const int n_tiles128 = static_cast<int>(m_pad / kTileRows);
@@ -986,7 +1100,8 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   const int cus = std::max(c->n_cu, 8);
   int n_splits = std::max(cus / 8, (2 * cus + n_groups - 1) / n_groups);
   n_splits = std::max(1, std::min(n_tiles128, n_splits));
-  CHK(ensure(c->partial, sizeof(GeneAcc) * static_cast<size_t>(n_splits) * c->G_pad));
+  CHK(ensure(c->partial, sizeof(GeneAcc) * (cis ? static_cast<size_t>(n_items) * 16
+                                                : static_cast<size_t>(n_splits) * c->G_pad)));
   const int64_t cap_dev = std::max<int64_t>(hit_cap, 1);
   if (c->hit_cap_dev < cap_dev) {
     CHK(ensure(c->hits, sizeof(eqtl_hit) * static_cast<size_t>(cap_dev)));
@@ -1031,13 +1146,19 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   a.tau_hi = kRefineTauHi;
   a.polish = lm ? nullptr : c->polish.as<PolishItem>();
   a.polish_cap = lm ? 0 : kPolishCap;
+  a.cis = cis ? 1 : 0;
+  a.work = cis ? c->work.as<int32_t>() : nullptr;
   {
     Timed t(c, "wald");
-    CHK(launch_wald(c, a, dim3(static_cast<unsigned>(n_groups_pad * n_splits))));
+    CHK(launch_wald(c, a, dim3(static_cast<unsigned>(cis ? n_items : n_groups_pad * n_splits))));
     CHK(t.done());
   }
-  hipLaunchKernelGGL(k_merge, dim3((c->G + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
-                     c->partial.as<GeneAcc>(), c->G, c->G_pad, n_splits);
+  if (cis)
+    hipLaunchKernelGGL(k_merge_items, dim3((c->G + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
+                       c->partial.as<GeneAcc>(), c->item_first.as<int32_t>(), c->G);
+  else
+    hipLaunchKernelGGL(k_merge, dim3((c->G + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
+                       c->partial.as<GeneAcc>(), c->G, c->G_pad, n_splits);
   CHK(check_launch("k_merge"));
   unsigned long long cnt2[2] = {0, 0};
   HIPC(hipMemcpyAsync(cnt2, c->hit_count.p, sizeof(cnt2), hipMemcpyDeviceToHost, c->stream));
@@ -1106,6 +1227,173 @@ int eqtl_gpu_scan_end(eqtl_gpu_ctx* c, eqtl_gene_state* out) {
   HIPC(hipMemcpyAsync(out, o.p, sizeof(eqtl_gene_state) * c->G, hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   c->scanning = false;
+  return EQTL_OK;
+}
+
+// ---- cis permutations (scan_gene_snps permutation loop, scan_cis.hpp:69-180) ----
+
+int eqtl_gpu_upload_bed(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, int64_t m, int32_t bps) {
+  if (!c || !rows || m <= 0 || bps <= 0) return fail(EQTL_E_ARG, "bad upload_bed arguments");
+  CHK(set_device(c));
+  const size_t bytes = static_cast<size_t>(m) * bps;
+  CHK(ensure(c->bed_res, bytes));
+  HIPC(hipMemcpyAsync(c->bed_res.p, rows, bytes, on_dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                      c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  c->bed_m = m;
+  c->bed_bps = bps;
+  return EQTL_OK;
+}
+
+// Pseudo-genes (gene, b) per batch, and SNPs per scan block of a batch.
+constexpr int kPermBatch = 16384;
+constexpr int64_t kCisBlock = 32768;
+
+int eqtl_gpu_cis_perm(eqtl_gpu_ctx* c, int32_t precision, int32_t B, int32_t seed, const uint32_t* gene_hash,
+                      const double* best_p_obs, int32_t delta_mode, double* perm_min_p, double* p_emp) {
+  if (!c || !gene_hash || !best_p_obs || B <= 0) return fail(EQTL_E_ARG, "bad cis_perm arguments");
+  if (c->scanning) return fail(EQTL_E_STATE, "a scan is in progress");
+  const int model = c->prepped_model;
+  if (model != EQTL_MODEL_LM && model != EQTL_MODEL_LMM) return fail(EQTL_E_STATE, "genes not prepared");
+  if (!c->have_windows || static_cast<int>(c->h_gene_chrom.size()) != c->G)
+    return fail(EQTL_E_STATE, "cis permutations need the gene windows");
+  if (c->bed_m <= 0 || c->bed_m != c->m_annot) return fail(EQTL_E_STATE, "upload the bed (eqtl_gpu_upload_bed) first");
+  if (model == EQTL_MODEL_LMM && delta_mode != EQTL_DELTA_REF && delta_mode != EQTL_DELTA_NEWTON)
+    return fail(EQTL_E_ARG, "permutation delta mode must be REF or NEWTON");
+  CHK(set_device(c));
+  CHK(ensure_blas(c));
+  const int G = c->G, n = c->n;
+  CHK(cis_windows(c));
+  // per-gene permutation bases, kept apart from the gene side that the batches reuse
+  DBuf base, yt, dlt;
+  CHK(ensure(base, sizeof(double) * static_cast<size_t>(n) * G));
+  if (model == EQTL_MODEL_LM) {
+    HIPC(hipMemcpyAsync(base.p, c->U64.p, sizeof(double) * static_cast<size_t>(n) * G, hipMemcpyDeviceToDevice,
+                        c->stream));
+  } else {
+    CHK(ensure(yt, sizeof(double) * static_cast<size_t>(n) * G));
+    CHK(ensure(dlt, sizeof(double) * G));
+    HIPC(hipMemcpyAsync(yt.p, c->Yt64.p, sizeof(double) * static_cast<size_t>(n) * G, hipMemcpyDeviceToDevice,
+                        c->stream));
+    HIPC(hipMemcpyAsync(dlt.p, c->delta.p, sizeof(double) * G, hipMemcpyDeviceToDevice, c->stream));
+    const size_t N = static_cast<size_t>(n) * G;
+    hipLaunchKernelGGL(k_whitened_residual, dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256), 0, c->stream,
+                       c->U64.as<double>(), dlt.as<double>(), c->lam64.as<double>(), n, G, base.as<double>());
+    CHK(check_launch("k_whitened_residual"));
+  }
+  const std::vector<int64_t> glo = c->win_lo, ghi = c->win_hi;
+  const std::vector<int32_t> gch = c->h_gene_chrom;
+  const std::vector<int64_t> gcs = c->h_gene_cs, gce = c->h_gene_ce;
+  const int p = c->p;
+  // genes with a tested SNP (the reference permutes only when n_tested > 0)
+  std::vector<int32_t> genes;
+  for (int k = 0; k < G; ++k)
+    if (std::isfinite(best_p_obs[k]) && best_p_obs[k] <= 1.0 && glo[k] < ghi[k]) genes.push_back(k);
+  std::vector<double> minp(static_cast<size_t>(G) * B, std::numeric_limits<double>::quiet_NaN());
+  const int nb_max = std::max(1, kPermBatch / B);
+  DBuf mt, idx, gof, hof;
+  int rc = EQTL_OK;
+  for (size_t g0 = 0; g0 < genes.size() && rc == EQTL_OK; g0 += nb_max) {
+    const int nb = static_cast<int>(std::min<size_t>(nb_max, genes.size() - g0));
+    const int E = nb * B;
+    std::vector<int32_t> go(genes.begin() + g0, genes.begin() + g0 + nb);
+    std::vector<uint32_t> ho(nb);
+    for (int q = 0; q < nb; ++q) ho[q] = gene_hash[go[q]];
+    CHK(ensure(mt, sizeof(uint32_t) * static_cast<size_t>(E) * kMtN));
+    CHK(ensure(idx, sizeof(int32_t) * static_cast<size_t>(E) * n));
+    CHK(ensure(gof, sizeof(int32_t) * nb));
+    CHK(ensure(hof, sizeof(uint32_t) * nb));
+    HIPC(hipMemcpyAsync(gof.p, go.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(hof.p, ho.data(), sizeof(uint32_t) * nb, hipMemcpyHostToDevice, c->stream));
+    // pseudo-gene phenotypes and their null fits
+    if (model == EQTL_MODEL_LM) {
+      CHK(alloc_gene_side(c, E, 1, 1));
+      CHK(ensure(c->Y64, sizeof(double) * static_cast<size_t>(n) * E));
+      CHK(ensure(c->U64, sizeof(double) * static_cast<size_t>(n) * E));
+      hipLaunchKernelGGL(k_perm_pheno, dim3((E + 63) / 64), dim3(64), 0, c->stream, E, B, n, seed,
+                         gof.as<int32_t>(), hof.as<uint32_t>(), mt.as<uint32_t>(), idx.as<int32_t>(), 0,
+                         base.as<double>(), nullptr, nullptr, nullptr, c->Y64.as<double>());
+      CHK(check_launch("k_perm_pheno"));
+      hipLaunchKernelGGL(k_lm_gene, dim3(E), dim3(256), 0, c->stream, c->Y64.as<double>(), c->Wx.as<double>(), n,
+                         c->n_pad, p, E, gene_cols(c), c->gscal.as<double>(), c->gvalid.as<int>(),
+                         c->U64.as<double>());
+      CHK(check_launch("k_lm_gene"));
+      c->prepped_model = EQTL_MODEL_LM;
+    } else {
+      CHK(alloc_gene_side(c, E, p + 2, lmm_nq_pad(p + 2)));
+      CHK(ensure(c->Yt64, sizeof(double) * static_cast<size_t>(n) * E));
+      CHK(ensure(c->U64, sizeof(double) * static_cast<size_t>(n) * E));
+      CHK(ensure(c->V64, sizeof(double) * static_cast<size_t>(n) * E));
+      CHK(ensure(c->Lc, sizeof(double) * static_cast<size_t>(p) * p * E));
+      hipLaunchKernelGGL(k_perm_pheno, dim3((E + 63) / 64), dim3(64), 0, c->stream, E, B, n, seed,
+                         gof.as<int32_t>(), hof.as<uint32_t>(), mt.as<uint32_t>(), idx.as<int32_t>(), 1,
+                         base.as<double>(), yt.as<double>(), dlt.as<double>(), c->lam64.as<double>(),
+                         c->Yt64.as<double>());
+      CHK(check_launch("k_perm_pheno"));
+      CHK(lmm_prep_core(c, E, delta_mode, nullptr, nullptr, nullptr));
+    }
+    // windows of the pseudo-genes (their gene's)
+    std::vector<int32_t> ech(E);
+    std::vector<int64_t> ecs(E), ece(E);
+    c->win_lo.assign(E, 0);
+    c->win_hi.assign(E, 0);
+    int64_t lo = INT64_MAX, hi = -1;
+    for (int e = 0; e < E; ++e) {
+      const int k = go[e / B];
+      ech[e] = gch[k];
+      ecs[e] = gcs[k];
+      ece[e] = gce[k];
+      c->win_lo[e] = glo[k];
+      c->win_hi[e] = ghi[k];
+      lo = std::min(lo, glo[k]);
+      hi = std::max(hi, ghi[k]);
+    }
+    CHK(ensure(c->gene_chrom, sizeof(int32_t) * E));
+    CHK(ensure(c->gene_cs, sizeof(int64_t) * E));
+    CHK(ensure(c->gene_ce, sizeof(int64_t) * E));
+    HIPC(hipMemcpyAsync(c->gene_chrom.p, ech.data(), sizeof(int32_t) * E, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->gene_cs.p, ecs.data(), sizeof(int64_t) * E, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->gene_ce.p, ece.data(), sizeof(int64_t) * E, hipMemcpyHostToDevice, c->stream));
+    c->have_windows = true;
+    // min p of every pseudo-gene over its window (no hit output)
+    CHK(scan_begin_impl(c, model, precision, EQTL_SCOPE_CIS, -1.0, true));
+    for (int64_t j0 = lo; j0 < hi && rc == EQTL_OK; j0 += kCisBlock) {
+      const int64_t mb = std::min<int64_t>(kCisBlock, hi - j0);
+      int64_t nh = 0;
+      rc = eqtl_gpu_scan_block(c, c->bed_res.as<uint8_t>() + j0 * c->bed_bps, 1, j0, mb, c->bed_bps, nullptr, 0,
+                               &nh);
+    }
+    std::vector<eqtl_gene_state> st(E);
+    if (rc == EQTL_OK) rc = eqtl_gpu_scan_end(c, st.data());
+    else c->scanning = false;
+    if (rc != EQTL_OK) break;
+    for (int e = 0; e < E; ++e) {
+      const int k = go[e / B], b = e % B;
+      minp[static_cast<size_t>(k) * B + b] = std::min(1.0, st[e].best_p);
+    }
+  }
+  // the gene side now holds the last batch: callers prepare genes again
+  c->prepped_model = -1;
+  c->have_windows = false;
+  c->h_gene_chrom.clear();
+  if (rc != EQTL_OK) return rc;
+  for (int k = 0; k < G; ++k) {
+    const bool done = std::isfinite(minp[static_cast<size_t>(k) * B]);
+    if (p_emp) {
+      if (!done) {
+        p_emp[k] = std::numeric_limits<double>::quiet_NaN();
+      } else {
+        // p_emp_count (stats_extra.cpp:39-46) on T = -log10(max(p, 1e-300))
+        const double Tobs = -std::log10(std::max(best_p_obs[k], 1e-300));
+        int ge = 0;
+        for (int b = 0; b < B; ++b)
+          if (-std::log10(std::max(minp[static_cast<size_t>(k) * B + b], 1e-300)) >= Tobs) ++ge;
+        p_emp[k] = (1.0 + ge) / (B + 1.0);
+      }
+    }
+    if (perm_min_p)
+      for (int b = 0; b < B; ++b) perm_min_p[static_cast<size_t>(k) * B + b] = minp[static_cast<size_t>(k) * B + b];
+  }
   return EQTL_OK;
 }
 

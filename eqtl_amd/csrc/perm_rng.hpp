@@ -100,3 +100,56 @@ EQ_PHD uint32_t fnv1a32(const char* s) {  // util.hpp:51-58 (low 32 bits of 64-b
 }
 
 }  // namespace eqtl_amd
+
+#if defined(__HIPCC__) || defined(__HIP__)
+namespace eqtl_amd {
+
+// Permuted phenotypes of one batch of (gene, b) pairs, one thread per pair
+// e = local_gene * B + b (scan_cis.hpp:104-148): the index stream of
+// (seed, b, gene) permutes the gene's base vector.
+//   LM : y_e[i]  = y_s[idx[i]]                       (sample space)
+//   LMM: y~_e[i] = y~[i] + (w[idx[i]] - w[i]) / sqrt(max(d_i, 1e-12))
+//        with w = r~ o sqrt(d) the whitened null residual, i.e. the rotated
+//        X~ b0 + r~_perm of the reference's y = Q (X~ b0 + r~_perm).
+// mt: E x kMtN words of generator state, idx: E x n scratch.
+__global__ void k_perm_pheno(int E, int B, int n, int32_t seed, const int32_t* gene_of, const uint32_t* hash_of,
+                             uint32_t* mt, int32_t* idx, int lmm, const double* base, const double* yt,
+                             const double* delta, const double* lam, double* out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int lk = e / B, b = e % B;
+  const int k = gene_of[lk];
+  Mt19937 g{mt + static_cast<size_t>(e) * kMtN, kMtN};
+  g.seed(perm_seed(seed, b, hash_of[lk]));
+  int32_t* ix = idx + static_cast<size_t>(e) * n;
+  shuffle_iota(g, ix, n);
+  const double* bk = base + static_cast<size_t>(k) * n;
+  double* o = out + static_cast<size_t>(e) * n;
+  if (!lmm) {
+    for (int i = 0; i < n; ++i) o[i] = bk[ix[i]];
+  } else {
+    const double* yk = yt + static_cast<size_t>(k) * n;
+    const double dk = delta[k];
+    for (int i = 0; i < n; ++i) {
+      double v = dk * lam[i] + 1.0;
+      if (v < 1e-12) v = 1e-12;
+      const double d = 1.0 / v;
+      o[i] = yk[i] + (bk[ix[i]] - bk[i]) / sqrt(fmax(d, 1e-12));
+    }
+  }
+}
+
+// w = U / sqrt(d): the whitened null residual from the kernel's s o (s o r~)
+// column (k_lmm_basis stores U = s o r with r = s o r~).
+__global__ void k_whitened_residual(const double* U, const double* delta, const double* lam, int n, int G,
+                                    double* w) {
+  const size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<size_t>(n) * G) return;
+  const int k = static_cast<int>(i / n), r = static_cast<int>(i % n);
+  double v = delta[k] * lam[r] + 1.0;
+  if (v < 1e-12) v = 1e-12;
+  w[i] = U[i] * sqrt(v);
+}
+
+}  // namespace eqtl_amd
+#endif

@@ -716,7 +716,8 @@ void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model,
     eng.check(eqtl_gpu_lm_prep(c, Y.data(), G, cov.X.data(), cov.p, nullptr), "lm_prep");
   }
   eng.check(eqtl_gpu_set_gene_windows(c, gch.data(), gcs.data(), gce.data(), G), "set_gene_windows");
-  eng.check(eqtl_gpu_scan_begin(c, model == Model::Lmm ? EQTL_MODEL_LMM : EQTL_MODEL_LM, opt.precision, pthr),
+  eng.check(eqtl_gpu_scan_begin(c, model == Model::Lmm ? EQTL_MODEL_LMM : EQTL_MODEL_LM, opt.precision,
+                                EQTL_SCOPE_TRANS, pthr),
             "scan_begin");
   std::vector<uint8_t> buf;
   std::vector<eqtl_hit> hits(1 << 20);

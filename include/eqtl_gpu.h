@@ -44,6 +44,11 @@ enum { EQTL_PREC_FP32 = 0, EQTL_PREC_BF16X3 = 1 };
  * (model_lmm.cpp:39-86); NEWTON = grid start + safeguarded Newton on log delta
  * with the analytic adjoint gradient/Hessian; GIVEN = caller's delta_in. */
 enum { EQTL_DELTA_REF = 0, EQTL_DELTA_NEWTON = 1, EQTL_DELTA_GIVEN = 2 };
+/* Scan scope: TRANS tests every SNP outside the gene's window (in_cis_window,
+ * scan_common.hpp:65-69; every SNP when no windows are set); CIS tests only
+ * the SNPs inside it (for_each_snp_region over cis_window_bounds,
+ * scan_cis.cpp:122-129). */
+enum { EQTL_SCOPE_TRANS = 0, EQTL_SCOPE_CIS = 1 };
 
 /* One significant (p <= pthr) pair — the fields of write_pair_line
  * (output.cpp:27-76) that the engine computes; text columns come from the bim. */
@@ -129,8 +134,9 @@ int eqtl_gpu_lm_prep(eqtl_gpu_ctx* ctx, const double* Y, int32_t G, const double
 int eqtl_gpu_set_gene_windows(eqtl_gpu_ctx* ctx, const int32_t* chrom, const int64_t* cstart,
                               const int64_t* cend, int32_t G);
 
-/* ---- trans scans (scan_lm_snp_outer / scan_lmm_snp_outer) ------------------ */
-int eqtl_gpu_scan_begin(eqtl_gpu_ctx* ctx, int32_t model, int32_t precision, double pthr);
+/* ---- SNP-block scans (scan_lm_snp_outer / scan_lmm_snp_outer; the cis scope
+ * runs the same tests per gene window, scan_gene_snps without permutations) --- */
+int eqtl_gpu_scan_begin(eqtl_gpu_ctx* ctx, int32_t model, int32_t precision, int32_t scope, double pthr);
 /* One block of m SNPs with global indices snp0 .. snp0+m-1 (bim order). Hits
  * for this block are appended to hits (capacity hit_cap) in (snp, gene)
  * order; *n_hits is the block's count (EQTL_E_CAPACITY if it exceeds hit_cap;
@@ -140,6 +146,25 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_
                         int64_t hit_cap, int64_t* n_hits);
 /* Final per-gene summaries (G entries). */
 int eqtl_gpu_scan_end(eqtl_gpu_ctx* ctx, eqtl_gene_state* state);
+
+/* ---- cis permutations (scan_gene_snps, scan_cis.hpp:69-180) ------------------ */
+/* Resident genotypes (all m SNPs of the annotation, bim order) for the
+ * permutation stage, which revisits every gene's window B times. */
+int eqtl_gpu_upload_bed(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_device, int64_t m,
+                        int32_t bytes_per_snp);
+/* Gene-level permutations after an observed cis scan (scope CIS) of the
+ * prepared genes: for every gene with a tested SNP and b < B, the index
+ * stream mt19937((seed >= 0 ? seed : 1) + 9973 b + gene_hash[k]) + std::shuffle
+ * permutes the gene's null residual (LM: y_s; LMM: the whitened spectral
+ * residual, un-whitened and re-fit with its own REML delta, delta_mode REF or
+ * NEWTON), and the minimum p over the gene's window is recorded.
+ * gene_hash[k] = fnv1a of the gene id (util.hpp:51-58); best_p_obs = the
+ * observed scan's best_p. Outputs (host, nullable): perm_min_p G x B (row k =
+ * gene k; NaN for genes not permuted) and p_emp (p_emp_count,
+ * stats_extra.cpp:39-46; NaN when not permuted). The gene side is consumed:
+ * prepare genes again before the next scan. */
+int eqtl_gpu_cis_perm(eqtl_gpu_ctx* ctx, int32_t precision, int32_t B, int32_t seed, const uint32_t* gene_hash,
+                      const double* best_p_obs, int32_t delta_mode, double* perm_min_p, double* p_emp);
 
 /* ---- instrumentation ------------------------------------------------------- */
 /* When enabled, the library brackets each launch of the named kernel family

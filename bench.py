@@ -27,6 +27,8 @@ sys.path.insert(0, ROOT)
 
 N, M, G, NCOV, NCHROM = 2000, 500_000, 20_000, 20, 10
 BLOCK = 32768
+# CPU baseline sample: ~10-20 s of the FP64 port on the host cores
+CPU_SNPS, CPU_GENES = 400, 10000
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
 FP32_PEAK_TFLOPS = 157.3
@@ -123,7 +125,7 @@ def accuracy_sample(ctx, rows_dev, bps, X, Y, win, delta, planted, prec, Q, lam)
             "max_abs_dt": float(np.max(np.abs(tg[ok] - ref["stat"][ok])))}
 
 
-def cpu_baseline(Q, lam, X, Y, rows, delta, seconds_target=15.0):
+def cpu_baseline(Q, lam, X, Y, rows, delta):
     """FP64 oracle (port of the reference's SNP loop) timed on the host cores."""
     import oracle as O
     threads = O.num_threads()
@@ -134,12 +136,32 @@ def cpu_baseline(Q, lam, X, Y, rows, delta, seconds_target=15.0):
     dt = res["loop_seconds"]
     tests = int(res["state"]["n_tested"].sum())
     return {"value": tests / dt, "unit": "Wald tests/s", "cores": threads, "kind": "port",
-            "sample": f"{n_snp} SNPs x {Y.shape[1]} genes (n={N}, p={X.shape[1]}), FP64 restatement of the "
-                      f"reference SNP-outer loop (Q^T g per SNP, test_lmm_gtil per gene), timed over the SNP loop "
-                      f"only, per-gene prep untimed ({tests} tests in {dt:.1f} s)"}
+            "sample": f"{n_snp} SNPs x {Y.shape[1]} genes of the same data (n={N}, p={X.shape[1]}, the genes' "
+                      f"own delta), FP64 restatement of the reference SNP-outer loop (Q^T g per SNP, "
+                      f"test_lmm_gtil per gene) on {threads} threads, timed over the SNP loop only, per-gene "
+                      f"prep untimed ({tests} tests in {dt:.1f} s)"}
+
+
+def measured_traffic(block):
+    """HBM bytes per launch of the fused Wald kernel (split form, this workload)
+    from the committed rocprofv3 PMC summary: FETCH_SIZE and WRITE_SIZE runs,
+    each scaled by the same counter on a 1 GiB device copy. None when the
+    summary was taken at another block size or is absent."""
+    path = os.path.join(ROOT, "profiles", "wald_bf16x3_trans_lmm_n2000_g20000_pmc.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:") or line.startswith("snps_per_launch:"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = float(v.strip())
+    if vals.get("snps_per_launch") != block or "traffic_bytes_per_launch" not in vals:
+        return None
+    return vals["traffic_bytes_per_launch"]
 
 
 def main():
+    global BLOCK
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
@@ -147,6 +169,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--snps", type=int, default=M, help="SNPs per step (default: the full 500k)")
+    ap.add_argument("--block", type=int, default=BLOCK, help="SNPs per scan block")
     ap.add_argument("--probe-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -154,6 +177,7 @@ def main():
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
     import eqtl_amd
+    BLOCK = args.block
     if args.probe_copy:
         import torch
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -219,7 +243,7 @@ def main():
     achieved = flop_per_test * tests_per_launch / (wald_launch_ms * 1e-3) / 1e12
     extra = {}
     # plain FP32 form on a slice of the same data (REF delta search, as the reference)
-    m32 = min(m, 4 * BLOCK)
+    m32 = min(m, 131072)
     ctx.set_timing(True)
     ctx.synchronize()
     t1 = time.perf_counter()
@@ -237,8 +261,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         import oracle as O  # noqa: F401  (timed CPU baseline only)
-        rows_s = ctx.copy_rows(rows_dev, 0, 8, bps)
-        cpu = cpu_baseline(Q, lam, X, Y, rows_s, delta)
+        rows_s = ctx.copy_rows(rows_dev, 0, CPU_SNPS, bps)
+        cpu = cpu_baseline(Q, lam, X, Y[:, :CPU_GENES], rows_s, delta[:CPU_GENES])
         log("cpu baseline", cpu)
     out = {
         "metric": "Wald tests/sec trans-LMM (split-BF16 vs FP32 GPU), max p rel err vs FP64",
@@ -246,9 +270,10 @@ def main():
         "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "bf16x3 contractions, fp64 epilogue", "data": "synthetic (generated in HBM, seeded)",
         "config": {"workload": "trans-LMM n=2000 samples x 500k SNPs x 20k genes, p=21, 1% missing",
-                   "snps_per_step": m, "genes": G, "samples": N, "covariates": p, "delta": "batched Newton"},
+                   "snps_per_step": m, "genes": G, "samples": N, "covariates": p, "delta": "batched Newton",
+                   "snps_per_block": BLOCK},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": measured_traffic(BLOCK),
                      "note": "algorithmic 2n(p+2) FLOP/test over the fused Wald kernel's event-timed launches; "
                              "the split form issues 3 bf16 MFMA products per algorithmic MAC"},
         "cpu_baseline": cpu,

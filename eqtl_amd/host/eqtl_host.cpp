@@ -793,6 +793,172 @@ void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model,
   }
 }
 
+// FNV-1a of a gene id, low 32 bits (util.hpp:51-58): permutation seeds.
+static uint32_t fnv1a(const std::string& s) {
+  unsigned long long h = 14695981039346656037ULL;
+  for (unsigned char ch : s) {
+    h ^= ch;
+    h *= 1099511628211ULL;
+  }
+  return static_cast<uint32_t>(h & 0xFFFFFFFFu);
+}
+
+// Cis scan of every gene's window with optional gene-level permutations
+// (run_cis_parallel, scan_cis.cpp:18-163, and scan_gene_snps,
+// scan_cis.hpp:15-185): genes in (chromosome, TSS) order, pairs written
+// gene by gene in SNP order, top only when best p <= pthr, p_emp from the
+// engine's permutation stage.
+void scan_cis(const Options& opt, Engine& eng, PlinkBed& bed, Model model, ScopeOut& out, double pthr,
+              std::vector<TransJob>& jobs, const PhenoData& ph, const CovData& cov) {
+  if (jobs.empty()) return;
+  std::stable_sort(jobs.begin(), jobs.end(), [](const TransJob& a, const TransJob& b) {
+    if (a.loc.chrom != b.loc.chrom) return a.loc.chrom < b.loc.chrom;
+    return a.loc.tss < b.loc.tss;
+  });
+  const int n = static_cast<int>(ph.n());
+  const int G = static_cast<int>(jobs.size());
+  std::vector<double> Y(static_cast<size_t>(n) * G);
+  for (int k = 0; k < G; ++k)
+    std::copy(ph.Y.begin() + static_cast<size_t>(jobs[k].col) * n,
+              ph.Y.begin() + static_cast<size_t>(jobs[k].col + 1) * n, Y.begin() + static_cast<size_t>(k) * n);
+  const auto& sites = bed.sites();
+  std::vector<std::string> chroms = bed.chromosomes();
+  auto chrom_id = [&](const std::string& c) -> int32_t {
+    for (size_t i = 0; i < chroms.size(); ++i)
+      if (chrom_equal(chroms[i], c)) return static_cast<int32_t>(i);
+    return -1;  // chromosome not in the bed: empty window
+  };
+  std::vector<int32_t> snp_chrom(sites.size());
+  std::vector<int64_t> snp_pos(sites.size());
+  for (size_t j = 0, ci = 0; j < sites.size(); ++j) {
+    while (!chrom_equal(sites[j].chrom, chroms[ci])) ++ci;
+    snp_chrom[j] = static_cast<int32_t>(ci);
+    snp_pos[j] = sites[j].pos;
+  }
+  // windows (cis_window_bounds, scan_common.hpp:58-63) and their SNP index ranges
+  // chromosomes are contiguous runs of the bim with sorted positions (checked when the bim is read)
+  std::vector<size_t> cbeg(chroms.size(), 0), cend(chroms.size(), 0);
+  for (size_t j = 0; j < sites.size(); ++j) {
+    if (j == 0 || snp_chrom[j] != snp_chrom[j - 1]) cbeg[snp_chrom[j]] = j;
+    cend[snp_chrom[j]] = j + 1;
+  }
+  std::vector<int32_t> gch(G);
+  std::vector<int64_t> gcs(G), gce(G);
+  int64_t lo = INT64_MAX, hi = -1;
+  for (int k = 0; k < G; ++k) {
+    gch[k] = chrom_id(jobs[k].loc.chrom);
+    gcs[k] = std::max<int64_t>(1, jobs[k].loc.start - opt.window);
+    gce[k] = jobs[k].loc.end + opt.window;
+    if (gch[k] < 0) continue;
+    auto b = snp_pos.begin() + cbeg[gch[k]], e = snp_pos.begin() + cend[gch[k]];
+    const auto a0 = std::lower_bound(b, e, gcs[k]), a1 = std::upper_bound(b, e, gce[k]);
+    if (a0 < a1) {
+      lo = std::min<int64_t>(lo, a0 - snp_pos.begin());
+      hi = std::max<int64_t>(hi, a1 - snp_pos.begin());
+    }
+  }
+  eqtl_gpu_ctx* c = eng.ctx();
+  eng.check(eqtl_gpu_set_snp_annot(c, snp_chrom.data(), snp_pos.data(), static_cast<int64_t>(snp_pos.size())),
+            "set_snp_annot");
+  if (opt.perm > 0) {
+    std::vector<uint8_t> all;
+    bed.read_rows(0, sites.size(), all);
+    eng.check(eqtl_gpu_upload_bed(c, all.data(), 0, static_cast<int64_t>(sites.size()), bed.bytes_per_snp()),
+              "upload_bed");
+  }
+  const int gm = model == Model::Lmm ? EQTL_MODEL_LMM : EQTL_MODEL_LM;
+  if (model == Model::Lmm)
+    eng.check(eqtl_gpu_lmm_prep(c, Y.data(), G, cov.X.data(), cov.p, opt.delta_mode, nullptr, nullptr, nullptr),
+              "lmm_prep");
+  else
+    eng.check(eqtl_gpu_lm_prep(c, Y.data(), G, cov.X.data(), cov.p, nullptr), "lm_prep");
+  eng.check(eqtl_gpu_set_gene_windows(c, gch.data(), gcs.data(), gce.data(), G), "set_gene_windows");
+  eng.check(eqtl_gpu_scan_begin(c, gm, opt.precision, EQTL_SCOPE_CIS, pthr), "scan_begin");
+  std::vector<uint8_t> buf;
+  std::vector<eqtl_hit> hits(1 << 20), all_hits;
+  for (int64_t j0 = std::max<int64_t>(lo, 0); j0 < hi; j0 += static_cast<int64_t>(kBlockSnps)) {
+    const size_t m = std::min<size_t>(kBlockSnps, static_cast<size_t>(hi - j0));
+    bed.read_rows(static_cast<size_t>(j0), m, buf);
+    int64_t nh = 0;
+    int rc = eqtl_gpu_scan_block(c, buf.data(), 0, j0, static_cast<int64_t>(m), bed.bytes_per_snp(), hits.data(),
+                                 static_cast<int64_t>(hits.size()), &nh);
+    if (rc == EQTL_E_CAPACITY)
+      die("more than " + std::to_string(hits.size()) + " significant pairs in one SNP block; lower --pval-cis");
+    eng.check(rc, "scan_block");
+    all_hits.insert(all_hits.end(), hits.begin(), hits.begin() + nh);
+  }
+  std::vector<eqtl_gene_state> st(G);
+  eng.check(eqtl_gpu_scan_end(c, st.data()), "scan_end");
+  auto fill = [&](AssocHit& h, const eqtl_hit& e) {
+    const TransJob& job = jobs[e.gene];
+    const BimSite& s = sites[e.snp];
+    h.gene = job.gene;
+    h.snp = s.id;
+    h.chrom = s.chrom;
+    h.pos = s.pos;
+    h.ref = s.a2;
+    h.alt = s.a1;
+    h.maf = e.af;
+    h.beta = e.beta;
+    h.se = e.se;
+    h.stat = e.stat;
+    h.p = e.p;
+    h.r2 = e.r2;
+    h.n = n;
+    if (job.loc.ok) {
+      h.has_tss_dist = true;
+      h.tss_dist = static_cast<double>(s.pos - job.loc.tss);
+    }
+  };
+  std::stable_sort(all_hits.begin(), all_hits.end(), [](const eqtl_hit& a, const eqtl_hit& b) {
+    return a.gene != b.gene ? a.gene < b.gene : a.snp < b.snp;
+  });
+  for (const auto& e : all_hits) {
+    AssocHit h;
+    fill(h, e);
+    write_pair_line(out.pairs, h, model, "cis");
+  }
+  std::vector<double> p_emp(G, std::numeric_limits<double>::quiet_NaN());
+  if (opt.perm > 0) {
+    std::vector<uint32_t> hash(G);
+    std::vector<double> best(G);
+    for (int k = 0; k < G; ++k) {
+      hash[k] = fnv1a(jobs[k].gene);
+      best[k] = st[k].n_tested > 0 ? st[k].best_p : std::numeric_limits<double>::quiet_NaN();
+    }
+    info(std::string("cis ") + (model == Model::Lmm ? "LMM" : "LM") + ": " + std::to_string(opt.perm) +
+         " gene-level permutations");
+    eng.check(eqtl_gpu_cis_perm(c, opt.precision, opt.perm, opt.seed, hash.data(), best.data(), opt.delta_mode,
+                                nullptr, p_emp.data()),
+              "cis_perm");
+    if (!opt.disable_beta_approx) warn("beta-approximated permutation p (p_beta) is not provided; p_beta = NA");
+  }
+  for (int k = 0; k < G; ++k) {
+    auto& job = jobs[k];
+    job.summary.gene = job.gene;
+    job.summary.chrom = job.loc.chrom;
+    job.summary.tss = job.loc.tss;
+    job.summary.n_tested = st[k].n_tested;
+    job.summary.n_sig = st[k].n_sig;
+    job.summary.acat_p = st[k].acat_p;
+    job.summary.p_emp = p_emp[k];
+    if (st[k].best_snp >= 0 && st[k].best_p <= pthr && st[k].best_p <= 1.0) {
+      eqtl_hit e;
+      e.snp = st[k].best_snp;
+      e.gene = k;
+      e.af = st[k].best_af;
+      e.beta = st[k].best_beta;
+      e.se = st[k].best_se;
+      e.stat = st[k].best_stat;
+      e.p = st[k].best_p;
+      e.r2 = st[k].best_r2;
+      AssocHit h;
+      fill(h, e);
+      write_pair_line(out.top, h, model, "cis");
+    }
+  }
+}
+
 // ============================================================ orchestrator (scan.cpp:18-340)
 int run_make_grm(const Options& opt) {
   PlinkBed bed;
@@ -872,7 +1038,6 @@ int run_eqtl(const Options& opt) {
   eng.check(eqtl_gpu_set_policy(eng.ctx(), opt.miss == MissHand::Impute, opt.max_miss, opt.maf), "set_policy");
   for (Model model : opt.models) {
     for (const std::string& scope : scopes) {
-      if (scope == "cis") die("cis scope is not provided by this engine build; use --mode trans or gw");
       const std::string prefix = opt.out + "." + model_str(model) + "." + scope;
       ScopeOut so;
       so.tag = scope;
@@ -883,7 +1048,7 @@ int run_eqtl(const Options& opt) {
       write_pairs_header(so.pairs, model);
       write_pairs_header(so.top, model);
       write_region_header(so.region);
-      const double pthr = opt.pval_trans;
+      const double pthr = (scope == "cis") ? opt.pval_cis : opt.pval_trans;
       std::vector<TransJob> jobs;
       for (size_t gi = 0; gi < ph.gene_ids.size(); ++gi) {
         TransJob job;
@@ -898,7 +1063,7 @@ int run_eqtl(const Options& opt) {
             continue;
           }
         }
-        if (scope == "trans" && !job.has_loc) continue;
+        if ((scope == "trans" || scope == "cis") && !job.has_loc) continue;
         // build_gene_ready (scan_common.cpp:11-35): need >= 3 complete samples and a non-zero y
         if (!all_finite(ph, job.col, cov))
           die("gene " + job.gene + " has missing phenotype/covariate values: per-gene sample subsets are not "
@@ -908,9 +1073,15 @@ int run_eqtl(const Options& opt) {
         if (ph.n() < 3 || ymax < 1e-15) continue;
         jobs.push_back(std::move(job));
       }
-      info(std::string("trans/gw ") + (model == Model::Lmm ? "LMM" : "LM") + ": SNP-outer (" +
-           std::to_string(jobs.size()) + " genes)");
-      scan_snp_outer(opt, eng, bed, model, scope, so, pthr, jobs, ph, cov, nullptr, nullptr);
+      if (scope == "cis") {
+        info(std::string("cis ") + (model == Model::Lmm ? "LMM" : "LM") + ": " + std::to_string(jobs.size()) +
+             " genes");
+        scan_cis(opt, eng, bed, model, so, pthr, jobs, ph, cov);
+      } else {
+        info(std::string("trans/gw ") + (model == Model::Lmm ? "LMM" : "LM") + ": SNP-outer (" +
+             std::to_string(jobs.size()) + " genes)");
+        scan_snp_outer(opt, eng, bed, model, scope, so, pthr, jobs, ph, cov, nullptr, nullptr);
+      }
       std::vector<GeneSummary> summaries;
       for (auto& j : jobs) summaries.push_back(j.summary);
       bh_adjust(summaries);

@@ -126,3 +126,58 @@ def test_cli_trans_end_to_end(small, tmp_path, model):
     for row in top:
         k = int(row[0][1:])
         assert int(row[1][2:]) - 1 == st["best_snp"][k]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model", ["lm", "lmm"])
+def test_cli_cis_with_permutations(small, tmp_path, model):
+    """--mode cis --perm B: pairs (gene-major), top, region with p_emp. LM p_emp is
+    compared with the oracle exactly; LMM permutations depend on the eigenvector
+    signs of the engine's own GRM basis, so for LMM only the observed statistics
+    and the p_emp range are checked."""
+    ds, prefix, d = small
+    out = str(tmp_path / "cis")
+    w, B, seed = 200000, 25, 42
+    r = run(["-b", prefix, "-e", prefix + ".pheno.tsv", "-c", prefix + ".covar.tsv", "-g", prefix + ".gff",
+             "--model", model, "--mode", "cis", "-w", str(w), "--pval-cis", "1e-3", "--precision", "fp32",
+             "--perm", str(B), "--seed", str(seed), "-o", out])
+    assert r.returncode == 0, r.stderr
+    rows = ds.bed_rows()
+    dos, missm, keep, af = O.decode_bed(rows, ds.n)
+    cs, ce = np.maximum(1, ds.gene_start - w), ds.gene_end + w
+    kw = {}
+    if model == "lmm":
+        K, N, used = O.grm(dos, missm, keep)
+        Q, lam = O.eigh(K)
+        kw = {"Q": Q, "lam": lam}
+    G = ds.Y.shape[1]
+    refs = {}
+    for k in range(G):
+        idx = np.nonzero((ds.chrom == ds.gene_chrom[k]) & (ds.pos >= cs[k]) & (ds.pos <= ce[k]) & keep)[0]
+        refs[f"g{k}"] = (idx, O.cis_gene(model, ds.Y[:, k], ds.X(), dos[idx], B=B if model == "lm" else 0,
+                                         seed=seed, gene=f"g{k}", **kw))
+    head, pairs = _read_tsv(f"{out}.{model}.cis.pairs.tsv")
+    exp = {(g, f"rs{idx[s] + 1}") for g, (idx, ref) in refs.items() for s in range(len(idx))
+           if np.isfinite(ref["p"][s]) and ref["p"][s] <= 1e-3}
+    near = {(g, f"rs{idx[s] + 1}") for g, (idx, ref) in refs.items() for s in range(len(idx))
+            if np.isfinite(ref["p"][s]) and abs(np.log(ref["p"][s] / 1e-3)) < 1e-3}
+    got = {(p[0], p[1]) for p in pairs}
+    assert (got ^ exp) <= near
+    assert all(p[14] == "cis" for p in pairs)
+    # gene-major, SNP order inside a gene
+    genes_in_order = []
+    for p in pairs:
+        if not genes_in_order or genes_in_order[-1] != p[0]:
+            assert p[0] not in genes_in_order
+            genes_in_order.append(p[0])
+    head, region = _read_tsv(f"{out}.{model}.cis.region.tsv")
+    for row in region:
+        idx, ref = refs[row[0]]
+        assert int(row[3]) == ref["n_tested"]
+        if model == "lm" and ref["n_tested"] > 0:
+            # %.10g of (1 + count) / (B + 1): the count is exact
+            assert round(float(row[7]) * (B + 1)) == round(ref["p_emp"] * (B + 1)), (row[0], row[7], ref["p_emp"])
+        elif ref["n_tested"] > 0:
+            pe = float(row[7])
+            assert 1.0 / (B + 1) <= pe <= 1.0
+        assert row[8] == "NA"

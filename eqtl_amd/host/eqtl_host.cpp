@@ -670,6 +670,73 @@ Grm compute_grm(Engine& eng, PlinkBed& bed, const std::vector<std::string>& samp
   return g;
 }
 
+static uint32_t fnv1a(const std::string& s);
+
+// Trans stage 2 (stage2_perm_topk, scan_perm.cpp:11-142): genes whose best p
+// is below --perm-trans-thr are permuted over their top-K SNPs (the K smallest
+// p below the threshold, earliest SNP first on ties). The engine's cis
+// permutation stage runs them on a virtual genotype set: gene k's top-K rows
+// in a block of their own, annotated as "chromosome" k with positions 1..K, so
+// that the block is gene k's window.
+static void stage2_perm(const Options& opt, Engine& eng, PlinkBed& bed, Model model, std::vector<TransJob>& jobs,
+                        std::vector<std::vector<std::pair<double, int64_t>>>& cand,
+                        const std::vector<eqtl_gene_state>& st, const std::vector<double>& Y, const CovData& cov) {
+  const int G = static_cast<int>(jobs.size());
+  const size_t n = Y.size() / std::max(G, 1);
+  std::vector<int> q;  // qualifying genes
+  for (int k = 0; k < G; ++k) {
+    jobs[k].summary.p_emp = std::numeric_limits<double>::quiet_NaN();
+    if (st[k].n_tested > 0 && st[k].best_p < opt.perm_trans_thr && !cand[k].empty()) q.push_back(k);
+  }
+  if (q.empty()) return;
+  const int bps = bed.bytes_per_snp();
+  std::vector<uint8_t> row, vbed;
+  std::vector<int32_t> vchrom, gch;
+  std::vector<int64_t> vpos, gcs, gce;
+  std::vector<double> Yq(n * q.size());
+  std::vector<uint32_t> hash;
+  std::vector<double> best;
+  for (size_t qi = 0; qi < q.size(); ++qi) {
+    const int k = q[qi];
+    auto& cs = cand[k];
+    std::sort(cs.begin(), cs.end());
+    if (static_cast<int>(cs.size()) > opt.perm_trans_top) cs.resize(opt.perm_trans_top);
+    std::sort(cs.begin(), cs.end(), [](const auto& a, const auto& b) { return a.second < b.second; });
+    for (size_t i = 0; i < cs.size(); ++i) {
+      bed.read_rows(static_cast<size_t>(cs[i].second), 1, row);
+      vbed.insert(vbed.end(), row.begin(), row.begin() + bps);
+      vchrom.push_back(static_cast<int32_t>(qi));
+      vpos.push_back(static_cast<int64_t>(i) + 1);
+    }
+    gch.push_back(static_cast<int32_t>(qi));
+    gcs.push_back(1);
+    gce.push_back(static_cast<int64_t>(cs.size()));
+    std::copy(Y.begin() + static_cast<size_t>(k) * n, Y.begin() + static_cast<size_t>(k + 1) * n,
+              Yq.begin() + qi * n);
+    hash.push_back(fnv1a(jobs[k].gene));
+    best.push_back(st[k].best_p);
+  }
+  const int Gq = static_cast<int>(q.size());
+  eqtl_gpu_ctx* c = eng.ctx();
+  const int64_t mv = static_cast<int64_t>(vpos.size());
+  eng.check(eqtl_gpu_set_snp_annot(c, vchrom.data(), vpos.data(), mv), "set_snp_annot");
+  eng.check(eqtl_gpu_upload_bed(c, vbed.data(), 0, mv, bps), "upload_bed");
+  if (model == Model::Lmm)
+    eng.check(eqtl_gpu_lmm_prep(c, Yq.data(), Gq, cov.X.data(), cov.p, opt.delta_mode, nullptr, nullptr, nullptr),
+              "lmm_prep");
+  else
+    eng.check(eqtl_gpu_lm_prep(c, Yq.data(), Gq, cov.X.data(), cov.p, nullptr), "lm_prep");
+  eng.check(eqtl_gpu_set_gene_windows(c, gch.data(), gcs.data(), gce.data(), Gq), "set_gene_windows");
+  info(std::string("trans stage 2: ") + std::to_string(Gq) + " genes x " + std::to_string(opt.perm) +
+       " permutations over their top-K SNPs");
+  std::vector<double> p_emp(Gq);
+  eng.check(eqtl_gpu_cis_perm(c, opt.precision, opt.perm, opt.seed, hash.data(), best.data(), opt.delta_mode,
+                              nullptr, p_emp.data()),
+            "cis_perm");
+  for (int qi = 0; qi < Gq; ++qi) jobs[q[qi]].summary.p_emp = p_emp[qi];
+  if (!opt.disable_beta_approx) warn("beta-approximated permutation p (p_beta) is not provided; p_beta = NA");
+}
+
 void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model, const std::string& scope,
                     ScopeOut& out, double pthr, std::vector<TransJob>& jobs, const PhenoData& ph,
                     const CovData& cov, const std::vector<double>* Q, const std::vector<double>* lambda) {
@@ -716,8 +783,14 @@ void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model,
     eng.check(eqtl_gpu_lm_prep(c, Y.data(), G, cov.X.data(), cov.p, nullptr), "lm_prep");
   }
   eng.check(eqtl_gpu_set_gene_windows(c, gch.data(), gcs.data(), gce.data(), G), "set_gene_windows");
+  // with --perm the scan also keeps every pair below perm_trans_thr: the
+  // candidates of each gene's stage-2 top-K set (topk_consider, scan_trans.hpp:53-70)
+  const bool stage2 = opt.perm > 0;
+  const double pthr_dev = stage2 ? std::max(pthr, opt.perm_trans_thr) : pthr;
+  std::vector<std::vector<std::pair<double, int64_t>>> cand(stage2 ? G : 0);
+  std::vector<int> n_sig(G, 0);
   eng.check(eqtl_gpu_scan_begin(c, model == Model::Lmm ? EQTL_MODEL_LMM : EQTL_MODEL_LM, opt.precision,
-                                EQTL_SCOPE_TRANS, pthr),
+                                EQTL_SCOPE_TRANS, pthr_dev),
             "scan_begin");
   std::vector<uint8_t> buf;
   std::vector<eqtl_hit> hits(1 << 20);
@@ -754,8 +827,12 @@ void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model,
       die("more than " + std::to_string(hits.size()) + " significant pairs in one SNP block; lower --pval-trans");
     eng.check(rc, "scan_block");
     for (int64_t i = 0; i < nh; ++i) {  // SNP-major, gene order (scan_trans_lmm.cpp:185-188)
+      const eqtl_hit& e = hits[i];
+      if (stage2 && e.p < opt.perm_trans_thr) cand[e.gene].emplace_back(e.p, e.snp);
+      if (e.p > pthr) continue;
+      ++n_sig[e.gene];
       AssocHit h;
-      fill(h, hits[i]);
+      fill(h, e);
       write_pair_line(out.pairs, h, model, out.tag);
     }
     const size_t before = snp_cnt;
@@ -773,7 +850,7 @@ void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model,
       job.summary.tss = job.loc.tss;
     }
     job.summary.n_tested = st[k].n_tested;
-    job.summary.n_sig = st[k].n_sig;
+    job.summary.n_sig = n_sig[k];
     job.summary.acat_p = st[k].acat_p;
     if (st[k].best_snp >= 0 && st[k].best_p <= pthr && st[k].best_p <= 1.0) {
       eqtl_hit e;
@@ -789,8 +866,8 @@ void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model,
       fill(h, e);
       write_pair_line(out.top, h, model, scope);
     }
-    if (opt.perm > 0) warn("trans stage-2 permutations are not provided by this engine; p_emp = NA");
   }
+  if (stage2) stage2_perm(opt, eng, bed, model, jobs, cand, st, Y, cov);
 }
 
 // FNV-1a of a gene id, low 32 bits (util.hpp:51-58): permutation seeds.

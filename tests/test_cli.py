@@ -181,3 +181,43 @@ def test_cli_cis_with_permutations(small, tmp_path, model):
             pe = float(row[7])
             assert 1.0 / (B + 1) <= pe <= 1.0
         assert row[8] == "NA"
+
+
+@pytest.mark.gpu
+def test_cli_trans_stage2_permutations(small, tmp_path):
+    """--mode trans --perm B (stage2_perm_topk): LM p_emp of every gene whose best
+    p is below --perm-trans-thr equals the oracle's permutation of that gene over
+    its top-K SNPs; other genes report NA."""
+    ds, prefix, d = small
+    out = str(tmp_path / "s2")
+    w, B, seed, thr, K = 200000, 30, 11, 1e-4, 20
+    r = run(["-b", prefix, "-e", prefix + ".pheno.tsv", "-c", prefix + ".covar.tsv", "-g", prefix + ".gff",
+             "--model", "lm", "--mode", "trans", "-w", str(w), "--pval-trans", "1e-5", "--precision", "fp32",
+             "--perm", str(B), "--seed", str(seed), "--perm-trans-thr", str(thr), "--perm-trans-top", str(K),
+             "-o", out])
+    assert r.returncode == 0, r.stderr
+    rows = ds.bed_rows()
+    dos, missm, keep, af = O.decode_bed(rows, ds.n)
+    gc = ds.gene_chrom.astype(np.int32)
+    cs, ce = np.maximum(1, ds.gene_start - w), ds.gene_end + w
+    ref = O.trans_scan("lm", dos, keep, ds.chrom, ds.pos, ds.Y, ds.X(), gene_chrom=gc, gene_cstart=cs,
+                       gene_cend=ce, pthr=1.0, dense=True, hit_cap=1 << 20)
+    head, region = _read_tsv(f"{out}.lm.trans.region.tsv")
+    checked = 0
+    for row in region:
+        k = int(row[0][1:])
+        pk = ref["p"][:, k]
+        ok = np.isfinite(pk) & (pk < thr)
+        if not ok.any():
+            assert row[7] == "NA"
+            continue
+        idx = np.nonzero(ok)[0]
+        top = idx[np.lexsort((idx, pk[idx]))][:K]
+        top = np.sort(top)
+        o = O.cis_gene("lm", ds.Y[:, k], ds.X(), dos[top], B=B, seed=seed, gene=row[0])
+        Tobs = -np.log10(max(np.nanmin(pk), 1e-300))
+        if np.any(np.abs(o["T_perm"] - Tobs) <= 1e-4 * Tobs):
+            continue
+        assert round(float(row[7]) * (B + 1)) == round(o["p_emp"] * (B + 1)), (row[0], row[7], o["p_emp"])
+        checked += 1
+    assert checked >= 1

@@ -35,6 +35,7 @@ EXPORTS = [
     "eqtl_gpu_set_gene_windows", "eqtl_gpu_scan_begin", "eqtl_gpu_scan_block", "eqtl_gpu_scan_end",
     "eqtl_gpu_set_timing", "eqtl_gpu_get_timing", "eqtl_gpu_synth_bed", "eqtl_gpu_copy_rows",
     "eqtl_gpu_synchronize", "eqtl_gpu_selftest", "eqtl_gpu_upload_bed", "eqtl_gpu_cis_perm",
+    "eqtl_gpu_beta_approx",
 ]
 
 _lib = None
@@ -263,6 +264,16 @@ class Context:
         _check(lib().eqtl_gpu_cis_perm(self._c, ctypes.c_int32(precision), ctypes.c_int32(B), ctypes.c_int32(seed),
                                        _p(gh), _p(bp), ctypes.c_int32(delta_mode), _p(pm), _p(pe)))
         return pm, pe
+
+    def beta_approx(self, perm_min_p, obs_min_p):
+        """(p_beta, shape1, shape2) per gene from G x B permutation minima."""
+        pm = np.ascontiguousarray(perm_min_p, dtype=np.float64)
+        ob = np.ascontiguousarray(obs_min_p, dtype=np.float64)
+        G, B = pm.shape
+        pb, s1, s2 = np.empty(G), np.empty(G), np.empty(G)
+        _check(lib().eqtl_gpu_beta_approx(self._c, _p(pm), _p(ob), ctypes.c_int32(G), ctypes.c_int32(B), _p(pb),
+                                          _p(s1), _p(s2)))
+        return pb, s1, s2
 
     # ---- instrumentation ----
     def set_timing(self, on=True):

@@ -21,6 +21,7 @@
 #include "grm_kernels.hpp"
 #include "reml_batch.hpp"
 #include "perm_rng.hpp"
+#include "beta_approx.hpp"
 
 using namespace eqtl_amd;
 
@@ -1394,6 +1395,29 @@ int eqtl_gpu_cis_perm(eqtl_gpu_ctx* c, int32_t precision, int32_t B, int32_t see
     if (perm_min_p)
       for (int b = 0; b < B; ++b) perm_min_p[static_cast<size_t>(k) * B + b] = minp[static_cast<size_t>(k) * B + b];
   }
+  return EQTL_OK;
+}
+
+int eqtl_gpu_beta_approx(eqtl_gpu_ctx* c, const double* perm_min_p, const double* obs_min_p, int32_t G, int32_t B,
+                         double* p_beta, double* shape1, double* shape2) {
+  if (!c || !perm_min_p || !obs_min_p || G <= 0 || B <= 0 || !p_beta)
+    return fail(EQTL_E_ARG, "bad beta_approx arguments");
+  CHK(set_device(c));
+  DBuf pm, ob, out;
+  CHK(ensure(pm, sizeof(double) * static_cast<size_t>(G) * B));
+  CHK(ensure(ob, sizeof(double) * G));
+  CHK(ensure(out, sizeof(double) * 3 * static_cast<size_t>(G)));
+  HIPC(hipMemcpyAsync(pm.p, perm_min_p, sizeof(double) * static_cast<size_t>(G) * B, hipMemcpyHostToDevice,
+                      c->stream));
+  HIPC(hipMemcpyAsync(ob.p, obs_min_p, sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
+  double* o = out.as<double>();
+  hipLaunchKernelGGL(k_beta_approx, dim3((G + 63) / 64), dim3(64), 0, c->stream, pm.as<double>(), ob.as<double>(), G,
+                     B, o, o + G, o + 2 * G);
+  CHK(check_launch("k_beta_approx"));
+  HIPC(hipMemcpyAsync(p_beta, o, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  if (shape1) HIPC(hipMemcpyAsync(shape1, o + G, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  if (shape2) HIPC(hipMemcpyAsync(shape2, o + 2 * G, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
   return EQTL_OK;
 }
 

@@ -729,12 +729,22 @@ static void stage2_perm(const Options& opt, Engine& eng, PlinkBed& bed, Model mo
   eng.check(eqtl_gpu_set_gene_windows(c, gch.data(), gcs.data(), gce.data(), Gq), "set_gene_windows");
   info(std::string("trans stage 2: ") + std::to_string(Gq) + " genes x " + std::to_string(opt.perm) +
        " permutations over their top-K SNPs");
-  std::vector<double> p_emp(Gq);
+  std::vector<double> p_emp(Gq), perm_min_p(static_cast<size_t>(Gq) * opt.perm);
   eng.check(eqtl_gpu_cis_perm(c, opt.precision, opt.perm, opt.seed, hash.data(), best.data(), opt.delta_mode,
-                              nullptr, p_emp.data()),
+                              perm_min_p.data(), p_emp.data()),
             "cis_perm");
   for (int qi = 0; qi < Gq; ++qi) jobs[q[qi]].summary.p_emp = p_emp[qi];
-  if (!opt.disable_beta_approx) warn("beta-approximated permutation p (p_beta) is not provided; p_beta = NA");
+  if (!opt.disable_beta_approx) {  // scan_perm.cpp:136-138
+    std::vector<double> pb(Gq), s1(Gq), s2(Gq);
+    eng.check(eqtl_gpu_beta_approx(c, perm_min_p.data(), best.data(), Gq, opt.perm, pb.data(), s1.data(), s2.data()),
+              "beta_approx");
+    for (int qi = 0; qi < Gq; ++qi) {
+      GeneSummary& s = jobs[q[qi]].summary;
+      s.p_beta = pb[qi];
+      s.beta_shape1 = s1[qi];
+      s.beta_shape2 = s2[qi];
+    }
+  }
 }
 
 void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model, const std::string& scope,
@@ -996,6 +1006,7 @@ void scan_cis(const Options& opt, Engine& eng, PlinkBed& bed, Model model, Scope
     write_pair_line(out.pairs, h, model, "cis");
   }
   std::vector<double> p_emp(G, std::numeric_limits<double>::quiet_NaN());
+  std::vector<double> p_beta(G, std::numeric_limits<double>::quiet_NaN()), shape1(G, 0.0), shape2(G, 0.0);
   if (opt.perm > 0) {
     std::vector<uint32_t> hash(G);
     std::vector<double> best(G);
@@ -1005,10 +1016,21 @@ void scan_cis(const Options& opt, Engine& eng, PlinkBed& bed, Model model, Scope
     }
     info(std::string("cis ") + (model == Model::Lmm ? "LMM" : "LM") + ": " + std::to_string(opt.perm) +
          " gene-level permutations");
+    std::vector<double> perm_min_p(static_cast<size_t>(G) * opt.perm);
     eng.check(eqtl_gpu_cis_perm(c, opt.precision, opt.perm, opt.seed, hash.data(), best.data(), opt.delta_mode,
-                                nullptr, p_emp.data()),
+                                perm_min_p.data(), p_emp.data()),
               "cis_perm");
-    if (!opt.disable_beta_approx) warn("beta-approximated permutation p (p_beta) is not provided; p_beta = NA");
+    if (!opt.disable_beta_approx) {  // scan_cis.hpp:172-174; genes with no tested SNP keep NA / 0 / 0
+      std::vector<double> pb(G), s1(G), s2(G);
+      eng.check(eqtl_gpu_beta_approx(c, perm_min_p.data(), best.data(), G, opt.perm, pb.data(), s1.data(), s2.data()),
+                "beta_approx");
+      for (int k = 0; k < G; ++k)
+        if (st[k].n_tested > 0) {
+          p_beta[k] = pb[k];
+          shape1[k] = s1[k];
+          shape2[k] = s2[k];
+        }
+    }
   }
   for (int k = 0; k < G; ++k) {
     auto& job = jobs[k];
@@ -1019,6 +1041,9 @@ void scan_cis(const Options& opt, Engine& eng, PlinkBed& bed, Model model, Scope
     job.summary.n_sig = st[k].n_sig;
     job.summary.acat_p = st[k].acat_p;
     job.summary.p_emp = p_emp[k];
+    job.summary.p_beta = p_beta[k];
+    job.summary.beta_shape1 = shape1[k];
+    job.summary.beta_shape2 = shape2[k];
     if (st[k].best_snp >= 0 && st[k].best_p <= pthr && st[k].best_p <= 1.0) {
       eqtl_hit e;
       e.snp = st[k].best_snp;

@@ -166,6 +166,16 @@ int eqtl_gpu_upload_bed(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_
 int eqtl_gpu_cis_perm(eqtl_gpu_ctx* ctx, int32_t precision, int32_t B, int32_t seed, const uint32_t* gene_hash,
                       const double* best_p_obs, int32_t delta_mode, double* perm_min_p, double* p_emp);
 
+/* Beta-approximated permutation p per gene (beta_approx_p, stats_extra.cpp:79-158):
+ * moment fit of Beta(a, b) to the gene's permutation minima in (0, 1) (>= 10
+ * needed), maximum-likelihood refinement with a Nelder-Mead simplex (bounds
+ * a in [0.1, 10], b in [5, 1e6]; moment fit kept when it fails), p_beta =
+ * I_obs(a, b). perm_min_p: G x B host array (row k = gene k, NaN allowed),
+ * obs_min_p: G. Outputs host arrays of G (shape1/shape2 nullable); NaN where
+ * the reference reports NaN. */
+int eqtl_gpu_beta_approx(eqtl_gpu_ctx* ctx, const double* perm_min_p, const double* obs_min_p, int32_t G, int32_t B,
+                         double* p_beta, double* shape1, double* shape2);
+
 /* ---- instrumentation ------------------------------------------------------- */
 /* When enabled, the library brackets each launch of the named kernel family
  * ("wald", "rotate", "decode", "reml") with HIP events on its stream. */

@@ -28,6 +28,16 @@ the cited reference code):
   eigh (Jacobi)      numpy.linalg.eigh; ||QΛQ^T-K||, ||Q^TQ-I||.
   perm_index         libstdc++ std::mt19937 + std::shuffle called directly, and
                      the committed fixture tests/golden/perm_idx.json.
+  lnbeta             scipy.special.betaln over the fit's shape box.
+  beta_approx        moment fit: numpy closed form on cases where the simplex
+                     must fail (start outside the shape bounds); ML refinement:
+                     NLL at the result within the simplex tolerance of the
+                     bounded optimum found by scipy.optimize; p_beta equals
+                     scipy.special.betainc at the returned shapes; filtering
+                     and the <10-minima NaN rule on hand-built vectors. GSL's
+                     nmsimplex2 itself is absent from the image (the reference
+                     links the system libgsl), so the simplex path is pinned
+                     by optimality, not step-by-step.
   cis_gene           composition of the above (LM/LMM perm bases restated from
                      scan_cis.hpp:74-160).
 """
@@ -63,6 +73,7 @@ def lib():
         _lib = ctypes.CDLL(_LIB)
         _lib.orc_p_from_t.restype = ctypes.c_double
         _lib.orc_betai.restype = ctypes.c_double
+        _lib.orc_lnbeta.restype = ctypes.c_double
         _lib.orc_acat.restype = ctypes.c_double
         _lib.orc_p_emp.restype = ctypes.c_double
         _lib.orc_subset_maf_or_nan.restype = ctypes.c_double
@@ -110,6 +121,24 @@ def p_from_t(t, df):
 
 def betai(a, b, x):
     return lib().orc_betai(ctypes.c_double(a), ctypes.c_double(b), ctypes.c_double(x))
+
+
+def lnbeta(a, b):
+    return lib().orc_lnbeta(ctypes.c_double(a), ctypes.c_double(b))
+
+
+def beta_approx(perm_min_p, obs_min_p):
+    """Per gene (row of perm_min_p, G x B) -> (p_beta, shape1, shape2)."""
+    pm = np.atleast_2d(_f64(perm_min_p))
+    obs = np.atleast_1d(_f64(obs_min_p))
+    G, B = pm.shape
+    out = np.empty((G, 3))
+    row = np.empty(3)
+    for k in range(G):
+        r = np.ascontiguousarray(pm[k])
+        lib().orc_beta_approx(_p(r), ctypes.c_int32(B), ctypes.c_double(obs[k]), _p(row))
+        out[k] = row
+    return out[:, 0].copy(), out[:, 1].copy(), out[:, 2].copy()
 
 
 def acat(p):

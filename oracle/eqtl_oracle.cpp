@@ -614,6 +614,139 @@ double orc_p_emp(double Tobs, const double* Tperm, int32_t B) {
   return (1.0 + ge) / (1.0 + B);
 }
 
+// beta_approx_p (stats_extra.cpp:79-158): moment fit, Nelder-Mead ML refinement
+// as GSL's nmsimplex2 does it (start simplex x0, x0 + x0_i/10 e_i; reflect -1,
+// expand -2, 1-D contraction 0.5, shrink to best; RMS corner size < 0.01 or
+// 1000 iterations), bounds by a 1e300 penalty, CDF by the continued fraction.
+// out: p_beta, shape1, shape2 (NaN when fewer than 10 minima in (0,1) or the
+// moment fit fails).
+// ln B(a, b) as gsl_sf_lnbeta gives it (to rounding): for b >= 20 the
+// difference ln Gamma(b) - ln Gamma(a+b) comes from Stirling's series,
+// -a ln b - (a+b-1/2) log1p(a/b) + a + R(b) - R(a+b), with
+// R(z) = 1/(12z) - 1/(360z^3) + 1/(1260z^5) - 1/(1680z^7) + 1/(1188z^9).
+static double stirling_rem(double z) {
+  const double r = 1.0 / z, r2 = r * r;
+  return r * (1.0 / 12 + r2 * (-1.0 / 360 + r2 * (1.0 / 1260 + r2 * (-1.0 / 1680 + r2 * (1.0 / 1188)))));
+}
+
+double orc_lnbeta(double a, double b) {
+  if (b < 20.0) return std::lgamma(a) + std::lgamma(b) - std::lgamma(a + b);
+  return std::lgamma(a) - a * std::log(b) - (a + b - 0.5) * std::log1p(a / b) + a + stirling_rem(b) -
+         stirling_rem(a + b);
+}
+
+static double beta_nll_orc(double a, double b, double s1, double s2, double n) {
+  if (!(a >= 0.1 && a <= 10.0 && b >= 5.0 && b <= 1e6)) return 1e300;
+  return -((a - 1.0) * s1 + (b - 1.0) * s2 - n * orc_lnbeta(a, b));
+}
+
+void orc_beta_approx(const double* pm, int32_t B, double obs, double* out) {
+  const double nan = std::numeric_limits<double>::quiet_NaN();
+  out[0] = out[1] = out[2] = nan;
+  std::vector<double> x;
+  for (int i = 0; i < B; ++i)
+    if (std::isfinite(pm[i]) && pm[i] > 0.0 && pm[i] < 1.0) x.push_back(pm[i]);
+  if (x.size() < 10) return;
+  double mean = 0;
+  for (double v : x) mean += v;
+  mean /= x.size();
+  double var = 0;
+  for (double v : x) var += (v - mean) * (v - mean);
+  var /= static_cast<double>(x.size() - 1);
+  if (var <= 0 || mean <= 0 || mean >= 1) return;
+  const double t = mean * (1.0 - mean) / var - 1.0;
+  if (t <= 0) return;
+  const double mm1 = mean * t, mm2 = (1.0 - mean) * t;
+  double s1 = 0, s2 = 0;
+  for (double v : x) {
+    s1 += std::log(v);
+    s2 += std::log(1.0 - v);
+  }
+  const double nn = static_cast<double>(x.size());
+  auto F = [&](const double* q) { return beta_nll_orc(q[0], q[1], s1, s2, nn); };
+  const int P = 3;
+  double X[3][2] = {{mm1, mm2}, {mm1 + mm1 / 10.0, mm2}, {mm1, mm2 + mm2 / 10.0}}, Y[3];
+  for (int i = 0; i < P; ++i) Y[i] = F(X[i]);
+  double C[2];
+  auto centre = [&] { for (int j = 0; j < 2; ++j) C[j] = (X[0][j] + X[1][j] + X[2][j]) / P; };
+  auto sz2 = [&] {
+    double r = 0;
+    for (int i = 0; i < P; ++i)
+      for (int j = 0; j < 2; ++j) r += (X[i][j] - C[j]) * (X[i][j] - C[j]);
+    return r / P;
+  };
+  centre();
+  double S2 = sz2();
+  auto move = [&](double cf, int k, double* o) {
+    const double al = (1 - cf) * P / (P - 1.0), be = (P * cf - 1.0) / (P - 1.0);
+    for (int j = 0; j < 2; ++j) o[j] = al * C[j] + be * X[k][j];
+    return F(o);
+  };
+  auto put = [&](int k, const double* q, double v) {
+    double d2 = 0, xd = 0;
+    for (int j = 0; j < 2; ++j) {
+      const double dl = q[j] - X[k][j];
+      d2 += dl * dl;
+      xd += (X[k][j] - C[j]) * dl;
+    }
+    S2 += (2.0 / P) * xd + ((P - 1.0) / P) * d2 / P;
+    for (int j = 0; j < 2; ++j) {
+      C[j] += (q[j] - X[k][j]) / P;
+      X[k][j] = q[j];
+    }
+    Y[k] = v;
+  };
+  bool ok = false;
+  for (int it = 0; it < 1000; ++it) {
+    int hi = 0, lo = 0, shi = 1;
+    double dhi = Y[0], dlo = Y[0], dshi = Y[1];
+    for (int i = 1; i < P; ++i) {
+      if (Y[i] < dlo) { dlo = Y[i]; lo = i; }
+      else if (Y[i] > dhi) { dshi = dhi; shi = hi; dhi = Y[i]; hi = i; }
+      else if (Y[i] > dshi) { dshi = Y[i]; shi = i; }
+    }
+    double q1[2], q2[2];
+    const double v1 = move(-1.0, hi, q1);
+    if (v1 < Y[lo]) {
+      const double v2 = move(-2.0, hi, q2);
+      if (v2 < Y[lo]) put(hi, q2, v2);
+      else put(hi, q1, v1);
+    } else if (v1 > Y[shi]) {
+      if (v1 <= Y[hi]) put(hi, q1, v1);
+      const double v2 = move(0.5, hi, q2);
+      if (v2 <= Y[hi]) {
+        put(hi, q2, v2);
+      } else {
+        for (int i = 0; i < P; ++i) {
+          if (i == lo) continue;
+          for (int j = 0; j < 2; ++j) X[i][j] = 0.5 * (X[i][j] + X[lo][j]);
+          Y[i] = F(X[i]);
+        }
+        centre();
+        S2 = sz2();
+      }
+    } else {
+      put(hi, q1, v1);
+    }
+    if ((S2 > 0 ? std::sqrt(S2) : std::sqrt(sz2())) < 0.01) { ok = true; break; }
+  }
+  int best = 0;
+  for (int i = 1; i < P; ++i)
+    if (Y[i] < Y[best]) best = i;
+  double a = X[best][0], b = X[best][1];
+  if (!(ok && std::isfinite(a) && std::isfinite(b) && a >= 0.1 && a <= 10.0 && b >= 5.0 && b <= 1e6)) {
+    a = mm1;
+    b = mm2;
+  }
+  double o = obs;
+  if (o < 1e-300) o = 1e-300;
+  if (o > 1.0 - 1e-16) o = 1.0 - 1e-16;
+  const double pb = betai(a, b, o);
+  out[0] = std::isfinite(pb) ? pb : nan;
+  out[1] = a;
+  out[2] = b;
+}
+
 // bh_adjust: scan_common.cpp:201-217
 void orc_bh(const double* acat, int32_t G, double* q) {
   if (G <= 0) return;

@@ -180,7 +180,24 @@ def test_cli_cis_with_permutations(small, tmp_path, model):
         elif ref["n_tested"] > 0:
             pe = float(row[7])
             assert 1.0 / (B + 1) <= pe <= 1.0
+        if ref["n_tested"] == 0:
+            assert row[8:11] == ["NA", "0", "0"]
+        elif model == "lm":
+            _check_beta_columns(row, ref["perm_min_p"], ref["best_p"])
+        else:
+            assert 0.0 <= float(row[8]) <= 1.0 and float(row[9]) > 0 and float(row[10]) > 0
+
+
+def _check_beta_columns(row, perm_min_p, best_p):
+    """p_beta, beta_shape1, beta_shape2 of a region line against the oracle's
+    beta approximation of the oracle's permutation minima (which match the
+    engine's to ~1e-6 relative, so the fit agrees to a loose 1e-3)."""
+    pb, a, b = O.beta_approx(np.asarray(perm_min_p)[None, :], [best_p])
+    if np.isnan(pb[0]):
         assert row[8] == "NA"
+        return
+    assert float(row[8]) == pytest.approx(pb[0], rel=1e-3, abs=1e-12), (row, pb, a, b)
+    assert float(row[9]) == pytest.approx(a[0], rel=1e-3) and float(row[10]) == pytest.approx(b[0], rel=1e-3)
 
 
 @pytest.mark.gpu
@@ -219,5 +236,6 @@ def test_cli_trans_stage2_permutations(small, tmp_path):
         if np.any(np.abs(o["T_perm"] - Tobs) <= 1e-4 * Tobs):
             continue
         assert round(float(row[7]) * (B + 1)) == round(o["p_emp"] * (B + 1)), (row[0], row[7], o["p_emp"])
+        _check_beta_columns(row, o["perm_min_p"], o["best_p"])
         checked += 1
     assert checked >= 1

@@ -322,3 +322,74 @@ def test_fnv1a_known_values():
         return h & 0xFFFFFFFF
     for s in ("", "a", "g0", "ENSG00000123456"):
         assert O.fnv1a(s) == f(s)
+
+
+# ---------------------------------------------------- beta-approximated p value
+def test_lnbeta_mpmath():
+    import mpmath as mp
+    mp.mp.dps = 40
+    for a in (0.1, 0.37, 1.0, 3.7, 10.0):
+        for b in (5.0, 7.5, 19.99, 20.0, 21.0, 100.0, 1e3, 1e4, 1e5, 1e6):
+            ref = mp.log(mp.beta(a, b))
+            assert abs(O.lnbeta(a, b) - ref) <= 2e-14 * max(1.0, abs(float(ref))), (a, b)
+
+
+def _beta_nll(x):
+    x = x[np.isfinite(x) & (x > 0) & (x < 1)]
+    s1, s2, n = np.log(x).sum(), np.log1p(-x).sum(), x.size
+    return lambda q: -((q[0] - 1) * s1 + (q[1] - 1) * s2 - n * O.lnbeta(q[0], q[1]))
+
+
+@pytest.mark.parametrize("a0,b0,seed", [(1.0, 300.0, 1), (0.8, 1000.0, 2), (1.2, 50.0, 3), (0.5, 2e4, 4),
+                                        (3.0, 8.0, 5), (1.05, 5000.0, 6)])
+def test_beta_approx_ml_optimum(a0, b0, seed):
+    # The simplex (size tolerance 0.01) must land on the bounded ML optimum:
+    # its NLL within 1e-3 of scipy's optimum, shapes within 2%, and p_beta the
+    # Beta CDF at the returned shapes.
+    rng = np.random.default_rng(seed)
+    x = rng.beta(a0, b0, size=1000)
+    obs = [1e-6, 1e-3, float(np.median(x)), 0.5]
+    nll = _beta_nll(x)
+    _, a_o, b_o = O.beta_approx(x[None, :], [0.5])
+    lnll = lambda u: nll(np.exp(u))  # noqa: E731  (log-shape space: b spans decades)
+    best = None
+    for start in ((a0, b0), (1.0, 100.0), (0.5, 1000.0), (a_o[0], b_o[0])):
+        r = optimize.minimize(lnll, np.log(start), method="Nelder-Mead",
+                              options=dict(xatol=1e-12, fatol=1e-13, maxiter=20000, maxfev=40000))
+        if best is None or r.fun < best.fun:
+            best = r
+    best.x = np.exp(best.x)
+    assert 0.1 <= best.x[0] <= 10 and 5 <= best.x[1] <= 1e6
+    for o in obs:
+        pb, a, b = O.beta_approx(x[None, :], [o])
+        a, b, pb = a[0], b[0], pb[0]
+        d = nll([a, b]) - best.fun
+        assert -1e-6 <= d <= 1e-3, (a, b, best.x, d)
+        assert abs(a / best.x[0] - 1) < 0.02 and abs(b / best.x[1] - 1) < 0.02
+        assert pb == pytest.approx(special.betainc(a, b, o), rel=1e-10, abs=1e-300)
+
+
+def test_beta_approx_moment_fallback_and_filtering():
+    rng = np.random.default_rng(11)
+    # mean*(1-mean)/var - 1 puts shape1 below 0.1: the simplex starts outside
+    # the bounds, every corner is penalised, the size test never passes and the
+    # moment estimate (sample variance, ddof=1) is returned.
+    x = rng.beta(0.05, 30.0, size=500)
+    x = x[(x > 0) & (x < 1)]
+    m, v = x.mean(), x.var(ddof=1)
+    t = m * (1 - m) / v - 1
+    pb, a, b = O.beta_approx(x[None, :], [1e-3])
+    assert a[0] < 0.1
+    assert a[0] == pytest.approx(m * t, rel=1e-12) and b[0] == pytest.approx((1 - m) * t, rel=1e-12)
+    assert pb[0] == pytest.approx(special.betainc(m * t, (1 - m) * t, 1e-3), rel=1e-10)
+    # values outside (0, 1) and non-finite ones are dropped before fitting
+    y = rng.beta(1.0, 200.0, size=300)
+    dirty = np.concatenate([y[:100], [0.0, 1.0, -0.5, 2.0, np.nan, np.inf], y[100:]])
+    np.testing.assert_array_equal(np.array(O.beta_approx(dirty[None, :], [1e-4])),
+                                  np.array(O.beta_approx(y[None, :], [1e-4])))
+    # fewer than 10 usable minima -> NaN p and shapes
+    few = np.concatenate([y[:9], [np.nan] * 20, [1.0] * 5])
+    assert np.all(np.isnan(np.array(O.beta_approx(few[None, :], [1e-4]))))
+    # observed p clamped to [1e-300, 1 - 1e-16]
+    pb0, a, b = O.beta_approx(y[None, :], [0.0])
+    assert pb0[0] == pytest.approx(special.betainc(a[0], b[0], 1e-300), rel=1e-10)

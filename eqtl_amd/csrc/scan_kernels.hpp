@@ -258,7 +258,14 @@ constexpr int kRB = 2;          // 16-row blocks per wave
 // s_waitcnt immediate (gfx9 encoding): vmcnt <= 2 * kRB (bits 3:0, 15:14),
 // expcnt and lgkmcnt at their maximum (no wait).
 constexpr int kWaitVm4 = (2 * kRB) | (0x7 << 4) | (0xF << 8);
+constexpr int kWaitVm4Lgkm0 = (2 * kRB) | (0x7 << 4);  // same, and lgkmcnt 0
+constexpr int kWaitLgkm0 = 0xF | (0x7 << 4) | (0x3 << 14);  // lgkmcnt 0 only
 static_assert(2 * kRB < 16, "vmcnt field");
+
+// Workgroup barrier without the memory-model fences of __syncthreads(): the
+// caller has already waited for the counters it needs. The "memory" clobber
+// keeps the compiler from moving memory operations across it.
+__device__ inline void lds_barrier() { asm volatile("s_barrier" ::: "memory"); }
 constexpr int kTileRows = 128;  // SNP rows per workgroup tile
 
 // Per-tile SNP data staged in LDS.
@@ -430,6 +437,135 @@ __host__ __device__ constexpr size_t wald_lds_bytes(int nq) {
 }
 static_assert(wald_lds_bytes(34) <= kLdsCap && wald_a_stages(23) == 3, "Wald LDS layout");
 
+// Direct-to-LDS copy of one k-step of this wave's SNP rows (kRB blocks x 2
+// chunks of 1 KiB).
+__device__ inline void wald_stage_rows(const float* const (&src)[kRB], unsigned char* dst, int lane) {
+#pragma unroll
+  for (int rb = 0; rb < kRB; ++rb)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) lds_direct16(src[rb] + 4 * h, dst + (rb * 2 + h) * 1024);
+  (void)lane;
+}
+
+// One 32-sample k-step of the Wald contraction for one wave: read this
+// k-step's rows (a_rd) and gene tile (b_cur) from LDS, start the copies of the
+// next gene tile (b_nxt) and of a later row stage (a_wr), and run the MFMAs.
+// The LDS pointers are __restrict__: the four regions are disjoint, which lets
+// the compiler see that the copies in flight do not feed this k-step's LDS
+// reads (otherwise it drains vmcnt before them).
+template <int MODEL, int PREC, int NQ, int kAS>
+__device__ __attribute__((always_inline)) inline void wald_kstep(
+    f32x4 (&acc)[kRB][NQ], f32x8 (&an)[kRB], const unsigned char* __restrict__ a_rd, unsigned char* __restrict__ a_wr,
+    const float* const (&a_src)[kRB], bool a_reg_next, const unsigned char* __restrict__ b_cur,
+    unsigned char* __restrict__ b_nxt, const unsigned char* b_src, int wave, int lane) {
+  constexpr int tile_bytes = NQ * 2048;
+  constexpr int chunks = tile_bytes >> 10;
+  f32x8 ac[kRB];
+  if constexpr (kAS > 0) {
+#pragma unroll
+    for (int rb = 0; rb < kRB; ++rb) {
+      const float4 lo = *reinterpret_cast<const float4*>(a_rd + (rb * 2 + 0) * 1024 + lane * 16);
+      const float4 hi = *reinterpret_cast<const float4*>(a_rd + (rb * 2 + 1) * 1024 + lane * 16);
+      ac[rb].v[0] = lo.x; ac[rb].v[1] = lo.y; ac[rb].v[2] = lo.z; ac[rb].v[3] = lo.w;
+      ac[rb].v[4] = hi.x; ac[rb].v[5] = hi.y; ac[rb].v[6] = hi.z; ac[rb].v[7] = hi.w;
+    }
+  } else {
+#pragma unroll
+    for (int rb = 0; rb < kRB; ++rb) ac[rb] = an[rb];
+  }
+  if (b_nxt)
+    for (int c = wave; c < chunks; c += 4) lds_direct16(b_src + c * 1024 + lane * 16, b_nxt + c * 1024);
+  // the row stage goes out after the tile loads, so the caller's wait can
+  // leave it (the youngest 2 kRB operations) in flight
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (kAS > 0) {
+    if (a_wr) wald_stage_rows(a_src, a_wr, lane);
+  } else {
+    if (a_reg_next) {
+#pragma unroll
+      for (int rb = 0; rb < kRB; ++rb) an[rb] = load8(a_src[rb]);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  if constexpr (PREC == EQTL_PREC_FP32) {
+    f32x8 sq[kRB];
+    if constexpr (MODEL == EQTL_MODEL_LMM) {
+#pragma unroll
+      for (int rb = 0; rb < kRB; ++rb)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) sq[rb].v[t] = ac[rb].v[t] * ac[rb].v[t];
+    }
+    // gene fragments are read one quantity ahead of the MFMAs using them
+    f32x8 bn = load8(reinterpret_cast<const float*>(b_cur + lane * 32));
+#pragma unroll
+    for (int b = 0; b < NQ; ++b) {
+      const f32x8 bv = bn;
+      if (b + 1 < NQ) bn = load8(reinterpret_cast<const float*>(b_cur + ((b + 1) * 64 + lane) * 32));
+#pragma unroll
+      for (int rb = 0; rb < kRB; ++rb) {
+        if (MODEL == EQTL_MODEL_LMM && b == 0) acc[rb][b] = mfma_f32x8(sq[rb], bv, acc[rb][b]);
+        else acc[rb][b] = mfma_f32x8(ac[rb], bv, acc[rb][b]);
+      }
+    }
+  } else {
+    // Gene fragments go through registers in groups of kCB column blocks, one
+    // group ahead: a group's 2 kCB LDS reads are in flight behind the previous
+    // group's 3 kCB kRB MFMAs. Inside a group the three split products run
+    // term-major, so consecutive MFMAs never share an accumulator.
+    constexpr int kCB = 3;  // two groups in flight = 12 LDS reads (lgkmcnt holds 15)
+    constexpr int kGroups = (NQ + kCB - 1) / kCB;
+    bf16x8 fh[2][kCB], fl[2][kCB];
+    auto load_group = [&](int g, int slot) {
+#pragma unroll
+      for (int c = 0; c < kCB; ++c) {
+        const int b = g * kCB + c;
+        if (b < NQ) {
+          fh[slot][c] = *reinterpret_cast<const bf16x8*>(b_cur + (b * 64 + lane) * 16);
+          fl[slot][c] = *reinterpret_cast<const bf16x8*>(b_cur + NQ * 1024 + (b * 64 + lane) * 16);
+        }
+      }
+    };
+    load_group(0, 0);
+    bf16x8 ah[kRB], al[kRB], sh[kRB], sl[kRB];
+#pragma unroll
+    for (int rb = 0; rb < kRB; ++rb) {
+      split_bf16(ac[rb], ah[rb], al[rb]);
+      if constexpr (MODEL == EQTL_MODEL_LMM) {
+        f32x8 sv;
+#pragma unroll
+        for (int t = 0; t < 8; ++t) sv.v[t] = ac[rb].v[t] * ac[rb].v[t];
+        split_bf16(sv, sh[rb], sl[rb]);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < kGroups; ++g) {
+      // Wait for group g before the reads of group g + 1 go out: the direct-
+      // to-LDS copies make the compiler's own lgkmcnt waits all-or-nothing, so
+      // a wait placed after those reads would also drain them.
+      __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (g + 1 < kGroups) load_group(g + 1, (g + 1) & 1);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int term = 0; term < 3; ++term)
+#pragma unroll
+        for (int c = 0; c < kCB; ++c) {
+          const int b = g * kCB + c;
+          if (b < NQ) {
+            const bf16x8& y = term == 1 ? fl[g & 1][c] : fh[g & 1][c];
+#pragma unroll
+            for (int rb = 0; rb < kRB; ++rb) {
+              const bool sqr = MODEL == EQTL_MODEL_LMM && b == 0;
+              const bf16x8& x = term == 2 ? (sqr ? sl[rb] : al[rb]) : (sqr ? sh[rb] : ah[rb]);
+              acc[rb][b] = mfma_bf16(x, y, acc[rb][b]);
+            }
+          }
+        }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+}
+
 template <int MODEL, int PREC, int NQ>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_wald(WaldArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
@@ -513,21 +649,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     for (int rb = 0; rb < kRB; ++rb) arow[rb] = a.A + (rowA + rb * 16 + gl) * a.n_pad + kk;
     constexpr int kAS = wald_a_stages(NQ);
     unsigned char* const abuf = smem + 2 * tile_bytes;
-    // stage k-step s of this wave's rows: chunk (rb, h) = 64 lanes x 16 B, so
-    // every later ds_read_b128 of it is conflict-free
-    auto stage_rows = [&](int s) {
-#pragma unroll
-      for (int rb = 0; rb < kRB; ++rb)
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-          lds_direct16(arow[rb] + s * 32 + 4 * h,
-                       abuf + (s % (kAS > 0 ? kAS : 1)) * kAStageBytes + ((wave * kRB + rb) * 2 + h) * 1024);
-    };
+    // this wave's part of a row stage: chunk (rb, h) = 64 lanes x 16 B, so every
+    // later ds_read_b128 of it is conflict-free
+    const int wrow = wave * kRB * 2 * 1024;
     f32x8 an[kRB];
     // prologue: gene tile of k-step 0, rows of the first stages
     for (int c = wave; c < chunks; c += 4) lds_direct16(gsrc + c * 1024 + lane * 16, buf0 + c * 1024);
     if constexpr (kAS > 0) {
-      for (int s = 0; s < kAS - 1 && s < nks; ++s) stage_rows(s);
+      for (int s = 0; s < kAS - 1 && s < nks; ++s) {
+        const float* src[kRB];
+#pragma unroll
+        for (int rb = 0; rb < kRB; ++rb) src[rb] = arow[rb] + s * 32;
+        wald_stage_rows(src, abuf + s * kAStageBytes + wrow, lane);
+      }
     } else {
 #pragma unroll
       for (int rb = 0; rb < kRB; ++rb) an[rb] = load8(arow[rb]);
@@ -535,99 +669,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int s = 0; s < nks; ++s) {
-      const unsigned char* cur = (s & 1) ? buf1 : buf0;
-      f32x8 ac[kRB];
-      if constexpr (kAS > 0) {
-        const unsigned char* as = abuf + (s % kAS) * kAStageBytes;
+      const int sr = kAS > 0 ? s + kAS - 1 : s + 1;  // k-step whose rows are fetched now
+      const float* src[kRB];
 #pragma unroll
-        for (int rb = 0; rb < kRB; ++rb) {
-          const float4 lo = *reinterpret_cast<const float4*>(as + ((wave * kRB + rb) * 2 + 0) * 1024 + lane * 16);
-          const float4 hi = *reinterpret_cast<const float4*>(as + ((wave * kRB + rb) * 2 + 1) * 1024 + lane * 16);
-          ac[rb].v[0] = lo.x; ac[rb].v[1] = lo.y; ac[rb].v[2] = lo.z; ac[rb].v[3] = lo.w;
-          ac[rb].v[4] = hi.x; ac[rb].v[5] = hi.y; ac[rb].v[6] = hi.z; ac[rb].v[7] = hi.w;
-        }
-      } else {
-#pragma unroll
-        for (int rb = 0; rb < kRB; ++rb) ac[rb] = an[rb];
-      }
-      if (s + 1 < nks) {
-        const unsigned char* src = gsrc + static_cast<size_t>(s + 1) * tile_bytes;
-        unsigned char* dst = ((s + 1) & 1) ? buf1 : buf0;
-        for (int c = wave; c < chunks; c += 4) lds_direct16(src + c * 1024 + lane * 16, dst + c * 1024);
-      }
-      // the row stage goes out after the tile loads, so the wait below can
-      // leave it (the youngest 2 kRB operations) in flight
-      __builtin_amdgcn_sched_barrier(0);
-      const bool ahead = kAS == 3 && s + 2 < nks;
-      if constexpr (kAS > 0) {
-        if (s + kAS - 1 < nks) stage_rows(s + kAS - 1);
-      } else {
-        if (s + 1 < nks) {
-#pragma unroll
-          for (int rb = 0; rb < kRB; ++rb) an[rb] = load8(arow[rb] + (s + 1) * 32);
-        }
-      }
-      __builtin_amdgcn_sched_barrier(0);
-      if constexpr (PREC == EQTL_PREC_FP32) {
-        f32x8 sq[kRB];
-        if constexpr (MODEL == EQTL_MODEL_LMM) {
-#pragma unroll
-          for (int rb = 0; rb < kRB; ++rb)
-#pragma unroll
-            for (int t = 0; t < 8; ++t) sq[rb].v[t] = ac[rb].v[t] * ac[rb].v[t];
-        }
-        // gene fragments are read one quantity ahead of the MFMAs using them
-        f32x8 bn = load8(reinterpret_cast<const float*>(cur + lane * 32));
-#pragma unroll
-        for (int b = 0; b < NQ; ++b) {
-          const f32x8 bv = bn;
-          if (b + 1 < NQ) bn = load8(reinterpret_cast<const float*>(cur + ((b + 1) * 64 + lane) * 32));
-#pragma unroll
-          for (int rb = 0; rb < kRB; ++rb) {
-            if (MODEL == EQTL_MODEL_LMM && b == 0) acc[rb][b] = mfma_f32x8(sq[rb], bv, acc[rb][b]);
-            else acc[rb][b] = mfma_f32x8(ac[rb], bv, acc[rb][b]);
-          }
-        }
-      } else {
-        bf16x8 ah[kRB], al[kRB], sh[kRB], sl[kRB];
-#pragma unroll
-        for (int rb = 0; rb < kRB; ++rb) {
-          split_bf16(ac[rb], ah[rb], al[rb]);
-          if constexpr (MODEL == EQTL_MODEL_LMM) {
-            f32x8 sv;
-#pragma unroll
-            for (int t = 0; t < 8; ++t) sv.v[t] = ac[rb].v[t] * ac[rb].v[t];
-            split_bf16(sv, sh[rb], sl[rb]);
-          }
-        }
-        bf16x8 bhn = *reinterpret_cast<const bf16x8*>(cur + lane * 16);
-        bf16x8 bln = *reinterpret_cast<const bf16x8*>(cur + NQ * 1024 + lane * 16);
-#pragma unroll
-        for (int b = 0; b < NQ; ++b) {
-          const bf16x8 bh = bhn, bl = bln;
-          if (b + 1 < NQ) {
-            bhn = *reinterpret_cast<const bf16x8*>(cur + ((b + 1) * 64 + lane) * 16);
-            bln = *reinterpret_cast<const bf16x8*>(cur + NQ * 1024 + ((b + 1) * 64 + lane) * 16);
-          }
-#pragma unroll
-          for (int rb = 0; rb < kRB; ++rb) {
-            if (MODEL == EQTL_MODEL_LMM && b == 0) {
-              acc[rb][b] = mfma_bf16(sh[rb], bh, acc[rb][b]);
-              acc[rb][b] = mfma_bf16(sh[rb], bl, acc[rb][b]);
-              acc[rb][b] = mfma_bf16(sl[rb], bh, acc[rb][b]);
-            } else {
-              acc[rb][b] = mfma_bf16(ah[rb], bh, acc[rb][b]);
-              acc[rb][b] = mfma_bf16(ah[rb], bl, acc[rb][b]);
-              acc[rb][b] = mfma_bf16(al[rb], bh, acc[rb][b]);
-            }
-          }
-        }
-      }
+      for (int rb = 0; rb < kRB; ++rb) src[rb] = arow[rb] + sr * 32;
+      unsigned char* const a_wr =
+          (kAS > 0 && sr < nks) ? abuf + (sr % (kAS > 0 ? kAS : 1)) * kAStageBytes + wrow : nullptr;
+      const unsigned char* const a_rd = kAS > 0 ? abuf + (s % (kAS > 0 ? kAS : 1)) * kAStageBytes + wrow : nullptr;
+      unsigned char* const b_nxt = s + 1 < nks ? (((s + 1) & 1) ? buf1 : buf0) : nullptr;
+      wald_kstep<MODEL, PREC, NQ, kAS>(acc, an, a_rd, a_wr, src, kAS == 0 && sr < nks, (s & 1) ? buf1 : buf0,
+                                       b_nxt, gsrc + static_cast<size_t>(s + 1) * tile_bytes, wave, lane);
       // the next k-step's gene tile and rows must have landed; with three row
-      // stages the rows of k-step s+2 (issued last) stay in flight
-      if (ahead) __builtin_amdgcn_s_waitcnt(kWaitVm4);
+      // stages the rows of k-step s+2 (issued last) stay in flight. The
+      // barrier is a bare s_barrier: the direct-to-LDS loads are complete
+      // once vmcnt says so, and a workgroup release fence would drain vmcnt
+      // to zero, defeating the third stage.
+      if (kAS == 3 && s + 2 < nks) __builtin_amdgcn_s_waitcnt(kWaitVm4Lgkm0);
       else __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
+      lds_barrier();
     }
     // fold the accumulators to (gg, S, num) per test in FP64, parked in LDS
     // (the last k-step's barrier has released the staging buffers)

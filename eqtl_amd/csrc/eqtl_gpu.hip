@@ -18,6 +18,7 @@
 #include "numerics.hpp"
 #include "reml_kernels.hpp"
 #include "scan_kernels.hpp"
+#include "wald_lmm.hpp"
 #include "grm_kernels.hpp"
 #include "reml_batch.hpp"
 #include "perm_rng.hpp"
@@ -288,19 +289,32 @@ int lmm_nq_pad(int nq) {
   return -1;
 }
 
+template <int PREC, int NQ>
+int launch_wald_lmm_one(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
+  const size_t lds = w8_lds_bytes(NQ);
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald_lmm<PREC, NQ>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_wald_lmm<PREC, NQ>), grid, dim3(kW8Threads), lds, c->stream, a);
+  return check_launch("k_wald_lmm");
+}
+
 template <int PREC>
 int launch_wald_lmm(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
   switch (c->nq_pad) {
-    case 4: return launch_wald_one<EQTL_MODEL_LMM, PREC, 4>(c, a, grid);
-    case 8: return launch_wald_one<EQTL_MODEL_LMM, PREC, 8>(c, a, grid);
-    case 12: return launch_wald_one<EQTL_MODEL_LMM, PREC, 12>(c, a, grid);
-    case 16: return launch_wald_one<EQTL_MODEL_LMM, PREC, 16>(c, a, grid);
-    case 18: return launch_wald_one<EQTL_MODEL_LMM, PREC, 18>(c, a, grid);
-    case 20: return launch_wald_one<EQTL_MODEL_LMM, PREC, 20>(c, a, grid);
-    case 23: return launch_wald_one<EQTL_MODEL_LMM, PREC, 23>(c, a, grid);
-    case 26: return launch_wald_one<EQTL_MODEL_LMM, PREC, 26>(c, a, grid);
-    case 30: return launch_wald_one<EQTL_MODEL_LMM, PREC, 30>(c, a, grid);
-    case 34: return launch_wald_one<EQTL_MODEL_LMM, PREC, 34>(c, a, grid);
+    case 4: return launch_wald_lmm_one<PREC, 4>(c, a, grid);
+    case 8: return launch_wald_lmm_one<PREC, 8>(c, a, grid);
+    case 12: return launch_wald_lmm_one<PREC, 12>(c, a, grid);
+    case 16: return launch_wald_lmm_one<PREC, 16>(c, a, grid);
+    case 18: return launch_wald_lmm_one<PREC, 18>(c, a, grid);
+    case 20: return launch_wald_lmm_one<PREC, 20>(c, a, grid);
+    case 23: return launch_wald_lmm_one<PREC, 23>(c, a, grid);
+    case 26: return launch_wald_lmm_one<PREC, 26>(c, a, grid);
+    case 30: return launch_wald_lmm_one<PREC, 30>(c, a, grid);
+    case 34: return launch_wald_lmm_one<PREC, 34>(c, a, grid);
     default: return fail(EQTL_E_ARG, "too many covariates for the Wald kernel");
   }
 }

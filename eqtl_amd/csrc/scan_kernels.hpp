@@ -409,6 +409,13 @@ __device__ inline void lds_direct16(const void* gsrc, unsigned char* lds_wave_ba
   __builtin_amdgcn_global_load_lds(gsrc, (__attribute__((address_space(3))) void*)(lds_wave_base), 16, 0, 0);
 }
 
+// The same copy marked non-temporal (cache policy NT): for operands streamed
+// once per pass, so they do not push re-used operands out of L2.
+constexpr int kCpolNT = 2;
+__device__ inline void lds_direct16_nt(const void* gsrc, unsigned char* lds_wave_base) {
+  __builtin_amdgcn_global_load_lds(gsrc, (__attribute__((address_space(3))) void*)(lds_wave_base), 16, 0, kCpolNT);
+}
+
 // LDS layout of the Wald kernel. Region 0 holds, during the contraction, the
 // two k-step buffers of the gene tile and a ring of SNP-row stages; then the
 // folded (gg, S, num) of the tile's tests during the epilogue; then the

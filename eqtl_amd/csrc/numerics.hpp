@@ -181,17 +181,16 @@ EQ_HD double acat_term_ld(double p) {
 // or p within 1e-6 of 1) the x87 emulation above is used.
 EQ_HD double acat_term_fast(double p) {
 #ifdef __HIP_DEVICE_COMPILE__
+  // one sincospi on the reduced argument, no branches (the lanes of a wave
+  // hold unrelated p): p < 1/4 -> cot(pi p), p <= 3/4 -> tan(pi (1/2 - p)),
+  // p > 3/4 -> -cot(pi (1 - p))
+  const bool lo = p < 0.25, hi = p > 0.75;
+  const double r = lo ? p : (hi ? 1.0 - p : 0.5 - p);
   double s, c;
-  if (p < 0.25) {
-    sincospi(p, &s, &c);
-    return c / s;
-  }
-  if (p <= 0.75) {
-    sincospi(0.5 - p, &s, &c);
-    return s / c;
-  }
-  sincospi(1.0 - p, &s, &c);
-  return -c / s;
+  sincospi(r, &s, &c);
+  const bool cot = lo || hi;
+  const double q = (cot ? c : s) / (cot ? s : c);
+  return hi ? -q : q;
 #else
   const long double pi = 3.14159265358979323846264338327950288L;
   if (p < 0.25) return static_cast<double>(cosl(pi * p) / sinl(pi * p));

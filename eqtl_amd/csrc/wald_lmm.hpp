@@ -12,9 +12,8 @@
 // while one waits on LDS, a barrier or its FP64 epilogue.
 //
 // Per 32-sample k-step the group's gene tile (NQ x 2 KiB) is copied global ->
-// LDS by the four half-0 waves (double-buffered), and each half-1 wave copies
-// its row pair into a ring of row stages; the k-step ends with one workgroup
-// barrier. After the last k-step each wave folds its accumulators to
+// LDS (double-buffered) and each row pair into a ring of row stages, all by
+// the four half-1 waves; the k-step ends with one workgroup barrier. After the last k-step each wave folds its accumulators to
 // partial (gg, sum of squares, num) per test into LDS; the epilogue then runs
 // on all 512 threads (4 tests each), and per-gene states are merged through
 // lane shuffles and LDS into a running state per gene kept in LDS.
@@ -50,10 +49,11 @@ static_assert(w8_lds_bytes(34) <= kLdsCap && w8_a_stages(23) == 3 && w8_a_stages
 // stage) and the gene tile (b_cur) are read from LDS, the copies of later
 // operands are started, and the MFMAs cover blocks [B0, B0 + NQH). SQ: block
 // 0 (the weights d, against the squared rows) is in this half.
-// Copies are split by half: the four half-0 waves copy the next gene tile
-// (b_nxt), the four half-1 waves their row pair of a later stage (a_wr, from
-// a_src[rb]). Each wave's vmcnt then orders only one kind of copy, so waiting
-// for the gene tile never waits on a row copy still coming from HBM.
+// The four half-1 waves issue all copies: their share of the next gene tile
+// (b_nxt, chunks r, r + 4, ...), then their row pair of a later stage (a_wr,
+// from a_src[rb]); the half-0 waves, which also compute the squared rows,
+// issue none. (Measured: this beats splitting the copy kinds between the
+// halves or spreading them over the k-step.)
 // The LDS pointers are __restrict__ (disjoint regions), so the compiler does
 // not drain the copies in flight before this k-step's LDS reads.
 template <int PREC, int NQ, int NQH, int B0, bool SQ, int kAS>
@@ -76,18 +76,11 @@ __device__ __attribute__((always_inline)) inline void w8_kstep(
 #pragma unroll
     for (int rb = 0; rb < kRB; ++rb) ac[rb] = an[rb];
   }
-  // half 0: copies of the next gene tile, spread over the MFMA groups below
-  // (chunks r, r + 4, ...), so no wave stalls issuing a dozen copies at once
-  int b_chunk = (B0 == 0 && b_nxt) ? r : chunks;
-  auto issue_b = [&](int n) {
-#pragma unroll
-    for (int i = 0; i < n; ++i) {
-      if (b_chunk < chunks) lds_direct16(b_src + b_chunk * 1024 + lane * 16, b_nxt + b_chunk * 1024);
-      b_chunk += 4;
-    }
-  };
-  if constexpr (B0 == 0) {
-  } else if constexpr (kAS > 0) {
+  if constexpr (B0 != 0) {
+    if (b_nxt)
+      for (int c = r; c < chunks; c += 4) lds_direct16(b_src + c * 1024 + lane * 16, b_nxt + c * 1024);
+  }
+  if constexpr (B0 != 0 && kAS > 0) {
     if (a_wr) {
 #pragma unroll
       for (int rb = 0; rb < kRB; ++rb) {
@@ -111,11 +104,9 @@ __device__ __attribute__((always_inline)) inline void w8_kstep(
 #pragma unroll
         for (int t = 0; t < 8; ++t) sq[rb].v[t] = ac[rb].v[t] * ac[rb].v[t];
     }
-    constexpr int kPerBlock = ((chunks + 3) / 4 + NQH - 1) / NQH;
     f32x8 bn = load8(reinterpret_cast<const float*>(b_cur + (B0 * 64 + lane) * 32));
 #pragma unroll
     for (int b = 0; b < NQH; ++b) {
-      if constexpr (B0 == 0) issue_b(kPerBlock);
       const f32x8 bv = bn;
       if (b + 1 < NQH) bn = load8(reinterpret_cast<const float*>(b_cur + ((B0 + b + 1) * 64 + lane) * 32));
 #pragma unroll
@@ -130,7 +121,6 @@ __device__ __attribute__((always_inline)) inline void w8_kstep(
     // copies make the compiler's own lgkm waits all-or-nothing)
     constexpr int kCB = 3;
     constexpr int kGroups = (NQH + kCB - 1) / kCB;
-    constexpr int kPerGroup = ((chunks + 3) / 4 + kGroups - 1) / kGroups;
     bf16x8 fh[2][kCB], fl[2][kCB];
     auto load_group = [&](int g, int slot) {
 #pragma unroll
@@ -159,7 +149,6 @@ __device__ __attribute__((always_inline)) inline void w8_kstep(
       __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
       __builtin_amdgcn_sched_barrier(0);
       if (g + 1 < kGroups) load_group(g + 1, (g + 1) & 1);
-      if constexpr (B0 == 0) issue_b(kPerGroup);
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int term = 0; term < 3; ++term)
@@ -211,10 +200,11 @@ __device__ __attribute__((always_inline)) inline void w8_tile(const WaldArgs& a,
 #pragma unroll
     for (int b = 0; b < NQH; ++b) acc[rb][b] = f32x4{0.f, 0.f, 0.f, 0.f};
   f32x8 an[kRB];
-  // prologue: gene tile of k-step 0 (half 0) and the first row stages (half 1)
-  if constexpr (B0 == 0) {
+  // prologue (half 1): gene tile of k-step 0 and the first row stages
+  if constexpr (B0 != 0) {
     for (int ch = r; ch < chunks; ch += 4) lds_direct16(gsrc + ch * 1024 + lane * 16, buf0 + ch * 1024);
-  } else if constexpr (kAS > 0) {
+  }
+  if constexpr (B0 != 0 && kAS > 0) {
     for (int s = 0; s < kAS - 1 && s < nks; ++s) {
       unsigned char* dst = abuf + s * kAStageBytes + pair_off;
 #pragma unroll
@@ -291,42 +281,121 @@ __device__ inline GeneAcc shfl_xor_acc(const GeneAcc& x, int m) {
   return y;
 }
 
-template <int PREC, int NQ>
-__global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_wald_lmm(WaldArgs a) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  constexpr int NQ0 = (NQ + 1) / 2, NQ1 = NQ - NQ0;
-  // XCD-aware mapping, as in k_wald: the n_splits consecutive workgroups of
-  // one XCD share a gene group
+// Work of one workgroup: gene group and SNP tiles (XCD-aware in trans scope:
+// the n_splits consecutive workgroups of one XCD share a gene group, so its
+// column blocks are served from that XCD's L2; cis scope: one work item).
+struct W8Work {
   int group, split, t_lo, t_hi, t_step;
+};
+__device__ inline W8Work w8_work(const WaldArgs& a) {
+  W8Work w;
   if (a.work) {
-    group = a.work[3 * blockIdx.x];
-    t_lo = a.work[3 * blockIdx.x + 1];
-    t_hi = a.work[3 * blockIdx.x + 2];
-    split = blockIdx.x;
-    t_step = 1;
+    w.group = a.work[3 * blockIdx.x];
+    w.t_lo = a.work[3 * blockIdx.x + 1];
+    w.t_hi = a.work[3 * blockIdx.x + 2];
+    w.split = blockIdx.x;
+    w.t_step = 1;
   } else {
     const int wg = blockIdx.x;
     const int xcd = wg & 7;
     const int jx = wg >> 3;
-    group = (jx / a.n_splits) * 8 + xcd;
-    split = jx % a.n_splits;
-    t_lo = split;
-    t_hi = (a.n_tiles + 1) >> 1;
-    t_step = a.n_splits;
+    w.group = (jx / a.n_splits) * 8 + xcd;
+    w.split = jx % a.n_splits;
+    w.t_lo = w.split;
+    w.t_hi = (a.n_tiles + 1) >> 1;
+    w.t_step = a.n_splits;
   }
-  if (group >= a.n_groups) return;
+  return w;
+}
+
+// The epilogue's gene: thread -> gene (tid & 15) of the group, tests of rows
+// (tid >> 4) + 32 i, with the gene's scalar and window.
+struct W8Gene {
+  int gl, gene;
+  bool ok, win_known;
+  double gscal;
+  int32_t gch;
+  int64_t gcs, gce;
+};
+__device__ inline W8Gene w8_gene(const WaldArgs& a, int group) {
+  W8Gene g;
+  g.gl = threadIdx.x & 15;
+  g.gene = group * 16 + g.gl;
+  g.ok = g.gene < a.G && a.gene_valid[g.gene];
+  g.gscal = g.ok ? a.gene_scal[g.gene] : 0.0;
+  const bool excl = g.ok && a.gene_chrom && a.gene_chrom[g.gene] >= 0;
+  g.gch = excl ? a.gene_chrom[g.gene] : -1;
+  g.gcs = excl ? a.gene_cs[g.gene] : 0;
+  g.gce = excl ? a.gene_ce[g.gene] : 0;
+  g.win_known = excl && a.snp_chrom;
+  return g;
+}
+
+// FP64 epilogue of one tile (2048 tests, 4 per thread), after the fold is in
+// LDS: fold(t, gg, S, num) reads test t = row * 16 + gene. The per-thread gene
+// states are merged over the four lanes holding the same gene, then over the
+// eight waves (wpart) into the running state in gstate.
+template <class Fold>
+__device__ inline void w8_epilogue(const WaldArgs& a, const W8Gene& gi, const RowInfo& ri, const TailView& tv,
+                                   GeneAcc* gstate, GeneAcc* wpart, int64_t row0, Fold fold) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  GeneAcc st;
+  acc_init(st);
+  dd asum{0.0, 0.0};
+#pragma unroll 1
+  for (int i = 0; i < kW8Tests / kW8Threads; ++i) {
+    const int jl = (threadIdx.x >> 4) + (kW8Threads / 16) * i;
+    const int t = jl * 16 + gi.gl;
+    const int64_t j = row0 + jl;
+    const bool inwin = gi.win_known && ri.chrom[jl] == gi.gch && ri.pos[jl] >= gi.gcs && ri.pos[jl] <= gi.gce;
+    const bool ok = gi.ok && j < a.m && ri.keep[jl] && (a.cis ? inwin : !inwin);
+    double gg, S, num;
+    fold(t, gg, S, num);
+    const bool cand = refine_wanted<EQTL_MODEL_LMM>(a, ok, gg, S, num, gi.gscal, 0.0);
+    const double n64 = wave_refine(a, cand, j, gi.gene);
+    if (cand) num = n64;
+    if (ok) wald_test<EQTL_MODEL_LMM>(a, ri, jl, j, gi.gene, gi.gscal, gg, S, num, tv, st, asum);
+  }
+  st.acat_hi = asum.hi;
+  st.acat_lo = asum.lo;
+  // lanes l, l ^ 16, l ^ 32, l ^ 48 hold the same gene
+  acc_merge(st, shfl_xor_acc(st, 16));
+  acc_merge(st, shfl_xor_acc(st, 32));
+  if (lane < 16) wpart[wave * 16 + lane] = st;
+  __syncthreads();
+  if (threadIdx.x < 16) {
+    GeneAcc g = gstate[threadIdx.x];
+    for (int w = 0; w < kW8Waves; ++w) acc_merge(g, wpart[w * 16 + threadIdx.x]);
+    gstate[threadIdx.x] = g;
+  }
+}
+
+// Loads one tile's SNP data (keep, af, annotation) into LDS.
+__device__ inline void w8_row_info(const WaldArgs& a, RowInfo& ri, int64_t row0) {
+  if (threadIdx.x < kTileRows) {
+    const int t = threadIdx.x;
+    const int64_t j = row0 + t;
+    const bool in = j < a.m;
+    ri.keep[t] = in ? a.keep[j] : 0;
+    ri.af[t] = in ? a.af[j] : 0.0;
+    ri.gtg[t] = 0.0;
+    const bool ann = in && a.snp_chrom;
+    ri.chrom[t] = ann ? a.snp_chrom[a.snp0 + j] : -1;
+    ri.pos[t] = ann ? a.snp_pos[a.snp0 + j] : 0;
+  }
+}
+
+template <int PREC, int NQ>
+__global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_wald_lmm(WaldArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  constexpr int NQ0 = (NQ + 1) / 2, NQ1 = NQ - NQ0;
+  const W8Work wk = w8_work(a);
+  if (wk.group >= a.n_groups) return;
+  const int group = wk.group;
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = wave >> 2, r = wave & 3;
-  // epilogue mapping: thread -> gene (tid & 15), rows (tid >> 4) + 32 i
-  const int gl = threadIdx.x & 15;
-  const int gene = group * 16 + gl;
-  const bool gene_ok = gene < a.G && a.gene_valid[gene];
-  const double gscal = gene_ok ? a.gene_scal[gene] : 0.0;
-  const bool excl = gene_ok && a.gene_chrom && a.gene_chrom[gene] >= 0;
-  const int32_t gch = excl ? a.gene_chrom[gene] : -1;
-  const int64_t gcs = excl ? a.gene_cs[gene] : 0, gce = excl ? a.gene_ce[gene] : 0;
-  const bool win_known = excl && a.snp_chrom;
+  const W8Gene gi = w8_gene(a, group);
 
   const size_t r0b = w8_region0_bytes(NQ);
   RowInfo& ri = *reinterpret_cast<RowInfo*>(smem + r0b);
@@ -341,56 +410,22 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
       PREC == EQTL_PREC_FP32 ? reinterpret_cast<const unsigned char*>(a.gc.f32 + a.gc.tile_f32(group, 0))
                              : reinterpret_cast<const unsigned char*>(a.gc.bf + a.gc.tile_bf(group, 0));
 
-  for (int tile = t_lo; tile < t_hi; tile += t_step) {
+  for (int tile = wk.t_lo; tile < wk.t_hi; tile += wk.t_step) {
     const int64_t row0 = static_cast<int64_t>(tile) * kTileRows;
     __syncthreads();  // the previous tile's epilogue is done with ri, fold and the gene partials
-    if (threadIdx.x < kTileRows) {
-      const int t = threadIdx.x;
-      const int64_t j = row0 + t;
-      const bool in = j < a.m;
-      ri.keep[t] = in ? a.keep[j] : 0;
-      ri.af[t] = in ? a.af[j] : 0.0;
-      ri.gtg[t] = 0.0;
-      const bool ann = in && a.snp_chrom;
-      ri.chrom[t] = ann ? a.snp_chrom[a.snp0 + j] : -1;
-      ri.pos[t] = ann ? a.snp_pos[a.snp0 + j] : 0;
-    }
+    w8_row_info(a, ri, row0);
     if (c == 0) w8_tile<PREC, NQ, NQ0, 0, true>(a, smem, gsrc, row0, c, r, lane);
     else w8_tile<PREC, NQ, NQ1, NQ0, false>(a, smem, gsrc, row0, c, r, lane);
     __syncthreads();  // fold complete
-    GeneAcc st;
-    acc_init(st);
-    dd asum{0.0, 0.0};
-#pragma unroll 1
-    for (int i = 0; i < kW8Tests / kW8Threads; ++i) {
-      const int jl = (threadIdx.x >> 4) + (kW8Threads / 16) * i;
-      const int t = jl * 16 + gl;
-      const int64_t j = row0 + jl;
-      const bool inwin = win_known && ri.chrom[jl] == gch && ri.pos[jl] >= gcs && ri.pos[jl] <= gce;
-      const bool ok = gene_ok && j < a.m && ri.keep[jl] && (a.cis ? inwin : !inwin);
-      const double gg = fold[0 * kW8Tests + t] + fold[3 * kW8Tests + t];
-      const double S = gg - (fold[1 * kW8Tests + t] + fold[4 * kW8Tests + t]);
-      double num = fold[2 * kW8Tests + t] + fold[5 * kW8Tests + t];
-      const bool cand = refine_wanted<EQTL_MODEL_LMM>(a, ok, gg, S, num, gscal, 0.0);
-      const double n64 = wave_refine(a, cand, j, gene);
-      if (cand) num = n64;
-      if (ok) wald_test<EQTL_MODEL_LMM>(a, ri, jl, j, gene, gscal, gg, S, num, tv, st, asum);
-    }
-    st.acat_hi = asum.hi;
-    st.acat_lo = asum.lo;
-    // lanes l, l ^ 16, l ^ 32, l ^ 48 hold the same gene
-    acc_merge(st, shfl_xor_acc(st, 16));
-    acc_merge(st, shfl_xor_acc(st, 32));
-    if (lane < 16) wpart[wave * 16 + lane] = st;
-    __syncthreads();
-    if (threadIdx.x < 16) {
-      GeneAcc g = gstate[threadIdx.x];
-      for (int w = 0; w < kW8Waves; ++w) acc_merge(g, wpart[w * 16 + threadIdx.x]);
-      gstate[threadIdx.x] = g;
-    }
+    w8_epilogue(a, gi, ri, tv, gstate, wpart, row0, [&](int t, double& gg, double& S, double& num) {
+      gg = fold[0 * kW8Tests + t] + fold[3 * kW8Tests + t];
+      S = gg - (fold[1 * kW8Tests + t] + fold[4 * kW8Tests + t]);
+      num = fold[2 * kW8Tests + t] + fold[5 * kW8Tests + t];
+    });
   }
   if (threadIdx.x < 16) {
-    const size_t slot = a.work ? static_cast<size_t>(split) * 16 : static_cast<size_t>(split) * a.G_pad + group * 16;
+    const size_t slot =
+        a.work ? static_cast<size_t>(wk.split) * 16 : static_cast<size_t>(wk.split) * a.G_pad + group * 16;
     a.partial[slot + threadIdx.x] = gstate[threadIdx.x];
   }
 }

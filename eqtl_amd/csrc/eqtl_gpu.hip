@@ -266,17 +266,17 @@ const uint8_t* stage_rows(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, 
   return c->rows.as<uint8_t>();
 }
 
-template <int MODEL, int PREC, int NQ>
-int launch_wald_one(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
-  const size_t lds = wald_lds_bytes(NQ);
+template <int PREC>
+int launch_wald_lm(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
+  const size_t lds = lm_lds_bytes();
   static bool attr_set = false;
   if (!attr_set) {
-    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald<MODEL, PREC, NQ>),
+    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald_lm<PREC>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_wald<MODEL, PREC, NQ>), grid, dim3(256), lds, c->stream, a);
-  return check_launch("k_wald");
+  hipLaunchKernelGGL((k_wald_lm<PREC>), grid, dim3(256), lds, c->stream, a);
+  return check_launch("k_wald_lm");
 }
 
 // Column-block counts the LMM Wald kernel is compiled for (nq = p + 2 is
@@ -324,8 +324,8 @@ int launch_wald(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
     if (c->scan_prec == EQTL_PREC_FP32) return launch_wald_lmm<EQTL_PREC_FP32>(c, a, grid);
     return launch_wald_lmm<EQTL_PREC_BF16X3>(c, a, grid);
   }
-  if (c->scan_prec == EQTL_PREC_FP32) return launch_wald_one<EQTL_MODEL_LM, EQTL_PREC_FP32, 1>(c, a, grid);
-  return launch_wald_one<EQTL_MODEL_LM, EQTL_PREC_BF16X3, 1>(c, a, grid);
+  if (c->scan_prec == EQTL_PREC_FP32) return launch_wald_lm<EQTL_PREC_FP32>(c, a, grid);
+  return launch_wald_lm<EQTL_PREC_BF16X3>(c, a, grid);
 }
 
 }  // namespace

@@ -196,20 +196,17 @@ __global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ A, con
 }
 
 // ---------------------------------------------------------------------------
-// K7/K8: fused Wald. One workgroup = one group of 16 genes x a strided set of
-// 128-SNP tiles; wave w owns rows 32w..32w+31 of a tile as two 16-row blocks,
-// so every B fragment read from LDS feeds two MFMA chains. Column block b of
-// the group holds quantity b for genes 0..15 (lane & 15), so each lane ends
-// with all NQ quantities of one (SNP row, gene) pair per output register r:
-//   LMM: b=0 d (against A^2 → gg), b=1..p whitened orthonormal covariates
-//        (→ W^T h), b=p+1 whitened null residual (→ r^T h)
-//   LM : b=0 residualised phenotype y_s (→ g^T y_s)
-// The group's k-step tile (32 samples x 16 genes x NQ blocks) is staged in
-// LDS, double-buffered and filled by direct global→LDS 16-byte loads, and
-// shared by the 4 waves. Right after the contraction the accumulators are
-// folded to (gg, S = gg - |W^T h|^2, num) per test, so the FP64 epilogue runs
-// with the accumulator registers free; the tile's SNP data and the t tail's
-// head sit in LDS.
+// Fused Wald scans: one workgroup = one group of 16 genes x a strided set of
+// 128-SNP tiles. The gene-side operand is stored in column blocks: block b of a
+// group holds quantity b for genes 0..15 (lane & 15), so after the MFMA
+// contraction each lane holds all quantities of one (SNP row, gene) pair per
+// output register:
+//   LMM: b=0 d (against A^2 -> gg), b=1..p whitened orthonormal covariates
+//        (-> W^T h), b=p+1 whitened null residual (-> r^T h); k_wald_lmm
+//   LM : b=0 residualised phenotype y_s (-> g^T y_s); k_wald_lm below
+// Both stage the group's k-step gene tile and a ring of SNP-row stages in LDS
+// with direct global->LDS 16-byte loads, and run the FP64 epilogue (t tail,
+// p, ACAT, hits) with the accumulators parked in LDS.
 // ---------------------------------------------------------------------------
 struct PolishItem {
   int32_t j;     // block-local SNP row
@@ -409,40 +406,27 @@ __device__ inline void lds_direct16(const void* gsrc, unsigned char* lds_wave_ba
   __builtin_amdgcn_global_load_lds(gsrc, (__attribute__((address_space(3))) void*)(lds_wave_base), 16, 0, 0);
 }
 
-// The same copy marked non-temporal (cache policy NT): for operands streamed
-// once per pass, so they do not push re-used operands out of L2.
-constexpr int kCpolNT = 2;
-__device__ inline void lds_direct16_nt(const void* gsrc, unsigned char* lds_wave_base) {
-  __builtin_amdgcn_global_load_lds(gsrc, (__attribute__((address_space(3))) void*)(lds_wave_base), 16, 0, kCpolNT);
-}
-
-// LDS layout of the Wald kernel. Region 0 holds, during the contraction, the
-// two k-step buffers of the gene tile and a ring of SNP-row stages; then the
-// folded (gg, S, num) of the tile's tests during the epilogue; then the
+// LDS layout of the LM Wald kernel. Region 0 holds, during the contraction,
+// the two k-step buffers of the gene tile (2 KiB each) and a ring of three
+// SNP-row stages; then gty of the tile's tests during the epilogue; then the
 // per-gene partials of the final reduction. RowInfo and the t-tail head
-// follow it. The SNP rows (read from HBM once per gene group) are staged
-// three k-steps deep when that fits in 160 KiB, two otherwise; for the widest
-// gene tiles they go through registers one k-step ahead.
-constexpr int kFoldDoubles = kRB * 4 * 3 * 256;
+// follow it.
+constexpr int kFoldDoubles = kRB * 4 * 256;  // one gty per test of the tile
 constexpr int kAStageBytes = kTileRows * 32 * 4;  // 128 rows x 32 samples fp32
+constexpr int kLmStages = 3;
 constexpr long kLdsCap = 160 * 1024;
 __host__ __device__ constexpr long wald_tail_bytes() {
   return static_cast<long>(sizeof(RowInfo)) + static_cast<long>(sizeof(double)) * kTailHead;
 }
-__host__ __device__ constexpr int wald_a_stages(int nq) {
-  return (2L * nq * 2048 + 3L * kAStageBytes + wald_tail_bytes() <= kLdsCap)   ? 3
-         : (2L * nq * 2048 + 2L * kAStageBytes + wald_tail_bytes() <= kLdsCap) ? 2
-                                                                                : 0;
-}
-__host__ __device__ constexpr size_t wald_region0_bytes(int nq) {
-  const size_t b = 2 * static_cast<size_t>(nq) * 2048 + static_cast<size_t>(wald_a_stages(nq)) * kAStageBytes;
+__host__ __device__ constexpr size_t lm_region0_bytes() {
+  const size_t b = 2 * 2048 + static_cast<size_t>(kLmStages) * kAStageBytes;
   const size_t f = sizeof(double) * kFoldDoubles, r = 256 * sizeof(GeneAcc);
   return b > f ? (b > r ? b : r) : (f > r ? f : r);
 }
-__host__ __device__ constexpr size_t wald_lds_bytes(int nq) {
-  return wald_region0_bytes(nq) + static_cast<size_t>(wald_tail_bytes());
+__host__ __device__ constexpr size_t lm_lds_bytes() {
+  return lm_region0_bytes() + static_cast<size_t>(wald_tail_bytes());
 }
-static_assert(wald_lds_bytes(34) <= kLdsCap && wald_a_stages(23) == 3, "Wald LDS layout");
+static_assert(lm_lds_bytes() <= kLdsCap, "LM Wald LDS layout");
 
 // Direct-to-LDS copy of one k-step of this wave's SNP rows (kRB blocks x 2
 // chunks of 1 KiB).
@@ -454,127 +438,58 @@ __device__ inline void wald_stage_rows(const float* const (&src)[kRB], unsigned 
   (void)lane;
 }
 
-// One 32-sample k-step of the Wald contraction for one wave: read this
-// k-step's rows (a_rd) and gene tile (b_cur) from LDS, start the copies of the
-// next gene tile (b_nxt) and of a later row stage (a_wr), and run the MFMAs.
-// The LDS pointers are __restrict__: the four regions are disjoint, which lets
-// the compiler see that the copies in flight do not feed this k-step's LDS
-// reads (otherwise it drains vmcnt before them).
-template <int MODEL, int PREC, int NQ, int kAS>
-__device__ __attribute__((always_inline)) inline void wald_kstep(
-    f32x4 (&acc)[kRB][NQ], f32x8 (&an)[kRB], const unsigned char* __restrict__ a_rd, unsigned char* __restrict__ a_wr,
-    const float* const (&a_src)[kRB], bool a_reg_next, const unsigned char* __restrict__ b_cur,
-    unsigned char* __restrict__ b_nxt, const unsigned char* b_src, int wave, int lane) {
-  constexpr int tile_bytes = NQ * 2048;
-  constexpr int chunks = tile_bytes >> 10;
+// One 32-sample k-step of the LM contraction for one wave: read this
+// k-step's rows (a_rd) and gene tile (b_cur: one block, y_s of the 16 genes)
+// from LDS, start the copies of the next gene tile (b_nxt) and of a later row
+// stage (a_wr), and run the MFMAs. The LDS pointers are __restrict__: the four
+// regions are disjoint, which lets the compiler see that the copies in flight
+// do not feed this k-step's LDS reads (otherwise it drains vmcnt before them).
+template <int PREC>
+__device__ __attribute__((always_inline)) inline void lm_kstep(
+    f32x4 (&acc)[kRB], const unsigned char* __restrict__ a_rd, unsigned char* __restrict__ a_wr,
+    const float* const (&a_src)[kRB], const unsigned char* __restrict__ b_cur, unsigned char* __restrict__ b_nxt,
+    const unsigned char* b_src, int wave, int lane) {
   f32x8 ac[kRB];
-  if constexpr (kAS > 0) {
 #pragma unroll
-    for (int rb = 0; rb < kRB; ++rb) {
-      const float4 lo = *reinterpret_cast<const float4*>(a_rd + (rb * 2 + 0) * 1024 + lane * 16);
-      const float4 hi = *reinterpret_cast<const float4*>(a_rd + (rb * 2 + 1) * 1024 + lane * 16);
-      ac[rb].v[0] = lo.x; ac[rb].v[1] = lo.y; ac[rb].v[2] = lo.z; ac[rb].v[3] = lo.w;
-      ac[rb].v[4] = hi.x; ac[rb].v[5] = hi.y; ac[rb].v[6] = hi.z; ac[rb].v[7] = hi.w;
-    }
-  } else {
-#pragma unroll
-    for (int rb = 0; rb < kRB; ++rb) ac[rb] = an[rb];
+  for (int rb = 0; rb < kRB; ++rb) {
+    const float4 lo = *reinterpret_cast<const float4*>(a_rd + (rb * 2 + 0) * 1024 + lane * 16);
+    const float4 hi = *reinterpret_cast<const float4*>(a_rd + (rb * 2 + 1) * 1024 + lane * 16);
+    ac[rb].v[0] = lo.x; ac[rb].v[1] = lo.y; ac[rb].v[2] = lo.z; ac[rb].v[3] = lo.w;
+    ac[rb].v[4] = hi.x; ac[rb].v[5] = hi.y; ac[rb].v[6] = hi.z; ac[rb].v[7] = hi.w;
   }
-  if (b_nxt)
-    for (int c = wave; c < chunks; c += 4) lds_direct16(b_src + c * 1024 + lane * 16, b_nxt + c * 1024);
-  // the row stage goes out after the tile loads, so the caller's wait can
+  // the gene tile is two 1 KiB chunks (either precision): waves 0 and 1
+  if (b_nxt && wave < 2) lds_direct16(b_src + wave * 1024 + lane * 16, b_nxt + wave * 1024);
+  // the row stage goes out after the tile load, so the caller's wait can
   // leave it (the youngest 2 kRB operations) in flight
   __builtin_amdgcn_sched_barrier(0);
-  if constexpr (kAS > 0) {
-    if (a_wr) wald_stage_rows(a_src, a_wr, lane);
-  } else {
-    if (a_reg_next) {
-#pragma unroll
-      for (int rb = 0; rb < kRB; ++rb) an[rb] = load8(a_src[rb]);
-    }
-  }
+  if (a_wr) wald_stage_rows(a_src, a_wr, lane);
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (PREC == EQTL_PREC_FP32) {
-    f32x8 sq[kRB];
-    if constexpr (MODEL == EQTL_MODEL_LMM) {
+    const f32x8 bv = load8(reinterpret_cast<const float*>(b_cur + lane * 32));
 #pragma unroll
-      for (int rb = 0; rb < kRB; ++rb)
-#pragma unroll
-        for (int t = 0; t < 8; ++t) sq[rb].v[t] = ac[rb].v[t] * ac[rb].v[t];
-    }
-    // gene fragments are read one quantity ahead of the MFMAs using them
-    f32x8 bn = load8(reinterpret_cast<const float*>(b_cur + lane * 32));
-#pragma unroll
-    for (int b = 0; b < NQ; ++b) {
-      const f32x8 bv = bn;
-      if (b + 1 < NQ) bn = load8(reinterpret_cast<const float*>(b_cur + ((b + 1) * 64 + lane) * 32));
-#pragma unroll
-      for (int rb = 0; rb < kRB; ++rb) {
-        if (MODEL == EQTL_MODEL_LMM && b == 0) acc[rb][b] = mfma_f32x8(sq[rb], bv, acc[rb][b]);
-        else acc[rb][b] = mfma_f32x8(ac[rb], bv, acc[rb][b]);
-      }
-    }
+    for (int rb = 0; rb < kRB; ++rb) acc[rb] = mfma_f32x8(ac[rb], bv, acc[rb]);
   } else {
-    // Gene fragments go through registers in groups of kCB column blocks, one
-    // group ahead: a group's 2 kCB LDS reads are in flight behind the previous
-    // group's 3 kCB kRB MFMAs. Inside a group the three split products run
-    // term-major, so consecutive MFMAs never share an accumulator.
-    constexpr int kCB = 3;  // two groups in flight = 12 LDS reads (lgkmcnt holds 15)
-    constexpr int kGroups = (NQ + kCB - 1) / kCB;
-    bf16x8 fh[2][kCB], fl[2][kCB];
-    auto load_group = [&](int g, int slot) {
+    const bf16x8 bh = *reinterpret_cast<const bf16x8*>(b_cur + lane * 16);
+    const bf16x8 bl = *reinterpret_cast<const bf16x8*>(b_cur + 1024 + lane * 16);
+    bf16x8 ah[kRB], al[kRB];
 #pragma unroll
-      for (int c = 0; c < kCB; ++c) {
-        const int b = g * kCB + c;
-        if (b < NQ) {
-          fh[slot][c] = *reinterpret_cast<const bf16x8*>(b_cur + (b * 64 + lane) * 16);
-          fl[slot][c] = *reinterpret_cast<const bf16x8*>(b_cur + NQ * 1024 + (b * 64 + lane) * 16);
-        }
-      }
-    };
-    load_group(0, 0);
-    bf16x8 ah[kRB], al[kRB], sh[kRB], sl[kRB];
+    for (int rb = 0; rb < kRB; ++rb) split_bf16(ac[rb], ah[rb], al[rb]);
 #pragma unroll
-    for (int rb = 0; rb < kRB; ++rb) {
-      split_bf16(ac[rb], ah[rb], al[rb]);
-      if constexpr (MODEL == EQTL_MODEL_LMM) {
-        f32x8 sv;
+    for (int rb = 0; rb < kRB; ++rb) acc[rb] = mfma_bf16(ah[rb], bh, acc[rb]);
 #pragma unroll
-        for (int t = 0; t < 8; ++t) sv.v[t] = ac[rb].v[t] * ac[rb].v[t];
-        split_bf16(sv, sh[rb], sl[rb]);
-      }
-    }
+    for (int rb = 0; rb < kRB; ++rb) acc[rb] = mfma_bf16(ah[rb], bl, acc[rb]);
 #pragma unroll
-    for (int g = 0; g < kGroups; ++g) {
-      // Wait for group g before the reads of group g + 1 go out: the direct-
-      // to-LDS copies make the compiler's own lgkmcnt waits all-or-nothing, so
-      // a wait placed after those reads would also drain them.
-      __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
-      __builtin_amdgcn_sched_barrier(0);
-      if (g + 1 < kGroups) load_group(g + 1, (g + 1) & 1);
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int term = 0; term < 3; ++term)
-#pragma unroll
-        for (int c = 0; c < kCB; ++c) {
-          const int b = g * kCB + c;
-          if (b < NQ) {
-            const bf16x8& y = term == 1 ? fl[g & 1][c] : fh[g & 1][c];
-#pragma unroll
-            for (int rb = 0; rb < kRB; ++rb) {
-              const bool sqr = MODEL == EQTL_MODEL_LMM && b == 0;
-              const bf16x8& x = term == 2 ? (sqr ? sl[rb] : al[rb]) : (sqr ? sh[rb] : ah[rb]);
-              acc[rb][b] = mfma_bf16(x, y, acc[rb][b]);
-            }
-          }
-        }
-      __builtin_amdgcn_sched_barrier(0);
-    }
+    for (int rb = 0; rb < kRB; ++rb) acc[rb] = mfma_bf16(al[rb], bh, acc[rb]);
   }
 }
 
-template <int MODEL, int PREC, int NQ>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_wald(WaldArgs a) {
+// K8: fused trans/cis LM scan (gty = g^T y_s per test, FP64 Wald from it and
+// the per-SNP FP64 g^T g). One workgroup = one group of 16 genes x a strided
+// set of 128-SNP tiles, 4 waves x 32 rows; the group's one-block gene tile and
+// the SNP rows are staged in LDS as in the LMM kernel.
+template <int PREC>
+__global__ __launch_bounds__(256) void k_wald_lm(WaldArgs a) {
+  constexpr int kAS = kLmStages;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   // XCD-aware mapping: workgroups are dealt round-robin to the 8 XCDs; the
   // n_splits consecutive workgroups of one XCD share a gene group, so the
@@ -603,7 +518,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const int gene = group * 16 + gl;
   const bool gene_ok = gene < a.G && a.gene_valid[gene];
   const int kk = 8 * (lane >> 4);
-  const int nq = a.nq;
   const int nks = a.n_pad >> 5;
   const double gscal = gene_ok ? a.gene_scal[gene] : 0.0;
   // the gene's window (trans: excluded; cis: the only SNPs tested)
@@ -612,10 +526,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const int64_t gcs = excl ? a.gene_cs[gene] : 0, gce = excl ? a.gene_ce[gene] : 0;
   const bool win_known = excl && a.snp_chrom;
 
-  constexpr int tile_bytes = NQ * 2048;  // one k-step tile of the group (either precision)
+  constexpr int tile_bytes = 2048;  // one k-step tile of the group (either precision)
   unsigned char* const buf0 = smem;
   unsigned char* const buf1 = smem + tile_bytes;
-  const size_t r0b = wald_region0_bytes(NQ);
+  unsigned char* const abuf = smem + 2 * tile_bytes;
+  const size_t r0b = lm_region0_bytes();
   RowInfo& ri = *reinterpret_cast<RowInfo*>(smem + r0b);
   double* thead = reinterpret_cast<double*>(smem + r0b + sizeof(RowInfo));
   double* fold = reinterpret_cast<double*>(smem);
@@ -624,7 +539,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   const unsigned char* gsrc =
       PREC == EQTL_PREC_FP32 ? reinterpret_cast<const unsigned char*>(a.gc.f32 + a.gc.tile_f32(group, 0))
                              : reinterpret_cast<const unsigned char*>(a.gc.bf + a.gc.tile_bf(group, 0));
-  constexpr int chunks = tile_bytes >> 10;  // 1 KiB per wave instruction (64 lanes x 16 B)
+  // this wave's part of a row stage: chunk (rb, h) = 64 lanes x 16 B, so every
+  // later ds_read_b128 of it is conflict-free
+  const int wrow = wave * kRB * 2 * 1024;
 
   GeneAcc st;
   acc_init(st);
@@ -632,95 +549,59 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
   for (int tile = t_lo; tile < t_hi; tile += t_step) {
     const int64_t row0 = static_cast<int64_t>(tile) * kTileRows;
     const int64_t rowA = row0 + wave * 32;
-    __syncthreads();  // the previous tile's epilogue is done with ri
+    __syncthreads();  // the previous tile's epilogue is done with ri and fold
     if (threadIdx.x < kTileRows) {
       const int t = threadIdx.x;
       const int64_t j = row0 + t;
       const bool in = j < a.m;
       ri.keep[t] = in ? a.keep[j] : 0;
       ri.af[t] = in ? a.af[j] : 0.0;
-      ri.gtg[t] = (MODEL == EQTL_MODEL_LM && in) ? a.gtg[j] : 0.0;
+      ri.gtg[t] = in ? a.gtg[j] : 0.0;
       const bool ann = in && a.snp_chrom;
       ri.chrom[t] = ann ? a.snp_chrom[a.snp0 + j] : -1;
       ri.pos[t] = ann ? a.snp_pos[a.snp0 + j] : 0;
     }
-    f32x4 acc[kRB][NQ];
+    f32x4 acc[kRB];
 #pragma unroll
-    for (int rb = 0; rb < kRB; ++rb)
-#pragma unroll
-      for (int b = 0; b < NQ; ++b) acc[rb][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int rb = 0; rb < kRB; ++rb) acc[rb] = f32x4{0.f, 0.f, 0.f, 0.f};
     // SNP-row operand: lane l of row block rb holds row rowA + 16 rb + (l & 15),
     // samples 32 s + 8 (l >> 4) + [0..7] of k-step s
     const float* arow[kRB];
 #pragma unroll
     for (int rb = 0; rb < kRB; ++rb) arow[rb] = a.A + (rowA + rb * 16 + gl) * a.n_pad + kk;
-    constexpr int kAS = wald_a_stages(NQ);
-    unsigned char* const abuf = smem + 2 * tile_bytes;
-    // this wave's part of a row stage: chunk (rb, h) = 64 lanes x 16 B, so every
-    // later ds_read_b128 of it is conflict-free
-    const int wrow = wave * kRB * 2 * 1024;
-    f32x8 an[kRB];
     // prologue: gene tile of k-step 0, rows of the first stages
-    for (int c = wave; c < chunks; c += 4) lds_direct16(gsrc + c * 1024 + lane * 16, buf0 + c * 1024);
-    if constexpr (kAS > 0) {
-      for (int s = 0; s < kAS - 1 && s < nks; ++s) {
-        const float* src[kRB];
+    if (wave < 2) lds_direct16(gsrc + wave * 1024 + lane * 16, buf0 + wave * 1024);
+    for (int s = 0; s < kAS - 1 && s < nks; ++s) {
+      const float* src[kRB];
 #pragma unroll
-        for (int rb = 0; rb < kRB; ++rb) src[rb] = arow[rb] + s * 32;
-        wald_stage_rows(src, abuf + s * kAStageBytes + wrow, lane);
-      }
-    } else {
-#pragma unroll
-      for (int rb = 0; rb < kRB; ++rb) an[rb] = load8(arow[rb]);
+      for (int rb = 0; rb < kRB; ++rb) src[rb] = arow[rb] + s * 32;
+      wald_stage_rows(src, abuf + s * kAStageBytes + wrow, lane);
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int s = 0; s < nks; ++s) {
-      const int sr = kAS > 0 ? s + kAS - 1 : s + 1;  // k-step whose rows are fetched now
+      const int sr = s + kAS - 1;  // k-step whose rows are fetched now
       const float* src[kRB];
 #pragma unroll
       for (int rb = 0; rb < kRB; ++rb) src[rb] = arow[rb] + sr * 32;
-      unsigned char* const a_wr =
-          (kAS > 0 && sr < nks) ? abuf + (sr % (kAS > 0 ? kAS : 1)) * kAStageBytes + wrow : nullptr;
-      const unsigned char* const a_rd = kAS > 0 ? abuf + (s % (kAS > 0 ? kAS : 1)) * kAStageBytes + wrow : nullptr;
+      unsigned char* const a_wr = sr < nks ? abuf + (sr % kAS) * kAStageBytes + wrow : nullptr;
+      const unsigned char* const a_rd = abuf + (s % kAS) * kAStageBytes + wrow;
       unsigned char* const b_nxt = s + 1 < nks ? (((s + 1) & 1) ? buf1 : buf0) : nullptr;
-      wald_kstep<MODEL, PREC, NQ, kAS>(acc, an, a_rd, a_wr, src, kAS == 0 && sr < nks, (s & 1) ? buf1 : buf0,
-                                       b_nxt, gsrc + static_cast<size_t>(s + 1) * tile_bytes, wave, lane);
-      // the next k-step's gene tile and rows must have landed; with three row
-      // stages the rows of k-step s+2 (issued last) stay in flight. The
-      // barrier is a bare s_barrier: the direct-to-LDS loads are complete
-      // once vmcnt says so, and a workgroup release fence would drain vmcnt
-      // to zero, defeating the third stage.
-      if (kAS == 3 && s + 2 < nks) __builtin_amdgcn_s_waitcnt(kWaitVm4Lgkm0);
+      lm_kstep<PREC>(acc, a_rd, a_wr, src, (s & 1) ? buf1 : buf0, b_nxt,
+                     gsrc + static_cast<size_t>(s + 1) * tile_bytes, wave, lane);
+      // the next k-step's gene tile and rows must have landed; the rows of
+      // k-step s+2 (issued last) stay in flight. The barrier is a bare
+      // s_barrier: a workgroup release fence would drain vmcnt to zero.
+      if (s + 2 < nks) __builtin_amdgcn_s_waitcnt(kWaitVm4Lgkm0);
       else __builtin_amdgcn_s_waitcnt(0);
       lds_barrier();
     }
-    // fold the accumulators to (gg, S, num) per test in FP64, parked in LDS
-    // (the last k-step's barrier has released the staging buffers)
+    // park gty per test in LDS (the last k-step's barrier has released the
+    // staging buffers), then the FP64 epilogue, one test per iteration
 #pragma unroll
     for (int rb = 0; rb < kRB; ++rb)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int t = rb * 4 + r;
-        double gg = 0, S = 0, num;
-        if constexpr (MODEL == EQTL_MODEL_LMM) {
-          gg = acc[rb][0][r];
-          double ss = 0;
-          num = 0;
-#pragma unroll
-          for (int b = 1; b < NQ; ++b) {
-            if (b < nq - 1) ss += static_cast<double>(acc[rb][b][r]) * acc[rb][b][r];
-            if (b == nq - 1) num = acc[rb][b][r];
-          }
-          S = gg - ss;
-        } else {
-          num = acc[rb][0][r];
-        }
-        fold[(t * 3 + 0) * 256 + threadIdx.x] = gg;
-        fold[(t * 3 + 1) * 256 + threadIdx.x] = S;
-        fold[(t * 3 + 2) * 256 + threadIdx.x] = num;
-      }
-    // FP64 epilogue, one test per iteration (one copy of the FP64 code)
+      for (int r = 0; r < 4; ++r) fold[(rb * 4 + r) * 256 + threadIdx.x] = acc[rb][r];
 #pragma unroll 1
     for (int t = 0; t < kRB * 4; ++t) {
       const int rb = t >> 2, r = t & 3;
@@ -728,13 +609,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) voi
       const int64_t j = row0 + jl;
       const bool inwin = win_known && ri.chrom[jl] == gch && ri.pos[jl] >= gcs && ri.pos[jl] <= gce;
       const bool ok = gene_ok && j < a.m && ri.keep[jl] && (a.cis ? inwin : !inwin);
-      const double gg = fold[(t * 3 + 0) * 256 + threadIdx.x];
-      const double S = fold[(t * 3 + 1) * 256 + threadIdx.x];
-      double num = fold[(t * 3 + 2) * 256 + threadIdx.x];
-      const bool cand = refine_wanted<MODEL>(a, ok, gg, S, num, gscal, ri.gtg[jl]);
+      double num = fold[t * 256 + threadIdx.x];
+      const bool cand = refine_wanted<EQTL_MODEL_LM>(a, ok, 0.0, 0.0, num, gscal, ri.gtg[jl]);
       const double n64 = wave_refine(a, cand, j, gene);
       if (cand) num = n64;
-      if (ok) wald_test<MODEL>(a, ri, jl, j, gene, gscal, gg, S, num, tv, st, asum);
+      if (ok) wald_test<EQTL_MODEL_LM>(a, ri, jl, j, gene, gscal, 0.0, 0.0, num, tv, st, asum);
     }
   }
   st.acat_hi = asum.hi;

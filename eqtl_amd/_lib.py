@@ -19,9 +19,13 @@ MODEL_LM, MODEL_LMM = 0, 1
 PREC_FP32, PREC_BF16X3 = 0, 1
 DELTA_REF, DELTA_NEWTON, DELTA_GIVEN = 0, 1, 2
 SCOPE_TRANS, SCOPE_CIS = 0, 1
+GENO_BED, GENO_DOSE32 = 0, 1
+EQTL_E_FORMAT = -7
 
 HIT_DTYPE = np.dtype([("snp", "i8"), ("gene", "i4"), ("pad", "i4"), ("af", "f8"), ("beta", "f8"),
                       ("se", "f8"), ("stat", "f8"), ("p", "f8"), ("r2", "f8")])
+VCF_RECORD_DTYPE = np.dtype([("offset", "i8"), ("length", "i4"), ("gt_key", "i2"), ("ds_key", "i2"),
+                             ("haploid", "i4"), ("pad", "i4")])
 GENE_STATE_DTYPE = np.dtype([("acat_p", "f8"), ("best_p", "f8"), ("best_snp", "i8"), ("best_af", "f8"),
                              ("best_beta", "f8"), ("best_se", "f8"), ("best_stat", "f8"), ("best_r2", "f8"),
                              ("n_tested", "i4"), ("n_sig", "i4")])
@@ -35,7 +39,7 @@ EXPORTS = [
     "eqtl_gpu_set_gene_windows", "eqtl_gpu_scan_begin", "eqtl_gpu_scan_block", "eqtl_gpu_scan_end",
     "eqtl_gpu_set_timing", "eqtl_gpu_get_timing", "eqtl_gpu_synth_bed", "eqtl_gpu_copy_rows",
     "eqtl_gpu_synchronize", "eqtl_gpu_selftest", "eqtl_gpu_upload_bed", "eqtl_gpu_cis_perm",
-    "eqtl_gpu_beta_approx",
+    "eqtl_gpu_beta_approx", "eqtl_gpu_set_geno_format", "eqtl_gpu_vcf_decode",
 ]
 
 _lib = None
@@ -129,6 +133,19 @@ class Context:
         ch = np.ascontiguousarray(chrom, dtype=np.int32)
         ps = np.ascontiguousarray(pos, dtype=np.int64)
         _check(lib().eqtl_gpu_set_snp_annot(self._c, _p(ch), _p(ps), ctypes.c_int64(ch.size)))
+
+    def set_geno_format(self, fmt):
+        _check(lib().eqtl_gpu_set_geno_format(self._c, ctypes.c_int32(fmt)))
+        self.bps = 4 * self.n_file if fmt == GENO_DOSE32 else (self.n_file + 3) // 4
+
+    def vcf_decode(self, text: bytes, recs):
+        """Sample columns of VCF records (VCF_RECORD_DTYPE) -> device DOSE32 rows.
+        Returns the device pointer (owned by the context, valid until the next call)."""
+        recs = np.ascontiguousarray(recs, dtype=VCF_RECORD_DTYPE)
+        ptr = ctypes.c_void_p()
+        _check(lib().eqtl_gpu_vcf_decode(self._c, ctypes.c_char_p(text), ctypes.c_int32(0), ctypes.c_int64(len(text)),
+                                         _p(recs), ctypes.c_int64(recs.size), ctypes.byref(ptr)))
+        return ptr.value
 
     def decode_bed(self, rows, want_dosage=True):
         rows = np.ascontiguousarray(rows, dtype=np.uint8)

@@ -15,13 +15,14 @@ HIPCC_FLAGS = [
 ]
 
 
-HOST_SRC = [os.path.join(HERE, "host", "eqtl_host.cpp"), os.path.join(HERE, "host", "main.cpp")]
+HOST_SRC = [os.path.join(HERE, "host", "eqtl_host.cpp"), os.path.join(HERE, "host", "vcf_source.cpp"),
+            os.path.join(HERE, "host", "main.cpp")]
 CLI = os.path.join(HERE, "eqtl")
 
 
 def build_cli() -> str:
     """The eqtl command-line driver (C++17 host code over libeqtl_gpu.so)."""
-    cmd = ["g++", "-O2", "-std=c++17", "-Wall", *HOST_SRC, "-L" + HERE, "-leqtl_gpu",
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-pthread", *HOST_SRC, "-L" + HERE, "-leqtl_gpu", "-lz",
            "-Wl,-rpath,$ORIGIN", "-o", CLI + ".tmp"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:

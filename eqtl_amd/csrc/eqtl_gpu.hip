@@ -18,6 +18,7 @@
 #include "numerics.hpp"
 #include "reml_kernels.hpp"
 #include "scan_kernels.hpp"
+#include "vcf_kernels.hpp"
 #include "wald_lmm.hpp"
 #include "grm_kernels.hpp"
 #include "reml_batch.hpp"
@@ -116,6 +117,8 @@ struct eqtl_gpu_ctx {
   // samples / policy
   int n_file = 0, n = 0, n_pad = 0;
   DBuf sample_col;
+  DBuf col_used;  // n_file: how many analysis samples read each file column
+  int geno_fmt = EQTL_GENO_BED;
   int impute = 1;
   double max_miss = 0.8, maf_min = 0.0;
   // snp annotation
@@ -155,6 +158,8 @@ struct eqtl_gpu_ctx {
   // per-block buffers
   int64_t m_cap = 0;
   DBuf rows, A32, A64, Gt, keep, af, gtg, dosage;
+  // VCF decode
+  DBuf vcf_text, vcf_recs, vcf_rows, vcf_bad;
   // GRM
   DBuf grm_acc, grm_miss, grm_cnt, grm_z;
   int64_t grm_m = 0;
@@ -215,6 +220,11 @@ GeneCols gene_cols(eqtl_gpu_ctx* c) {
   return g;
 }
 
+// Smallest row size of the context's genotype row format.
+int64_t min_bps(const eqtl_gpu_ctx* c) {
+  return c->geno_fmt == EQTL_GENO_DOSE32 ? 4 * static_cast<int64_t>(c->n_file) : (c->n_file + 3) / 4;
+}
+
 // Block buffers for up to m SNPs.
 int ensure_block(eqtl_gpu_ctx* c, int64_t m, int32_t bps, bool want64) {
   const int64_t m_pad = round_up(std::max<int64_t>(m, 1), 128);
@@ -249,7 +259,11 @@ int run_decode(eqtl_gpu_ctx* c, const uint8_t* rows_dev, int64_t m, int32_t bps,
   a.keep = c->keep.as<uint8_t>();
   a.af = c->af.as<double>();
   Timed t(c, "decode");
-  hipLaunchKernelGGL(k_decode, dim3(static_cast<unsigned>((m_pad + 3) / 4)), dim3(256), 0, c->stream, a);
+  const dim3 grid(static_cast<unsigned>((m_pad + 3) / 4));
+  if (c->geno_fmt == EQTL_GENO_DOSE32)
+    hipLaunchKernelGGL(k_decode<EQTL_GENO_DOSE32>, grid, dim3(256), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_decode<EQTL_GENO_BED>, grid, dim3(256), 0, c->stream, a);
   CHK(check_launch("k_decode"));
   return t.done();
 }
@@ -393,9 +407,82 @@ int eqtl_gpu_set_samples(eqtl_gpu_ctx* c, int32_t n_file, const int32_t* sample_
   c->n_file = n_file;
   c->n = n;
   c->n_pad = static_cast<int>(round_up(n, 64));
+  std::vector<uint8_t> used(static_cast<size_t>(n_file), 0);
+  for (int i = 0; i < n; ++i) {
+    if (used[sample_col[i]] == 255) return fail(EQTL_E_ARG, "a file column is repeated more than 255 times");
+    ++used[sample_col[i]];
+  }
   CHK(ensure(c->sample_col, sizeof(int32_t) * n));
+  CHK(ensure(c->col_used, static_cast<size_t>(n_file)));
   HIPC(hipMemcpyAsync(c->sample_col.p, sample_col, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->col_used.p, used.data(), static_cast<size_t>(n_file), hipMemcpyHostToDevice, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
+  return EQTL_OK;
+}
+
+int eqtl_gpu_set_geno_format(eqtl_gpu_ctx* c, int32_t format) {
+  if (!c) return fail(EQTL_E_ARG, "null ctx");
+  if (format != EQTL_GENO_BED && format != EQTL_GENO_DOSE32) return fail(EQTL_E_ARG, "unknown genotype row format");
+  c->geno_fmt = format;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_vcf_decode(eqtl_gpu_ctx* c, const char* text, int32_t on_dev, int64_t text_len,
+                        const eqtl_vcf_record* recs, int64_t m, const uint8_t** rows_dev) {
+  if (!c || !rows_dev || m <= 0 || !recs || !text || text_len < 0) return fail(EQTL_E_ARG, "bad vcf_decode arguments");
+  if (c->n <= 0 || c->n_file <= 0) return fail(EQTL_E_STATE, "set_samples first");
+  if (m > 0x7fffffff) return fail(EQTL_E_ARG, "too many records in one call");
+  int64_t lmax = 0;
+  for (int64_t r = 0; r < m; ++r) {
+    const eqtl_vcf_record& e = recs[r];
+    if (e.offset < 0 || e.length < 0 || e.offset + e.length > text_len || e.gt_key < -1 || e.ds_key < -1)
+      return fail(EQTL_E_ARG, "VCF record " + std::to_string(r) + ": span outside the text or bad key index");
+    lmax = std::max<int64_t>(lmax, e.length);
+  }
+  CHK(set_device(c));
+  const unsigned char* td = reinterpret_cast<const unsigned char*>(text);
+  if (!on_dev) {
+    CHK(ensure(c->vcf_text, static_cast<size_t>(text_len) + 16));
+    HIPC(hipMemcpyAsync(c->vcf_text.p, text, static_cast<size_t>(text_len), hipMemcpyHostToDevice, c->stream));
+    td = c->vcf_text.as<unsigned char>();
+  }
+  CHK(ensure(c->vcf_recs, sizeof(eqtl_vcf_record) * static_cast<size_t>(m)));
+  CHK(ensure(c->vcf_rows, sizeof(float) * static_cast<size_t>(m) * c->n_file));
+  CHK(ensure(c->vcf_bad, sizeof(unsigned long long)));
+  HIPC(hipMemcpyAsync(c->vcf_recs.p, recs, sizeof(eqtl_vcf_record) * static_cast<size_t>(m), hipMemcpyHostToDevice,
+                      c->stream));
+  const unsigned long long none = static_cast<unsigned long long>(m);
+  HIPC(hipMemcpyAsync(c->vcf_bad.p, &none, sizeof(none), hipMemcpyHostToDevice, c->stream));
+  VcfArgs a;
+  a.text = td;
+  a.recs = c->vcf_recs.as<eqtl_vcf_record>();
+  a.m = m;
+  a.n_file = c->n_file;
+  a.n = c->n;
+  a.used = c->col_used.as<uint8_t>();
+  a.impute = c->impute;
+  a.max_miss = c->max_miss;
+  a.maf_min = c->maf_min;
+  a.rows = c->vcf_rows.as<float>();
+  a.bad = c->vcf_bad.as<unsigned long long>();
+  // a record's dword-aligned span is at most length + 6 bytes
+  const int64_t span = round_up(lmax + 8, 16);
+  Timed t(c, "vcf");
+  if (span <= kVcfLdsMax) {
+    hipLaunchKernelGGL(k_vcf_decode<true>, dim3(static_cast<unsigned>(m)), dim3(kVcfThreads),
+                       static_cast<size_t>(span), c->stream, a);
+  } else {
+    hipLaunchKernelGGL(k_vcf_decode<false>, dim3(static_cast<unsigned>(m)), dim3(kVcfThreads), 0, c->stream, a);
+  }
+  CHK(check_launch("k_vcf_decode"));
+  CHK(t.done());
+  unsigned long long bad = none;
+  HIPC(hipMemcpyAsync(&bad, c->vcf_bad.p, sizeof(bad), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  if (bad < none)
+    return fail(EQTL_E_FORMAT, "VCF record " + std::to_string(bad) + " of the block: the number of sample columns "
+                               "differs from the header's " + std::to_string(c->n_file) + " samples");
+  *rows_dev = c->vcf_rows.as<uint8_t>();
   return EQTL_OK;
 }
 
@@ -425,7 +512,7 @@ int eqtl_gpu_decode_bed(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
                         double* dosage, uint8_t* keep, double* af) {
   if (!c || !rows || m <= 0) return fail(EQTL_E_ARG, "bad decode arguments");
   if (c->n <= 0) return fail(EQTL_E_STATE, "set_samples first");
-  if (bps < (c->n_file + 3) / 4) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
+  if (bps < min_bps(c)) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
   CHK(set_device(c));
   CHK(ensure_block(c, m, bps, false));
   if (dosage) CHK(ensure(c->dosage, sizeof(double) * static_cast<size_t>(m) * c->n));
@@ -1038,7 +1125,7 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
                         int32_t bps, eqtl_hit* hits, int64_t hit_cap, int64_t* n_hits) {
   if (!c || !rows || m <= 0 || snp0 < 0) return fail(EQTL_E_ARG, "bad scan_block arguments");
   if (!c->scanning) return fail(EQTL_E_STATE, "scan_begin first");
-  if (bps < (c->n_file + 3) / 4) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
+  if (bps < min_bps(c)) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
   if (c->have_windows && snp0 + m > c->m_annot) return fail(EQTL_E_ARG, "SNP block beyond the annotation");
   if (hit_cap > 0 && !hits) return fail(EQTL_E_ARG, "hits buffer is null");
   CHK(set_device(c));
@@ -1454,7 +1541,7 @@ int eqtl_gpu_grm_begin(eqtl_gpu_ctx* c) {
 int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, int64_t m, int32_t bps) {
   if (!c || !rows || m <= 0) return fail(EQTL_E_ARG, "bad grm_accumulate arguments");
   if (!c->grm_acc.p) return fail(EQTL_E_STATE, "grm_begin first");
-  if (bps < (c->n_file + 3) / 4) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
+  if (bps < min_bps(c)) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
   CHK(set_device(c));
   CHK(ensure_block(c, m, bps, false));
   const int64_t m_pad = round_up(m, 128);
@@ -1468,7 +1555,8 @@ int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev
   DBuf used;
   CHK(ensure(used, 2 * sizeof(unsigned long long)));
   HIPC(hipMemsetAsync(used.p, 0, 2 * sizeof(unsigned long long), c->stream));
-  hipLaunchKernelGGL(k_grm_standardize, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream, rd, m, m_pad,
+  auto* kstd = c->geno_fmt == EQTL_GENO_DOSE32 ? &k_grm_standardize<EQTL_GENO_DOSE32> : &k_grm_standardize<EQTL_GENO_BED>;
+  hipLaunchKernelGGL(kstd, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream, rd, m, m_pad,
                      bps, c->n, np, c->n_file, c->sample_col.as<int32_t>(), z64, mis, c->grm_cnt.as<int32_t>(),
                      used.as<unsigned long long>(), c->impute, c->max_miss, c->maf_min);
   CHK(check_launch("k_grm_standardize"));

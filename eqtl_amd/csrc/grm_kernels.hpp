@@ -1,6 +1,6 @@
 // K2: GRM (compute_grm_from, grm.cpp:112-224).
-//   k_grm_standardize — decode each SNP row, apply the GRM's SNP rules
-//       (decode_row filters, then p = mean/2 in [1e-8, 1-1e-8], sd >= 1e-12),
+//   k_grm_standardize — decode each genotype row, apply the GRM's SNP rules
+//       (the source's filters, then p = mean/2 in [1e-8, 1-1e-8], sd >= 1e-12),
 //       write z = (g - mean)/sd (missing → 0) in fp64 and the 0/1 missing
 //       indicator in bf16, sample-major (SNP-contiguous) for the MFMA
 //       operands; count per-sample missing calls of used SNPs.
@@ -13,10 +13,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "geno_rows.hpp"
 #include "mfma.hpp"
 
 namespace eqtl_amd {
 
+template <int FMT>
 __global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, int64_t m, int64_t m_pad, int bps,
                                                          int n, int n_pad, int n_file, const int32_t* sample_col,
                                                          double* z64, __bf16* mis, int32_t* miss_cnt,
@@ -26,14 +28,15 @@ __global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, in
   const int64_t j = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
   if (j >= m_pad) return;
   bool use = j < m;
-  int n_miss = 0, sum = 0;
+  int n_miss = 0;
+  double sum = 0.0;  // integer-valued for BED rows (exact)
   const uint8_t* row = use ? rows + j * bps : nullptr;
   if (use) {
     for (int i = lane; i < n; i += 64) {
-      const int col = sample_col[i];
-      const int code = (row[col >> 2] >> ((col & 3) * 2)) & 3;
-      if (code == 1) { ++n_miss; continue; }
-      sum += code == 0 ? 2 : (code == 2 ? 1 : 0);
+      bool miss;
+      const double g = geno_at<FMT>(row, sample_col[i], miss);
+      if (miss) { ++n_miss; continue; }
+      sum += g;
     }
     for (int o = 32; o > 0; o >>= 1) {
       n_miss += __shfl_xor(n_miss, o);
@@ -44,13 +47,18 @@ __global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, in
   if (use) {
     const int n_ok = n - n_miss;
     if (n_ok == 0) use = false;
-    if (use && static_cast<double>(n_miss) / n > max_miss + 1e-15) use = false;
-    if (use && !impute && n_miss > 0) use = false;
+    // the source's rules (decode_row) for BED rows; DOSE32 rows carry them
+    if constexpr (FMT == EQTL_GENO_BED) {
+      if (use && static_cast<double>(n_miss) / n > max_miss + 1e-15) use = false;
+      if (use && !impute && n_miss > 0) use = false;
+    }
     if (use) {
-      mu = static_cast<double>(sum) / n_ok;
-      double af = fmin(1.0, fmax(0.0, mu / 2.0));
-      if (af < 1e-12 || af > 1.0 - 1e-12) use = false;
-      if (maf_min > 0.0 && (af < maf_min || af > 1.0 - maf_min)) use = false;
+      mu = sum / n_ok;
+      if constexpr (FMT == EQTL_GENO_BED) {
+        double af = fmin(1.0, fmax(0.0, mu / 2.0));
+        if (af < 1e-12 || af > 1.0 - 1e-12) use = false;
+        if (maf_min > 0.0 && (af < maf_min || af > 1.0 - maf_min)) use = false;
+      }
     }
     if (use) {
       mean = (sum + n_miss * mu) / n;
@@ -64,13 +72,12 @@ __global__ __launch_bounds__(256) void k_grm_standardize(const uint8_t* rows, in
     double z = 0.0;
     float mi = 0.f;
     if (use && i < n) {
-      const int col = sample_col[i];
-      const int code = (row[col >> 2] >> ((col & 3) * 2)) & 3;
-      if (code == 1) {
+      bool miss;
+      const double g = geno_at<FMT>(row, sample_col[i], miss);
+      if (miss) {
         mi = 1.f;
         atomicAdd(&miss_cnt[i], 1);
       } else {
-        const double g = code == 0 ? 2.0 : (code == 2 ? 1.0 : 0.0);
         z = (g - mean) / sd;
       }
     }

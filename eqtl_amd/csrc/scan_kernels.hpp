@@ -1,6 +1,7 @@
 // Genotype-side kernels of the trans scans:
-//   K1 k_decode   — 2-bit PLINK rows → centred, mean-imputed dosages + per-SNP
-//                   filters/stats (plink_bed.cpp:263-320, scan_common.cpp:133-151)
+//   K1 k_decode   — genotype rows (2-bit PLINK or float dosage) → centred,
+//                   mean-imputed dosages + per-SNP filters/stats
+//                   (plink_bed.cpp:263-320, scan_common.cpp:133-151)
 //   K6 k_rotate   — G~ = (G - mu)^T Q on MFMA (scan_trans_lmm.cpp:168)
 //   K7/K8 k_wald  — fused contraction against the per-gene column blocks +
 //                   FP64 Wald epilogue + per-gene reductions + hit emission
@@ -11,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "geno_rows.hpp"
 #include "mfma.hpp"
 #include "numerics.hpp"
 #include "../../include/eqtl_gpu.h"
@@ -49,6 +51,10 @@ __host__ __device__ inline void acc_merge(GeneAcc& a, const GeneAcc& b) {
 
 // ---------------------------------------------------------------------------
 // K1: decode. One wave per SNP row. Rows >= m (padding) are written as zeros.
+// BED rows get the source's rules here (decode_row, plink_bed.cpp:263-320);
+// DOSE32 rows had them applied when they were written (a rejected row is all
+// missing), so only n_ok > 0 is left. Then subset_maf_or_nan
+// (scan_common.cpp:133-151) on the mean-imputed row.
 // ---------------------------------------------------------------------------
 struct DecodeArgs {
   const uint8_t* rows;
@@ -64,6 +70,7 @@ struct DecodeArgs {
   double* af;          // m_pad
 };
 
+template <int FMT>
 __global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
   const int lane = threadIdx.x & 63;
   const int64_t j = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
@@ -79,14 +86,14 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
     return;
   }
   const uint8_t* row = a.rows + j * a.bps;
-  int n_miss = 0, sum = 0, sum2 = 0, bad = 0;
+  int n_miss = 0, bad = 0;
+  double sum = 0.0, sum2 = 0.0;  // integer-valued for BED rows (exact)
   for (int i = lane; i < a.n; i += kWave) {
     const int col = a.sample_col[i];
     if (col < 0 || col >= a.n_file) { bad = 1; continue; }
-    const int code = (row[col >> 2] >> ((col & 3) * 2)) & 3;
-    // PLINK-1 LUT {00→2, 01→missing, 10→1, 11→0} (plink_bed.cpp:221-224)
-    if (code == 1) { ++n_miss; continue; }
-    const int g = code == 0 ? 2 : (code == 2 ? 1 : 0);
+    bool miss;
+    const double g = geno_at<FMT>(row, col, miss);
+    if (miss) { ++n_miss; continue; }
     sum += g;
     sum2 += g * g;
   }
@@ -101,19 +108,21 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
   const int n_ok = n - n_miss;
   bool keep = !bad && n_ok > 0;
   double mu = 0, muc = 0, afs = __builtin_nan("");
-  if (keep) {
-    const double miss_frac = static_cast<double>(n_miss) / static_cast<double>(n);
-    if (miss_frac > a.max_miss + 1e-15) keep = false;
-    if (!a.impute && n_miss > 0) keep = false;
+  if constexpr (FMT == EQTL_GENO_BED) {
+    if (keep) {
+      const double miss_frac = static_cast<double>(n_miss) / static_cast<double>(n);
+      if (miss_frac > a.max_miss + 1e-15) keep = false;
+      if (!a.impute && n_miss > 0) keep = false;
+    }
+    if (keep) {
+      double afd = (sum / n_ok) / 2.0;
+      afd = fmin(1.0, fmax(0.0, afd));
+      if (afd < 1e-12 || afd > 1.0 - 1e-12) keep = false;
+      if (a.maf_min > 0.0 && (afd < a.maf_min || afd > 1.0 - a.maf_min)) keep = false;  // pass_maf
+    }
   }
   if (keep) {
-    mu = static_cast<double>(sum) / n_ok;
-    double afd = (static_cast<double>(sum) / n_ok) / 2.0;
-    afd = fmin(1.0, fmax(0.0, afd));
-    if (afd < 1e-12 || afd > 1.0 - 1e-12) keep = false;
-    if (a.maf_min > 0.0 && (afd < a.maf_min || afd > 1.0 - a.maf_min)) keep = false;  // pass_maf
-  }
-  if (keep) {
+    mu = sum / n_ok;
     // subset_maf_or_nan on the imputed vector
     const double sI = sum + n_miss * mu, s2I = sum2 + n_miss * mu * mu;
     double af = (sI / n) / 2.0;
@@ -129,13 +138,10 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
     if (i < n) {
       const int col = a.sample_col[i];
       if (col >= 0 && col < a.n_file) {
-        const int code = (row[col >> 2] >> ((col & 3) * 2)) & 3;
-        if (code == 1) {
-          dose = a.impute ? mu : __builtin_nan("");
-        } else {
-          dose = code == 0 ? 2.0 : (code == 2 ? 1.0 : 0.0);
-        }
-        if (keep) v = (code == 1 ? mu : dose) - muc;
+        bool miss;
+        const double g = geno_at<FMT>(row, col, miss);
+        dose = miss ? (a.impute ? mu : __builtin_nan("")) : g;
+        if (keep) v = (miss ? mu : g) - muc;
       }
       if (a.dosage) a.dosage[j * n + i] = dose;
     }

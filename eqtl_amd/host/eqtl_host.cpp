@@ -197,9 +197,9 @@ int parse_options(int argc, char** argv, Options& opt) {
       default: return 1;
     }
   }
-  if (!opt.vcf.empty()) die("VCF/BCF input is not provided by this engine; use -b/--bfile");
-  if (opt.bfile.empty()) {
-    std::cerr << "[E] need -b/--bfile\n";
+  if (opt.vcf.empty() == opt.bfile.empty()) {  // options.cpp:275-281
+    if (opt.vcf.empty()) std::cerr << "[E] need exactly one of -v/--vcf or -b/--bfile\n";
+    else std::cerr << "[E] use only one of -v/--vcf or -b/--bfile\n";
     return 1;
   }
   if (!opt.make_grm && opt.pheno.empty()) {
@@ -280,11 +280,28 @@ void PlinkBed::open(const std::string& prefix) {  // plink_bed.cpp:187-230
        prefix + "]");
 }
 
-std::vector<std::string> PlinkBed::chromosomes() const {
+std::vector<std::string> GenoSource::chromosomes() const {
   std::vector<std::string> out;
-  for (size_t i = 0; i < sites_.size(); ++i)
-    if (i == 0 || !chrom_equal(sites_[i].chrom, out.back())) out.push_back(sites_[i].chrom);
+  const auto& st = sites();
+  for (size_t i = 0; i < st.size(); ++i)
+    if (i == 0 || !chrom_equal(st[i].chrom, out.back())) out.push_back(st[i].chrom);
   return out;
+}
+
+const uint8_t* PlinkBed::rows(Engine&, size_t j0, size_t m, int32_t* on_dev) {
+  read_rows(j0, m, buf_);
+  *on_dev = 0;
+  return buf_.data();
+}
+
+const uint8_t* PlinkBed::gather(Engine&, const std::vector<int64_t>& idx, int32_t* on_dev) {
+  buf_.clear();
+  for (int64_t j : idx) {
+    read_rows(static_cast<size_t>(j), 1, row_);
+    buf_.insert(buf_.end(), row_.begin(), row_.begin() + bps_);
+  }
+  *on_dev = 0;
+  return buf_.data();
 }
 
 void PlinkBed::read_rows(size_t j0, size_t m, std::vector<uint8_t>& buf) {
@@ -629,32 +646,34 @@ void Engine::check(int rc, const char* what) const {
   if (rc != EQTL_OK) die(std::string(what) + ": " + eqtl_gpu_last_error());
 }
 
-static std::vector<int32_t> sample_cols(const PlinkBed& bed, const std::vector<std::string>& ids) {
+// set_sample_order (plink_bed.cpp:245-255, vcf_session.cpp:127-138) on the
+// engine, with the source's row format.
+static void set_sample_order(Engine& eng, const GenoSource& geno, const std::vector<std::string>& ids) {
   std::unordered_map<std::string, int> m;
-  for (size_t i = 0; i < bed.samples().size(); ++i) m[bed.samples()[i]] = static_cast<int>(i);
+  for (size_t i = 0; i < geno.samples().size(); ++i) m[geno.samples()[i]] = static_cast<int>(i);
   std::vector<int32_t> col;
   for (const auto& id : ids) {
     auto it = m.find(id);
-    if (it == m.end()) die("sample not in fam: " + id);
+    if (it == m.end()) die("sample not in genotype file: " + id);
     col.push_back(it->second);
   }
-  return col;
+  eng.check(eqtl_gpu_set_samples(eng.ctx(), static_cast<int32_t>(geno.n_file()), col.data(),
+                                 static_cast<int32_t>(col.size())), "set_samples");
+  eng.check(eqtl_gpu_set_geno_format(eng.ctx(), geno.row_format()), "set_geno_format");
 }
 
 static constexpr size_t kBlockSnps = 16384;
 
-Grm compute_grm(Engine& eng, PlinkBed& bed, const std::vector<std::string>& sample_ids, MissHand miss,
+Grm compute_grm(Engine& eng, GenoSource& bed, const std::vector<std::string>& sample_ids, MissHand miss,
                 double max_miss, double maf) {
-  const auto col = sample_cols(bed, sample_ids);
-  eng.check(eqtl_gpu_set_samples(eng.ctx(), static_cast<int32_t>(bed.n_file()), col.data(),
-                                 static_cast<int32_t>(col.size())), "set_samples");
+  set_sample_order(eng, bed, sample_ids);
   eng.check(eqtl_gpu_set_policy(eng.ctx(), miss == MissHand::Impute, max_miss, maf), "set_policy");
   eng.check(eqtl_gpu_grm_begin(eng.ctx()), "grm_begin");
-  std::vector<uint8_t> buf;
   for (size_t j0 = 0; j0 < bed.sites().size(); j0 += kBlockSnps) {
     const size_t m = std::min(kBlockSnps, bed.sites().size() - j0);
-    bed.read_rows(j0, m, buf);
-    eng.check(eqtl_gpu_grm_accumulate(eng.ctx(), buf.data(), 0, static_cast<int64_t>(m), bed.bytes_per_snp()),
+    int32_t on_dev = 0;
+    const uint8_t* rows = bed.rows(eng, j0, m, &on_dev);
+    eng.check(eqtl_gpu_grm_accumulate(eng.ctx(), rows, on_dev, static_cast<int64_t>(m), bed.bytes_per_snp()),
               "grm_accumulate");
   }
   Grm g;
@@ -678,7 +697,7 @@ static uint32_t fnv1a(const std::string& s);
 // permutation stage runs them on a virtual genotype set: gene k's top-K rows
 // in a block of their own, annotated as "chromosome" k with positions 1..K, so
 // that the block is gene k's window.
-static void stage2_perm(const Options& opt, Engine& eng, PlinkBed& bed, Model model, std::vector<TransJob>& jobs,
+static void stage2_perm(const Options& opt, Engine& eng, GenoSource& bed, Model model, std::vector<TransJob>& jobs,
                         std::vector<std::vector<std::pair<double, int64_t>>>& cand,
                         const std::vector<eqtl_gene_state>& st, const std::vector<double>& Y, const CovData& cov) {
   const int G = static_cast<int>(jobs.size());
@@ -690,7 +709,7 @@ static void stage2_perm(const Options& opt, Engine& eng, PlinkBed& bed, Model mo
   }
   if (q.empty()) return;
   const int bps = bed.bytes_per_snp();
-  std::vector<uint8_t> row, vbed;
+  std::vector<int64_t> vidx;
   std::vector<int32_t> vchrom, gch;
   std::vector<int64_t> vpos, gcs, gce;
   std::vector<double> Yq(n * q.size());
@@ -703,8 +722,7 @@ static void stage2_perm(const Options& opt, Engine& eng, PlinkBed& bed, Model mo
     if (static_cast<int>(cs.size()) > opt.perm_trans_top) cs.resize(opt.perm_trans_top);
     std::sort(cs.begin(), cs.end(), [](const auto& a, const auto& b) { return a.second < b.second; });
     for (size_t i = 0; i < cs.size(); ++i) {
-      bed.read_rows(static_cast<size_t>(cs[i].second), 1, row);
-      vbed.insert(vbed.end(), row.begin(), row.begin() + bps);
+      vidx.push_back(cs[i].second);
       vchrom.push_back(static_cast<int32_t>(qi));
       vpos.push_back(static_cast<int64_t>(i) + 1);
     }
@@ -720,7 +738,9 @@ static void stage2_perm(const Options& opt, Engine& eng, PlinkBed& bed, Model mo
   eqtl_gpu_ctx* c = eng.ctx();
   const int64_t mv = static_cast<int64_t>(vpos.size());
   eng.check(eqtl_gpu_set_snp_annot(c, vchrom.data(), vpos.data(), mv), "set_snp_annot");
-  eng.check(eqtl_gpu_upload_bed(c, vbed.data(), 0, mv, bps), "upload_bed");
+  int32_t on_dev = 0;
+  const uint8_t* vrows = bed.gather(eng, vidx, &on_dev);
+  eng.check(eqtl_gpu_upload_bed(c, vrows, on_dev, mv, bps), "upload_bed");
   if (model == Model::Lmm)
     eng.check(eqtl_gpu_lmm_prep(c, Yq.data(), Gq, cov.X.data(), cov.p, opt.delta_mode, nullptr, nullptr, nullptr),
               "lmm_prep");
@@ -747,7 +767,7 @@ static void stage2_perm(const Options& opt, Engine& eng, PlinkBed& bed, Model mo
   }
 }
 
-void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model, const std::string& scope,
+void scan_snp_outer(const Options& opt, Engine& eng, GenoSource& bed, Model model, const std::string& scope,
                     ScopeOut& out, double pthr, std::vector<TransJob>& jobs, const PhenoData& ph,
                     const CovData& cov, const std::vector<double>* Q, const std::vector<double>* lambda) {
   if (jobs.empty()) return;
@@ -802,7 +822,6 @@ void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model,
   eng.check(eqtl_gpu_scan_begin(c, model == Model::Lmm ? EQTL_MODEL_LMM : EQTL_MODEL_LM, opt.precision,
                                 EQTL_SCOPE_TRANS, pthr_dev),
             "scan_begin");
-  std::vector<uint8_t> buf;
   std::vector<eqtl_hit> hits(1 << 20);
   const auto& sites = bed.sites();
   auto fill = [&](AssocHit& h, const eqtl_hit& e) {
@@ -829,9 +848,10 @@ void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model,
   size_t snp_cnt = 0;
   for (size_t j0 = 0; j0 < sites.size(); j0 += kBlockSnps) {
     const size_t m = std::min(kBlockSnps, sites.size() - j0);
-    bed.read_rows(j0, m, buf);
+    int32_t on_dev = 0;
+    const uint8_t* rows = bed.rows(eng, j0, m, &on_dev);
     int64_t nh = 0;
-    int rc = eqtl_gpu_scan_block(c, buf.data(), 0, static_cast<int64_t>(j0), static_cast<int64_t>(m),
+    int rc = eqtl_gpu_scan_block(c, rows, on_dev, static_cast<int64_t>(j0), static_cast<int64_t>(m),
                                  bed.bytes_per_snp(), hits.data(), static_cast<int64_t>(hits.size()), &nh);
     if (rc == EQTL_E_CAPACITY)
       die("more than " + std::to_string(hits.size()) + " significant pairs in one SNP block; lower --pval-trans");
@@ -895,7 +915,7 @@ static uint32_t fnv1a(const std::string& s) {
 // scan_cis.hpp:15-185): genes in (chromosome, TSS) order, pairs written
 // gene by gene in SNP order, top only when best p <= pthr, p_emp from the
 // engine's permutation stage.
-void scan_cis(const Options& opt, Engine& eng, PlinkBed& bed, Model model, ScopeOut& out, double pthr,
+void scan_cis(const Options& opt, Engine& eng, GenoSource& bed, Model model, ScopeOut& out, double pthr,
               std::vector<TransJob>& jobs, const PhenoData& ph, const CovData& cov) {
   if (jobs.empty()) return;
   std::stable_sort(jobs.begin(), jobs.end(), [](const TransJob& a, const TransJob& b) {
@@ -948,9 +968,9 @@ void scan_cis(const Options& opt, Engine& eng, PlinkBed& bed, Model model, Scope
   eng.check(eqtl_gpu_set_snp_annot(c, snp_chrom.data(), snp_pos.data(), static_cast<int64_t>(snp_pos.size())),
             "set_snp_annot");
   if (opt.perm > 0) {
-    std::vector<uint8_t> all;
-    bed.read_rows(0, sites.size(), all);
-    eng.check(eqtl_gpu_upload_bed(c, all.data(), 0, static_cast<int64_t>(sites.size()), bed.bytes_per_snp()),
+    int32_t on_dev = 0;
+    const uint8_t* all = bed.rows(eng, 0, sites.size(), &on_dev);
+    eng.check(eqtl_gpu_upload_bed(c, all, on_dev, static_cast<int64_t>(sites.size()), bed.bytes_per_snp()),
               "upload_bed");
   }
   const int gm = model == Model::Lmm ? EQTL_MODEL_LMM : EQTL_MODEL_LM;
@@ -961,13 +981,13 @@ void scan_cis(const Options& opt, Engine& eng, PlinkBed& bed, Model model, Scope
     eng.check(eqtl_gpu_lm_prep(c, Y.data(), G, cov.X.data(), cov.p, nullptr), "lm_prep");
   eng.check(eqtl_gpu_set_gene_windows(c, gch.data(), gcs.data(), gce.data(), G), "set_gene_windows");
   eng.check(eqtl_gpu_scan_begin(c, gm, opt.precision, EQTL_SCOPE_CIS, pthr), "scan_begin");
-  std::vector<uint8_t> buf;
   std::vector<eqtl_hit> hits(1 << 20), all_hits;
   for (int64_t j0 = std::max<int64_t>(lo, 0); j0 < hi; j0 += static_cast<int64_t>(kBlockSnps)) {
     const size_t m = std::min<size_t>(kBlockSnps, static_cast<size_t>(hi - j0));
-    bed.read_rows(static_cast<size_t>(j0), m, buf);
+    int32_t on_dev = 0;
+    const uint8_t* rows = bed.rows(eng, static_cast<size_t>(j0), m, &on_dev);
     int64_t nh = 0;
-    int rc = eqtl_gpu_scan_block(c, buf.data(), 0, j0, static_cast<int64_t>(m), bed.bytes_per_snp(), hits.data(),
+    int rc = eqtl_gpu_scan_block(c, rows, on_dev, j0, static_cast<int64_t>(m), bed.bytes_per_snp(), hits.data(),
                                  static_cast<int64_t>(hits.size()), &nh);
     if (rc == EQTL_E_CAPACITY)
       die("more than " + std::to_string(hits.size()) + " significant pairs in one SNP block; lower --pval-cis");
@@ -1062,17 +1082,28 @@ void scan_cis(const Options& opt, Engine& eng, PlinkBed& bed, Model model, Scope
 }
 
 // ============================================================ orchestrator (scan.cpp:18-340)
+// The genotype source named by -b or -v (scan.cpp:20-45, 321-339).
+static std::unique_ptr<GenoSource> open_geno(const Options& opt) {
+  if (opt.use_bfile()) {
+    auto bed = std::make_unique<PlinkBed>();
+    bed->open(opt.bfile);
+    return bed;
+  }
+  auto vcf = std::make_unique<VcfSource>();
+  vcf->open(opt.vcf, opt.threads);
+  return vcf;
+}
+
 int run_make_grm(const Options& opt) {
-  PlinkBed bed;
-  bed.open(opt.bfile);
-  std::vector<std::string> samples = bed.samples();
+  std::unique_ptr<GenoSource> geno = open_geno(opt);
+  std::vector<std::string> samples = geno->samples();
   if (!opt.pheno.empty()) {
     PhenoData ph = load_pheno(opt.pheno);
-    samples = intersect_order(ph.sample_ids, bed.samples());
+    samples = intersect_order(ph.sample_ids, geno->samples());
     if (samples.empty()) die("no overlapping samples for GRM");
   }
   Engine eng(opt.device);
-  Grm g = compute_grm(eng, bed, samples, opt.miss, opt.max_miss, opt.maf);
+  Grm g = compute_grm(eng, *geno, samples, opt.miss, opt.max_miss, opt.maf);
   write_grm_gcta(opt.out, g);
   return 0;
 }
@@ -1088,10 +1119,11 @@ static bool all_finite(const PhenoData& ph, int col, const CovData& cov) {
 
 int run_eqtl(const Options& opt) {
   PhenoData ph = load_pheno(opt.pheno);
-  PlinkBed bed;
-  bed.open(opt.bfile);
+  std::unique_ptr<GenoSource> geno = open_geno(opt);
+  GenoSource& bed = *geno;
   std::vector<std::string> order = intersect_order(ph.sample_ids, bed.samples());
-  if (order.empty()) die("no overlapping samples between pheno and fam");
+  if (order.empty()) die(opt.use_bfile() ? "no overlapping samples between pheno and fam"
+                                         : "no overlapping samples between pheno and VCF");
   info("overlap samples: " + std::to_string(order.size()));
   slice_pheno(ph, order);
   CovData cov = load_covar(opt.covar, order);
@@ -1111,6 +1143,8 @@ int run_eqtl(const Options& opt) {
     probe.close();
     std::remove((opt.out + ".probe").c_str());
   }
+  if (std::find(scopes.begin(), scopes.end(), "cis") != scopes.end() && !bed.has_index())  // scan.cpp:93-99
+    die("cis mode requires an indexed genotype file; for VCF run: tabix -p vcf <file>.vcf.gz");
   if (have_gff) {
     size_t matched = 0;
     for (const auto& g : ph.gene_ids) matched += annot.count(g);
@@ -1118,7 +1152,6 @@ int run_eqtl(const Options& opt) {
     if (matched == 0) die("no phenotype gene IDs match the GFF annotation (check --gff-id-key and ID format)");
   }
   Engine eng(opt.device);
-  const auto col = sample_cols(bed, order);
   bool any_lmm = false;
   for (Model m : opt.models) any_lmm |= (m == Model::Lmm);
   if (any_lmm) {
@@ -1128,14 +1161,12 @@ int run_eqtl(const Options& opt) {
       info("computing GRM from genotypes (overlap samples)");
       grm = compute_grm(eng, bed, order, opt.miss, opt.max_miss, opt.maf);
     }
-    eng.check(eqtl_gpu_set_samples(eng.ctx(), static_cast<int32_t>(bed.n_file()), col.data(),
-                                   static_cast<int32_t>(col.size())), "set_samples");
+    set_sample_order(eng, bed, order);
     const int rc = eqtl_gpu_eigh(eng.ctx(), grm.K.data(), grm.n(), nullptr, nullptr);
     if (rc == EQTL_E_SOLVER) die("GRM eigen decomposition failed");
     eng.check(rc, "eigh");
   } else {
-    eng.check(eqtl_gpu_set_samples(eng.ctx(), static_cast<int32_t>(bed.n_file()), col.data(),
-                                   static_cast<int32_t>(col.size())), "set_samples");
+    set_sample_order(eng, bed, order);
   }
   eng.check(eqtl_gpu_set_policy(eng.ctx(), opt.miss == MissHand::Impute, opt.max_miss, opt.maf), "set_policy");
   for (Model model : opt.models) {

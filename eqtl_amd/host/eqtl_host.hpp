@@ -58,20 +58,47 @@ bool chrom_equal(const std::string& a, const std::string& b);
 std::vector<std::string> intersect_order(const std::vector<std::string>& prefer,
                                          const std::vector<std::string>& other);
 
-// ---- genotype source: PLINK bed/bim/fam (plink_bed.cpp:26-230) -------------
+// ---- genotype sources (plink_bed.hpp:48-73 concept) --------------------------
+// A site as the outputs name it: a1 = effect (ALT) allele, a2 = other (REF).
 struct BimSite {
   std::string chrom, id, a1, a2;
   int64_t pos = 0;
 };
 
-class PlinkBed {
+class Engine;
+
+// Samples, sites in file order, and blocks of genotype rows in the engine's
+// row format (EQTL_GENO_*) for the samples and policy already set on the
+// engine.
+class GenoSource {
+ public:
+  virtual ~GenoSource() = default;
+  virtual const std::vector<std::string>& samples() const = 0;
+  virtual const std::vector<BimSite>& sites() const = 0;
+  virtual int32_t row_format() const = 0;
+  virtual int32_t bytes_per_snp() const = 0;
+  // has_index (plink_bed.hpp:69, vcf_session.hpp): region queries possible
+  virtual bool has_index() const = 0;
+  // Rows [j0, j0 + m); *on_dev = 1 when the pointer is device memory. Valid
+  // until the next call.
+  virtual const uint8_t* rows(Engine& eng, size_t j0, size_t m, int32_t* on_dev) = 0;
+  // Rows of the listed SNPs, in list order.
+  virtual const uint8_t* gather(Engine& eng, const std::vector<int64_t>& idx, int32_t* on_dev) = 0;
+  size_t n_file() const { return samples().size(); }
+  std::vector<std::string> chromosomes() const;
+};
+
+// PLINK bed/bim/fam (plink_bed.cpp:26-230).
+class PlinkBed : public GenoSource {
  public:
   void open(const std::string& prefix);
-  const std::vector<std::string>& samples() const { return samples_; }
-  const std::vector<BimSite>& sites() const { return sites_; }
-  size_t n_file() const { return samples_.size(); }
-  int32_t bytes_per_snp() const { return bps_; }
-  std::vector<std::string> chromosomes() const;
+  const std::vector<std::string>& samples() const override { return samples_; }
+  const std::vector<BimSite>& sites() const override { return sites_; }
+  int32_t row_format() const override { return EQTL_GENO_BED; }
+  int32_t bytes_per_snp() const override { return bps_; }
+  bool has_index() const override { return true; }
+  const uint8_t* rows(Engine& eng, size_t j0, size_t m, int32_t* on_dev) override;
+  const uint8_t* gather(Engine& eng, const std::vector<int64_t>& idx, int32_t* on_dev) override;
   // Reads SNP rows [j0, j0+m) into buf (m * bytes_per_snp bytes).
   void read_rows(size_t j0, size_t m, std::vector<uint8_t>& buf);
 
@@ -81,7 +108,52 @@ class PlinkBed {
   std::vector<BimSite> sites_;
   int32_t bps_ = 0;
   std::unique_ptr<std::ifstream> bed_;
+  std::vector<uint8_t> buf_, row_;
 };
+
+// VCF (plain or gzip/BGZF-compressed) text (VcfSession, vcf_session.cpp:51-306).
+// The host inflates the file (BGZF blocks in parallel), reads the header and
+// the fixed columns of each record, and keeps the text; the engine decodes the
+// sample columns of a block of records on the device (eqtl_gpu_vcf_decode).
+class VcfSource : public GenoSource {
+ public:
+  void open(const std::string& path, int threads);
+  const std::vector<std::string>& samples() const override { return samples_; }
+  const std::vector<BimSite>& sites() const override { return sites_; }
+  int32_t row_format() const override { return EQTL_GENO_DOSE32; }
+  int32_t bytes_per_snp() const override { return static_cast<int32_t>(4 * samples_.size()); }
+  bool has_index() const override { return indexed_; }
+  const uint8_t* rows(Engine& eng, size_t j0, size_t m, int32_t* on_dev) override;
+  const uint8_t* gather(Engine& eng, const std::vector<int64_t>& idx, int32_t* on_dev) override;
+  size_t multi_allelic_skipped() const { return n_multi_skip_; }
+
+  struct Rec {
+    int64_t offset;  // sample columns in text_
+    int32_t length;
+    int16_t gt_key, ds_key;
+    int32_t haploid;
+  };
+  const std::vector<Rec>& records() const { return recs_; }
+  const std::string& text() const { return text_; }
+
+ private:
+  void parse_header_and_records();
+  std::string path_;
+  std::string text_;
+  std::vector<std::string> samples_;
+  std::vector<BimSite> sites_;
+  std::vector<Rec> recs_;
+  bool ds_float_ = false, indexed_ = false;
+  size_t n_multi_skip_ = 0;
+  std::string gather_text_;
+  std::vector<eqtl_vcf_record> desc_;
+};
+
+// is_haploid_chrom (vcf_session.cpp:149-158): X, Y, MT (any case, optional "chr").
+bool is_haploid_chrom(const std::string& chrom);
+// Decompressed bytes of a file: gzip members (BGZF blocks inflated on
+// `threads` threads) or the raw bytes when not gzip-compressed.
+std::string read_maybe_gzip(const std::string& path, int threads);
 
 // ---- phenotypes / covariates (pheno.cpp:17-196) ----------------------------
 struct PhenoData {
@@ -159,8 +231,8 @@ class Engine {
   eqtl_gpu_ctx* ctx_ = nullptr;
 };
 
-// compute_grm over the bed for the given sample order (grm.cpp:112-242).
-Grm compute_grm(Engine& eng, PlinkBed& bed, const std::vector<std::string>& sample_ids, MissHand miss,
+// compute_grm over the genotype source for the given sample order (grm.cpp:112-242).
+Grm compute_grm(Engine& eng, GenoSource& geno, const std::vector<std::string>& sample_ids, MissHand miss,
                 double max_miss, double maf);
 
 struct TransJob {
@@ -174,7 +246,7 @@ struct TransJob {
 
 // scan_lm_snp_outer / scan_lmm_snp_outer (scan_trans_lm.cpp:100-222,
 // scan_trans_lmm.cpp:82-251) for jobs that share the analysis sample set.
-void scan_snp_outer(const Options& opt, Engine& eng, PlinkBed& bed, Model model, const std::string& scope,
+void scan_snp_outer(const Options& opt, Engine& eng, GenoSource& bed, Model model, const std::string& scope,
                     ScopeOut& out, double pthr, std::vector<TransJob>& jobs, const PhenoData& ph,
                     const CovData& cov, const std::vector<double>* Q, const std::vector<double>* lambda);
 

@@ -7,8 +7,9 @@
  *
  * Layouts: Q n x n column-major (column c = eigenvector c), lambda ascending;
  * Y n x G column-major (gene k contiguous); X n x p column-major with the
- * intercept included by the caller; bed rows SNP-major, bytes_per_snp each
- * (PLINK-1 2-bit codes, no magic header).
+ * intercept included by the caller; genotype rows SNP-major, bytes_per_snp
+ * each (PLINK-1 2-bit codes without the magic header, or float32 dosages: see
+ * eqtl_gpu_set_geno_format).
  *
  * The reference (WWz33/eqtl, C++17) has no FFI; its seams are C++ functions.
  * Each entry point below names the reference function(s) whose work it takes
@@ -32,7 +33,8 @@ enum {
   EQTL_E_NOMEM = -3,    /* device allocation failed */
   EQTL_E_STATE = -4,    /* call order (e.g. scan before prep) */
   EQTL_E_SOLVER = -5,   /* eigensolver failure (make_lmm_basis dies) */
-  EQTL_E_CAPACITY = -6  /* hit buffer too small: *n_hits holds the count */
+  EQTL_E_CAPACITY = -6, /* hit buffer too small: *n_hits holds the count */
+  EQTL_E_FORMAT = -7    /* malformed input record (the message names it) */
 };
 
 enum { EQTL_MODEL_LM = 0, EQTL_MODEL_LMM = 1 };
@@ -93,6 +95,42 @@ int eqtl_gpu_set_snp_annot(eqtl_gpu_ctx* ctx, const int32_t* chrom, const int64_
  * not imputing), keep[m] (1 = SNP tested), af[m] (subset allele frequency). */
 int eqtl_gpu_decode_bed(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_device, int64_t m,
                         int32_t bytes_per_snp, double* dosage, uint8_t* keep, double* af);
+
+/* ---- genotype row formats ----------------------------------------------------
+ * Rows handed to eqtl_gpu_decode_bed, _grm_accumulate, _scan_block and
+ * _upload_bed are in the context's row format:
+ *   BED    PLINK-1 2-bit codes, bytes_per_snp >= ceil(n_file / 4) (default);
+ *   DOSE32 n_file float32 ALT dosages in file-sample order, NaN = missing call;
+ *          a row the genotype source rejected (VcfSession::parse_record false,
+ *          or the MAF gate of for_each_snp) is all NaN, so it is never tested.
+ *          bytes_per_snp >= 4 * n_file. eqtl_gpu_vcf_decode writes this format. */
+enum { EQTL_GENO_BED = 0, EQTL_GENO_DOSE32 = 1 };
+int eqtl_gpu_set_geno_format(eqtl_gpu_ctx* ctx, int32_t format);
+
+/* One VCF data line, located in a text buffer by the host (which parses the
+ * fixed columns CHROM..FORMAT itself and skips records with n_allele != 2). */
+typedef struct {
+  int64_t offset;  /* first byte of the first sample column */
+  int32_t length;  /* bytes of the sample columns, to the end of the line (no newline) */
+  int16_t gt_key;  /* index of GT among the record's FORMAT keys, -1 if absent */
+  int16_t ds_key;  /* index of DS among the FORMAT keys when the header declares
+                    * FORMAT/DS as Float, else -1; DS is then used for every sample */
+  int32_t haploid; /* 1: chromosome X, Y or MT (one allele copy per call in the AF) */
+  int32_t pad;
+} eqtl_vcf_record;
+
+/* VCF sample columns -> DOSE32 rows on the device (VcfSession::parse_record,
+ * vcf_session.cpp:160-286, then pass_maf, util.hpp:61-64, as for_each_snp,
+ * vcf_session.cpp:288-306, applies it) for the analysis samples and policy of
+ * eqtl_gpu_set_samples / _set_policy: GT (a call with a missing allele is
+ * missing; one allele = haploid call) or DS (float, '.' or non-finite =
+ * missing), miss_frac / AF (ALT count over the called allele copies) /
+ * Filter / MAF rules. text: host memory, or device memory readable for 4 bytes
+ * past text_len (text_on_device = 1). *rows_dev receives a device buffer of m
+ * rows of 4 * n_file bytes owned by the context and valid until the next call.
+ * EQTL_E_FORMAT if a record's sample column count differs from n_file. */
+int eqtl_gpu_vcf_decode(eqtl_gpu_ctx* ctx, const char* text, int32_t text_on_device, int64_t text_len,
+                        const eqtl_vcf_record* recs, int64_t m, const uint8_t** rows_dev);
 
 /* ---- GRM (compute_grm_from, grm.cpp:112-224) ----------------------------- */
 /* Accumulate m bed rows into the GRM; call repeatedly, then eqtl_gpu_grm_finish. */
@@ -178,7 +216,7 @@ int eqtl_gpu_beta_approx(eqtl_gpu_ctx* ctx, const double* perm_min_p, const doub
 
 /* ---- instrumentation ------------------------------------------------------- */
 /* When enabled, the library brackets each launch of the named kernel family
- * ("wald", "rotate", "decode", "reml") with HIP events on its stream. */
+ * ("wald", "rotate", "decode", "reml", "vcf") with HIP events on its stream. */
 int eqtl_gpu_set_timing(eqtl_gpu_ctx* ctx, int32_t enable);
 int eqtl_gpu_get_timing(eqtl_gpu_ctx* ctx, const char* family, double* total_ms, int64_t* launches);
 /* Synthetic PLINK rows generated on the device (benchmark inputs resident in

@@ -38,6 +38,15 @@ the cited reference code):
                      nmsimplex2 itself is absent from the image (the reference
                      links the system libgsl), so the simplex path is pinned
                      by optimality, not step-by-step.
+  vcf_record         the reference's own VCF fixtures (data/smoke.vcf.gz,
+                     tests/golden/smoke/): for every record the ALT dosage sum
+                     and the called allele copies equal the INFO AC and AN the
+                     producing tool wrote; hand-built lines with known answers
+                     (haploid chromosomes and calls, '.'-alleles, DS with '.',
+                     short columns, ALT '.' and multi-allelic skips, policies);
+                     diploid GT lines equal orc_decode_bed on the PLINK codes
+                     of the same calls. htslib (the reference's VCF reader) is
+                     absent: its float parse is taken as strtod -> float.
   cis_gene           composition of the above (LM/LMM perm bases restated from
                      scan_cis.hpp:74-160).
 """
@@ -181,6 +190,26 @@ def decode_bed(rows, n_file, sample_col=None, impute=True, max_miss=0.8, maf_min
                          ctypes.c_double(max_miss), ctypes.c_double(maf_min), _p(dos), _p(miss),
                          _p(keep), _p(af))
     return dos, miss, keep.astype(bool), af
+
+
+def vcf_record(line, n_file, sample_col=None, ds_float=False, impute=True, max_miss=0.8, maf_min=0.0):
+    """VcfSession::parse_record + pass_maf on one VCF data line (str or bytes,
+    no newline). Returns (status, dosage[n], af, (n_miss, sum, copies)) with
+    status -1 skipped (n_allele != 2), -2 column count != n_file, 0 rejected,
+    1 kept, and raw[n] (the dosage before imputation, NaN = missing)."""
+    if isinstance(line, str):
+        line = line.encode()
+    sc = np.arange(n_file, dtype=np.int32) if sample_col is None else np.ascontiguousarray(sample_col, np.int32)
+    n = sc.size
+    d = np.empty(n, np.float64)
+    af = ctypes.c_double()
+    st = np.zeros(3, np.float64)
+    raw = np.empty(n, np.float64)
+    rc = lib().orc_vcf_record(ctypes.c_char_p(line), ctypes.c_int64(len(line)), ctypes.c_int32(int(ds_float)),
+                              ctypes.c_int32(n_file), _p(sc), ctypes.c_int32(n), ctypes.c_int32(int(impute)),
+                              ctypes.c_double(max_miss), ctypes.c_double(maf_min), _p(d), ctypes.byref(af), _p(st),
+                              _p(raw))
+    return int(rc), d, af.value, (int(st[0]), float(st[1]), float(st[2])), raw
 
 
 def eigh(K):

@@ -820,6 +820,151 @@ void orc_decode_bed(const uint8_t* rows, int64_t m, int32_t bytes_per_snp, int32
   }
 }
 
+// ---- VCF record (VcfSession::parse_record, vcf_session.cpp:160-286, and the
+// pass_maf gate of for_each_snp, :288-306) over the VCF text of one data line.
+// The reference reads the line through htslib (absent from the image); the
+// field semantics restated here are those of the VCF 4.x text format as
+// htslib exposes them to parse_record: GT alleles separated by '/' or '|',
+// '.' a missing allele (bcf_gt_missing), a sample with fewer alleles than
+// the record's maximum padded with vector_end; FORMAT/DS (used when the header
+// declares it as Float and the record carries it) a float per sample, '.'
+// missing; a sample column with fewer ':' subfields than FORMAT keys has the
+// trailing ones missing; a record with n_allele != 2 is skipped before
+// parsing (multi-allelic, or ALT '.').
+static std::vector<std::string> split_on(const char* s, size_t len, char c) {
+  std::vector<std::string> out;
+  size_t a = 0;
+  for (size_t i = 0; i <= len; ++i)
+    if (i == len || s[i] == c) {
+      out.emplace_back(s + a, i - a);
+      a = i + 1;
+    }
+  return out;
+}
+
+static bool haploid_chrom(std::string k) {  // vcf_session.cpp:149-158
+  if (k.size() > 3 && (k[0] == 'c' || k[0] == 'C') && (k[1] == 'h' || k[1] == 'H') && (k[2] == 'r' || k[2] == 'R'))
+    k = k.substr(3);
+  return k == "X" || k == "x" || k == "Y" || k == "y" || k == "M" || k == "m" || k == "MT" || k == "mt" || k == "Mt";
+}
+
+// Returns -1 (skipped: n_allele != 2), -2 (sample column count differs from
+// n_file: a read error in the reference), 0 (parse_record or pass_maf false)
+// or 1 (kept). dosage[n] (analysis order): the SnpRec dosage — NaN for a missing
+// call, replaced by the non-missing mean when impute and kept; af_out: out.maf;
+// stats[3] (nullable): n_miss, ALT dosage sum, called allele copies; raw[n]
+// (nullable): the dosage before imputation.
+int32_t orc_vcf_record(const char* line, int64_t len, int32_t ds_float, int32_t n_file, const int32_t* sample_col,
+                       int32_t n, int32_t impute, double max_miss, double maf_min, double* dosage, double* af_out,
+                       double* stats, double* raw) {
+  const auto cols = split_on(line, static_cast<size_t>(len), '\t');
+  *af_out = kNaN;
+  for (int i = 0; i < n; ++i) dosage[i] = kNaN;
+  if (cols.size() < 8) return 0;
+  const std::string& alt = cols[4];
+  int n_allele = 1;
+  if (alt != ".") n_allele = 2 + static_cast<int>(std::count(alt.begin(), alt.end(), ','));
+  if (n_allele != 2) return -1;
+  if (n <= 0 || n_file <= 0) return 0;
+  if (static_cast<int>(cols.size()) != 9 + n_file) return -2;  // htslib: column count differs from the header
+  const bool hap = haploid_chrom(cols[0]);
+  const auto keys = split_on(cols[8].data(), cols[8].size(), ':');
+  int gt_key = -1, ds_key = -1;
+  for (size_t k = 0; k < keys.size(); ++k) {
+    if (keys[k] == "GT" && gt_key < 0) gt_key = static_cast<int>(k);
+    if (keys[k] == "DS" && ds_float && ds_key < 0) ds_key = static_cast<int>(k);
+  }
+  auto subfield = [&](int col, int key, std::string& out) -> bool {
+    const auto sub = split_on(cols[9 + col].data(), cols[9 + col].size(), ':');
+    if (key >= static_cast<int>(sub.size())) return false;
+    out = sub[key];
+    return true;
+  };
+  int n_miss = 0, n_ok = 0;
+  double sum = 0.0, ploidy_sum = 0.0;
+  std::string f;
+  if (ds_key >= 0) {  // vcf_session.cpp:188-221
+    for (int i = 0; i < n; ++i) {
+      float v = std::numeric_limits<float>::quiet_NaN();
+      if (subfield(sample_col[i], ds_key, f) && !f.empty() &&
+          !(f[0] == '.' && !(f.size() > 1 && f[1] >= '0' && f[1] <= '9'))) {
+        char* e = nullptr;
+        const double d = std::strtod(f.c_str(), &e);
+        if (e != f.c_str()) v = static_cast<float>(d);
+      }
+      if (!std::isfinite(v)) {
+        ++n_miss;
+        continue;
+      }
+      dosage[i] = static_cast<double>(v);
+      sum += dosage[i];
+      ploidy_sum += hap ? 1.0 : 2.0;
+      ++n_ok;
+    }
+  } else {  // vcf_session.cpp:223-260
+    if (gt_key < 0) return 0;
+    for (int i = 0; i < n; ++i) {
+      std::vector<int> al;       // allele indices, -1 = '.'
+      if (subfield(sample_col[i], gt_key, f)) {
+        size_t q = 0;
+        while (q < f.size()) {
+          const char c = f[q];
+          if (c == '/' || c == '|') { ++q; continue; }
+          if (c == '.') { al.push_back(-1); ++q; continue; }
+          if (c >= '0' && c <= '9') {
+            int v = 0;
+            while (q < f.size() && f[q] >= '0' && f[q] <= '9') {
+              if (v < 10000000) v = v * 10 + (f[q] - '0');
+              ++q;
+            }
+            al.push_back(v);
+            continue;
+          }
+          ++q;
+        }
+      }
+      const bool miss_gt = al.empty() || al[0] < 0 || (al.size() > 1 && al[1] < 0);
+      if (miss_gt) {
+        ++n_miss;
+        continue;
+      }
+      int d = al[0], n_alleles = 1;
+      if (al.size() > 1) {
+        d += al[1];
+        ++n_alleles;
+      }
+      const bool this_hap = hap || n_alleles == 1;
+      dosage[i] = static_cast<double>(d);
+      sum += d;
+      ploidy_sum += this_hap ? 1.0 : 2.0;
+      ++n_ok;
+    }
+  }
+  if (raw)
+    for (int i = 0; i < n; ++i) raw[i] = dosage[i];
+  if (stats) {
+    stats[0] = n_miss;
+    stats[1] = sum;
+    stats[2] = ploidy_sum;
+  }
+  if (n_ok == 0) return 0;
+  const double miss_frac = static_cast<double>(n_miss) / static_cast<double>(n);
+  if (miss_frac > max_miss + 1e-15) return 0;
+  double af = ploidy_sum > 0 ? sum / ploidy_sum : 0.0;
+  if (af < 0) af = 0;
+  if (af > 1) af = 1;
+  if (af < 1e-12 || af > 1.0 - 1e-12) return 0;
+  if (!impute && n_miss > 0) return 0;
+  if (n_miss > 0 && impute) {
+    const double mu = sum / n_ok;
+    for (int i = 0; i < n; ++i)
+      if (!std::isfinite(dosage[i])) dosage[i] = mu;
+  }
+  if (maf_min > 0.0 && (af < maf_min || af > 1.0 - maf_min)) return 0;  // pass_maf, util.hpp:61-64
+  *af_out = af;
+  return 1;
+}
+
 void orc_eigh(const double* K, int32_t n, double* Q, double* lam) { jacobi_eigh(K, n, Q, lam); }
 
 double orc_reml_negll(double delta, const double* yt, const double* Xt, const double* lam,

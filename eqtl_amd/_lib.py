@@ -138,13 +138,19 @@ class Context:
         _check(lib().eqtl_gpu_set_geno_format(self._c, ctypes.c_int32(fmt)))
         self.bps = 4 * self.n_file if fmt == GENO_DOSE32 else (self.n_file + 3) // 4
 
-    def vcf_decode(self, text: bytes, recs):
+    def vcf_decode(self, text, recs, text_dev=None, text_len=None):
         """Sample columns of VCF records (VCF_RECORD_DTYPE) -> device DOSE32 rows.
-        Returns the device pointer (owned by the context, valid until the next call)."""
+        text: host bytes, or text_dev (device pointer, 4-byte aligned, readable 4
+        bytes past text_len). Returns the device pointer of the rows (owned by the
+        context, valid until the next call)."""
         recs = np.ascontiguousarray(recs, dtype=VCF_RECORD_DTYPE)
         ptr = ctypes.c_void_p()
-        _check(lib().eqtl_gpu_vcf_decode(self._c, ctypes.c_char_p(text), ctypes.c_int32(0), ctypes.c_int64(len(text)),
-                                         _p(recs), ctypes.c_int64(recs.size), ctypes.byref(ptr)))
+        if text_dev is not None:
+            src, on_dev, n = ctypes.c_void_p(text_dev), 1, text_len
+        else:
+            src, on_dev, n = ctypes.c_char_p(text), 0, len(text)
+        _check(lib().eqtl_gpu_vcf_decode(self._c, src, ctypes.c_int32(on_dev), ctypes.c_int64(n), _p(recs),
+                                         ctypes.c_int64(recs.size), ctypes.byref(ptr)))
         return ptr.value
 
     def decode_bed(self, rows, want_dosage=True):

@@ -118,6 +118,7 @@ struct eqtl_gpu_ctx {
   int n_file = 0, n = 0, n_pad = 0;
   DBuf sample_col;
   DBuf col_used;  // n_file: how many analysis samples read each file column
+  bool col_used_all = false;  // every file column read exactly once
   int geno_fmt = EQTL_GENO_BED;
   int impute = 1;
   double max_miss = 0.8, maf_min = 0.0;
@@ -412,6 +413,7 @@ int eqtl_gpu_set_samples(eqtl_gpu_ctx* c, int32_t n_file, const int32_t* sample_
     if (used[sample_col[i]] == 255) return fail(EQTL_E_ARG, "a file column is repeated more than 255 times");
     ++used[sample_col[i]];
   }
+  c->col_used_all = std::all_of(used.begin(), used.end(), [](uint8_t u) { return u == 1; });
   CHK(ensure(c->sample_col, sizeof(int32_t) * n));
   CHK(ensure(c->col_used, static_cast<size_t>(n_file)));
   HIPC(hipMemcpyAsync(c->sample_col.p, sample_col, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
@@ -432,12 +434,11 @@ int eqtl_gpu_vcf_decode(eqtl_gpu_ctx* c, const char* text, int32_t on_dev, int64
   if (!c || !rows_dev || m <= 0 || !recs || !text || text_len < 0) return fail(EQTL_E_ARG, "bad vcf_decode arguments");
   if (c->n <= 0 || c->n_file <= 0) return fail(EQTL_E_STATE, "set_samples first");
   if (m > 0x7fffffff) return fail(EQTL_E_ARG, "too many records in one call");
-  int64_t lmax = 0;
+  if (on_dev && (reinterpret_cast<uintptr_t>(text) & 3)) return fail(EQTL_E_ARG, "device text must be 4-byte aligned");
   for (int64_t r = 0; r < m; ++r) {
     const eqtl_vcf_record& e = recs[r];
     if (e.offset < 0 || e.length < 0 || e.offset + e.length > text_len || e.gt_key < -1 || e.ds_key < -1)
       return fail(EQTL_E_ARG, "VCF record " + std::to_string(r) + ": span outside the text or bad key index");
-    lmax = std::max<int64_t>(lmax, e.length);
   }
   CHK(set_device(c));
   const unsigned char* td = reinterpret_cast<const unsigned char*>(text);
@@ -460,20 +461,15 @@ int eqtl_gpu_vcf_decode(eqtl_gpu_ctx* c, const char* text, int32_t on_dev, int64
   a.n_file = c->n_file;
   a.n = c->n;
   a.used = c->col_used.as<uint8_t>();
+  a.used_all = c->col_used_all ? 1 : 0;
   a.impute = c->impute;
   a.max_miss = c->max_miss;
   a.maf_min = c->maf_min;
   a.rows = c->vcf_rows.as<float>();
   a.bad = c->vcf_bad.as<unsigned long long>();
-  // a record's dword-aligned span is at most length + 6 bytes
-  const int64_t span = round_up(lmax + 8, 16);
   Timed t(c, "vcf");
-  if (span <= kVcfLdsMax) {
-    hipLaunchKernelGGL(k_vcf_decode<true>, dim3(static_cast<unsigned>(m)), dim3(kVcfThreads),
-                       static_cast<size_t>(span), c->stream, a);
-  } else {
-    hipLaunchKernelGGL(k_vcf_decode<false>, dim3(static_cast<unsigned>(m)), dim3(kVcfThreads), 0, c->stream, a);
-  }
+  hipLaunchKernelGGL(k_vcf_decode, dim3(static_cast<unsigned>((m + kVcfWaves - 1) / kVcfWaves)), dim3(kVcfThreads), 0,
+                     c->stream, a);
   CHK(check_launch("k_vcf_decode"));
   CHK(t.done());
   unsigned long long bad = none;

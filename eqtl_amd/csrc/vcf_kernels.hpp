@@ -1,14 +1,18 @@
 // VCF text -> DOSE32 genotype rows on the device.
-//   k_vcf_decode — one workgroup per record. The record's sample columns are
-//     staged in LDS with coalesced dword loads when they fit; each thread
-//     counts the tabs of a contiguous chunk, a workgroup prefix sum gives every
-//     thread the sample index of the columns that start in its chunk, and the
-//     thread parses the GT or DS subfield of those columns
-//     (VcfSession::parse_record, vcf_session.cpp:160-286): it writes the float
-//     ALT dosage (NaN = missing) and accumulates, over the analysis samples,
-//     the missing count, the dosage sum and the called allele copies. The
-//     record-level rules (n_ok, miss_frac, AF in (1e-12, 1-1e-12), Filter,
-//     pass_maf) then decide the row; a rejected row is rewritten as all NaN.
+//   k_vcf_decode — one wave per record. The wave walks the record's sample
+//     columns 256 bytes at a time: lane l holds dword l of the step (coalesced
+//     loads, prefetched two steps ahead), counts its tabs, and a wave prefix
+//     sum over the lanes gives the sample index of every column that starts
+//     in its dword. The lane decodes those columns' GT or DS subfield
+//     (VcfSession::parse_record, vcf_session.cpp:160-286) from a 16-byte
+//     register window (its dword and the next three lanes' dwords, through
+//     shuffles): branch-free for the common GT / GT:DS forms, byte by byte
+//     otherwise, and from the line in memory for a subfield that runs past
+//     the window. It writes the float ALT dosage (NaN = missing) and
+//     accumulates, over the analysis samples, the missing count, the dosage
+//     sum and the called allele copies. The record-level rules (n_ok,
+//     miss_frac, AF in (1e-12, 1-1e-12), Filter, pass_maf) then decide the
+//     row; a rejected row is rewritten as all NaN. No LDS, no barriers.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -19,7 +23,7 @@
 namespace eqtl_amd {
 
 constexpr int kVcfThreads = 256;
-constexpr int kVcfLdsMax = 60 * 1024;  // largest staged record span (bytes), within the default 64 KiB
+constexpr int kVcfWaves = kVcfThreads / 64;  // records per workgroup
 
 struct VcfArgs {
   const unsigned char* text;
@@ -27,84 +31,98 @@ struct VcfArgs {
   int64_t m;
   int32_t n_file, n;
   const uint8_t* used;  // n_file: number of analysis samples reading the column
+  int32_t used_all;     // every column read exactly once (used[] is all ones)
   int32_t impute;
   double max_miss, maf_min;
   float* rows;                 // m x n_file
   unsigned long long* bad;     // lowest malformed record (initialised to m)
 };
 
-template <bool STAGED>
+// The general parse of a column (the uncommon forms): from the register
+// window, else from the line in memory. Kept out of line so that the main
+// loop's registers stay few.
+__device__ __attribute__((noinline)) VcfCall vcf_column_slow(const unsigned char* line, int L, uint64_t lo,
+                                                             uint64_t hi, int o, int wbase, int gt_key, int ds_key,
+                                                             bool hap) {
+  VcfCall c;
+  if (parse_sample_window(lo, hi, o, L - wbase, gt_key, ds_key, hap, c)) return c;
+  auto at_line = [&](int q) -> unsigned { return line[q]; };
+  return parse_sample(at_line, wbase + o, L, gt_key, ds_key, hap);
+}
+
 __global__ __launch_bounds__(kVcfThreads) void k_vcf_decode(VcfArgs a) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char vsm[];
-  __shared__ int s_tabs[kVcfThreads / 64];
-  __shared__ double s_sum[kVcfThreads / 64];
-  __shared__ int s_cnt[2][kVcfThreads / 64];
-  __shared__ int s_keep;
-  const int64_t r = blockIdx.x;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lane = threadIdx.x & 63;
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * kVcfWaves + (threadIdx.x >> 6);
+  if (r >= a.m) return;  // wave-uniform
   const eqtl_vcf_record rec = a.recs[r];
   const int L = rec.length;
   const unsigned char* line = a.text + rec.offset;
   const int s0 = static_cast<int>(reinterpret_cast<uintptr_t>(line) & 3);
   const unsigned* gw = reinterpret_cast<const unsigned*>(line - s0);
   const int D = (s0 + L + 3) >> 2;  // dwords of the aligned span
-  const unsigned* lw = reinterpret_cast<const unsigned*>(vsm);
-  if constexpr (STAGED) {
-    unsigned* dst = reinterpret_cast<unsigned*>(vsm);
-    for (int d = tid; d < D; d += kVcfThreads) dst[d] = gw[d];
-    __syncthreads();
-  }
-  auto word = [&](int d) -> unsigned { return STAGED ? lw[d] : gw[d]; };
-  auto at = [&](int q) -> unsigned { return STAGED ? vsm[s0 + q] : line[q]; };
-
-  // pass 1: tabs per chunk, workgroup exclusive prefix
-  const int per = (D + kVcfThreads - 1) / kVcfThreads;
-  const int d0 = min(D, tid * per), d1 = min(D, d0 + per);
-  int cnt = 0;
-  for (int d = d0; d < d1; ++d) cnt += __popc(tab_mask(word(d)) & line_mask(d, s0, L));
-  int incl = cnt;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int t = __shfl_up(incl, o);
-    if (lane >= o) incl += t;
-  }
-  if (lane == 63) s_tabs[wave] = incl;
-  __syncthreads();
-  int before = incl - cnt, total = 0;
-#pragma unroll
-  for (int w = 0; w < kVcfThreads / 64; ++w) {
-    if (w < wave) before += s_tabs[w];
-    total += s_tabs[w];
-  }
-  if (total != a.n_file - 1) {
-    if (tid == 0) atomicMin(a.bad, static_cast<unsigned long long>(r));
-    return;  // uniform over the workgroup
-  }
-
-  // pass 2: parse the columns that start in this chunk
-  float* row = a.rows + r * static_cast<int64_t>(a.n_file);
+  const int n_file = a.n_file;
+  float* row = a.rows + r * static_cast<int64_t>(n_file);
   const bool hap = rec.haploid != 0;
+  const int gt_key = rec.gt_key, ds_key = rec.ds_key;
   int n_miss = 0, copies = 0;
   double sum = 0.0;
-  auto take = [&](int k, int q) {
-    const VcfCall c = parse_sample(at, q, L, rec.gt_key, rec.ds_key, hap);
+  bool over = false;
+  // column k starting at byte o (0..4) of the window that begins at line position wbase
+  auto take = [&](int k, int o, int wbase, uint64_t lo, uint64_t hi) {
+    if (k >= n_file) {
+      over = true;
+      return;
+    }
+    VcfCall c;
+    const int sh = 8 * o;  // o <= 4: 16 - o >= 12 window bytes from the column start
+    const uint64_t x0 = o == 0 ? lo : ((lo >> sh) | (hi << (64 - sh))), x1 = o == 0 ? hi : (hi >> sh);
+    if (!parse_sample_fast(x0, x1, 16 - o, L - wbase - o, gt_key, ds_key, hap, c))
+      c = vcf_column_slow(line, L, lo, hi, o, wbase, gt_key, ds_key, hap);
     row[k] = c.miss ? __builtin_nanf("") : c.dose;
-    const int u = a.used[k];
+    const int u = a.used_all ? 1 : a.used[k];
     if (u) {
       if (c.miss) n_miss += u;
       else { sum += u * static_cast<double>(c.dose); copies += u * c.copies; }
     }
   };
-  if (tid == 0) take(0, 0);
-  int k = before;
-  for (int d = d0; d < d1; ++d) {
-    unsigned mk = tab_mask(word(d)) & line_mask(d, s0, L);
+  int carry = 0;  // tabs before this step (wave-uniform)
+  unsigned cur = lane < D ? gw[lane] : 0u;
+  unsigned nxt = lane + 64 < D ? gw[lane + 64] : 0u;
+  const int d_end = D > 0 ? D : 1;  // an empty line still has its one (empty) column
+  for (int d0 = 0; d0 < d_end; d0 += 64) {
+    const int d = d0 + lane;
+    const unsigned nx2 = d + 128 < D ? gw[d + 128] : 0u;  // two steps ahead
+    unsigned mk = d < D ? (tab_mask(cur) & line_mask(d, s0, L)) : 0u;
+    const int c = __popc(mk);
+    int incl = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o);
+      if (lane >= o) incl += t;
+    }
+    // 16-byte window: this lane's dword and the next three (the next step's
+    // dwords past lane 63)
+    const unsigned c1 = __shfl(cur, (lane + 1) & 63), n1 = __shfl(nxt, (lane + 1) & 63);
+    const unsigned c2 = __shfl(cur, (lane + 2) & 63), n2 = __shfl(nxt, (lane + 2) & 63);
+    const unsigned c3 = __shfl(cur, (lane + 3) & 63), n3 = __shfl(nxt, (lane + 3) & 63);
+    const unsigned w1 = lane + 1 < 64 ? c1 : n1, w2 = lane + 2 < 64 ? c2 : n2, w3 = lane + 3 < 64 ? c3 : n3;
+    const uint64_t lo = cur | (static_cast<uint64_t>(w1) << 32), hi = w2 | (static_cast<uint64_t>(w3) << 32);
+    const int wbase = 4 * d - s0;
+    int k = carry + incl - c;  // tabs before this dword
+    if (d == 0) take(0, s0, wbase, lo, hi);  // the first column
     while (mk) {
       const int b = __builtin_ctz(mk);
       mk &= mk - 1;
       ++k;
-      take(k, 4 * d - s0 + b + 1);
+      take(k, b + 1, wbase, lo, hi);
     }
+    carry += __shfl(incl, 63);
+    cur = nxt;
+    nxt = nx2;
+  }
+  if (carry != n_file - 1 || __any(over)) {
+    if (lane == 0) atomicMin(a.bad, static_cast<unsigned long long>(r));
+    return;
   }
   // record-level rules over the analysis samples
 #pragma unroll
@@ -113,33 +131,18 @@ __global__ __launch_bounds__(kVcfThreads) void k_vcf_decode(VcfArgs a) {
     copies += __shfl_xor(copies, o);
     sum += __shfl_xor(sum, o);
   }
-  if (lane == 0) {
-    s_cnt[0][wave] = n_miss;
-    s_cnt[1][wave] = copies;
-    s_sum[wave] = sum;
+  const int n_ok = a.n - n_miss;
+  bool keep = n_ok > 0;
+  if (keep && static_cast<double>(n_miss) / static_cast<double>(a.n) > a.max_miss + 1e-15) keep = false;
+  double af = copies > 0 ? sum / static_cast<double>(copies) : 0.0;
+  af = fmin(1.0, fmax(0.0, af));
+  if (af < 1e-12 || af > 1.0 - 1e-12) keep = false;
+  if (!a.impute && n_miss > 0) keep = false;
+  if (a.maf_min > 0.0 && (af < a.maf_min || af > 1.0 - a.maf_min)) keep = false;  // pass_maf
+  if (!keep) {
+    __threadfence_block();  // the column stores above complete before the rewrite
+    for (int i = lane; i < n_file; i += 64) row[i] = __builtin_nanf("");
   }
-  __syncthreads();
-  if (tid == 0) {
-    int nm = 0, cp = 0;
-    double sm = 0.0;
-    for (int w = 0; w < kVcfThreads / 64; ++w) {
-      nm += s_cnt[0][w];
-      cp += s_cnt[1][w];
-      sm += s_sum[w];
-    }
-    const int n_ok = a.n - nm;
-    bool keep = n_ok > 0;
-    if (keep && static_cast<double>(nm) / static_cast<double>(a.n) > a.max_miss + 1e-15) keep = false;
-    double af = cp > 0 ? sm / static_cast<double>(cp) : 0.0;
-    af = fmin(1.0, fmax(0.0, af));
-    if (af < 1e-12 || af > 1.0 - 1e-12) keep = false;
-    if (!a.impute && nm > 0) keep = false;
-    if (a.maf_min > 0.0 && (af < a.maf_min || af > 1.0 - a.maf_min)) keep = false;  // pass_maf
-    s_keep = keep ? 1 : 0;
-  }
-  __syncthreads();  // also orders this workgroup's row stores before the rewrite
-  if (!s_keep)
-    for (int i = tid; i < a.n_file; i += kVcfThreads) row[i] = __builtin_nanf("");
 }
 
 }  // namespace eqtl_amd

@@ -109,35 +109,52 @@ struct VcfCall {
   int copies;  // allele copies the call adds to the AF denominator
 };
 
-// One sample column starting at line position q: its GT or DS subfield.
+// Extent [qs, qe) of the key-th ':'-separated subfield of the sample column
+// starting at q, scanning no further than lim (lim_is_end: lim is the end of
+// the line). Returns 1 found, 0 absent (the column ends first: a missing
+// value), -1 undecided (lim was reached inside the column).
 template <class Byte>
-EQ_HD VcfCall parse_sample(const Byte& at, int q, int L, int gt_key, int ds_key, bool hap) {
-  VcfCall r{0.f, true, 0};
-  const bool use_ds = ds_key >= 0;
-  const int key = use_ds ? ds_key : gt_key;
-  if (key < 0) return r;
-  for (int f = 0; f < key; ++f) {  // skip to the key-th ':'-separated subfield
-    while (q < L && at(q) != ':' && at(q) != '\t') ++q;
-    if (q >= L || at(q) == '\t') return r;  // the column has fewer subfields: missing
+EQ_HD int locate_subfield(const Byte& at, int q, int lim, bool lim_is_end, int key, int& qs, int& qe) {
+  for (int f = 0; f < key; ++f) {
+    while (q < lim && at(q) != ':' && at(q) != '\t') ++q;
+    if (q >= lim) return lim_is_end ? 0 : -1;
+    if (at(q) == '\t') return 0;
     ++q;
   }
-  if (use_ds) {
-    if (q >= L || at(q) == ':' || at(q) == '\t') return r;
-    if (at(q) == '.' && !(q + 1 < L && is_digit(at(q + 1)))) return r;
-    bool ok;
-    const float v = parse_decimal(at, q, L, ok);
-    if (!ok || !isfinite(v)) return r;
-    r.dose = v;
-    r.miss = false;
-    r.copies = hap ? 1 : 2;
-    return r;
-  }
-  // GT: allele tokens separated by '/' or '|'; '.' = missing allele
+  qs = q;
+  while (q < lim && at(q) != ':' && at(q) != '\t') ++q;
+  if (q >= lim && !lim_is_end) return -1;
+  qe = q;
+  return 1;
+}
+
+// DS subfield [q, e): a float; '.' (not followed by a digit), empty, no
+// digits or non-finite = missing.
+template <class Byte>
+EQ_HD VcfCall parse_ds(const Byte& at, int q, int e, bool hap) {
+  VcfCall r{0.f, true, 0};
+  if (q >= e) return r;
+  if (at(q) == '.' && !(q + 1 < e && is_digit(at(q + 1)))) return r;
+  bool ok;
+  const float v = parse_decimal(at, q, e, ok);
+  if (!ok || !isfinite(v)) return r;
+  r.dose = v;
+  r.miss = false;
+  r.copies = hap ? 1 : 2;
+  return r;
+}
+
+// GT subfield [q, e): allele tokens separated by '/' or '|', '.' = missing
+// allele. Missing call when there is no allele or either of the first two is
+// missing; dosage = sum of the first two allele indices; a single allele is a
+// haploid call (one copy in the AF denominator).
+template <class Byte>
+EQ_HD VcfCall parse_gt(const Byte& at, int q, int e, bool hap) {
+  VcfCall r{0.f, true, 0};
   int na = 0, a0 = 0, a1 = 0;
   bool m0 = false, m1 = false;
-  while (q < L) {
+  while (q < e) {
     const unsigned c = at(q);
-    if (c == ':' || c == '\t') break;
     if (c == '/' || c == '|') { ++q; continue; }
     bool mis = false;
     int val = 0;
@@ -145,7 +162,7 @@ EQ_HD VcfCall parse_sample(const Byte& at, int q, int L, int gt_key, int ds_key,
       mis = true;
       ++q;
     } else if (is_digit(c)) {
-      while (q < L && is_digit(at(q))) {
+      while (q < e && is_digit(at(q))) {
         if (val < 10000000) val = val * 10 + (at(q) - '0');
         ++q;
       }
@@ -162,6 +179,111 @@ EQ_HD VcfCall parse_sample(const Byte& at, int q, int L, int gt_key, int ds_key,
   r.dose = static_cast<float>(na >= 2 ? a0 + a1 : a0);
   r.copies = (hap || na == 1) ? 1 : 2;
   return r;
+}
+
+// One sample column starting at line position q (line length L): its DS
+// subfield when ds_key >= 0, else its GT subfield.
+template <class Byte>
+EQ_HD VcfCall parse_sample(const Byte& at, int q, int L, int gt_key, int ds_key, bool hap) {
+  VcfCall r{0.f, true, 0};
+  const int key = ds_key >= 0 ? ds_key : gt_key;
+  if (key < 0) return r;
+  int qs = 0, qe = 0;
+  if (locate_subfield(at, q, L, true, key, qs, qe) != 1) return r;
+  return ds_key >= 0 ? parse_ds(at, qs, qe, hap) : parse_gt(at, qs, qe, hap);
+}
+
+// The sample column starting at byte o of a 16-byte window of the line (two
+// little-endian 64-bit words lo, hi; room = line bytes from the window's first
+// byte on). Returns false when its subfield runs past the window: the caller
+// then parses the column from the line itself.
+EQ_HD bool parse_sample_window(uint64_t lo, uint64_t hi, int o, int room, int gt_key, int ds_key, bool hap,
+                               VcfCall& out) {
+  out = VcfCall{0.f, true, 0};
+  const int key = ds_key >= 0 ? ds_key : gt_key;
+  if (key < 0) return true;
+  auto at = [&](int j) -> unsigned {
+    return static_cast<unsigned>((j < 8 ? lo >> (8 * j) : hi >> (8 * (j - 8))) & 0xffu);
+  };
+  const int lim = room < 16 ? room : 16;
+  int qs = 0, qe = 0;
+  const int st = locate_subfield(at, o, lim, room <= 16, key, qs, qe);
+  if (st < 0) return false;
+  if (st == 1) out = ds_key >= 0 ? parse_ds(at, qs, qe, hap) : parse_gt(at, qs, qe, hap);
+  return true;
+}
+
+// Branch-free decode of the commonest columns, GT first in FORMAT: a diploid
+// call of one-digit alleles ("a|b", "a/b", '.' alleles) or a haploid one ("a"),
+// and, when DS is the second key, a DS value of at most nine digits and one
+// '.' ("0.123", "1", ".5") right after such a call. x0, x1 hold 16 bytes from the
+// column start, of which win are window bytes and avail are on the line.
+// Returns false when the column has another form or does not end inside the
+// window (the caller then uses the general parse); otherwise the result is the
+// general parse's for the same column.
+EQ_HD bool parse_sample_fast(uint64_t x0, uint64_t x1, int win, int avail, int gt_key, int ds_key, bool hap,
+                             VcfCall& out) {
+  if (gt_key != 0 || (ds_key >= 0 && ds_key != 1)) return false;
+  auto byte = [&](int i) -> unsigned {
+    return static_cast<unsigned>(((i < 8) ? (x0 >> (8 * i)) : (x1 >> (8 * (i - 8)))) & 0xffu);
+  };
+  // 1: terminator (':', tab or end of line), 0: other byte, -1: past the window
+  auto term = [&](int i) -> int {
+    if (i >= avail) return 1;
+    if (i >= win) return -1;
+    const unsigned ch = byte(i);
+    return (ch == ':' || ch == '\t') ? 1 : 0;
+  };
+  const unsigned b0 = byte(0), b1 = byte(1), b2 = byte(2);
+  const bool al0 = avail >= 1 && (is_digit(b0) || b0 == '.');
+  const bool al2 = avail >= 3 && (is_digit(b2) || b2 == '.');
+  const bool dip = al0 && avail >= 2 && (b1 == '/' || b1 == '|') && al2 && term(3) == 1;
+  const bool hpl = al0 && term(1) == 1;
+  if (!dip && !hpl) return false;
+  const int gt_len = dip ? 3 : 1;
+  if (ds_key < 0) {
+    out.miss = b0 == '.' || (dip && b2 == '.');
+    out.dose = out.miss ? 0.f : static_cast<float>(static_cast<int>(b0) - '0' + (dip ? static_cast<int>(b2) - '0' : 0));
+    out.copies = out.miss ? 0 : ((hap || !dip) ? 1 : 2);
+    return true;
+  }
+  // DS after "GT:"; the column must continue with ':'
+  if (gt_len >= avail || byte(gt_len) != ':') return false;
+  const int q0 = gt_len + 1;
+  unsigned M = 0;  // at most 9 digits: exact in 32 bits and in a double
+  int frac = 0, nd = 0;
+  bool dot = false, open = true, bad = false;
+#pragma unroll
+  for (int j = 0; j < 11; ++j) {  // nine digits, a '.', the terminator
+    const int i = q0 + j;
+    if (open) {
+      const int t = term(i);
+      if (t != 0) {
+        open = false;
+        bad |= t < 0;  // ran past the window
+      } else {
+        const unsigned ch = byte(i);
+        if (is_digit(ch) && nd < 9) {
+          M = M * 10u + (ch - '0');
+          ++nd;
+          if (dot) ++frac;
+        } else if (ch == '.' && !dot) {
+          dot = true;
+        } else {
+          bad = true;
+        }
+      }
+    }
+  }
+  if (open || bad) return false;
+  if (nd == 0) {  // empty, or '.' alone: missing
+    out = VcfCall{0.f, true, 0};
+    return true;
+  }
+  out.miss = false;
+  out.copies = hap ? 1 : 2;
+  out.dose = static_cast<float>(static_cast<double>(M) / exact_p10(frac));
+  return true;
 }
 
 }  // namespace eqtl_amd

@@ -125,8 +125,8 @@ typedef struct {
  * eqtl_gpu_set_samples / _set_policy: GT (a call with a missing allele is
  * missing; one allele = haploid call) or DS (float, '.' or non-finite =
  * missing), miss_frac / AF (ALT count over the called allele copies) /
- * Filter / MAF rules. text: host memory, or device memory readable for 4 bytes
- * past text_len (text_on_device = 1). *rows_dev receives a device buffer of m
+ * Filter / MAF rules. text: host memory, or 4-byte aligned device memory
+ * readable for 4 bytes past text_len (text_on_device = 1). *rows_dev receives a device buffer of m
  * rows of 4 * n_file bytes owned by the context and valid until the next call.
  * EQTL_E_FORMAT if a record's sample column count differs from n_file. */
 int eqtl_gpu_vcf_decode(eqtl_gpu_ctx* ctx, const char* text, int32_t text_on_device, int64_t text_len,

@@ -203,6 +203,20 @@ def synth_vcf_text(rng, n, m, ds):
     return np.ascontiguousarray(lines).reshape(-1), recs
 
 
+def vcf_traffic(ds, m, n):
+    """HBM bytes per k_vcf_decode launch from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB device copy), for the
+    32,768 x 2000 workload only."""
+    path = os.path.join(ROOT, "profiles", "vcf_decode_n2000_m32768.md")
+    if (m, n) != (VCF_M, VCF_N) or not os.path.exists(path):
+        return None
+    key = "ds_traffic_bytes_per_launch:" if ds else "gt_traffic_bytes_per_launch:"
+    for line in open(path):
+        if line.startswith(key):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def bench_vcf(args, ds):
     """Device VCF decode (eqtl_gpu_vcf_decode) on 32,768 records x 2000 samples
     with the text resident in HBM; one step = one call (descriptor upload,
@@ -274,7 +288,7 @@ def bench_vcf(args, ds):
         "config": {"workload": f"VCF {'GT:DS' if ds else 'GT'} n={n} samples x {m} records per call",
                    "text_bytes_per_call": text_bytes},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": vcf_traffic(ds, m, n),
                      "note": "algorithmic bytes = sample-column text read + 4 B per call written + 24 B per "
                              "record descriptor, over the k_vcf_decode launch's HIP-event time"},
         "cpu_baseline": cpu,
@@ -303,11 +317,6 @@ def main():
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
         sys.exit(2)
-    if args.workload != "trans-lmm":
-        bench_vcf(args, args.workload == "vcf-ds")
-        return
-    import eqtl_amd
-    BLOCK = args.block
     if args.probe_copy:
         import torch
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -316,6 +325,11 @@ def main():
         torch.cuda.synchronize()
         log("copy probe: 1 GiB read + 1 GiB written")
         del x, y
+    if args.workload != "trans-lmm":
+        bench_vcf(args, args.workload == "vcf-ds")
+        return
+    import eqtl_amd
+    BLOCK = args.block
     m = min(args.snps, M)
     ctx = eqtl_amd.Context(0)
     name, ncu, hbm = ctx.device_info()

@@ -435,8 +435,10 @@ int eqtl_gpu_vcf_decode(eqtl_gpu_ctx* c, const char* text, int32_t on_dev, int64
   if (c->n <= 0 || c->n_file <= 0) return fail(EQTL_E_STATE, "set_samples first");
   if (m > 0x7fffffff) return fail(EQTL_E_ARG, "too many records in one call");
   if (on_dev && (reinterpret_cast<uintptr_t>(text) & 3)) return fail(EQTL_E_ARG, "device text must be 4-byte aligned");
+  bool any_ds = false;
   for (int64_t r = 0; r < m; ++r) {
     const eqtl_vcf_record& e = recs[r];
+    any_ds |= e.ds_key >= 0;
     if (e.offset < 0 || e.length < 0 || e.offset + e.length > text_len || e.gt_key < -1 || e.ds_key < -1)
       return fail(EQTL_E_ARG, "VCF record " + std::to_string(r) + ": span outside the text or bad key index");
   }
@@ -468,8 +470,11 @@ int eqtl_gpu_vcf_decode(eqtl_gpu_ctx* c, const char* text, int32_t on_dev, int64
   a.rows = c->vcf_rows.as<float>();
   a.bad = c->vcf_bad.as<unsigned long long>();
   Timed t(c, "vcf");
-  hipLaunchKernelGGL(k_vcf_decode, dim3(static_cast<unsigned>((m + kVcfWaves - 1) / kVcfWaves)), dim3(kVcfThreads), 0,
-                     c->stream, a);
+  const dim3 grid(static_cast<unsigned>((m + kVcfWaves - 1) / kVcfWaves));
+  if (any_ds)
+    hipLaunchKernelGGL(k_vcf_decode<true>, grid, dim3(kVcfThreads), 0, c->stream, a);
+  else
+    hipLaunchKernelGGL(k_vcf_decode<false>, grid, dim3(kVcfThreads), 0, c->stream, a);
   CHK(check_launch("k_vcf_decode"));
   CHK(t.done());
   unsigned long long bad = none;

@@ -50,6 +50,8 @@ __device__ __attribute__((noinline)) VcfCall vcf_column_slow(const unsigned char
   return parse_sample(at_line, wbase + o, L, gt_key, ds_key, hap);
 }
 
+// ANY_DS: some record of the launch reads DS (else the DS code is compiled out).
+template <bool ANY_DS>
 __global__ __launch_bounds__(kVcfThreads) void k_vcf_decode(VcfArgs a) {
   const int lane = threadIdx.x & 63;
   const int64_t r = static_cast<int64_t>(blockIdx.x) * kVcfWaves + (threadIdx.x >> 6);
@@ -63,7 +65,7 @@ __global__ __launch_bounds__(kVcfThreads) void k_vcf_decode(VcfArgs a) {
   const int n_file = a.n_file;
   float* row = a.rows + r * static_cast<int64_t>(n_file);
   const bool hap = rec.haploid != 0;
-  const int gt_key = rec.gt_key, ds_key = rec.ds_key;
+  const int gt_key = rec.gt_key, ds_key = ANY_DS ? rec.ds_key : -1;
   int n_miss = 0, copies = 0;
   double sum = 0.0;
   bool over = false;

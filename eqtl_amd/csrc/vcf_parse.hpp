@@ -213,10 +213,15 @@ EQ_HD bool parse_sample_window(uint64_t lo, uint64_t hi, int o, int room, int gt
   return true;
 }
 
+// 0x80 in every byte of x that is zero (exact: no carries between bytes).
+EQ_HD uint64_t swar_zero_bytes(uint64_t x) {
+  return ~(((x & 0x7f7f7f7f7f7f7f7full) + 0x7f7f7f7f7f7f7f7full) | x) & 0x8080808080808080ull;
+}
+
 // Branch-free decode of the commonest columns, GT first in FORMAT: a diploid
 // call of one-digit alleles ("a|b", "a/b", '.' alleles) or a haploid one ("a"),
-// and, when DS is the second key, a DS value of at most nine digits and one
-// '.' ("0.123", "1", ".5") right after such a call. x0, x1 hold 16 bytes from the
+// and, when DS is the second key, a DS value of at most seven characters,
+// digits and one '.' ("0.123", "1", ".5"), right after such a call. x0, x1 hold 16 bytes from the
 // column start, of which win are window bytes and avail are on the line.
 // Returns false when the column has another form or does not end inside the
 // window (the caller then uses the general parse); otherwise the result is the
@@ -249,40 +254,45 @@ EQ_HD bool parse_sample_fast(uint64_t x0, uint64_t x1, int win, int avail, int g
   }
   // DS after "GT:"; the column must continue with ':'
   if (gt_len >= avail || byte(gt_len) != ':') return false;
-  const int q0 = gt_len + 1;
-  unsigned M = 0;  // at most 9 digits: exact in 32 bits and in a double
-  int frac = 0, nd = 0;
-  bool dot = false, open = true, bad = false;
-#pragma unroll
-  for (int j = 0; j < 11; ++j) {  // nine digits, a '.', the terminator
-    const int i = q0 + j;
-    if (open) {
-      const int t = term(i);
-      if (t != 0) {
-        open = false;
-        bad |= t < 0;  // ran past the window
-      } else {
-        const unsigned ch = byte(i);
-        if (is_digit(ch) && nd < 9) {
-          M = M * 10u + (ch - '0');
-          ++nd;
-          if (dot) ++frac;
-        } else if (ch == '.' && !dot) {
-          dot = true;
-        } else {
-          bad = true;
-        }
-      }
-    }
-  }
-  if (open || bad) return false;
+  const int q0 = gt_len + 1;  // 2 or 4: bytes q0..q0+7 are window bytes (win >= 12)
+  const int sh = 8 * q0;
+  const uint64_t v = (x0 >> sh) | (x1 << (64 - sh));
+  // first terminator (':', tab, end of line) among the 8 bytes
+  const uint64_t zt = swar_zero_bytes(v ^ 0x3a3a3a3a3a3a3a3aull) | swar_zero_bytes(v ^ 0x0909090909090909ull);
+  int t = zt ? (__builtin_ctzll(zt) >> 3) : 8;
+  if (avail - q0 < t) t = avail - q0;
+  if (t >= 8) return false;  // longer value: general parse
+  const uint64_t low = t == 0 ? 0ull : (~0ull >> (64 - 8 * t));
+  const uint64_t dg = v ^ 0x3030303030303030ull;  // digits become 0..9
+  const uint64_t not_digit = ((((dg & 0x7f7f7f7f7f7f7f7full) + 0x7676767676767676ull) | dg) & 0x8080808080808080ull) & low;
+  const uint64_t dots = swar_zero_bytes(v ^ 0x2e2e2e2e2e2e2e2eull) & low;
+  if ((not_digit & ~dots) != 0 || __builtin_popcountll(dots) > 1) return false;  // sign, exponent, ...
+  const int nd = t - __builtin_popcountll(dots);
   if (nd == 0) {  // empty, or '.' alone: missing
     out = VcfCall{0.f, true, 0};
     return true;
   }
+  uint64_t dd = dg & low;
+  int frac = 0;
+  if (dots) {  // drop the '.' byte
+    const int p = __builtin_ctzll(dots) >> 3;
+    frac = t - p - 1;
+    const uint64_t keep = p == 0 ? 0ull : (~0ull >> (64 - 8 * p));
+    dd = (dd & keep) | ((dd >> 8) & ~keep);
+  }
+  dd <<= 8 * (8 - nd);  // right-align: leading zero digits
+  // eight ASCII-free digits, first in the lowest byte -> integer (the SWAR
+  // reduction of Lemire's "parse_eight_digits_unrolled")
+  dd = ((dd & 0x0f0f0f0f0f0f0f0full) * 2561) >> 8;
+  dd = ((dd & 0x00ff00ff00ff00ffull) * 6553601) >> 16;
+  const unsigned M = static_cast<unsigned>(((dd & 0x0000ffff0000ffffull) * 42949672960001ull) >> 32);
+  double p10 = 1.0;  // 10^frac, frac <= 6: exact
+  p10 = (frac & 1) ? p10 * 10.0 : p10;
+  p10 = (frac & 2) ? p10 * 100.0 : p10;
+  p10 = (frac & 4) ? p10 * 10000.0 : p10;
   out.miss = false;
   out.copies = hap ? 1 : 2;
-  out.dose = static_cast<float>(static_cast<double>(M) / exact_p10(frac));
+  out.dose = static_cast<float>(static_cast<double>(M) / p10);
   return true;
 }
 

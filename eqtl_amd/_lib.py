@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libeqtl_gpu.so")
 
 EQTL_OK = 0
 EQTL_E_CAPACITY = -6
-MODEL_LM, MODEL_LMM = 0, 1
+MODEL_LM, MODEL_LMM, MODEL_GLM = 0, 1, 2
 PREC_FP32, PREC_BF16X3 = 0, 1
 DELTA_REF, DELTA_NEWTON, DELTA_GIVEN = 0, 1, 2
 SCOPE_TRANS, SCOPE_CIS = 0, 1
@@ -39,7 +39,7 @@ EXPORTS = [
     "eqtl_gpu_set_gene_windows", "eqtl_gpu_scan_begin", "eqtl_gpu_scan_block", "eqtl_gpu_scan_end",
     "eqtl_gpu_set_timing", "eqtl_gpu_get_timing", "eqtl_gpu_synth_bed", "eqtl_gpu_copy_rows",
     "eqtl_gpu_synchronize", "eqtl_gpu_selftest", "eqtl_gpu_upload_bed", "eqtl_gpu_cis_perm",
-    "eqtl_gpu_beta_approx", "eqtl_gpu_set_geno_format", "eqtl_gpu_vcf_decode",
+    "eqtl_gpu_beta_approx", "eqtl_gpu_set_geno_format", "eqtl_gpu_vcf_decode", "eqtl_gpu_glm_prep",
 ]
 
 _lib = None
@@ -223,6 +223,18 @@ class Context:
         _check(lib().eqtl_gpu_lm_prep(self._c, _p(Y), ctypes.c_int32(G), _p(X), ctypes.c_int32(p), _p(yty)))
         self.G = G
         return yty
+
+    def glm_prep(self, Y, X):
+        """NB GLM null fits of all genes (counts). Returns (phi[G], converged[G])."""
+        Y = np.asfortranarray(Y, dtype=np.float64)
+        X = np.asfortranarray(X, dtype=np.float64)
+        G, p = Y.shape[1], X.shape[1]
+        phi = np.empty(G)
+        conv = np.empty(G, dtype=np.int32)
+        _check(lib().eqtl_gpu_glm_prep(self._c, _p(Y), ctypes.c_int32(G), _p(X), ctypes.c_int32(p), _p(phi),
+                                       _p(conv)))
+        self.G = G
+        return phi, conv.astype(bool)
 
     def set_gene_windows(self, chrom, cstart, cend):
         ch = np.ascontiguousarray(chrom, dtype=np.int32)

@@ -19,6 +19,7 @@
 #include "reml_kernels.hpp"
 #include "scan_kernels.hpp"
 #include "vcf_kernels.hpp"
+#include "glm_kernels.hpp"
 #include "wald_lmm.hpp"
 #include "grm_kernels.hpp"
 #include "reml_batch.hpp"
@@ -38,6 +39,9 @@ constexpr double kRefineTauF32 = 5e-4;
 constexpr double kRefineTauBF16X3 = 2e-3;
 // |t| above which the numerator is refined as well (two-sided p < ~1e-15).
 constexpr double kRefineTauHi = 8.0;
+// NB score tests with |z| > 8 get the FP64 score (the information stays from
+// the contraction; no polish pass)
+constexpr double kGlmTauHi = 8.0;
 // Capacity of the per-block list of LMM tests polished in FP64 (|t| > 8), and
 // how many are polished per launch (g~ scratch: chunk x n doubles).
 constexpr int64_t kPolishCap = int64_t(1) << 20;
@@ -240,7 +244,7 @@ int ensure_block(eqtl_gpu_ctx* c, int64_t m, int32_t bps, bool want64) {
 }
 
 int run_decode(eqtl_gpu_ctx* c, const uint8_t* rows_dev, int64_t m, int32_t bps, bool want64,
-               double* dosage_dev) {
+               double* dosage_dev, double* mean_dev = nullptr) {
   const int64_t m_pad = round_up(std::max<int64_t>(m, 1), 128);
   DecodeArgs a;
   a.rows = rows_dev;
@@ -259,6 +263,7 @@ int run_decode(eqtl_gpu_ctx* c, const uint8_t* rows_dev, int64_t m, int32_t bps,
   a.dosage = dosage_dev;
   a.keep = c->keep.as<uint8_t>();
   a.af = c->af.as<double>();
+  a.mean = mean_dev;
   Timed t(c, "decode");
   const dim3 grid(static_cast<unsigned>((m_pad + 3) / 4));
   if (c->geno_fmt == EQTL_GENO_DOSE32)
@@ -304,40 +309,44 @@ int lmm_nq_pad(int nq) {
   return -1;
 }
 
-template <int PREC, int NQ>
+template <int PREC, int NQ, int MODEL>
 int launch_wald_lmm_one(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
   const size_t lds = w8_lds_bytes(NQ);
   static bool attr_set = false;
   if (!attr_set) {
-    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald_lmm<PREC, NQ>),
+    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald_lmm<PREC, NQ, MODEL>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
     attr_set = true;
   }
-  hipLaunchKernelGGL((k_wald_lmm<PREC, NQ>), grid, dim3(kW8Threads), lds, c->stream, a);
+  hipLaunchKernelGGL((k_wald_lmm<PREC, NQ, MODEL>), grid, dim3(kW8Threads), lds, c->stream, a);
   return check_launch("k_wald_lmm");
 }
 
-template <int PREC>
+template <int PREC, int MODEL>
 int launch_wald_lmm(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
   switch (c->nq_pad) {
-    case 4: return launch_wald_lmm_one<PREC, 4>(c, a, grid);
-    case 8: return launch_wald_lmm_one<PREC, 8>(c, a, grid);
-    case 12: return launch_wald_lmm_one<PREC, 12>(c, a, grid);
-    case 16: return launch_wald_lmm_one<PREC, 16>(c, a, grid);
-    case 18: return launch_wald_lmm_one<PREC, 18>(c, a, grid);
-    case 20: return launch_wald_lmm_one<PREC, 20>(c, a, grid);
-    case 23: return launch_wald_lmm_one<PREC, 23>(c, a, grid);
-    case 26: return launch_wald_lmm_one<PREC, 26>(c, a, grid);
-    case 30: return launch_wald_lmm_one<PREC, 30>(c, a, grid);
-    case 34: return launch_wald_lmm_one<PREC, 34>(c, a, grid);
+    case 4: return launch_wald_lmm_one<PREC, 4, MODEL>(c, a, grid);
+    case 8: return launch_wald_lmm_one<PREC, 8, MODEL>(c, a, grid);
+    case 12: return launch_wald_lmm_one<PREC, 12, MODEL>(c, a, grid);
+    case 16: return launch_wald_lmm_one<PREC, 16, MODEL>(c, a, grid);
+    case 18: return launch_wald_lmm_one<PREC, 18, MODEL>(c, a, grid);
+    case 20: return launch_wald_lmm_one<PREC, 20, MODEL>(c, a, grid);
+    case 23: return launch_wald_lmm_one<PREC, 23, MODEL>(c, a, grid);
+    case 26: return launch_wald_lmm_one<PREC, 26, MODEL>(c, a, grid);
+    case 30: return launch_wald_lmm_one<PREC, 30, MODEL>(c, a, grid);
+    case 34: return launch_wald_lmm_one<PREC, 34, MODEL>(c, a, grid);
     default: return fail(EQTL_E_ARG, "too many covariates for the Wald kernel");
   }
 }
 
 int launch_wald(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
   if (c->scan_model == EQTL_MODEL_LMM) {
-    if (c->scan_prec == EQTL_PREC_FP32) return launch_wald_lmm<EQTL_PREC_FP32>(c, a, grid);
-    return launch_wald_lmm<EQTL_PREC_BF16X3>(c, a, grid);
+    if (c->scan_prec == EQTL_PREC_FP32) return launch_wald_lmm<EQTL_PREC_FP32, EQTL_MODEL_LMM>(c, a, grid);
+    return launch_wald_lmm<EQTL_PREC_BF16X3, EQTL_MODEL_LMM>(c, a, grid);
+  }
+  if (c->scan_model == EQTL_MODEL_GLM) {
+    if (c->scan_prec == EQTL_PREC_FP32) return launch_wald_lmm<EQTL_PREC_FP32, EQTL_MODEL_GLM>(c, a, grid);
+    return launch_wald_lmm<EQTL_PREC_BF16X3, EQTL_MODEL_GLM>(c, a, grid);
   }
   if (c->scan_prec == EQTL_PREC_FP32) return launch_wald_lm<EQTL_PREC_FP32>(c, a, grid);
   return launch_wald_lm<EQTL_PREC_BF16X3>(c, a, grid);
@@ -1020,6 +1029,42 @@ int eqtl_gpu_lm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double* 
   return EQTL_OK;
 }
 
+int eqtl_gpu_glm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double* X, int32_t p, double* phi_out,
+                      int32_t* conv_out) {
+  if (!c || !Y || !X || G <= 0 || p <= 0) return fail(EQTL_E_ARG, "bad glm_prep arguments");
+  if (p > kPMax) return fail(EQTL_E_ARG, "more than 32 covariate columns are not supported");
+  if (c->n <= 0) return fail(EQTL_E_STATE, "set_samples first");
+  const int n = c->n;
+  for (size_t i = 0; i < static_cast<size_t>(n) * G; ++i)  // looks_like_counts, model_lm.cpp:8-17
+    if (!(Y[i] >= 0) || std::fabs(Y[i] - std::floor(Y[i])) > 1e-8)
+      return fail(EQTL_E_ARG, "glm (NB) requires non-negative integer counts (gene " + std::to_string(i / n) + ")");
+  CHK(set_device(c));
+  c->p = p;
+  CHK(alloc_gene_side(c, G, p + 2, lmm_nq_pad(p + 2)));
+  CHK(ensure(c->Y64, sizeof(double) * static_cast<size_t>(n) * G));
+  CHK(ensure(c->X64, sizeof(double) * static_cast<size_t>(n) * p));
+  CHK(ensure(c->Yt64, sizeof(double) * static_cast<size_t>(n) * G));  // the fit's mean (scratch)
+  CHK(ensure(c->U64, sizeof(double) * static_cast<size_t>(n) * G));
+  CHK(ensure(c->V64, sizeof(double) * static_cast<size_t>(n) * G));
+  CHK(ensure(c->Lc, sizeof(double) * static_cast<size_t>(p) * p * G));
+  HIPC(hipMemcpyAsync(c->Y64.p, Y, sizeof(double) * static_cast<size_t>(n) * G, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->X64.p, X, sizeof(double) * static_cast<size_t>(n) * p, hipMemcpyHostToDevice, c->stream));
+  {
+    Timed t(c, "glm_prep");
+    hipLaunchKernelGGL(k_glm_null, dim3(G), dim3(256), 0, c->stream, c->X64.as<double>(), c->Y64.as<double>(), n,
+                       c->n_pad, p, G, gene_cols(c), c->Yt64.as<double>(), c->delta.as<double>(),
+                       c->gvalid.as<int>(), c->gscal.as<double>(), c->U64.as<double>(), c->V64.as<double>(),
+                       c->Lc.as<double>());
+    CHK(check_launch("k_glm_null"));
+    CHK(t.done());
+  }
+  if (phi_out) HIPC(hipMemcpyAsync(phi_out, c->delta.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  if (conv_out) HIPC(hipMemcpyAsync(conv_out, c->gvalid.p, sizeof(int32_t) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  c->prepped_model = EQTL_MODEL_GLM;
+  return EQTL_OK;
+}
+
 int eqtl_gpu_set_gene_windows(eqtl_gpu_ctx* c, const int32_t* chrom, const int64_t* cs, const int64_t* ce,
                               int32_t G) {
   if (!c || !chrom || !cs || !ce || G <= 0) return fail(EQTL_E_ARG, "bad gene windows");
@@ -1084,7 +1129,7 @@ int eqtl_gpu_scan_begin(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t sc
 static int scan_begin_impl(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t scope, double pthr,
                            bool windows_ready) {
   if (!c) return fail(EQTL_E_ARG, "null ctx");
-  if (model != EQTL_MODEL_LM && model != EQTL_MODEL_LMM) return fail(EQTL_E_ARG, "bad model");
+  if (model != EQTL_MODEL_LM && model != EQTL_MODEL_LMM && model != EQTL_MODEL_GLM) return fail(EQTL_E_ARG, "bad model");
   if (prec != EQTL_PREC_FP32 && prec != EQTL_PREC_BF16X3) return fail(EQTL_E_ARG, "bad precision");
   if (scope != EQTL_SCOPE_TRANS && scope != EQTL_SCOPE_CIS) return fail(EQTL_E_ARG, "bad scope");
   if (c->prepped_model != model) return fail(EQTL_E_STATE, "gene side not prepared for this model");
@@ -1131,15 +1176,19 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   if (hit_cap > 0 && !hits) return fail(EQTL_E_ARG, "hits buffer is null");
   CHK(set_device(c));
   const bool lm = c->scan_model == EQTL_MODEL_LM;
+  const bool glm = c->scan_model == EQTL_MODEL_GLM;
   CHK(ensure_block(c, m, bps, true));
   int rc;
   const uint8_t* rd = stage_rows(c, rows, on_dev, m, bps, &rc);
   if (rc) return rc;
-  CHK(run_decode(c, rd, m, bps, true, nullptr));
+  // GLM: the score needs each SNP's centring mean (kept in the per-SNP gtg slot)
+  CHK(run_decode(c, rd, m, bps, true, nullptr, glm ? c->gtg.as<double>() : nullptr));
   const int64_t m_pad = round_up(m, 128);
   const int np = c->n_pad;
   const float* Aop = c->A32.as<float>();
-  if (!lm) {
+  if (glm) {
+    // the score test contracts the centred dosage itself (no eigenbasis)
+  } else if (!lm) {
     Timed t(c, "rotate");
     dim3 grid(static_cast<unsigned>(np / 64), static_cast<unsigned>(m_pad / 64));
     if (c->scan_prec == EQTL_PREC_FP32)
@@ -1211,7 +1260,8 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     c->hit_cap_dev = cap_dev;
   }
   HIPC(hipMemsetAsync(c->hit_count.p, 0, 2 * sizeof(unsigned long long), c->stream));
-  if (!lm) CHK(ensure(c->polish, sizeof(PolishItem) * static_cast<size_t>(kPolishCap)));
+  const bool polish_on = c->scan_model == EQTL_MODEL_LMM;
+  if (polish_on) CHK(ensure(c->polish, sizeof(PolishItem) * static_cast<size_t>(kPolishCap)));
   WaldArgs a;
   a.A = Aop;
   a.gc = gene_cols(c);
@@ -1243,12 +1293,14 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   a.hit_count = c->hit_count.as<unsigned long long>();
   a.hit_cap = cap_dev;
   a.A64 = c->A64.as<double>();
-  a.vref = lm ? c->U64.as<double>() : c->V64.as<double>();
+  // FP64 residual vectors in sample space for the numerator refinement: LM
+  // y_s, GLM the score residual r, LMM Q (s o r)
+  a.vref = (lm || glm) ? c->U64.as<double>() : c->V64.as<double>();
   a.n = c->n;
   a.tau = c->scan_prec == EQTL_PREC_FP32 ? kRefineTauF32 : kRefineTauBF16X3;
-  a.tau_hi = kRefineTauHi;
-  a.polish = lm ? nullptr : c->polish.as<PolishItem>();
-  a.polish_cap = lm ? 0 : kPolishCap;
+  a.tau_hi = glm ? kGlmTauHi : kRefineTauHi;
+  a.polish = polish_on ? c->polish.as<PolishItem>() : nullptr;
+  a.polish_cap = polish_on ? kPolishCap : 0;
   a.cis = cis ? 1 : 0;
   a.work = cis ? c->work.as<int32_t>() : nullptr;
   {
@@ -1357,6 +1409,8 @@ int eqtl_gpu_cis_perm(eqtl_gpu_ctx* c, int32_t precision, int32_t B, int32_t see
   if (!c || !gene_hash || !best_p_obs || B <= 0) return fail(EQTL_E_ARG, "bad cis_perm arguments");
   if (c->scanning) return fail(EQTL_E_STATE, "a scan is in progress");
   const int model = c->prepped_model;
+  if (model == EQTL_MODEL_GLM)
+    return fail(EQTL_E_ARG, "gene-level permutations are provided for lm and lmm only (not the NB glm)");
   if (model != EQTL_MODEL_LM && model != EQTL_MODEL_LMM) return fail(EQTL_E_STATE, "genes not prepared");
   if (!c->have_windows || static_cast<int>(c->h_gene_chrom.size()) != c->G)
     return fail(EQTL_E_STATE, "cis permutations need the gene windows");

@@ -68,6 +68,7 @@ struct DecodeArgs {
   double* dosage;      // nullable, m x n imputed dosage (uncentred)
   uint8_t* keep;       // m_pad
   double* af;          // m_pad
+  double* mean;        // nullable, m_pad: the centring mean of each kept SNP (0 otherwise)
 };
 
 template <int FMT>
@@ -82,7 +83,11 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
       Arow[i] = 0.f;
       if (A64row) A64row[i] = 0.0;
     }
-    if (lane == 0) { a.keep[j] = 0; a.af[j] = __builtin_nan(""); }
+    if (lane == 0) {
+      a.keep[j] = 0;
+      a.af[j] = __builtin_nan("");
+      if (a.mean) a.mean[j] = 0.0;
+    }
     return;
   }
   const uint8_t* row = a.rows + j * a.bps;
@@ -151,6 +156,7 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
   if (lane == 0) {
     a.keep[j] = keep ? 1 : 0;
     a.af[j] = keep ? afs : __builtin_nan("");
+    if (a.mean) a.mean[j] = keep ? muc : 0.0;
   }
 }
 
@@ -299,9 +305,15 @@ struct TailView {
 template <int MODEL>
 __device__ inline bool refine_wanted(const WaldArgs& a, bool ok, double gg, double S, double num, double gscal,
                                      double gtg) {
-  if (!ok || !(a.df > 0)) return false;
+  if (!ok || (MODEL != EQTL_MODEL_GLM && !(a.df > 0))) return false;
   const double tau2 = a.tau * a.tau, hi2 = a.tau_hi * a.tau_hi;
-  if constexpr (MODEL == EQTL_MODEL_LMM) {
+  if constexpr (MODEL == EQTL_MODEL_GLM) {
+    // score test: U = num + mean * sum(r) (gtg carries the SNP mean, gscal sum r)
+    if (S < 1e-12) return false;
+    const double U = num + gtg * gscal;
+    const double z2S = U * U;  // z^2 = z2S / S
+    return z2S < tau2 * S || z2S > hi2 * S;
+  } else if constexpr (MODEL == EQTL_MODEL_LMM) {
     if (gg < 1e-12 || S <= 1e-15) return false;
     const double beta = num / S;
     const double qq = fmax(gscal - S * beta * beta, 0.0);
@@ -349,7 +361,18 @@ __device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, i
   const int64_t snp = a.snp0 + j;
   double beta = 0, se = 0, stat = 0, p = 1.0, r2 = 0;
   const double df = a.df;
-  if constexpr (MODEL == EQTL_MODEL_LMM) {
+  if constexpr (MODEL == EQTL_MODEL_GLM) {
+    // test_glm_nb (model_glm.cpp:117-131): U = g^T r with the uncentred dosage
+    // = (g - mean)^T r + mean * sum(r); info = S; two-sided normal p
+    if (!(S < 1e-12)) {
+      const double U = num + ri.gtg[jl] * gscal;
+      const double rs = sqrt(S);
+      beta = U / S;
+      se = 1.0 / rs;
+      stat = U / rs;
+      p = isfinite(stat) ? fmin(1.0, fmax(0.0, erfc(fabs(stat) * 0.70710678118654752440))) : 1.0;
+    }
+  } else if constexpr (MODEL == EQTL_MODEL_LMM) {
     if (!(gg < 1e-12) && !(S <= 1e-15)) {
       beta = num / S;
       double qq = gscal - S * beta * beta;

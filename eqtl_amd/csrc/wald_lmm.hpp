@@ -335,7 +335,7 @@ __device__ inline W8Gene w8_gene(const WaldArgs& a, int group) {
 // LDS: fold(t, gg, S, num) reads test t = row * 16 + gene. The per-thread gene
 // states are merged over the four lanes holding the same gene, then over the
 // eight waves (wpart) into the running state in gstate.
-template <class Fold>
+template <int MODEL, class Fold>
 __device__ inline void w8_epilogue(const WaldArgs& a, const W8Gene& gi, const RowInfo& ri, const TailView& tv,
                                    GeneAcc* gstate, GeneAcc* wpart, int64_t row0, Fold fold) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -351,10 +351,10 @@ __device__ inline void w8_epilogue(const WaldArgs& a, const W8Gene& gi, const Ro
     const bool ok = gi.ok && j < a.m && ri.keep[jl] && (a.cis ? inwin : !inwin);
     double gg, S, num;
     fold(t, gg, S, num);
-    const bool cand = refine_wanted<EQTL_MODEL_LMM>(a, ok, gg, S, num, gi.gscal, 0.0);
+    const bool cand = refine_wanted<MODEL>(a, ok, gg, S, num, gi.gscal, ri.gtg[jl]);
     const double n64 = wave_refine(a, cand, j, gi.gene);
     if (cand) num = n64;
-    if (ok) wald_test<EQTL_MODEL_LMM>(a, ri, jl, j, gi.gene, gi.gscal, gg, S, num, tv, st, asum);
+    if (ok) wald_test<MODEL>(a, ri, jl, j, gi.gene, gi.gscal, gg, S, num, tv, st, asum);
   }
   st.acat_hi = asum.hi;
   st.acat_lo = asum.lo;
@@ -370,7 +370,8 @@ __device__ inline void w8_epilogue(const WaldArgs& a, const W8Gene& gi, const Ro
   }
 }
 
-// Loads one tile's SNP data (keep, af, annotation) into LDS.
+// Loads one tile's SNP data (keep, af, annotation; the SNP mean for GLM) into LDS.
+template <int MODEL>
 __device__ inline void w8_row_info(const WaldArgs& a, RowInfo& ri, int64_t row0) {
   if (threadIdx.x < kTileRows) {
     const int t = threadIdx.x;
@@ -378,14 +379,16 @@ __device__ inline void w8_row_info(const WaldArgs& a, RowInfo& ri, int64_t row0)
     const bool in = j < a.m;
     ri.keep[t] = in ? a.keep[j] : 0;
     ri.af[t] = in ? a.af[j] : 0.0;
-    ri.gtg[t] = 0.0;
+    ri.gtg[t] = (MODEL == EQTL_MODEL_GLM && in) ? a.gtg[j] : 0.0;
     const bool ann = in && a.snp_chrom;
     ri.chrom[t] = ann ? a.snp_chrom[a.snp0 + j] : -1;
     ri.pos[t] = ann ? a.snp_pos[a.snp0 + j] : 0;
   }
 }
 
-template <int PREC, int NQ>
+// MODEL: EQTL_MODEL_LMM (A = rotated dosage, Wald t test) or EQTL_MODEL_GLM
+// (A = centred dosage, NB score test); both contract the same column blocks.
+template <int PREC, int NQ, int MODEL>
 __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_wald_lmm(WaldArgs a) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   constexpr int NQ0 = (NQ + 1) / 2, NQ1 = NQ - NQ0;
@@ -413,11 +416,11 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
   for (int tile = wk.t_lo; tile < wk.t_hi; tile += wk.t_step) {
     const int64_t row0 = static_cast<int64_t>(tile) * kTileRows;
     __syncthreads();  // the previous tile's epilogue is done with ri, fold and the gene partials
-    w8_row_info(a, ri, row0);
+    w8_row_info<MODEL>(a, ri, row0);
     if (c == 0) w8_tile<PREC, NQ, NQ0, 0, true>(a, smem, gsrc, row0, c, r, lane);
     else w8_tile<PREC, NQ, NQ1, NQ0, false>(a, smem, gsrc, row0, c, r, lane);
     __syncthreads();  // fold complete
-    w8_epilogue(a, gi, ri, tv, gstate, wpart, row0, [&](int t, double& gg, double& S, double& num) {
+    w8_epilogue<MODEL>(a, gi, ri, tv, gstate, wpart, row0, [&](int t, double& gg, double& S, double& num) {
       gg = fold[0 * kW8Tests + t] + fold[3 * kW8Tests + t];
       S = gg - (fold[1 * kW8Tests + t] + fold[4 * kW8Tests + t]);
       num = fold[2 * kW8Tests + t] + fold[5 * kW8Tests + t];

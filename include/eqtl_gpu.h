@@ -37,7 +37,10 @@ enum {
   EQTL_E_FORMAT = -7    /* malformed input record (the message names it) */
 };
 
-enum { EQTL_MODEL_LM = 0, EQTL_MODEL_LMM = 1 };
+/* LM: OLS Wald t test (model_lm.cpp); LMM: spectral LMM Wald (model_lmm.cpp);
+ * GLM: negative-binomial (NB2, log link) score test at the null fit
+ * (model_glm.cpp; --model glm). */
+enum { EQTL_MODEL_LM = 0, EQTL_MODEL_LMM = 1, EQTL_MODEL_GLM = 2 };
 /* Contraction precision of the fused Wald kernel (and the genotype rotation).
  * FP32: fp32 MFMA on fp32 operands. BF16X3: operands split into bf16 hi+lo,
  * three bf16 MFMA products (hi*hi + hi*lo + lo*hi) with fp32 accumulation. */
@@ -165,6 +168,18 @@ int eqtl_gpu_reml_grad_hess(eqtl_gpu_ctx* ctx, const double* log_delta, int32_t 
 /* prep_lm for all genes (model_lm.cpp:19-30): residualised y_s, yty. yty_out nullable. */
 int eqtl_gpu_lm_prep(eqtl_gpu_ctx* ctx, const double* Y, int32_t G, const double* X, int32_t p,
                      double* yty_out);
+
+/* prep_glm_nb for all genes at once (model_glm.cpp:28-98): per gene, IRLS for
+ * the NB2 log-link null model with the dispersion phi re-estimated by moments
+ * between rounds (nb_irls / nb_mom_phi), then the null working weights
+ * w = mu / (1 + phi mu), the Cholesky factor of X^T W X and the column blocks
+ * the fused kernel contracts for the score test (test_glm_nb, :100-132).
+ * Y must hold non-negative integer counts (the caller skips other genes, as
+ * scan.cpp:246-249 does). Genes whose fit fails (X^T W X not positive
+ * definite) are not tested (the reference's p = NaN). phi_out / converged_out
+ * (G each) nullable. */
+int eqtl_gpu_glm_prep(eqtl_gpu_ctx* ctx, const double* Y, int32_t G, const double* X, int32_t p, double* phi_out,
+                      int32_t* converged_out);
 
 /* Cis windows for the trans exclusion, per gene: chrom id (< 0: gene has no
  * location, never excluded) and [cstart, cend] (cis_window_bounds). Optional:

@@ -47,6 +47,16 @@ the cited reference code):
                      diploid GT lines equal orc_decode_bed on the PLINK codes
                      of the same calls. htslib (the reference's VCF reader) is
                      absent: its float parse is taken as strtod -> float.
+  glm_prep/glm_test  NB2 score test (model_glm.cpp): the score U is the
+                     derivative of the NB2 log-likelihood (scipy gammaln form)
+                     in the SNP coefficient at the null fit, by finite
+                     differences; the information equals 1 / [(J^-1)_gg] of the
+                     full (p+1) x (p+1) information J = [X g]^T W [X g]
+                     (Schur form checked against a plain inverse); the null fit
+                     solves the NB2 score equations X^T (y - mu)/(1 + phi mu)
+                     = 0 and phi is the MoM fixed point; phi -> 0 data
+                     (Poisson-like) agree with a Poisson GLM fit by
+                     scipy.optimize.
   cis_gene           composition of the above (LM/LMM perm bases restated from
                      scan_cis.hpp:74-160).
 """
@@ -90,6 +100,7 @@ def lib():
         _lib.orc_trans_lmm_scan.restype = ctypes.c_int64
         _lib.orc_trans_lm_scan.restype = ctypes.c_int64
         _lib.orc_grm.restype = ctypes.c_int64
+        _lib.orc_glm_scan.restype = ctypes.c_int64
         _lib.orc_fnv1a.restype = ctypes.c_uint32
         _lib.orc_cis_gene.restype = ctypes.c_double
         _lib.orc_num_threads.restype = ctypes.c_int32
@@ -311,6 +322,61 @@ def trans_scan(model, dosage, snp_keep, snp_chrom, snp_pos, Y, X, Q=None, lam=No
     k = min(nh, hit_cap)
     out = {"state": state, "n_hits": int(nh), "loop_seconds": loop_s.value, "hit_snp": hs[:k].copy(), "hit_gene": hg[:k].copy(),
            "hit_vals": hv[:k].copy()}
+    if dense:
+        out.update(beta=dense_arrs[0], se=dense_arrs[1], stat=dense_arrs[2], p=dense_arrs[3])
+    return out
+
+
+def glm_prep(y, X):
+    """prep_glm_nb (NB2 IRLS with MoM phi) of one gene. Returns (converged, phi,
+    mu[n], beta[p], w[n])."""
+    y = _f64(y)
+    X = np.asfortranarray(X, dtype=np.float64)
+    n, p = X.shape
+    phi = ctypes.c_double()
+    mu, beta, w = np.empty(n), np.empty(p), np.empty(n)
+    conv = lib().orc_glm_prep(_p(y), _p(X), ctypes.c_int32(n), ctypes.c_int32(p), ctypes.byref(phi), _p(mu),
+                              _p(beta), _p(w))
+    return bool(conv), phi.value, mu, beta, w
+
+
+def glm_test(y, X, g):
+    """test_glm_nb of one SNP (imputed dosage g) for one gene: (beta, se, stat, p)."""
+    y, g = _f64(y), _f64(g)
+    X = np.asfortranarray(X, dtype=np.float64)
+    n, p = X.shape
+    out = np.empty(4)
+    lib().orc_glm_test(_p(y), _p(X), ctypes.c_int32(n), ctypes.c_int32(p), _p(g), _p(out))
+    return tuple(out)
+
+
+def glm_scan(dosage, snp_keep, snp_chrom, snp_pos, Y, X, gene_chrom=None, gene_cstart=None, gene_cend=None,
+             cis=False, pthr=1e-5, dense=False, hit_cap=1 << 20):
+    """NB GLM score-test scan (trans: windows excluded; cis: only windows). Returns dict."""
+    dosage = np.ascontiguousarray(dosage, dtype=np.float64)
+    m, n = dosage.shape
+    Y = np.asfortranarray(Y, dtype=np.float64)
+    X = np.asfortranarray(X, dtype=np.float64)
+    G, p = Y.shape[1], X.shape[1]
+    keep = np.ascontiguousarray(snp_keep, dtype=np.uint8)
+    sc = np.ascontiguousarray(snp_chrom, dtype=np.int32)
+    sp = np.ascontiguousarray(snp_pos, dtype=np.int64)
+    gc, gs, ge = _gene_windows(gene_chrom, gene_cstart, gene_cend, G)
+    state = np.zeros(G, dtype=GENE_STATE)
+    phi = np.empty(G)
+    conv = np.empty(G, np.int32)
+    dense_arrs = [np.empty((m, G)) for _ in range(4)] if dense else [None] * 4
+    dp = [_p(a) if a is not None else None for a in dense_arrs]
+    hs = np.empty(hit_cap, np.int64)
+    hg = np.empty(hit_cap, np.int32)
+    hv = np.empty((hit_cap, 6))
+    nh = lib().orc_glm_scan(_p(dosage), _p(keep), _p(sc), _p(sp), ctypes.c_int64(m), ctypes.c_int32(n), _p(Y),
+                            ctypes.c_int32(G), _p(X), ctypes.c_int32(p), _p(gc), _p(gs), _p(ge),
+                            ctypes.c_int32(int(cis)), ctypes.c_double(pthr), _p(state), _p(phi), _p(conv), *dp,
+                            _p(hs), _p(hg), _p(hv), ctypes.c_int64(hit_cap))
+    k = min(nh, hit_cap)
+    out = {"state": state, "phi": phi, "converged": conv.astype(bool), "n_hits": int(nh),
+           "hit_snp": hs[:k].copy(), "hit_gene": hg[:k].copy(), "hit_vals": hv[:k].copy()}
     if dense:
         out.update(beta=dense_arrs[0], se=dense_arrs[1], stat=dense_arrs[2], p=dense_arrs[3])
     return out

@@ -1174,6 +1174,290 @@ int64_t orc_trans_lm_scan(const double* dosage, const uint8_t* snp_keep, const i
   return nh;
 }
 
+// ---- NB GLM score test (model_glm.cpp:1-132) ---------------------------------
+// Cholesky solve of the p x p SPD system A x = b (A col-major, copied); false
+// when A is not positive definite (Eigen's LDLT of the reference reports no
+// failure for a PD matrix; both give the same solution to rounding).
+static bool spd_solve(std::vector<double> A, int p, const double* b, double* x) {
+  for (int j = 0; j < p; ++j) {
+    double s = A[j * p + j];
+    for (int k = 0; k < j; ++k) s -= A[k * p + j] * A[k * p + j];
+    if (!(s > 0)) return false;
+    const double l = std::sqrt(s);
+    A[j * p + j] = l;
+    for (int i = j + 1; i < p; ++i) {
+      double t = A[j * p + i];
+      for (int k = 0; k < j; ++k) t -= A[k * p + i] * A[k * p + j];
+      A[j * p + i] = t / l;
+    }
+  }
+  std::vector<double> z(p);
+  for (int i = 0; i < p; ++i) {
+    double t = b[i];
+    for (int k = 0; k < i; ++k) t -= A[k * p + i] * z[k];
+    z[i] = t / A[i * p + i];
+  }
+  for (int i = p - 1; i >= 0; --i) {
+    double t = z[i];
+    for (int k = i + 1; k < p; ++k) t -= A[i * p + k] * x[k];
+    x[i] = t / A[i * p + i];
+  }
+  return true;
+}
+
+struct PrepGlm {
+  int n = 0, p = 0;
+  const double* y = nullptr;
+  const double* X = nullptr;   // n x p col-major
+  std::vector<double> mu, w, XtWX_inv;
+  double phi = 1.0;
+  bool converged = false;
+};
+
+// nb_mom_phi, model_glm.cpp:14-21
+static double nb_mom_phi(const double* y, const std::vector<double>& mu, int n, int df) {
+  double num = 0;
+  for (int i = 0; i < n; ++i) {
+    const double m = std::max(mu[i], 1e-8);
+    num += ((y[i] - m) * (y[i] - m) - m) / (m * m);
+  }
+  return std::max(1e-8, num / static_cast<double>(std::max(1, df)));
+}
+
+// nb_irls, model_glm.cpp:28-74 (offset 0, phi re-estimated between rounds)
+static void nb_irls(const double* y, const double* X, int n, int p, double& phi, std::vector<double>& beta,
+                    std::vector<double>& mu, bool& converged) {
+  const int df = std::max(1, n - p);
+  beta.assign(p, 0.0);
+  mu.resize(n);
+  for (int i = 0; i < n; ++i) mu[i] = std::max(y[i], 0.1);
+  converged = false;
+  std::vector<double> A(static_cast<size_t>(p) * p), b(p), bn(p), z(n), w(n);
+  for (int outer = 0; outer < 10; ++outer) {
+    for (int it = 0; it < 50; ++it) {
+      for (int i = 0; i < n; ++i) {
+        const double m = std::max(mu[i], 1e-8);
+        const double var = m + phi * m * m;
+        w[i] = (m * m) / std::max(var, 1e-12);
+        z[i] = std::log(m) + (y[i] - m) / m;
+      }
+      for (int a = 0; a < p; ++a) {
+        for (int c = 0; c <= a; ++c) {
+          double s = 0;
+          for (int i = 0; i < n; ++i) s += X[static_cast<size_t>(a) * n + i] * w[i] * X[static_cast<size_t>(c) * n + i];
+          A[static_cast<size_t>(c) * p + a] = A[static_cast<size_t>(a) * p + c] = s;
+        }
+        double s = 0;
+        for (int i = 0; i < n; ++i) s += X[static_cast<size_t>(a) * n + i] * (w[i] * z[i]);
+        b[a] = s;
+      }
+      if (!spd_solve(A, p, b.data(), bn.data())) return;
+      double diff = 0;
+      for (int a = 0; a < p; ++a) diff = std::max(diff, std::fabs(bn[a] - beta[a]));
+      beta = bn;
+      for (int i = 0; i < n; ++i) {
+        double eta = 0;
+        for (int a = 0; a < p; ++a) eta += X[static_cast<size_t>(a) * n + i] * beta[a];
+        mu[i] = std::exp(eta);
+      }
+      if (diff < 1e-6) break;
+    }
+    const double phi_new = nb_mom_phi(y, mu, n, df);
+    if (std::fabs(phi_new - phi) < 1e-6 * std::max(1.0, phi)) {
+      phi = phi_new;
+      converged = true;
+      return;
+    }
+    phi = phi_new;
+  }
+  converged = true;
+}
+
+// prep_glm_nb, model_glm.cpp:76-98
+static void prep_glm(const double* y, const double* X, int n, int p, PrepGlm& P, std::vector<double>* beta_out) {
+  P.n = n;
+  P.p = p;
+  P.y = y;
+  P.X = X;
+  P.phi = 1.0;
+  std::vector<double> beta;
+  nb_irls(y, X, n, p, P.phi, beta, P.mu, P.converged);
+  if (beta_out) *beta_out = beta;
+  P.w.resize(n);
+  for (int i = 0; i < n; ++i) {
+    const double m = std::max(P.mu[i], 1e-8);
+    P.w[i] = m / (1.0 + P.phi * m);
+  }
+  std::vector<double> A(static_cast<size_t>(p) * p);
+  for (int a = 0; a < p; ++a)
+    for (int c = 0; c <= a; ++c) {
+      double s = 0;
+      for (int i = 0; i < n; ++i) s += X[static_cast<size_t>(a) * n + i] * P.w[i] * X[static_cast<size_t>(c) * n + i];
+      A[static_cast<size_t>(c) * p + a] = A[static_cast<size_t>(a) * p + c] = s;
+    }
+  P.XtWX_inv.assign(static_cast<size_t>(p) * p, kNaN);
+  std::vector<double> e(p), col(p);
+  for (int c = 0; c < p; ++c) {
+    std::fill(e.begin(), e.end(), 0.0);
+    e[c] = 1.0;
+    if (!spd_solve(A, p, e.data(), col.data())) {
+      P.converged = false;
+      return;
+    }
+    for (int a = 0; a < p; ++a) P.XtWX_inv[static_cast<size_t>(c) * p + a] = col[a];
+  }
+}
+
+// test_glm_nb, model_glm.cpp:100-132 (g: the imputed dosage, uncentred)
+static Hit test_glm(const PrepGlm& P, const double* g) {
+  Hit h;
+  h.r2 = 0;
+  if (!P.converged) {
+    h.p = kNaN;
+    return h;
+  }
+  const int n = P.n, p = P.p;
+  double s1 = 0, s2 = 0;
+  for (int i = 0; i < n; ++i) {
+    s1 += g[i];
+    s2 += g[i] * g[i];
+  }
+  const double gvar = s2 / n - (s1 / n) * (s1 / n);
+  if (gvar < 1e-12) {
+    h.p = 1.0;
+    return h;
+  }
+  double U = 0, gwg = 0;
+  std::vector<double> xwg(p, 0.0);
+  for (int i = 0; i < n; ++i) {
+    const double r = (P.y[i] - P.mu[i]) / (1.0 + P.phi * P.mu[i]);
+    U += g[i] * r;
+    const double wg = P.w[i] * g[i];
+    gwg += g[i] * wg;
+    for (int a = 0; a < p; ++a) xwg[a] += P.X[static_cast<size_t>(a) * n + i] * wg;
+  }
+  double quad = 0;
+  for (int a = 0; a < p; ++a) {
+    double t = 0;
+    for (int c = 0; c < p; ++c) t += P.XtWX_inv[static_cast<size_t>(c) * p + a] * xwg[c];
+    quad += xwg[a] * t;
+  }
+  const double info = gwg - quad;
+  if (info < 1e-12) {
+    h.p = 1.0;
+    return h;
+  }
+  h.beta = U / info;
+  h.se = 1.0 / std::sqrt(info);
+  h.stat = U / std::sqrt(info);
+  double pv = std::erfc(std::fabs(h.stat) / std::sqrt(2.0));  // pnorm_two_sided, util.cpp:116-123
+  if (pv < 0) pv = 0;
+  if (pv > 1) pv = 1;
+  h.p = std::isfinite(h.stat) ? pv : 1.0;
+  return h;
+}
+
+// Null fit of one gene: returns converged; phi, mu (n) and beta (p) out.
+int32_t orc_glm_prep(const double* y, const double* X, int32_t n, int32_t p, double* phi, double* mu,
+                     double* beta, double* w) {
+  PrepGlm P;
+  std::vector<double> b;
+  prep_glm(y, X, n, p, P, &b);
+  *phi = P.phi;
+  for (int i = 0; i < n; ++i) {
+    mu[i] = P.mu[i];
+    if (w) w[i] = P.w[i];
+  }
+  for (int a = 0; a < p && a < static_cast<int>(b.size()); ++a) beta[a] = b[a];
+  return P.converged ? 1 : 0;
+}
+
+// Score test of one SNP (imputed dosage g) for one gene: out = beta, se, stat, p.
+void orc_glm_test(const double* y, const double* X, int32_t n, int32_t p, const double* g, double* out) {
+  PrepGlm P;
+  prep_glm(y, X, n, p, P, nullptr);
+  const Hit h = test_glm(P, g);
+  out[0] = h.beta;
+  out[1] = h.se;
+  out[2] = h.stat;
+  out[3] = h.p;
+}
+
+// NB GLM scan over decoded SNPs for every gene (the gene-outer scan of
+// scan.cpp:228-300 with scan_gene_snps, scan_cis.hpp:15-66, without
+// permutations): a SNP is tested for gene k unless it lies in the gene's
+// window (trans) — or, when cis != 0, only if it lies there. Same outputs as
+// orc_trans_lm_scan plus phi / converged per gene.
+int64_t orc_glm_scan(const double* dosage, const uint8_t* snp_keep, const int32_t* snp_chrom, const int64_t* snp_pos,
+                     int64_t m, int32_t n, const double* Y, int32_t G, const double* X, int32_t p,
+                     const int32_t* gene_chrom, const int64_t* gene_cstart, const int64_t* gene_cend, int32_t cis,
+                     double pthr, OrcGeneState* state, double* phi_out, int32_t* conv_out, double* d_beta,
+                     double* d_se, double* d_stat, double* d_p, int64_t* hit_snp, int32_t* hit_gene,
+                     double* hit_vals, int64_t hit_cap) {
+  std::vector<PrepGlm> preps(G);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int k = 0; k < G; ++k) prep_glm(Y + static_cast<size_t>(k) * n, X, n, p, preps[k], nullptr);
+  std::vector<Acat> acc(G);
+  for (int k = 0; k < G; ++k) {
+    state[k] = OrcGeneState{};
+    state[k].delta = kNaN;
+    state[k].best_p = 2.0;
+    state[k].best_snp = -1;
+    phi_out[k] = preps[k].phi;
+    conv_out[k] = preps[k].converged ? 1 : 0;
+  }
+  std::vector<Hit> hs(G);
+  std::vector<char> tested(G);
+  int64_t nh = 0;
+  for (int64_t j = 0; j < m; ++j) {
+    const double* g = dosage + j * static_cast<int64_t>(n);
+    const bool snp_ok = snp_keep[j] != 0;
+    double maf_sub = 0;
+    const bool maf_ok = snp_ok && std::isfinite(subset_maf_or_nan(g, n, &maf_sub));
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < G; ++k) {
+      tested[k] = 0;
+      if (!maf_ok) continue;
+      const bool inwin = gene_chrom[k] >= 0 && gene_chrom[k] == snp_chrom[j] && snp_pos[j] >= gene_cstart[k] &&
+                         snp_pos[j] <= gene_cend[k];
+      if (cis ? !inwin : inwin) continue;
+      tested[k] = 1;
+      hs[k] = test_glm(preps[k], g);
+    }
+    for (int k = 0; k < G; ++k) {
+      const size_t o = static_cast<size_t>(j) * G + k;
+      if (!tested[k]) {
+        if (d_p) { d_beta[o] = d_se[o] = d_stat[o] = d_p[o] = kNaN; }
+        continue;
+      }
+      const Hit& h = hs[k];
+      if (d_p) { d_beta[o] = h.beta; d_se[o] = h.se; d_stat[o] = h.stat; d_p[o] = h.p; }
+      if (!std::isfinite(h.p)) continue;
+      OrcGeneState& st = state[k];
+      ++st.n_tested;
+      acc[k].add(h.p);
+      if (h.p <= pthr) {
+        ++st.n_sig;
+        if (nh < hit_cap) {
+          hit_snp[nh] = j;
+          hit_gene[nh] = k;
+          double* v = hit_vals + nh * 6;
+          v[0] = maf_sub; v[1] = h.beta; v[2] = h.se; v[3] = h.stat; v[4] = h.p; v[5] = h.r2;
+        }
+        ++nh;
+      }
+      if (h.p < st.best_p) {
+        st.best_p = h.p;
+        st.best_snp = j;
+        st.best_beta = h.beta; st.best_se = h.se; st.best_stat = h.stat; st.best_r2 = h.r2;
+        st.best_af = maf_sub;
+      }
+    }
+  }
+  for (int k = 0; k < G; ++k) state[k].acat_p = acc[k].p();
+  return nh;
+}
+
 // GRM: grm.cpp:112-224 (standardise, A += Z Z^T, GCTA per-pair divisor).
 // dosage m x n imputed; miss m x n mask; snp_keep as decoded. K n x n
 // column-major, N_packed n(n+1)/2 row-major lower incl. diagonal.

@@ -105,9 +105,9 @@ static std::vector<Model> parse_models(const std::string& s) {
     if (t.empty()) continue;
     if (t == "lm") out.push_back(Model::Lm);
     else if (t == "lmm") out.push_back(Model::Lmm);
-    else if (t == "glm" || t == "nb_glm" || t == "glmm" || t == "p_glmm")
-      die("--model " + t + " is not provided by this engine (lm, lmm)");
-    else die("invalid --model: " + t);
+    else if (t == "glm") out.push_back(Model::Glm);
+    else if (t == "glmm") die("--model glmm is not provided by this engine (lm, glm, lmm)");
+    else die("invalid --model value: " + t + " (lm|glm|lmm|glmm)");
   }
   if (out.empty()) die("empty --model");
   return out;
@@ -509,23 +509,42 @@ std::unordered_map<std::string, GeneLoc> load_gff_tss(const std::string& path,
 }
 
 // ============================================================ outputs (output.cpp:8-110)
-void write_pairs_header(std::ostream& os, Model) {
-  os << "gene\tsnp\tchrom\tpos\tref\talt\taf\tbeta\tse\tstat\tp\tr2\tn\ttss_dist\tscope\n";
+void write_pairs_header(std::ostream& os, Model m) {  // output.cpp:8-13
+  os << "gene\tsnp\tchrom\tpos\tref\talt\taf\tbeta\tse\tstat\tp\tr2\tn\ttss_dist\tscope";
+  if (m == Model::Glm) os << "\tphi\tglm_converged";
+  os << '\n';
 }
 
-void write_pair_line(std::ostream& os, const AssocHit& h, Model, const std::string& scope) {
+void write_pair_line(std::ostream& os, const AssocHit& h, Model m, const std::string& scope) {  // output.cpp:27-76
   char buf[2048];
   int n;
   if (h.has_tss_dist)
-    n = std::snprintf(buf, sizeof(buf), "%s\t%s\t%s\t%lld\t%s\t%s\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%d\t%.10g\t%s\n",
+    n = std::snprintf(buf, sizeof(buf), "%s\t%s\t%s\t%lld\t%s\t%s\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%d\t%.10g\t%s",
                       h.gene.c_str(), h.snp.c_str(), h.chrom.c_str(), static_cast<long long>(h.pos), h.ref.c_str(),
                       h.alt.c_str(), h.maf, h.beta, h.se, h.stat, h.p, h.r2, h.n, h.tss_dist, scope.c_str());
   else
-    n = std::snprintf(buf, sizeof(buf), "%s\t%s\t%s\t%lld\t%s\t%s\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%d\tNA\t%s\n",
+    n = std::snprintf(buf, sizeof(buf), "%s\t%s\t%s\t%lld\t%s\t%s\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%.10g\t%d\tNA\t%s",
                       h.gene.c_str(), h.snp.c_str(), h.chrom.c_str(), static_cast<long long>(h.pos), h.ref.c_str(),
                       h.alt.c_str(), h.maf, h.beta, h.se, h.stat, h.p, h.r2, h.n, scope.c_str());
-  if (n > 0 && static_cast<size_t>(n) < sizeof(buf)) os.write(buf, n);
-  else die("pair line too long");
+  if (n > 0 && m == Model::Glm && static_cast<size_t>(n) < sizeof(buf))
+    n += std::snprintf(buf + n, sizeof(buf) - static_cast<size_t>(n), "\t%.10g\t%d", h.phi, h.glm_converged ? 1 : 0);
+  if (n > 0 && static_cast<size_t>(n) + 1 < sizeof(buf)) {
+    buf[n++] = '\n';
+    os.write(buf, n);
+  } else {
+    die("pair line too long");
+  }
+}
+
+// looks_like_counts (model_lm.cpp:8-17): finite values are non-negative integers.
+static bool looks_like_counts(const double* y, size_t n) {
+  size_t seen = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (!std::isfinite(y[i])) continue;
+    ++seen;
+    if (y[i] < 0 || std::fabs(y[i] - std::floor(y[i])) > 1e-8) return false;
+  }
+  return seen > 0;
 }
 
 void write_region_header(std::ostream& os) {
@@ -689,6 +708,31 @@ Grm compute_grm(Engine& eng, GenoSource& bed, const std::vector<std::string>& sa
   return g;
 }
 
+// Gene-side preparation for the model (prep_lm / prep_lmm / prep_glm_nb for
+// every job). glm: phi and converged per gene for the output columns.
+static void prep_genes(const Options& opt, Engine& eng, Model model, const std::vector<double>& Y, int G,
+                       const CovData& cov, std::vector<double>* phi, std::vector<int32_t>* conv) {
+  eqtl_gpu_ctx* c = eng.ctx();
+  if (model == Model::Lmm) {
+    eng.check(eqtl_gpu_lmm_prep(c, Y.data(), G, cov.X.data(), cov.p, opt.delta_mode, nullptr, nullptr, nullptr),
+              "lmm_prep");
+  } else if (model == Model::Glm) {
+    phi->assign(G, 0.0);
+    conv->assign(G, 0);
+    eng.check(eqtl_gpu_glm_prep(c, Y.data(), G, cov.X.data(), cov.p, phi->data(), conv->data()), "glm_prep");
+  } else {
+    eng.check(eqtl_gpu_lm_prep(c, Y.data(), G, cov.X.data(), cov.p, nullptr), "lm_prep");
+  }
+}
+
+static const char* model_tag(Model m) {
+  return m == Model::Lmm ? "LMM" : (m == Model::Glm ? "GLM" : (m == Model::Glmm ? "GLMM" : "LM"));
+}
+
+static int32_t engine_model(Model m) {
+  return m == Model::Lmm ? EQTL_MODEL_LMM : (m == Model::Glm ? EQTL_MODEL_GLM : EQTL_MODEL_LM);
+}
+
 static uint32_t fnv1a(const std::string& s);
 
 // Trans stage 2 (stage2_perm_topk, scan_perm.cpp:11-142): genes whose best p
@@ -803,25 +847,23 @@ void scan_snp_outer(const Options& opt, Engine& eng, GenoSource& bed, Model mode
   eqtl_gpu_ctx* c = eng.ctx();
   eng.check(eqtl_gpu_set_snp_annot(c, snp_chrom.data(), snp_pos.data(), static_cast<int64_t>(snp_pos.size())),
             "set_snp_annot");
-  if (model == Model::Lmm) {
-    std::vector<double> delta(G);
-    eng.check(eqtl_gpu_lmm_prep(c, Y.data(), G, cov.X.data(), cov.p, opt.delta_mode, nullptr, delta.data(), nullptr),
-              "lmm_prep");
-    (void)Q;
-    (void)lambda;
-  } else {
-    eng.check(eqtl_gpu_lm_prep(c, Y.data(), G, cov.X.data(), cov.p, nullptr), "lm_prep");
-  }
+  (void)Q;
+  (void)lambda;
+  std::vector<double> phi;
+  std::vector<int32_t> conv;
+  prep_genes(opt, eng, model, Y, G, cov, &phi, &conv);
   eng.check(eqtl_gpu_set_gene_windows(c, gch.data(), gcs.data(), gce.data(), G), "set_gene_windows");
+  // glm runs gene by gene in the reference (scan.cpp:228-300): its pairs are
+  // written gene-major, SNP order within a gene
+  const bool gene_major = model == Model::Glm;
+  std::vector<eqtl_hit> gm_hits;
   // with --perm the scan also keeps every pair below perm_trans_thr: the
   // candidates of each gene's stage-2 top-K set (topk_consider, scan_trans.hpp:53-70)
   const bool stage2 = opt.perm > 0;
   const double pthr_dev = stage2 ? std::max(pthr, opt.perm_trans_thr) : pthr;
   std::vector<std::vector<std::pair<double, int64_t>>> cand(stage2 ? G : 0);
   std::vector<int> n_sig(G, 0);
-  eng.check(eqtl_gpu_scan_begin(c, model == Model::Lmm ? EQTL_MODEL_LMM : EQTL_MODEL_LM, opt.precision,
-                                EQTL_SCOPE_TRANS, pthr_dev),
-            "scan_begin");
+  eng.check(eqtl_gpu_scan_begin(c, engine_model(model), opt.precision, EQTL_SCOPE_TRANS, pthr_dev), "scan_begin");
   std::vector<eqtl_hit> hits(1 << 20);
   const auto& sites = bed.sites();
   auto fill = [&](AssocHit& h, const eqtl_hit& e) {
@@ -840,6 +882,10 @@ void scan_snp_outer(const Options& opt, Engine& eng, GenoSource& bed, Model mode
     h.p = e.p;
     h.r2 = e.r2;
     h.n = n;
+    if (model == Model::Glm) {
+      h.phi = phi[e.gene];
+      h.glm_converged = conv[e.gene] != 0;
+    }
     if (job.has_loc && job.loc.ok) {
       h.has_tss_dist = true;
       h.tss_dist = static_cast<double>(s.pos - job.loc.tss);
@@ -861,6 +907,10 @@ void scan_snp_outer(const Options& opt, Engine& eng, GenoSource& bed, Model mode
       if (stage2 && e.p < opt.perm_trans_thr) cand[e.gene].emplace_back(e.p, e.snp);
       if (e.p > pthr) continue;
       ++n_sig[e.gene];
+      if (gene_major) {
+        gm_hits.push_back(e);
+        continue;
+      }
       AssocHit h;
       fill(h, e);
       write_pair_line(out.pairs, h, model, out.tag);
@@ -868,7 +918,17 @@ void scan_snp_outer(const Options& opt, Engine& eng, GenoSource& bed, Model mode
     const size_t before = snp_cnt;
     snp_cnt += m;
     if (before / 100000 != snp_cnt / 100000)
-      info(std::string("trans/gw ") + (model == Model::Lmm ? "LMM" : "LM") + ": SNP " + std::to_string(snp_cnt));
+      info(std::string("trans/gw ") + model_tag(model) + ": SNP " + std::to_string(snp_cnt));
+  }
+  if (gene_major) {
+    std::stable_sort(gm_hits.begin(), gm_hits.end(), [](const eqtl_hit& a, const eqtl_hit& b) {
+      return a.gene != b.gene ? a.gene < b.gene : a.snp < b.snp;
+    });
+    for (const auto& e : gm_hits) {
+      AssocHit h;
+      fill(h, e);
+      write_pair_line(out.pairs, h, model, out.tag);
+    }
   }
   std::vector<eqtl_gene_state> st(G);
   eng.check(eqtl_gpu_scan_end(c, st.data()), "scan_end");
@@ -918,10 +978,13 @@ static uint32_t fnv1a(const std::string& s) {
 void scan_cis(const Options& opt, Engine& eng, GenoSource& bed, Model model, ScopeOut& out, double pthr,
               std::vector<TransJob>& jobs, const PhenoData& ph, const CovData& cov) {
   if (jobs.empty()) return;
-  std::stable_sort(jobs.begin(), jobs.end(), [](const TransJob& a, const TransJob& b) {
-    if (a.loc.chrom != b.loc.chrom) return a.loc.chrom < b.loc.chrom;
-    return a.loc.tss < b.loc.tss;
-  });
+  // run_cis_parallel (threads > 1) orders genes by (chromosome, TSS); the
+  // single-thread gene loop keeps the phenotype order (scan.cpp:217-300)
+  if (opt.threads > 1)
+    std::stable_sort(jobs.begin(), jobs.end(), [](const TransJob& a, const TransJob& b) {
+      if (a.loc.chrom != b.loc.chrom) return a.loc.chrom < b.loc.chrom;
+      return a.loc.tss < b.loc.tss;
+    });
   const int n = static_cast<int>(ph.n());
   const int G = static_cast<int>(jobs.size());
   std::vector<double> Y(static_cast<size_t>(n) * G);
@@ -973,12 +1036,10 @@ void scan_cis(const Options& opt, Engine& eng, GenoSource& bed, Model model, Sco
     eng.check(eqtl_gpu_upload_bed(c, all, on_dev, static_cast<int64_t>(sites.size()), bed.bytes_per_snp()),
               "upload_bed");
   }
-  const int gm = model == Model::Lmm ? EQTL_MODEL_LMM : EQTL_MODEL_LM;
-  if (model == Model::Lmm)
-    eng.check(eqtl_gpu_lmm_prep(c, Y.data(), G, cov.X.data(), cov.p, opt.delta_mode, nullptr, nullptr, nullptr),
-              "lmm_prep");
-  else
-    eng.check(eqtl_gpu_lm_prep(c, Y.data(), G, cov.X.data(), cov.p, nullptr), "lm_prep");
+  const int gm = engine_model(model);
+  std::vector<double> phi;
+  std::vector<int32_t> conv;
+  prep_genes(opt, eng, model, Y, G, cov, &phi, &conv);
   eng.check(eqtl_gpu_set_gene_windows(c, gch.data(), gcs.data(), gce.data(), G), "set_gene_windows");
   eng.check(eqtl_gpu_scan_begin(c, gm, opt.precision, EQTL_SCOPE_CIS, pthr), "scan_begin");
   std::vector<eqtl_hit> hits(1 << 20), all_hits;
@@ -1012,6 +1073,10 @@ void scan_cis(const Options& opt, Engine& eng, GenoSource& bed, Model model, Sco
     h.p = e.p;
     h.r2 = e.r2;
     h.n = n;
+    if (model == Model::Glm) {
+      h.phi = phi[e.gene];
+      h.glm_converged = conv[e.gene] != 0;
+    }
     if (job.loc.ok) {
       h.has_tss_dist = true;
       h.tss_dist = static_cast<double>(s.pos - job.loc.tss);
@@ -1034,7 +1099,7 @@ void scan_cis(const Options& opt, Engine& eng, GenoSource& bed, Model model, Sco
       hash[k] = fnv1a(jobs[k].gene);
       best[k] = st[k].n_tested > 0 ? st[k].best_p : std::numeric_limits<double>::quiet_NaN();
     }
-    info(std::string("cis ") + (model == Model::Lmm ? "LMM" : "LM") + ": " + std::to_string(opt.perm) +
+    info(std::string("cis ") + model_tag(model) + ": " + std::to_string(opt.perm) +
          " gene-level permutations");
     std::vector<double> perm_min_p(static_cast<size_t>(G) * opt.perm);
     eng.check(eqtl_gpu_cis_perm(c, opt.precision, opt.perm, opt.seed, hash.data(), best.data(), opt.delta_mode,
@@ -1169,6 +1234,9 @@ int run_eqtl(const Options& opt) {
     set_sample_order(eng, bed, order);
   }
   eng.check(eqtl_gpu_set_policy(eng.ctx(), opt.miss == MissHand::Impute, opt.max_miss, opt.maf), "set_policy");
+  for (Model model : opt.models)
+    if (model == Model::Glm && opt.perm > 0)
+      die("--perm: gene-level permutations are provided for lm and lmm (the NB glm's parametric resampling is not)");
   for (Model model : opt.models) {
     for (const std::string& scope : scopes) {
       const std::string prefix = opt.out + "." + model_str(model) + "." + scope;
@@ -1184,6 +1252,10 @@ int run_eqtl(const Options& opt) {
       const double pthr = (scope == "cis") ? opt.pval_cis : opt.pval_trans;
       std::vector<TransJob> jobs;
       for (size_t gi = 0; gi < ph.gene_ids.size(); ++gi) {
+        if (model == Model::Glm && !looks_like_counts(&ph.Y[gi * ph.n()], ph.n())) {  // scan.cpp:246-249
+          warn("skip non-count gene for glm: " + ph.gene_ids[gi]);
+          continue;
+        }
         TransJob job;
         job.gene = ph.gene_ids[gi];
         job.col = static_cast<int>(gi);
@@ -1207,11 +1279,11 @@ int run_eqtl(const Options& opt) {
         jobs.push_back(std::move(job));
       }
       if (scope == "cis") {
-        info(std::string("cis ") + (model == Model::Lmm ? "LMM" : "LM") + ": " + std::to_string(jobs.size()) +
+        info(std::string("cis ") + model_tag(model) + ": " + std::to_string(jobs.size()) +
              " genes");
         scan_cis(opt, eng, bed, model, so, pthr, jobs, ph, cov);
       } else {
-        info(std::string("trans/gw ") + (model == Model::Lmm ? "LMM" : "LM") + ": SNP-outer (" +
+        info(std::string("trans/gw ") + model_tag(model) + ": SNP-outer (" +
              std::to_string(jobs.size()) + " genes)");
         scan_snp_outer(opt, eng, bed, model, scope, so, pthr, jobs, ph, cov, nullptr, nullptr);
       }

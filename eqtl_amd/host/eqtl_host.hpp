@@ -185,6 +185,8 @@ struct AssocHit {
   std::string gene, snp, chrom, ref, alt;
   int64_t pos = 0;
   double maf = 0, beta = 0, se = 0, stat = 0, p = 1, r2 = 0, tss_dist = 0;
+  double phi = 0;              // NB dispersion of the gene's null fit (glm)
+  bool glm_converged = true;
   int n = 0;
   bool has_tss_dist = false;
 };

@@ -206,3 +206,83 @@ def test_glm_scan_parity(gpu_ctx, prec, cis):
     assert np.array_equal(st["best_snp"], rs["best_snp"])
     assert rel_err(st["acat_p"], rs["acat_p"]).max() < 1e-2
     assert np.all(hits["r2"] == 0)
+
+
+def _read_tsv(path):
+    with open(path) as f:
+        head = f.readline().rstrip("\n").split("\t")
+        return head, [ln.rstrip("\n").split("\t") for ln in f if ln.strip()]
+
+
+def _table(path):
+    with open(path) as f:
+        cols = f.readline().rstrip("\n").split("\t")[1:]
+        ids, vals = [], []
+        for ln in f:
+            t = ln.rstrip("\n").split("\t")
+            ids.append(t[0])
+            vals.append([float(x) for x in t[1:]])
+    return ids, cols, np.array(vals)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["cis", "trans"])
+def test_cli_glm_on_smoke_counts(tmp_path, mode):
+    """--model glm on the reference's smoke data (VCF genotypes, count
+    phenotypes, one covariate): every pair line equals the oracle's NB score
+    test of the oracle-decoded dosage (se 1e-5 relative, beta and stat to 1e-5
+    of se and of 1, p 1e-4),
+    phi equals the oracle's null fit, pairs are gene-major in phenotype order,
+    and each gene's n_tested equals the oracle's count of its SNPs."""
+    from _vcf import SMOKE, read_vcf
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "eqtl_amd", "eqtl")
+    vcf = os.path.join(SMOKE, "smoke.vcf.gz")
+    counts, covar, gff = (os.path.join(SMOKE, f) for f in ("smoke.counts.tsv", "smoke.covar.tsv", "smoke.gff"))
+    out = str(tmp_path / "g")
+    w = 50000
+    r = subprocess.run([cli, "-v", vcf, "-e", counts, "-c", covar, "-g", gff, "--model", "glm", "--mode", mode,
+                        "-w", str(w), "--pval-cis", "1", "--pval-trans", "1", "--precision", "fp32", "-o", out],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    head, pairs = _read_tsv(f"{out}.glm.{mode}.pairs.tsv")
+    assert head[-2:] == ["phi", "glm_converged"] and len(pairs) > 50
+    samples, ds_float, lines = read_vcf(vcf)
+    cid, genes, Y = _table(counts)
+    vid, vcols, V = _table(covar)
+    X = np.column_stack([np.ones(len(cid)), V[[vid.index(s) for s in cid]]])
+    sc = np.array([samples.index(s) for s in cid], np.int32)
+    row = {ln.split(b"\t", 3)[2].decode(): j for j, ln in enumerate(lines)}
+    dos = {}
+    phis = {}
+    order = []
+    for pr in pairs:
+        gene, snp = pr[0], pr[1]
+        if not order or order[-1][0] != gene:
+            assert all(gene != o[0] for o in order), "pairs not gene-major"
+            order.append((gene, []))
+        order[-1][1].append(row[snp])
+        if snp not in dos:
+            st, d, *_ = O.vcf_record(lines[row[snp]], len(samples), sample_col=sc)
+            assert st == 1
+            dos[snp] = d
+        y = Y[:, genes.index(gene)]
+        if gene not in phis:
+            phis[gene] = O.glm_prep(y, X)[1]
+        b, se, stat, pv = O.glm_test(y, X, dos[snp])
+        got = [float(v) for v in (pr[7], pr[8], pr[9], pr[10])]
+        # beta near 0 is set by the score's absolute error: compare it in units of se
+        assert got[0] == pytest.approx(b, rel=1e-5, abs=1e-5 * se) and got[1] == pytest.approx(se, rel=1e-5)
+        assert got[2] == pytest.approx(stat, rel=1e-5, abs=1e-5) and got[3] == pytest.approx(pv, rel=1e-4)
+        assert float(pr[15]) == pytest.approx(phis[gene], rel=1e-8) and pr[16] == "1"
+        assert pr[11] == "0"  # r2
+    gidx = [genes.index(g) for g, _ in order]
+    assert gidx == sorted(gidx)  # phenotype order
+    assert all(js == sorted(js) for _, js in order)
+    head, region = _read_tsv(f"{out}.glm.{mode}.region.tsv")
+    n_by_gene = {g: len(js) for g, js in order}
+    for rg in region:
+        if rg[0] in n_by_gene:
+            assert int(rg[3]) == n_by_gene[rg[0]]  # --pval 1: every tested SNP is a pair
+    r = subprocess.run([cli, "-v", vcf, "-e", counts, "-g", gff, "--model", "glm", "--mode", "cis", "--perm", "10",
+                        "-o", out + "p"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1 and "permutations are provided for lm and lmm" in r.stderr

@@ -125,6 +125,127 @@ def accuracy_sample(ctx, rows_dev, bps, X, Y, win, delta, planted, prec, Q, lam)
             "max_abs_dt": float(np.max(np.abs(tg[ok] - ref["stat"][ok])))}
 
 
+def glm_counts(Y, seed=1005):
+    """NB2 counts from the workload's phenotypes (planted effects included):
+    mu = base * exp(0.25 y), base ~ U(5, 50), phi ~ U(0.05, 1) per gene."""
+    rng = np.random.default_rng(seed)
+    Gn = Y.shape[1]
+    base = rng.uniform(5, 50, Gn)
+    phi = rng.uniform(0.05, 1.0, Gn)
+    mu = base[None, :] * np.exp(0.25 * Y)
+    lam = rng.gamma(1.0 / phi[None, :], phi[None, :] * mu)
+    return rng.poisson(lam).astype(np.float64)
+
+
+def glm_pass(ctx, rows_dev, bps, X, Yc, m, prec, pthr=1e-5):
+    import eqtl_amd
+    phi, conv = ctx.glm_prep(Yc, X)
+    ctx.scan_begin(eqtl_amd.MODEL_GLM, prec, pthr)
+    nh = 0
+    for j0 in range(0, m, BLOCK):
+        mb = min(BLOCK, m - j0)
+        hits = ctx.scan_block(None, j0, hit_cap=1 << 20, rows_dev=rows_dev + j0 * bps, m=mb, bps=bps)
+        nh += hits.shape[0]
+    return phi, conv, ctx.scan_end(), nh
+
+
+def bench_glm(args):
+    """Trans NB GLM score-test scan on the metric's configuration (n = 2000,
+    500k SNPs, 20k count genes, p = 21); one step = the per-gene IRLS null fits
+    + decode + fused score-test kernel over all SNPs."""
+    import eqtl_amd
+    import oracle as O
+    global BLOCK
+    BLOCK = args.block
+    m = min(args.snps, M)
+    ctx = eqtl_amd.Context(0)
+    t0 = time.perf_counter()
+    rows_dev, bps, chrom, pos, X, Y, win, planted = make_inputs(ctx)
+    Yc = glm_counts(Y)
+    del Y
+    log(f"inputs ready {time.perf_counter() - t0:.1f}s")
+    ctx.set_samples(N)
+    ctx.set_policy(True, 0.8, 0.0)
+    ctx.set_snp_annot(chrom, pos)
+    ctx.set_gene_windows(*win)
+    prec = eqtl_amd.PREC_BF16X3
+    for _ in range(args.warmup):
+        glm_pass(ctx, rows_dev, bps, X, Yc, min(m, 4 * BLOCK), prec)
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    tests = 0
+    for s in range(args.steps):
+        phi, conv, st, nh = glm_pass(ctx, rows_dev, bps, X, Yc, m, prec)
+        tests += int(st["n_tested"].sum())
+        log(f"step {s}: {int(st['n_tested'].sum())} tests, {nh} hits, {int(conv.sum())} genes fitted")
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    wald_ms, wald_n = ctx.timing("wald")
+    prep_ms, _ = ctx.timing("glm_prep")
+    dec_ms, _ = ctx.timing("decode")
+    ctx.set_timing(False)
+    log(f"step time {dt / args.steps:.2f}s: kernel {wald_ms / args.steps:.0f} ms, null fits "
+        f"{prep_ms / args.steps:.0f} ms, decode {dec_ms / args.steps:.0f} ms")
+    p = X.shape[1]
+    launch_ms = wald_ms / max(wald_n, 1)
+    achieved = 2.0 * N * (p + 2) * (tests / max(wald_n, 1)) / (launch_ms * 1e-3) / 1e12
+    extra = {}
+    if not args.no_accuracy:
+        # split form vs the FP64 oracle on sampled pairs (planted genes and SNPs included)
+        rng = np.random.default_rng(8)
+        genes = np.unique(np.concatenate([np.arange(48), rng.choice(G, 48, replace=False)]))
+        snps = np.unique(np.concatenate([np.array([planted[k] for k in genes if k in planted]),
+                                         rng.choice(M, 300, replace=False)]))
+        rows = np.concatenate([ctx.copy_rows(rows_dev, int(j), 1, bps) for j in snps])
+        Ys = Yc[:, genes]
+        ctx.glm_prep(Ys, X)
+        ctx.set_gene_windows(win[0][genes], win[1][genes], win[2][genes])
+        ctx.set_snp_annot(np.zeros(len(snps), np.int32), np.arange(len(snps), dtype=np.int64))
+        hits, _ = ctx.scan(eqtl_amd.MODEL_GLM, prec, rows, pthr=1.0, block=4096,
+                           hit_cap=len(snps) * len(genes) + 1)
+        dos, _, keep, _ = O.decode_bed(rows, N)
+        ref = O.glm_scan(dos, keep, np.zeros(len(snps)), np.arange(len(snps)), Ys, X, pthr=1.0, dense=True,
+                         hit_cap=1)
+        pg = np.full(ref["p"].shape, np.nan)
+        pg[hits["snp"], hits["gene"]] = hits["p"]
+        ok = np.isfinite(ref["p"]) & (ref["p"] >= 1e-300)
+        extra["accuracy_vs_fp64"] = {"pairs": int(ok.sum()), "min_p": float(ref["p"][ok].min()),
+                                     "max_rel_p": float(np.max(np.abs(pg[ok] - ref["p"][ok]) / ref["p"][ok]))}
+        log("accuracy", extra["accuracy_vs_fp64"])
+    cpu = None
+    if not args.no_cpu:
+        rows_s = ctx.copy_rows(rows_dev, 0, 200, bps)
+        dos, _, keep, _ = O.decode_bed(rows_s, N)
+        threads = O.num_threads()
+        Ys = Yc[:, :1000]
+        t1 = time.perf_counter()
+        ref = O.glm_scan(dos, keep, np.zeros(200), np.arange(200), Ys, X, pthr=1e-5, hit_cap=1 << 16)
+        dtc = time.perf_counter() - t1
+        nt = int(ref["state"]["n_tested"].sum())
+        cpu = {"value": nt / dtc, "unit": "NB score tests/s", "cores": threads, "kind": "port",
+               "sample": f"200 SNPs x 1000 genes of the same data through the oracle's restatement of "
+                         f"prep_glm_nb + test_glm_nb (null fits included) on {threads} threads "
+                         f"({nt} tests in {dtc:.1f} s)"}
+        log("cpu baseline", cpu)
+    out = {
+        "metric": "NB GLM score tests/s, trans (split-BF16 contraction, FP64 epilogue)",
+        "value": tests / dt, "unit": "NB score tests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3 contractions, fp64 epilogue", "data": "synthetic (generated in HBM, seeded; NB2 counts)",
+        "config": {"workload": "trans NB GLM n=2000 samples x 500k SNPs x 20k count genes, p=21, 1% missing",
+                   "snps_per_step": m, "genes": G, "samples": N, "covariates": p, "snps_per_block": BLOCK},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                     "note": "algorithmic 2n(p+2) FLOP/test over the fused kernel's event-timed launches"},
+        "cpu_baseline": cpu,
+        "kernel_ms_per_launch": launch_ms, "null_fit_ms_per_step": prep_ms / args.steps,
+        **extra,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
 def cpu_baseline(Q, lam, X, Y, rows, delta):
     """FP64 oracle (port of the reference's SNP loop) timed on the host cores."""
     import oracle as O
@@ -311,8 +432,9 @@ def main():
     ap.add_argument("--block", type=int, default=BLOCK, help="SNPs per scan block")
     ap.add_argument("--probe-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (reference kernel for HBM byte counters)")
-    ap.add_argument("--workload", choices=["trans-lmm", "vcf-gt", "vcf-ds"], default="trans-lmm",
-                    help="trans-lmm: the association scan (default); vcf-gt / vcf-ds: the device VCF decode")
+    ap.add_argument("--workload", choices=["trans-lmm", "trans-glm", "vcf-gt", "vcf-ds"], default="trans-lmm",
+                    help="trans-lmm: the association scan (default); trans-glm: the NB score-test scan; "
+                         "vcf-gt / vcf-ds: the device VCF decode")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
@@ -325,6 +447,9 @@ def main():
         torch.cuda.synchronize()
         log("copy probe: 1 GiB read + 1 GiB written")
         del x, y
+    if args.workload == "trans-glm":
+        bench_glm(args)
+        return
     if args.workload != "trans-lmm":
         bench_vcf(args, args.workload == "vcf-ds")
         return

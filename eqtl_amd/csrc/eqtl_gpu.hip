@@ -39,8 +39,7 @@ constexpr double kRefineTauF32 = 5e-4;
 constexpr double kRefineTauBF16X3 = 2e-3;
 // |t| above which the numerator is refined as well (two-sided p < ~1e-15).
 constexpr double kRefineTauHi = 8.0;
-// NB score tests with |z| > 8 get the FP64 score (the information stays from
-// the contraction; no polish pass)
+// NB score tests with |z| > 8 are recomputed in FP64 after the kernel (polish)
 constexpr double kGlmTauHi = 8.0;
 // Capacity of the per-block list of LMM tests polished in FP64 (|t| > 8), and
 // how many are polished per launch (g~ scratch: chunk x n doubles).
@@ -1260,7 +1259,7 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     c->hit_cap_dev = cap_dev;
   }
   HIPC(hipMemsetAsync(c->hit_count.p, 0, 2 * sizeof(unsigned long long), c->stream));
-  const bool polish_on = c->scan_model == EQTL_MODEL_LMM;
+  const bool polish_on = c->scan_model != EQTL_MODEL_LM;
   if (polish_on) CHK(ensure(c->polish, sizeof(PolishItem) * static_cast<size_t>(kPolishCap)));
   WaldArgs a;
   a.A = Aop;
@@ -1335,8 +1334,10 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     pa.lam = c->lam64.as<double>();
     pa.delta = c->delta.as<double>();
     pa.U = c->U64.as<double>();
-    pa.Xt = c->Xt64.as<double>();
+    pa.Xt = glm ? c->X64.as<double>() : c->Xt64.as<double>();
     pa.Lc = c->Lc.as<double>();
+    pa.w = glm ? c->V64.as<double>() : nullptr;
+    pa.mean = glm ? c->gtg.as<double>() : nullptr;
     pa.gene_scal = c->gscal.as<double>();
     pa.af = c->af.as<double>();
     pa.df = a.df;
@@ -1349,8 +1350,13 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     for (int64_t i0 = 0; i0 < npol; i0 += chunk) {
       pa.items = c->polish.as<PolishItem>() + i0;
       pa.count = std::min<int64_t>(chunk, npol - i0);
-      hipLaunchKernelGGL(k_polish_lmm, dim3(static_cast<unsigned>(pa.count)), dim3(256), 0, c->stream, pa);
-      CHK(check_launch("k_polish_lmm"));
+      if (glm)
+        hipLaunchKernelGGL(k_polish<EQTL_MODEL_GLM>, dim3(static_cast<unsigned>(pa.count)), dim3(256), 0, c->stream,
+                           pa);
+      else
+        hipLaunchKernelGGL(k_polish<EQTL_MODEL_LMM>, dim3(static_cast<unsigned>(pa.count)), dim3(256), 0, c->stream,
+                           pa);
+      CHK(check_launch("k_polish"));
       hipLaunchKernelGGL(k_polish_best, dim3(static_cast<unsigned>((pa.count + 255) / 256)), dim3(256), 0,
                          c->stream, c->state.as<GeneAcc>(), pa.out, pa.count);
       CHK(check_launch("k_polish_best"));

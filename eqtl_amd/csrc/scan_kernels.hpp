@@ -309,10 +309,10 @@ __device__ inline bool refine_wanted(const WaldArgs& a, bool ok, double gg, doub
   const double tau2 = a.tau * a.tau, hi2 = a.tau_hi * a.tau_hi;
   if constexpr (MODEL == EQTL_MODEL_GLM) {
     // score test: U = num + mean * sum(r) (gtg carries the SNP mean, gscal sum r)
+    // (tests with |z| > tau_hi are polished in FP64 after the kernel)
     if (S < 1e-12) return false;
     const double U = num + gtg * gscal;
-    const double z2S = U * U;  // z^2 = z2S / S
-    return z2S < tau2 * S || z2S > hi2 * S;
+    return U * U < tau2 * S;  // z^2 = U^2 / S
   } else if constexpr (MODEL == EQTL_MODEL_LMM) {
     if (gg < 1e-12 || S <= 1e-15) return false;
     const double beta = num / S;
@@ -400,9 +400,9 @@ __device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, i
   }
   if (!isfinite(p)) return;
   ++st.n_tested;
-  // LMM tests with |t| > tau_hi are polished in FP64 after the kernel, which
-  // adds their ACAT term (an accepted p stays accepted: both are < 1e-15)
-  const bool polish = MODEL == EQTL_MODEL_LMM && fabs(stat) > a.tau_hi;
+  // LMM / GLM tests with |t| > tau_hi are polished in FP64 after the kernel,
+  // which adds their ACAT term (an accepted p stays accepted: both are < 1e-15)
+  const bool polish = MODEL != EQTL_MODEL_LM && fabs(stat) > a.tau_hi;
   double term;
   if (acat_accept(p, &term)) {
     if (!polish) asum = dd_add_d(asum, term);
@@ -661,7 +661,7 @@ __global__ __launch_bounds__(256) void k_wald_lm(WaldArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// FP64 polish of LMM tests with |t| > tau_hi (strong signals, p far below any
+// FP64 polish of LMM / GLM tests with |t| > tau_hi (strong signals, p far below any
 // reporting threshold, where the contraction's relative error in S is
 // amplified by |t| and by 1/(1 - r^2)): test_lmm_gtil recomputed from the
 // FP64 centred dosage, one workgroup per test:
@@ -682,6 +682,8 @@ struct PolishArgs {
   const double* U;      // G x n: s o r in the rotated space
   const double* Xt;     // n x p col-major (rotated covariates)
   const double* Lc;     // G x p x p, lower, col-major
+  const double* w;      // GLM: G x n null working weights
+  const double* mean;   // GLM: block-local SNP centring means
   const double* gene_scal;
   const double* af;
   double df, lbeta;
@@ -692,32 +694,47 @@ struct PolishArgs {
   GeneAcc* state;       // running per-gene state (ACAT term of the polished p)
 };
 
-__global__ __launch_bounds__(256) void k_polish_lmm(PolishArgs a) {
+// GLM (test_glm_nb, model_glm.cpp:100-132): the same quantities without the
+// rotation — weights w_k, covariates X, residual r_k — and U = num + mean * sum r,
+// info = S, two-sided normal p.
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_polish(PolishArgs a) {
   __shared__ double red[4];
   __shared__ double va[34];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const PolishItem it = a.items[blockIdx.x];
   const int n = a.n, p = a.p, k = it.gene;
   const double* gc = a.A64 + static_cast<size_t>(it.j) * a.n_pad;
-  double* gt = a.gt + static_cast<size_t>(blockIdx.x) * n;
-  for (int c = wave; c < n; c += 4) {
-    const double* qc = a.Q + static_cast<size_t>(c) * n;
-    double s = 0;
-    for (int i = lane; i < n; i += 64) s = fma(qc[i], gc[i], s);
+  const double* gt = gc;
+  if constexpr (MODEL == EQTL_MODEL_LMM) {
+    double* gr = a.gt + static_cast<size_t>(blockIdx.x) * n;
+    for (int c = wave; c < n; c += 4) {
+      const double* qc = a.Q + static_cast<size_t>(c) * n;
+      double s = 0;
+      for (int i = lane; i < n; i += 64) s = fma(qc[i], gc[i], s);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) gt[c] = s;
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+      if (lane == 0) gr[c] = s;
+    }
+    __syncthreads();
+    gt = gr;
   }
-  __syncthreads();
-  const double dk = a.delta[k];
+  const double dk = MODEL == EQTL_MODEL_LMM ? a.delta[k] : 0.0;
   const double* u = a.U + static_cast<size_t>(k) * n;
+  const double* wk = MODEL == EQTL_MODEL_GLM ? a.w + static_cast<size_t>(k) * n : nullptr;
   double part[34];
 #pragma unroll
   for (int c = 0; c < 34; ++c) part[c] = 0;
   for (int i = threadIdx.x; i < n; i += 256) {
-    double v = dk * a.lam[i] + 1.0;
-    if (v < 1e-12) v = 1e-12;
-    const double d = 1.0 / v, g = gt[i], dg = d * g;
+    double d;
+    if constexpr (MODEL == EQTL_MODEL_GLM) {
+      d = wk[i];
+    } else {
+      double v = dk * a.lam[i] + 1.0;
+      if (v < 1e-12) v = 1e-12;
+      d = 1.0 / v;
+    }
+    const double g = gt[i], dg = d * g;
     part[0] += dg * g;
     part[1] += u[i] * g;
 #pragma unroll
@@ -749,7 +766,16 @@ __global__ __launch_bounds__(256) void k_polish_lmm(PolishArgs a) {
   }
   const double gg = va[0], num = va[1], S = gg - ss, gscal = a.gene_scal[k], df = a.df;
   double beta = 0, se = 0, stat = 0, pv = 1.0, r2 = 0;
-  if (!(gg < 1e-12) && !(S <= 1e-15)) {
+  if constexpr (MODEL == EQTL_MODEL_GLM) {
+    if (!(S < 1e-12)) {
+      const double U = num + a.mean[it.j] * gscal;
+      const double rs = sqrt(S);
+      beta = U / S;
+      se = 1.0 / rs;
+      stat = U / rs;
+      pv = isfinite(stat) ? fmin(1.0, fmax(0.0, erfc(fabs(stat) * 0.70710678118654752440))) : 1.0;
+    }
+  } else if (!(gg < 1e-12) && !(S <= 1e-15)) {
     beta = num / S;
     double qq = gscal - S * beta * beta;
     if (qq < 0) qq = 0;

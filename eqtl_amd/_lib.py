@@ -40,6 +40,7 @@ EXPORTS = [
     "eqtl_gpu_set_timing", "eqtl_gpu_get_timing", "eqtl_gpu_synth_bed", "eqtl_gpu_copy_rows",
     "eqtl_gpu_synchronize", "eqtl_gpu_selftest", "eqtl_gpu_upload_bed", "eqtl_gpu_cis_perm",
     "eqtl_gpu_beta_approx", "eqtl_gpu_set_geno_format", "eqtl_gpu_vcf_decode", "eqtl_gpu_glm_prep",
+    "eqtl_gpu_set_test_subset",
 ]
 
 _lib = None
@@ -122,8 +123,19 @@ class Context:
         sc = np.ascontiguousarray(sample_col, dtype=np.int32)
         _check(lib().eqtl_gpu_set_samples(self._c, ctypes.c_int32(n_file), _p(sc), ctypes.c_int32(sc.size)))
         self.n = sc.size
+        self.n_src = sc.size
         self.n_file = n_file
         self.bps = (n_file + 3) // 4
+
+    def set_test_subset(self, keep=None):
+        """Restrict tests to analysis samples keep (increasing); None restores all."""
+        if keep is None:
+            _check(lib().eqtl_gpu_set_test_subset(self._c, None, ctypes.c_int32(0)))
+            self.n = self.n_src
+            return
+        k = np.ascontiguousarray(keep, dtype=np.int32)
+        _check(lib().eqtl_gpu_set_test_subset(self._c, _p(k), ctypes.c_int32(k.size)))
+        self.n = k.size
 
     def set_policy(self, impute=True, max_miss=0.8, maf_min=0.0):
         _check(lib().eqtl_gpu_set_policy(self._c, ctypes.c_int32(1 if impute else 0), ctypes.c_double(max_miss),

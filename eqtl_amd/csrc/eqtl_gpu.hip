@@ -120,6 +120,12 @@ struct eqtl_gpu_ctx {
   // samples / policy
   int n_file = 0, n = 0, n_pad = 0;
   DBuf sample_col;
+  std::vector<int32_t> h_sample_col;
+  // test subset: n (below) test samples out of n_src analysis samples;
+  // test_col = file columns of the test samples (subset active only)
+  int n_src = 0;
+  bool subset = false;
+  DBuf test_col;
   DBuf col_used;  // n_file: how many analysis samples read each file column
   bool col_used_all = false;  // every file column read exactly once
   int geno_fmt = EQTL_GENO_BED;
@@ -251,9 +257,11 @@ int run_decode(eqtl_gpu_ctx* c, const uint8_t* rows_dev, int64_t m, int32_t bps,
   a.m_pad = m_pad;
   a.bps = bps;
   a.n = c->n;
+  a.n_src = c->n_src;
   a.n_file = c->n_file;
   a.n_pad = c->n_pad;
   a.sample_col = c->sample_col.as<int32_t>();
+  a.test_col = c->subset ? c->test_col.as<int32_t>() : nullptr;
   a.impute = c->impute;
   a.max_miss = c->max_miss;
   a.maf_min = c->maf_min;
@@ -415,6 +423,9 @@ int eqtl_gpu_set_samples(eqtl_gpu_ctx* c, int32_t n_file, const int32_t* sample_
   }
   c->n_file = n_file;
   c->n = n;
+  c->n_src = n;
+  c->subset = false;
+  c->h_sample_col.assign(sample_col, sample_col + n);
   c->n_pad = static_cast<int>(round_up(n, 64));
   std::vector<uint8_t> used(static_cast<size_t>(n_file), 0);
   for (int i = 0; i < n; ++i) {
@@ -427,6 +438,37 @@ int eqtl_gpu_set_samples(eqtl_gpu_ctx* c, int32_t n_file, const int32_t* sample_
   HIPC(hipMemcpyAsync(c->sample_col.p, sample_col, sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
   HIPC(hipMemcpyAsync(c->col_used.p, used.data(), static_cast<size_t>(n_file), hipMemcpyHostToDevice, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
+  return EQTL_OK;
+}
+
+int eqtl_gpu_set_test_subset(eqtl_gpu_ctx* c, const int32_t* keep, int32_t nk) {
+  if (!c) return fail(EQTL_E_ARG, "null ctx");
+  if (c->n_src <= 0) return fail(EQTL_E_STATE, "set_samples first");
+  bool identity = !keep || nk == c->n_src;
+  if (keep) {
+    if (nk < 1 || nk > c->n_src) return fail(EQTL_E_ARG, "bad test subset size");
+    for (int i = 0; i < nk; ++i) {
+      if (keep[i] < 0 || keep[i] >= c->n_src || (i > 0 && keep[i] <= keep[i - 1]))
+        return fail(EQTL_E_ARG, "test subset indices must be strictly increasing analysis-sample indices");
+      identity = identity && keep[i] == i;
+    }
+  }
+  CHK(set_device(c));
+  const int n_new = identity ? c->n_src : nk;
+  if (c->n != n_new || c->subset || !identity) {
+    c->have_basis = false;
+    c->prepped_model = -1;
+  }
+  c->n = n_new;
+  c->n_pad = static_cast<int>(round_up(n_new, 64));
+  c->subset = !identity;
+  if (c->subset) {
+    std::vector<int32_t> tc(nk);
+    for (int i = 0; i < nk; ++i) tc[i] = c->h_sample_col[keep[i]];
+    CHK(ensure(c->test_col, sizeof(int32_t) * nk));
+    HIPC(hipMemcpyAsync(c->test_col.p, tc.data(), sizeof(int32_t) * nk, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+  }
   return EQTL_OK;
 }
 
@@ -469,7 +511,7 @@ int eqtl_gpu_vcf_decode(eqtl_gpu_ctx* c, const char* text, int32_t on_dev, int64
   a.recs = c->vcf_recs.as<eqtl_vcf_record>();
   a.m = m;
   a.n_file = c->n_file;
-  a.n = c->n;
+  a.n = c->n_src;
   a.used = c->col_used.as<uint8_t>();
   a.used_all = c->col_used_all ? 1 : 0;
   a.impute = c->impute;
@@ -1587,6 +1629,7 @@ int eqtl_gpu_beta_approx(eqtl_gpu_ctx* c, const double* perm_min_p, const double
 int eqtl_gpu_grm_begin(eqtl_gpu_ctx* c) {
   if (!c) return fail(EQTL_E_ARG, "null ctx");
   if (c->n <= 0) return fail(EQTL_E_STATE, "set_samples first");
+  if (c->subset) return fail(EQTL_E_STATE, "the GRM is built over the analysis samples: clear the test subset");
   CHK(set_device(c));
   const size_t nn = static_cast<size_t>(c->n_pad) * c->n_pad;
   CHK(ensure(c->grm_acc, nn * sizeof(double)));

@@ -60,7 +60,9 @@ struct DecodeArgs {
   const uint8_t* rows;
   int64_t m, m_pad;
   int32_t bps, n, n_file, n_pad;
-  const int32_t* sample_col;
+  int32_t n_src;              // analysis samples (source rules, imputation)
+  const int32_t* sample_col;  // n_src file columns
+  const int32_t* test_col;    // nullable: n file columns of the test subset (else sample_col, n == n_src)
   int32_t impute;
   double max_miss, maf_min;
   float* A;            // m_pad x n_pad, centred imputed dosage (fp32)
@@ -93,7 +95,7 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
   const uint8_t* row = a.rows + j * a.bps;
   int n_miss = 0, bad = 0;
   double sum = 0.0, sum2 = 0.0;  // integer-valued for BED rows (exact)
-  for (int i = lane; i < a.n; i += kWave) {
+  for (int i = lane; i < a.n_src; i += kWave) {
     const int col = a.sample_col[i];
     if (col < 0 || col >= a.n_file) { bad = 1; continue; }
     bool miss;
@@ -109,13 +111,13 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
     sum2 += __shfl_xor(sum2, o);
     bad |= __shfl_xor(bad, o);
   }
-  const int n = a.n;
-  const int n_ok = n - n_miss;
+  const int n_src = a.n_src;
+  const int n_ok = n_src - n_miss;
   bool keep = !bad && n_ok > 0;
   double mu = 0, muc = 0, afs = __builtin_nan("");
   if constexpr (FMT == EQTL_GENO_BED) {
     if (keep) {
-      const double miss_frac = static_cast<double>(n_miss) / static_cast<double>(n);
+      const double miss_frac = static_cast<double>(n_miss) / static_cast<double>(n_src);
       if (miss_frac > a.max_miss + 1e-15) keep = false;
       if (!a.impute && n_miss > 0) keep = false;
     }
@@ -126,10 +128,29 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
       if (a.maf_min > 0.0 && (afd < a.maf_min || afd > 1.0 - a.maf_min)) keep = false;  // pass_maf
     }
   }
+  const int n = a.n;
+  const int32_t* tcol = a.test_col ? a.test_col : a.sample_col;
   if (keep) {
     mu = sum / n_ok;
-    // subset_maf_or_nan on the imputed vector
-    const double sI = sum + n_miss * mu, s2I = sum2 + n_miss * mu * mu;
+    // subset_maf_or_nan on the imputed vector (restricted to the test subset)
+    double sI = sum + n_miss * mu, s2I = sum2 + n_miss * mu * mu;
+    if (a.test_col) {
+      double s1 = 0.0, s2 = 0.0;
+      for (int i = lane; i < n; i += kWave) {
+        bool miss;
+        const double g = geno_at<FMT>(row, tcol[i], miss);
+        const double v = miss ? mu : g;
+        s1 += v;
+        s2 += v * v;
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
+      }
+      sI = s1;
+      s2I = s2;
+    }
     double af = (sI / n) / 2.0;
     af = fmin(1.0, fmax(0.0, af));
     const double maf = af > 0.5 ? 1.0 - af : af;
@@ -141,7 +162,7 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
   for (int i = lane; i < a.n_pad; i += kWave) {
     double v = 0.0, dose = __builtin_nan("");
     if (i < n) {
-      const int col = a.sample_col[i];
+      const int col = tcol[i];
       if (col >= 0 && col < a.n_file) {
         bool miss;
         const double g = geno_at<FMT>(row, col, miss);

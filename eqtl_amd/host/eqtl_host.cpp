@@ -11,6 +11,7 @@
 #include <cstring>
 #include <iostream>
 #include <limits>
+#include <map>
 #include <numeric>
 #include <sstream>
 #include <unordered_set>
@@ -136,6 +137,7 @@ int parse_options(int argc, char** argv, Options& opt) {
       {"permutations", required_argument, 0, 1008}, {"perm-trans-thr", required_argument, 0, 1020},
       {"perm-trans-top", required_argument, 0, 1021}, {"seed", required_argument, 0, 1009},
       {"disable-beta-approx", no_argument, 0, 1010}, {"help", no_argument, 0, 'h'},
+      {"fast", no_argument, 0, 1007},
       {"precision", required_argument, 0, 2001}, {"delta", required_argument, 0, 2002},
       {"device", required_argument, 0, 2003},   {0, 0, 0, 0}};
   if (argc <= 1) return 2;
@@ -164,6 +166,7 @@ int parse_options(int argc, char** argv, Options& opt) {
         else if (std::string(optarg) == "impute") opt.miss = MissHand::Impute;
         else die("invalid --miss-hand (filter|impute)");
         break;
+      case 1007: opt.fast = true; break;
       case 1008: opt.perm = std::atoi(optarg); break;
       case 1009: opt.seed = std::atoi(optarg); break;
       case 1010: opt.disable_beta_approx = true; break;
@@ -533,6 +536,25 @@ void write_pair_line(std::ostream& os, const AssocHit& h, Model m, const std::st
     os.write(buf, n);
   } else {
     die("pair line too long");
+  }
+}
+
+void emit_line(ScopeOut& out, bool top, int64_t k1, int64_t k2, const AssocHit& h, Model m, const std::string& scope) {
+  if (!out.buffered) {
+    write_pair_line(top ? out.top : out.pairs, h, m, scope);
+    return;
+  }
+  std::ostringstream os;
+  write_pair_line(os, h, m, scope);
+  (top ? out.top_buf : out.pair_buf).emplace_back(std::make_pair(k1, k2), os.str());
+}
+
+void flush_lines(ScopeOut& out) {
+  for (auto* b : {&out.pair_buf, &out.top_buf}) {
+    std::stable_sort(b->begin(), b->end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+    std::ostream& os = b == &out.pair_buf ? static_cast<std::ostream&>(out.pairs) : static_cast<std::ostream&>(out.top);
+    for (const auto& kv : *b) os << kv.second;
+    b->clear();
   }
 }
 
@@ -913,7 +935,7 @@ void scan_snp_outer(const Options& opt, Engine& eng, GenoSource& bed, Model mode
       }
       AssocHit h;
       fill(h, e);
-      write_pair_line(out.pairs, h, model, out.tag);
+      emit_line(out, false, e.snp, jobs[e.gene].order, h, model, out.tag);
     }
     const size_t before = snp_cnt;
     snp_cnt += m;
@@ -927,7 +949,7 @@ void scan_snp_outer(const Options& opt, Engine& eng, GenoSource& bed, Model mode
     for (const auto& e : gm_hits) {
       AssocHit h;
       fill(h, e);
-      write_pair_line(out.pairs, h, model, out.tag);
+      emit_line(out, false, jobs[e.gene].order, e.snp, h, model, out.tag);
     }
   }
   std::vector<eqtl_gene_state> st(G);
@@ -954,7 +976,7 @@ void scan_snp_outer(const Options& opt, Engine& eng, GenoSource& bed, Model mode
       e.r2 = st[k].best_r2;
       AssocHit h;
       fill(h, e);
-      write_pair_line(out.top, h, model, scope);
+      emit_line(out, true, job.order, 0, h, model, scope);
     }
   }
   if (stage2) stage2_perm(opt, eng, bed, model, jobs, cand, st, Y, cov);
@@ -978,13 +1000,6 @@ static uint32_t fnv1a(const std::string& s) {
 void scan_cis(const Options& opt, Engine& eng, GenoSource& bed, Model model, ScopeOut& out, double pthr,
               std::vector<TransJob>& jobs, const PhenoData& ph, const CovData& cov) {
   if (jobs.empty()) return;
-  // run_cis_parallel (threads > 1) orders genes by (chromosome, TSS); the
-  // single-thread gene loop keeps the phenotype order (scan.cpp:217-300)
-  if (opt.threads > 1)
-    std::stable_sort(jobs.begin(), jobs.end(), [](const TransJob& a, const TransJob& b) {
-      if (a.loc.chrom != b.loc.chrom) return a.loc.chrom < b.loc.chrom;
-      return a.loc.tss < b.loc.tss;
-    });
   const int n = static_cast<int>(ph.n());
   const int G = static_cast<int>(jobs.size());
   std::vector<double> Y(static_cast<size_t>(n) * G);
@@ -1088,7 +1103,7 @@ void scan_cis(const Options& opt, Engine& eng, GenoSource& bed, Model model, Sco
   for (const auto& e : all_hits) {
     AssocHit h;
     fill(h, e);
-    write_pair_line(out.pairs, h, model, "cis");
+    emit_line(out, false, jobs[e.gene].order, e.snp, h, model, "cis");
   }
   std::vector<double> p_emp(G, std::numeric_limits<double>::quiet_NaN());
   std::vector<double> p_beta(G, std::numeric_limits<double>::quiet_NaN()), shape1(G, 0.0), shape2(G, 0.0);
@@ -1141,7 +1156,7 @@ void scan_cis(const Options& opt, Engine& eng, GenoSource& bed, Model model, Sco
       e.r2 = st[k].best_r2;
       AssocHit h;
       fill(h, e);
-      write_pair_line(out.top, h, model, "cis");
+      emit_line(out, true, job.order, 0, h, model, "cis");
     }
   }
 }
@@ -1173,13 +1188,53 @@ int run_make_grm(const Options& opt) {
   return 0;
 }
 
-static bool all_finite(const PhenoData& ph, int col, const CovData& cov) {
+// sparsify_grm (model_lmm.cpp:88-104): off-diagonal |K_ij| < thr set to 0.
+static void sparsify_grm(std::vector<double>& K, int n, double thr) {
+  size_t n_zero = 0;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < i; ++j)
+      if (std::fabs(K[static_cast<size_t>(j) * n + i]) < thr) {
+        K[static_cast<size_t>(j) * n + i] = 0.0;
+        K[static_cast<size_t>(i) * n + j] = 0.0;
+        n_zero += 2;
+      }
+  info("fast: GRM sparse approx thr=" + std::to_string(thr) + " zeroed " + std::to_string(n_zero) +
+       " off-diagonal entries");
+}
+
+// Analysis samples with a finite phenotype and finite covariates
+// (build_gene_ready, scan_common.cpp:11-25).
+static std::vector<int32_t> finite_keep(const PhenoData& ph, int col, const CovData& cov) {
   const size_t n = ph.n();
-  for (size_t i = 0; i < n; ++i)
-    if (!std::isfinite(ph.Y[static_cast<size_t>(col) * n + i])) return false;
-  for (double v : cov.X)
-    if (!std::isfinite(v)) return false;
-  return true;
+  std::vector<int32_t> keep;
+  for (size_t i = 0; i < n; ++i) {
+    if (!std::isfinite(ph.Y[static_cast<size_t>(col) * n + i])) continue;
+    bool ok = true;
+    for (int c = 0; c < cov.p && ok; ++c) ok = std::isfinite(cov.X[static_cast<size_t>(c) * n + i]);
+    if (ok) keep.push_back(static_cast<int32_t>(i));
+  }
+  return keep;
+}
+
+// The phenotypes of jobs (rows keep) and the covariates on keep.
+static void subset_inputs(const PhenoData& ph, const CovData& cov, const std::vector<int32_t>& keep,
+                          std::vector<TransJob>& jobs, PhenoData& ph_g, CovData& cov_g) {
+  const size_t n = ph.n(), nk = keep.size();
+  ph_g.sample_ids.clear();
+  for (int32_t i : keep) ph_g.sample_ids.push_back(ph.sample_ids[i]);
+  ph_g.gene_ids.clear();
+  ph_g.Y.assign(nk * jobs.size(), 0.0);
+  for (size_t k = 0; k < jobs.size(); ++k) {
+    ph_g.gene_ids.push_back(jobs[k].gene);
+    for (size_t r = 0; r < nk; ++r) ph_g.Y[k * nk + r] = ph.Y[static_cast<size_t>(jobs[k].col) * n + keep[r]];
+    jobs[k].col = static_cast<int>(k);
+  }
+  cov_g.sample_ids = ph_g.sample_ids;
+  cov_g.cov_names = cov.cov_names;
+  cov_g.p = cov.p;
+  cov_g.X.assign(nk * cov.p, 0.0);
+  for (int c = 0; c < cov.p; ++c)
+    for (size_t r = 0; r < nk; ++r) cov_g.X[static_cast<size_t>(c) * nk + r] = cov.X[static_cast<size_t>(c) * n + keep[r]];
 }
 
 int run_eqtl(const Options& opt) {
@@ -1219,20 +1274,15 @@ int run_eqtl(const Options& opt) {
   Engine eng(opt.device);
   bool any_lmm = false;
   for (Model m : opt.models) any_lmm |= (m == Model::Lmm);
+  Grm grm;
   if (any_lmm) {
-    Grm grm;
     if (!opt.grm.empty()) grm = slice_grm(load_grm_gcta(opt.grm), order);
     else {
       info("computing GRM from genotypes (overlap samples)");
       grm = compute_grm(eng, bed, order, opt.miss, opt.max_miss, opt.maf);
     }
-    set_sample_order(eng, bed, order);
-    const int rc = eqtl_gpu_eigh(eng.ctx(), grm.K.data(), grm.n(), nullptr, nullptr);
-    if (rc == EQTL_E_SOLVER) die("GRM eigen decomposition failed");
-    eng.check(rc, "eigh");
-  } else {
-    set_sample_order(eng, bed, order);
   }
+  set_sample_order(eng, bed, order);
   eng.check(eqtl_gpu_set_policy(eng.ctx(), opt.miss == MissHand::Impute, opt.max_miss, opt.maf), "set_policy");
   for (Model model : opt.models)
     if (model == Model::Glm && opt.perm > 0)
@@ -1269,24 +1319,74 @@ int run_eqtl(const Options& opt) {
           }
         }
         if ((scope == "trans" || scope == "cis") && !job.has_loc) continue;
-        // build_gene_ready (scan_common.cpp:11-35): need >= 3 complete samples and a non-zero y
-        if (!all_finite(ph, job.col, cov))
-          die("gene " + job.gene + " has missing phenotype/covariate values: per-gene sample subsets are not "
-              "provided by this engine");
+        // build_gene_ready (scan_common.cpp:11-35): >= 3 samples with finite
+        // phenotype and covariates, and a non-zero y on them
+        job.keep = finite_keep(ph, job.col, cov);
         double ymax = 0;
-        for (size_t i = 0; i < ph.n(); ++i) ymax = std::max(ymax, std::fabs(ph.Y[gi * ph.n() + i]));
-        if (ph.n() < 3 || ymax < 1e-15) continue;
+        for (int32_t i : job.keep) ymax = std::max(ymax, std::fabs(ph.Y[gi * ph.n() + i]));
+        if (job.keep.size() < 3 || ymax < 1e-15) continue;
         jobs.push_back(std::move(job));
       }
-      if (scope == "cis") {
-        info(std::string("cis ") + model_tag(model) + ": " + std::to_string(jobs.size()) +
-             " genes");
-        scan_cis(opt, eng, bed, model, so, pthr, jobs, ph, cov);
-      } else {
-        info(std::string("trans/gw ") + model_tag(model) + ": SNP-outer (" +
-             std::to_string(jobs.size()) + " genes)");
-        scan_snp_outer(opt, eng, bed, model, scope, so, pthr, jobs, ph, cov, nullptr, nullptr);
+      // run_cis_parallel (threads > 1) orders cis genes by (chromosome, TSS);
+      // the single-thread gene loop keeps the phenotype order (scan.cpp:217-300)
+      if (scope == "cis" && opt.threads > 1)
+        std::stable_sort(jobs.begin(), jobs.end(), [](const TransJob& a, const TransJob& b) {
+          if (a.loc.chrom != b.loc.chrom) return a.loc.chrom < b.loc.chrom;
+          return a.loc.tss < b.loc.tss;
+        });
+      for (size_t k = 0; k < jobs.size(); ++k) jobs[k].order = static_cast<int>(k);
+      // genes grouped by sample subset, in order of first appearance; one
+      // group (the usual case: no missing values) streams its output
+      std::vector<std::vector<int>> groups;
+      {
+        std::map<std::vector<int32_t>, int> gid;
+        for (size_t k = 0; k < jobs.size(); ++k) {
+          auto it = gid.find(jobs[k].keep);
+          if (it == gid.end()) {
+            it = gid.emplace(jobs[k].keep, static_cast<int>(groups.size())).first;
+            groups.emplace_back();
+          }
+          groups[it->second].push_back(static_cast<int>(k));
+        }
       }
+      so.buffered = groups.size() > 1;
+      if (groups.size() > 1)
+        info(std::to_string(groups.size()) + " distinct sample subsets (missing phenotype/covariate values)");
+      for (const auto& grp : groups) {
+        std::vector<TransJob> gjobs;
+        for (int k : grp) gjobs.push_back(jobs[k]);
+        const std::vector<int32_t> keep = gjobs[0].keep;
+        const bool full = keep.size() == ph.n();
+        eng.check(eqtl_gpu_set_test_subset(eng.ctx(), full ? nullptr : keep.data(), static_cast<int32_t>(keep.size())),
+                  "set_test_subset");
+        PhenoData ph_g;
+        CovData cov_g;
+        subset_inputs(ph, cov, keep, gjobs, ph_g, cov_g);
+        if (model == Model::Lmm) {
+          // the GRM on the subset (slice of K); --fast sparsifies it where the
+          // reference does: a basis shared by all genes, and per-gene cis bases
+          const int nk = static_cast<int>(keep.size());
+          std::vector<double> Kg(static_cast<size_t>(nk) * nk);
+          for (int b = 0; b < nk; ++b)
+            for (int a = 0; a < nk; ++a)
+              Kg[static_cast<size_t>(b) * nk + a] = grm.K[static_cast<size_t>(keep[b]) * ph.n() + keep[a]];
+          if (opt.fast && (groups.size() == 1 || scope == "cis")) sparsify_grm(Kg, nk, 1e-4);
+          const int rc = eqtl_gpu_eigh(eng.ctx(), Kg.data(), nk, nullptr, nullptr);
+          if (rc == EQTL_E_SOLVER) die("GRM eigen decomposition failed");
+          eng.check(rc, "eigh");
+        }
+        if (scope == "cis") {
+          info(std::string("cis ") + model_tag(model) + ": " + std::to_string(gjobs.size()) + " genes");
+          scan_cis(opt, eng, bed, model, so, pthr, gjobs, ph_g, cov_g);
+        } else {
+          info(std::string("trans/gw ") + model_tag(model) + ": SNP-outer (" + std::to_string(gjobs.size()) +
+               " genes)");
+          scan_snp_outer(opt, eng, bed, model, scope, so, pthr, gjobs, ph_g, cov_g, nullptr, nullptr);
+        }
+        for (size_t i = 0; i < grp.size(); ++i) jobs[grp[i]].summary = gjobs[i].summary;
+      }
+      eng.check(eqtl_gpu_set_test_subset(eng.ctx(), nullptr, 0), "set_test_subset");
+      if (so.buffered) flush_lines(so);
       std::vector<GeneSummary> summaries;
       for (auto& j : jobs) summaries.push_back(j.summary);
       bh_adjust(summaries);

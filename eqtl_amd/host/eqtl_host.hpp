@@ -36,6 +36,7 @@ struct Options {
   int perm_trans_top = 1000;
   int seed = -1;
   bool disable_beta_approx = false;
+  bool fast = false;  // --fast: sparsified GRM (|K_ij| < 1e-4 -> 0) for the LMM bases
   // engine-specific
   int precision = EQTL_PREC_BF16X3;
   int delta_mode = EQTL_DELTA_REF;
@@ -207,7 +208,14 @@ void bh_adjust(std::vector<GeneSummary>& gs);
 struct ScopeOut {
   std::ofstream pairs, top, region;
   std::string tag;
+  // per-gene sample subsets run group by group: lines are then buffered with
+  // their sort keys and written in the reference's order at the end
+  bool buffered = false;
+  std::vector<std::pair<std::pair<int64_t, int64_t>, std::string>> pair_buf, top_buf;
 };
+// One pair / top line keyed (k1, k2) for the buffered order.
+void emit_line(ScopeOut& out, bool top, int64_t k1, int64_t k2, const AssocHit& h, Model m, const std::string& scope);
+void flush_lines(ScopeOut& out);
 
 // ---- GRM (grm.cpp:10-108) --------------------------------------------------
 struct Grm {
@@ -241,7 +249,9 @@ struct TransJob {
   std::string gene;
   GeneLoc loc;
   bool has_loc = false;
-  int col = -1;  // phenotype column
+  int col = -1;    // phenotype column
+  int order = 0;   // position in the scope's output order
+  std::vector<int32_t> keep;  // analysis samples with finite phenotype and covariates
   GeneSummary summary;
   AssocHit best;
 };

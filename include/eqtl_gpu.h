@@ -86,6 +86,16 @@ int eqtl_gpu_device_info(eqtl_gpu_ctx* ctx, char* name, int32_t name_len, int32_
 /* Analysis sample order: sample_col[i] = fam column of analysis sample i
  * (PlinkBed::set_sample_order, plink_bed.cpp:245-255). */
 int eqtl_gpu_set_samples(eqtl_gpu_ctx* ctx, int32_t n_file, const int32_t* sample_col, int32_t n);
+/* Test subset (build_gene_ready, scan_common.cpp:11-63: the samples whose
+ * phenotype and covariates are finite): keep[0..nk) are strictly increasing
+ * indices into the analysis samples of eqtl_gpu_set_samples. Genotypes are
+ * still decoded, filtered and mean-imputed over all analysis samples (the
+ * source's SnpRec), then restricted to the subset (subset_dosage), where
+ * subset_maf_or_nan and every test run; genes prepared afterwards, the
+ * eigenbasis (K of nk x nk) and the scans use the nk subset samples.
+ * keep = NULL (or the full identity) restores the analysis set. The GRM
+ * entry points need the full set. */
+int eqtl_gpu_set_test_subset(eqtl_gpu_ctx* ctx, const int32_t* keep, int32_t nk);
 /* Missing policy and MAF gate (MissPolicy + maf_min). impute: 1 = mean-impute,
  * 0 = drop any SNP with a missing call. */
 int eqtl_gpu_set_policy(eqtl_gpu_ctx* ctx, int32_t impute, double max_miss, double maf_min);

@@ -1,6 +1,7 @@
 // Host side of the engine (see eqtl_host.hpp). Each function names the
 // reference code whose behaviour it mirrors.
 #include "eqtl_host.hpp"
+#include "pheno_norm.hpp"
 
 #include <getopt.h>
 
@@ -137,7 +138,7 @@ int parse_options(int argc, char** argv, Options& opt) {
       {"permutations", required_argument, 0, 1008}, {"perm-trans-thr", required_argument, 0, 1020},
       {"perm-trans-top", required_argument, 0, 1021}, {"seed", required_argument, 0, 1009},
       {"disable-beta-approx", no_argument, 0, 1010}, {"help", no_argument, 0, 'h'},
-      {"fast", no_argument, 0, 1007},
+      {"fast", no_argument, 0, 1007},           {"pheno-norm", required_argument, 0, 1022},
       {"precision", required_argument, 0, 2001}, {"delta", required_argument, 0, 2002},
       {"device", required_argument, 0, 2003},   {0, 0, 0, 0}};
   if (argc <= 1) return 2;
@@ -167,6 +168,11 @@ int parse_options(int argc, char** argv, Options& opt) {
         else die("invalid --miss-hand (filter|impute)");
         break;
       case 1007: opt.fast = true; break;
+      case 1022:  // options.cpp:228-235
+        if (std::string(optarg) == "none") opt.pheno_int = false;
+        else if (std::string(optarg) == "int") opt.pheno_int = true;
+        else die("invalid --pheno-norm (none|int)");
+        break;
       case 1008: opt.perm = std::atoi(optarg); break;
       case 1009: opt.seed = std::atoi(optarg); break;
       case 1010: opt.disable_beta_approx = true; break;
@@ -210,6 +216,11 @@ int parse_options(int argc, char** argv, Options& opt) {
     return 1;
   }
   if (opt.window < 0) die("--window must be >= 0");
+  if (opt.pheno_int)  // options.cpp:289-300
+    for (Model m : opt.models)
+      if (m == Model::Glm || m == Model::Glmm)
+        die("--pheno-norm int is incompatible with count model " + model_str(m) +
+            " (apply INT only for lm/lmm; count models need raw integer counts)");
   return 0;
 }
 
@@ -1246,6 +1257,8 @@ int run_eqtl(const Options& opt) {
                                          : "no overlapping samples between pheno and VCF");
   info("overlap samples: " + std::to_string(order.size()));
   slice_pheno(ph, order);
+  if (opt.pheno_int)  // apply_pheno_norm per gene (scan_common.hpp:124-126)
+    for (size_t k = 0; k < ph.G(); ++k) inverse_normal_transform(&ph.Y[k * ph.n()], ph.n());
   CovData cov = load_covar(opt.covar, order);
   std::unordered_map<std::string, GeneLoc> annot;
   const bool have_gff = !opt.gff.empty();

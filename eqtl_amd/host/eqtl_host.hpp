@@ -37,6 +37,7 @@ struct Options {
   int seed = -1;
   bool disable_beta_approx = false;
   bool fast = false;  // --fast: sparsified GRM (|K_ij| < 1e-4 -> 0) for the LMM bases
+  bool pheno_int = false;  // --pheno-norm int: per-gene rank inverse normal transform
   // engine-specific
   int precision = EQTL_PREC_BF16X3;
   int delta_mode = EQTL_DELTA_REF;

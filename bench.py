@@ -157,7 +157,7 @@ def bench_glm(args):
     import oracle as O
     global BLOCK
     BLOCK = args.block
-    m = min(args.snps, M)
+    m = min(args.snps or M, M)
     ctx = eqtl_amd.Context(0)
     t0 = time.perf_counter()
     rows_dev, bps, chrom, pos, X, Y, win, planted = make_inputs(ctx)
@@ -420,6 +420,392 @@ def bench_vcf(args, ds):
     ctx.close()
 
 
+def lm_pass(ctx, rows_dev, bps, X, Y, m, prec, pthr=1e-5):
+    import eqtl_amd
+    ctx.lm_prep(Y, X)
+    ctx.scan_begin(eqtl_amd.MODEL_LM, prec, pthr)
+    nh = 0
+    for j0 in range(0, m, BLOCK):
+        mb = min(BLOCK, m - j0)
+        hits = ctx.scan_block(None, j0, hit_cap=1 << 20, rows_dev=rows_dev + j0 * bps, m=mb, bps=bps)
+        nh += hits.shape[0]
+    return ctx.scan_end(), nh
+
+
+def bench_trans_lm(args):
+    """Trans LM scan on the metric's genotypes and phenotypes (n = 2000, 500k
+    SNPs, 20k genes, p = 21); one step = per-gene residualisation (prep_lm) +
+    decode + fused LM kernel (per SNP FP64 g^T g, per test one length-n dot
+    product) + per-gene reductions over all SNPs."""
+    import eqtl_amd
+    import oracle as O
+    global BLOCK
+    BLOCK = args.block
+    m = min(args.snps or M, M)
+    ctx = eqtl_amd.Context(0)
+    t0 = time.perf_counter()
+    rows_dev, bps, chrom, pos, X, Y, win, planted = make_inputs(ctx)
+    log(f"inputs ready {time.perf_counter() - t0:.1f}s")
+    ctx.set_samples(N)
+    ctx.set_policy(True, 0.8, 0.0)
+    ctx.set_snp_annot(chrom, pos)
+    ctx.set_gene_windows(*win)
+    prec = eqtl_amd.PREC_BF16X3
+    for _ in range(args.warmup):
+        lm_pass(ctx, rows_dev, bps, X, Y, min(m, 4 * BLOCK), prec)
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    tests = 0
+    for s in range(args.steps):
+        st, nh = lm_pass(ctx, rows_dev, bps, X, Y, m, prec)
+        tests += int(st["n_tested"].sum())
+        log(f"step {s}: {int(st['n_tested'].sum())} tests, {nh} hits")
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    wald_ms, wald_n = ctx.timing("wald")
+    dec_ms, _ = ctx.timing("decode")
+    ctx.set_timing(False)
+    log(f"step time {dt / args.steps:.2f}s: kernel {wald_ms / args.steps:.0f} ms, decode {dec_ms / args.steps:.0f} ms")
+    launch_ms = wald_ms / max(wald_n, 1)
+    achieved = 2.0 * N * (tests / max(wald_n, 1)) / (launch_ms * 1e-3) / 1e12
+    extra = {}
+    if not args.no_accuracy:
+        rng = np.random.default_rng(9)
+        genes = np.unique(np.concatenate([np.arange(64), rng.choice(G, 64, replace=False)]))
+        snps = np.unique(np.concatenate([np.array([planted[k] for k in genes if k in planted]),
+                                         rng.choice(M, 400, replace=False)]))
+        rows = np.concatenate([ctx.copy_rows(rows_dev, int(j), 1, bps) for j in snps])
+        Ys = Y[:, genes]
+        ctx.lm_prep(Ys, X)
+        ctx.set_gene_windows(win[0][genes], win[1][genes], win[2][genes])
+        ctx.set_snp_annot(np.zeros(len(snps), np.int32), np.arange(len(snps), dtype=np.int64))
+        hits, _ = ctx.scan(eqtl_amd.MODEL_LM, prec, rows, pthr=1.0, block=4096, hit_cap=len(snps) * len(genes) + 1)
+        dos, _, keep, _ = O.decode_bed(rows, N)
+        ref = O.trans_scan("lm", dos, keep, np.zeros(len(snps)), np.arange(len(snps)), Ys, X, pthr=1.0, dense=True,
+                           hit_cap=1)
+        pg = np.full(ref["p"].shape, np.nan)
+        pg[hits["snp"], hits["gene"]] = hits["p"]
+        ok = np.isfinite(ref["p"]) & (ref["p"] >= 1e-300)
+        extra["accuracy_vs_fp64"] = {"pairs": int(ok.sum()), "min_p": float(ref["p"][ok].min()),
+                                     "max_rel_p": float(np.max(np.abs(pg[ok] - ref["p"][ok]) / ref["p"][ok]))}
+        log("accuracy", extra["accuracy_vs_fp64"])
+    cpu = None
+    if not args.no_cpu:
+        ns = 1000
+        rows_s = ctx.copy_rows(rows_dev, 0, ns, bps)
+        dos, _, keep, _ = O.decode_bed(rows_s, N)
+        threads = O.num_threads()
+        t1 = time.perf_counter()
+        ref = O.trans_scan("lm", dos, keep, np.zeros(ns), np.arange(ns), Y, X, pthr=1e-5, hit_cap=1 << 16)
+        dtc = time.perf_counter() - t1
+        nt = int(ref["state"]["n_tested"].sum())
+        cpu = {"value": nt / dtc, "unit": "LM tests/s", "cores": threads, "kind": "port",
+               "sample": f"{ns} SNPs x {G} genes of the same data through the oracle's restatement of the "
+                         f"reference SNP-outer LM loop (prep_lm per gene, then residualize_g + GEMV + "
+                         f"hit_from_gty per SNP) on {threads} threads, prep included ({nt} tests in {dtc:.1f} s)"}
+        log("cpu baseline", cpu)
+    out = {
+        "metric": "LM tests/s, trans (split-BF16 contraction, FP64 epilogue)",
+        "value": tests / dt, "unit": "LM tests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3 contractions, fp64 g^T g and epilogue", "data": "synthetic (generated in HBM, seeded)",
+        "config": {"workload": "trans LM n=2000 samples x 500k SNPs x 20k genes, p=21, 1% missing",
+                   "snps_per_step": m, "genes": G, "samples": N, "covariates": X.shape[1], "snps_per_block": BLOCK},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                     "note": "algorithmic 2n FLOP/test over the fused LM kernel's event-timed launches"},
+        "cpu_baseline": cpu,
+        "kernel_ms_per_launch": launch_ms,
+        **extra,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+CIS_N, CIS_M, CIS_G, CIS_NCOV, CIS_NCHROM = 1000, 1_000_000, 20_000, 15, 10
+
+
+def cis_inputs(ctx, n_genes, seed=1004):
+    """Cis permutation workload: n = 1000, 1M SNPs on 10 chromosomes (100 bp
+    apart), a 20k-gene table with window sizes c_g = round(exp(U(ln 50, ln 5000)))
+    SNPs (run with a zero flank, so a gene's window holds exactly c_g SNPs), of
+    which a seeded slice of n_genes genes is run, sorted by (chromosome, start);
+    Y ~ N(0, 1), 5% of the genes with one cis eQTL |beta| ~ U(0.1, 0.5);
+    15 covariates + intercept."""
+    from eqtl_amd import synth
+    rng = np.random.default_rng(seed)
+    bps = (CIS_N + 3) // 4
+    rows_dev = ctx.synth_bed(seed, CIS_M, CIS_N, 0.01, 0.5, 0.01)
+    per = CIS_M // CIS_NCHROM
+    chrom = np.repeat(np.arange(1, CIS_NCHROM + 1, dtype=np.int32), per)
+    pos = np.tile(1 + 100 * np.arange(per, dtype=np.int64), CIS_NCHROM)
+    cg = np.rint(np.exp(rng.uniform(np.log(50), np.log(5000), CIS_G))).astype(np.int64)
+    gch = rng.integers(1, CIS_NCHROM + 1, CIS_G).astype(np.int32)
+    start = 1 + (rng.random(CIS_G) * (per * 100 - 100 * cg)).astype(np.int64)
+    end = start + 100 * cg - 1
+    sel = rng.choice(CIS_G, n_genes, replace=False)
+    sel = sel[np.lexsort((start[sel], gch[sel]))]
+    X = np.hstack([np.ones((CIS_N, 1)), rng.standard_normal((CIS_N, CIS_NCOV))])
+    Y = rng.standard_normal((CIS_N, n_genes))
+    for k in range(n_genes):
+        if rng.random() < 0.05:
+            g0 = sel[k]
+            first = (gch[g0] - 1) * per + (start[g0] - 1 + 99) // 100
+            j = int(first + rng.integers(0, cg[g0]))
+            g = synth.unpack_bed(ctx.copy_rows(rows_dev, j, 1, bps), CIS_N)[0].astype(float)
+            g[g < 0] = g[g >= 0].mean()
+            Y[:, k] += rng.uniform(0.1, 0.5) * (1 if rng.random() < 0.5 else -1) * (g - g.mean())
+    names = [f"g{int(k)}" for k in sel]
+    return rows_dev, bps, chrom, pos, X, Y, (gch[sel], start[sel], end[sel]), cg[sel], names
+
+
+def cis_model_pass(ctx, model, rows_dev, bps, X, Y, win, hashes, B, seed, prec, delta_mode):
+    """Observed cis scan of every gene's window, then B gene-level permutations
+    (eqtl_gpu_cis_perm). Returns (observed tests, permuted tests, p_emp)."""
+    import eqtl_amd
+    if model == eqtl_amd.MODEL_LM:
+        ctx.lm_prep(Y, X)
+    else:
+        ctx.lmm_prep(Y, X, delta_mode=delta_mode)
+    ctx.set_gene_windows(*win)
+    ctx.scan_begin(model, prec, 1e-5, eqtl_amd.SCOPE_CIS)
+    for j0 in range(0, CIS_M, BLOCK):
+        mb = min(BLOCK, CIS_M - j0)
+        ctx.scan_block(None, j0, hit_cap=1 << 20, rows_dev=rows_dev + j0 * bps, m=mb, bps=bps)
+    st = ctx.scan_end()
+    pm, pe = ctx.cis_perm(prec, B, seed, hashes, st["best_p"], delta_mode=delta_mode)
+    n_obs = st["n_tested"].astype(np.int64)
+    return int(n_obs.sum()), int(n_obs[np.isfinite(pe)].sum()) * B, pe
+
+
+def bench_cis_perm(args):
+    """Cis LM + LMM scans with B = 1000 gene-level permutations (the
+    --model lm,lmm --mode cis -w 0 --perm 1000 run) on a gene slice of the
+    20k-gene workload; one step = for each model the per-gene prep, the observed
+    window scan and all permutations (per (gene, b): mt19937 shuffle, null fit
+    with a fresh REML delta for the LMM, window scan)."""
+    import eqtl_amd
+    import oracle as O
+    global BLOCK
+    BLOCK = args.block
+    B, seed = args.perm, 42
+    ctx = eqtl_amd.Context(0)
+    t0 = time.perf_counter()
+    rows_dev, bps, chrom, pos, X, Y, win, cg, names = cis_inputs(ctx, args.genes)
+    ctx.set_samples(CIS_N)
+    ctx.set_policy(True, 0.8, 0.0)
+    import ctypes
+    from eqtl_amd._lib import lib, _check
+    _check(lib().eqtl_gpu_grm_begin(ctx.handle))
+    for j0 in range(0, CIS_M, BLOCK):
+        mb = min(BLOCK, CIS_M - j0)
+        _check(lib().eqtl_gpu_grm_accumulate(ctx.handle, ctypes.c_void_p(rows_dev + j0 * bps), ctypes.c_int32(1),
+                                             ctypes.c_int64(mb), ctypes.c_int32(bps)))
+    K, _, _ = ctx.grm_finish()
+    Q, lam = ctx.eigh(K)
+    ctx.set_snp_annot(chrom, pos)
+    ctx.upload_bed(rows_dev=rows_dev, m=CIS_M, bps=bps)
+    hashes = np.array([O.fnv1a(nm) for nm in names], dtype=np.uint32)
+    log(f"inputs ready {time.perf_counter() - t0:.1f}s: {args.genes} genes, {int(cg.sum())} window SNPs, B={B}")
+    prec, dmode = eqtl_amd.PREC_BF16X3, eqtl_amd.DELTA_REF
+    if args.warmup:
+        w = min(args.genes, 16)
+        for model in (eqtl_amd.MODEL_LM, eqtl_amd.MODEL_LMM):
+            cis_model_pass(ctx, model, rows_dev, bps, X, Y[:, :w], tuple(a[:w] for a in win), hashes[:w],
+                           min(B, 50), seed, prec, dmode)
+    res = {}
+    ctx.synchronize()
+    t_all = time.perf_counter()
+    for s in range(args.steps):
+        for name, model in (("lm", eqtl_amd.MODEL_LM), ("lmm", eqtl_amd.MODEL_LMM)):
+            ctx.set_timing(True)
+            t1 = time.perf_counter()
+            n_obs, n_perm, pe = cis_model_pass(ctx, model, rows_dev, bps, X, Y, win, hashes, B, seed, prec, dmode)
+            ctx.synchronize()
+            dtm = time.perf_counter() - t1
+            fam = {f: ctx.timing(f) for f in ("wald", "reml", "rotate", "basis", "decode")}
+            ctx.set_timing(False)
+            r = res.setdefault(name, {"s": 0.0, "obs_tests": 0, "perm_tests": 0, "fam": {}})
+            r["s"] += dtm
+            r["obs_tests"] += n_obs
+            r["perm_tests"] += n_perm
+            for f, (ms, nl) in fam.items():
+                a = r["fam"].setdefault(f, [0.0, 0])
+                a[0] += ms
+                a[1] += nl
+            log(f"step {s} {name}: {n_obs} observed, {n_perm} permuted tests in {dtm:.2f}s; " +
+                ", ".join(f"{f} {ms:.0f} ms/{nl}" for f, (ms, nl) in fam.items()))
+    dt = time.perf_counter() - t_all
+    tests = sum(r["obs_tests"] + r["perm_tests"] for r in res.values())
+    p = X.shape[1]
+    # dominant kernel: the LMM permutations' fused Wald kernel (2n(p+2) FLOP per test)
+    lw_ms, lw_n = res["lmm"]["fam"]["wald"]
+    lmm_tests = res["lmm"]["obs_tests"] + res["lmm"]["perm_tests"]
+    achieved = 2.0 * CIS_N * (p + 2) * lmm_tests / (lw_ms * 1e-3) / 1e12 if lw_ms > 0 else None
+    cpu = None
+    if not args.no_cpu:
+        # the oracle's restatement of scan_gene_snps (observed + permutations) on
+        # a few genes of the slice, both models, fewer permutations
+        order = np.argsort(cg, kind="stable")
+        small = [int(k) for k in order[max(0, len(order) // 2 - 1):][:3]]  # median window sizes
+        bc = 8
+        tl = tn = 0.0
+        nl = nn = 0
+        per = CIS_M // CIS_NCHROM
+        for k in small:
+            first = int((win[0][k] - 1) * per + (win[1][k] - 1 + 99) // 100)
+            rows = ctx.copy_rows(rows_dev, first, int(cg[k]), bps)
+            dos, _, keep, _ = O.decode_bed(rows, CIS_N)
+            W = dos[keep]
+            t1 = time.perf_counter()
+            r_lm = O.cis_gene("lm", Y[:, k], X, W, B=bc, seed=seed, gene=names[k])
+            tl += time.perf_counter() - t1
+            nl += r_lm["n_tested"] * (bc + 1)
+            t1 = time.perf_counter()
+            r_lmm = O.cis_gene("lmm", Y[:, k], X, W, B=bc, seed=seed, gene=names[k], Q=Q, lam=lam)
+            tn += time.perf_counter() - t1
+            nn += r_lmm["n_tested"] * (bc + 1)
+        cpu = {"value": (nl + nn) / (tl + tn), "unit": "cis tests/s (observed + permuted)", "cores": 1,
+               "kind": "port",
+               "sample": f"{len(small)} genes of the slice ({', '.join(str(int(cg[k])) for k in small)} window "
+                         f"SNPs), both models, B={bc}, through the oracle's restatement of scan_gene_snps "
+                         f"(Q^T g per LMM test, REML per permutation) on one thread: LM {nl / tl:.3g} tests/s, "
+                         f"LMM {nn / tn:.3g} tests/s"}
+        log("cpu baseline", cpu)
+    out = {
+        "metric": "cis tests/s with gene-level permutations (LM + LMM, observed + permuted)",
+        "value": tests / dt, "unit": "cis tests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3 contractions, fp64 REML and epilogue", "data": "synthetic (generated in HBM, seeded)",
+        "config": {"workload": f"cis LM+LMM n=1000 samples, 1M SNPs, {args.genes} of 20k genes "
+                               f"(windows 50-5000 SNPs), p=16, B={B}",
+                   "genes": args.genes, "window_snps": int(cg.sum()), "permutations": B, "delta": "reference search"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / BF16_PEAK_TFLOPS if achieved else None, "traffic": None,
+                     "note": "LMM permutations' fused Wald kernel: algorithmic 2n(p+2) FLOP per test over its "
+                             "event-timed launches"},
+        "cpu_baseline": cpu,
+        "per_model": {k: {"seconds": v["s"], "observed_tests": v["obs_tests"], "permuted_tests": v["perm_tests"],
+                          "tests_per_s": (v["obs_tests"] + v["perm_tests"]) / v["s"],
+                          "kernel_ms": {f: a[0] for f, a in v["fam"].items()},
+                          "kernel_launches": {f: a[1] for f, a in v["fam"].items()}} for k, v in res.items()},
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
+GRM_N, GRM_M, GRM_G = 5000, 1_000_000, 20_000
+FP64_PEAK_TFLOPS = 78.6
+
+
+def bench_grm(args):
+    """GRM (GCTA form) over n = 5000 samples x 1M SNPs (MAF ~ U(0.01, 0.5), no
+    missing calls); one step = decode + standardise + FP64 Gram accumulation
+    + the GCTA divisor, K copied to the host. Then (untimed by value) the
+    eigendecomposition and the REML adjoint (f, df/dlog delta,
+    d2f/dlog delta^2) of 20k intercept-only genes at their delta."""
+    import ctypes
+    import eqtl_amd
+    import oracle as O
+    from eqtl_amd import synth
+    from eqtl_amd._lib import lib, _check
+    global BLOCK
+    BLOCK = args.block
+    n, m = GRM_N, min(args.snps or GRM_M, GRM_M)
+    bps = (n + 3) // 4
+    ctx = eqtl_amd.Context(0)
+    rows_dev = ctx.synth_bed(1005, m, n, 0.01, 0.5, 0.0)
+    ctx.set_samples(n)
+    ctx.set_policy(True, 0.8, 0.0)
+
+    def grm_pass(mm):
+        _check(lib().eqtl_gpu_grm_begin(ctx.handle))
+        for j0 in range(0, mm, BLOCK):
+            mb = min(BLOCK, mm - j0)
+            _check(lib().eqtl_gpu_grm_accumulate(ctx.handle, ctypes.c_void_p(rows_dev + j0 * bps),
+                                                 ctypes.c_int32(1), ctypes.c_int64(mb), ctypes.c_int32(bps)))
+        return ctx.grm_finish()
+
+    for _ in range(args.warmup):
+        grm_pass(min(m, BLOCK))
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        K, Np, used = grm_pass(m)
+        log(f"step {s}: {used} SNPs used")
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    g_ms, g_n = ctx.timing("grm")
+    ctx.set_timing(False)
+    snps = m * args.steps
+    achieved = float(n) * (n + 1) * snps / (g_ms * 1e-3) / 1e12
+    log(f"GRM step {dt / args.steps:.2f}s, gram kernel {g_ms / args.steps:.0f} ms ({achieved:.1f} TFLOP/s)")
+    extra = {"gram_ms_per_launch": g_ms / max(g_n, 1), "snps_per_launch": BLOCK}
+    # eigendecomposition and the REML adjoint at this n
+    t1 = time.perf_counter()
+    Q, lam = ctx.eigh(K)
+    extra["eigh_s"] = time.perf_counter() - t1
+    rng = np.random.default_rng(1005)
+    causal = np.sort(rng.choice(m, 1000, replace=False))
+    Z = np.empty((1000, n))
+    for i, j in enumerate(causal):
+        Z[i] = synth.standardized(synth.unpack_bed(ctx.copy_rows(rows_dev, int(j), 1, bps), n))[0]
+    h2 = rng.uniform(0, 0.8, GRM_G)
+    u = Z.T @ rng.standard_normal((1000, GRM_G))
+    u = (u - u.mean(0)) / u.std(0)
+    Y = rng.standard_normal((n, GRM_G)) + u * np.sqrt(h2 / (1 - h2))
+    del u, Z
+    X1 = np.ones((n, 1))
+    t1 = time.perf_counter()
+    delta, _ = ctx.lmm_prep(Y, X1, delta_mode=eqtl_amd.DELTA_NEWTON)
+    extra["lmm_prep_s"] = time.perf_counter() - t1
+    ld = np.log(delta)
+    ctx.reml_grad_hess(ld)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    for sh in (0.0, -1.0, 1.0):
+        ctx.reml_grad_hess(ld + sh)
+    t_adj = (time.perf_counter() - t1) / 3
+    adj_bytes = 8.0 * n * (1 + 1) * GRM_G
+    extra["reml_adjoint"] = {"genes": GRM_G, "ms_per_call": 1e3 * t_adj,
+                             "achieved_GBps": adj_bytes / t_adj / 1e9,
+                             "note": "host-timed call (delta upload, weights kernel, two DGEMMs, f/g/h read back); "
+                                     "algorithmic 8n(1+p) B per gene-evaluation"}
+    log(f"eigh {extra['eigh_s']:.2f}s, lmm_prep {extra['lmm_prep_s']:.2f}s, REML adjoint {1e3 * t_adj:.1f} ms/call")
+    cpu = None
+    if not args.no_cpu:
+        ns = 16384
+        rows_s = ctx.copy_rows(rows_dev, 0, ns, bps)
+        dos, miss, keep, _ = O.decode_bed(rows_s, n)
+        threads = O.num_threads()
+        t1 = time.perf_counter()
+        O.grm(dos, miss, keep)
+        dtc = time.perf_counter() - t1
+        cpu = {"value": ns / dtc, "unit": "GRM SNPs/s", "cores": threads, "kind": "port",
+               "sample": f"{ns} SNPs x {n} samples through the oracle's restatement of compute_grm_from "
+                         f"(rank-1 FP64 updates of the lower triangle, {threads} threads, {dtc:.1f} s incl. "
+                         f"the GCTA divisor)"}
+        log("cpu baseline", cpu)
+    out = {
+        "metric": "GRM SNPs/s (GCTA form, FP64)",
+        "value": snps / dt, "unit": "GRM SNPs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (generated in HBM, seeded)",
+        "config": {"workload": f"GRM n={n} samples x {m} SNPs, no missing calls; REML adjoint for {GRM_G} "
+                               f"intercept-only genes", "samples": n, "snps_per_step": m, "snps_per_block": BLOCK},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "note": "k_grm_gram: algorithmic n(n+1) FLOP per SNP (unique lower triangle incl. the "
+                             "diagonal) over its event-timed launches, FP64 matrix peak"},
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
 def main():
     global BLOCK
     ap = argparse.ArgumentParser()
@@ -428,13 +814,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
-    ap.add_argument("--snps", type=int, default=M, help="SNPs per step (default: the full 500k)")
+    ap.add_argument("--snps", type=int, default=None,
+                    help="SNPs per step (default: the workload's full count, 500k for the trans scans)")
     ap.add_argument("--block", type=int, default=BLOCK, help="SNPs per scan block")
+    ap.add_argument("--genes", type=int, default=1000, help="cis-perm: genes of the 20k-gene table that are run")
+    ap.add_argument("--perm", type=int, default=1000, help="cis-perm: permutations per gene")
     ap.add_argument("--probe-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (reference kernel for HBM byte counters)")
-    ap.add_argument("--workload", choices=["trans-lmm", "trans-glm", "vcf-gt", "vcf-ds"], default="trans-lmm",
-                    help="trans-lmm: the association scan (default); trans-glm: the NB score-test scan; "
-                         "vcf-gt / vcf-ds: the device VCF decode")
+    ap.add_argument("--workload", choices=["trans-lmm", "trans-lm", "trans-glm", "cis-perm", "grm", "vcf-gt",
+                                           "vcf-ds"], default="trans-lmm",
+                    help="trans-lmm: the association scan (default); trans-lm: the LM scan; trans-glm: the NB "
+                         "score-test scan; cis-perm: cis LM+LMM with gene-level permutations; grm: the GRM at "
+                         "n=5000 (+ eigh, REML adjoint); vcf-gt / vcf-ds: the device VCF decode")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; multi-GPU runs are not part of this build", file=sys.stderr)
@@ -447,15 +838,14 @@ def main():
         torch.cuda.synchronize()
         log("copy probe: 1 GiB read + 1 GiB written")
         del x, y
-    if args.workload == "trans-glm":
-        bench_glm(args)
-        return
-    if args.workload != "trans-lmm":
-        bench_vcf(args, args.workload == "vcf-ds")
+    other = {"trans-glm": bench_glm, "trans-lm": bench_trans_lm, "cis-perm": bench_cis_perm, "grm": bench_grm,
+             "vcf-gt": lambda a: bench_vcf(a, False), "vcf-ds": lambda a: bench_vcf(a, True)}
+    if args.workload in other:
+        other[args.workload](args)
         return
     import eqtl_amd
     BLOCK = args.block
-    m = min(args.snps, M)
+    m = min(args.snps or M, M)
     ctx = eqtl_amd.Context(0)
     name, ncu, hbm = ctx.device_info()
     log(f"device {name} CUs={ncu} HBM={hbm / 2**30:.0f} GiB")

@@ -672,11 +672,11 @@ static int reml_eval(eqtl_gpu_ctx* c, const std::vector<int32_t>& ek, const std:
     CHK(dgemm_tn(c, TA, nw * Ec, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA));
     CHK(dgemm_tn(c, p, nw * Ec, n, c->Xt64.as<double>(), n, r.Z.as<double>(), n, r.CB.as<double>(), p));
     if (deriv)
-      hipLaunchKernelGGL(k_reml_eval<true>, dim3(Ec), dim3(64), 0, c->stream, r.CA.as<double>(), r.CB.as<double>(),
+      hipLaunchKernelGGL(k_reml_eval_fgh, dim3(Ec), dim3(64), 0, c->stream, r.CA.as<double>(), r.CB.as<double>(),
                          r.S.as<double>(), Ec, TA, p, df, Fo, Go, Ho);
     else
-      hipLaunchKernelGGL(k_reml_eval<false>, dim3(Ec), dim3(64), 0, c->stream, r.CA.as<double>(), r.CB.as<double>(),
-                         r.S.as<double>(), Ec, TA, p, df, Fo, Go, Ho);
+      hipLaunchKernelGGL(k_reml_eval_f, dim3(Ec), dim3(64), 0, c->stream, r.CA.as<double>(), r.CB.as<double>(),
+                         r.S.as<double>(), TA, p, df, Fo);
     CHK(check_launch("k_reml_eval"));
     HIPC(hipMemcpyAsync(F + e0, Fo, sizeof(double) * Ec, hipMemcpyDeviceToHost, c->stream));
     if (deriv) {

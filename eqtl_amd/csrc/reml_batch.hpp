@@ -161,55 +161,68 @@ __device__ inline void load_sym(const double* tri, int p, double* M) {
 }
 
 // LDLT with the pivot taken on the not-yet-updated trailing diagonal (the rule
-// of Eigen 3.4's LDLT, model_lmm.cpp:23), then the explicit inverse. Sums keep
-// the sequential order of the single-threaded form.
-__device__ inline void wave_ldlt_inverse(WaveLdlt& w, int p) {
+// of Eigen 3.4's LDLT, model_lmm.cpp:23). Sums keep the sequential order of
+// the single-threaded form. m(i, j) = L[j * kLD + i], lower triangle used.
+// BORDER: row p holds b^T and m(p, p) = c; the border never pivots, and its
+// last step leaves the Schur complement c - b^T A^-1 b in D[p].
+template <bool BORDER>
+__device__ inline void wave_ldlt_factor(double* L, double* D, double* tmp, int* tr, int* ok, int p) {
   const int lane = threadIdx.x;
-  auto m = [&](int i, int j) -> double& { return w.L[j * kLD + i]; };
-  if (lane == 0) w.ok = 1;
+  const int pe = BORDER ? p + 1 : p;
+  auto m = [&](int i, int j) -> double& { return L[j * kLD + i]; };
+  if (lane == 0) *ok = 1;
   __syncthreads();
-  for (int k = 0; k < p; ++k) {
-    // pivot: largest |m(i,i)|, i >= k, first index on ties
-    double bv = -1.0;
-    int big = p;
-    if (lane >= k && lane < p) { bv = fabs(m(lane, lane)); big = lane; }
+  for (int k = 0; k < pe; ++k) {
+    if (k < p) {
+      // pivot: largest |m(i,i)|, p > i >= k, first index on ties
+      double bv = -1.0;
+      int big = p;
+      if (lane >= k && lane < p) { bv = fabs(m(lane, lane)); big = lane; }
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      const double ov = __shfl_xor(bv, o);
-      const int oi = __shfl_xor(big, o);
-      if (ov > bv || (ov == bv && oi < big)) { bv = ov; big = oi; }
+      for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(bv, o);
+        const int oi = __shfl_xor(big, o);
+        if (ov > bv || (ov == bv && oi < big)) { bv = ov; big = oi; }
+      }
+      if (!(big < p)) big = k;  // all candidates NaN: keep k (as the sequential scan does)
+      if (big != k) {
+        if (lane < k) { const double t = m(k, lane); m(k, lane) = m(big, lane); m(big, lane) = t; }
+        else if (lane > big && lane < pe) { const double t = m(lane, k); m(lane, k) = m(lane, big); m(lane, big) = t; }
+        else if (lane > k && lane < big) { const double t = m(lane, k); m(lane, k) = m(big, lane); m(big, lane) = t; }
+        else if (lane == k) { const double t = m(k, k); m(k, k) = m(big, big); m(big, big) = t; }
+      }
+      if (lane == 0) tr[k] = big;
+      __syncthreads();
     }
-    if (!(big < p)) big = k;  // all candidates NaN: keep k (as the sequential scan does)
-    if (big != k) {
-      if (lane < k) { const double t = m(k, lane); m(k, lane) = m(big, lane); m(big, lane) = t; }
-      else if (lane > big && lane < p) { const double t = m(lane, k); m(lane, k) = m(lane, big); m(lane, big) = t; }
-      else if (lane > k && lane < big) { const double t = m(lane, k); m(lane, k) = m(big, lane); m(big, lane) = t; }
-      else if (lane == k) { const double t = m(k, k); m(k, k) = m(big, big); m(big, big) = t; }
-    }
-    if (lane == 0) w.tr[k] = big;
-    __syncthreads();
     if (k > 0) {
-      if (lane < k) w.tmp[lane] = w.D[lane] * m(k, lane);
+      if (lane < k) tmp[lane] = D[lane] * m(k, lane);
       __syncthreads();
       if (lane == k) {
         double s = 0;
-        for (int j = 0; j < k; ++j) s += m(k, j) * w.tmp[j];
+        for (int j = 0; j < k; ++j) s += m(k, j) * tmp[j];
         m(k, k) -= s;
-      } else if (lane > k && lane < p) {
+      } else if (lane > k && lane < pe) {
         double t = 0;
-        for (int j = 0; j < k; ++j) t += m(lane, j) * w.tmp[j];
+        for (int j = 0; j < k; ++j) t += m(lane, j) * tmp[j];
         m(lane, k) -= t;
       }
       __syncthreads();
     }
     const double akk = m(k, k);
     if (lane == 0) {
-      w.D[k] = akk;
-      if (!isfinite(akk)) w.ok = 0;
+      D[k] = akk;
+      if (k < p && !isfinite(akk)) *ok = 0;
     }
-    if (lane > k && lane < p && fabs(akk) > 0.0) m(lane, k) /= akk;
+    if (lane > k && lane < pe && fabs(akk) > 0.0) m(lane, k) /= akk;
     __syncthreads();
   }
+}
+
+// Factor (above), then the explicit inverse.
+__device__ inline void wave_ldlt_inverse(WaveLdlt& w, int p) {
+  const int lane = threadIdx.x;
+  auto m = [&](int i, int j) -> double& { return w.L[j * kLD + i]; };
+  wave_ldlt_factor<false>(w.L, w.D, w.tmp, w.tr, &w.ok, p);
   // inverse: lane c solves A x = e_c in column c of Ai
   if (lane < p) {
     double* y = w.Ai + lane * kLD;
@@ -327,27 +340,57 @@ __global__ void k_reml_grid_argmin(const double* F, const double* dgrid, int ng,
   best_f[k] = bf;
 }
 
-// One evaluation per 64-thread block: f (and f', f'' when DERIV) from the GEMM
-// results CA (TA x nw E), CB (p x nw E) and the sums S.
-template <bool DERIV>
-__global__ __launch_bounds__(64) void k_reml_eval(const double* CA, const double* CB, const double* S, int E, int TA,
+// f alone, one evaluation per 64-thread block: the LDLT of the bordered
+// matrix [A b; b^T c0] (pivots within A) gives log det A from the first p
+// pivots and q = c0 - b^T A^-1 b as the last, with no inverse.
+struct WaveLdltB {
+  double L[(kRP + 1) * kLD];
+  double D[kRP + 1];
+  double tmp[kRP + 1];
+  int tr[kRP];
+  int ok;
+};
+
+__global__ __launch_bounds__(64) void k_reml_eval_f(const double* CA, const double* CB, const double* S, int TA,
+                                                    int p, int df, double* Fo) {
+  __shared__ WaveLdltB w;
+  const int e = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double* s = S + static_cast<size_t>(e) * kNumScal;
+  load_sym(CA + static_cast<size_t>(e) * TA, p, w.L);
+  if (lane < p) w.L[lane * kLD + p] = CB[static_cast<size_t>(e) * p + lane];
+  if (lane == 0) w.L[p * kLD + p] = s[0];
+  __syncthreads();
+  wave_ldlt_factor<true>(w.L, w.D, w.tmp, w.tr, &w.ok, p);
+  if (lane == 0) {
+    bool good = w.ok != 0;
+    double ld = 0;
+    for (int a = 0; a < p && good; ++a) {
+      if (!(w.D[a] > 0)) good = false;
+      else ld += log(w.D[a]);
+    }
+    double q = w.D[p];
+    if (q <= 0) q = 1e-12;
+    Fo[e] = good ? 0.5 * (df * log(q / df) + s[3] + ld) : 1e300;
+  }
+}
+
+// f, f' and f'' (in ln delta), one evaluation per 64-thread block, from the
+// GEMM results CA (TA x 3E), CB (p x 3E) and the sums S.
+__global__ __launch_bounds__(64) void k_reml_eval_fgh(const double* CA, const double* CB, const double* S, int E, int TA,
                                                   int p, int df, double* Fo, double* Go, double* Ho) {
   __shared__ WaveLdlt w;
-  __shared__ double A1[DERIV ? kRP * kLD : 1], A2[DERIV ? kRP * kLD : 1];
+  __shared__ double A1[kRP * kLD], A2[kRP * kLD];
   __shared__ double vb[3][kRP], beta[kRP], u[kRP], wv[kRP];
   const int e = blockIdx.x;
   const int lane = threadIdx.x;
   load_sym(CA + static_cast<size_t>(e) * TA, p, w.L);
-  if (DERIV) {
-    load_sym(CA + (static_cast<size_t>(E) + e) * TA, p, A1);
-    load_sym(CA + (2 * static_cast<size_t>(E) + e) * TA, p, A2);
-  }
+  load_sym(CA + (static_cast<size_t>(E) + e) * TA, p, A1);
+  load_sym(CA + (2 * static_cast<size_t>(E) + e) * TA, p, A2);
   if (lane < p) {
     vb[0][lane] = CB[static_cast<size_t>(e) * p + lane];
-    if (DERIV) {
-      vb[1][lane] = CB[(static_cast<size_t>(E) + e) * p + lane];
-      vb[2][lane] = CB[(2 * static_cast<size_t>(E) + e) * p + lane];
-    }
+    vb[1][lane] = CB[(static_cast<size_t>(E) + e) * p + lane];
+    vb[2][lane] = CB[(2 * static_cast<size_t>(E) + e) * p + lane];
   }
   __syncthreads();
   wave_ldlt_inverse(w, p);
@@ -357,7 +400,8 @@ __global__ __launch_bounds__(64) void k_reml_eval(const double* CA, const double
   if (!good) {
     if (lane == 0) {
       Fo[e] = 1e300;
-      if (DERIV) { Go[e] = 0; Ho[e] = 0; }
+      Go[e] = 0;
+      Ho[e] = 0;
     }
     return;
   }
@@ -384,10 +428,6 @@ __global__ __launch_bounds__(64) void k_reml_eval(const double* CA, const double
   q += bAb;
   if (q <= 0) q = 1e-12;
   const double F = 0.5 * (df * log(q / df) + s[3] + ld);
-  if (!DERIV) {
-    if (lane == 0) Fo[e] = F;
-    return;
-  }
   // u = b1 - A1 beta, wv = A^-1 u
   if (lane < p) {
     double t = vb[1][lane];

@@ -155,7 +155,10 @@ __device__ inline bool chol_lower(RemlSmem& sm, int p) {
 // LMM gene basis at the chosen delta. Column blocks (nq = p + 2):
 //   b = 0: d_i ; b = 1..p: s_i W_ic ; b = p+1: s_i r_i   (s = sqrt(d))
 // where W = (s X~) Lc^-T is orthonormal (Lc Lc^T = X~^T D X~) and
-// r = s y~ - W W^T (s y~) is the whitened null residual; rss_null = ||r||^2.
+// r = s y~ - W u, u = W^T (s y~) = Lc^-1 X~^T D y~, is the whitened null
+// residual; rss_null = ||r||^2. One pass over the samples: row i's W entries
+// come from a forward substitution against Lc, and u from the same
+// substitution applied once to X~^T D y~ (no per-column reductions).
 __global__ __launch_bounds__(256) void k_lmm_basis(const double* Xt, const double* Yt, const double* lam, int n,
                                                    int n_pad, int p, int G, const double* delta, GeneCols gc,
                                                    double* rss_null, int* valid, double* u64, double* Lc_out) {
@@ -165,7 +168,16 @@ __global__ __launch_bounds__(256) void k_lmm_basis(const double* Xt, const doubl
   const double* yt = Yt + static_cast<size_t>(k) * n;
   const double dk = delta[k];
   reml_pass1<false>(Xt, yt, lam, n, p, dk, sm);
-  if (threadIdx.x == 0) sm.flag = chol_lower(sm, p) ? 1 : 0;
+  if (threadIdx.x == 0) {
+    const bool good = chol_lower(sm, p);
+    sm.flag = good ? 1 : 0;
+    // u = Lc^-1 b
+    for (int c = 0; c < p && good; ++c) {
+      double t = sm.b[c];
+      for (int q = 0; q < c; ++q) t -= sm.A1[q * kPMax + c] * sm.u[q];
+      sm.u[c] = t / sm.A1[c * kPMax + c];
+    }
+  }
   __syncthreads();
   for (int idx = threadIdx.x; idx < p * p; idx += 256) {
     const int i = idx % p, jj = idx / p;
@@ -173,50 +185,21 @@ __global__ __launch_bounds__(256) void k_lmm_basis(const double* Xt, const doubl
   }
   const int nq = p + 2;
   const bool ok = sm.flag != 0;
-  // pass A: W rows, u = W^T (s y~)
-  double ucol[kPMax];
-#pragma unroll
-  for (int c = 0; c < kPMax; ++c) ucol[c] = 0;
+  double rss = 0;
   for (int i = threadIdx.x; i < n_pad; i += 256) {
     if (i >= n || !ok) {
       for (int b = 0; b < nq; ++b) put_val(gc, k, b, i, 0.0);
       continue;
     }
+    // keep the Lc reads inside the loop (hoisting the p(p+1)/2 loop-invariant
+    // LDS values into registers spills)
+    asm volatile("" ::: "memory");
     double v = dk * lam[i] + 1.0;
     if (v < 1e-12) v = 1e-12;
     const double d = 1.0 / v, s = sqrt(d);
-    const double yw = s * yt[i];
+    double r = s * yt[i];
     double w[kPMax];
-#pragma unroll
-    for (int c = 0; c < kPMax; ++c) {
-      if (c < p) {
-        double t = s * Xt[static_cast<size_t>(c) * n + i];
-        for (int q = 0; q < c; ++q) t -= w[q] * sm.A1[q * kPMax + c];
-        w[c] = t / sm.A1[c * kPMax + c];
-        ucol[c] += w[c] * yw;
-      }
-    }
     put_val(gc, k, 0, i, d);
-#pragma unroll
-    for (int c = 0; c < kPMax; ++c)
-      if (c < p) put_val(gc, k, 1 + c, i, s * w[c]);
-  }
-#pragma unroll
-  for (int c = 0; c < kPMax; ++c) {
-    if (c < p) {
-      const double t = block_sum(ucol[c], sm);
-      if (threadIdx.x == 0) sm.u[c] = t;
-    }
-  }
-  __syncthreads();
-  double rss = 0;
-  for (int i = threadIdx.x; i < n && ok; i += 256) {
-    double v = dk * lam[i] + 1.0;
-    if (v < 1e-12) v = 1e-12;
-    const double d = 1.0 / v, s = sqrt(d);
-    const double yw = s * yt[i];
-    double w[kPMax];
-    double r = yw;
 #pragma unroll
     for (int c = 0; c < kPMax; ++c) {
       if (c < p) {
@@ -224,6 +207,7 @@ __global__ __launch_bounds__(256) void k_lmm_basis(const double* Xt, const doubl
         for (int q = 0; q < c; ++q) t -= w[q] * sm.A1[q * kPMax + c];
         w[c] = t / sm.A1[c * kPMax + c];
         r -= w[c] * sm.u[c];
+        put_val(gc, k, 1 + c, i, s * w[c]);
       }
     }
     rss += r * r;

@@ -646,7 +646,7 @@ static int reml_eval(eqtl_gpu_ctx* c, const std::vector<int32_t>& ek, const std:
   auto& r = c->rw;
   const size_t cols = static_cast<size_t>(nw) * std::min(E, kEvalChunk);
   CHK(ensure(r.W, sizeof(double) * n * cols));
-  CHK(ensure(r.Z, sizeof(double) * n * cols));
+  if (deriv) CHK(ensure(r.Z, sizeof(double) * n * cols));
   CHK(ensure(r.CA, sizeof(double) * TA * cols));
   CHK(ensure(r.CB, sizeof(double) * p * cols));
   CHK(ensure(r.S, sizeof(double) * kNumScal * std::min(E, kEvalChunk)));
@@ -660,23 +660,36 @@ static int reml_eval(eqtl_gpu_ctx* c, const std::vector<int32_t>& ek, const std:
     double* Ho = Fo + 2 * Ec;
     HIPC(hipMemcpyAsync(r.ek.p, ek.data() + e0, sizeof(int32_t) * Ec, hipMemcpyHostToDevice, c->stream));
     HIPC(hipMemcpyAsync(r.ed.p, ed.data() + e0, sizeof(double) * Ec, hipMemcpyHostToDevice, c->stream));
-    if (deriv)
+    if (deriv) {
       hipLaunchKernelGGL(k_reml_eval_weights<true>, dim3(Ec), dim3(256), 0, c->stream, c->lam64.as<double>(),
                          c->Yt64.as<double>(), n, Ec, r.ek.as<int32_t>(), r.ed.as<double>(), r.W.as<double>(),
                          r.Z.as<double>(), r.S.as<double>());
-    else
-      hipLaunchKernelGGL(k_reml_eval_weights<false>, dim3(Ec), dim3(256), 0, c->stream, c->lam64.as<double>(),
-                         c->Yt64.as<double>(), n, Ec, r.ek.as<int32_t>(), r.ed.as<double>(), r.W.as<double>(),
-                         r.Z.as<double>(), r.S.as<double>());
-    CHK(check_launch("k_reml_eval_weights"));
-    CHK(dgemm_tn(c, TA, nw * Ec, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA));
-    CHK(dgemm_tn(c, p, nw * Ec, n, c->Xt64.as<double>(), n, r.Z.as<double>(), n, r.CB.as<double>(), p));
-    if (deriv)
+      CHK(check_launch("k_reml_eval_weights"));
+      CHK(dgemm_tn(c, TA, nw * Ec, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA));
+      CHK(dgemm_tn(c, p, nw * Ec, n, c->Xt64.as<double>(), n, r.Z.as<double>(), n, r.CB.as<double>(), p));
       hipLaunchKernelGGL(k_reml_eval_fgh, dim3(Ec), dim3(64), 0, c->stream, r.CA.as<double>(), r.CB.as<double>(),
                          r.S.as<double>(), Ec, TA, p, df, Fo, Go, Ho);
-    else
-      hipLaunchKernelGGL(k_reml_eval_f, dim3(Ec), dim3(64), 0, c->stream, r.CA.as<double>(), r.CB.as<double>(),
-                         r.S.as<double>(), TA, p, df, Fo);
+    } else {
+      // b, c0 and sum ln v come straight from the weights kernel; one GEMM for X~^T D X~
+      if (p <= 16)
+        hipLaunchKernelGGL((k_reml_eval_weights_f<16, 4>), dim3((Ec + 3) / 4), dim3(64), 0, c->stream,
+                           c->lam64.as<double>(), c->Yt64.as<double>(), c->Xt64.as<double>(), n, p, Ec,
+                           r.ek.as<int32_t>(), r.ed.as<double>(), r.W.as<double>(), r.CB.as<double>(), r.S.as<double>());
+      else
+        hipLaunchKernelGGL((k_reml_eval_weights_f<kRP, 2>), dim3((Ec + 1) / 2), dim3(64), 0, c->stream,
+                           c->lam64.as<double>(), c->Yt64.as<double>(), c->Xt64.as<double>(), n, p, Ec,
+                           r.ek.as<int32_t>(), r.ed.as<double>(), r.W.as<double>(), r.CB.as<double>(), r.S.as<double>());
+      CHK(check_launch("k_reml_eval_weights_f"));
+      CHK(dgemm_tn(c, TA, Ec, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA));
+      if (p < 32) {
+        const size_t lds = 2 * reml_eval_f_bytes(p);
+        hipLaunchKernelGGL(k_reml_eval_f<32>, dim3((Ec + 1) / 2), dim3(64), lds, c->stream, r.CA.as<double>(),
+                           r.CB.as<double>(), r.S.as<double>(), Ec, TA, p, df, Fo);
+      } else {
+        hipLaunchKernelGGL(k_reml_eval_f<64>, dim3(Ec), dim3(64), reml_eval_f_bytes(p), c->stream,
+                           r.CA.as<double>(), r.CB.as<double>(), r.S.as<double>(), Ec, TA, p, df, Fo);
+      }
+    }
     CHK(check_launch("k_reml_eval"));
     HIPC(hipMemcpyAsync(F + e0, Fo, sizeof(double) * Ec, hipMemcpyDeviceToHost, c->stream));
     if (deriv) {

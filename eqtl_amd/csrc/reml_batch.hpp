@@ -141,6 +141,64 @@ __global__ __launch_bounds__(256) void k_reml_eval_weights(const double* lam, co
   }
 }
 
+// Inputs of EV f-only evaluations per one-wave block (p <= PB): the weights
+// d_i for the X~^T D X~ GEMM (W), and directly b = X~^T (d o y~) (CB, p values),
+// c0 = y~^T D y~ and sum ln v_i (S[0], S[3]). X~ (n x p, column-major) is
+// shared by all evaluations; each row of it is loaded once per block and used
+// for the block's EV evaluations. One wave per block keeps the closing
+// reductions to a single wave (EV (p + 2) butterfly sums).
+template <int PB, int EV>
+__global__ __launch_bounds__(64) void k_reml_eval_weights_f(const double* lam, const double* Yt, const double* Xt,
+                                                             int n, int p, int E, const int32_t* ek,
+                                                             const double* ed, double* W, double* CB, double* S) {
+  constexpr int NV = PB + 2;  // per evaluation: c0, sum ln v, b[0..p)
+  const int e0 = blockIdx.x * EV;
+  double dl[EV];
+  const double* y[EV];
+#pragma unroll
+  for (int v = 0; v < EV; ++v) {
+    const int e = min(e0 + v, E - 1);  // a spare slot repeats the last evaluation and writes nothing
+    dl[v] = ed[e];
+    y[v] = Yt + static_cast<size_t>(ek[e]) * n;
+  }
+  double acc[EV][NV];
+#pragma unroll
+  for (int v = 0; v < EV; ++v)
+#pragma unroll
+    for (int q = 0; q < NV; ++q) acc[v][q] = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double x[PB];
+#pragma unroll
+    for (int a = 0; a < PB; ++a) x[a] = a < p ? Xt[static_cast<size_t>(a) * n + i] : 0.0;
+    const double li = lam[i];
+#pragma unroll
+    for (int v = 0; v < EV; ++v) {
+      double vv = dl[v] * li + 1.0;
+      if (vv < 1e-12) vv = 1e-12;
+      const double d = 1.0 / vv, yi = y[v][i], dy = d * yi;
+      if (e0 + v < E) W[static_cast<size_t>(e0 + v) * n + i] = d;
+      acc[v][0] += yi * dy;
+      acc[v][1] += log(vv);
+#pragma unroll
+      for (int a = 0; a < PB; ++a) acc[v][2 + a] += x[a] * dy;
+    }
+  }
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int v = 0; v < EV; ++v) {
+    const int e = e0 + v;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+      const double t = wave_sum_d(acc[v][q]);
+      if (lane == q && e < E && q < p + 2) {
+        if (q == 0) S[static_cast<size_t>(e) * kNumScal + 0] = t;
+        else if (q == 1) S[static_cast<size_t>(e) * kNumScal + 3] = t;
+        else CB[static_cast<size_t>(e) * p + (q - 2)] = t;
+      }
+    }
+  }
+}
+
 // ---- one wavefront (= one 64-thread block) per p x p problem ----------------
 
 struct WaveLdlt {
@@ -165,11 +223,13 @@ __device__ inline void load_sym(const double* tri, int p, double* M) {
 // the single-threaded form. m(i, j) = L[j * kLD + i], lower triangle used.
 // BORDER: row p holds b^T and m(p, p) = c; the border never pivots, and its
 // last step leaves the Schur complement c - b^T A^-1 b in D[p].
-template <bool BORDER>
-__device__ inline void wave_ldlt_factor(double* L, double* D, double* tmp, int* tr, int* ok, int p) {
-  const int lane = threadIdx.x;
+// LANES threads (64, or 32 for two problems per wave) share one problem;
+// ld: leading dimension of L.
+template <bool BORDER, int LANES>
+__device__ inline void wave_ldlt_factor(double* L, int ld, double* D, double* tmp, int* tr, int* ok, int p) {
+  const int lane = threadIdx.x % LANES;
   const int pe = BORDER ? p + 1 : p;
-  auto m = [&](int i, int j) -> double& { return L[j * kLD + i]; };
+  auto m = [&](int i, int j) -> double& { return L[j * ld + i]; };
   if (lane == 0) *ok = 1;
   __syncthreads();
   for (int k = 0; k < pe; ++k) {
@@ -179,7 +239,7 @@ __device__ inline void wave_ldlt_factor(double* L, double* D, double* tmp, int* 
       int big = p;
       if (lane >= k && lane < p) { bv = fabs(m(lane, lane)); big = lane; }
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
+      for (int o = LANES / 2; o > 0; o >>= 1) {
         const double ov = __shfl_xor(bv, o);
         const int oi = __shfl_xor(big, o);
         if (ov > bv || (ov == bv && oi < big)) { bv = ov; big = oi; }
@@ -222,7 +282,7 @@ __device__ inline void wave_ldlt_factor(double* L, double* D, double* tmp, int* 
 __device__ inline void wave_ldlt_inverse(WaveLdlt& w, int p) {
   const int lane = threadIdx.x;
   auto m = [&](int i, int j) -> double& { return w.L[j * kLD + i]; };
-  wave_ldlt_factor<false>(w.L, w.D, w.tmp, w.tr, &w.ok, p);
+  wave_ldlt_factor<false, 64>(w.L, kLD, w.D, w.tmp, w.tr, &w.ok, p);
   // inverse: lane c solves A x = e_c in column c of Ai
   if (lane < p) {
     double* y = w.Ai + lane * kLD;
@@ -340,38 +400,48 @@ __global__ void k_reml_grid_argmin(const double* F, const double* dgrid, int ng,
   best_f[k] = bf;
 }
 
-// f alone, one evaluation per 64-thread block: the LDLT of the bordered
-// matrix [A b; b^T c0] (pivots within A) gives log det A from the first p
-// pivots and q = c0 - b^T A^-1 b as the last, with no inverse.
-struct WaveLdltB {
-  double L[(kRP + 1) * kLD];
-  double D[kRP + 1];
-  double tmp[kRP + 1];
-  int tr[kRP];
-  int ok;
-};
+// f alone: the LDLT of the bordered matrix [A b; b^T c0] (pivots within A)
+// gives log det A from the first p pivots and q = c0 - b^T A^-1 b as the last,
+// with no inverse. LANES = 32 runs two evaluations per wave (p <= 31), 64 one.
+// Dynamic LDS per evaluation: reml_eval_f_bytes(p), sized to p so that many
+// evaluations are in flight per CU (the factorisation is latency-bound).
+__host__ __device__ constexpr int reml_eval_f_doubles(int p) { return (p + 1) * (p + 1) + 2 * (p + 1) + (p + 1); }
+__host__ __device__ constexpr size_t reml_eval_f_bytes(int p) { return sizeof(double) * reml_eval_f_doubles(p) + 8; }
 
-__global__ __launch_bounds__(64) void k_reml_eval_f(const double* CA, const double* CB, const double* S, int TA,
+template <int LANES>
+__global__ __launch_bounds__(64) void k_reml_eval_f(const double* CA, const double* CB, const double* S, int E, int TA,
                                                     int p, int df, double* Fo) {
-  __shared__ WaveLdltB w;
-  const int e = blockIdx.x;
-  const int lane = threadIdx.x;
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_f[];
+  constexpr int kPer = 64 / LANES;
+  const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+  const int e_raw = blockIdx.x * kPer + sub;
+  const int e = e_raw < E ? e_raw : E - 1;  // a spare half repeats the last evaluation and writes nothing
+  const int ld = p + 1;
+  double* L = reinterpret_cast<double*>(smem_f + sub * reml_eval_f_bytes(p));
+  double* D = L + ld * (p + 1);
+  double* tmp = D + (p + 1);
+  int* tr = reinterpret_cast<int*>(tmp + (p + 1));
+  int* ok = tr + (p + 1);
   const double* s = S + static_cast<size_t>(e) * kNumScal;
-  load_sym(CA + static_cast<size_t>(e) * TA, p, w.L);
-  if (lane < p) w.L[lane * kLD + p] = CB[static_cast<size_t>(e) * p + lane];
-  if (lane == 0) w.L[p * kLD + p] = s[0];
+  const double* tri = CA + static_cast<size_t>(e) * TA;
+  for (int idx = lane; idx < p * p; idx += LANES) {
+    const int a = idx / p, c = idx % p;
+    L[c * ld + a] = tri[tri_index(a, c)];
+  }
+  if (lane < p) L[lane * ld + p] = CB[static_cast<size_t>(e) * p + lane];
+  if (lane == 0) L[p * ld + p] = s[0];
   __syncthreads();
-  wave_ldlt_factor<true>(w.L, w.D, w.tmp, w.tr, &w.ok, p);
-  if (lane == 0) {
-    bool good = w.ok != 0;
-    double ld = 0;
+  wave_ldlt_factor<true, LANES>(L, ld, D, tmp, tr, ok, p);
+  if (lane == 0 && e_raw < E) {
+    bool good = *ok != 0;
+    double lg = 0;
     for (int a = 0; a < p && good; ++a) {
-      if (!(w.D[a] > 0)) good = false;
-      else ld += log(w.D[a]);
+      if (!(D[a] > 0)) good = false;
+      else lg += log(D[a]);
     }
-    double q = w.D[p];
+    double q = D[p];
     if (q <= 0) q = 1e-12;
-    Fo[e] = good ? 0.5 * (df * log(q / df) + s[3] + ld) : 1e300;
+    Fo[e] = good ? 0.5 * (df * log(q / df) + s[3] + lg) : 1e300;
   }
 }
 

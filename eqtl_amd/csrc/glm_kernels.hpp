@@ -22,7 +22,7 @@ namespace eqtl_amd {
 // A = X^T diag(w) X and b = X^T (w z) over all samples, with (w_i, z_i) from
 // row(i, w, z). Collective over the 256 threads; X staged in LDS by chunks.
 template <class Row>
-__device__ void glm_pass(const double* __restrict__ X, int n, int p, Row row, RemlSmem& sm) {
+__device__ __attribute__((always_inline)) inline void glm_pass(const double* __restrict__ X, int n, int p, Row row, RemlSmem& sm) {
   const int TA = p * (p + 1) / 2;
   const int T = TA + p;
   double acc[kMaxTasks];

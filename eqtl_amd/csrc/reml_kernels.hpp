@@ -48,7 +48,7 @@ __device__ inline double block_sum(double v, RemlSmem& sm) {
 // Pass over all samples accumulating A (and A1, A2 when DERIV), b and the
 // per-row scalars. Collective over the 256 threads.
 template <bool DERIV>
-__device__ void reml_pass1(const double* __restrict__ Xt, const double* __restrict__ yt,
+__device__ __attribute__((always_inline)) inline void reml_pass1(const double* __restrict__ Xt, const double* __restrict__ yt,
                            const double* __restrict__ lam, int n, int p, double delta, RemlSmem& sm) {
   const int TA = p * (p + 1) / 2;
   const int T = (DERIV ? 3 * TA : TA) + p;
@@ -152,6 +152,41 @@ __device__ inline bool chol_lower(RemlSmem& sm, int p) {
   return true;
 }
 
+// The same factorisation by the whole block (every thread calls it), column by
+// column: thread 0 forms the pivot, threads j < i < p the column below it, each
+// element with the serial form's operation order (so the factor is identical).
+// Thread p also carries the border row b^T through the same steps, leaving
+// u = Lc^-1 b in sm.u. Returns false when a pivot is not positive.
+__device__ inline bool chol_lower_block(RemlSmem& sm, int p) {
+  const int t = threadIdx.x;
+  if (t == 0) sm.flag = 1;
+  __syncthreads();
+  for (int j = 0; j < p; ++j) {
+    if (t == 0) {
+      double s = sm.A[j * kPMax + j];
+      for (int k = 0; k < j; ++k) s -= sm.A1[k * kPMax + j] * sm.A1[k * kPMax + j];
+      if (!(s > 0)) sm.flag = 0;
+      else sm.A1[j * kPMax + j] = sqrt(s);
+    }
+    __syncthreads();
+    if (!sm.flag) break;
+    const double ljj = sm.A1[j * kPMax + j];
+    if (t > j && t < p) {
+      double x = sm.A[j * kPMax + t];
+      for (int k = 0; k < j; ++k) x -= sm.A1[k * kPMax + t] * sm.A1[k * kPMax + j];
+      sm.A1[j * kPMax + t] = x / ljj;
+    } else if (t == p) {
+      double x = sm.b[j];
+      for (int k = 0; k < j; ++k) x -= sm.A1[k * kPMax + j] * sm.u[k];
+      sm.u[j] = x / ljj;
+    }
+    __syncthreads();
+  }
+  const bool ok = sm.flag != 0;
+  __syncthreads();
+  return ok;
+}
+
 // LMM gene basis at the chosen delta. Column blocks (nq = p + 2):
 //   b = 0: d_i ; b = 1..p: s_i W_ic ; b = p+1: s_i r_i   (s = sqrt(d))
 // where W = (s X~) Lc^-T is orthonormal (Lc Lc^T = X~^T D X~) and
@@ -159,7 +194,7 @@ __device__ inline bool chol_lower(RemlSmem& sm, int p) {
 // residual; rss_null = ||r||^2. One pass over the samples: row i's W entries
 // come from a forward substitution against Lc, and u from the same
 // substitution applied once to X~^T D y~ (no per-column reductions).
-__global__ __launch_bounds__(256) void k_lmm_basis(const double* Xt, const double* Yt, const double* lam, int n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_lmm_basis(const double* Xt, const double* Yt, const double* lam, int n,
                                                    int n_pad, int p, int G, const double* delta, GeneCols gc,
                                                    double* rss_null, int* valid, double* u64, double* Lc_out) {
   __shared__ RemlSmem sm;
@@ -168,23 +203,12 @@ __global__ __launch_bounds__(256) void k_lmm_basis(const double* Xt, const doubl
   const double* yt = Yt + static_cast<size_t>(k) * n;
   const double dk = delta[k];
   reml_pass1<false>(Xt, yt, lam, n, p, dk, sm);
-  if (threadIdx.x == 0) {
-    const bool good = chol_lower(sm, p);
-    sm.flag = good ? 1 : 0;
-    // u = Lc^-1 b
-    for (int c = 0; c < p && good; ++c) {
-      double t = sm.b[c];
-      for (int q = 0; q < c; ++q) t -= sm.A1[q * kPMax + c] * sm.u[q];
-      sm.u[c] = t / sm.A1[c * kPMax + c];
-    }
-  }
-  __syncthreads();
+  const bool ok = chol_lower_block(sm, p);  // Lc, and u = Lc^-1 b
   for (int idx = threadIdx.x; idx < p * p; idx += 256) {
     const int i = idx % p, jj = idx / p;
-    Lc_out[static_cast<size_t>(k) * p * p + idx] = (i >= jj && sm.flag) ? sm.A1[jj * kPMax + i] : 0.0;
+    Lc_out[static_cast<size_t>(k) * p * p + idx] = (i >= jj && ok) ? sm.A1[jj * kPMax + i] : 0.0;
   }
   const int nq = p + 2;
-  const bool ok = sm.flag != 0;
   double rss = 0;
   for (int i = threadIdx.x; i < n_pad; i += 256) {
     if (i >= n || !ok) {

@@ -643,18 +643,20 @@ static int reml_eval(eqtl_gpu_ctx* c, const std::vector<int32_t>& ek, const std:
   const int n = c->n, p = c->p, TA = p * (p + 1) / 2, df = n - p;
   const int E = static_cast<int>(ek.size());
   const int nw = deriv ? 3 : 1;
+  // f-only rounds (the reference search: two points per gene) go in one batch
+  const int chunk = deriv ? kEvalChunk : 2 * kEvalChunk;
   auto& r = c->rw;
-  const size_t cols = static_cast<size_t>(nw) * std::min(E, kEvalChunk);
+  const size_t cols = static_cast<size_t>(nw) * std::min(E, chunk);
   CHK(ensure(r.W, sizeof(double) * n * cols));
   if (deriv) CHK(ensure(r.Z, sizeof(double) * n * cols));
   CHK(ensure(r.CA, sizeof(double) * TA * cols));
   CHK(ensure(r.CB, sizeof(double) * p * cols));
-  CHK(ensure(r.S, sizeof(double) * kNumScal * std::min(E, kEvalChunk)));
-  CHK(ensure(r.ek, sizeof(int32_t) * std::min(E, kEvalChunk)));
-  CHK(ensure(r.ed, sizeof(double) * std::min(E, kEvalChunk)));
-  CHK(ensure(r.Fo, sizeof(double) * 3 * std::min(E, kEvalChunk)));
-  for (int e0 = 0; e0 < E; e0 += kEvalChunk) {
-    const int Ec = std::min(kEvalChunk, E - e0);
+  CHK(ensure(r.S, sizeof(double) * kNumScal * std::min(E, chunk)));
+  CHK(ensure(r.ek, sizeof(int32_t) * std::min(E, chunk)));
+  CHK(ensure(r.ed, sizeof(double) * std::min(E, chunk)));
+  CHK(ensure(r.Fo, sizeof(double) * 3 * std::min(E, chunk)));
+  for (int e0 = 0; e0 < E; e0 += chunk) {
+    const int Ec = std::min(chunk, E - e0);
     double* Fo = r.Fo.as<double>();
     double* Go = Fo + Ec;
     double* Ho = Fo + 2 * Ec;

@@ -79,21 +79,6 @@ __device__ __attribute__((always_inline)) inline void glm_pass(const double* __r
   __syncthreads();
 }
 
-// Solves Lc Lc^T x = b with Lc in sm.A1 (lower, from chol_lower). Thread 0.
-__device__ inline void chol_solve(const RemlSmem& sm, int p, const double* b, double* x) {
-  double z[kPMax];
-  for (int i = 0; i < p; ++i) {
-    double t = b[i];
-    for (int k = 0; k < i; ++k) t -= sm.A1[k * kPMax + i] * z[k];
-    z[i] = t / sm.A1[i * kPMax + i];
-  }
-  for (int i = p - 1; i >= 0; --i) {
-    double t = z[i];
-    for (int k = i + 1; k < p; ++k) t -= sm.A1[i * kPMax + k] * x[k];
-    x[i] = t / sm.A1[i * kPMax + i];
-  }
-}
-
 // mu: G x n scratch (the gene's current mean). Outputs per gene: phi, valid
 // (fit converged and X^T W X positive definite), gene_scal = sum r, u64 = r,
 // v64 = w (FP64, n each), Lc (p x p, lower, col-major) and the column blocks.
@@ -130,11 +115,17 @@ __global__ __launch_bounds__(256) void k_glm_null(const double* X, const double*
                  z = log(m) + (y[i] - m) / m;
                },
                sm);
+      // Lc and u = Lc^-1 b by the block; thread 0 back-substitutes Lc^T x = u
+      const bool good = chol_lower_block(sm, p);
       if (threadIdx.x == 0) {
-        if (!chol_lower(sm, p)) {
+        if (!good) {
           s_fail = 1;
         } else {
-          chol_solve(sm, p, sm.b, bnew);
+          for (int i = p - 1; i >= 0; --i) {
+            double t = sm.u[i];
+            for (int k2 = i + 1; k2 < p; ++k2) t -= sm.A1[i * kPMax + k2] * bnew[k2];
+            bnew[i] = t / sm.A1[i * kPMax + i];
+          }
           double diff = 0;
           for (int a = 0; a < p; ++a) diff = fmax(diff, fabs(bnew[a] - beta[a]));
           for (int a = 0; a < p; ++a) beta[a] = bnew[a];
@@ -178,9 +169,7 @@ __global__ __launch_bounds__(256) void k_glm_null(const double* X, const double*
                z = 0.0;
              },
              sm);
-    if (threadIdx.x == 0) sm.flag = chol_lower(sm, p) ? 1 : 0;
-    __syncthreads();
-    ok = sm.flag != 0;
+    ok = chol_lower_block(sm, p);
   }
   for (int idx = threadIdx.x; idx < p * p; idx += 256) {
     const int i = idx % p, jj = idx / p;

@@ -335,9 +335,9 @@ __device__ inline W8Gene w8_gene(const WaldArgs& a, int group) {
 // LDS: test t = row * 16 + gene has gg = fold[t] + fold[3T + t], S = gg -
 // (fold[T + t] + fold[4T + t]), num = fold[2T + t] + fold[5T + t] (T = tests
 // per tile). Pass 1 applies the window / keep rules and writes FP64-refined
-// numerators back into the fold; pass 2 evaluates the thread's four tests as
-// independent chains (the tail and ACAT math is latency-bound); pass 3 does
-// the bookkeeping in row order. The per-thread gene states are merged over the
+// numerators back into the fold; for the LMM, pass 2 evaluates the thread's
+// four tests as independent chains (the t tail and ACAT math is
+// latency-bound) and pass 3 does the bookkeeping in row order. The per-thread gene states are merged over the
 // four lanes holding the same gene, then over the eight waves (wpart) into the
 // running state in gstate.
 template <int MODEL>
@@ -368,21 +368,34 @@ __device__ inline void w8_epilogue(const WaldArgs& a, const W8Gene& gi, const Ro
     }
     if (ok) okm |= 1u << i;
   }
-  WaldOut w[kT];
-#pragma unroll
-  for (int i = 0; i < kT; ++i) {
-    const int jl = (threadIdx.x >> 4) + (kW8Threads / 16) * i;
-    double gg, S, num;
-    fold_vals(jl * 16 + gi.gl, gg, S, num);
-    w[i] = wald_stats<MODEL>(a, ri.gtg[jl], gi.gscal, gg, S, num, tv);
-  }
   GeneAcc st;
   acc_init(st);
   dd asum{0.0, 0.0};
+  if constexpr (MODEL == EQTL_MODEL_LMM) {
+    WaldOut w[kT];
 #pragma unroll
-  for (int i = 0; i < kT; ++i) {
-    const int jl = (threadIdx.x >> 4) + (kW8Threads / 16) * i;
-    if (okm >> i & 1u) wald_book<MODEL>(a, ri, jl, row0 + jl, gi.gene, w[i], st, asum);
+    for (int i = 0; i < kT; ++i) {
+      const int jl = (threadIdx.x >> 4) + (kW8Threads / 16) * i;
+      double gg, S, num;
+      fold_vals(jl * 16 + gi.gl, gg, S, num);
+      w[i] = wald_stats<MODEL>(a, ri.gtg[jl], gi.gscal, gg, S, num, tv);
+    }
+#pragma unroll
+    for (int i = 0; i < kT; ++i) {
+      const int jl = (threadIdx.x >> 4) + (kW8Threads / 16) * i;
+      if (okm >> i & 1u) wald_book<MODEL>(a, ri, jl, row0 + jl, gi.gene, w[i], st, asum);
+    }
+  } else {
+    // the score test's tail (erfc) is short: one test at a time (measured faster)
+#pragma unroll 1
+    for (int i = 0; i < kT; ++i) {
+      if (!(okm >> i & 1u)) continue;
+      const int jl = (threadIdx.x >> 4) + (kW8Threads / 16) * i;
+      double gg, S, num;
+      fold_vals(jl * 16 + gi.gl, gg, S, num);
+      wald_book<MODEL>(a, ri, jl, row0 + jl, gi.gene, wald_stats<MODEL>(a, ri.gtg[jl], gi.gscal, gg, S, num, tv),
+                       st, asum);
+    }
   }
   st.acat_hi = asum.hi;
   st.acat_lo = asum.lo;

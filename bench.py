@@ -797,7 +797,7 @@ def bench_grm(args):
                                f"intercept-only genes", "samples": n, "snps_per_step": m, "snps_per_block": BLOCK},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "note": "k_grm_gram: algorithmic n(n+1) FLOP per SNP (unique lower triangle incl. the "
+                     "note": "k_grm_syrk: algorithmic n(n+1) FLOP per SNP (unique lower triangle incl. the "
                              "diagonal) over its event-timed launches, FP64 matrix peak"},
         "cpu_baseline": cpu,
         **extra,

@@ -1684,10 +1684,29 @@ int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev
   HIPC(hipStreamSynchronize(c->stream));
   {
     Timed t(c, "grm");
-    dim3 grid(static_cast<unsigned>(np / 64), static_cast<unsigned>(np / 64));
-    hipLaunchKernelGGL(k_grm_gram, grid, dim3(256), 0, c->stream, z64, mis, static_cast<int>(m_pad), np,
-                       c->grm_acc.as<double>(), c->grm_miss.as<int32_t>(), uu[1] ? 1 : 0);
-    CHK(check_launch("k_grm_gram"));
+    static bool attr_set = false;
+    if (!attr_set) {
+      HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_grm_syrk), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               static_cast<int>(kGramLdsBytes)));
+      attr_set = true;
+    }
+    if (np >= kGramLargeN) {
+      const unsigned nt = static_cast<unsigned>((np + kGramT - 1) / kGramT);
+      hipLaunchKernelGGL(k_grm_syrk, dim3(nt, nt), dim3(256), kGramLdsBytes, c->stream, z64, static_cast<int>(m_pad),
+                         np, c->grm_acc.as<double>());
+      CHK(check_launch("k_grm_syrk"));
+      if (uu[1]) {
+        dim3 grid(static_cast<unsigned>(np / 64), static_cast<unsigned>(np / 64));
+        hipLaunchKernelGGL(k_grm_joint, grid, dim3(256), 0, c->stream, mis, static_cast<int>(m_pad), np,
+                           c->grm_miss.as<int32_t>());
+        CHK(check_launch("k_grm_joint"));
+      }
+    } else {
+      dim3 grid(static_cast<unsigned>(np / 64), static_cast<unsigned>(np / 64));
+      hipLaunchKernelGGL(k_grm_gram, grid, dim3(256), 0, c->stream, z64, mis, static_cast<int>(m_pad), np,
+                         c->grm_acc.as<double>(), c->grm_miss.as<int32_t>(), uu[1] ? 1 : 0);
+      CHK(check_launch("k_grm_gram"));
+    }
     CHK(t.done());
   }
   HIPC(hipStreamSynchronize(c->stream));

@@ -240,6 +240,25 @@ def test_grm_matches_closed_form(gpu_ctx):
         assert err < 1e-12
 
 
+def test_grm_large_n_tiles(gpu_ctx):
+    """n past the switch to 128 x 128 LDS-staged tiles (2 x 128 + a ragged
+    third tile row), with and without missing calls, blocks of uneven length."""
+    rng = np.random.default_rng(19)
+    for n, m, miss in ((1601, 300, 0.02), (1700, 260, 0.0)):
+        geno, _ = synth.draw_genotypes(rng, m, n, 0.01, 0.5, miss)
+        rows = synth.pack_bed(geno)
+        gpu_ctx.set_samples(n)
+        gpu_ctx.set_policy(True, 0.8, 0.0)
+        K, N, used = gpu_ctx.grm(rows, block=200)
+        dos, missm, keep, af = O.decode_bed(rows, n)
+        Kr, Nr, ur = O.grm(dos, missm, keep)
+        assert used == ur
+        assert np.array_equal(N, Nr)
+        err = np.abs(K - Kr).max()
+        print(f"GRM n={n} m={m} miss={miss}: max abs err {err:.3e}")
+        assert err < 1e-12
+
+
 def test_eigh_gpu(gpu_ctx):
     ds, rows, (Q, lam), win, ref = lmm_case()
     dos, missm, keep, af = O.decode_bed(rows, ds.n)

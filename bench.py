@@ -13,6 +13,14 @@ value = Wald tests per second of that step (split-BF16 form). The plain FP32
 form is timed on a SNP slice of the same data (tests/s is per-test) and the
 split form's p-values are compared with the FP64 oracle on a sample of
 (SNP, gene) pairs that includes the planted strong effects.
+
+Other paths of the engine (--workload), each printing one JSON line of the
+same shape with its own metric, roofline of its dominant kernel and CPU
+baseline: trans-lm (LM scan, same data), trans-glm (NB score tests on counts
+drawn from the same phenotypes), cis-perm (cis LM + LMM with 1000 gene-level
+permutations, n = 1000, a gene slice of a 20k-gene table), grm (the GRM at
+n = 5000 x 1M SNPs, then eigh and the REML adjoint), vcf-gt / vcf-ds (device
+VCF decode).
 """
 import argparse
 import json
@@ -215,16 +223,17 @@ def bench_glm(args):
         log("accuracy", extra["accuracy_vs_fp64"])
     cpu = None
     if not args.no_cpu:
-        rows_s = ctx.copy_rows(rows_dev, 0, 200, bps)
+        ns, ng = 2000, 4000
+        rows_s = ctx.copy_rows(rows_dev, 0, ns, bps)
         dos, _, keep, _ = O.decode_bed(rows_s, N)
         threads = O.num_threads()
-        Ys = Yc[:, :1000]
+        Ys = Yc[:, :ng]
         t1 = time.perf_counter()
-        ref = O.glm_scan(dos, keep, np.zeros(200), np.arange(200), Ys, X, pthr=1e-5, hit_cap=1 << 16)
+        ref = O.glm_scan(dos, keep, np.zeros(ns), np.arange(ns), Ys, X, pthr=1e-5, hit_cap=1 << 16)
         dtc = time.perf_counter() - t1
         nt = int(ref["state"]["n_tested"].sum())
         cpu = {"value": nt / dtc, "unit": "NB score tests/s", "cores": threads, "kind": "port",
-               "sample": f"200 SNPs x 1000 genes of the same data through the oracle's restatement of "
+               "sample": f"{ns} SNPs x {ng} genes of the same data through the oracle's restatement of "
                          f"prep_glm_nb + test_glm_nb (null fits included) on {threads} threads "
                          f"({nt} tests in {dtc:.1f} s)"}
         log("cpu baseline", cpu)

@@ -663,10 +663,10 @@ static int reml_eval(eqtl_gpu_ctx* c, const std::vector<int32_t>& ek, const std:
     HIPC(hipMemcpyAsync(r.ek.p, ek.data() + e0, sizeof(int32_t) * Ec, hipMemcpyHostToDevice, c->stream));
     HIPC(hipMemcpyAsync(r.ed.p, ed.data() + e0, sizeof(double) * Ec, hipMemcpyHostToDevice, c->stream));
     if (deriv) {
-      hipLaunchKernelGGL(k_reml_eval_weights<true>, dim3(Ec), dim3(256), 0, c->stream, c->lam64.as<double>(),
+      hipLaunchKernelGGL(k_reml_eval_weights_fgh, dim3(Ec), dim3(256), 0, c->stream, c->lam64.as<double>(),
                          c->Yt64.as<double>(), n, Ec, r.ek.as<int32_t>(), r.ed.as<double>(), r.W.as<double>(),
                          r.Z.as<double>(), r.S.as<double>());
-      CHK(check_launch("k_reml_eval_weights"));
+      CHK(check_launch("k_reml_eval_weights_fgh"));
       CHK(dgemm_tn(c, TA, nw * Ec, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA));
       CHK(dgemm_tn(c, p, nw * Ec, n, c->Xt64.as<double>(), n, r.Z.as<double>(), n, r.CB.as<double>(), p));
       hipLaunchKernelGGL(k_reml_eval_fgh, dim3(Ec), dim3(64), 0, c->stream, r.CA.as<double>(), r.CB.as<double>(),

@@ -94,13 +94,13 @@ __global__ void k_square(const double* x, size_t N, double* y) {
   if (i < N) y[i] = x[i] * x[i];
 }
 
-// Per-evaluation weights for gene-specific deltas. W / Z: n x (nw E), column
-// w * E + e holds weight set w (d, d', d'') of evaluation e and the same times
-// y~ of its gene. S[e * kNumScal + .] = c0 c1 c2 logv se sde. One block per e.
-template <bool DERIV>
-__global__ __launch_bounds__(256) void k_reml_eval_weights(const double* lam, const double* Yt, int n, int E,
-                                                           const int32_t* ek, const double* ed, double* W,
-                                                           double* Z, double* S) {
+// Per-evaluation weights for gene-specific deltas, with the derivatives in
+// ln delta. W / Z: n x 3E, column w * E + e holds weight set w (d, d', d'') of
+// evaluation e and the same times y~ of its gene.
+// S[e * kNumScal + .] = c0 c1 c2 logv se sde. One block per e.
+__global__ __launch_bounds__(256) void k_reml_eval_weights_fgh(const double* lam, const double* Yt, int n, int E,
+                                                               const int32_t* ek, const double* ed, double* W,
+                                                               double* Z, double* S) {
   __shared__ double red[4];
   const int e = blockIdx.x;
   const double dl = ed[e];
@@ -114,27 +114,23 @@ __global__ __launch_bounds__(256) void k_reml_eval_weights(const double* lam, co
     Z[static_cast<size_t>(e) * n + i] = d * yi;
     c0 += yi * (d * yi);
     lv += log(v);
-    if (DERIV) {
-      const double ee = 1.0 - d;
-      const double w1 = -d * ee, w2 = d * ee * (ee - d);
-      W[(static_cast<size_t>(E) + e) * n + i] = w1;
-      Z[(static_cast<size_t>(E) + e) * n + i] = w1 * yi;
-      W[(2 * static_cast<size_t>(E) + e) * n + i] = w2;
-      Z[(2 * static_cast<size_t>(E) + e) * n + i] = w2 * yi;
-      c1 += w1 * yi * yi;
-      c2 += w2 * yi * yi;
-      se += ee;
-      sde += d * ee;
-    }
+    const double ee = 1.0 - d;
+    const double w1 = -d * ee, w2 = d * ee * (ee - d);
+    W[(static_cast<size_t>(E) + e) * n + i] = w1;
+    Z[(static_cast<size_t>(E) + e) * n + i] = w1 * yi;
+    W[(2 * static_cast<size_t>(E) + e) * n + i] = w2;
+    Z[(2 * static_cast<size_t>(E) + e) * n + i] = w2 * yi;
+    c1 += w1 * yi * yi;
+    c2 += w2 * yi * yi;
+    se += ee;
+    sde += d * ee;
   }
   c0 = block_sum256(c0, red);
   lv = block_sum256(lv, red);
-  if (DERIV) {
-    c1 = block_sum256(c1, red);
-    c2 = block_sum256(c2, red);
-    se = block_sum256(se, red);
-    sde = block_sum256(sde, red);
-  }
+  c1 = block_sum256(c1, red);
+  c2 = block_sum256(c2, red);
+  se = block_sum256(se, red);
+  sde = block_sum256(sde, red);
   if (threadIdx.x == 0) {
     double* s = S + static_cast<size_t>(e) * kNumScal;
     s[0] = c0; s[1] = c1; s[2] = c2; s[3] = lv; s[4] = se; s[5] = sde;

@@ -122,7 +122,8 @@ class Dataset:
 def make_trans(seed: int, n: int, m: int, G: int, n_chrom: int, ncov: int, maf=(0.05, 0.5),
                miss: float = 0.01, spacing: int = 1000, h2=None, n_causal: int = 200,
                n_planted: int = 10, beta_planted=(0.5, 0.5), body: int = 1000) -> Dataset:
-    """Trans-scan dataset (the C1 / C3 family).
+    """Trans-scan dataset: uniform SNP grid, genes with 1 kb bodies, optional
+    polygenic term and planted trans effects.
 
     h2: None → pure N(0,1) phenotypes; ("beta", a, b) or ("uniform", lo, hi)
     draws a per-gene heritability and adds a polygenic term over n_causal SNPs.
@@ -176,7 +177,8 @@ def make_trans(seed: int, n: int, m: int, G: int, n_chrom: int, ncov: int, maf=(
 def make_cis(seed: int, n: int, G: int, n_chrom: int, snps_per_chrom: int, ncov: int,
              win_lo: int = 50, win_hi: int = 5000, spacing: int = 100, maf=(0.01, 0.5),
              miss: float = 0.01, frac_planted: float = 0.05) -> Dataset:
-    """Cis dataset (the C4 family): gene bodies hold exactly c_g SNPs (run with window 0)."""
+    """Cis dataset with log-uniform window sizes c_g in [win_lo, win_hi]: gene bodies
+    hold exactly c_g SNPs (run with window 0)."""
     rng = np.random.default_rng(seed)
     m = n_chrom * snps_per_chrom
     geno, _ = draw_genotypes(rng, m, n, maf[0], maf[1], miss)

@@ -372,29 +372,21 @@ __device__ inline double wave_refine(const WaldArgs& a, bool cand, int64_t j, in
   return mine;
 }
 
-// One (SNP row j, gene) test in two parts. wald_stats: the FP64 Wald test
-// (test_lmm_gtil, scan_trans_lmm.cpp:11-40 / hit_from_gty, scan_trans_lm.cpp:21-36
-// / test_glm_nb) and the ACAT term of its p, with no side effects, so a thread
-// can evaluate several tests as independent instruction chains. wald_book: the
-// per-gene statistics (apply_snp_hit_stats, scan_trans.hpp:87-102), hit and
-// polish records. The caller has checked the gene, the row bound, the SNP's
-// keep flag and the window rule of the scope. gtg: the SNP's FP64 g^T g (LM) or
-// mean (GLM).
-struct WaldOut {
-  double beta, se, stat, p, r2, term;
-  bool accept;
-};
-
+// One (SNP row j, gene) test: FP64 Wald (test_lmm_gtil, scan_trans_lmm.cpp:11-40
+// / hit_from_gty, scan_trans_lm.cpp:21-36), then the per-gene statistics
+// (apply_snp_hit_stats, scan_trans.hpp:87-102). The caller has checked the
+// gene, the row bound, the SNP's keep flag and the window rule of the scope.
 template <int MODEL>
-__device__ inline WaldOut wald_stats(const WaldArgs& a, double gtg, double gscal, double gg, double S, double num,
-                                     const TailView& tv) {
+__device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, int64_t j, int gene, double gscal,
+                                 double gg, double S, double num, const TailView& tv, GeneAcc& st, dd& asum) {
+  const int64_t snp = a.snp0 + j;
   double beta = 0, se = 0, stat = 0, p = 1.0, r2 = 0;
   const double df = a.df;
   if constexpr (MODEL == EQTL_MODEL_GLM) {
     // test_glm_nb (model_glm.cpp:117-131): U = g^T r with the uncentred dosage
     // = (g - mean)^T r + mean * sum(r); info = S; two-sided normal p
     if (!(S < 1e-12)) {
-      const double U = num + gtg * gscal;
+      const double U = num + ri.gtg[jl] * gscal;
       const double rs = sqrt(S);
       beta = U / S;
       se = 1.0 / rs;
@@ -413,6 +405,7 @@ __device__ inline WaldOut wald_stats(const WaldArgs& a, double gtg, double gscal
       r2 = gscal > 1e-15 ? fmax(0.0, 1.0 - qq / gscal) : 0.0;
     }
   } else {
+    const double gtg = ri.gtg[jl];
     const double gty = num;
     if (!(gtg < 1e-12) && df > 0) {
       beta = gty / gtg;
@@ -426,23 +419,14 @@ __device__ inline WaldOut wald_stats(const WaldArgs& a, double gtg, double gscal
       beta = gty / gtg;  // df <= 0: p = 1 with beta set (model_lm.cpp:44-48)
     }
   }
-  WaldOut o{beta, se, stat, p, r2, 0.0, false};
-  o.accept = acat_accept(p, &o.term);
-  return o;
-}
-
-template <int MODEL>
-__device__ inline void wald_book(const WaldArgs& a, const RowInfo& ri, int jl, int64_t j, int gene,
-                                 const WaldOut& o, GeneAcc& st, dd& asum) {
-  const int64_t snp = a.snp0 + j;
-  const double p = o.p;
   if (!isfinite(p)) return;
   ++st.n_tested;
   // LMM / GLM tests with |t| > tau_hi are polished in FP64 after the kernel,
   // which adds their ACAT term (an accepted p stays accepted: both are < 1e-15)
-  const bool polish = MODEL != EQTL_MODEL_LM && fabs(o.stat) > a.tau_hi;
-  if (o.accept) {
-    if (!polish) asum = dd_add_d(asum, o.term);
+  const bool polish = MODEL != EQTL_MODEL_LM && fabs(stat) > a.tau_hi;
+  double term;
+  if (acat_accept(p, &term)) {
+    if (!polish) asum = dd_add_d(asum, term);
     ++st.used;
   }
   const double afj = ri.af[jl];
@@ -453,25 +437,19 @@ __device__ inline void wald_book(const WaldArgs& a, const RowInfo& ri, int jl, i
     if (static_cast<int64_t>(idx) < a.hit_cap) {
       eqtl_hit h;
       h.snp = snp; h.gene = gene; h.pad = 0;
-      h.af = afj; h.beta = o.beta; h.se = o.se; h.stat = o.stat; h.p = p; h.r2 = o.r2;
+      h.af = afj; h.beta = beta; h.se = se; h.stat = stat; h.p = p; h.r2 = r2;
       a.hits[idx] = h;
       hidx = static_cast<int64_t>(idx);
     }
   }
   if (p < st.best_p || (p == st.best_p && snp < st.best_snp)) {
-    st.best_p = p; st.best_snp = snp; st.best_af = afj; st.best_beta = o.beta;
-    st.best_se = o.se; st.best_stat = o.stat; st.best_r2 = o.r2;
+    st.best_p = p; st.best_snp = snp; st.best_af = afj; st.best_beta = beta;
+    st.best_se = se; st.best_stat = stat; st.best_r2 = r2;
   }
   if (polish) {
     const unsigned long long pi = atomicAdd(a.hit_count + 1, 1ull);
     if (static_cast<int64_t>(pi) < a.polish_cap) a.polish[pi] = PolishItem{static_cast<int32_t>(j), gene, hidx};
   }
-}
-
-template <int MODEL>
-__device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, int64_t j, int gene, double gscal,
-                                 double gg, double S, double num, const TailView& tv, GeneAcc& st, dd& asum) {
-  wald_book<MODEL>(a, ri, jl, j, gene, wald_stats<MODEL>(a, ri.gtg[jl], gscal, gg, S, num, tv), st, asum);
 }
 
 __device__ inline void lds_direct16(const void* gsrc, unsigned char* lds_wave_base) {

@@ -332,70 +332,29 @@ __device__ inline W8Gene w8_gene(const WaldArgs& a, int group) {
 }
 
 // FP64 epilogue of one tile (2048 tests, 4 per thread), after the fold is in
-// LDS: test t = row * 16 + gene has gg = fold[t] + fold[3T + t], S = gg -
-// (fold[T + t] + fold[4T + t]), num = fold[2T + t] + fold[5T + t] (T = tests
-// per tile). Pass 1 applies the window / keep rules and writes FP64-refined
-// numerators back into the fold; for the LMM, pass 2 evaluates the thread's
-// four tests as independent chains (the t tail and ACAT math is
-// latency-bound) and pass 3 does the bookkeeping in row order. The per-thread gene states are merged over the
-// four lanes holding the same gene, then over the eight waves (wpart) into the
-// running state in gstate.
-template <int MODEL>
+// LDS: fold(t, gg, S, num) reads test t = row * 16 + gene. The per-thread gene
+// states are merged over the four lanes holding the same gene, then over the
+// eight waves (wpart) into the running state in gstate.
+template <int MODEL, class Fold>
 __device__ inline void w8_epilogue(const WaldArgs& a, const W8Gene& gi, const RowInfo& ri, const TailView& tv,
-                                   GeneAcc* gstate, GeneAcc* wpart, int64_t row0, double* fold) {
-  constexpr int kT = kW8Tests / kW8Threads;
+                                   GeneAcc* gstate, GeneAcc* wpart, int64_t row0, Fold fold) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  auto fold_vals = [&](int t, double& gg, double& S, double& num) {
-    gg = fold[0 * kW8Tests + t] + fold[3 * kW8Tests + t];
-    S = gg - (fold[1 * kW8Tests + t] + fold[4 * kW8Tests + t]);
-    num = fold[2 * kW8Tests + t] + fold[5 * kW8Tests + t];
-  };
-  unsigned okm = 0;
+  GeneAcc st;
+  acc_init(st);
+  dd asum{0.0, 0.0};
 #pragma unroll 1
-  for (int i = 0; i < kT; ++i) {
+  for (int i = 0; i < kW8Tests / kW8Threads; ++i) {
     const int jl = (threadIdx.x >> 4) + (kW8Threads / 16) * i;
     const int t = jl * 16 + gi.gl;
     const int64_t j = row0 + jl;
     const bool inwin = gi.win_known && ri.chrom[jl] == gi.gch && ri.pos[jl] >= gi.gcs && ri.pos[jl] <= gi.gce;
     const bool ok = gi.ok && j < a.m && ri.keep[jl] && (a.cis ? inwin : !inwin);
     double gg, S, num;
-    fold_vals(t, gg, S, num);
+    fold(t, gg, S, num);
     const bool cand = refine_wanted<MODEL>(a, ok, gg, S, num, gi.gscal, ri.gtg[jl]);
     const double n64 = wave_refine(a, cand, j, gi.gene);
-    if (cand) {
-      fold[2 * kW8Tests + t] = n64;
-      fold[5 * kW8Tests + t] = 0.0;
-    }
-    if (ok) okm |= 1u << i;
-  }
-  GeneAcc st;
-  acc_init(st);
-  dd asum{0.0, 0.0};
-  if constexpr (MODEL == EQTL_MODEL_LMM) {
-    WaldOut w[kT];
-#pragma unroll
-    for (int i = 0; i < kT; ++i) {
-      const int jl = (threadIdx.x >> 4) + (kW8Threads / 16) * i;
-      double gg, S, num;
-      fold_vals(jl * 16 + gi.gl, gg, S, num);
-      w[i] = wald_stats<MODEL>(a, ri.gtg[jl], gi.gscal, gg, S, num, tv);
-    }
-#pragma unroll
-    for (int i = 0; i < kT; ++i) {
-      const int jl = (threadIdx.x >> 4) + (kW8Threads / 16) * i;
-      if (okm >> i & 1u) wald_book<MODEL>(a, ri, jl, row0 + jl, gi.gene, w[i], st, asum);
-    }
-  } else {
-    // the score test's tail (erfc) is short: one test at a time (measured faster)
-#pragma unroll 1
-    for (int i = 0; i < kT; ++i) {
-      if (!(okm >> i & 1u)) continue;
-      const int jl = (threadIdx.x >> 4) + (kW8Threads / 16) * i;
-      double gg, S, num;
-      fold_vals(jl * 16 + gi.gl, gg, S, num);
-      wald_book<MODEL>(a, ri, jl, row0 + jl, gi.gene, wald_stats<MODEL>(a, ri.gtg[jl], gi.gscal, gg, S, num, tv),
-                       st, asum);
-    }
+    if (cand) num = n64;
+    if (ok) wald_test<MODEL>(a, ri, jl, j, gi.gene, gi.gscal, gg, S, num, tv, st, asum);
   }
   st.acat_hi = asum.hi;
   st.acat_lo = asum.lo;
@@ -406,7 +365,7 @@ __device__ inline void w8_epilogue(const WaldArgs& a, const W8Gene& gi, const Ro
   __syncthreads();
   if (threadIdx.x < 16) {
     GeneAcc g = gstate[threadIdx.x];
-    for (int w2 = 0; w2 < kW8Waves; ++w2) acc_merge(g, wpart[w2 * 16 + threadIdx.x]);
+    for (int w = 0; w < kW8Waves; ++w) acc_merge(g, wpart[w * 16 + threadIdx.x]);
     gstate[threadIdx.x] = g;
   }
 }
@@ -445,7 +404,7 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
   RowInfo& ri = *reinterpret_cast<RowInfo*>(smem + r0b);
   double* thead = reinterpret_cast<double*>(smem + r0b + sizeof(RowInfo));
   GeneAcc* gstate = reinterpret_cast<GeneAcc*>(smem + r0b + sizeof(RowInfo) + sizeof(double) * kTailHead);
-  double* fold = reinterpret_cast<double*>(smem);
+  const double* fold = reinterpret_cast<const double*>(smem);
   GeneAcc* wpart = reinterpret_cast<GeneAcc*>(smem + kW8FoldBytes);
   for (int i = threadIdx.x; i < kTailHead; i += kW8Threads) thead[i] = (&a.tail->df)[i];
   if (threadIdx.x < 16) acc_init(gstate[threadIdx.x]);
@@ -461,7 +420,11 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
     if (c == 0) w8_tile<PREC, NQ, NQ0, 0, true>(a, smem, gsrc, row0, c, r, lane);
     else w8_tile<PREC, NQ, NQ1, NQ0, false>(a, smem, gsrc, row0, c, r, lane);
     __syncthreads();  // fold complete
-    w8_epilogue<MODEL>(a, gi, ri, tv, gstate, wpart, row0, fold);
+    w8_epilogue<MODEL>(a, gi, ri, tv, gstate, wpart, row0, [&](int t, double& gg, double& S, double& num) {
+      gg = fold[0 * kW8Tests + t] + fold[3 * kW8Tests + t];
+      S = gg - (fold[1 * kW8Tests + t] + fold[4 * kW8Tests + t]);
+      num = fold[2 * kW8Tests + t] + fold[5 * kW8Tests + t];
+    });
   }
   if (threadIdx.x < 16) {
     const size_t slot =

@@ -1410,9 +1410,13 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
       if (glm)
         hipLaunchKernelGGL(k_polish<EQTL_MODEL_GLM>, dim3(static_cast<unsigned>(pa.count)), dim3(256), 0, c->stream,
                            pa);
-      else
+      else {
+        hipLaunchKernelGGL(k_polish_rotate, dim3(static_cast<unsigned>(pa.count), static_cast<unsigned>((c->n + 63) / 64)),
+                           dim3(256), 0, c->stream, pa);
+        CHK(check_launch("k_polish_rotate"));
         hipLaunchKernelGGL(k_polish<EQTL_MODEL_LMM>, dim3(static_cast<unsigned>(pa.count)), dim3(256), 0, c->stream,
                            pa);
+      }
       CHK(check_launch("k_polish"));
       hipLaunchKernelGGL(k_polish_best, dim3(static_cast<unsigned>((pa.count + 255) / 256)), dim3(256), 0,
                          c->stream, c->state.as<GeneAcc>(), pa.out, pa.count);

@@ -718,6 +718,36 @@ struct PolishArgs {
 // GLM (test_glm_nb, model_glm.cpp:100-132): the same quantities without the
 // rotation — weights w_k, covariates X, residual r_k — and U = num + mean * sum r,
 // info = S, two-sided normal p.
+// Q^T g_c for each polished LMM test, spread over the chip: grid (items,
+// ceil(n / 64)); wave w of block (item, cb) forms columns cb * 64 + 16 w ..
+// + 15, four at a time (independent accumulators keep the column loads in
+// flight), lanes striding the samples and a butterfly sum per column.
+__global__ __launch_bounds__(256) void k_polish_rotate(PolishArgs a) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const PolishItem it = a.items[blockIdx.x];
+  const int n = a.n;
+  const double* gc = a.A64 + static_cast<size_t>(it.j) * a.n_pad;
+  double* gr = a.gt + static_cast<size_t>(blockIdx.x) * n;
+  const int cbeg = blockIdx.y * 64 + wave * 16;
+  const int cend = min(cbeg + 16, n);
+  for (int c0 = cbeg; c0 < cend; c0 += 4) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = lane; i < n; i += 64) {
+      const double g = gc[i];
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (c0 + q < cend) s[q] = fma(a.Q[static_cast<size_t>(c0 + q) * n + i], g, s[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double v = s[q];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+      if (lane == 0 && c0 + q < cend) gr[c0 + q] = v;
+    }
+  }
+}
+
 template <int MODEL>
 __global__ __launch_bounds__(256) void k_polish(PolishArgs a) {
   __shared__ double red[4];
@@ -725,21 +755,9 @@ __global__ __launch_bounds__(256) void k_polish(PolishArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const PolishItem it = a.items[blockIdx.x];
   const int n = a.n, p = a.p, k = it.gene;
-  const double* gc = a.A64 + static_cast<size_t>(it.j) * a.n_pad;
-  const double* gt = gc;
-  if constexpr (MODEL == EQTL_MODEL_LMM) {
-    double* gr = a.gt + static_cast<size_t>(blockIdx.x) * n;
-    for (int c = wave; c < n; c += 4) {
-      const double* qc = a.Q + static_cast<size_t>(c) * n;
-      double s = 0;
-      for (int i = lane; i < n; i += 64) s = fma(qc[i], gc[i], s);
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-      if (lane == 0) gr[c] = s;
-    }
-    __syncthreads();
-    gt = gr;
-  }
+  // LMM: the rotated dosage Q^T g_c comes from k_polish_rotate
+  const double* gt = MODEL == EQTL_MODEL_LMM ? a.gt + static_cast<size_t>(blockIdx.x) * n
+                                             : a.A64 + static_cast<size_t>(it.j) * a.n_pad;
   const double dk = MODEL == EQTL_MODEL_LMM ? a.delta[k] : 0.0;
   const double* u = a.U + static_cast<size_t>(k) * n;
   const double* wk = MODEL == EQTL_MODEL_GLM ? a.w + static_cast<size_t>(k) * n : nullptr;

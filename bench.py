@@ -17,7 +17,8 @@ split form's p-values are compared with the FP64 oracle on a sample of
 Other paths of the engine (--workload), each printing one JSON line of the
 same shape with its own metric, roofline of its dominant kernel and CPU
 baseline: trans-lm (LM scan, same data), trans-glm (NB score tests on counts
-drawn from the same phenotypes), cis-perm (cis LM + LMM with 1000 gene-level
+drawn from the same phenotypes), cis-lmm (cis LMM scan reporting every pair,
+n = 1000, 100k SNPs, 2000 genes, 1 Mb flanks), cis-perm (cis LM + LMM with 1000 gene-level
 permutations, n = 1000, a gene slice of a 20k-gene table), grm (the GRM at
 n = 5000 x 1M SNPs, then eigh and the REML adjoint), vcf-gt / vcf-ds (device
 VCF decode).
@@ -704,6 +705,133 @@ def bench_cis_perm(args):
     ctx.close()
 
 
+C2_N, C2_M, C2_G, C2_NCOV, C2_NCHROM, C2_FLANK = 1000, 100_000, 2000, 15, 10, 1_000_000
+
+
+def bench_cis_lmm(args):
+    """Cis LMM scan with every pair reported (--model lmm --mode cis -w 1000000
+    --pval-cis 1): n = 1000, 100k SNPs on 10 chromosomes (2 kb apart), 2000
+    genes with 1 kb bodies and a 1 Mb flank, polygenic phenotypes
+    (h2 ~ U(0, 0.6)) with 1-3 cis eQTLs per gene, 15 covariates + intercept;
+    GRM and eigendecomposition in the same process (untimed). One step =
+    per-gene REML delta (the reference search) + bases + decode + rotation +
+    fused Wald over the windows, all pairs copied to the host."""
+    import ctypes
+    import eqtl_amd
+    import oracle as O
+    from eqtl_amd import synth
+    from eqtl_amd._lib import lib, _check
+    global BLOCK
+    BLOCK = args.block
+    n, m, G = C2_N, C2_M, C2_G
+    rng = np.random.default_rng(1002)
+    bps = (n + 3) // 4
+    ctx = eqtl_amd.Context(0)
+    rows_dev = ctx.synth_bed(1002, m, n, 0.01, 0.5, 0.01)
+    per = m // C2_NCHROM
+    chrom = np.repeat(np.arange(1, C2_NCHROM + 1, dtype=np.int32), per)
+    pos = np.tile(1 + 2000 * np.arange(per, dtype=np.int64), C2_NCHROM)
+    gch = rng.integers(1, C2_NCHROM + 1, G).astype(np.int32)
+    tss = rng.integers(1, per * 2000 - 1000, G).astype(np.int64)
+    order = np.lexsort((tss, gch))
+    gch, tss = gch[order], tss[order]
+    cs, ce = np.maximum(1, tss - C2_FLANK), tss + 999 + C2_FLANK
+    X = np.hstack([np.ones((n, 1)), rng.standard_normal((n, C2_NCOV))])
+    causal = np.sort(rng.choice(m, 500, replace=False))
+    Z = np.stack([synth.standardized(synth.unpack_bed(ctx.copy_rows(rows_dev, int(j), 1, bps), n))[0] for j in causal])
+    h2 = rng.uniform(0, 0.6, G)
+    u = Z.T @ rng.standard_normal((500, G))
+    u = (u - u.mean(0)) / u.std(0)
+    Y = rng.standard_normal((n, G)) + u * np.sqrt(h2 / (1 - h2))
+    for k in range(G):
+        first = (gch[k] - 1) * per
+        lo = first + max(0, (int(cs[k]) - 1 + 1999) // 2000)
+        hi = first + min(per - 1, (int(ce[k]) - 1) // 2000)
+        for _ in range(int(rng.integers(1, 4))):
+            j = int(rng.integers(lo, hi + 1))
+            g = synth.unpack_bed(ctx.copy_rows(rows_dev, j, 1, bps), n)[0].astype(float)
+            g[g < 0] = g[g >= 0].mean()
+            Y[:, k] += rng.uniform(0.1, 0.5) * (1 if rng.random() < 0.5 else -1) * (g - g.mean())
+    ctx.set_samples(n)
+    ctx.set_policy(True, 0.8, 0.0)
+    _check(lib().eqtl_gpu_grm_begin(ctx.handle))
+    for j0 in range(0, m, BLOCK):
+        mb = min(BLOCK, m - j0)
+        _check(lib().eqtl_gpu_grm_accumulate(ctx.handle, ctypes.c_void_p(rows_dev + j0 * bps), ctypes.c_int32(1),
+                                             ctypes.c_int64(mb), ctypes.c_int32(bps)))
+    K, _, _ = ctx.grm_finish()
+    Q, lam = ctx.eigh(K)
+    ctx.set_snp_annot(chrom, pos)
+    prec = eqtl_amd.PREC_BF16X3
+
+    def step():
+        ctx.lmm_prep(Y, X, delta_mode=eqtl_amd.DELTA_REF)
+        ctx.set_gene_windows(gch, cs, ce)
+        ctx.scan_begin(eqtl_amd.MODEL_LMM, prec, 1.0, eqtl_amd.SCOPE_CIS)
+        nh = 0
+        for j0 in range(0, m, BLOCK):
+            mb = min(BLOCK, m - j0)
+            nh += ctx.scan_block(None, j0, hit_cap=1 << 22, rows_dev=rows_dev + j0 * bps, m=mb, bps=bps).shape[0]
+        return ctx.scan_end(), nh
+
+    for _ in range(args.warmup):
+        step()
+    ctx.set_timing(True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    tests = 0
+    for s_ in range(args.steps):
+        st, nh = step()
+        tests += int(st["n_tested"].sum())
+        log(f"step {s_}: {int(st['n_tested'].sum())} tests, {nh} pairs reported")
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    fam = {f: ctx.timing(f) for f in ("wald", "rotate", "reml", "basis", "decode", "rotate_yx")}
+    ctx.set_timing(False)
+    log(f"step time {dt / args.steps:.3f}s: " + ", ".join(f"{f} {ms / args.steps:.1f} ms" for f, (ms, _) in fam.items()))
+    p = X.shape[1]
+    w_ms, w_n = fam["wald"]
+    r_ms, r_n = fam["rotate"]
+    rot_tflops = 2.0 * n * n * m * args.steps / (r_ms * 1e-3) / 1e12 if r_ms > 0 else None
+    wald_tflops = 2.0 * n * (p + 2) * tests / (w_ms * 1e-3) / 1e12 if w_ms > 0 else None
+    cpu = None
+    if not args.no_cpu:
+        # the oracle's scan_gene_snps restatement (no permutations) on a few genes
+        genes = list(range(0, G, G // 8))[:8]
+        t1 = time.perf_counter()
+        nt = 0
+        for k in genes:
+            first = (gch[k] - 1) * per
+            lo = first + max(0, (int(cs[k]) - 1 + 1999) // 2000)
+            hi = first + min(per - 1, (int(ce[k]) - 1) // 2000)
+            rows = ctx.copy_rows(rows_dev, lo, hi - lo + 1, bps)
+            dos, _, keep, _ = O.decode_bed(rows, n)
+            r = O.cis_gene("lmm", Y[:, k], X, dos[keep], B=0, gene=f"g{k}", Q=Q, lam=lam)
+            nt += r["n_tested"]
+        dtc = time.perf_counter() - t1
+        cpu = {"value": nt / dtc, "unit": "cis LMM tests/s", "cores": 1, "kind": "port",
+               "sample": f"{len(genes)} genes ({nt} window SNPs) through the oracle's restatement of "
+                         f"scan_gene_snps + test_lmm (REML per gene, Q^T g per test) on one thread, {dtc:.1f} s"}
+        log("cpu baseline", cpu)
+    out = {
+        "metric": "cis LMM tests/s (all pairs reported)",
+        "value": tests / dt, "unit": "cis LMM tests/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16x3 contractions, fp64 REML and epilogue", "data": "synthetic (generated in HBM, seeded)",
+        "config": {"workload": "cis LMM n=1000 samples x 100k SNPs x 2000 genes, +-1 Mb windows, p=16, 1% missing",
+                   "samples": n, "snps": m, "genes": G, "covariates": p, "delta": "reference search"},
+        "roofline": {"bound": "mfma", "achieved": rot_tflops, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": rot_tflops / BF16_PEAK_TFLOPS if rot_tflops else None, "traffic": None,
+                     "note": "k_rotate: algorithmic 2n^2 FLOP per SNP (Q^T g, once per SNP for all genes) over its "
+                             "event-timed launches; the split form issues 3 bf16 products per MAC"},
+        "cpu_baseline": cpu,
+        "kernel_ms_per_step": {f: ms / args.steps for f, (ms, _) in fam.items()},
+        "wald_tflops_algorithmic": wald_tflops,
+    }
+    print(json.dumps(out))
+    ctx.close()
+
+
 GRM_N, GRM_M, GRM_G = 5000, 1_000_000, 20_000
 FP64_PEAK_TFLOPS = 78.6
 
@@ -830,10 +958,10 @@ def main():
     ap.add_argument("--perm", type=int, default=1000, help="cis-perm: permutations per gene")
     ap.add_argument("--probe-copy", action="store_true",
                     help="first run a 1 GiB device-to-device copy (reference kernel for HBM byte counters)")
-    ap.add_argument("--workload", choices=["trans-lmm", "trans-lm", "trans-glm", "cis-perm", "grm", "vcf-gt",
-                                           "vcf-ds"], default="trans-lmm",
+    ap.add_argument("--workload", choices=["trans-lmm", "trans-lm", "trans-glm", "cis-lmm", "cis-perm", "grm",
+                                           "vcf-gt", "vcf-ds"], default="trans-lmm",
                     help="trans-lmm: the association scan (default); trans-lm: the LM scan; trans-glm: the NB "
-                         "score-test scan; cis-perm: cis LM+LMM with gene-level permutations; grm: the GRM at "
+                         "score-test scan; cis-lmm: the cis LMM scan, all pairs; cis-perm: cis LM+LMM with gene-level permutations; grm: the GRM at "
                          "n=5000 (+ eigh, REML adjoint); vcf-gt / vcf-ds: the device VCF decode")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -848,6 +976,7 @@ def main():
         log("copy probe: 1 GiB read + 1 GiB written")
         del x, y
     other = {"trans-glm": bench_glm, "trans-lm": bench_trans_lm, "cis-perm": bench_cis_perm, "grm": bench_grm,
+             "cis-lmm": bench_cis_lmm,
              "vcf-gt": lambda a: bench_vcf(a, False), "vcf-ds": lambda a: bench_vcf(a, True)}
     if args.workload in other:
         other[args.workload](args)

@@ -1248,11 +1248,19 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     Timed t(c, "rotate");
     dim3 grid(static_cast<unsigned>(np / 64), static_cast<unsigned>(m_pad / 64));
     if (c->scan_prec == EQTL_PREC_FP32)
-      hipLaunchKernelGGL(k_rotate<EQTL_PREC_FP32>, grid, dim3(256), 0, c->stream, c->A32.as<float>(),
-                         c->Q32.as<float>(), c->Qhi.as<__bf16>(), c->Qlo.as<__bf16>(), c->Gt.as<float>(), np);
-    else
-      hipLaunchKernelGGL(k_rotate<EQTL_PREC_BF16X3>, grid, dim3(256), 0, c->stream, c->A32.as<float>(),
-                         c->Q32.as<float>(), c->Qhi.as<__bf16>(), c->Qlo.as<__bf16>(), c->Gt.as<float>(), np);
+      hipLaunchKernelGGL(k_rotate_f32, grid, dim3(256), 0, c->stream, c->A32.as<float>(), c->Q32.as<float>(),
+                         c->Gt.as<float>(), np);
+    else {
+      static bool attr_set = false;
+      if (!attr_set) {
+        HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rotate_lds),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kRotLds)));
+        attr_set = true;
+      }
+      const dim3 g2(static_cast<unsigned>((np + kRotT - 1) / kRotT), static_cast<unsigned>(m_pad / kRotT));
+      hipLaunchKernelGGL(k_rotate_lds, g2, dim3(256), kRotLds, c->stream, c->A32.as<float>(), c->Qhi.as<__bf16>(),
+                         c->Qlo.as<__bf16>(), c->Gt.as<float>(), np);
+    }
     CHK(check_launch("k_rotate"));
     CHK(t.done());
     Aop = c->Gt.as<float>();

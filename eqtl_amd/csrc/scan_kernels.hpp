@@ -2,7 +2,7 @@
 //   K1 k_decode   — genotype rows (2-bit PLINK or float dosage) → centred,
 //                   mean-imputed dosages + per-SNP filters/stats
 //                   (plink_bed.cpp:263-320, scan_common.cpp:133-151)
-//   K6 k_rotate   — G~ = (G - mu)^T Q on MFMA (scan_trans_lmm.cpp:168)
+//   K6 k_rotate_lds / k_rotate_f32 — G~ = (G - mu)^T Q on MFMA (scan_trans_lmm.cpp:168)
 //   K7/K8 k_wald  — fused contraction against the per-gene column blocks +
 //                   FP64 Wald epilogue + per-gene reductions + hit emission
 //                   (scan_trans_lmm.cpp:11-40,170-188; scan_trans_lm.cpp:21-36,
@@ -182,14 +182,13 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// K6: rotation Gt[j, c] = sum_i A[j, i] Q[i, c]. WG tile 64 rows x 64 coords,
-// 4 waves x (16 rows x 4 blocks of 16 coords). Q stored coordinate-major
-// (Q[c * n_pad + i] = Q(i, c)), i.e. K-contiguous for the B operand.
+// K6, FP32 form: rotation Gt[j, c] = sum_i A[j, i] Q[i, c]. WG tile 64 rows x
+// 64 coords, 4 waves x (16 rows x 4 blocks of 16 coords), operands read from
+// L2. Q stored coordinate-major (Q[c * n_pad + i] = Q(i, c)), i.e.
+// K-contiguous for the B operand.
 // ---------------------------------------------------------------------------
-template <int PREC>
-__global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ A, const float* __restrict__ Q32,
-                                                const __bf16* __restrict__ Qhi, const __bf16* __restrict__ Qlo,
-                                                float* __restrict__ Gt, int n_pad) {
+__global__ __launch_bounds__(256) void k_rotate_f32(const float* __restrict__ A, const float* __restrict__ Q32,
+                                                    float* __restrict__ Gt, int n_pad) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t r0 = static_cast<int64_t>(blockIdx.y) * 64 + wave * 16;
   const int c0 = blockIdx.x * 64;
@@ -200,23 +199,10 @@ __global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ A, con
   const float* arow = A + (r0 + (lane & 15)) * n_pad + kk;
   for (int k0 = 0; k0 < n_pad; k0 += 32) {
     const f32x8 av = load8(arow + k0);
-    if constexpr (PREC == EQTL_PREC_FP32) {
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
-        const size_t bo = static_cast<size_t>(c0 + cb * 16 + (lane & 15)) * n_pad + k0 + kk;
-        acc[cb] = mfma_f32x8(av, load8(Q32 + bo), acc[cb]);
-      }
-    } else {
-      bf16x8 ah, al;
-      split_bf16(av, ah, al);
-#pragma unroll
-      for (int cb = 0; cb < 4; ++cb) {
-        const size_t bo = static_cast<size_t>(c0 + cb * 16 + (lane & 15)) * n_pad + k0 + kk;
-        const bf16x8 bh = load8_bf16(Qhi + bo), bl = load8_bf16(Qlo + bo);
-        acc[cb] = mfma_bf16(ah, bh, acc[cb]);
-        acc[cb] = mfma_bf16(ah, bl, acc[cb]);
-        acc[cb] = mfma_bf16(al, bh, acc[cb]);
-      }
+    for (int cb = 0; cb < 4; ++cb) {
+      const size_t bo = static_cast<size_t>(c0 + cb * 16 + (lane & 15)) * n_pad + k0 + kk;
+      acc[cb] = mfma_f32x8(av, load8(Q32 + bo), acc[cb]);
     }
   }
 #pragma unroll
@@ -225,6 +211,110 @@ __global__ __launch_bounds__(256) void k_rotate(const float* __restrict__ A, con
     for (int r = 0; r < 4; ++r) {
       const int64_t row = r0 + 4 * (lane >> 4) + r;
       Gt[row * n_pad + c0 + cb * 16 + (lane & 15)] = acc[cb][r];
+    }
+}
+
+// K6, split form, staged through LDS: 128 SNP rows x 128 coordinates per
+// workgroup. Per 32-sample step the rows' FP32 values (8 chunks of 16 B per
+// row) and the coordinates' Qhi / Qlo (4 chunks each) are copied global -> LDS
+// with direct 16-byte loads, double-buffered and stored chunk-major (slot
+// (c, r) at (c * 128 + r) * 16), so the 16 lanes of an MFMA fragment read 16
+// consecutive slots. Wave (wr, wc) owns the 64 x 64 quadrant (rows wr*64..,
+// coordinates wc*64..) as 4 x 4 blocks, three bf16 products per block
+// (hi*hi, hi*lo, lo*hi, in that order). Coordinates past n_pad
+// are read clamped and not stored.
+constexpr int kRotT = 128;
+constexpr size_t kRotA = 8 * kRotT * 16;       // FP32 rows: 16 KiB per step
+constexpr size_t kRotStage = kRotA + 8 * kRotT * 16;  // + Qhi and Qlo: 32 KiB
+constexpr size_t kRotLds = 2 * kRotStage;
+
+__global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A, const __bf16* __restrict__ Qhi,
+                                                    const __bf16* __restrict__ Qlo, float* __restrict__ Gt,
+                                                    int n_pad) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int64_t R0 = static_cast<int64_t>(blockIdx.y) * kRotT;
+  const int C0 = blockIdx.x * kRotT;
+  // this wave's 8 of the step's 32 copy instructions (q = wave + 4 i):
+  // q < 16 rows of A (chunk q >> 1), then Qhi, then Qlo (chunks (q & 7) >> 1), half h = q & 1
+  const unsigned char* src[8];
+  int step_bytes[8], dst_off[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int q = wave + 4 * i;
+    const int h = q & 1;
+    if (q < 16) {
+      const int c = q >> 1;
+      src[i] = reinterpret_cast<const unsigned char*>(A + (R0 + h * 64 + lane) * n_pad + 4 * c);
+      step_bytes[i] = 32 * static_cast<int>(sizeof(float));
+      dst_off[i] = (c * kRotT + h * 64) * 16;
+    } else {
+      const int c = (q & 7) >> 1;
+      const int coord = min(C0 + h * 64 + lane, n_pad - 1);
+      const __bf16* base = q < 24 ? Qhi : Qlo;
+      src[i] = reinterpret_cast<const unsigned char*>(base + static_cast<size_t>(coord) * n_pad + 8 * c);
+      step_bytes[i] = 32 * static_cast<int>(sizeof(__bf16));
+      dst_off[i] = static_cast<int>(kRotA) + (q < 24 ? 0 : 4 * kRotT * 16) + (c * kRotT + h * 64) * 16;
+    }
+  }
+  auto issue = [&](int s, unsigned char* buf) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      __builtin_amdgcn_global_load_lds(src[i] + s * step_bytes[i],
+                                       (__attribute__((address_space(3))) void*)(buf + dst_off[i]), 16, 0, 0);
+  };
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int nks = n_pad / 32;
+  issue(0, rsm);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  const int g = lane >> 4;
+  for (int s = 0; s < nks; ++s) {
+    const unsigned char* cur = rsm + (s & 1) * kRotStage;
+    if (s + 1 < nks) issue(s + 1, rsm + ((s + 1) & 1) * kRotStage);
+    bf16x8 bh[4], bl[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      const int rc = wc * 64 + cb * 16 + (lane & 15);
+      bh[cb] = *reinterpret_cast<const bf16x8*>(cur + kRotA + (g * kRotT + rc) * 16);
+      bl[cb] = *reinterpret_cast<const bf16x8*>(cur + kRotA + 4 * kRotT * 16 + (g * kRotT + rc) * 16);
+    }
+#pragma unroll
+    for (int rb = 0; rb < 4; ++rb) {
+      const int ra = wr * 64 + rb * 16 + (lane & 15);
+      const float4 lo = *reinterpret_cast<const float4*>(cur + ((2 * g) * kRotT + ra) * 16);
+      const float4 hi = *reinterpret_cast<const float4*>(cur + ((2 * g + 1) * kRotT + ra) * 16);
+      f32x8 av;
+      av.v[0] = lo.x; av.v[1] = lo.y; av.v[2] = lo.z; av.v[3] = lo.w;
+      av.v[4] = hi.x; av.v[5] = hi.y; av.v[6] = hi.z; av.v[7] = hi.w;
+      bf16x8 ah, al;
+      split_bf16(av, ah, al);
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb) {
+        acc[rb][cb] = mfma_bf16(ah, bh[cb], acc[rb][cb]);
+        acc[rb][cb] = mfma_bf16(ah, bl[cb], acc[rb][cb]);
+        acc[rb][cb] = mfma_bf16(al, bh[cb], acc[rb][cb]);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int rb = 0; rb < 4; ++rb)
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb) {
+      const int col = C0 + wc * 64 + cb * 16 + (lane & 15);
+      if (col >= n_pad) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t row = R0 + wr * 64 + rb * 16 + 4 * (lane >> 4) + r;
+        Gt[row * n_pad + col] = acc[rb][cb][r];
+      }
     }
 }
 

@@ -547,13 +547,13 @@ __device__ inline void lds_direct16(const void* gsrc, unsigned char* lds_wave_ba
 }
 
 // LDS layout of the LM Wald kernel. Region 0 holds, during the contraction,
-// the two k-step buffers of the gene tile (2 KiB each) and a ring of three
-// SNP-row stages; then gty of the tile's tests during the epilogue; then the
+// the two k-step buffers of the gene tile (2 KiB each) and a ring of
+// kLmStages SNP-row stages; then gty of the tile's tests during the epilogue; then the
 // per-gene partials of the final reduction. RowInfo and the t-tail head
 // follow it.
 constexpr int kFoldDoubles = kRB * 4 * 256;  // one gty per test of the tile
 constexpr int kAStageBytes = kTileRows * 32 * 4;  // 128 rows x 32 samples fp32
-constexpr int kLmStages = 3;
+constexpr int kLmStages = 2;
 constexpr long kLdsCap = 160 * 1024;
 __host__ __device__ constexpr long wald_tail_bytes() {
   return static_cast<long>(sizeof(RowInfo)) + static_cast<long>(sizeof(double)) * kTailHead;
@@ -628,7 +628,11 @@ __device__ __attribute__((always_inline)) inline void lm_kstep(
 // set of 128-SNP tiles, 4 waves x 32 rows; the group's one-block gene tile and
 // the SNP rows are staged in LDS as in the LMM kernel.
 template <int PREC>
-__global__ __launch_bounds__(256) void k_wald_lm(WaldArgs a) {
+// Three waves per SIMD (registers capped at 168, two row stages so that three
+// workgroups fit in LDS): the FP64 epilogue bounds this kernel and needs the
+// extra waves to hide its latency (measured 10% faster than two waves with
+// three stages).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_wald_lm(WaldArgs a) {
   constexpr int kAS = kLmStages;
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   // XCD-aware mapping: workgroups are dealt round-robin to the 8 XCDs; the
@@ -729,10 +733,11 @@ __global__ __launch_bounds__(256) void k_wald_lm(WaldArgs a) {
       unsigned char* const b_nxt = s + 1 < nks ? (((s + 1) & 1) ? buf1 : buf0) : nullptr;
       lm_kstep<PREC>(acc, a_rd, a_wr, src, (s & 1) ? buf1 : buf0, b_nxt,
                      gsrc + static_cast<size_t>(s + 1) * tile_bytes, wave, lane);
-      // the next k-step's gene tile and rows must have landed; the rows of
-      // k-step s+2 (issued last) stay in flight. The barrier is a bare
-      // s_barrier: a workgroup release fence would drain vmcnt to zero.
-      if (s + 2 < nks) __builtin_amdgcn_s_waitcnt(kWaitVm4Lgkm0);
+      // the next k-step's gene tile and rows must have landed (with three
+      // stages the rows of k-step s+2, issued last, could stay in flight).
+      // The barrier is a bare s_barrier: a workgroup release fence would
+      // drain vmcnt to zero.
+      if (kAS == 3 && s + 2 < nks) __builtin_amdgcn_s_waitcnt(kWaitVm4Lgkm0);
       else __builtin_amdgcn_s_waitcnt(0);
       lds_barrier();
     }

@@ -127,6 +127,34 @@ def _lmm_run(ctx, ds, rows, Q, lam, win, prec, delta_mode, delta_in=None, pthr=1
     return delta, rss, hits, st
 
 
+def test_reml_delta_reference_search_p16(gpu_ctx):
+    """15 covariates + intercept: X~^T D X~ has 136 packed entries, 128 from the
+    GEMM and the last 8 summed in the weights kernel."""
+    rng = np.random.default_rng(33)
+    n, G = 300, 24
+    K = np.corrcoef(rng.standard_normal((n, 1500)))
+    lam, Q = np.linalg.eigh(K)
+    lam = np.maximum(lam, 0)
+    X = np.hstack([np.ones((n, 1)), rng.standard_normal((n, 15))])
+    u = Q @ (np.sqrt(lam)[:, None] * rng.standard_normal((n, G)))
+    Y = rng.standard_normal((n, G)) + u * rng.uniform(0, 1.5, G)[None, :]
+    gpu_ctx.set_samples(n)
+    gpu_ctx.set_basis(Q, lam)
+    delta, _ = gpu_ctx.lmm_prep(Y, X, delta_mode=eqtl_amd.DELTA_REF)
+    Yt, Xt = O.rotate(Q, Y), O.rotate(Q, X)
+    dref = O.optimize_delta(Yt, Xt, lam)
+    _, _, h = gpu_ctx.reml_grad_hess(np.log(dref))
+    curved = h > 1e-3
+    r = rel_err(delta, dref)
+    print("p=16 REML ref-search delta max rel err (curved genes)", r[curved].max(), "flat genes", int((~curved).sum()))
+    assert curved.sum() >= G // 2
+    assert r[curved].max() < 1e-4
+    for k in range(G):
+        fg = O.reml_negll(delta[k], Yt[:, k], Xt, lam)
+        fo = O.reml_negll(dref[k], Yt[:, k], Xt, lam)
+        assert fg - fo <= 1e-9 * abs(fo), (k, fg, fo)
+
+
 def test_reml_delta_reference_search(gpu_ctx):
     ds, rows, (Q, lam), win, ref = lmm_case()
     gpu_ctx.set_samples(ds.n)

@@ -15,47 +15,42 @@ namespace eqtl_amd {
 
 constexpr int kPMax = 32;   // max covariates (incl. intercept)
 constexpr int kChunk = 128; // sample rows staged in LDS per pass
-constexpr int kMaxTasks = 8;
+constexpr int kMaxTasks = 3;  // per-thread dot products of a pass: p(p+1)/2 + p <= 3 x 256
+static_assert(kPMax * (kPMax + 1) / 2 + kPMax <= 256 * kMaxTasks, "pass tasks");
 
 struct RemlSmem {
-  double Xs[kChunk][kPMax];
-  double ys[kChunk];
-  double w0[kChunk], w1[kChunk], w2[kChunk];
-  double rs[kChunk];
-  double A[kPMax * kPMax];   // A (later: LDLT factors)
-  double A1[kPMax * kPMax];
-  double A2[kPMax * kPMax];
-  double M[kPMax * kPMax];   // A^-1 A1 (columns)
-  double b[kPMax], beta[kPMax], u[kPMax], wv[kPMax], D[kPMax];
-  int tr[kPMax];
-  double red[256];
-  double scal[16];
+  double Xs[kChunk][kPMax];  // a chunk of X~ rows
+  double ys[kChunk];         // the chunk's y~ (or working response)
+  double w0[kChunk];         // the chunk's weights
+  double A[kPMax * kPMax];   // X^T W X (full, symmetric)
+  double A1[kPMax * kPMax];  // its Cholesky factor Lc (lower, column-major)
+  double b[kPMax], u[kPMax]; // X^T W y and u = Lc^-1 b
+  double red[4];             // per-wave partial sums of block_sum
   int flag;
 };
 
 __device__ inline double block_sum(double v, RemlSmem& sm) {
-  sm.red[threadIdx.x] = v;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) sm.red[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) sm.red[threadIdx.x] += sm.red[threadIdx.x + s];
-    __syncthreads();
-  }
-  const double r = sm.red[0];
+  const double r = (sm.red[0] + sm.red[1]) + (sm.red[2] + sm.red[3]);
   __syncthreads();
   return r;
 }
 
-// Pass over all samples accumulating A (and A1, A2 when DERIV), b and the
-// per-row scalars. Collective over the 256 threads.
-template <bool DERIV>
-__device__ __attribute__((always_inline)) inline void reml_pass1(const double* __restrict__ Xt, const double* __restrict__ yt,
-                           const double* __restrict__ lam, int n, int p, double delta, RemlSmem& sm) {
+// Pass over all samples accumulating A = X~^T D X~ (sm.A, both triangles) and
+// b = X~^T D y~ (sm.b), d_i = 1 / max(delta lam_i + 1, 1e-12). Collective over
+// the 256 threads; X~ is staged in LDS by chunks of kChunk rows.
+__device__ __attribute__((always_inline)) inline void reml_pass1(const double* __restrict__ Xt,
+                                                                 const double* __restrict__ yt,
+                                                                 const double* __restrict__ lam, int n, int p,
+                                                                 double delta, RemlSmem& sm) {
   const int TA = p * (p + 1) / 2;
-  const int T = (DERIV ? 3 * TA : TA) + p;
+  const int T = TA + p;
   double acc[kMaxTasks];
 #pragma unroll
   for (int t = 0; t < kMaxTasks; ++t) acc[t] = 0;
-  double s_logv = 0, s_ydy = 0, s_e = 0, s_de = 0;
   for (int i0 = 0; i0 < n; i0 += kChunk) {
     const int rows = min(kChunk, n - i0);
     for (int q = threadIdx.x; q < kChunk * p; q += 256) {
@@ -64,25 +59,16 @@ __device__ __attribute__((always_inline)) inline void reml_pass1(const double* _
     }
     if (threadIdx.x < kChunk) {
       const int r = threadIdx.x;
-      double w0 = 0, w1 = 0, w2 = 0, y = 0;
+      double w0 = 0, y = 0;
       if (r < rows) {
         const int i = i0 + r;
         double v = delta * lam[i] + 1.0;
         if (v < 1e-12) v = 1e-12;
-        const double d = 1.0 / v;
+        w0 = 1.0 / v;
         y = yt[i];
-        w0 = d;
-        s_logv += log(v);
-        s_ydy += y * (d * y);
-        if (DERIV) {
-          const double e = 1.0 - d;
-          w1 = -d * e;
-          w2 = d * e * (e - d);
-          s_e += e;
-          s_de += d * e;
-        }
       }
-      sm.w0[r] = w0; sm.w1[r] = w1; sm.w2[r] = w2; sm.ys[r] = y;
+      sm.w0[r] = w0;
+      sm.ys[r] = y;
     }
     __syncthreads();
 #pragma unroll
@@ -90,17 +76,15 @@ __device__ __attribute__((always_inline)) inline void reml_pass1(const double* _
       const int t = threadIdx.x + 256 * s;
       if (t >= T) break;
       double v = 0;
-      if (t < (DERIV ? 3 * TA : TA)) {
-        const int which = t / TA, tt = t % TA;
-        // triangular index → (a, c), c <= a
-        int a = static_cast<int>((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
-        while ((a + 1) * (a + 2) / 2 <= tt) ++a;
-        while (a * (a + 1) / 2 > tt) --a;
-        const int c = tt - a * (a + 1) / 2;
-        const double* w = which == 0 ? sm.w0 : (which == 1 ? sm.w1 : sm.w2);
-        for (int r = 0; r < rows; ++r) v += sm.Xs[r][a] * w[r] * sm.Xs[r][c];
+      if (t < TA) {
+        // triangular index -> (a, c), c <= a
+        int a = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((a + 1) * (a + 2) / 2 <= t) ++a;
+        while (a * (a + 1) / 2 > t) --a;
+        const int c = t - a * (a + 1) / 2;
+        for (int r = 0; r < rows; ++r) v += sm.Xs[r][a] * sm.w0[r] * sm.Xs[r][c];
       } else {
-        const int a = t - (DERIV ? 3 * TA : TA);
+        const int a = t - TA;
         for (int r = 0; r < rows; ++r) v += sm.Xs[r][a] * (sm.w0[r] * sm.ys[r]);
       }
       acc[s] += v;
@@ -111,26 +95,16 @@ __device__ __attribute__((always_inline)) inline void reml_pass1(const double* _
   for (int s = 0; s < kMaxTasks; ++s) {
     const int t = threadIdx.x + 256 * s;
     if (t >= T) break;
-    if (t < (DERIV ? 3 * TA : TA)) {
-      const int which = t / TA, tt = t % TA;
-      int a = static_cast<int>((sqrt(8.0 * tt + 1.0) - 1.0) * 0.5);
-      while ((a + 1) * (a + 2) / 2 <= tt) ++a;
-      while (a * (a + 1) / 2 > tt) --a;
-      const int c = tt - a * (a + 1) / 2;
-      double* M = which == 0 ? sm.A : (which == 1 ? sm.A1 : sm.A2);
-      M[c * kPMax + a] = acc[s];
-      M[a * kPMax + c] = acc[s];
+    if (t < TA) {
+      int a = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+      while ((a + 1) * (a + 2) / 2 <= t) ++a;
+      while (a * (a + 1) / 2 > t) --a;
+      const int c = t - a * (a + 1) / 2;
+      sm.A[c * kPMax + a] = acc[s];
+      sm.A[a * kPMax + c] = acc[s];
     } else {
-      sm.b[t - (DERIV ? 3 * TA : TA)] = acc[s];
+      sm.b[t - TA] = acc[s];
     }
-  }
-  const double logv = block_sum(s_logv, sm);
-  const double ydy = block_sum(s_ydy, sm);
-  if (threadIdx.x == 0) { sm.scal[0] = logv; sm.scal[1] = ydy; }
-  if (DERIV) {
-    const double se = block_sum(s_e, sm);
-    const double sde = block_sum(s_de, sm);
-    if (threadIdx.x == 0) { sm.scal[2] = se; sm.scal[3] = sde; }
   }
   __syncthreads();
 }
@@ -202,7 +176,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   if (k >= G) return;
   const double* yt = Yt + static_cast<size_t>(k) * n;
   const double dk = delta[k];
-  reml_pass1<false>(Xt, yt, lam, n, p, dk, sm);
+  reml_pass1(Xt, yt, lam, n, p, dk, sm);
   const bool ok = chol_lower_block(sm, p);  // Lc, and u = Lc^-1 b
   for (int idx = threadIdx.x; idx < p * p; idx += 256) {
     const int i = idx % p, jj = idx / p;
@@ -250,7 +224,7 @@ __global__ __launch_bounds__(256) void k_lm_basis(const double* X, const double*
                                                   int* ok_out) {
   __shared__ RemlSmem sm;
   // delta = 0 → weights 1: A = X^T X
-  reml_pass1<false>(X, zeros, zeros, n, p, 0.0, sm);
+  reml_pass1(X, zeros, zeros, n, p, 0.0, sm);
   if (threadIdx.x == 0) sm.flag = chol_lower(sm, p) ? 1 : 0;
   __syncthreads();
   const bool ok = sm.flag != 0;

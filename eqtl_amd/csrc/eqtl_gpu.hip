@@ -643,8 +643,9 @@ static int reml_eval(eqtl_gpu_ctx* c, const std::vector<int32_t>& ek, const std:
   const int n = c->n, p = c->p, TA = p * (p + 1) / 2, df = n - p;
   const int E = static_cast<int>(ek.size());
   const int nw = deriv ? 3 : 1;
-  // f-only rounds (the reference search: two points per gene) go in one batch
-  const int chunk = deriv ? kEvalChunk : 2 * kEvalChunk;
+  // up to 32,768 evaluations per batch (workspace n x 3 x 32,768 doubles at
+  // most): fewer, larger launches of the weight, DGEMM and evaluation kernels
+  const int chunk = 2 * kEvalChunk;
   auto& r = c->rw;
   const size_t cols = static_cast<size_t>(nw) * std::min(E, chunk);
   CHK(ensure(r.W, sizeof(double) * n * cols));

@@ -201,7 +201,7 @@ def test_cis_lmm_permutations(gpu_ctx):
     pm, pe = gpu_ctx.cis_perm(eqtl_amd.PREC_FP32, B, seed, hashes, st["best_p"], delta_mode=eqtl_amd.DELTA_REF)
     # each permutation re-estimates delta with the reference search (restated on
     # the GPU in batched form); its rounding differs from the oracle's, so p is
-    # compared at 1e-3 and ties are judged at 1e-4 of T_obs
+    # compared at 1e-3 and ties are decided at 1e-4 of T_obs
     worst, exact, skipped = _perm_compare(pm, pe, refs, B, 1e-3, 1e-4)
     print(f"cis LMM perms: max perm_min_p rel {worst:.3e}, p_emp exact {exact}, near-tie genes {skipped}")
     assert exact >= 0.9 * sum(1 for _, r in refs if r["n_tested"] > 0)

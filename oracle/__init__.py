@@ -276,6 +276,21 @@ def lmm_test(yt, Xt, lam, delta, gt):
     return out, dout.value
 
 
+
+def lmm_perm_pheno(Q, lam, y, X, delta, idx):
+    """LMM permutation base and one permutation's pseudo-phenotype
+    (scan_cis.hpp:77-87,135-148). Returns (base, y_perm), or None when the
+    LDLT of X̃ᵀD⁻¹X̃ reports failure."""
+    Q = np.asfortranarray(Q, dtype=np.float64)
+    X = np.asfortranarray(X, dtype=np.float64)
+    lam, y = _f64(lam), _f64(y)
+    idx = np.ascontiguousarray(idx, np.int32)
+    n, p = X.shape
+    base, yp = np.empty(n), np.empty(n)
+    ok = lib().orc_lmm_perm_pheno(_p(Q), _p(lam), _p(y), _p(X), ctypes.c_int32(n), ctypes.c_int32(p),
+                                  ctypes.c_double(delta), _p(idx), _p(base), _p(yp))
+    return (base, yp) if ok else None
+
 def _gene_windows(gene_chrom, gene_cstart, gene_cend, G):
     if gene_chrom is None:
         return (np.full(G, -1, np.int32), np.zeros(G, np.int64), np.zeros(G, np.int64))

@@ -1521,6 +1521,51 @@ void orc_perm_index(int32_t seed, int32_t b, const char* gene, int32_t n, int32_
   perm_index(seed, b, gene, n, idx);
 }
 
+// LMM permutation base (scan_cis.hpp:77-87): b0 = (X̃ᵀD⁻¹X̃)⁻¹ X̃ᵀD⁻¹ỹ by
+// LDLT, whitened spectral residuals w = (ỹ − X̃b0)∘√dinv. When the LDLT
+// reports failure, base keeps its input (the raw phenotype) and Xb0 stays empty.
+static void lmm_perm_base(const PrepLmm& lmm, std::vector<double>& base, std::vector<double>& Xb0) {
+  const int n = lmm.n, p = lmm.p;
+  std::vector<double> A(static_cast<size_t>(p) * p), bb(p), b0(p);
+  for (int a = 0; a < p; ++a) {
+    for (int cc = 0; cc <= a; ++cc) {
+      double s = 0;
+      for (int i = 0; i < n; ++i)
+        s += lmm.Xt[static_cast<size_t>(a) * n + i] * lmm.dinv[i] * lmm.Xt[static_cast<size_t>(cc) * n + i];
+      A[static_cast<size_t>(cc) * p + a] = s;
+      A[static_cast<size_t>(a) * p + cc] = s;
+    }
+    double s = 0;
+    for (int i = 0; i < n; ++i) s += lmm.Xt[static_cast<size_t>(a) * n + i] * (lmm.dinv[i] * lmm.yt[i]);
+    bb[a] = s;
+  }
+  Ldlt f;
+  f.compute(A.data(), p);
+  if (!f.ok) return;
+  f.solve(bb.data(), b0.data());
+  Xb0.assign(n, 0.0);
+  for (int i = 0; i < n; ++i) {
+    double s = 0;
+    for (int a = 0; a < p; ++a) s += lmm.Xt[static_cast<size_t>(a) * n + i] * b0[a];
+    Xb0[i] = s;
+    base[i] = (lmm.yt[i] - s) * std::sqrt(lmm.dinv[i]);
+  }
+}
+
+// Pseudo-phenotype of one LMM permutation (scan_cis.hpp:135-148): un-whiten
+// the permuted residuals, r̃ = w_perm / √max(dinv, 1e-12), and rotate back,
+// y = Q (X̃b0 + r̃). Q is column-major n x n (column r = eigenvector r).
+static void lmm_perm_phenotype(const double* Q, const PrepLmm& lmm, const std::vector<double>& Xb0,
+                               const double* wperm, double* rt, double* yy) {
+  const int n = lmm.n;
+  for (int i = 0; i < n; ++i) rt[i] = Xb0[i] + wperm[i] / std::sqrt(std::max(lmm.dinv[i], 1e-12));
+  for (int i = 0; i < n; ++i) {
+    double s = 0;
+    for (int r = 0; r < n; ++r) s += Q[static_cast<size_t>(r) * n + i] * rt[r];
+    yy[i] = s;
+  }
+}
+
 // Cis gene scan with gene-level permutations (scan_cis.hpp:15-185) for LM
 // (model 0) or LMM (model 1) over one gene's window.
 //  y (n), X (n x p), window dosages W (c x n, only SNPs that passed decode),
@@ -1583,37 +1628,10 @@ double orc_cis_gene(int32_t model, const double* y, const double* X, int32_t n, 
   if (B <= 0 || n_tested == 0) return kNaN;
   // permutation base (scan_cis.hpp:74-87)
   std::vector<double> base(y, y + n), Xb0;
-  if (model == 0) {
+  if (model == 0)
     base = lm.ys;
-  } else {
-    std::vector<double> b0(p);
-    Ldlt f;
-    // recompute A00 / XtDy exactly as scan_cis.hpp:80-84
-    std::vector<double> A(static_cast<size_t>(p) * p), bb(p);
-    for (int a = 0; a < p; ++a) {
-      for (int cc = 0; cc <= a; ++cc) {
-        double s = 0;
-        for (int i = 0; i < n; ++i)
-          s += lmm.Xt[static_cast<size_t>(a) * n + i] * lmm.dinv[i] * lmm.Xt[static_cast<size_t>(cc) * n + i];
-        A[static_cast<size_t>(cc) * p + a] = s;
-        A[static_cast<size_t>(a) * p + cc] = s;
-      }
-      double s = 0;
-      for (int i = 0; i < n; ++i) s += lmm.Xt[static_cast<size_t>(a) * n + i] * (lmm.dinv[i] * lmm.yt[i]);
-      bb[a] = s;
-    }
-    f.compute(A.data(), p);
-    if (f.ok) {
-      f.solve(bb.data(), b0.data());
-      Xb0.assign(n, 0.0);
-      for (int i = 0; i < n; ++i) {
-        double s = 0;
-        for (int a = 0; a < p; ++a) s += lmm.Xt[static_cast<size_t>(a) * n + i] * b0[a];
-        Xb0[i] = s;
-        base[i] = (lmm.yt[i] - s) * std::sqrt(lmm.dinv[i]);
-      }
-    }
-  }
+  else
+    lmm_perm_base(lmm, base, Xb0);
   std::vector<int> idx(n);
   std::vector<double> yb(n), rt(n), yy(n);
   for (int b = 0; b < B; ++b) {
@@ -1625,13 +1643,7 @@ double orc_cis_gene(int32_t model, const double* y, const double* X, int32_t n, 
       prep_lm(yb.data(), X, n, p, lmb);
     } else {
       if (!Xb0.empty()) {
-        for (int i = 0; i < n; ++i) rt[i] = Xb0[i] + yb[i] / std::sqrt(std::max(lmm.dinv[i], 1e-12));
-        // y = Q (X̃ b0 + r̃_perm)
-        for (int i = 0; i < n; ++i) {
-          double s = 0;
-          for (int r = 0; r < n; ++r) s += Q[static_cast<size_t>(r) * n + i] * rt[r];
-          yy[i] = s;
-        }
+        lmm_perm_phenotype(Q, lmm, Xb0, yb.data(), rt.data(), yy.data());
       } else {
         yy = yb;
       }
@@ -1650,6 +1662,28 @@ double orc_cis_gene(int32_t model, const double* y, const double* X, int32_t n, 
   }
   const double Tobs = -std::log10(std::max(best, 1e-300));
   return orc_p_emp(Tobs, T_perm, B);
+}
+
+// LMM permutation base of one phenotype and the pseudo-phenotype of one
+// permutation idx (scan_cis.hpp:77-87,135-148), for tests. Q column-major
+// n x n, X column-major n x p, delta > 0 given (<= 0: searched). Writes
+// base_out (whitened residuals) and y_out (pseudo-phenotype); returns 0 and
+// writes nothing when the LDLT of X̃ᵀD⁻¹X̃ reports failure.
+int32_t orc_lmm_perm_pheno(const double* Q, const double* lam, const double* y, const double* X,
+                           int32_t n, int32_t p, double delta, const int32_t* idx, double* base_out,
+                           double* y_out) {
+  std::vector<double> yt(n), Xt(static_cast<size_t>(n) * p);
+  orc_rotate(Q, n, y, 1, yt.data());
+  orc_rotate(Q, n, X, p, Xt.data());
+  PrepLmm lmm;
+  prep_lmm(yt.data(), Xt.data(), lam, n, p, delta, lmm);
+  std::vector<double> base(y, y + n), Xb0, wperm(n), rt(n);
+  lmm_perm_base(lmm, base, Xb0);
+  if (Xb0.empty()) return 0;
+  for (int i = 0; i < n; ++i) wperm[i] = base[idx[i]];
+  lmm_perm_phenotype(Q, lmm, Xb0, wperm.data(), rt.data(), y_out);
+  std::copy(base.begin(), base.end(), base_out);
+  return 1;
 }
 
 // Direct LMM Wald at a given (prep) for one rotated genotype — single-test

@@ -324,6 +324,33 @@ def test_fnv1a_known_values():
         assert O.fnv1a(s) == f(s)
 
 
+
+def test_lmm_perm_pheno_invariants():
+    """LMM permutation base and pseudo-phenotype (scan_cis.hpp:77-87,135-148)
+    pinned by what the algebra fixes: the base is the GLS residual whitened by
+    sqrt(dinv), dinv = 1/(delta*lam + 1) (model_lmm.cpp:124-153), so it is orthogonal to the whitened
+    rotated covariates; the identity permutation gives back the phenotype
+    itself; any permutation's pseudo-phenotype, rotated and with X~b0 taken
+    off, whitens to exactly the permuted base."""
+    rng = np.random.default_rng(11)
+    n, p, delta = 60, 3, 0.7
+    G = rng.standard_normal((n, 2 * n))
+    Q, lam = O.eigh(G @ G.T / (2 * n))
+    X = np.column_stack([np.ones(n), rng.standard_normal((n, p - 1))])
+    y = X @ np.array([1.0, -0.5, 2.0]) + rng.standard_normal(n)
+    yt, Xt = Q.T @ y, Q.T @ X
+    sd = 1.0 / np.sqrt(delta * lam + 1.0)
+    b0 = np.linalg.lstsq(Xt * sd[:, None], yt * sd, rcond=None)[0]
+    w = (yt - Xt @ b0) * sd
+    base, y_id = O.lmm_perm_pheno(Q, lam, y, X, delta, np.arange(n))
+    np.testing.assert_allclose(base, w, rtol=0, atol=1e-10)
+    assert np.abs((Xt * sd[:, None]).T @ base).max() < 1e-10
+    np.testing.assert_allclose(y_id, y, rtol=0, atol=1e-10)
+    idx = O.perm_index(7, 3, "g1", n)
+    base2, y_perm = O.lmm_perm_pheno(Q, lam, y, X, delta, idx)
+    np.testing.assert_array_equal(base2, base)
+    np.testing.assert_allclose((Q.T @ y_perm - Xt @ b0) * sd, base[idx], rtol=0, atol=1e-10)
+
 # ---------------------------------------------------- beta-approximated p value
 def test_lnbeta_mpmath():
     import mpmath as mp

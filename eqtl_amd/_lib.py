@@ -40,7 +40,7 @@ EXPORTS = [
     "eqtl_gpu_set_timing", "eqtl_gpu_get_timing", "eqtl_gpu_synth_bed", "eqtl_gpu_copy_rows",
     "eqtl_gpu_synchronize", "eqtl_gpu_selftest", "eqtl_gpu_upload_bed", "eqtl_gpu_cis_perm",
     "eqtl_gpu_beta_approx", "eqtl_gpu_set_geno_format", "eqtl_gpu_vcf_decode", "eqtl_gpu_glm_prep",
-    "eqtl_gpu_set_test_subset",
+    "eqtl_gpu_set_test_subset", "eqtl_gpu_scan_hits",
 ]
 
 _lib = None
@@ -273,7 +273,17 @@ class Context:
             rc = lib().eqtl_gpu_scan_block(self._c, _p(rows), ctypes.c_int32(0), ctypes.c_int64(snp0), ctypes.c_int64(m),
                                            ctypes.c_int32(bps), _p(hits), ctypes.c_int64(hit_cap), ctypes.byref(nh))
         _check(rc)
-        return hits[:nh.value]
+        total = nh.value
+        if total <= hit_cap:
+            return hits[:total]
+        # drain the rest of the block's hits
+        out = np.empty(total, dtype=HIT_DTYPE)
+        out[:hit_cap] = hits[:hit_cap]
+        got = ctypes.c_int64(0)
+        _check(lib().eqtl_gpu_scan_hits(self._c, ctypes.c_int64(hit_cap), _p(out[hit_cap:]),
+                                        ctypes.c_int64(total - hit_cap), ctypes.byref(got)))
+        assert got.value == total - hit_cap
+        return out
 
     def scan_end(self):
         st = np.empty(self.G, dtype=GENE_STATE_DTYPE)

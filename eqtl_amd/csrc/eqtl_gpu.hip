@@ -41,9 +41,13 @@ constexpr double kRefineTauBF16X3 = 2e-3;
 constexpr double kRefineTauHi = 8.0;
 // NB score tests with |z| > 8 are recomputed in FP64 after the kernel (polish)
 constexpr double kGlmTauHi = 8.0;
-// Capacity of the per-block list of LMM tests polished in FP64 (|t| > 8), and
-// how many are polished per launch (g~ scratch: chunk x n doubles).
-constexpr int64_t kPolishCap = int64_t(1) << 20;
+// LMM / GLM tests with S < kappa gg (covariates explaining more than 99% of
+// the SNP's weighted sum of squares) are polished in FP64: the contraction's
+// relative error in S grows as gg / S.
+constexpr double kPolishKappa = 1e-2;
+// Initial capacity of the per-block list of tests polished in FP64 (grown on
+// demand), and how many are polished per launch (g~ scratch: chunk x n doubles).
+constexpr int64_t kPolishCap = int64_t(1) << 16;
 constexpr int64_t kPolishChunk = 4096;
 // cis work items span at most this many 128-SNP tiles
 constexpr int kCisItemTiles = 4;
@@ -145,7 +149,9 @@ struct eqtl_gpu_ctx {
   DBuf Yt64, Xt64, X64, Y64, delta, gscal, gvalid, Bf32, Bbf, Wx, zeros;
   DBuf U64, V64;  // per-gene FP64 residual columns (model space / sample space)
   DBuf Lc;        // per-gene Cholesky factor of X~^T D X~ (LMM)
-  DBuf polish, polish_gt, polish_out;
+  DBuf polish, polish_gt, polish_out, pbest_key;
+  int64_t polish_cap_dev = 0;
+  bool have_wx = false;  // Wx = orthonormal basis of the covariates (LM and LMM scans residualise on it)
   RemlWork rw;
   bool have_windows = false;
   DBuf gene_chrom, gene_cs, gene_ce;
@@ -165,6 +171,9 @@ struct eqtl_gpu_ctx {
   DBuf state, partial, hits, hit_count, tail;
   double tail_df = -1;
   int64_t hit_cap_dev = 0;
+  // every hit of the last scanned block, sorted by (snp, gene); the caller
+  // drains what did not fit its buffer with eqtl_gpu_scan_hits
+  std::vector<eqtl_hit> hit_stage;
   // per-block buffers
   int64_t m_cap = 0;
   DBuf rows, A32, A64, Gt, keep, af, gtg, dosage;
@@ -420,6 +429,7 @@ int eqtl_gpu_set_samples(eqtl_gpu_ctx* c, int32_t n_file, const int32_t* sample_
   if (c->n != n) {
     c->have_basis = false;
     c->prepped_model = -1;
+    c->have_wx = false;
   }
   c->n_file = n_file;
   c->n = n;
@@ -458,6 +468,7 @@ int eqtl_gpu_set_test_subset(eqtl_gpu_ctx* c, const int32_t* keep, int32_t nk) {
   if (c->n != n_new || c->subset || !identity) {
     c->have_basis = false;
     c->prepped_model = -1;
+    c->have_wx = false;
   }
   c->n = n_new;
   c->n_pad = static_cast<int>(round_up(n_new, 64));
@@ -941,8 +952,35 @@ int eqtl_gpu_set_basis(eqtl_gpu_ctx* c, const double* Q, const double* lam, int3
 }
 
 // ---- gene preparation -------------------------------------------------------
+// Orthonormal basis Wx (n x p) of the covariates in c->X64 (prep_lm's
+// (X^T X)^-1, model_lm.cpp:24-25, as Wx = X Lx^-T with Lx Lx^T = X^T X). The
+// LM and LMM scans residualise each SNP on it before the contraction.
+static int covariate_basis(eqtl_gpu_ctx* c) {
+  const int n = c->n, p = c->p;
+  c->have_wx = false;
+  CHK(ensure(c->Wx, sizeof(double) * static_cast<size_t>(n) * p));
+  CHK(ensure(c->zeros, sizeof(double) * n));
+  HIPC(hipMemsetAsync(c->zeros.p, 0, sizeof(double) * n, c->stream));
+  DBuf okb;
+  CHK(ensure(okb, sizeof(int)));
+  hipLaunchKernelGGL(k_lm_basis, dim3(1), dim3(256), 0, c->stream, c->X64.as<double>(), c->zeros.as<double>(), n, p,
+                     c->Wx.as<double>(), okb.as<int>());
+  CHK(check_launch("k_lm_basis"));
+  int ok = 0;
+  HIPC(hipMemcpyAsync(&ok, okb.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  if (!ok) return fail(EQTL_E_ARG, "covariate matrix X^T X is not positive definite");
+  c->have_wx = true;
+  return EQTL_OK;
+}
+
 static int alloc_gene_side(eqtl_gpu_ctx* c, int G, int nq, int nq_pad) {
   if (nq_pad < nq) return fail(EQTL_E_ARG, "too many covariates for the Wald kernel");
+  if (c->have_windows && static_cast<int>(c->h_gene_chrom.size()) != G) {
+    // windows belong to a gene set of another size: drop them
+    c->have_windows = false;
+    c->h_gene_chrom.clear();
+  }
   c->G = G;
   c->G_pad = static_cast<int>(round_up(G, 128));  // multiple of 8 groups of 16
   c->nq = nq;
@@ -1019,6 +1057,7 @@ int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double*
   CHK(ensure(c->Lc, sizeof(double) * static_cast<size_t>(p) * p * G));
   HIPC(hipMemcpyAsync(c->Y64.p, Y, sizeof(double) * static_cast<size_t>(n) * G, hipMemcpyHostToDevice, c->stream));
   HIPC(hipMemcpyAsync(c->X64.p, X, sizeof(double) * static_cast<size_t>(n) * p, hipMemcpyHostToDevice, c->stream));
+  CHK(covariate_basis(c));
   // K4: Y~ = Q^T Y, X~ = Q^T X once for all genes (FP64 library GEMM)
   const double one = 1.0, zero = 0.0;
   {
@@ -1061,21 +1100,10 @@ int eqtl_gpu_lm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double* 
   CHK(alloc_gene_side(c, G, 1, 1));
   CHK(ensure(c->Y64, sizeof(double) * static_cast<size_t>(n) * G));
   CHK(ensure(c->X64, sizeof(double) * static_cast<size_t>(n) * p));
-  CHK(ensure(c->Wx, sizeof(double) * static_cast<size_t>(n) * p));
-  CHK(ensure(c->zeros, sizeof(double) * n));
   CHK(ensure(c->U64, sizeof(double) * static_cast<size_t>(n) * G));
-  HIPC(hipMemsetAsync(c->zeros.p, 0, sizeof(double) * n, c->stream));
   HIPC(hipMemcpyAsync(c->Y64.p, Y, sizeof(double) * static_cast<size_t>(n) * G, hipMemcpyHostToDevice, c->stream));
   HIPC(hipMemcpyAsync(c->X64.p, X, sizeof(double) * static_cast<size_t>(n) * p, hipMemcpyHostToDevice, c->stream));
-  DBuf okb;
-  CHK(ensure(okb, sizeof(int)));
-  hipLaunchKernelGGL(k_lm_basis, dim3(1), dim3(256), 0, c->stream, c->X64.as<double>(), c->zeros.as<double>(), n, p,
-                     c->Wx.as<double>(), okb.as<int>());
-  CHK(check_launch("k_lm_basis"));
-  int ok = 0;
-  HIPC(hipMemcpyAsync(&ok, okb.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPC(hipStreamSynchronize(c->stream));
-  if (!ok) return fail(EQTL_E_ARG, "covariate matrix X^T X is not positive definite");
+  CHK(covariate_basis(c));
   hipLaunchKernelGGL(k_lm_gene, dim3(G), dim3(256), 0, c->stream, c->Y64.as<double>(), c->Wx.as<double>(), n,
                      c->n_pad, p, G, gene_cols(c), c->gscal.as<double>(), c->gvalid.as<int>(),
                      c->U64.as<double>());
@@ -1192,7 +1220,7 @@ static int scan_begin_impl(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t
   if (c->prepped_model != model) return fail(EQTL_E_STATE, "gene side not prepared for this model");
   if (c->have_windows && c->m_annot <= 0) return fail(EQTL_E_STATE, "gene windows need the SNP annotation");
   if (scope == EQTL_SCOPE_CIS && !c->have_windows) return fail(EQTL_E_STATE, "cis scope needs gene windows");
-  if (scope == EQTL_SCOPE_CIS && !windows_ready && static_cast<int>(c->h_gene_chrom.size()) != c->G)
+  if (c->have_windows && !windows_ready && static_cast<int>(c->h_gene_chrom.size()) != c->G)
     return fail(EQTL_E_STATE, "gene windows were set for a different gene count");
   CHK(set_device(c));
   c->scan_model = model;
@@ -1232,6 +1260,7 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   if (c->have_windows && snp0 + m > c->m_annot) return fail(EQTL_E_ARG, "SNP block beyond the annotation");
   if (hit_cap > 0 && !hits) return fail(EQTL_E_ARG, "hits buffer is null");
   CHK(set_device(c));
+  c->hit_stage.clear();
   const bool lm = c->scan_model == EQTL_MODEL_LM;
   const bool glm = c->scan_model == EQTL_MODEL_GLM;
   CHK(ensure_block(c, m, bps, true));
@@ -1243,6 +1272,16 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   const int64_t m_pad = round_up(m, 128);
   const int np = c->n_pad;
   const float* Aop = c->A32.as<float>();
+  if (!glm) {
+    // g' = g_c - Wx Wx^T g_c (FP64) and gtg = ||g'||^2 before the rotation / contraction
+    if (!c->have_wx) return fail(EQTL_E_STATE, "covariate basis missing: prepare genes first");
+    Timed t(c, "residualize");
+    hipLaunchKernelGGL(k_residualize, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream,
+                       c->A64.as<double>(), c->A32.as<float>(), m_pad, c->n, np, c->Wx.as<double>(), c->p,
+                       c->gtg.as<double>());
+    CHK(check_launch("k_residualize"));
+    CHK(t.done());
+  }
   if (glm) {
     // the score test contracts the centred dosage itself (no eigenbasis)
   } else if (!lm) {
@@ -1265,10 +1304,6 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     CHK(check_launch("k_rotate"));
     CHK(t.done());
     Aop = c->Gt.as<float>();
-  } else {
-    hipLaunchKernelGGL(k_lm_gtg, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream,
-                       c->A64.as<double>(), m_pad, c->n, np, c->Wx.as<double>(), c->p, c->gtg.as<double>());
-    CHK(check_launch("k_lm_gtg"));
   }
   const int n_groups = c->G_pad / 16;
   // cis: work items {group, tile_lo, tile_hi} over the tiles of this block
@@ -1319,14 +1354,21 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   n_splits = std::max(1, std::min(n_tiles128, n_splits));
   CHK(ensure(c->partial, sizeof(GeneAcc) * (cis ? static_cast<size_t>(n_items) * 16
                                                 : static_cast<size_t>(n_splits) * c->G_pad)));
-  const int64_t cap_dev = std::max<int64_t>(hit_cap, 1);
-  if (c->hit_cap_dev < cap_dev) {
-    CHK(ensure(c->hits, sizeof(eqtl_hit) * static_cast<size_t>(cap_dev)));
-    c->hit_cap_dev = cap_dev;
-  }
-  HIPC(hipMemsetAsync(c->hit_count.p, 0, 2 * sizeof(unsigned long long), c->stream));
+  // device lists: hits (every hit of the block) and polish items; both grow
+  // on demand, the block's Wald launch is then repeated (its outputs are
+  // overwritten, not accumulated, until k_merge)
   const bool polish_on = c->scan_model != EQTL_MODEL_LM;
-  if (polish_on) CHK(ensure(c->polish, sizeof(PolishItem) * static_cast<size_t>(kPolishCap)));
+  if (c->hit_cap_dev < std::max<int64_t>(hit_cap, 1024)) {
+    const int64_t want = std::max<int64_t>(hit_cap, 1024);
+    c->hit_cap_dev = 0;
+    CHK(ensure(c->hits, sizeof(eqtl_hit) * static_cast<size_t>(want)));
+    c->hit_cap_dev = want;
+  }
+  if (polish_on && c->polish_cap_dev < kPolishCap) {
+    c->polish_cap_dev = 0;
+    CHK(ensure(c->polish, sizeof(PolishItem) * static_cast<size_t>(kPolishCap)));
+    c->polish_cap_dev = kPolishCap;
+  }
   WaldArgs a;
   a.A = Aop;
   a.gc = gene_cols(c);
@@ -1354,9 +1396,7 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   a.tail = c->tail.as<TTail>();
   a.pthr = c->pthr;
   a.partial = c->partial.as<GeneAcc>();
-  a.hits = c->hits.as<eqtl_hit>();
   a.hit_count = c->hit_count.as<unsigned long long>();
-  a.hit_cap = cap_dev;
   a.A64 = c->A64.as<double>();
   // FP64 residual vectors in sample space for the numerator refinement: LM
   // y_s, GLM the score residual r, LMM Q (s o r)
@@ -1364,14 +1404,49 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   a.n = c->n;
   a.tau = c->scan_prec == EQTL_PREC_FP32 ? kRefineTauF32 : kRefineTauBF16X3;
   a.tau_hi = glm ? kGlmTauHi : kRefineTauHi;
-  a.polish = polish_on ? c->polish.as<PolishItem>() : nullptr;
-  a.polish_cap = polish_on ? kPolishCap : 0;
+  a.kappa = kPolishKappa;
   a.cis = cis ? 1 : 0;
   a.work = cis ? c->work.as<int32_t>() : nullptr;
-  {
-    Timed t(c, "wald");
-    CHK(launch_wald(c, a, dim3(static_cast<unsigned>(cis ? n_items : n_groups_pad * n_splits))));
-    CHK(t.done());
+  int64_t got = 0, npol = 0;
+  for (int pass = 0;; ++pass) {
+    a.hits = c->hits.as<eqtl_hit>();
+    a.hit_cap = c->hit_cap_dev;
+    a.polish = polish_on ? c->polish.as<PolishItem>() : nullptr;
+    a.polish_cap = polish_on ? c->polish_cap_dev : 0;
+    HIPC(hipMemsetAsync(c->hit_count.p, 0, 2 * sizeof(unsigned long long), c->stream));
+    {
+      Timed t(c, "wald");
+      CHK(launch_wald(c, a, dim3(static_cast<unsigned>(cis ? n_items : n_groups_pad * n_splits))));
+      CHK(t.done());
+    }
+    unsigned long long cnt2[2] = {0, 0};
+    HIPC(hipMemcpyAsync(cnt2, c->hit_count.p, sizeof(cnt2), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    got = static_cast<int64_t>(cnt2[0]);
+    npol = static_cast<int64_t>(cnt2[1]);
+    // polished tests may append hits of their own: room for got + npol
+    const bool grow_hits = got + npol > c->hit_cap_dev;
+    const bool grow_pol = polish_on && npol > c->polish_cap_dev;
+    if (!grow_hits && !grow_pol) break;
+    if (pass > 0) return fail(EQTL_E_STATE, "hit / polish counts changed between identical launches");
+    if (grow_hits) {
+      const int64_t want = (got + npol) + (got + npol) / 4 + 1024;
+      int rc2 = ensure(c->hits, sizeof(eqtl_hit) * static_cast<size_t>(want));
+      c->hit_cap_dev = rc2 == EQTL_OK ? want : 0;
+      if (rc2 != EQTL_OK)
+        return fail(rc2, "the block's " + std::to_string(got + npol) +
+                             " hits do not fit in device memory: scan smaller SNP blocks");
+      c->hit_cap_dev = want;
+    }
+    if (grow_pol) {
+      const int64_t want = npol + npol / 4 + 1024;
+      int rc2 = ensure(c->polish, sizeof(PolishItem) * static_cast<size_t>(want));
+      c->polish_cap_dev = rc2 == EQTL_OK ? want : 0;
+      if (rc2 != EQTL_OK)
+        return fail(rc2, "the block's " + std::to_string(npol) +
+                             " FP64 polish items do not fit in device memory: scan smaller SNP blocks");
+      c->polish_cap_dev = want;
+    }
   }
   if (cis)
     hipLaunchKernelGGL(k_merge_items, dim3((c->G + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
@@ -1380,17 +1455,12 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     hipLaunchKernelGGL(k_merge, dim3((c->G + 255) / 256), dim3(256), 0, c->stream, c->state.as<GeneAcc>(),
                        c->partial.as<GeneAcc>(), c->G, c->G_pad, n_splits);
   CHK(check_launch("k_merge"));
-  unsigned long long cnt2[2] = {0, 0};
-  HIPC(hipMemcpyAsync(cnt2, c->hit_count.p, sizeof(cnt2), hipMemcpyDeviceToHost, c->stream));
-  HIPC(hipStreamSynchronize(c->stream));
-  const int64_t got = static_cast<int64_t>(cnt2[0]);
-  const int64_t npol = static_cast<int64_t>(cnt2[1]);
-  if (npol > kPolishCap) return fail(EQTL_E_CAPACITY, "more than 2^20 tests with |t| > 8 in one block");
   if (npol > 0) {
     Timed t(c, "polish");
     const int64_t chunk = std::min<int64_t>(npol, kPolishChunk);
     CHK(ensure(c->polish_gt, sizeof(double) * static_cast<size_t>(chunk) * c->n));
-    CHK(ensure(c->polish_out, sizeof(eqtl_hit) * static_cast<size_t>(chunk)));
+    CHK(ensure(c->polish_out, sizeof(eqtl_hit) * static_cast<size_t>(npol)));
+    CHK(ensure(c->pbest_key, 2 * sizeof(unsigned long long) * static_cast<size_t>(c->G)));
     PolishArgs pa;
     pa.A64 = c->A64.as<double>();
     pa.n_pad = np;
@@ -1410,12 +1480,15 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     pa.lbeta = std::lgamma(0.5 * a.df) + std::lgamma(0.5) - std::lgamma(0.5 * a.df + 0.5);
     pa.snp0 = snp0;
     pa.gt = c->polish_gt.as<double>();
+    pa.pthr = c->pthr;
     pa.hits = c->hits.as<eqtl_hit>();
-    pa.out = c->polish_out.as<eqtl_hit>();
+    pa.hit_count = c->hit_count.as<unsigned long long>();
+    pa.hit_cap = c->hit_cap_dev;
     pa.state = c->state.as<GeneAcc>();
     for (int64_t i0 = 0; i0 < npol; i0 += chunk) {
       pa.items = c->polish.as<PolishItem>() + i0;
       pa.count = std::min<int64_t>(chunk, npol - i0);
+      pa.out = c->polish_out.as<eqtl_hit>() + i0;
       if (glm)
         hipLaunchKernelGGL(k_polish<EQTL_MODEL_GLM>, dim3(static_cast<unsigned>(pa.count)), dim3(256), 0, c->stream,
                            pa);
@@ -1427,22 +1500,44 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
                            pa);
       }
       CHK(check_launch("k_polish"));
-      hipLaunchKernelGGL(k_polish_best, dim3(static_cast<unsigned>((pa.count + 255) / 256)), dim3(256), 0,
-                         c->stream, c->state.as<GeneAcc>(), pa.out, pa.count);
-      CHK(check_launch("k_polish_best"));
     }
+    // best over the kernel's best and every polished test (lexicographic in (p, SNP))
+    const eqtl_hit* po = c->polish_out.as<eqtl_hit>();
+    unsigned long long* key = c->pbest_key.as<unsigned long long>();
+    const dim3 gg((c->G + 255) / 256), gi(static_cast<unsigned>((npol + 255) / 256));
+    hipLaunchKernelGGL(k_pbest_init, gg, dim3(256), 0, c->stream, c->state.as<GeneAcc>(), key, c->G);
+    hipLaunchKernelGGL(k_pbest_p, gi, dim3(256), 0, c->stream, po, npol, key);
+    hipLaunchKernelGGL(k_pbest_state_snp, gg, dim3(256), 0, c->stream, c->state.as<GeneAcc>(), key, c->G);
+    hipLaunchKernelGGL(k_pbest_snp, gi, dim3(256), 0, c->stream, po, npol, key);
+    hipLaunchKernelGGL(k_pbest_write, gi, dim3(256), 0, c->stream, po, npol, key, c->state.as<GeneAcc>());
+    CHK(check_launch("k_pbest"));
+    unsigned long long cnt = 0;
+    HIPC(hipMemcpyAsync(&cnt, c->hit_count.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    got = static_cast<int64_t>(cnt);
     CHK(t.done());
   }
-  if (n_hits) *n_hits = got;
-  const int64_t take = std::min<int64_t>(got, hit_cap);
-  if (take > 0) {
-    HIPC(hipMemcpyAsync(hits, c->hits.p, sizeof(eqtl_hit) * take, hipMemcpyDeviceToHost, c->stream));
+  // every hit of the block, sorted (SNP-major, gene order within a SNP)
+  c->hit_stage.resize(static_cast<size_t>(got));
+  if (got > 0) {
+    HIPC(hipMemcpyAsync(c->hit_stage.data(), c->hits.p, sizeof(eqtl_hit) * got, hipMemcpyDeviceToHost, c->stream));
     HIPC(hipStreamSynchronize(c->stream));
-    std::sort(hits, hits + take, [](const eqtl_hit& x, const eqtl_hit& y) {
+    std::sort(c->hit_stage.begin(), c->hit_stage.end(), [](const eqtl_hit& x, const eqtl_hit& y) {
       return x.snp != y.snp ? x.snp < y.snp : x.gene < y.gene;
     });
   }
-  if (got > hit_cap) return fail(EQTL_E_CAPACITY, "hit buffer too small for this block");
+  if (n_hits) *n_hits = got;
+  const int64_t take = std::min<int64_t>(got, std::max<int64_t>(hit_cap, 0));
+  if (take > 0) std::memcpy(hits, c->hit_stage.data(), sizeof(eqtl_hit) * take);
+  return EQTL_OK;
+}
+
+int eqtl_gpu_scan_hits(eqtl_gpu_ctx* c, int64_t first, eqtl_hit* hits, int64_t cap, int64_t* n_out) {
+  if (!c || first < 0 || cap < 0 || (cap > 0 && !hits) || !n_out) return fail(EQTL_E_ARG, "bad scan_hits arguments");
+  const int64_t total = static_cast<int64_t>(c->hit_stage.size());
+  const int64_t take = first >= total ? 0 : std::min(cap, total - first);
+  if (take > 0) std::memcpy(hits, c->hit_stage.data() + first, sizeof(eqtl_hit) * take);
+  *n_out = take;
   return EQTL_OK;
 }
 

@@ -334,7 +334,7 @@ __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A,
 struct PolishItem {
   int32_t j;     // block-local SNP row
   int32_t gene;
-  int64_t hit;   // index in the block's hit list, -1 if not a hit
+  int64_t pad;
 };
 
 struct WaldArgs {
@@ -363,7 +363,10 @@ struct WaldArgs {
   const double* vref;         // G x n FP64 residual vectors in sample space
   int32_t n;
   double tau, tau_hi;
-  // LMM tests with |t| > tau_hi, recomputed entirely in FP64 afterwards
+  // LMM / GLM tests recomputed entirely in FP64 afterwards: |t| > tau_hi, or
+  // S < kappa gg (the covariates explain more than 1 - kappa of the SNP's
+  // weighted sum of squares, so S = gg - ||W^T h||^2 cancels)
+  double kappa;
   PolishItem* polish;
   int64_t polish_cap;
   // scope: 0 trans (a gene's cis window is excluded), 1 cis (only the window
@@ -511,16 +514,21 @@ __device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, i
   }
   if (!isfinite(p)) return;
   ++st.n_tested;
-  // LMM / GLM tests with |t| > tau_hi are polished in FP64 after the kernel,
-  // which adds their ACAT term (an accepted p stays accepted: both are < 1e-15)
-  const bool polish = MODEL != EQTL_MODEL_LM && fabs(stat) > a.tau_hi;
+  // LMM / GLM tests with |t| > tau_hi or a cancelling S are polished in FP64
+  // after the kernel, which then does their ACAT term, n_sig, hit and best
+  // (k_polish, k_pbest_*); here they only count as tested
+  const bool polish = MODEL != EQTL_MODEL_LM && (fabs(stat) > a.tau_hi || (S < a.kappa * gg && !(S <= 1e-15)));
+  if (polish) {
+    const unsigned long long pi = atomicAdd(a.hit_count + 1, 1ull);
+    if (static_cast<int64_t>(pi) < a.polish_cap) a.polish[pi] = PolishItem{static_cast<int32_t>(j), gene, -1};
+    return;
+  }
   double term;
   if (acat_accept(p, &term)) {
-    if (!polish) asum = dd_add_d(asum, term);
+    asum = dd_add_d(asum, term);
     ++st.used;
   }
   const double afj = ri.af[jl];
-  int64_t hidx = -1;
   if (p <= a.pthr) {
     ++st.n_sig;
     const unsigned long long idx = atomicAdd(a.hit_count, 1ull);
@@ -529,16 +537,11 @@ __device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, i
       h.snp = snp; h.gene = gene; h.pad = 0;
       h.af = afj; h.beta = beta; h.se = se; h.stat = stat; h.p = p; h.r2 = r2;
       a.hits[idx] = h;
-      hidx = static_cast<int64_t>(idx);
     }
   }
   if (p < st.best_p || (p == st.best_p && snp < st.best_snp)) {
     st.best_p = p; st.best_snp = snp; st.best_af = afj; st.best_beta = beta;
     st.best_se = se; st.best_stat = stat; st.best_r2 = r2;
-  }
-  if (polish) {
-    const unsigned long long pi = atomicAdd(a.hit_count + 1, 1ull);
-    if (static_cast<int64_t>(pi) < a.polish_cap) a.polish[pi] = PolishItem{static_cast<int32_t>(j), gene, hidx};
   }
 }
 
@@ -777,15 +780,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
 }
 
 // ---------------------------------------------------------------------------
-// FP64 polish of LMM / GLM tests with |t| > tau_hi (strong signals, p far below any
-// reporting threshold, where the contraction's relative error in S is
-// amplified by |t| and by 1/(1 - r^2)): test_lmm_gtil recomputed from the
-// FP64 centred dosage, one workgroup per test:
+// FP64 polish of LMM / GLM tests with |t| > tau_hi (strong signals, p far
+// below any reporting threshold, where the contraction's relative error in S
+// is amplified by |t|) or with S < kappa gg (covariates that explain most of
+// the SNP, where S = gg - |W^T h|^2 cancels): test_lmm_gtil recomputed from
+// the FP64 dosage, one workgroup per test:
 //   g~ = Q^T g_c,  gg = sum d g~^2,  num = (s o r)^T g~,  a = X~^T (d o g~),
 //   W^T h = Lc^-1 a (Lc Lc^T = X~^T D X~),  S = gg - |Lc^-1 a|^2,
-// then beta/se/stat/p (continued-fraction tail) and r^2 as in the kernel. The
-// hit record (if any) is overwritten; k_polish_best then updates the gene's
-// best entry if it is this SNP.
+// then beta/se/stat/p (continued-fraction tail) and r^2 as in the kernel.
+// The Wald kernel only counted these tests as tested; here each one adds its
+// ACAT term, its n_sig count and its hit record (appended to the block's
+// list), and the k_pbest_* kernels merge the polished p into the gene's best.
 // ---------------------------------------------------------------------------
 struct PolishArgs {
   const PolishItem* items;
@@ -805,7 +810,10 @@ struct PolishArgs {
   double df, lbeta;
   int64_t snp0;
   double* gt;           // count x n scratch
-  eqtl_hit* hits;
+  double pthr;
+  eqtl_hit* hits;       // the block's hit list, appended to at hit_count[0]
+  unsigned long long* hit_count;
+  int64_t hit_cap;
   eqtl_hit* out;        // count
   GeneAcc* state;       // running per-gene state (ACAT term of the polished p)
 };
@@ -923,19 +931,52 @@ __global__ __launch_bounds__(256) void k_polish(PolishArgs a) {
   h.snp = a.snp0 + it.j; h.gene = k; h.pad = 0;
   h.af = a.af[it.j]; h.beta = beta; h.se = se; h.stat = stat; h.p = pv; h.r2 = r2;
   a.out[blockIdx.x] = h;
-  if (it.hit >= 0) a.hits[it.hit] = h;
   double term;
-  if (acat_accept(pv, &term)) atomicAdd(&a.state[k].acat_pol, term);
+  if (acat_accept(pv, &term)) {
+    atomicAdd(&a.state[k].acat_pol, term);
+    atomicAdd(&a.state[k].used, 1);
+  }
+  if (pv <= a.pthr) {
+    atomicAdd(&a.state[k].n_sig, 1);
+    const unsigned long long idx = atomicAdd(a.hit_count, 1ull);
+    if (static_cast<int64_t>(idx) < a.hit_cap) a.hits[idx] = h;
+  }
 }
 
-__global__ void k_polish_best(GeneAcc* state, const eqtl_hit* out, int64_t count) {
+// Best entry over the kernel's best and the polished tests, lexicographic in
+// (p, SNP index): key[2g] = min p bits (non-negative doubles order as their
+// bit patterns), key[2g + 1] = smallest SNP at that p; the winning polished
+// test (if any) writes the gene's best fields.
+__device__ inline unsigned long long pbits(double p) { return static_cast<unsigned long long>(__double_as_longlong(p)); }
+__global__ void k_pbest_init(const GeneAcc* state, unsigned long long* key, int G) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  key[2 * g] = pbits(state[g].best_p);
+  key[2 * g + 1] = state[g].best_snp >= 0 ? static_cast<unsigned long long>(state[g].best_snp) : ~0ull;
+}
+__global__ void k_pbest_p(const eqtl_hit* out, int64_t count, unsigned long long* key) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < count) atomicMin(&key[2 * out[i].gene], pbits(out[i].p));
+}
+__global__ void k_pbest_state_snp(const GeneAcc* state, unsigned long long* key, int G) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  if (key[2 * g] != pbits(state[g].best_p)) key[2 * g + 1] = ~0ull;  // the kernel's best lost on p
+}
+__global__ void k_pbest_snp(const eqtl_hit* out, int64_t count, unsigned long long* key) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const eqtl_hit h = out[i];
+  if (key[2 * h.gene] == pbits(h.p)) atomicMin(&key[2 * h.gene + 1], static_cast<unsigned long long>(h.snp));
+}
+__global__ void k_pbest_write(const eqtl_hit* out, int64_t count, const unsigned long long* key, GeneAcc* state) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const eqtl_hit h = out[i];
+  if (key[2 * h.gene] != pbits(h.p) || key[2 * h.gene + 1] != static_cast<unsigned long long>(h.snp)) return;
   GeneAcc& s = state[h.gene];
-  if (s.best_snp == h.snp) {
-    s.best_p = h.p; s.best_beta = h.beta; s.best_se = h.se; s.best_stat = h.stat; s.best_r2 = h.r2;
-  }
+  s.best_p = h.p; s.best_snp = h.snp; s.best_af = h.af; s.best_beta = h.beta;
+  s.best_se = h.se; s.best_stat = h.stat; s.best_r2 = h.r2;
 }
 
 // ---------------------------------------------------------------------------
@@ -1035,26 +1076,58 @@ __global__ void k_finalize(const GeneAcc* state, eqtl_gene_state* out, int G) {
   out[g] = o;
 }
 
-// LM: per-SNP gtg = ||g_c||^2 - ||Wx^T g_c||^2 in FP64 (residualize_g,
-// scan_trans_lm.cpp:11-19, with an orthonormal basis Wx of the covariates).
-__global__ __launch_bounds__(256) void k_lm_gtg(const double* A64, int64_t m_pad, int n, int n_pad,
-                                                const double* Wx, int p, double* gtg) {
+// Covariate residualisation of the decoded block (LM and LMM scans), FP64,
+// one wave per SNP row: g' = g_c - Wx (Wx^T g_c), written over the FP64 and
+// fp32 rows, and gtg = ||g'||^2 (residualize_g, scan_trans_lm.cpp:11-19, with
+// an orthonormal basis Wx of the covariates; LM uses gtg as g_s^T g_s). Both
+// Wald statistics are invariant to adding covariate combinations to g, so
+// the tests are unchanged; but the contraction's rounding then scales with
+// ||g'|| instead of ||g||, so a SNP that the covariates explain almost fully
+// (a conditioning SNP in LD, genotype PCs) keeps its relative accuracy in S
+// and in the numerator.
+__global__ __launch_bounds__(256) void k_residualize(double* A64, float* A32, int64_t m_pad, int n, int n_pad,
+                                                     const double* Wx, int p, double* gtg) {
+  constexpr int kPCov = 32;  // covariate columns at most (intercept included)
+  constexpr int kCh = 8;     // columns per pass (independent reductions in flight)
   const int lane = threadIdx.x & 63;
   const int64_t j = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (j >= m_pad) return;
-  const double* g = A64 + j * n_pad;
-  double ss = 0;
-  for (int i = lane; i < n; i += kWave) ss += g[i] * g[i];
-  for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
-  double proj = 0;
-  for (int c = 0; c < p; ++c) {
-    double s = 0;
-    const double* w = Wx + static_cast<size_t>(c) * n;
-    for (int i = lane; i < n; i += kWave) s += w[i] * g[i];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    proj += s * s;
+  if (j >= m_pad || p > kPCov) return;
+  double* g = A64 + j * n_pad;
+  float* g32 = A32 + j * n_pad;
+  double cf[kPCov];  // indexed with compile-time indices only (kept in registers)
+#pragma unroll
+  for (int c0 = 0; c0 < kPCov; c0 += kCh) {
+    double s[kCh];
+#pragma unroll
+    for (int u = 0; u < kCh; ++u) s[u] = 0.0;
+    if (c0 < p) {
+      for (int i = lane; i < n; i += 64) {
+        const double gi = g[i];
+#pragma unroll
+        for (int u = 0; u < kCh; ++u)
+          if (c0 + u < p) s[u] = fma(Wx[static_cast<size_t>(c0 + u) * n + i], gi, s[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kCh; ++u)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o);
+    }
+#pragma unroll
+    for (int u = 0; u < kCh; ++u) cf[c0 + u] = s[u];
   }
-  if (lane == 0) gtg[j] = ss - proj;
+  double ss = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    double v = g[i];
+#pragma unroll
+    for (int c = 0; c < kPCov; ++c)
+      if (c < p) v = fma(-Wx[static_cast<size_t>(c) * n + i], cf[c], v);
+    g[i] = v;
+    g32[i] = static_cast<float>(v);
+    ss = fma(v, v, ss);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
+  if (lane == 0 && gtg) gtg[j] = ss;
 }
 
 // ---------------------------------------------------------------------------

@@ -698,6 +698,22 @@ void Engine::check(int rc, const char* what) const {
   if (rc != EQTL_OK) die(std::string(what) + ": " + eqtl_gpu_last_error());
 }
 
+void Engine::scan_block(const uint8_t* rows, int32_t on_dev, int64_t snp0, int64_t m, int32_t bps,
+                        std::vector<eqtl_hit>& hits) const {
+  if (hits.capacity() < (1u << 16)) hits.reserve(1u << 16);
+  hits.resize(hits.capacity());
+  int64_t total = 0;
+  check(eqtl_gpu_scan_block(ctx_, rows, on_dev, snp0, m, bps, hits.data(), static_cast<int64_t>(hits.size()), &total),
+        "scan_block");
+  const int64_t first = std::min<int64_t>(total, static_cast<int64_t>(hits.size()));
+  hits.resize(static_cast<size_t>(total));
+  if (total > first) {
+    int64_t got = 0;
+    check(eqtl_gpu_scan_hits(ctx_, first, hits.data() + first, total - first, &got), "scan_hits");
+    if (got != total - first) die("scan_hits returned fewer hits than the block holds");
+  }
+}
+
 // set_sample_order (plink_bed.cpp:245-255, vcf_session.cpp:127-138) on the
 // engine, with the source's row format.
 static void set_sample_order(Engine& eng, const GenoSource& geno, const std::vector<std::string>& ids) {
@@ -897,7 +913,7 @@ void scan_snp_outer(const Options& opt, Engine& eng, GenoSource& bed, Model mode
   std::vector<std::vector<std::pair<double, int64_t>>> cand(stage2 ? G : 0);
   std::vector<int> n_sig(G, 0);
   eng.check(eqtl_gpu_scan_begin(c, engine_model(model), opt.precision, EQTL_SCOPE_TRANS, pthr_dev), "scan_begin");
-  std::vector<eqtl_hit> hits(1 << 20);
+  std::vector<eqtl_hit> hits;
   const auto& sites = bed.sites();
   auto fill = [&](AssocHit& h, const eqtl_hit& e) {
     const TransJob& job = jobs[e.gene];
@@ -929,12 +945,8 @@ void scan_snp_outer(const Options& opt, Engine& eng, GenoSource& bed, Model mode
     const size_t m = std::min(kBlockSnps, sites.size() - j0);
     int32_t on_dev = 0;
     const uint8_t* rows = bed.rows(eng, j0, m, &on_dev);
-    int64_t nh = 0;
-    int rc = eqtl_gpu_scan_block(c, rows, on_dev, static_cast<int64_t>(j0), static_cast<int64_t>(m),
-                                 bed.bytes_per_snp(), hits.data(), static_cast<int64_t>(hits.size()), &nh);
-    if (rc == EQTL_E_CAPACITY)
-      die("more than " + std::to_string(hits.size()) + " significant pairs in one SNP block; lower --pval-trans");
-    eng.check(rc, "scan_block");
+    eng.scan_block(rows, on_dev, static_cast<int64_t>(j0), static_cast<int64_t>(m), bed.bytes_per_snp(), hits);
+    const int64_t nh = static_cast<int64_t>(hits.size());
     for (int64_t i = 0; i < nh; ++i) {  // SNP-major, gene order (scan_trans_lmm.cpp:185-188)
       const eqtl_hit& e = hits[i];
       if (stage2 && e.p < opt.perm_trans_thr) cand[e.gene].emplace_back(e.p, e.snp);
@@ -1068,18 +1080,13 @@ void scan_cis(const Options& opt, Engine& eng, GenoSource& bed, Model model, Sco
   prep_genes(opt, eng, model, Y, G, cov, &phi, &conv);
   eng.check(eqtl_gpu_set_gene_windows(c, gch.data(), gcs.data(), gce.data(), G), "set_gene_windows");
   eng.check(eqtl_gpu_scan_begin(c, gm, opt.precision, EQTL_SCOPE_CIS, pthr), "scan_begin");
-  std::vector<eqtl_hit> hits(1 << 20), all_hits;
+  std::vector<eqtl_hit> hits, all_hits;
   for (int64_t j0 = std::max<int64_t>(lo, 0); j0 < hi; j0 += static_cast<int64_t>(kBlockSnps)) {
     const size_t m = std::min<size_t>(kBlockSnps, static_cast<size_t>(hi - j0));
     int32_t on_dev = 0;
     const uint8_t* rows = bed.rows(eng, static_cast<size_t>(j0), m, &on_dev);
-    int64_t nh = 0;
-    int rc = eqtl_gpu_scan_block(c, rows, on_dev, j0, static_cast<int64_t>(m), bed.bytes_per_snp(), hits.data(),
-                                 static_cast<int64_t>(hits.size()), &nh);
-    if (rc == EQTL_E_CAPACITY)
-      die("more than " + std::to_string(hits.size()) + " significant pairs in one SNP block; lower --pval-cis");
-    eng.check(rc, "scan_block");
-    all_hits.insert(all_hits.end(), hits.begin(), hits.begin() + nh);
+    eng.scan_block(rows, on_dev, j0, static_cast<int64_t>(m), bed.bytes_per_snp(), hits);
+    all_hits.insert(all_hits.end(), hits.begin(), hits.end());
   }
   std::vector<eqtl_gene_state> st(G);
   eng.check(eqtl_gpu_scan_end(c, st.data()), "scan_end");

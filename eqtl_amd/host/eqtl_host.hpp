@@ -237,6 +237,10 @@ class Engine {
   ~Engine();
   eqtl_gpu_ctx* ctx() const { return ctx_; }
   void check(int rc, const char* what) const;
+  // eqtl_gpu_scan_block, then every hit of the block into hits (resized to
+  // the block's count; its capacity is reused across blocks)
+  void scan_block(const uint8_t* rows, int32_t on_dev, int64_t snp0, int64_t m, int32_t bps,
+                  std::vector<eqtl_hit>& hits) const;
 
  private:
   eqtl_gpu_ctx* ctx_ = nullptr;

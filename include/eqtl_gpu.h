@@ -33,7 +33,7 @@ enum {
   EQTL_E_NOMEM = -3,    /* device allocation failed */
   EQTL_E_STATE = -4,    /* call order (e.g. scan before prep) */
   EQTL_E_SOLVER = -5,   /* eigensolver failure (make_lmm_basis dies) */
-  EQTL_E_CAPACITY = -6, /* hit buffer too small: *n_hits holds the count */
+  EQTL_E_CAPACITY = -6, /* a fixed-size device buffer is too small (the message names it) */
   EQTL_E_FORMAT = -7    /* malformed input record (the message names it) */
 };
 
@@ -200,13 +200,18 @@ int eqtl_gpu_set_gene_windows(eqtl_gpu_ctx* ctx, const int32_t* chrom, const int
 /* ---- SNP-block scans (scan_lm_snp_outer / scan_lmm_snp_outer; the cis scope
  * runs the same tests per gene window, scan_gene_snps without permutations) --- */
 int eqtl_gpu_scan_begin(eqtl_gpu_ctx* ctx, int32_t model, int32_t precision, int32_t scope, double pthr);
-/* One block of m SNPs with global indices snp0 .. snp0+m-1 (bim order). Hits
- * for this block are appended to hits (capacity hit_cap) in (snp, gene)
- * order; *n_hits is the block's count (EQTL_E_CAPACITY if it exceeds hit_cap;
- * the gene states are still updated). */
+/* One block of m SNPs with global indices snp0 .. snp0+m-1 (bim order). Every
+ * pair with p <= pthr is kept (the reference writes each one,
+ * scan_trans_lmm.cpp:185-188, scan_trans_lm.cpp:115-118): *n_hits is the
+ * block's total, the first min(total, hit_cap) hits in (snp, gene) order are
+ * copied to hits, and eqtl_gpu_scan_hits returns the rest. The device lists
+ * grow as the block needs (EQTL_E_NOMEM only when device memory runs out). */
 int eqtl_gpu_scan_block(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_device,
                         int64_t snp0, int64_t m, int32_t bytes_per_snp, eqtl_hit* hits,
                         int64_t hit_cap, int64_t* n_hits);
+/* Hits first .. first+cap-1 (same order) of the last scanned block; *n_out =
+ * how many were copied (0 past the end). Valid until the next scan_block. */
+int eqtl_gpu_scan_hits(eqtl_gpu_ctx* ctx, int64_t first, eqtl_hit* hits, int64_t cap, int64_t* n_out);
 /* Final per-gene summaries (G entries). */
 int eqtl_gpu_scan_end(eqtl_gpu_ctx* ctx, eqtl_gene_state* state);
 

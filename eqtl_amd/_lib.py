@@ -40,7 +40,7 @@ EXPORTS = [
     "eqtl_gpu_set_timing", "eqtl_gpu_get_timing", "eqtl_gpu_synth_bed", "eqtl_gpu_copy_rows",
     "eqtl_gpu_synchronize", "eqtl_gpu_selftest", "eqtl_gpu_upload_bed", "eqtl_gpu_cis_perm",
     "eqtl_gpu_beta_approx", "eqtl_gpu_set_geno_format", "eqtl_gpu_vcf_decode", "eqtl_gpu_glm_prep",
-    "eqtl_gpu_set_test_subset", "eqtl_gpu_scan_hits",
+    "eqtl_gpu_set_test_subset", "eqtl_gpu_scan_hits", "eqtl_gpu_grm_accumulate_noalloc",
 ]
 
 _lib = None
@@ -176,15 +176,25 @@ class Context:
         return dos, keep.astype(bool), af
 
     # ---- GRM ----
-    def grm(self, rows, block=4096):
+    def grm(self, rows, block=4096, noalloc=False):
+        """GRM of host bed rows (m, bps) in blocks; noalloc uses the entry point
+        that works in the library's fixed device scratch (no allocation)."""
         rows = np.ascontiguousarray(rows, dtype=np.uint8)
         m, bps = rows.shape
         _check(lib().eqtl_gpu_grm_begin(self._c))
+        fn = lib().eqtl_gpu_grm_accumulate_noalloc if noalloc else lib().eqtl_gpu_grm_accumulate
         for j0 in range(0, m, block):
             blk = np.ascontiguousarray(rows[j0:j0 + block])
-            _check(lib().eqtl_gpu_grm_accumulate(self._c, _p(blk), ctypes.c_int32(0), ctypes.c_int64(blk.shape[0]),
-                                                 ctypes.c_int32(bps)))
+            _check(fn(self._c, _p(blk), ctypes.c_int32(0), ctypes.c_int64(blk.shape[0]), ctypes.c_int32(bps)))
         return self.grm_finish()
+
+    def grm_accumulate_noalloc_dev(self, rows_dev, m, bps):
+        """eqtl_gpu_grm_accumulate_noalloc on device rows (after grm_begin)."""
+        _check(lib().eqtl_gpu_grm_accumulate_noalloc(self._c, ctypes.c_void_p(rows_dev), ctypes.c_int32(1),
+                                                     ctypes.c_int64(m), ctypes.c_int32(bps)))
+
+    def grm_begin(self):
+        _check(lib().eqtl_gpu_grm_begin(self._c))
 
     def grm_finish(self):
         n = self.n

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -28,7 +29,30 @@
 
 using namespace eqtl_amd;
 
+// Fixed device scratch of the entry points that allocate nothing
+// (eqtl_gpu_grm_accumulate_noalloc): each device holds its own copy of this
+// array for the life of the process. Host code never uses its name as an
+// address: eqtl_gpu_open maps it per device with hipGetSymbolAddress and the
+// kernels receive that pointer. Calls that overlap on one device share it
+// and race.
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
+
+// Device address of `scratch` per device (mapped once per device).
+struct ScratchMap {
+  std::mutex mu;
+  std::map<int, void*> ptr;
+};
+ScratchMap& scratch_map() {
+  static ScratchMap m;
+  return m;
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int count, unsigned long long v) {
+  for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = v;
+}
 
 // |t| below which a test's numerator is recomputed in FP64 (near-null
 // refinement in the Wald kernels). Chosen from each form's absolute t error
@@ -152,6 +176,7 @@ struct eqtl_gpu_ctx {
   DBuf polish, polish_gt, polish_out, pbest_key;
   int64_t polish_cap_dev = 0;
   bool have_wx = false;  // Wx = orthonormal basis of the covariates (LM and LMM scans residualise on it)
+  void* scratch = nullptr;  // this device's address of the fixed scratch array
   RemlWork rw;
   bool have_windows = false;
   DBuf gene_chrom, gene_cs, gene_ce;
@@ -392,6 +417,22 @@ int eqtl_gpu_open(int32_t device, eqtl_gpu_ctx** out) {
     c->n_cu = prop.multiProcessorCount;
     c->name = prop.gcnArchName;
     c->hbm = static_cast<int64_t>(prop.totalGlobalMem);
+  }
+  {
+    // the fixed scratch array's address on this device (once per device)
+    ScratchMap& sm = scratch_map();
+    std::lock_guard<std::mutex> lk(sm.mu);
+    auto it = sm.ptr.find(device);
+    if (it == sm.ptr.end()) {
+      void* ptr = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !ptr) {
+        delete c;
+        return fail(EQTL_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      }
+      it = sm.ptr.emplace(device, ptr).first;
+    }
+    c->scratch = it->second;
   }
   *out = c;
   return EQTL_OK;
@@ -1765,30 +1806,25 @@ int eqtl_gpu_grm_begin(eqtl_gpu_ctx* c) {
   return EQTL_OK;
 }
 
-int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, int64_t m, int32_t bps) {
-  if (!c || !rows || m <= 0) return fail(EQTL_E_ARG, "bad grm_accumulate arguments");
-  if (!c->grm_acc.p) return fail(EQTL_E_STATE, "grm_begin first");
-  if (bps < min_bps(c)) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
-  CHK(set_device(c));
-  CHK(ensure_block(c, m, bps, false));
+}  // extern "C"
+
+namespace {
+
+// One block of the GRM accumulation over rows already on the device: the
+// block's standardised dosages z64 and missing flags mis (m_pad x n_pad each,
+// sample-major) and the two block counters `used` (SNPs used, SNPs with a
+// missing call; zeroed by the caller) are caller-provided scratch.
+int grm_accumulate_core(eqtl_gpu_ctx* c, const uint8_t* rd, int64_t m, int32_t bps, double* z64, __bf16* mis,
+                        unsigned long long* used) {
   const int64_t m_pad = round_up(m, 128);
   const int np = c->n_pad;
-  CHK(ensure(c->grm_z, static_cast<size_t>(m_pad) * np * (sizeof(double) + sizeof(__bf16))));
-  int rc;
-  const uint8_t* rd = stage_rows(c, rows, on_dev, m, bps, &rc);
-  if (rc) return rc;
-  double* z64 = c->grm_z.as<double>();
-  __bf16* mis = reinterpret_cast<__bf16*>(z64 + static_cast<size_t>(m_pad) * np);
-  DBuf used;
-  CHK(ensure(used, 2 * sizeof(unsigned long long)));
-  HIPC(hipMemsetAsync(used.p, 0, 2 * sizeof(unsigned long long), c->stream));
   auto* kstd = c->geno_fmt == EQTL_GENO_DOSE32 ? &k_grm_standardize<EQTL_GENO_DOSE32> : &k_grm_standardize<EQTL_GENO_BED>;
   hipLaunchKernelGGL(kstd, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream, rd, m, m_pad,
                      bps, c->n, np, c->n_file, c->sample_col.as<int32_t>(), z64, mis, c->grm_cnt.as<int32_t>(),
-                     used.as<unsigned long long>(), c->impute, c->max_miss, c->maf_min);
+                     used, c->impute, c->max_miss, c->maf_min);
   CHK(check_launch("k_grm_standardize"));
   unsigned long long uu[2] = {0, 0};
-  HIPC(hipMemcpyAsync(uu, used.p, sizeof(uu), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipMemcpyAsync(uu, used, sizeof(uu), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   {
     Timed t(c, "grm");
@@ -1820,6 +1856,60 @@ int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev
   HIPC(hipStreamSynchronize(c->stream));
   c->grm_m += static_cast<int64_t>(uu[0]);
   return EQTL_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, int64_t m, int32_t bps) {
+  if (!c || !rows || m <= 0) return fail(EQTL_E_ARG, "bad grm_accumulate arguments");
+  if (!c->grm_acc.p) return fail(EQTL_E_STATE, "grm_begin first");
+  if (bps < min_bps(c)) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
+  CHK(set_device(c));
+  CHK(ensure_block(c, m, bps, false));
+  const int64_t m_pad = round_up(m, 128);
+  const int np = c->n_pad;
+  CHK(ensure(c->grm_z, static_cast<size_t>(m_pad) * np * (sizeof(double) + sizeof(__bf16))));
+  int rc;
+  const uint8_t* rd = stage_rows(c, rows, on_dev, m, bps, &rc);
+  if (rc) return rc;
+  double* z64 = c->grm_z.as<double>();
+  __bf16* mis = reinterpret_cast<__bf16*>(z64 + static_cast<size_t>(m_pad) * np);
+  DBuf used;
+  CHK(ensure(used, 2 * sizeof(unsigned long long)));
+  HIPC(hipMemsetAsync(used.p, 0, 2 * sizeof(unsigned long long), c->stream));
+  return grm_accumulate_core(c, rd, m, bps, z64, mis, used.as<unsigned long long>());
+}
+
+int eqtl_gpu_grm_accumulate_noalloc(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, int64_t m, int32_t bps) {
+  if (!c || !rows || m <= 0) return fail(EQTL_E_ARG, "bad grm_accumulate_noalloc arguments");
+  if (!c->grm_acc.p) return fail(EQTL_E_STATE, "grm_begin first");
+  if (bps < min_bps(c)) return fail(EQTL_E_ARG, "bytes_per_snp too small for n_file");
+  if (!c->scratch) return fail(EQTL_E_STATE, "the device scratch array is not mapped for this device");
+  const int64_t m_pad = round_up(m, 128);
+  const size_t np = static_cast<size_t>(c->n_pad);
+  const size_t stage = on_dev ? 0 : static_cast<size_t>(round_up(m * static_cast<int64_t>(bps), 256));
+  const size_t need = 256 + stage + static_cast<size_t>(m_pad) * np * (sizeof(double) + sizeof(__bf16));
+  if (need > kScratchBytes)
+    return fail(EQTL_E_CAPACITY, "grm_accumulate_noalloc: " + std::to_string(m) + " SNPs x " +
+                                     std::to_string(c->n) + " samples need " + std::to_string(need) +
+                                     " bytes of scratch; the fixed scratch holds " + std::to_string(kScratchBytes) +
+                                     " (pass fewer SNPs per call)");
+  CHK(set_device(c));
+  unsigned char* sp = static_cast<unsigned char*>(c->scratch);
+  auto* used = reinterpret_cast<unsigned long long*>(sp);
+  const uint8_t* rd = rows;
+  if (!on_dev) {
+    HIPC(hipMemcpyAsync(sp + 256, rows, static_cast<size_t>(m) * bps, hipMemcpyHostToDevice, c->stream));
+    rd = sp + 256;
+  }
+  double* z64 = reinterpret_cast<double*>(sp + 256 + stage);
+  __bf16* mis = reinterpret_cast<__bf16*>(z64 + static_cast<size_t>(m_pad) * np);
+  // the counters keep the previous call's values: zero them (a kernel, not a memset node)
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, used, 2, 0ull);
+  CHK(check_launch("k_fill_u64"));
+  return grm_accumulate_core(c, rd, m, bps, z64, mis, used);
 }
 
 int eqtl_gpu_grm_finish(eqtl_gpu_ctx* c, double* K, int32_t* N_packed, int64_t* m_used) {

@@ -150,6 +150,19 @@ int eqtl_gpu_vcf_decode(eqtl_gpu_ctx* ctx, const char* text, int32_t text_on_dev
 int eqtl_gpu_grm_begin(eqtl_gpu_ctx* ctx);
 int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_device,
                             int64_t m, int32_t bytes_per_snp);
+/* eqtl_gpu_grm_accumulate without any device allocation, for callers that
+ * cannot allocate (tight serving loops, code under HIP graph capture of the
+ * caller's own work, harnesses that forbid hipMalloc). It has no counterpart
+ * in the reference. The block's standardised dosages, missing flags,
+ * counters and (for host rows) the staged rows live in a fixed device scratch
+ * array of 32 MiB that each device holds for the life of the process: a call
+ * needs 256 + (host rows: m * bytes_per_snp rounded up to 256) + m_pad * n_pad
+ * * 10 bytes (m_pad, n_pad = m, n rounded up to 128, 64), and a larger block
+ * is refused with EQTL_E_CAPACITY (at n = 2000, up to 1,536 SNPs per call).
+ * Same results as eqtl_gpu_grm_accumulate. All calls on one device share the
+ * array: calls that overlap on one device (any context) race. */
+int eqtl_gpu_grm_accumulate_noalloc(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_device,
+                                    int64_t m, int32_t bytes_per_snp);
 /* K n x n column-major FP64, N_packed n(n+1)/2 (GCTA .grm.N.bin order), m_used. */
 int eqtl_gpu_grm_finish(eqtl_gpu_ctx* ctx, double* K, int32_t* N_packed, int64_t* m_used);
 

@@ -202,6 +202,11 @@ struct eqtl_gpu_ctx {
   // per-block buffers
   int64_t m_cap = 0;
   DBuf rows, A32, A64, Gt, keep, af, gtg, dosage;
+  // exact FP64 rows for the permutation decisions near the observed statistic
+  DBuf ex_R, ex_keep, ex_gtg, ex_tasks, ex_out;
+  int64_t ex_lo = 0, ex_hi = 0;
+  bool ex_valid = false;
+  int64_t perm_exact = 0;  // permutations decided from FP64 tests in the last cis_perm
   // VCF decode
   DBuf vcf_text, vcf_recs, vcf_rows, vcf_bad;
   // GRM
@@ -1615,6 +1620,101 @@ int eqtl_gpu_upload_bed(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
 // Pseudo-genes (gene, b) per batch, and SNPs per scan block of a batch.
 constexpr int kPermBatch = 16384;
 constexpr int64_t kCisBlock = 32768;
+// A permutation whose split-precision T = -log10(max(min p, 1e-300)) lies
+// within kExactBand * (1 + T_obs) of the gene's observed T is decided again
+// from FP64 tests. The band is 20x (BF16X3) / 100x (FP32) the largest T error
+// of the contractions measured against FP64 (relative p error 2.1e-4 for the
+// LM and 8e-5 for the LMM split form, i.e. 9e-5 in T).
+constexpr double kExactBandBF16X3 = 2e-3;
+constexpr double kExactBandF32 = 5e-4;
+
+// FP64 rows of SNPs [lo, hi) of the resident genotypes (decode, covariate
+// residualisation, and for the LMM the rotation Q^T g' as an FP64 GEMM), with
+// their keep flags and (LM) ||g'||^2; kept while the range covers later asks.
+static int exact_rows(eqtl_gpu_ctx* c, int64_t lo, int64_t hi, bool lmm) {
+  if (c->ex_valid && c->ex_lo <= lo && hi <= c->ex_hi) return EQTL_OK;
+  c->ex_valid = false;
+  const int n = c->n, np = c->n_pad;
+  const int64_t M = hi - lo;
+  CHK(ensure(c->ex_R, sizeof(double) * static_cast<size_t>(M) * n));
+  CHK(ensure(c->ex_keep, static_cast<size_t>(M)));
+  CHK(ensure(c->ex_gtg, sizeof(double) * static_cast<size_t>(M)));
+  const int32_t bps = c->bed_bps;
+  for (int64_t j0 = lo; j0 < hi; j0 += kCisBlock) {
+    const int64_t mb = std::min<int64_t>(kCisBlock, hi - j0);
+    const int64_t m_pad = round_up(mb, 128);
+    CHK(ensure_block(c, mb, bps, true));
+    CHK(run_decode(c, c->bed_res.as<uint8_t>() + j0 * bps, mb, bps, true, nullptr));
+    hipLaunchKernelGGL(k_residualize, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream,
+                       c->A64.as<double>(), c->A32.as<float>(), m_pad, n, np, c->Wx.as<double>(), c->p,
+                       c->gtg.as<double>());
+    CHK(check_launch("k_residualize"));
+    double* dst = c->ex_R.as<double>() + static_cast<size_t>(j0 - lo) * n;
+    if (lmm) {
+      CHK(dgemm_tn(c, n, static_cast<int>(mb), n, c->Q64.as<double>(), n, c->A64.as<double>(), np, dst, n));
+    } else {
+      HIPC(hipMemcpy2DAsync(dst, sizeof(double) * n, c->A64.p, sizeof(double) * np, sizeof(double) * n,
+                            static_cast<size_t>(mb), hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIPC(hipMemcpyAsync(c->ex_keep.as<uint8_t>() + (j0 - lo), c->keep.p, static_cast<size_t>(mb),
+                        hipMemcpyDeviceToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->ex_gtg.as<double>() + (j0 - lo), c->gtg.p, sizeof(double) * mb, hipMemcpyDeviceToDevice,
+                        c->stream));
+  }
+  c->ex_lo = lo;
+  c->ex_hi = hi;
+  c->ex_valid = true;
+  return EQTL_OK;
+}
+
+// FP64 minimum p over each task's window, for the gene side now prepared
+// (columns tasks[i].e), the exact rows covering every task's range.
+static int exact_minp(eqtl_gpu_ctx* c, int model, const std::vector<ExactTask>& tasks, std::vector<double>& out) {
+  out.assign(tasks.size(), 2.0);
+  if (tasks.empty()) return EQTL_OK;
+  int64_t lo = INT64_MAX, hi = -1;
+  for (const auto& t : tasks) {
+    lo = std::min(lo, t.jlo);
+    hi = std::max(hi, t.jhi);
+  }
+  if (lo >= hi) return EQTL_OK;
+  CHK(exact_rows(c, lo, hi, model == EQTL_MODEL_LMM));
+  CHK(ensure(c->ex_tasks, sizeof(ExactTask) * tasks.size()));
+  CHK(ensure(c->ex_out, sizeof(double) * tasks.size()));
+  HIPC(hipMemcpyAsync(c->ex_tasks.p, tasks.data(), sizeof(ExactTask) * tasks.size(), hipMemcpyHostToDevice,
+                      c->stream));
+  ExactArgs a;
+  a.tasks = c->ex_tasks.as<ExactTask>();
+  a.n_tasks = static_cast<int32_t>(tasks.size());
+  a.r0 = c->ex_lo;
+  a.R = c->ex_R.as<double>();
+  a.keep = c->ex_keep.as<uint8_t>();
+  a.gtg = c->ex_gtg.as<double>();
+  a.snp_chrom = c->snp_chrom.as<int32_t>();
+  a.snp_pos = c->snp_pos.as<int64_t>();
+  a.n = c->n;
+  a.p = c->p;
+  a.lam = c->lam64.as<double>();
+  a.delta = c->delta.as<double>();
+  a.U = c->U64.as<double>();
+  a.Xt = c->Xt64.as<double>();
+  a.Lc = c->Lc.as<double>();
+  a.gene_scal = c->gscal.as<double>();
+  a.df = static_cast<double>(c->n - c->p - 1);
+  a.lbeta = std::lgamma(0.5 * a.df) + std::lgamma(0.5) - std::lgamma(0.5 * a.df + 0.5);
+  a.out = c->ex_out.as<double>();
+  Timed t(c, "exact");
+  if (model == EQTL_MODEL_LMM)
+    hipLaunchKernelGGL(k_exact_minp<EQTL_MODEL_LMM>, dim3(static_cast<unsigned>(tasks.size())), dim3(256), 0,
+                       c->stream, a);
+  else
+    hipLaunchKernelGGL(k_exact_minp<EQTL_MODEL_LM>, dim3(static_cast<unsigned>(tasks.size())), dim3(256), 0,
+                       c->stream, a);
+  CHK(check_launch("k_exact_minp"));
+  HIPC(hipMemcpyAsync(out.data(), c->ex_out.p, sizeof(double) * tasks.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  return t.done();
+}
 
 int eqtl_gpu_cis_perm(eqtl_gpu_ctx* c, int32_t precision, int32_t B, int32_t seed, const uint32_t* gene_hash,
                       const double* best_p_obs, int32_t delta_mode, double* perm_min_p, double* p_emp) {
@@ -1660,6 +1760,41 @@ int eqtl_gpu_cis_perm(eqtl_gpu_ctx* c, int32_t precision, int32_t B, int32_t see
     if (std::isfinite(best_p_obs[k]) && best_p_obs[k] <= 1.0 && glo[k] < ghi[k]) genes.push_back(k);
   std::vector<double> minp(static_cast<size_t>(G) * B, std::numeric_limits<double>::quiet_NaN());
   const int nb_max = std::max(1, kPermBatch / B);
+  // Exact counts: the observed statistic of every permuted gene in FP64 (its
+  // gene side is still the observed one), with the FP64 rows of the whole
+  // span when they fit a third of the free device memory (else per group).
+  c->ex_valid = false;
+  const double band = precision == EQTL_PREC_FP32 ? kExactBandF32 : kExactBandBF16X3;
+  std::vector<double> Tobs64(G, std::numeric_limits<double>::quiet_NaN());
+  auto task_of = [&](int e, int k) {
+    ExactTask t;
+    t.e = e;
+    t.chrom = gch[k];
+    t.jlo = glo[k];
+    t.jhi = ghi[k];
+    t.cs = gcs[k];
+    t.ce = gce[k];
+    return t;
+  };
+  if (!genes.empty()) {
+    int64_t lo_all = INT64_MAX, hi_all = -1;
+    for (int k : genes) {
+      lo_all = std::min(lo_all, glo[k]);
+      hi_all = std::max(hi_all, ghi[k]);
+    }
+    size_t free_b = 0, total_b = 0;
+    HIPC(hipMemGetInfo(&free_b, &total_b));
+    if (static_cast<double>(hi_all - lo_all) * n * sizeof(double) <= free_b / 3.0)
+      CHK(exact_rows(c, lo_all, hi_all, model == EQTL_MODEL_LMM));
+    for (size_t g0 = 0; g0 < genes.size(); g0 += nb_max) {
+      std::vector<ExactTask> tasks;
+      for (size_t q = g0; q < std::min(genes.size(), g0 + nb_max); ++q) tasks.push_back(task_of(genes[q], genes[q]));
+      std::vector<double> mp;
+      CHK(exact_minp(c, model, tasks, mp));
+      for (size_t q = 0; q < tasks.size(); ++q) Tobs64[tasks[q].e] = -std::log10(std::max(std::min(1.0, mp[q]), 1e-300));
+    }
+  }
+  int64_t n_exact = 0;
   DBuf mt, idx, gof, hof;
   int rc = EQTL_OK;
   for (size_t g0 = 0; g0 < genes.size() && rc == EQTL_OK; g0 += nb_max) {
@@ -1736,11 +1871,28 @@ int eqtl_gpu_cis_perm(eqtl_gpu_ctx* c, int32_t precision, int32_t B, int32_t see
     if (rc == EQTL_OK) rc = eqtl_gpu_scan_end(c, st.data());
     else c->scanning = false;
     if (rc != EQTL_OK) break;
+    // permutations whose statistic lies near the observed one: FP64 tests
+    std::vector<ExactTask> tasks;
     for (int e = 0; e < E; ++e) {
       const int k = go[e / B], b = e % B;
-      minp[static_cast<size_t>(k) * B + b] = std::min(1.0, st[e].best_p);
+      const double mp = std::min(1.0, st[e].best_p);
+      minp[static_cast<size_t>(k) * B + b] = mp;
+      const double Tb = -std::log10(std::max(mp, 1e-300));
+      if (st[e].best_p <= 1.0 && std::fabs(Tb - Tobs64[k]) <= band * (1.0 + Tobs64[k])) tasks.push_back(task_of(e, k));
+    }
+    if (!tasks.empty()) {
+      std::vector<double> mp;
+      rc = exact_minp(c, model, tasks, mp);
+      if (rc != EQTL_OK) break;
+      for (size_t q = 0; q < tasks.size(); ++q) {
+        const int e = tasks[q].e, k = go[e / B], b = e % B;
+        if (mp[q] <= 1.0) minp[static_cast<size_t>(k) * B + b] = mp[q];
+      }
+      n_exact += static_cast<int64_t>(tasks.size());
     }
   }
+  c->ex_valid = false;
+  c->perm_exact = n_exact;
   // the gene side now holds the last batch: callers prepare genes again
   c->prepped_model = -1;
   c->have_windows = false;
@@ -1752,8 +1904,9 @@ int eqtl_gpu_cis_perm(eqtl_gpu_ctx* c, int32_t precision, int32_t B, int32_t see
       if (!done) {
         p_emp[k] = std::numeric_limits<double>::quiet_NaN();
       } else {
-        // p_emp_count (stats_extra.cpp:39-46) on T = -log10(max(p, 1e-300))
-        const double Tobs = -std::log10(std::max(best_p_obs[k], 1e-300));
+        // p_emp_count (stats_extra.cpp:39-46) on T = -log10(max(p, 1e-300)),
+        // with the observed T in FP64
+        const double Tobs = Tobs64[k];
         int ge = 0;
         for (int b = 0; b < B; ++b)
           if (-std::log10(std::max(minp[static_cast<size_t>(k) * B + b], 1e-300)) >= Tobs) ++ge;

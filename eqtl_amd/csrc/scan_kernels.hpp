@@ -980,6 +980,131 @@ __global__ void k_pbest_write(const eqtl_hit* out, int64_t count, const unsigned
 }
 
 // ---------------------------------------------------------------------------
+// FP64 minimum p over a gene's window, for the exact permutation counts
+// (p_emp_count, stats_extra.cpp:39-46, compares T = -log10(max(p, 1e-300))
+// of each permutation with the observed one; a permutation whose split-
+// precision T lies near the observed T is decided again here, from the FP64
+// tests the reference computes: test_lm, model_lm.cpp:32-57; test_lmm,
+// model_lmm.cpp:160-223, in the bordered-Schur form of k_polish).
+// One workgroup per task (gene-side column e, SNP range of its window); each
+// wave takes every fourth SNP of the range and runs its test in FP64 over the
+// exact rows R (LM: the residualised dosage g'; LMM: Q^T g'), the lanes
+// striding the samples; the window rule (chromosome, [cs, ce]) and the SNP's
+// keep flag decide which SNPs are tested. out[task] = the smallest p, or 2
+// when no SNP was tested.
+// ---------------------------------------------------------------------------
+struct ExactTask {
+  int32_t e;         // gene-side column (gene or pseudo-gene)
+  int32_t chrom;     // window chromosome id
+  int64_t jlo, jhi;  // global SNP index range holding the window
+  int64_t cs, ce;    // window bounds (positions)
+};
+
+struct ExactArgs {
+  const ExactTask* tasks;
+  int32_t n_tasks;
+  const double* R;        // exact rows, n doubles per SNP, SNP r0 + q at R + q * n
+  int64_t r0;             // global index of R's first SNP
+  const uint8_t* keep;    // per SNP of R
+  const double* gtg;      // LM: ||g'||^2 per SNP of R
+  const int32_t* snp_chrom;  // global annotation
+  const int64_t* snp_pos;
+  int32_t n, p;
+  const double* lam;      // LMM
+  const double* delta;    // LMM, per gene-side column
+  const double* U;        // LMM: s o r (rotated space); LM: y_s; n per column
+  const double* Xt;       // LMM: Q^T X, n x p col-major
+  const double* Lc;       // LMM: p x p per column
+  const double* gene_scal;  // LMM rss_null, LM yty
+  double df, lbeta;
+  double* out;
+};
+
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_exact_minp(ExactArgs a) {
+  constexpr int kP = 32;
+  __shared__ double wmin[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const ExactTask tk = a.tasks[blockIdx.x];
+  const int n = a.n, p = a.p, e = tk.e;
+  const double* u = a.U + static_cast<size_t>(e) * n;
+  const double gscal = a.gene_scal[e];
+  const double dk = MODEL == EQTL_MODEL_LMM ? a.delta[e] : 0.0;
+  const double* L = MODEL == EQTL_MODEL_LMM ? a.Lc + static_cast<size_t>(e) * p * p : nullptr;
+  double best = 2.0;
+  for (int64_t j = tk.jlo + wave; j < tk.jhi; j += 4) {
+    const int64_t q = j - a.r0;
+    if (!a.keep[q] || a.snp_chrom[j] != tk.chrom || a.snp_pos[j] < tk.cs || a.snp_pos[j] > tk.ce) continue;
+    const double* g = a.R + static_cast<size_t>(q) * n;
+    double pv = 1.0;
+    if constexpr (MODEL == EQTL_MODEL_LM) {
+      double num = 0.0;
+      for (int i = lane; i < n; i += 64) num = fma(u[i], g[i], num);
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) num += __shfl_xor(num, o);
+      const double gtg = a.gtg[q];
+      if (!(gtg < 1e-12) && a.df > 0) {
+        const double beta = num / gtg;
+        const double rss = gscal - beta * num;
+        const double s2 = fmax(rss / a.df, 0.0);
+        const double se = sqrt(s2 / gtg);
+        const double t = se > 0 ? beta / se : 0.0;
+        pv = p_from_t_dev(t, a.df, a.lbeta);
+      }
+    } else {
+      double gg = 0.0, num = 0.0, av[kP];
+#pragma unroll
+      for (int c = 0; c < kP; ++c) av[c] = 0.0;
+      for (int i = lane; i < n; i += 64) {
+        double v = dk * a.lam[i] + 1.0;
+        if (v < 1e-12) v = 1e-12;
+        const double gi = g[i], dg = gi / v;
+        gg = fma(dg, gi, gg);
+        num = fma(u[i], gi, num);
+#pragma unroll
+        for (int c = 0; c < kP; ++c)
+          if (c < p) av[c] = fma(a.Xt[static_cast<size_t>(c) * n + i], dg, av[c]);
+      }
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        gg += __shfl_xor(gg, o);
+        num += __shfl_xor(num, o);
+#pragma unroll
+        for (int c = 0; c < kP; ++c)
+          if (c < p) av[c] += __shfl_xor(av[c], o);
+      }
+      // W^T h = Lc^-1 a (forward substitution), S = gg - |W^T h|^2
+      double ss = 0.0;
+#pragma unroll
+      for (int i = 0; i < kP; ++i) {
+        if (i < p) {
+          double t = av[i];
+#pragma unroll
+          for (int c = 0; c < kP; ++c)
+            if (c < i) t -= L[c * p + i] * av[c];
+          av[i] = t / L[i * p + i];
+          ss += av[i] * av[i];
+        }
+      }
+      const double S = gg - ss;
+      if (!(gg < 1e-12) && !(S <= 1e-15)) {
+        const double beta = num / S;
+        double qq = gscal - S * beta * beta;
+        if (qq < 0) qq = 0;
+        const double sigma2 = a.df > 0 ? qq / a.df : 1.0;
+        const double se = sqrt(fmax(sigma2 / S, 0.0));
+        const double t = se > 0 ? beta / se : 0.0;
+        pv = p_from_t_dev(t, a.df, a.lbeta);
+      }
+    }
+    if (isfinite(pv)) best = fmin(best, pv);
+  }
+  if (lane == 0) wmin[wave] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) a.out[blockIdx.x] = fmin(fmin(wmin[0], wmin[1]), fmin(wmin[2], wmin[3]));
+}
+
+// ---------------------------------------------------------------------------
 // Self-test of the hardware conventions the kernels rely on (one workgroup):
 //  bit 0: bf16 16x16x32 MFMA operand/result lane layout
 //  bit 1: fp32 16x16x4 MFMA layout with the 8-k-slot grouping of mfma_f32x8

@@ -3,7 +3,13 @@ gene-level permutations) against the FP64 oracle's restatement of
 scan_gene_snps (scan_cis.hpp:15-185).
 
 Tolerances: tested SNP sets, n_tested and the best SNP per gene exact; Wald t
-|dt| <= 5e-6 (1+|t|) (FP32) / 5e-5 (1+|t|) (BF16X3); p relative 1e-3."""
+|dt| <= 5e-6 (1+|t|) (FP32) / 5e-5 (1+|t|) (BF16X3); p relative 1e-3;
+permutation minima relative 1e-3 and p_emp identical for every gene (LM at
+B = 1000, LMM with the oracle's eigenbasis injected at B = 300)."""
+import functools
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import pytest
 
@@ -92,10 +98,12 @@ def test_cis_lmm_observed(gpu_ctx, prec):
     print(f"cis LMM prec={prec}: max |dt|/(1+|t|) = {worst:.3e}")
 
 
-def _perm_compare(pm_g, pe_g, refs, B, tol_rel, tie_rel):
-    """perm_min_p within tol_rel of the oracle; p_emp identical for genes whose
-    permutation statistics all stay more than tie_rel away from T_obs."""
-    worst, exact, skipped = 0.0, 0, 0
+def _perm_compare(pm_g, pe_g, refs, B, tol_rel):
+    """perm_min_p within tol_rel of the oracle; p_emp identical for every gene
+    (p_emp_count, stats_extra.cpp:39-46). Also returns how many genes have a
+    permutation statistic within 1e-4 of T_obs (near ties, which the engine
+    decides from FP64 tests) and the closest such distance."""
+    worst, near, closest = 0.0, 0, np.inf
     for k, (idx, ref) in enumerate(refs):
         if ref["n_tested"] == 0:
             assert np.isnan(pe_g[k])
@@ -105,25 +113,36 @@ def _perm_compare(pm_g, pe_g, refs, B, tol_rel, tie_rel):
         worst = max(worst, rel.max())
         assert rel.max() < tol_rel, (k, rel.max())
         Tobs = -np.log10(max(ref["best_p"], 1e-300))
-        if np.any(np.abs(ref["T_perm"] - Tobs) <= tie_rel * max(Tobs, 1e-12)):
-            skipped += 1
-            continue
-        assert pe_g[k] == ref["p_emp"], (k, pe_g[k], ref["p_emp"])
-        exact += 1
-    return worst, exact, skipped
+        d = np.abs(ref["T_perm"] - Tobs)
+        closest = min(closest, d.min() / max(Tobs, 1e-12))
+        near += int(np.any(d <= 1e-4 * max(Tobs, 1e-12)))
+        assert pe_g[k] == ref["p_emp"], (k, pe_g[k], ref["p_emp"], d.min())
+    return worst, near, closest
+
+
+@functools.lru_cache(maxsize=None)
+def _lm_perm_refs(B, seed):
+    """Oracle cis LM with B permutations for every gene of the LM case (genes in
+    parallel threads: the oracle's calls release the GIL)."""
+    ds, rows, (gc, gs, ge), _ = trans_lm_case()
+    dos, _, keep, _ = O.decode_bed(rows, ds.n)
+    G = ds.Y.shape[1]
+
+    def one(k):
+        idx = _window_snps(ds, keep, gc, gs, ge, k)
+        return idx, O.cis_gene("lm", ds.Y[:, k], ds.X(), dos[idx], B=B, seed=seed, gene=f"g{k}")
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(one, range(G)))
 
 
 @pytest.mark.parametrize("prec", [eqtl_amd.PREC_FP32, eqtl_amd.PREC_BF16X3])
 def test_cis_lm_permutations(gpu_ctx, prec):
     ds, rows, (gc, gs, ge), _ = trans_lm_case()
-    dos, _, keep, _ = O.decode_bed(rows, ds.n)
-    B, seed = 30, 42
+    B, seed = 1000, 42
     G = ds.Y.shape[1]
     names = [f"g{k}" for k in range(G)]
-    refs = []
-    for k in range(G):
-        idx = _window_snps(ds, keep, gc, gs, ge, k)
-        refs.append((idx, O.cis_gene("lm", ds.Y[:, k], ds.X(), dos[idx], B=B, seed=seed, gene=names[k])))
+    refs = _lm_perm_refs(B, seed)
     gpu_ctx.set_samples(ds.n)
     gpu_ctx.set_policy(True, 0.8, 0.0)
     gpu_ctx.set_snp_annot(ds.chrom, ds.pos)
@@ -133,9 +152,9 @@ def test_cis_lm_permutations(gpu_ctx, prec):
     _, st = gpu_ctx.scan(eqtl_amd.MODEL_LM, prec, rows, pthr=1e-5, block=1024, scope=eqtl_amd.SCOPE_CIS)
     hashes = np.array([O.fnv1a(nm) for nm in names], dtype=np.uint32)
     pm, pe = gpu_ctx.cis_perm(prec, B, seed, hashes, st["best_p"])
-    worst, exact, skipped = _perm_compare(pm, pe, refs, B, 1e-3, 1e-4)
-    print(f"cis LM perms prec={prec}: max perm_min_p rel {worst:.3e}, p_emp exact {exact}, near-tie genes {skipped}")
-    assert exact >= 0.9 * sum(1 for _, r in refs if r["n_tested"] > 0)
+    worst, near, closest = _perm_compare(pm, pe, refs, B, 1e-3)
+    print(f"cis LM perms prec={prec} B={B}: max perm_min_p rel {worst:.3e}, p_emp equal for every gene; "
+          f"genes with a near tie (1e-4 T_obs) {near}, closest {closest:.2e}")
     # beta approximation of the same minima, GPU against the oracle
     _beta_compare(gpu_ctx, pm, st["best_p"])
 
@@ -176,18 +195,27 @@ def test_beta_approx_synthetic(gpu_ctx):
     assert n_fit == pm.shape[0] - 1
 
 
-def test_cis_lmm_permutations(gpu_ctx):
+@functools.lru_cache(maxsize=None)
+def _lmm_perm_refs(B, seed, G):
     ds, rows, (Q, lam), (gc, gs, ge), _ = lmm_case()
     dos, _, keep, _ = O.decode_bed(rows, ds.n)
-    B, seed = 12, 7
+
+    def one(k):
+        idx = _window_snps(ds, keep, gc, gs, ge, k)
+        return idx, O.cis_gene("lmm", ds.Y[:, k], ds.X(), dos[idx], B=B, seed=seed, gene=f"g{k}", Q=Q, lam=lam)
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        return list(ex.map(one, range(G)))
+
+
+@pytest.mark.parametrize("prec", [eqtl_amd.PREC_FP32, eqtl_amd.PREC_BF16X3])
+def test_cis_lmm_permutations(gpu_ctx, prec):
+    ds, rows, (Q, lam), (gc, gs, ge), _ = lmm_case()
+    B, seed = 300, 7
     G = 16
     names = [f"g{k}" for k in range(G)]
-    refs, deltas = [], np.empty(G)
-    for k in range(G):
-        idx = _window_snps(ds, keep, gc, gs, ge, k)
-        r = O.cis_gene("lmm", ds.Y[:, k], ds.X(), dos[idx], B=B, seed=seed, gene=names[k], Q=Q, lam=lam)
-        refs.append((idx, r))
-        deltas[k] = r["delta"]
+    refs = _lmm_perm_refs(B, seed, G)
+    deltas = np.array([r["delta"] for _, r in refs])
     gpu_ctx.set_samples(ds.n)
     gpu_ctx.set_policy(True, 0.8, 0.0)
     gpu_ctx.set_snp_annot(ds.chrom, ds.pos)
@@ -195,13 +223,11 @@ def test_cis_lmm_permutations(gpu_ctx):
     gpu_ctx.set_basis(Q, lam)
     gpu_ctx.lmm_prep(ds.Y[:, :G], ds.X(), delta_mode=eqtl_amd.DELTA_GIVEN, delta_in=deltas)
     gpu_ctx.set_gene_windows(gc[:G], gs[:G], ge[:G])
-    _, st = gpu_ctx.scan(eqtl_amd.MODEL_LMM, eqtl_amd.PREC_FP32, rows, pthr=1e-5, block=1024,
-                         scope=eqtl_amd.SCOPE_CIS)
+    _, st = gpu_ctx.scan(eqtl_amd.MODEL_LMM, prec, rows, pthr=1e-5, block=1024, scope=eqtl_amd.SCOPE_CIS)
     hashes = np.array([O.fnv1a(nm) for nm in names], dtype=np.uint32)
-    pm, pe = gpu_ctx.cis_perm(eqtl_amd.PREC_FP32, B, seed, hashes, st["best_p"], delta_mode=eqtl_amd.DELTA_REF)
+    pm, pe = gpu_ctx.cis_perm(prec, B, seed, hashes, st["best_p"], delta_mode=eqtl_amd.DELTA_REF)
     # each permutation re-estimates delta with the reference search (restated on
-    # the GPU in batched form); its rounding differs from the oracle's, so p is
-    # compared at 1e-3 and ties are decided at 1e-4 of T_obs
-    worst, exact, skipped = _perm_compare(pm, pe, refs, B, 1e-3, 1e-4)
-    print(f"cis LMM perms: max perm_min_p rel {worst:.3e}, p_emp exact {exact}, near-tie genes {skipped}")
-    assert exact >= 0.9 * sum(1 for _, r in refs if r["n_tested"] > 0)
+    # the GPU in batched form; the same steps in FP64), so p is compared at 1e-3
+    worst, near, closest = _perm_compare(pm, pe, refs, B, 1e-3)
+    print(f"cis LMM perms prec={prec} B={B}: max perm_min_p rel {worst:.3e}, p_emp equal for every gene; "
+          f"genes with a near tie (1e-4 T_obs) {near}, closest {closest:.2e}")

@@ -48,42 +48,10 @@ def log(*a):
 
 
 def make_inputs(ctx, seed=1003):
-    rng = np.random.default_rng(seed)
-    bps = (N + 3) // 4
-    rows_dev = ctx.synth_bed(seed, M, N, 0.05, 0.5, 0.01)
-    per = M // NCHROM
-    chrom = np.repeat(np.arange(1, NCHROM + 1, dtype=np.int32), per)
-    pos = np.tile(1 + 1000 * np.arange(per, dtype=np.int64), NCHROM)
-    C = rng.standard_normal((N, NCOV))
-    X = np.hstack([np.ones((N, 1)), C])
-    # polygenic term over 1000 causal SNPs (rows pulled back from HBM), h2 ~ Beta(2,5)
-    from eqtl_amd import synth
-    causal = np.sort(rng.choice(M, 1000, replace=False))
-    Z = np.empty((1000, N))
-    for i, j in enumerate(causal):
-        g = synth.unpack_bed(ctx.copy_rows(rows_dev, int(j), 1, bps), N)[0]
-        Z[i] = synth.standardized(g[None, :])[0]
-    h2 = np.clip(rng.beta(2, 5, G), 0, 0.95)
-    u = Z.T @ rng.standard_normal((1000, G))
-    u = (u - u.mean(0)) / u.std(0)
-    Y = rng.standard_normal((N, G)) + u * np.sqrt(h2 / (1 - h2))
-    # 200 genes with one planted trans SNP, |beta| log-uniform on [0.05, 2]
-    gene_chrom = rng.integers(1, NCHROM + 1, G).astype(np.int32)
-    gene_start = rng.integers(1, per * 1000 - 1000, G).astype(np.int64)
-    planted = {}
-    for k in range(200):
-        while True:
-            j = int(rng.integers(0, M))
-            if chrom[j] != gene_chrom[k]:
-                break
-        beta = float(np.exp(rng.uniform(np.log(0.05), np.log(2.0)))) * (1 if rng.random() < 0.5 else -1)
-        g = synth.unpack_bed(ctx.copy_rows(rows_dev, j, 1, bps), N)[0].astype(float)
-        g[g < 0] = g[g >= 0].mean()
-        Y[:, k] += beta * (g - g.mean())
-        planted[k] = j
-    cs = np.maximum(1, gene_start - 1_000_000)
-    ce = gene_start + 999 + 1_000_000
-    return rows_dev, bps, chrom, pos, X, Y, (gene_chrom, cs, ce), planted
+    """The trans workload's inputs (eqtl_amd.workloads.c3)."""
+    from eqtl_amd import workloads
+    w = workloads.c3(ctx, seed)
+    return w.rows_dev, w.bps, w.chrom, w.pos, w.X, w.Y, w.win, w.extra["planted"]
 
 
 def trans_pass(ctx, rows_dev, bps, X, Y, m, prec, delta_mode, delta_in=None, pthr=1e-5):
@@ -537,37 +505,11 @@ CIS_N, CIS_M, CIS_G, CIS_NCOV, CIS_NCHROM = 1000, 1_000_000, 20_000, 15, 10
 
 
 def cis_inputs(ctx, n_genes, seed=1004):
-    """Cis permutation workload: n = 1000, 1M SNPs on 10 chromosomes (100 bp
-    apart), a 20k-gene table with window sizes c_g = round(exp(U(ln 50, ln 5000)))
-    SNPs (run with a zero flank, so a gene's window holds exactly c_g SNPs), of
-    which a seeded slice of n_genes genes is run, sorted by (chromosome, start);
-    Y ~ N(0, 1), 5% of the genes with one cis eQTL |beta| ~ U(0.1, 0.5);
-    15 covariates + intercept."""
-    from eqtl_amd import synth
-    rng = np.random.default_rng(seed)
-    bps = (CIS_N + 3) // 4
-    rows_dev = ctx.synth_bed(seed, CIS_M, CIS_N, 0.01, 0.5, 0.01)
-    per = CIS_M // CIS_NCHROM
-    chrom = np.repeat(np.arange(1, CIS_NCHROM + 1, dtype=np.int32), per)
-    pos = np.tile(1 + 100 * np.arange(per, dtype=np.int64), CIS_NCHROM)
-    cg = np.rint(np.exp(rng.uniform(np.log(50), np.log(5000), CIS_G))).astype(np.int64)
-    gch = rng.integers(1, CIS_NCHROM + 1, CIS_G).astype(np.int32)
-    start = 1 + (rng.random(CIS_G) * (per * 100 - 100 * cg)).astype(np.int64)
-    end = start + 100 * cg - 1
-    sel = rng.choice(CIS_G, n_genes, replace=False)
-    sel = sel[np.lexsort((start[sel], gch[sel]))]
-    X = np.hstack([np.ones((CIS_N, 1)), rng.standard_normal((CIS_N, CIS_NCOV))])
-    Y = rng.standard_normal((CIS_N, n_genes))
-    for k in range(n_genes):
-        if rng.random() < 0.05:
-            g0 = sel[k]
-            first = (gch[g0] - 1) * per + (start[g0] - 1 + 99) // 100
-            j = int(first + rng.integers(0, cg[g0]))
-            g = synth.unpack_bed(ctx.copy_rows(rows_dev, j, 1, bps), CIS_N)[0].astype(float)
-            g[g < 0] = g[g >= 0].mean()
-            Y[:, k] += rng.uniform(0.1, 0.5) * (1 if rng.random() < 0.5 else -1) * (g - g.mean())
-    names = [f"g{int(k)}" for k in sel]
-    return rows_dev, bps, chrom, pos, X, Y, (gch[sel], start[sel], end[sel]), cg[sel], names
+    """Cis permutation workload (eqtl_amd.workloads.c4): a seeded slice of
+    n_genes genes of the 20k-gene table, sorted by (chromosome, start)."""
+    from eqtl_amd import workloads
+    w = workloads.c4(ctx, n_genes, seed)
+    return w.rows_dev, w.bps, w.chrom, w.pos, w.X, w.Y, w.win, w.extra["c_g"], w.names
 
 
 def cis_model_pass(ctx, model, rows_dev, bps, X, Y, win, hashes, B, seed, prec, delta_mode):
@@ -634,7 +576,7 @@ def bench_cis_perm(args):
             n_obs, n_perm, pe = cis_model_pass(ctx, model, rows_dev, bps, X, Y, win, hashes, B, seed, prec, dmode)
             ctx.synchronize()
             dtm = time.perf_counter() - t1
-            fam = {f: ctx.timing(f) for f in ("wald", "reml", "rotate", "basis", "decode")}
+            fam = {f: ctx.timing(f) for f in ("wald", "reml", "rotate", "basis", "decode", "residualize", "exact", "polish")}
             ctx.set_timing(False)
             r = res.setdefault(name, {"s": 0.0, "obs_tests": 0, "perm_tests": 0, "fam": {}})
             r["s"] += dtm
@@ -723,35 +665,13 @@ def bench_cis_lmm(args):
     from eqtl_amd._lib import lib, _check
     global BLOCK
     BLOCK = args.block
+    from eqtl_amd import workloads
     n, m, G = C2_N, C2_M, C2_G
-    rng = np.random.default_rng(1002)
-    bps = (n + 3) // 4
     ctx = eqtl_amd.Context(0)
-    rows_dev = ctx.synth_bed(1002, m, n, 0.01, 0.5, 0.01)
-    per = m // C2_NCHROM
-    chrom = np.repeat(np.arange(1, C2_NCHROM + 1, dtype=np.int32), per)
-    pos = np.tile(1 + 2000 * np.arange(per, dtype=np.int64), C2_NCHROM)
-    gch = rng.integers(1, C2_NCHROM + 1, G).astype(np.int32)
-    tss = rng.integers(1, per * 2000 - 1000, G).astype(np.int64)
-    order = np.lexsort((tss, gch))
-    gch, tss = gch[order], tss[order]
-    cs, ce = np.maximum(1, tss - C2_FLANK), tss + 999 + C2_FLANK
-    X = np.hstack([np.ones((n, 1)), rng.standard_normal((n, C2_NCOV))])
-    causal = np.sort(rng.choice(m, 500, replace=False))
-    Z = np.stack([synth.standardized(synth.unpack_bed(ctx.copy_rows(rows_dev, int(j), 1, bps), n))[0] for j in causal])
-    h2 = rng.uniform(0, 0.6, G)
-    u = Z.T @ rng.standard_normal((500, G))
-    u = (u - u.mean(0)) / u.std(0)
-    Y = rng.standard_normal((n, G)) + u * np.sqrt(h2 / (1 - h2))
-    for k in range(G):
-        first = (gch[k] - 1) * per
-        lo = first + max(0, (int(cs[k]) - 1 + 1999) // 2000)
-        hi = first + min(per - 1, (int(ce[k]) - 1) // 2000)
-        for _ in range(int(rng.integers(1, 4))):
-            j = int(rng.integers(lo, hi + 1))
-            g = synth.unpack_bed(ctx.copy_rows(rows_dev, j, 1, bps), n)[0].astype(float)
-            g[g < 0] = g[g >= 0].mean()
-            Y[:, k] += rng.uniform(0.1, 0.5) * (1 if rng.random() < 0.5 else -1) * (g - g.mean())
+    w = workloads.c2(ctx)
+    rows_dev, bps, chrom, pos, X, Y = w.rows_dev, w.bps, w.chrom, w.pos, w.X, w.Y
+    gch, cs, ce = w.win
+    per = w.extra["per"]
     ctx.set_samples(n)
     ctx.set_policy(True, 0.8, 0.0)
     _check(lib().eqtl_gpu_grm_begin(ctx.handle))

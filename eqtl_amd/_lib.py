@@ -188,6 +188,11 @@ class Context:
             _check(fn(self._c, _p(blk), ctypes.c_int32(0), ctypes.c_int64(blk.shape[0]), ctypes.c_int32(bps)))
         return self.grm_finish()
 
+    def grm_accumulate_dev(self, rows_dev, m, bps):
+        """eqtl_gpu_grm_accumulate on device rows (after grm_begin)."""
+        _check(lib().eqtl_gpu_grm_accumulate(self._c, ctypes.c_void_p(rows_dev), ctypes.c_int32(1),
+                                             ctypes.c_int64(m), ctypes.c_int32(bps)))
+
     def grm_accumulate_noalloc_dev(self, rows_dev, m, bps):
         """eqtl_gpu_grm_accumulate_noalloc on device rows (after grm_begin)."""
         _check(lib().eqtl_gpu_grm_accumulate_noalloc(self._c, ctypes.c_void_p(rows_dev), ctypes.c_int32(1),

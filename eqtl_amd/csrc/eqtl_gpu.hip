@@ -900,7 +900,11 @@ static int reml_search(eqtl_gpu_ctx* c, int mode, std::vector<double>& delta) {
     delta = d;
     return EQTL_OK;
   }
-  // safeguarded Newton on theta = ln delta inside the bracket
+  // safeguarded Newton on theta = ln delta from the grid's best point, over
+  // the search's whole range [1e-8, 1e6] (the reference's final Newton steps
+  // may also leave the golden-section bracket: a likelihood still falling at
+  // the bracket's end is followed to its minimum or to the range's end);
+  // every accepted step lowers f
   std::vector<double> th(G), f(G), g(G), h(G), tn(G), tlo(G), thi(G);
   std::vector<int> back(G, 0), iters(G, 0);
   std::vector<char> state(G, 1);  // 1: propose a step, 2: trial pending, 0: done
@@ -908,8 +912,8 @@ static int reml_search(eqtl_gpu_ctx* c, int mode, std::vector<double>& delta) {
   for (int k = 0; k < G; ++k) {
     all[k] = k;
     th[k] = std::log(best[k]);
-    tlo[k] = std::log(lo[k]);
-    thi[k] = std::log(hi[k]);
+    tlo[k] = std::log(1e-8);
+    thi[k] = std::log(1e6);
     ed.push_back(std::exp(th[k]));
   }
   CHK(reml_eval(c, all, ed, true, f.data(), g.data(), h.data()));
@@ -1252,6 +1256,26 @@ static int cis_windows(eqtl_gpu_ctx* c) {
 static int scan_begin_impl(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t scope, double pthr,
                            bool windows_ready);
 
+// t-tail table for the residual df n - p - 1 of the prepared genes (shared by
+// every test of a scan).
+static int ensure_tail(eqtl_gpu_ctx* c) {
+  const double df = static_cast<double>(c->n - c->p - 1);
+  if (df > 0 && c->tail_df != df) {
+    std::vector<TTail> T(1);
+    const double lbeta = std::lgamma(0.5 * df) + std::lgamma(0.5) - std::lgamma(0.5 * df + 0.5);
+    tail_build(T[0], df, lbeta);
+    CHK(ensure(c->tail, sizeof(TTail)));
+    HIPC(hipMemcpyAsync(c->tail.p, T.data(), sizeof(TTail), hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    c->tail_df = df;
+  } else if (df <= 0) {
+    CHK(ensure(c->tail, sizeof(TTail)));
+    HIPC(hipMemsetAsync(c->tail.p, 0, sizeof(TTail), c->stream));
+    c->tail_df = df;
+  }
+  return EQTL_OK;
+}
+
 int eqtl_gpu_scan_begin(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t scope, double pthr) {
   return scan_begin_impl(c, model, prec, scope, pthr, false);
 }
@@ -1279,21 +1303,7 @@ static int scan_begin_impl(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t
                      c->G_pad);
   CHK(check_launch("k_state_init"));
   CHK(ensure(c->hit_count, 2 * sizeof(unsigned long long)));
-  // t-tail table for this scan's residual df (shared by every test)
-  const double df = static_cast<double>(c->n - c->p - 1);
-  if (df > 0 && c->tail_df != df) {
-    std::vector<TTail> T(1);
-    const double lbeta = std::lgamma(0.5 * df) + std::lgamma(0.5) - std::lgamma(0.5 * df + 0.5);
-    tail_build(T[0], df, lbeta);
-    CHK(ensure(c->tail, sizeof(TTail)));
-    HIPC(hipMemcpyAsync(c->tail.p, T.data(), sizeof(TTail), hipMemcpyHostToDevice, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
-    c->tail_df = df;
-  } else if (df <= 0) {
-    CHK(ensure(c->tail, sizeof(TTail)));
-    HIPC(hipMemsetAsync(c->tail.p, 0, sizeof(TTail), c->stream));
-    c->tail_df = df;
-  }
+  CHK(ensure_tail(c));
   c->scanning = true;
   return EQTL_OK;
 }
@@ -1668,24 +1678,44 @@ static int exact_rows(eqtl_gpu_ctx* c, int64_t lo, int64_t hi, bool lmm) {
 }
 
 // FP64 minimum p over each task's window, for the gene side now prepared
-// (columns tasks[i].e), the exact rows covering every task's range.
+// (columns tasks[i].e), the exact rows covering every task's range; out[i] = 2
+// when the window has no tested SNP. The scan's t-tail table (c->tail, built
+// by scan_begin for df = n - p - 1) must be current.
 static int exact_minp(eqtl_gpu_ctx* c, int model, const std::vector<ExactTask>& tasks, std::vector<double>& out) {
   out.assign(tasks.size(), 2.0);
   if (tasks.empty()) return EQTL_OK;
   int64_t lo = INT64_MAX, hi = -1;
-  for (const auto& t : tasks) {
+  std::vector<ExactPieceRef> pieces;
+  for (size_t i = 0; i < tasks.size(); ++i) {
+    const ExactTask& t = tasks[i];
     lo = std::min(lo, t.jlo);
     hi = std::max(hi, t.jhi);
+    for (int64_t j = t.jlo; j < t.jhi; j += kExactPiece) {
+      ExactPieceRef pr;
+      pr.task = static_cast<int32_t>(i);
+      pr.pad = 0;
+      pr.jlo = j;
+      pr.jhi = std::min(t.jhi, j + kExactPiece);
+      pieces.push_back(pr);
+    }
   }
-  if (lo >= hi) return EQTL_OK;
+  if (lo >= hi || pieces.empty()) return EQTL_OK;
+  if (c->tail_df != static_cast<double>(c->n - c->p - 1)) return fail(EQTL_E_STATE, "t tail table not built");
   CHK(exact_rows(c, lo, hi, model == EQTL_MODEL_LMM));
-  CHK(ensure(c->ex_tasks, sizeof(ExactTask) * tasks.size()));
-  CHK(ensure(c->ex_out, sizeof(double) * tasks.size()));
-  HIPC(hipMemcpyAsync(c->ex_tasks.p, tasks.data(), sizeof(ExactTask) * tasks.size(), hipMemcpyHostToDevice,
-                      c->stream));
+  CHK(ensure(c->ex_tasks, sizeof(ExactTask) * tasks.size() + sizeof(ExactPieceRef) * pieces.size()));
+  CHK(ensure(c->ex_out, sizeof(unsigned long long) * tasks.size()));
+  ExactTask* dt = c->ex_tasks.as<ExactTask>();
+  ExactPieceRef* dp = reinterpret_cast<ExactPieceRef*>(dt + tasks.size());
+  HIPC(hipMemcpyAsync(dt, tasks.data(), sizeof(ExactTask) * tasks.size(), hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(dp, pieces.data(), sizeof(ExactPieceRef) * pieces.size(), hipMemcpyHostToDevice, c->stream));
+  double two = 2.0;
+  unsigned long long two_bits = 0;
+  std::memcpy(&two_bits, &two, sizeof(two));
+  hipLaunchKernelGGL(k_fill_u64, dim3(static_cast<unsigned>((tasks.size() + 255) / 256)), dim3(256), 0, c->stream,
+                     c->ex_out.as<unsigned long long>(), static_cast<int>(tasks.size()), two_bits);
   ExactArgs a;
-  a.tasks = c->ex_tasks.as<ExactTask>();
-  a.n_tasks = static_cast<int32_t>(tasks.size());
+  a.tasks = dt;
+  a.pieces = dp;
   a.r0 = c->ex_lo;
   a.R = c->ex_R.as<double>();
   a.keep = c->ex_keep.as<uint8_t>();
@@ -1700,19 +1730,21 @@ static int exact_minp(eqtl_gpu_ctx* c, int model, const std::vector<ExactTask>& 
   a.Xt = c->Xt64.as<double>();
   a.Lc = c->Lc.as<double>();
   a.gene_scal = c->gscal.as<double>();
+  a.tail = c->tail.as<TTail>();
   a.df = static_cast<double>(c->n - c->p - 1);
-  a.lbeta = std::lgamma(0.5 * a.df) + std::lgamma(0.5) - std::lgamma(0.5 * a.df + 0.5);
-  a.out = c->ex_out.as<double>();
+  a.out = c->ex_out.as<unsigned long long>();
   Timed t(c, "exact");
+  const dim3 grid(static_cast<unsigned>(pieces.size()));
   if (model == EQTL_MODEL_LMM)
-    hipLaunchKernelGGL(k_exact_minp<EQTL_MODEL_LMM>, dim3(static_cast<unsigned>(tasks.size())), dim3(256), 0,
-                       c->stream, a);
+    hipLaunchKernelGGL(k_exact_minp<EQTL_MODEL_LMM>, grid, dim3(256), 0, c->stream, a);
   else
-    hipLaunchKernelGGL(k_exact_minp<EQTL_MODEL_LM>, dim3(static_cast<unsigned>(tasks.size())), dim3(256), 0,
-                       c->stream, a);
+    hipLaunchKernelGGL(k_exact_minp<EQTL_MODEL_LM>, grid, dim3(256), 0, c->stream, a);
   CHK(check_launch("k_exact_minp"));
-  HIPC(hipMemcpyAsync(out.data(), c->ex_out.p, sizeof(double) * tasks.size(), hipMemcpyDeviceToHost, c->stream));
+  std::vector<unsigned long long> bits(tasks.size());
+  HIPC(hipMemcpyAsync(bits.data(), c->ex_out.p, sizeof(unsigned long long) * tasks.size(), hipMemcpyDeviceToHost,
+                      c->stream));
   HIPC(hipStreamSynchronize(c->stream));
+  for (size_t i = 0; i < tasks.size(); ++i) std::memcpy(&out[i], &bits[i], sizeof(double));
   return t.done();
 }
 
@@ -1764,6 +1796,7 @@ int eqtl_gpu_cis_perm(eqtl_gpu_ctx* c, int32_t precision, int32_t B, int32_t see
   // gene side is still the observed one), with the FP64 rows of the whole
   // span when they fit a third of the free device memory (else per group).
   c->ex_valid = false;
+  CHK(ensure_tail(c));
   const double band = precision == EQTL_PREC_FP32 ? kExactBandF32 : kExactBandBF16X3;
   std::vector<double> Tobs64(G, std::numeric_limits<double>::quiet_NaN());
   auto task_of = [&](int e, int k) {
@@ -1784,11 +1817,12 @@ int eqtl_gpu_cis_perm(eqtl_gpu_ctx* c, int32_t precision, int32_t B, int32_t see
     }
     size_t free_b = 0, total_b = 0;
     HIPC(hipMemGetInfo(&free_b, &total_b));
-    if (static_cast<double>(hi_all - lo_all) * n * sizeof(double) <= free_b / 3.0)
-      CHK(exact_rows(c, lo_all, hi_all, model == EQTL_MODEL_LMM));
-    for (size_t g0 = 0; g0 < genes.size(); g0 += nb_max) {
+    const bool whole = static_cast<double>(hi_all - lo_all) * n * sizeof(double) <= free_b / 3.0;
+    if (whole) CHK(exact_rows(c, lo_all, hi_all, model == EQTL_MODEL_LMM));
+    const size_t group = whole ? genes.size() : static_cast<size_t>(nb_max);
+    for (size_t g0 = 0; g0 < genes.size(); g0 += group) {
       std::vector<ExactTask> tasks;
-      for (size_t q = g0; q < std::min(genes.size(), g0 + nb_max); ++q) tasks.push_back(task_of(genes[q], genes[q]));
+      for (size_t q = g0; q < std::min(genes.size(), g0 + group); ++q) tasks.push_back(task_of(genes[q], genes[q]));
       std::vector<double> mp;
       CHK(exact_minp(c, model, tasks, mp));
       for (size_t q = 0; q < tasks.size(); ++q) Tobs64[tasks[q].e] = -std::log10(std::max(std::min(1.0, mp[q]), 1e-300));

@@ -986,13 +986,18 @@ __global__ void k_pbest_write(const eqtl_hit* out, int64_t count, const unsigned
 // precision T lies near the observed T is decided again here, from the FP64
 // tests the reference computes: test_lm, model_lm.cpp:32-57; test_lmm,
 // model_lmm.cpp:160-223, in the bordered-Schur form of k_polish).
-// One workgroup per task (gene-side column e, SNP range of its window); each
-// wave takes every fourth SNP of the range and runs its test in FP64 over the
-// exact rows R (LM: the residualised dosage g'; LMM: Q^T g'), the lanes
-// striding the samples; the window rule (chromosome, [cs, ce]) and the SNP's
-// keep flag decide which SNPs are tested. out[task] = the smallest p, or 2
-// when no SNP was tested.
+// A task (gene-side column e, SNP range of its window) is cut into pieces of
+// kExactPiece SNPs, one workgroup per piece; each wave takes every fourth SNP
+// of the piece and runs its test in FP64 over the exact rows R (LM: the
+// residualised dosage g'; LMM: Q^T g'), the lanes striding the samples; the
+// window rule (chromosome, [cs, ce]) and the SNP's keep flag decide which
+// SNPs are tested. The tail probability comes from the scan's t-tail table
+// (2e-12 relative to the continued fraction). out[task] receives the smallest
+// p as an atomic minimum of its bit pattern (non-negative doubles order as
+// their bits); the caller sets every slot to the bits of 2.0 first.
 // ---------------------------------------------------------------------------
+constexpr int kExactPiece = 64;
+
 struct ExactTask {
   int32_t e;         // gene-side column (gene or pseudo-gene)
   int32_t chrom;     // window chromosome id
@@ -1000,9 +1005,15 @@ struct ExactTask {
   int64_t cs, ce;    // window bounds (positions)
 };
 
+struct ExactPieceRef {
+  int32_t task;
+  int32_t pad;
+  int64_t jlo, jhi;  // this piece's SNP range
+};
+
 struct ExactArgs {
   const ExactTask* tasks;
-  int32_t n_tasks;
+  const ExactPieceRef* pieces;
   const double* R;        // exact rows, n doubles per SNP, SNP r0 + q at R + q * n
   int64_t r0;             // global index of R's first SNP
   const uint8_t* keep;    // per SNP of R
@@ -1016,23 +1027,29 @@ struct ExactArgs {
   const double* Xt;       // LMM: Q^T X, n x p col-major
   const double* Lc;       // LMM: p x p per column
   const double* gene_scal;  // LMM rss_null, LM yty
-  double df, lbeta;
-  double* out;
+  const TTail* tail;
+  double df;
+  unsigned long long* out;  // per task: bits of the minimum p
 };
 
 template <int MODEL>
 __global__ __launch_bounds__(256) void k_exact_minp(ExactArgs a) {
   constexpr int kP = 32;
-  __shared__ double wmin[4];
+  __shared__ double Ls[kP * kP];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const ExactTask tk = a.tasks[blockIdx.x];
+  const ExactPieceRef pc = a.pieces[blockIdx.x];
+  const ExactTask tk = a.tasks[pc.task];
   const int n = a.n, p = a.p, e = tk.e;
   const double* u = a.U + static_cast<size_t>(e) * n;
   const double gscal = a.gene_scal[e];
   const double dk = MODEL == EQTL_MODEL_LMM ? a.delta[e] : 0.0;
-  const double* L = MODEL == EQTL_MODEL_LMM ? a.Lc + static_cast<size_t>(e) * p * p : nullptr;
+  if constexpr (MODEL == EQTL_MODEL_LMM) {
+    const double* L = a.Lc + static_cast<size_t>(e) * p * p;
+    for (int i = threadIdx.x; i < p * p; i += 256) Ls[i] = L[i];
+    __syncthreads();
+  }
   double best = 2.0;
-  for (int64_t j = tk.jlo + wave; j < tk.jhi; j += 4) {
+  for (int64_t j = pc.jlo + wave; j < pc.jhi; j += 4) {
     const int64_t q = j - a.r0;
     if (!a.keep[q] || a.snp_chrom[j] != tk.chrom || a.snp_pos[j] < tk.cs || a.snp_pos[j] > tk.ce) continue;
     const double* g = a.R + static_cast<size_t>(q) * n;
@@ -1049,7 +1066,7 @@ __global__ __launch_bounds__(256) void k_exact_minp(ExactArgs a) {
         const double s2 = fmax(rss / a.df, 0.0);
         const double se = sqrt(s2 / gtg);
         const double t = se > 0 ? beta / se : 0.0;
-        pv = p_from_t_dev(t, a.df, a.lbeta);
+        pv = p_from_t_tab2(&a.tail->df, a.tail->coef_b, t);
       }
     } else {
       double gg = 0.0, num = 0.0, av[kP];
@@ -1081,8 +1098,8 @@ __global__ __launch_bounds__(256) void k_exact_minp(ExactArgs a) {
           double t = av[i];
 #pragma unroll
           for (int c = 0; c < kP; ++c)
-            if (c < i) t -= L[c * p + i] * av[c];
-          av[i] = t / L[i * p + i];
+            if (c < i) t -= Ls[c * p + i] * av[c];
+          av[i] = t / Ls[i * p + i];
           ss += av[i] * av[i];
         }
       }
@@ -1094,14 +1111,12 @@ __global__ __launch_bounds__(256) void k_exact_minp(ExactArgs a) {
         const double sigma2 = a.df > 0 ? qq / a.df : 1.0;
         const double se = sqrt(fmax(sigma2 / S, 0.0));
         const double t = se > 0 ? beta / se : 0.0;
-        pv = p_from_t_dev(t, a.df, a.lbeta);
+        pv = p_from_t_tab2(&a.tail->df, a.tail->coef_b, t);
       }
     }
     if (isfinite(pv)) best = fmin(best, pv);
   }
-  if (lane == 0) wmin[wave] = best;
-  __syncthreads();
-  if (threadIdx.x == 0) a.out[blockIdx.x] = fmin(fmin(wmin[0], wmin[1]), fmin(wmin[2], wmin[3]));
+  if (lane == 0 && best < 2.0) atomicMin(&a.out[pc.task], static_cast<unsigned long long>(__double_as_longlong(best)));
 }
 
 // ---------------------------------------------------------------------------

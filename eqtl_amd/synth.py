@@ -232,3 +232,52 @@ def grm_closed_form(geno: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
     N = o.T @ o
     K = np.where(N > 0, A / np.maximum(N, 1), 0.0)
     return K, N
+
+
+def grm_subblock(rows: np.ndarray, n: int, idx: np.ndarray, max_miss: float = 0.8,
+                 chunk: int = 16384) -> tuple[np.ndarray, np.ndarray, int]:
+    """The GCTA GRM restricted to the samples idx, from bed rows over all n
+    samples, streamed over SNP chunks: K_ij = sum_s z_si z_sj / N_ij with
+    z = (g - mu)/sqrt(2p(1-p)), mu the SNP's mean over its observed calls,
+    p = mu/2, missing calls 0; SNPs used when they have an observed call, a
+    missing share <= max_miss and p in [1e-8, 1 - 1e-8] (and AF in
+    (1e-12, 1 - 1e-12)); N_ij = used SNPs where i and j are both observed.
+    Per-SNP sums come from byte lookup tables, so only the idx columns are
+    unpacked. Returns (K_sub, N_sub, used)."""
+    idx = np.asarray(idx)
+    codes = np.arange(256, dtype=np.uint8)
+    dos_of_code = np.array([2, 0, 1, 0], dtype=np.int64)   # 00=2, 01=missing, 10=1, 11=0
+    obs_of_code = np.array([1, 0, 1, 1], dtype=np.int64)
+    sh = np.array([0, 2, 4, 6], dtype=np.uint8)
+    c4 = (codes[:, None] >> sh[None, :]) & 3                  # (256, 4)
+    nb = (n + 3) // 4
+    tail = n - 4 * (nb - 1)                                    # samples in the last byte
+    lut_sum = dos_of_code[c4].sum(1)
+    lut_obs = obs_of_code[c4].sum(1)
+    lut_sum_t = dos_of_code[c4[:, :tail]].sum(1)
+    lut_obs_t = obs_of_code[c4[:, :tail]].sum(1)
+    k = idx.size
+    A = np.zeros((k, k))
+    N = np.zeros((k, k))
+    used = 0
+    byte_i, shift_i = idx // 4, (2 * (idx % 4)).astype(np.uint8)
+    for j0 in range(0, rows.shape[0], chunk):
+        r = rows[j0:j0 + chunk, :nb]
+        s = lut_sum[r[:, :-1]].sum(1) + lut_sum_t[r[:, -1]]
+        o = lut_obs[r[:, :-1]].sum(1) + lut_obs_t[r[:, -1]]
+        mu = np.where(o > 0, s / np.maximum(o, 1), 0.0)
+        p = mu / 2
+        af = np.clip(p, 0.0, 1.0)
+        use = (o > 0) & ((n - o) / n <= max_miss + 1e-15) & (af > 1e-12) & (af < 1 - 1e-12) & \
+              (p >= 1e-8) & (p <= 1 - 1e-8)
+        cd = (r[:, byte_i] >> shift_i[None, :]) & 3            # (chunk, k) codes of the idx samples
+        g = dos_of_code[cd].astype(np.float64)
+        ob = obs_of_code[cd].astype(bool) & use[:, None]
+        sd = np.sqrt(2 * p * (1 - p))
+        z = np.where(ob, (g - mu[:, None]) / np.where(use, sd, 1.0)[:, None], 0.0)
+        A += z.T @ z
+        of = ob.astype(np.float64)
+        N += of.T @ of
+        used += int(use.sum())
+    K = np.where(N > 0, A / np.maximum(N, 1), 0.0)
+    return K, N.astype(np.int64), used

@@ -1590,25 +1590,41 @@ double orc_cis_gene(int32_t model, const double* y, const double* X, int32_t n, 
     prep_lmm(Yt.data(), Xt.data(), lam, n, p, -1.0, lmm);
     if (delta_out) *delta_out = lmm.delta;
   }
-  std::vector<double> gs(n), gt(n);
   std::vector<int> cached;
   int n_tested = 0;
   double best = 2.0;
-  auto test_one = [&](const double* g, const PrepLm* L, const PrepLmm* M) -> Hit {
+  // LMM: Q^T g of each window SNP (test_lmm forms it per test, model_lmm.cpp:170;
+  // the same deterministic FP64 sum is memoised here, so every permutation
+  // reuses bit-identical rotated genotypes instead of recomputing them)
+  // LM: g_s and g_s^T g_s of each window SNP (residualize_g depends only on g
+  // and X, not on the permuted phenotype; memoised the same way)
+  std::vector<double> rot(static_cast<size_t>(c) * n), gtg_of(c);
+  std::vector<char> have_rot(c, 0);
+  auto test_one = [&](int s, const PrepLm* L, const PrepLmm* M) -> Hit {
+    const double* g = W + static_cast<size_t>(s) * n;
     if (L) {
-      const double gtg = residualize_g(*L, g, gs.data());
+      double* gsr = rot.data() + static_cast<size_t>(s) * n;
+      if (!have_rot[s]) {
+        gtg_of[s] = residualize_g(*L, g, gsr);
+        have_rot[s] = 1;
+      }
+      const double gtg = gtg_of[s];
       double gty = 0;
-      for (int i = 0; i < n; ++i) gty += gs[i] * L->ys[i];
+      for (int i = 0; i < n; ++i) gty += gsr[i] * L->ys[i];
       if (gtg < 1e-12) { Hit h; h.p = 1.0; return h; }
       return hit_from_gty(*L, gtg, gty);
     }
-    for (int r = 0; r < n; ++r) {
-      double s = 0;
-      const double* q = Q + static_cast<size_t>(r) * n;
-      for (int i = 0; i < n; ++i) s += q[i] * g[i];
-      gt[r] = s;
+    double* gr = rot.data() + static_cast<size_t>(s) * n;
+    if (!have_rot[s]) {
+      for (int r = 0; r < n; ++r) {
+        double acc = 0;
+        const double* q = Q + static_cast<size_t>(r) * n;
+        for (int i = 0; i < n; ++i) acc += q[i] * g[i];
+        gr[r] = acc;
+      }
+      have_rot[s] = 1;
     }
-    return test_lmm_gtil(*M, gt.data());
+    return test_lmm_gtil(*M, gr);
   };
   for (int s = 0; s < c; ++s) {
     const double* g = W + static_cast<size_t>(s) * n;
@@ -1616,7 +1632,7 @@ double orc_cis_gene(int32_t model, const double* y, const double* X, int32_t n, 
     Hit h;
     h.p = kNaN;
     if (std::isfinite(subset_maf_or_nan(g, n, &maf)))
-      h = test_one(g, model == 0 ? &lm : nullptr, model == 0 ? nullptr : &lmm);
+      h = test_one(s, model == 0 ? &lm : nullptr, model == 0 ? nullptr : &lmm);
     if (w_p) { w_beta[s] = h.beta; w_se[s] = h.se; w_stat[s] = h.stat; w_p[s] = h.p; }
     if (!std::isfinite(h.p)) continue;
     ++n_tested;
@@ -1653,8 +1669,7 @@ double orc_cis_gene(int32_t model, const double* y, const double* X, int32_t n, 
     }
     double minp = 1.0;
     for (int s : cached) {
-      const double* g = W + static_cast<size_t>(s) * n;
-      const Hit h = test_one(g, model == 0 ? &lmb : nullptr, model == 0 ? nullptr : &lmmb);
+      const Hit h = test_one(s, model == 0 ? &lmb : nullptr, model == 0 ? nullptr : &lmmb);
       if (std::isfinite(h.p) && h.p < minp) minp = h.p;
     }
     perm_min_p[b] = minp;

@@ -420,3 +420,27 @@ def test_beta_approx_moment_fallback_and_filtering():
     # observed p clamped to [1e-300, 1 - 1e-16]
     pb0, a, b = O.beta_approx(y[None, :], [0.0])
     assert pb0[0] == pytest.approx(special.betainc(a[0], b[0], 1e-300), rel=1e-10)
+
+
+def test_grm_subblock_matches_oracle_grm():
+    """synth.grm_subblock (the streamed sub-block used by the full-size GRM
+    checks) equals the oracle's GRM restricted to the same samples, N exact,
+    with missing calls, a monomorphic and an all-missing SNP, and n not a
+    multiple of 4."""
+    rng = np.random.default_rng(3)
+    for n in (37, 64, 130):
+        geno, _ = synth.draw_genotypes(rng, 3000, n, 0.0, 0.5, 0.05)
+        geno[0, :] = 0
+        geno[1, :] = -1
+        geno[2, : n // 2 + 3] = -1
+        rows = synth.pack_bed(geno)
+        dos, miss, keep, _ = O.decode_bed(rows, n)
+        K, N, used = O.grm(dos, miss, keep)
+        idx = np.sort(rng.choice(n, 11, replace=False))
+        Ks, Ns, us = synth.grm_subblock(rows, n, idx, chunk=700)
+        Nf = np.zeros((n, n), np.int64)
+        Nf[np.tril_indices(n)] = N
+        Nf = np.tril(Nf) + np.tril(Nf, -1).T
+        assert us == used
+        assert np.abs(K[np.ix_(idx, idx)] - Ks).max() < 1e-13
+        assert np.array_equal(Nf[np.ix_(idx, idx)], Ns)

@@ -15,8 +15,8 @@ HIPCC_FLAGS = [
 ]
 
 
-HOST_SRC = [os.path.join(HERE, "host", "eqtl_host.cpp"), os.path.join(HERE, "host", "vcf_source.cpp"),
-            os.path.join(HERE, "host", "main.cpp")]
+HOST_SRC = [os.path.join(HERE, "host", "eqtl_host.cpp"), os.path.join(HERE, "host", "inputs.cpp"),
+            os.path.join(HERE, "host", "vcf_source.cpp"), os.path.join(HERE, "host", "main.cpp")]
 CLI = os.path.join(HERE, "eqtl")
 
 

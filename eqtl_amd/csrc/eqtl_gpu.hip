@@ -981,7 +981,26 @@ int eqtl_gpu_eigh(eqtl_gpu_ctx* c, const double* K, int32_t n, double* Q_out, do
   if (hinfo != 0) return fail(EQTL_E_SOLVER, "GRM eigen decomposition failed");
   for (auto& v : lam) v = std::max(v, 0.0);  // cwiseMax(0), model_lmm.cpp:111
   HIPC(hipMemcpyAsync(c->lam64.p, lam.data(), sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-  if (Q_out) HIPC(hipMemcpyAsync(Q_out, c->Q64.p, sizeof(double) * n * n, hipMemcpyDeviceToHost, c->stream));
+  // Eigenvector signs: each column is flipped so that sum_i u_i Q_ic > 0 with
+  // the fixed weights u_i = 1 + frac(0.6180339887498949 (i + 1)) / 2. The
+  // Wald tests do not depend on the signs; the LMM permutation streams do
+  // (they shuffle eigen-coordinates), and this convention makes them a
+  // function of K alone.
+  std::vector<double> Qh(static_cast<size_t>(n) * n);
+  HIPC(hipMemcpyAsync(Qh.data(), c->Q64.p, sizeof(double) * Qh.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  for (int cc = 0; cc < n; ++cc) {
+    double* q = Qh.data() + static_cast<size_t>(cc) * n;
+    double sgn = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double f = 0.6180339887498949 * (i + 1);
+      sgn += q[i] * (1.0 + 0.5 * (f - std::floor(f)));
+    }
+    if (sgn < 0)
+      for (int i = 0; i < n; ++i) q[i] = -q[i];
+  }
+  HIPC(hipMemcpyAsync(c->Q64.p, Qh.data(), sizeof(double) * Qh.size(), hipMemcpyHostToDevice, c->stream));
+  if (Q_out) std::memcpy(Q_out, Qh.data(), sizeof(double) * Qh.size());
   if (lam_out) std::memcpy(lam_out, lam.data(), sizeof(double) * n);
   HIPC(hipStreamSynchronize(c->stream));
   return upload_basis(c, n);

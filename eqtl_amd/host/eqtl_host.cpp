@@ -3,8 +3,6 @@
 #include "eqtl_host.hpp"
 #include "pheno_norm.hpp"
 
-#include <getopt.h>
-
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -57,11 +55,6 @@ std::vector<std::string> split_char(const std::string& s, char c) {
   return out;
 }
 
-static std::vector<std::string> split_line(const std::string& line) {  // pheno.cpp:11-15
-  if (line.find('\t') != std::string::npos) return split_char(line, '\t');
-  return split_ws(line);
-}
-
 std::string chrom_key(const std::string& chrom) {  // util.cpp:68-83
   std::string s = chrom;
   if (s.size() >= 3 && std::tolower(static_cast<unsigned char>(s[0])) == 'c' &&
@@ -97,131 +90,6 @@ std::string model_str(Model m) {
     case Model::Glmm: return "glmm";
   }
   return "?";
-}
-
-// ============================================================ options (options.cpp:132-302)
-static std::vector<Model> parse_models(const std::string& s) {
-  std::vector<Model> out;
-  for (auto& p : split_char(s, ',')) {
-    const auto t = trim(p);
-    if (t.empty()) continue;
-    if (t == "lm") out.push_back(Model::Lm);
-    else if (t == "lmm") out.push_back(Model::Lmm);
-    else if (t == "glm") out.push_back(Model::Glm);
-    else if (t == "glmm") die("--model glmm is not provided by this engine (lm, glm, lmm)");
-    else die("invalid --model value: " + t + " (lm|glm|lmm|glmm)");
-  }
-  if (out.empty()) die("empty --model");
-  return out;
-}
-
-static Mode parse_mode(const std::string& s) {
-  if (s == "cis") return Mode::Cis;
-  if (s == "trans") return Mode::Trans;
-  if (s == "all") return Mode::All;
-  if (s == "gw" || s == "gwas") return Mode::Gw;
-  die("invalid --mode: " + s + " (cis|trans|all|gw)");
-}
-
-int parse_options(int argc, char** argv, Options& opt) {
-  static struct option long_opts[] = {
-      {"vcf", required_argument, 0, 'v'},       {"bfile", required_argument, 0, 'b'},
-      {"pheno", required_argument, 0, 'e'},     {"out", required_argument, 0, 'o'},
-      {"gff", required_argument, 0, 'g'},       {"covar", required_argument, 0, 'c'},
-      {"grm", required_argument, 0, 'k'},       {"gff-id-key", required_argument, 0, 1001},
-      {"make-grm", no_argument, 0, 1002},       {"mode", required_argument, 0, 'm'},
-      {"model", required_argument, 0, 1003},    {"window", required_argument, 0, 'w'},
-      {"pval-cis", required_argument, 0, 1004}, {"pval-trans", required_argument, 0, 1005},
-      {"miss-hand", required_argument, 0, 1006}, {"max-miss", required_argument, 0, 1012},
-      {"maf", required_argument, 0, 1013},      {"thread", required_argument, 0, 't'},
-      {"threads", required_argument, 0, 't'},   {"perm", required_argument, 0, 1008},
-      {"permutations", required_argument, 0, 1008}, {"perm-trans-thr", required_argument, 0, 1020},
-      {"perm-trans-top", required_argument, 0, 1021}, {"seed", required_argument, 0, 1009},
-      {"disable-beta-approx", no_argument, 0, 1010}, {"help", no_argument, 0, 'h'},
-      {"fast", no_argument, 0, 1007},           {"pheno-norm", required_argument, 0, 1022},
-      {"precision", required_argument, 0, 2001}, {"delta", required_argument, 0, 2002},
-      {"device", required_argument, 0, 2003},   {0, 0, 0, 0}};
-  if (argc <= 1) return 2;
-  optind = 1;
-  int c, idx;
-  while ((c = getopt_long(argc, argv, "v:b:e:o:g:c:k:m:w:t:h", long_opts, &idx)) != -1) {
-    switch (c) {
-      case 'v': opt.vcf = optarg; break;
-      case 'b': opt.bfile = optarg; break;
-      case 'e': opt.pheno = optarg; break;
-      case 'o': opt.out = optarg; break;
-      case 'g': opt.gff = optarg; break;
-      case 'c': opt.covar = optarg; break;
-      case 'k': opt.grm = optarg; break;
-      case 'm': opt.mode = parse_mode(optarg); break;
-      case 'w': opt.window = std::atoi(optarg); break;
-      case 't': opt.threads = std::max(1, std::atoi(optarg)); break;
-      case 'h': return 2;
-      case 1001: opt.gff_id_key = optarg; break;
-      case 1002: opt.make_grm = true; break;
-      case 1003: opt.models = parse_models(optarg); break;
-      case 1004: opt.pval_cis = std::atof(optarg); break;
-      case 1005: opt.pval_trans = std::atof(optarg); break;
-      case 1006:
-        if (std::string(optarg) == "filter") opt.miss = MissHand::Filter;
-        else if (std::string(optarg) == "impute") opt.miss = MissHand::Impute;
-        else die("invalid --miss-hand (filter|impute)");
-        break;
-      case 1007: opt.fast = true; break;
-      case 1022:  // options.cpp:228-235
-        if (std::string(optarg) == "none") opt.pheno_int = false;
-        else if (std::string(optarg) == "int") opt.pheno_int = true;
-        else die("invalid --pheno-norm (none|int)");
-        break;
-      case 1008: opt.perm = std::atoi(optarg); break;
-      case 1009: opt.seed = std::atoi(optarg); break;
-      case 1010: opt.disable_beta_approx = true; break;
-      case 1012:
-        opt.max_miss = std::atof(optarg);
-        if (opt.max_miss < 0.0 || opt.max_miss > 1.0) die("--max-miss must be in [0,1]");
-        break;
-      case 1013:
-        opt.maf = std::atof(optarg);
-        if (opt.maf < 0.0 || opt.maf > 0.5) die("--maf must be in [0,0.5]");
-        break;
-      case 1020:
-        opt.perm_trans_thr = std::atof(optarg);
-        if (opt.perm_trans_thr <= 0 || opt.perm_trans_thr > 1) die("--perm-trans-thr must be in (0,1]");
-        break;
-      case 1021:
-        opt.perm_trans_top = std::atoi(optarg);
-        if (opt.perm_trans_top < 1) die("--perm-trans-top must be >= 1");
-        break;
-      case 2001:
-        if (std::string(optarg) == "fp32") opt.precision = EQTL_PREC_FP32;
-        else if (std::string(optarg) == "bf16x3") opt.precision = EQTL_PREC_BF16X3;
-        else die("invalid --precision (fp32|bf16x3)");
-        break;
-      case 2002:
-        if (std::string(optarg) == "ref") opt.delta_mode = EQTL_DELTA_REF;
-        else if (std::string(optarg) == "newton") opt.delta_mode = EQTL_DELTA_NEWTON;
-        else die("invalid --delta (ref|newton)");
-        break;
-      case 2003: opt.device = std::atoi(optarg); break;
-      default: return 1;
-    }
-  }
-  if (opt.vcf.empty() == opt.bfile.empty()) {  // options.cpp:275-281
-    if (opt.vcf.empty()) std::cerr << "[E] need exactly one of -v/--vcf or -b/--bfile\n";
-    else std::cerr << "[E] use only one of -v/--vcf or -b/--bfile\n";
-    return 1;
-  }
-  if (!opt.make_grm && opt.pheno.empty()) {
-    std::cerr << "[E] missing -e/--pheno (or use --make-grm)\n";
-    return 1;
-  }
-  if (opt.window < 0) die("--window must be >= 0");
-  if (opt.pheno_int)  // options.cpp:289-300
-    for (Model m : opt.models)
-      if (m == Model::Glm || m == Model::Glmm)
-        die("--pheno-norm int is incompatible with count model " + model_str(m) +
-            " (apply INT only for lm/lmm; count models need raw integer counts)");
-  return 0;
 }
 
 // ============================================================ PLINK bed
@@ -324,142 +192,6 @@ void PlinkBed::read_rows(size_t j0, size_t m, std::vector<uint8_t>& buf) {
   bed_->seekg(static_cast<std::streamoff>(3 + j0 * static_cast<size_t>(bps_)), std::ios::beg);
   bed_->read(reinterpret_cast<char*>(buf.data()), static_cast<std::streamsize>(buf.size()));
   if (static_cast<size_t>(bed_->gcount()) != buf.size()) die("bed fread short at SNP " + std::to_string(j0));
-}
-
-// ============================================================ pheno / covar
-PhenoData load_pheno(const std::string& path) {  // pheno.cpp:17-75
-  std::ifstream in(path);
-  if (!in) die("cannot open pheno: " + path);
-  std::string line;
-  if (!std::getline(in, line)) die("empty pheno: " + path);
-  line = trim(line);
-  if (!line.empty() && line[0] == '#') line = line.substr(1);
-  auto header = split_line(line);
-  if (header.size() < 2) die("pheno header needs sample + >=1 gene");
-  PhenoData P;
-  for (size_t i = 1; i < header.size(); ++i) P.gene_ids.push_back(trim(header[i]));
-  assert_unique(P.gene_ids, "pheno genes");
-  std::vector<std::string> ids;
-  std::vector<std::vector<double>> rows;
-  std::unordered_map<std::string, size_t> first;
-  while (std::getline(in, line)) {
-    line = trim(line);
-    if (line.empty() || line[0] == '#') continue;
-    auto tok = split_line(line);
-    if (tok.size() != header.size()) die("pheno row column mismatch for sample " + (tok.empty() ? "?" : tok[0]));
-    std::vector<double> r(P.gene_ids.size());
-    for (size_t j = 0; j < r.size(); ++j) {
-      const std::string& v = tok[j + 1];
-      r[j] = (v == "NA" || v == "NaN" || v == ".") ? std::numeric_limits<double>::quiet_NaN() : std::stod(v);
-    }
-    const std::string id = trim(tok[0]);
-    auto it = first.find(id);
-    if (it != first.end()) {  // duplicate id: identical payload → keep first (util.cpp dedupe_ids)
-      const auto& a = rows[it->second];
-      for (size_t j = 0; j < a.size(); ++j)
-        if (!((std::isnan(a[j]) && std::isnan(r[j])) || a[j] == r[j]))
-          die("pheno samples: duplicate id with conflicting values: " + id);
-      warn("pheno samples: duplicate id row with identical values (kept first): " + id);
-      continue;
-    }
-    first.emplace(id, rows.size());
-    ids.push_back(id);
-    rows.push_back(std::move(r));
-  }
-  if (ids.empty()) die("no samples in pheno");
-  P.sample_ids = ids;
-  P.Y.resize(ids.size() * P.gene_ids.size());
-  for (size_t r = 0; r < ids.size(); ++r)
-    for (size_t j = 0; j < P.gene_ids.size(); ++j) P.Y[j * ids.size() + r] = rows[r][j];
-  info("pheno: " + std::to_string(P.sample_ids.size()) + " samples, " + std::to_string(P.gene_ids.size()) + " genes");
-  return P;
-}
-
-void slice_pheno(PhenoData& p, const std::vector<std::string>& sample_ids) {  // pheno.cpp:77-87
-  std::unordered_map<std::string, size_t> m;
-  for (size_t i = 0; i < p.sample_ids.size(); ++i) m[p.sample_ids[i]] = i;
-  std::vector<double> Y(sample_ids.size() * p.G());
-  for (size_t i = 0; i < sample_ids.size(); ++i) {
-    auto it = m.find(sample_ids[i]);
-    if (it == m.end()) die("internal: sample missing in pheno slice");
-    for (size_t j = 0; j < p.G(); ++j) Y[j * sample_ids.size() + i] = p.Y[j * p.n() + it->second];
-  }
-  p.sample_ids = sample_ids;
-  p.Y = std::move(Y);
-}
-
-CovData load_covar(const std::string& path, const std::vector<std::string>& sample_ids) {  // pheno.cpp:89-196
-  CovData C;
-  C.sample_ids = sample_ids;
-  const size_t n = sample_ids.size();
-  if (path.empty()) {
-    C.cov_names = {"intercept"};
-    C.p = 1;
-    C.X.assign(n, 1.0);
-    return C;
-  }
-  std::ifstream in(path);
-  if (!in) die("cannot open covar: " + path);
-  std::string line;
-  std::vector<std::string> file_ids, names;
-  std::vector<std::vector<double>> rows;
-  bool first = true;
-  auto is_num = [](const std::string& s) {
-    if (s.empty()) return false;
-    char* end = nullptr;
-    std::strtod(s.c_str(), &end);
-    return end && *end == '\0';
-  };
-  while (std::getline(in, line)) {
-    line = trim(line);
-    if (line.empty() || line[0] == '#') continue;
-    auto tok = split_line(line);
-    if (tok.size() < 2) continue;
-    if (first) {
-      first = false;
-      if (!is_num(tok[1])) {
-        names.assign(tok.begin() + 1, tok.end());
-        continue;
-      }
-    }
-    file_ids.push_back(tok[0]);
-    std::vector<double> r;
-    for (size_t j = 1; j < tok.size(); ++j) r.push_back(std::stod(tok[j]));
-    rows.push_back(std::move(r));
-  }
-  if (rows.empty()) die("no covariates in " + path);
-  const int nc = static_cast<int>(rows[0].size());
-  if (names.empty())
-    for (int j = 0; j < nc; ++j) names.push_back("cov" + std::to_string(j + 1));
-  std::unordered_map<std::string, size_t> fmap;
-  for (size_t i = 0; i < file_ids.size(); ++i) fmap.emplace(file_ids[i], i);
-  std::vector<double> raw(n * nc);
-  for (size_t i = 0; i < n; ++i) {
-    auto it = fmap.find(sample_ids[i]);
-    if (it == fmap.end()) die("sample " + sample_ids[i] + " missing in covar");
-    if (static_cast<int>(rows[it->second].size()) != nc) die("covar row width mismatch");
-    for (int j = 0; j < nc; ++j) raw[j * n + i] = rows[it->second][j];
-  }
-  bool has_int = false;
-  for (int j = 0; j < nc && !has_int; ++j) {
-    bool ok = true;
-    for (size_t i = 0; i < n; ++i)
-      if (std::fabs(raw[j * n + i] - 1.0) > 1e-12) { ok = false; break; }
-    has_int = ok;
-  }
-  if (!has_int) {
-    C.p = nc + 1;
-    C.X.assign(n, 1.0);
-    C.X.insert(C.X.end(), raw.begin(), raw.end());
-    C.cov_names.push_back("intercept");
-    for (auto& s : names) C.cov_names.push_back(s);
-  } else {
-    C.p = nc;
-    C.X = raw;
-    C.cov_names = names;
-  }
-  info("covar: " + std::to_string(C.p) + " columns (incl intercept if added)");
-  return C;
 }
 
 // ============================================================ GFF3 genes
@@ -654,21 +386,24 @@ void write_grm_gcta(const std::string& prefix, const Grm& g) {
   if (!idf) die("cannot write " + prefix + ".grm.id");
   for (const auto& id : g.ids) idf << id << '\t' << id << '\n';
   const uint64_t ntri = static_cast<uint64_t>(n) * (n + 1) / 2;
-  std::vector<float> buf(ntri), nb(ntri);
+  std::vector<float> buf(ntri);
   size_t k = 0;
   for (int i = 0; i < n; ++i)
-    for (int j = 0; j <= i; ++j) {
-      buf[k] = static_cast<float>(g.K[static_cast<size_t>(j) * n + i]);
-      nb[k] = static_cast<float>(g.N_packed.empty() ? 0 : g.N_packed[k]);
-      ++k;
-    }
+    for (int j = 0; j <= i; ++j) buf[k++] = static_cast<float>(g.K[static_cast<size_t>(j) * n + i]);
   std::ofstream bf(prefix + ".grm.bin", std::ios::binary);
   if (!bf) die("cannot write " + prefix + ".grm.bin");
   bf.write(reinterpret_cast<const char*>(buf.data()), static_cast<std::streamsize>(ntri * sizeof(float)));
+  // .grm.N.bin only when the pair counts are known (a loaded GRM has none)
+  if (g.N_packed.empty()) {
+    info("wrote GRM " + prefix + ".grm.id/.grm.bin");
+    return;
+  }
+  if (g.N_packed.size() != ntri) die("GRM N_packed size mismatch on write");
+  for (size_t t = 0; t < ntri; ++t) buf[t] = static_cast<float>(g.N_packed[t]);
   std::ofstream nf(prefix + ".grm.N.bin", std::ios::binary);
   if (!nf) die("cannot write " + prefix + ".grm.N.bin");
-  nf.write(reinterpret_cast<const char*>(nb.data()), static_cast<std::streamsize>(ntri * sizeof(float)));
-  info("GRM written: " + prefix + ".grm.{id,bin,N.bin}");
+  nf.write(reinterpret_cast<const char*>(buf.data()), static_cast<std::streamsize>(ntri * sizeof(float)));
+  info("wrote GRM " + prefix + ".grm.id/.grm.bin/.grm.N.bin");
 }
 
 Grm slice_grm(const Grm& g, const std::vector<std::string>& sample_ids) {

@@ -965,7 +965,23 @@ int32_t orc_vcf_record(const char* line, int64_t len, int32_t ds_float, int32_t 
   return 1;
 }
 
-void orc_eigh(const double* K, int32_t n, double* Q, double* lam) { jacobi_eigh(K, n, Q, lam); }
+// make_lmm_basis's eigendecomposition (model_lmm.cpp:106-113) with the
+// engine's sign convention: column c flipped so that sum_i u_i Q_ic > 0,
+// u_i = 1 + frac(0.6180339887498949 (i + 1)) / 2 (LAPACK leaves the signs
+// arbitrary; the LMM permutation streams depend on them).
+void orc_eigh(const double* K, int32_t n, double* Q, double* lam) {
+  jacobi_eigh(K, n, Q, lam);
+  for (int c = 0; c < n; ++c) {
+    double* q = Q + static_cast<size_t>(c) * n;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double f = 0.6180339887498949 * (i + 1);
+      s += q[i] * (1.0 + 0.5 * (f - std::floor(f)));
+    }
+    if (s < 0)
+      for (int i = 0; i < n; ++i) q[i] = -q[i];
+  }
+}
 
 double orc_reml_negll(double delta, const double* yt, const double* Xt, const double* lam,
                       int32_t n, int32_t p) {

@@ -204,7 +204,8 @@ def test_glm_scan_parity(gpu_ctx, prec, cis):
     assert np.array_equal(st["n_tested"], rs["n_tested"])
     assert np.array_equal(st["n_sig"], rs["n_sig"])
     assert np.array_equal(st["best_snp"], rs["best_snp"])
-    assert rel_err(st["acat_p"], rs["acat_p"]).max() < 1e-2
+    print(f"ACAT vs oracle: max rel {rel_err(st['acat_p'], rs['acat_p']).max():.3e}")
+    assert rel_err(st["acat_p"], rs["acat_p"]).max() < 2e-3
     assert np.all(hits["r2"] == 0)
 
 

@@ -7,8 +7,9 @@ Tolerances (stated per quantity):
       (beta where |t| > 0.1); p relative 1e-3 for p >= 1e-300 (both forms)
   ACAT machinery                        acat_p equals the reference formula
       applied to the engine's own p values (rel 1e-12, or one 2^-54 step)
-  ACAT vs oracle                        relative 1e-2 (terms -1/(pi(1-p)) for p
-      near 1 amplify the contraction's absolute t error)
+  ACAT vs oracle                        relative 2e-3 (terms -1/(pi(1-p)) for p
+      near 1 carry the relative error of |t|: the FP64 numerator refinement
+      below |t| = tau leaves the contraction's relative error of S)
   REML delta (reference search)         f(delta_gpu) - f(delta_oracle) <= 1e-9|f|
       and, where f'' in ln(delta) exceeds 1e-3, delta relative 1e-4 (the search
       is only defined to its own floating-point noise; on flat likelihoods that
@@ -113,7 +114,8 @@ def test_trans_lm_parity(gpu_ctx, prec):
     assert np.array_equal(st["n_sig"], rs["n_sig"])
     assert np.array_equal(st["best_snp"], rs["best_snp"])
     _acat_consistent(st, hits, G)
-    assert rel_err(st["acat_p"], rs["acat_p"]).max() < 1e-2
+    print(f"ACAT vs oracle: max rel {rel_err(st['acat_p'], rs['acat_p']).max():.3e}")
+    assert rel_err(st["acat_p"], rs["acat_p"]).max() < 2e-3
 
 
 def _lmm_run(ctx, ds, rows, Q, lam, win, prec, delta_mode, delta_in=None, pthr=1.0):
@@ -201,7 +203,8 @@ def test_trans_lmm_parity(gpu_ctx, prec):
     assert np.array_equal(st["n_sig"], rs["n_sig"])
     assert np.array_equal(st["best_snp"], rs["best_snp"])
     _acat_consistent(st, hits, G)
-    assert rel_err(st["acat_p"], rs["acat_p"]).max() < 1e-2
+    print(f"ACAT vs oracle: max rel {rel_err(st['acat_p'], rs['acat_p']).max():.3e}")
+    assert rel_err(st["acat_p"], rs["acat_p"]).max() < 2e-3
 
 
 def test_reml_grad_hess_fd(gpu_ctx):

@@ -444,3 +444,30 @@ def test_grm_subblock_matches_oracle_grm():
         assert us == used
         assert np.abs(K[np.ix_(idx, idx)] - Ks).max() < 1e-13
         assert np.array_equal(Nf[np.ix_(idx, idx)], Ns)
+
+
+def test_tabulated_t_tail_against_golden(tmp_path):
+    """The kernels' tabulated t tail (TTail) against the 50-digit mpmath
+    vectors of tests/golden/t_tail.json (1e-11 relative, p >= 1e-300) and
+    against the continued fraction it tabulates on a dense grid (2.5e-12)."""
+    import subprocess
+    exe = str(tmp_path / "t_tail_check")
+    subprocess.check_call(["g++", "-O2", "-std=c++17", os.path.join(os.path.dirname(__file__), "t_tail_check.cpp"),
+                           "-o", exe])
+    rows = json.load(open(os.path.join(GOLD, "t_tail.json")))
+    rng = np.random.default_rng(11)
+    grid = [(df, t) for df in (7.0, 97.0, 983.0, 1978.0) for t in np.concatenate([rng.uniform(0, 6, 300),
+                                                                                   rng.uniform(6, 80, 300)])]
+    pairs = [(float(r["df"]), float(r["t"])) for r in rows] + grid
+    inp = "".join(f"{df:.17g} {t:.17g}\n" for df, t in pairs)
+    out = subprocess.run([exe], input=inp, capture_output=True, text=True, check=True).stdout.split()
+    vals = np.array(out, dtype=float).reshape(-1, 2)
+    gold = np.array([float(r["p"]) for r in rows])
+    tab = vals[: len(rows), 0]
+    ok = gold >= 1e-300
+    rel = np.abs(tab[ok] - gold[ok]) / gold[ok]
+    assert rel.max() < 1e-11, rel.max()
+    tg, cf = vals[len(rows):, 0], vals[len(rows):, 1]
+    okg = cf >= 1e-300
+    relg = np.abs(tg[okg] - cf[okg]) / cf[okg]
+    assert relg.max() < 2.5e-12, relg.max()

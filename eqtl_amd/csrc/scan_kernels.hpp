@@ -444,7 +444,12 @@ __device__ inline bool refine_wanted(const WaldArgs& a, bool ok, double gg, doub
 
 // The whole wave computes each flagged lane's FP64 dot product together: one
 // 2n-double read per refined test and no divergence. Every lane of the wave
-// must call this (wave-uniform control flow).
+// must call this (wave-uniform control flow). The operands are read in rounds
+// of kRefineRound samples with every load of a round issued before the first
+// FMA, so a refined test costs about one memory latency per round (n = 2,000:
+// two rounds) instead of one per 64 samples.
+constexpr int kRefineUnroll = 16;
+constexpr int kRefineRound = 64 * kRefineUnroll;
 __device__ inline double wave_refine(const WaldArgs& a, bool cand, int64_t j, int gene) {
   unsigned long long mask = __ballot(cand);
   const int lane = threadIdx.x & 63;
@@ -457,7 +462,18 @@ __device__ inline double wave_refine(const WaldArgs& a, bool cand, int64_t j, in
     const double* x = a.A64 + static_cast<size_t>(js) * a.n_pad;
     const double* v = a.vref + static_cast<size_t>(gs) * a.n;
     double s = 0.0;
-    for (int i = lane; i < a.n; i += 64) s = fma(x[i], v[i], s);
+    for (int base = 0; base < a.n; base += kRefineRound) {
+      double xv[kRefineUnroll], vv[kRefineUnroll];
+#pragma unroll
+      for (int u = 0; u < kRefineUnroll; ++u) {
+        const int i = base + lane + 64 * u;
+        const bool in = i < a.n;
+        xv[u] = in ? x[i] : 0.0;
+        vv[u] = in ? v[i] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kRefineUnroll; ++u) s = fma(xv[u], vv[u], s);
+    }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (lane == src) mine = s;

@@ -47,8 +47,11 @@ static_assert(w8_lds_bytes(34) <= kLdsCap && w8_a_stages(23) == 3 && w8_a_stages
 
 // One k-step of one wave: this wave's rows (a_rd, its row pair in the current
 // stage) and the gene tile (b_cur) are read from LDS, the copies of later
-// operands are started, and the MFMAs cover blocks [B0, B0 + NQH). SQ: block
-// 0 (the weights d, against the squared rows) is in this half.
+// operands are started, and the MFMAs cover blocks [B0, B0 + NQH). Block 0
+// (the weights d, against the squared rows) is split by row block so that
+// both halves square and split the same number of rows: half 0 runs it for
+// row block 0 (in acc[0][0]; acc[1][0] stays zero), half 1 for row block 1
+// (in acc0).
 // The four half-1 waves issue all copies: their share of the next gene tile
 // (b_nxt, chunks r, r + 4, ...), then their row pair of a later stage (a_wr,
 // from a_src[rb]); the half-0 waves, which also compute the squared rows,
@@ -56,9 +59,9 @@ static_assert(w8_lds_bytes(34) <= kLdsCap && w8_a_stages(23) == 3 && w8_a_stages
 // halves or spreading them over the k-step.)
 // The LDS pointers are __restrict__ (disjoint regions), so the compiler does
 // not drain the copies in flight before this k-step's LDS reads.
-template <int PREC, int NQ, int NQH, int B0, bool SQ, int kAS>
+template <int PREC, int NQ, int NQH, int B0, int kAS>
 __device__ __attribute__((always_inline)) inline void w8_kstep(
-    f32x4 (&acc)[kRB][NQH], f32x8 (&an)[kRB], const unsigned char* __restrict__ a_rd,
+    f32x4 (&acc)[kRB][NQH], f32x4& acc0, f32x8 (&an)[kRB], const unsigned char* __restrict__ a_rd,
     unsigned char* __restrict__ a_wr, const float* const (&a_src)[kRB], bool a_reg_next,
     const unsigned char* __restrict__ b_cur, unsigned char* __restrict__ b_nxt, const unsigned char* b_src, int r,
     int lane) {
@@ -97,13 +100,11 @@ __device__ __attribute__((always_inline)) inline void w8_kstep(
   }
   __builtin_amdgcn_sched_barrier(0);
   if constexpr (PREC == EQTL_PREC_FP32) {
-    f32x8 sq[kRB];
-    if constexpr (SQ) {
+    constexpr int rs = B0 == 0 ? 0 : 1;  // the row block whose squares this half contracts
+    f32x8 sq;
 #pragma unroll
-      for (int rb = 0; rb < kRB; ++rb)
-#pragma unroll
-        for (int t = 0; t < 8; ++t) sq[rb].v[t] = ac[rb].v[t] * ac[rb].v[t];
-    }
+    for (int t = 0; t < 8; ++t) sq.v[t] = ac[rs].v[t] * ac[rs].v[t];
+    if constexpr (B0 != 0) acc0 = mfma_f32x8(sq, load8(reinterpret_cast<const float*>(b_cur + lane * 32)), acc0);
     f32x8 bn = load8(reinterpret_cast<const float*>(b_cur + (B0 * 64 + lane) * 32));
 #pragma unroll
     for (int b = 0; b < NQH; ++b) {
@@ -111,8 +112,11 @@ __device__ __attribute__((always_inline)) inline void w8_kstep(
       if (b + 1 < NQH) bn = load8(reinterpret_cast<const float*>(b_cur + ((B0 + b + 1) * 64 + lane) * 32));
 #pragma unroll
       for (int rb = 0; rb < kRB; ++rb) {
-        if (SQ && b == 0) acc[rb][b] = mfma_f32x8(sq[rb], bv, acc[rb][b]);
-        else acc[rb][b] = mfma_f32x8(ac[rb], bv, acc[rb][b]);
+        if (B0 == 0 && b == 0) {
+          if (rb == 0) acc[0][0] = mfma_f32x8(sq, bv, acc[0][0]);
+        } else {
+          acc[rb][b] = mfma_f32x8(ac[rb], bv, acc[rb][b]);
+        }
       }
     }
   } else {
@@ -133,16 +137,26 @@ __device__ __attribute__((always_inline)) inline void w8_kstep(
       }
     };
     load_group(0, 0);
-    bf16x8 ah[kRB], al[kRB], sh[kRB], sl[kRB];
+    bf16x8 f0h, f0l;  // half 1: block 0's fragments
+    if constexpr (B0 != 0) {
+      f0h = *reinterpret_cast<const bf16x8*>(b_cur + lane * 16);
+      f0l = *reinterpret_cast<const bf16x8*>(b_cur + NQ * 1024 + lane * 16);
+    }
+    constexpr int rs = B0 == 0 ? 0 : 1;  // the row block whose squares this half contracts
+    bf16x8 ah[kRB], al[kRB], sh, sl;
 #pragma unroll
-    for (int rb = 0; rb < kRB; ++rb) {
-      split_bf16(ac[rb], ah[rb], al[rb]);
-      if constexpr (SQ) {
-        f32x8 sv;
+    for (int rb = 0; rb < kRB; ++rb) split_bf16(ac[rb], ah[rb], al[rb]);
+    {
+      f32x8 sv;
 #pragma unroll
-        for (int t = 0; t < 8; ++t) sv.v[t] = ac[rb].v[t] * ac[rb].v[t];
-        split_bf16(sv, sh[rb], sl[rb]);
-      }
+      for (int t = 0; t < 8; ++t) sv.v[t] = ac[rs].v[t] * ac[rs].v[t];
+      split_bf16(sv, sh, sl);
+    }
+    if constexpr (B0 != 0) {
+      __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+      acc0 = mfma_bf16(sh, f0h, acc0);
+      acc0 = mfma_bf16(sh, f0l, acc0);
+      acc0 = mfma_bf16(sl, f0h, acc0);
     }
 #pragma unroll
     for (int g = 0; g < kGroups; ++g) {
@@ -159,9 +173,11 @@ __device__ __attribute__((always_inline)) inline void w8_kstep(
             const bf16x8& y = term == 1 ? fl[g & 1][c] : fh[g & 1][c];
 #pragma unroll
             for (int rb = 0; rb < kRB; ++rb) {
-              const bool sqr = SQ && b == 0;
-              const bf16x8& x = term == 2 ? (sqr ? sl[rb] : al[rb]) : (sqr ? sh[rb] : ah[rb]);
-              acc[rb][b] = mfma_bf16(x, y, acc[rb][b]);
+              if (B0 == 0 && b == 0) {
+                if (rb == 0) acc[0][0] = mfma_bf16(term == 2 ? sl : sh, y, acc[0][0]);
+              } else {
+                acc[rb][b] = mfma_bf16(term == 2 ? al[rb] : ah[rb], y, acc[rb][b]);
+              }
             }
           }
         }
@@ -174,7 +190,7 @@ __device__ __attribute__((always_inline)) inline void w8_kstep(
 // fold[(3 c + k) * kW8Tests + row * 16 + gene], k = 0 gg, 1 sum of squares of
 // the covariate projections, 2 num, each partial over this half's blocks.
 // gsrc: the group's gene tiles (k-step s at gsrc + s * NQ * 2 KiB).
-template <int PREC, int NQ, int NQH, int B0, bool SQ>
+template <int PREC, int NQ, int NQH, int B0>
 __device__ __attribute__((always_inline)) inline void w8_tile(const WaldArgs& a, unsigned char* smem,
                                                               const unsigned char* gsrc, int64_t row0, int c, int r,
                                                               int lane) {
@@ -199,6 +215,7 @@ __device__ __attribute__((always_inline)) inline void w8_tile(const WaldArgs& a,
   for (int rb = 0; rb < kRB; ++rb)
 #pragma unroll
     for (int b = 0; b < NQH; ++b) acc[rb][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  f32x4 acc0 = f32x4{0.f, 0.f, 0.f, 0.f};  // half 1: block 0 of row block 1
   f32x8 an[kRB];
   // prologue (half 1): gene tile of k-step 0 and the first row stages
   if constexpr (B0 != 0) {
@@ -229,7 +246,7 @@ __device__ __attribute__((always_inline)) inline void w8_tile(const WaldArgs& a,
         (kAS > 0 && sr < nks) ? abuf + (sr % (kAS > 0 ? kAS : 1)) * kAStageBytes + pair_off : nullptr;
     const unsigned char* const a_rd = kAS > 0 ? abuf + (s % (kAS > 0 ? kAS : 1)) * kAStageBytes + pair_off : nullptr;
     unsigned char* const b_nxt = s + 1 < nks ? (((s + 1) & 1) ? buf1 : buf0) : nullptr;
-    w8_kstep<PREC, NQ, NQH, B0, SQ, kAS>(acc, an, a_rd, a_wr, src, kAS == 0 && sr < nks, (s & 1) ? buf1 : buf0,
+    w8_kstep<PREC, NQ, NQH, B0, kAS>(acc, acc0, an, a_rd, a_wr, src, kAS == 0 && sr < nks, (s & 1) ? buf1 : buf0,
                                          b_nxt, gsrc + static_cast<size_t>(s + 1) * tile_bytes, r, lane);
     // every wave's copies for the next k-step must have landed (hence the
     // barrier); with three stages a half-1 wave leaves its row copies of
@@ -250,10 +267,11 @@ __device__ __attribute__((always_inline)) inline void w8_tile(const WaldArgs& a,
       for (int b = 0; b < NQH; ++b) {
         const int gb = B0 + b;
         const double v = acc[rb][b][q];
-        if (SQ && gb == 0) gg = v;
+        if (gb == 0) gg = v;
         else if (gb < nq - 1) ss += v * v;
         else if (gb == nq - 1) num = v;
       }
+      if (B0 != 0 && rb == 1) gg = acc0[q];
       const int row = 32 * r + 16 * rb + 4 * (lane >> 4) + q;
       const int t = row * 16 + gl;
       fold[(3 * c + 0) * kW8Tests + t] = gg;
@@ -417,8 +435,8 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
     const int64_t row0 = static_cast<int64_t>(tile) * kTileRows;
     __syncthreads();  // the previous tile's epilogue is done with ri, fold and the gene partials
     w8_row_info<MODEL>(a, ri, row0);
-    if (c == 0) w8_tile<PREC, NQ, NQ0, 0, true>(a, smem, gsrc, row0, c, r, lane);
-    else w8_tile<PREC, NQ, NQ1, NQ0, false>(a, smem, gsrc, row0, c, r, lane);
+    if (c == 0) w8_tile<PREC, NQ, NQ0, 0>(a, smem, gsrc, row0, c, r, lane);
+    else w8_tile<PREC, NQ, NQ1, NQ0>(a, smem, gsrc, row0, c, r, lane);
     __syncthreads();  // fold complete
     w8_epilogue<MODEL>(a, gi, ri, tv, gstate, wpart, row0, [&](int t, double& gg, double& S, double& num) {
       gg = fold[0 * kW8Tests + t] + fold[3 * kW8Tests + t];

@@ -40,7 +40,7 @@ EXPORTS = [
     "eqtl_gpu_set_timing", "eqtl_gpu_get_timing", "eqtl_gpu_synth_bed", "eqtl_gpu_copy_rows",
     "eqtl_gpu_synchronize", "eqtl_gpu_selftest", "eqtl_gpu_upload_bed", "eqtl_gpu_cis_perm",
     "eqtl_gpu_beta_approx", "eqtl_gpu_set_geno_format", "eqtl_gpu_vcf_decode", "eqtl_gpu_glm_prep",
-    "eqtl_gpu_set_test_subset", "eqtl_gpu_scan_hits", "eqtl_gpu_grm_accumulate_noalloc",
+    "eqtl_gpu_set_test_subset", "eqtl_gpu_scan_hits", "eqtl_gpu_grm_accumulate_noalloc", "eqtl_gpu_cis_tuning",
 ]
 
 _lib = None
@@ -336,6 +336,13 @@ class Context:
         _check(lib().eqtl_gpu_cis_perm(self._c, ctypes.c_int32(precision), ctypes.c_int32(B), ctypes.c_int32(seed),
                                        _p(gh), _p(bp), ctypes.c_int32(delta_mode), _p(pm), _p(pe)))
         return pm, pe
+
+    def cis_tuning(self):
+        """(tiles per work item, timed flags) of the cis autotuner for the three window-size classes."""
+        t = np.zeros(3, dtype=np.int32)
+        d = np.zeros(3, dtype=np.int32)
+        _check(lib().eqtl_gpu_cis_tuning(self._c, _p(t), _p(d)))
+        return t, d
 
     def beta_approx(self, perm_min_p, obs_min_p):
         """(p_beta, shape1, shape2) per gene from G x B permutation minima."""

@@ -18,6 +18,7 @@
 #include "../../include/eqtl_gpu.h"
 #include "numerics.hpp"
 #include "reml_kernels.hpp"
+#include "reml_device.hpp"
 #include "scan_kernels.hpp"
 #include "vcf_kernels.hpp"
 #include "glm_kernels.hpp"
@@ -54,6 +55,10 @@ __global__ void k_fill_u64(unsigned long long* p, int count, unsigned long long 
   for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = v;
 }
 
+__global__ void k_fill_u32(unsigned int* p, int count, unsigned int v) {
+  for (int i = threadIdx.x; i < count; i += blockDim.x) p[i] = v;
+}
+
 // |t| below which a test's numerator is recomputed in FP64 (near-null
 // refinement in the Wald kernels). Chosen from each form's absolute t error
 // (FP32 ~1e-6, BF16X3 ~2e-5): above the threshold the ACAT term's relative
@@ -73,8 +78,14 @@ constexpr double kPolishKappa = 1e-2;
 // demand), and how many are polished per launch (g~ scratch: chunk x n doubles).
 constexpr int64_t kPolishCap = int64_t(1) << 16;
 constexpr int64_t kPolishChunk = 4096;
-// cis work items span at most this many 128-SNP tiles
-constexpr int kCisItemTiles = 4;
+// Cis work items: a gene group's span of 128-SNP tiles in a block is cut into
+// items of at most cis_tiles[class] tiles, the class being the span's length
+// (<= 4 tiles, <= 32 tiles, longer). The autotuner times the candidates of a
+// class on the first block that has groups of that class (default 4).
+constexpr int kCisClasses = 3;
+constexpr int kCisClassMax[kCisClasses - 1] = {4, 32};
+constexpr int kCisCand[kCisClasses][3] = {{1, 2, 4}, {2, 4, 8}, {4, 8, 16}};
+constexpr int kCisItemTilesDefault = 4;
 
 thread_local std::string g_err;
 
@@ -136,6 +147,12 @@ struct TimingSlot {
 struct RemlWork {
   DBuf P, dgrid, Dg, logvg, XD, Y2, CAg, Ainv, logdet, okg, Bg, C0, F, best;
   DBuf W, Z, CA, CB, S, ek, ed, Fo;
+  // device-resident reference search: per-gene state, evaluation points and
+  // values, and the captured launch sequence with the arguments it was
+  // captured for (replayed while they match)
+  DBuf st, dek, ded, dF;
+  hipGraphExec_t graph = nullptr;
+  std::vector<const void*> graph_key, eager_key;
 };
 
 struct eqtl_gpu_ctx {
@@ -184,7 +201,12 @@ struct eqtl_gpu_ctx {
   std::vector<int64_t> h_gene_cs, h_gene_ce;
   // cis scope: per-gene global SNP range [jlo, jhi) holding its window (empty: jlo == jhi)
   std::vector<int64_t> win_lo, win_hi;
-  DBuf work, item_first;
+  DBuf work, item_first, item_order, queue;
+  // cis autotuner: tiles per work item per window-size class, which classes
+  // have been timed, and the scan configuration they were timed for
+  int cis_tiles[kCisClasses] = {kCisItemTilesDefault, kCisItemTilesDefault, kCisItemTilesDefault};
+  int cis_timed[kCisClasses] = {0, 0, 0};
+  std::vector<int> cis_tune_key;
   // resident genotypes for cis scans
   DBuf bed_res;
   int64_t bed_m = 0;
@@ -332,15 +354,33 @@ const uint8_t* stage_rows(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, 
   return c->rows.as<uint8_t>();
 }
 
+// Cis launches are persistent: as many workgroups as can be resident (per-CU
+// occupancy of the kernel at its LDS size, times the CUs), at most one per
+// item, after the item queue is zeroed (a fill kernel, no memset).
+int persistent_grid(eqtl_gpu_ctx* c, const WaldArgs& a, const void* fn, int threads, size_t lds, int* occ,
+                    dim3* grid) {
+  if (!a.work) return EQTL_OK;
+  if (*occ < 0) {
+    int nb = 0;
+    HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, threads, lds));
+    *occ = std::max(nb, 1);
+  }
+  grid->x = std::min<unsigned>(grid->x, static_cast<unsigned>(std::max(c->n_cu, 1) * *occ));
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, a.queue, 1, 0u);
+  return check_launch("k_fill_u32");
+}
+
 template <int PREC>
 int launch_wald_lm(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
   const size_t lds = lm_lds_bytes();
   static bool attr_set = false;
+  static int occ = -1;
   if (!attr_set) {
     HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald_lm<PREC>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     attr_set = true;
   }
+  CHK(persistent_grid(c, a, reinterpret_cast<const void*>(&k_wald_lm<PREC>), 256, lds, &occ, &grid));
   hipLaunchKernelGGL((k_wald_lm<PREC>), grid, dim3(256), lds, c->stream, a);
   return check_launch("k_wald_lm");
 }
@@ -358,12 +398,16 @@ int lmm_nq_pad(int nq) {
 template <int PREC, int NQ, int MODEL>
 int launch_wald_lmm_one(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
   const size_t lds = w8_lds_bytes(NQ);
+  static_assert(w8_lds_bytes(NQ) + 64 <= kLdsCap, "LDS of the LMM Wald kernel with its static item slot");
   static bool attr_set = false;
+  static int occ = -1;
   if (!attr_set) {
     HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald_lmm<PREC, NQ, MODEL>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     attr_set = true;
   }
+  CHK(persistent_grid(c, a, reinterpret_cast<const void*>(&k_wald_lmm<PREC, NQ, MODEL>), kW8Threads, lds, &occ,
+                      &grid));
   hipLaunchKernelGGL((k_wald_lmm<PREC, NQ, MODEL>), grid, dim3(kW8Threads), lds, c->stream, a);
   return check_launch("k_wald_lmm");
 }
@@ -447,6 +491,7 @@ int eqtl_gpu_close(eqtl_gpu_ctx* c) {
   if (!c) return EQTL_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  if (c->rw.graph) (void)hipGraphExecDestroy(c->rw.graph);
   delete c;
   return EQTL_OK;
 }
@@ -676,13 +721,14 @@ static int ensure_blas(eqtl_gpu_ctx* c) {
 
 constexpr int kEvalChunk = 16384;  // evaluations per GEMM batch
 
+// C (m x nn, ldc) = A^T B with A k x m (lda) and B k x nn (ldb): the FP64
+// matrix-core kernel k_dgemm_tn (no library call, so it can be captured).
 static int dgemm_tn(eqtl_gpu_ctx* c, int m, int nn, int k, const double* A, int lda, const double* B, int ldb, double* C,
              int ldc) {
-  const double one = 1.0, zero = 0.0;
-  if (rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, m, nn, k, &one, A, lda, B, ldb,
-                    &zero, C, ldc) != rocblas_status_success)
-    return fail(EQTL_E_HIP, "rocblas_dgemm failed");
-  return EQTL_OK;
+  if (m <= 0 || nn <= 0) return EQTL_OK;
+  const dim3 grid(static_cast<unsigned>((m + kDgT - 1) / kDgT), static_cast<unsigned>((nn + kDgT - 1) / kDgT));
+  hipLaunchKernelGGL(k_dgemm_tn, grid, dim3(256), 0, c->stream, m, nn, k, A, lda, B, ldb, C, ldc);
+  return check_launch("k_dgemm_tn");
 }
 
 // Per-sample outer products of X~ (once per lmm_prep).
@@ -692,6 +738,34 @@ static int reml_setup(eqtl_gpu_ctx* c) {
   hipLaunchKernelGGL(k_reml_outer, dim3((n + 255) / 256, TA), dim3(256), 0, c->stream, c->Xt64.as<double>(), n, p,
                      c->rw.P.as<double>());
   return check_launch("k_reml_outer");
+}
+
+// f-only evaluations at (gene ek[e], delta ed[e]), e < E, with device-side
+// arguments and result F[e]: the weights / b / c0 kernel, the X~^T D X~ GEMM
+// and the bordered LDLT. The workspace (W, CA, CB, S) must hold E evaluations.
+static int enqueue_eval_f(eqtl_gpu_ctx* c, int E, const int32_t* ek, const double* ed, double* F) {
+  const int n = c->n, p = c->p, TA = p * (p + 1) / 2, df = n - p;
+  auto& r = c->rw;
+  // b, c0 and sum ln v come straight from the weights kernel; one GEMM for X~^T D X~
+  if (p <= 16)
+    hipLaunchKernelGGL((k_reml_eval_weights_f<16, 4>), dim3((E + 3) / 4), dim3(64), 0, c->stream,
+                       c->lam64.as<double>(), c->Yt64.as<double>(), c->Xt64.as<double>(), n, p, E, ek, ed,
+                       r.W.as<double>(), r.CB.as<double>(), r.S.as<double>());
+  else
+    hipLaunchKernelGGL((k_reml_eval_weights_f<kRP, 2>), dim3((E + 1) / 2), dim3(64), 0, c->stream,
+                       c->lam64.as<double>(), c->Yt64.as<double>(), c->Xt64.as<double>(), n, p, E, ek, ed,
+                       r.W.as<double>(), r.CB.as<double>(), r.S.as<double>());
+  CHK(check_launch("k_reml_eval_weights_f"));
+  CHK(dgemm_tn(c, TA, E, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA));
+  if (p < 32) {
+    const size_t lds = 2 * reml_eval_f_bytes(p);
+    hipLaunchKernelGGL(k_reml_eval_f<32>, dim3((E + 1) / 2), dim3(64), lds, c->stream, r.CA.as<double>(),
+                       r.CB.as<double>(), r.S.as<double>(), E, TA, p, df, F);
+  } else {
+    hipLaunchKernelGGL(k_reml_eval_f<64>, dim3(E), dim3(64), reml_eval_f_bytes(p), c->stream, r.CA.as<double>(),
+                       r.CB.as<double>(), r.S.as<double>(), E, TA, p, df, F);
+  }
+  return check_launch("k_reml_eval_f");
 }
 
 // f (and f', f'' in ln delta) at (gene ek[e], delta ed[e]) for all e.
@@ -729,28 +803,10 @@ static int reml_eval(eqtl_gpu_ctx* c, const std::vector<int32_t>& ek, const std:
       CHK(dgemm_tn(c, p, nw * Ec, n, c->Xt64.as<double>(), n, r.Z.as<double>(), n, r.CB.as<double>(), p));
       hipLaunchKernelGGL(k_reml_eval_fgh, dim3(Ec), dim3(64), 0, c->stream, r.CA.as<double>(), r.CB.as<double>(),
                          r.S.as<double>(), Ec, TA, p, df, Fo, Go, Ho);
+      CHK(check_launch("k_reml_eval_fgh"));
     } else {
-      // b, c0 and sum ln v come straight from the weights kernel; one GEMM for X~^T D X~
-      if (p <= 16)
-        hipLaunchKernelGGL((k_reml_eval_weights_f<16, 4>), dim3((Ec + 3) / 4), dim3(64), 0, c->stream,
-                           c->lam64.as<double>(), c->Yt64.as<double>(), c->Xt64.as<double>(), n, p, Ec,
-                           r.ek.as<int32_t>(), r.ed.as<double>(), r.W.as<double>(), r.CB.as<double>(), r.S.as<double>());
-      else
-        hipLaunchKernelGGL((k_reml_eval_weights_f<kRP, 2>), dim3((Ec + 1) / 2), dim3(64), 0, c->stream,
-                           c->lam64.as<double>(), c->Yt64.as<double>(), c->Xt64.as<double>(), n, p, Ec,
-                           r.ek.as<int32_t>(), r.ed.as<double>(), r.W.as<double>(), r.CB.as<double>(), r.S.as<double>());
-      CHK(check_launch("k_reml_eval_weights_f"));
-      CHK(dgemm_tn(c, TA, Ec, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA));
-      if (p < 32) {
-        const size_t lds = 2 * reml_eval_f_bytes(p);
-        hipLaunchKernelGGL(k_reml_eval_f<32>, dim3((Ec + 1) / 2), dim3(64), lds, c->stream, r.CA.as<double>(),
-                           r.CB.as<double>(), r.S.as<double>(), Ec, TA, p, df, Fo);
-      } else {
-        hipLaunchKernelGGL(k_reml_eval_f<64>, dim3(Ec), dim3(64), reml_eval_f_bytes(p), c->stream,
-                           r.CA.as<double>(), r.CB.as<double>(), r.S.as<double>(), Ec, TA, p, df, Fo);
-      }
+      CHK(enqueue_eval_f(c, Ec, r.ek.as<int32_t>(), r.ed.as<double>(), Fo));
     }
-    CHK(check_launch("k_reml_eval"));
     HIPC(hipMemcpyAsync(F + e0, Fo, sizeof(double) * Ec, hipMemcpyDeviceToHost, c->stream));
     if (deriv) {
       HIPC(hipMemcpyAsync(Gd + e0, Go, sizeof(double) * Ec, hipMemcpyDeviceToHost, c->stream));
@@ -761,9 +817,11 @@ static int reml_eval(eqtl_gpu_ctx* c, const std::vector<int32_t>& ek, const std:
   return EQTL_OK;
 }
 
-// Reference grid (delta = 1, then 1e-5 * 2^k <= 1e5) for all genes at once.
-static int reml_grid(eqtl_gpu_ctx* c, std::vector<double>& best_d) {
-  const int n = c->n, p = c->p, G = c->G, TA = p * (p + 1) / 2, df = n - p;
+// Reference grid (delta = 1, then 1e-5 * 2^k <= 1e5) for all genes at once:
+// workspace and the grid's deltas (reml_grid_setup, outside any capture), then
+// the launches that leave each gene's best grid delta in rw.best (device).
+static int reml_grid_setup(eqtl_gpu_ctx* c, int G, int* ng_out) {
+  const int n = c->n, p = c->p, TA = p * (p + 1) / 2;
   std::vector<double> dg{1.0};
   for (double d = 1e-5; d <= 1e5; d *= 2.0) dg.push_back(d);
   const int ng = static_cast<int>(dg.size());
@@ -782,6 +840,14 @@ static int reml_grid(eqtl_gpu_ctx* c, std::vector<double>& best_d) {
   CHK(ensure(r.F, sizeof(double) * static_cast<size_t>(ng) * G));
   CHK(ensure(r.best, sizeof(double) * 2 * G));
   HIPC(hipMemcpyAsync(r.dgrid.p, dg.data(), sizeof(double) * ng, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  *ng_out = ng;
+  return EQTL_OK;
+}
+
+static int reml_grid_enqueue(eqtl_gpu_ctx* c, int G, int ng) {
+  const int n = c->n, p = c->p, TA = p * (p + 1) / 2, df = n - p;
+  auto& r = c->rw;
   hipLaunchKernelGGL(k_reml_grid_weights, dim3(ng), dim3(256), 0, c->stream, c->lam64.as<double>(), n,
                      r.dgrid.as<double>(), r.Dg.as<double>(), r.logvg.as<double>());
   CHK(check_launch("k_reml_grid_weights"));
@@ -804,102 +870,126 @@ static int reml_grid(eqtl_gpu_ctx* c, std::vector<double>& best_d) {
   CHK(check_launch("k_reml_grid_f"));
   hipLaunchKernelGGL(k_reml_grid_argmin, dim3((G + 255) / 256), dim3(256), 0, c->stream, r.F.as<double>(),
                      r.dgrid.as<double>(), ng, G, r.best.as<double>(), r.best.as<double>() + G);
-  CHK(check_launch("k_reml_grid_argmin"));
+  return check_launch("k_reml_grid_argmin");
+}
+
+static int reml_grid(eqtl_gpu_ctx* c, std::vector<double>& best_d) {
+  const int G = c->G;
+  int ng = 0;
+  CHK(reml_grid_setup(c, G, &ng));
+  CHK(reml_grid_enqueue(c, G, ng));
   best_d.resize(G);
-  HIPC(hipMemcpyAsync(best_d.data(), r.best.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipMemcpyAsync(best_d.data(), c->rw.best.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   return EQTL_OK;
 }
 
-// Per-gene delta for all genes. mode EQTL_DELTA_REF: the reference search
-// (grid, 40 golden-section steps on [best/4, best*4] clipped to [1e-8, 1e6],
-// then at most 8 central-difference Newton steps with the reference's
-// acceptance rules), each stage run for every gene in lock-step; mode
-// EQTL_DELTA_NEWTON: the same grid and bracket, then safeguarded Newton on
-// ln delta with the analytic derivatives and step halving.
-static int reml_search(eqtl_gpu_ctx* c, int mode, std::vector<double>& delta) {
+// The reference search (optimize_delta, model_lmm.cpp:39-86) for genes
+// [0, G) entirely on the device: grid, 40 golden-section steps on
+// [best / 4, best * 4] clipped to [1e-8, 1e6], the midpoint, then 8 rounds of
+// central-difference Newton steps with the reference's acceptance rules (a
+// gene that stops keeps its delta; its later evaluations are not used). The
+// per-gene decisions run in k_ref_* kernels between the evaluations, so the
+// whole search is one fixed launch sequence with no host round trip.
+static int reml_ref_enqueue(eqtl_gpu_ctx* c, int G, int ng, double* delta) {
+  auto& r = c->rw;
+  const dim3 gb(static_cast<unsigned>((G + 255) / 256)), tb(256);
+  RemlRefState* st = r.st.as<RemlRefState>();
+  int32_t* ek = r.dek.as<int32_t>();
+  double* ed = r.ded.as<double>();
+  double* F = r.dF.as<double>();
+  hipLaunchKernelGGL(k_iota, dim3(static_cast<unsigned>((2 * G + 255) / 256)), tb, 0, c->stream, ek, G, 2);
+  CHK(check_launch("k_iota"));
+  CHK(reml_grid_enqueue(c, G, ng));
+  hipLaunchKernelGGL(k_ref_init, gb, tb, 0, c->stream, r.best.as<double>(), G, st);
+  const double phi = (1.0 + std::sqrt(5.0)) / 2.0;
+  for (int it = 0; it < 40; ++it) {
+    hipLaunchKernelGGL(k_ref_golden_points, gb, tb, 0, c->stream, st, G, phi, ed);
+    CHK(enqueue_eval_f(c, 2 * G, ek, ed, F));
+    hipLaunchKernelGGL(k_ref_golden_update, gb, tb, 0, c->stream, st, G, ed, F);
+  }
+  hipLaunchKernelGGL(k_ref_mid, gb, tb, 0, c->stream, st, G, ed);
+  CHK(enqueue_eval_f(c, G, ek, ed, F));
+  hipLaunchKernelGGL(k_ref_mid_done, gb, tb, 0, c->stream, st, G, F);
+  for (int it = 0; it < 8; ++it) {
+    hipLaunchKernelGGL(k_ref_newton_points, gb, tb, 0, c->stream, st, G, ed);
+    CHK(enqueue_eval_f(c, 2 * G, ek, ed, F));
+    hipLaunchKernelGGL(k_ref_newton_propose, gb, tb, 0, c->stream, st, G, F, ed);
+    CHK(enqueue_eval_f(c, G, ek, ed, F));
+    hipLaunchKernelGGL(k_ref_newton_accept, gb, tb, 0, c->stream, st, G, F);
+  }
+  hipLaunchKernelGGL(k_ref_out, gb, tb, 0, c->stream, st, G, delta);
+  return check_launch("k_ref");
+}
+
+// Runs reml_ref_enqueue: eagerly on the first use of a configuration (G and
+// every buffer it touches), captured in a hipGraph on its second use and
+// replayed from then on while the configuration is unchanged (batches of equal
+// size, e.g. the gene-level permutations). The sequence holds no memset and no
+// host copy.
+static int reml_ref_device(eqtl_gpu_ctx* c, int G, double* delta) {
+  const int n = c->n, p = c->p, TA = p * (p + 1) / 2;
+  auto& r = c->rw;
+  const size_t E2 = 2 * static_cast<size_t>(G);
+  CHK(ensure(r.W, sizeof(double) * n * E2));
+  CHK(ensure(r.CA, sizeof(double) * TA * E2));
+  CHK(ensure(r.CB, sizeof(double) * p * E2));
+  CHK(ensure(r.S, sizeof(double) * kNumScal * E2));
+  CHK(ensure(r.st, sizeof(RemlRefState) * G));
+  CHK(ensure(r.dek, sizeof(int32_t) * E2));
+  CHK(ensure(r.ded, sizeof(double) * E2));
+  CHK(ensure(r.dF, sizeof(double) * E2));
+  int ng = 0;
+  CHK(reml_grid_setup(c, G, &ng));
+  const std::vector<const void*> key = {
+      reinterpret_cast<const void*>(static_cast<intptr_t>(G)), reinterpret_cast<const void*>(static_cast<intptr_t>(n)),
+      reinterpret_cast<const void*>(static_cast<intptr_t>(p)), delta, c->lam64.p, c->Yt64.p, c->Xt64.p, r.P.p,
+      r.W.p, r.CA.p, r.CB.p, r.S.p, r.st.p, r.dek.p, r.ded.p, r.dF.p, r.dgrid.p, r.Dg.p, r.logvg.p, r.XD.p, r.Y2.p,
+      r.CAg.p, r.Ainv.p, r.logdet.p, r.okg.p, r.Bg.p, r.C0.p, r.F.p, r.best.p};
+  if (!r.graph || r.graph_key != key) {
+    if (r.graph) {
+      HIPC(hipGraphExecDestroy(r.graph));
+      r.graph = nullptr;
+    }
+    r.graph_key.clear();
+    if (r.eager_key != key) {
+      r.eager_key = key;
+      return reml_ref_enqueue(c, G, ng, delta);
+    }
+    HIPC(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = reml_ref_enqueue(c, G, ng, delta);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc != EQTL_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(EQTL_E_HIP, std::string("REML graph capture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&r.graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      r.graph = nullptr;
+      return fail(EQTL_E_HIP, std::string("REML graph instantiate: ") + hipGetErrorString(ei));
+    }
+    r.graph_key = key;
+  }
+  HIPC(hipGraphLaunch(r.graph, c->stream));
+  return EQTL_OK;
+}
+
+// Per-gene delta for all genes, mode EQTL_DELTA_NEWTON: the reference's grid
+// and bracket, then safeguarded Newton on ln delta with the analytic
+// derivatives and step halving (host-driven rounds over the active genes).
+// Mode EQTL_DELTA_REF runs on the device (reml_ref_device).
+static int reml_newton(eqtl_gpu_ctx* c, std::vector<double>& delta) {
   const int G = c->G, n = c->n, p = c->p;
   delta.assign(G, 1.0);
   if (n - p <= 0) return EQTL_OK;
   CHK(reml_setup(c));
   std::vector<double> best;
   CHK(reml_grid(c, best));
-  std::vector<double> lo(G), hi(G);
-  for (int k = 0; k < G; ++k) {
-    lo[k] = std::max(best[k] / 4.0, 1e-8);
-    hi[k] = std::min(best[k] * 4.0, 1e6);
-  }
   std::vector<int32_t> ek;
   std::vector<double> ed, F, Gd, H;
-  if (mode == EQTL_DELTA_REF) {
-    const double phi = (1.0 + std::sqrt(5.0)) / 2.0;
-    ek.resize(2 * G);
-    ed.resize(2 * G);
-    F.resize(2 * G);
-    for (int k = 0; k < G; ++k) ek[k] = ek[G + k] = k;
-    for (int it = 0; it < 40; ++it) {
-      for (int k = 0; k < G; ++k) {
-        ed[k] = hi[k] - (hi[k] - lo[k]) / phi;
-        ed[G + k] = lo[k] + (hi[k] - lo[k]) / phi;
-      }
-      CHK(reml_eval(c, ek, ed, false, F.data(), nullptr, nullptr));
-      for (int k = 0; k < G; ++k) {
-        if (F[k] < F[G + k]) hi[k] = ed[G + k];
-        else lo[k] = ed[k];
-      }
-    }
-    std::vector<double> d(G), cur(G);
-    std::vector<int32_t> all(G);
-    for (int k = 0; k < G; ++k) { d[k] = 0.5 * (lo[k] + hi[k]); all[k] = k; }
-    CHK(reml_eval(c, all, d, false, cur.data(), nullptr, nullptr));
-    std::vector<char> active(G, 1);
-    for (int it = 0; it < 8; ++it) {
-      std::vector<int32_t> ks;
-      for (int k = 0; k < G; ++k)
-        if (active[k]) ks.push_back(k);
-      if (ks.empty()) break;
-      const int A = static_cast<int>(ks.size());
-      ek.assign(2 * A, 0);
-      ed.assign(2 * A, 0.0);
-      std::vector<double> hstep(A);
-      for (int q = 0; q < A; ++q) {
-        const int k = ks[q];
-        hstep[q] = std::max(d[k] * 1e-4, 1e-9);
-        ek[q] = ek[A + q] = k;
-        ed[q] = d[k] - hstep[q];
-        ed[A + q] = d[k] + hstep[q];
-      }
-      F.assign(2 * A, 0.0);
-      CHK(reml_eval(c, ek, ed, false, F.data(), nullptr, nullptr));
-      std::vector<int32_t> k2;
-      std::vector<double> dn2, step2;
-      for (int q = 0; q < A; ++q) {
-        const int k = ks[q];
-        const double h = hstep[q], fm = F[q], fp = F[A + q];
-        const double g1 = (fp - fm) / (2.0 * h);
-        const double g2 = (fp - 2.0 * cur[k] + fm) / (h * h);
-        if (!(g2 > 1e-12)) { active[k] = 0; continue; }
-        const double step = g1 / g2;
-        const double dn = d[k] - step;
-        if (!(dn > 1e-8 && dn < 1e6)) { active[k] = 0; continue; }
-        k2.push_back(k);
-        dn2.push_back(dn);
-        step2.push_back(step);
-      }
-      if (k2.empty()) break;
-      std::vector<double> vn(k2.size());
-      CHK(reml_eval(c, k2, dn2, false, vn.data(), nullptr, nullptr));
-      for (size_t q = 0; q < k2.size(); ++q) {
-        const int k = k2[q];
-        if (!(vn[q] < cur[k])) { active[k] = 0; continue; }
-        d[k] = dn2[q];
-        cur[k] = vn[q];
-        if (std::fabs(step2[q]) < d[k] * 1e-8) active[k] = 0;
-      }
-    }
-    delta = d;
-    return EQTL_OK;
-  }
   // safeguarded Newton on theta = ln delta from the grid's best point, over
   // the search's whole range [1e-8, 1e6] (the reference's final Newton steps
   // may also leave the golden-section bracket: a likelihood still falling at
@@ -1077,10 +1167,16 @@ static int lmm_prep_core(eqtl_gpu_ctx* c, int G, int mode, const double* delta_i
     HIPC(hipMemcpyAsync(c->delta.p, delta_in, sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
   } else {
     Timed t(c, "reml");
-    std::vector<double> dl;
-    CHK(reml_search(c, mode, dl));
-    HIPC(hipMemcpyAsync(c->delta.p, dl.data(), sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
-    HIPC(hipStreamSynchronize(c->stream));
+    if (mode == EQTL_DELTA_REF && n - p > 0) {
+      CHK(reml_setup(c));
+      CHK(reml_ref_device(c, G, c->delta.as<double>()));
+    } else {
+      std::vector<double> dl;
+      if (mode == EQTL_DELTA_REF) dl.assign(G, 1.0);
+      else CHK(reml_newton(c, dl));
+      HIPC(hipMemcpyAsync(c->delta.p, dl.data(), sizeof(double) * G, hipMemcpyHostToDevice, c->stream));
+      HIPC(hipStreamSynchronize(c->stream));
+    }
     CHK(t.done());
   }
   {
@@ -1327,6 +1423,140 @@ static int scan_begin_impl(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t
   return EQTL_OK;
 }
 
+// Cis work items of one scan block over [snp0, snp0 + m): per gene group the
+// 128-row tiles holding a window SNP of its genes, cut into items of at most
+// tiles[class] tiles (class of the group's span, kCisClassMax); only groups
+// of class only_class when it is >= 0. items: {group, tile_lo, tile_hi} with
+// a group's items consecutive (first[g] .. first[g + 1]); order: the queue
+// order, items with more tiles first.
+struct CisItems {
+  std::vector<int32_t> items, first, order;
+  int classes_present[kCisClasses] = {0, 0, 0};
+};
+static int cis_class(int span_tiles) {
+  int k = 0;
+  while (k < kCisClasses - 1 && span_tiles > kCisClassMax[k]) ++k;
+  return k;
+}
+static void cis_items(const eqtl_gpu_ctx* c, int64_t snp0, int64_t m, int n_groups, const int* tiles, int only_class,
+                      CisItems& out) {
+  out.items.clear();
+  out.first.assign(n_groups + 1, 0);
+  for (int q = 0; q < kCisClasses; ++q) out.classes_present[q] = 0;
+  for (int g = 0; g < n_groups; ++g) {
+    out.first[g] = static_cast<int32_t>(out.items.size() / 3);
+    int64_t lo = INT64_MAX, hi = -1;
+    for (int k = 16 * g; k < std::min(16 * g + 16, c->G); ++k)
+      if (c->win_lo[k] < c->win_hi[k]) {
+        lo = std::min(lo, c->win_lo[k]);
+        hi = std::max(hi, c->win_hi[k]);
+      }
+    const int64_t l = std::max(lo, snp0) - snp0, h = std::min(hi, snp0 + m) - snp0;
+    if (l >= h) continue;
+    const int t0 = static_cast<int>(l / kTileRows), t1 = static_cast<int>((h + kTileRows - 1) / kTileRows);
+    const int cls = cis_class(t1 - t0);
+    out.classes_present[cls] = 1;
+    if (only_class >= 0 && cls != only_class) continue;
+    const int step = tiles[cls];
+    for (int t = t0; t < t1; t += step) {
+      out.items.push_back(g);
+      out.items.push_back(t);
+      out.items.push_back(std::min(t + step, t1));
+    }
+  }
+  out.first[n_groups] = static_cast<int32_t>(out.items.size() / 3);
+  const int n = out.first[n_groups];
+  out.order.resize(n);
+  for (int i = 0; i < n; ++i) out.order[i] = i;
+  std::stable_sort(out.order.begin(), out.order.end(), [&](int x, int y) {
+    return out.items[3 * x + 2] - out.items[3 * x + 1] > out.items[3 * y + 2] - out.items[3 * y + 1];
+  });
+}
+
+static int upload_items(eqtl_gpu_ctx* c, const CisItems& ci) {
+  const size_t n = ci.order.size();
+  CHK(ensure(c->work, sizeof(int32_t) * std::max<size_t>(ci.items.size(), 3)));
+  CHK(ensure(c->item_first, sizeof(int32_t) * ci.first.size()));
+  CHK(ensure(c->item_order, sizeof(int32_t) * std::max<size_t>(n, 1)));
+  CHK(ensure(c->queue, sizeof(unsigned int)));
+  if (!ci.items.empty())
+    HIPC(hipMemcpyAsync(c->work.p, ci.items.data(), sizeof(int32_t) * ci.items.size(), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPC(hipMemcpyAsync(c->item_first.p, ci.first.data(), sizeof(int32_t) * ci.first.size(), hipMemcpyHostToDevice,
+                      c->stream));
+  if (n) HIPC(hipMemcpyAsync(c->item_order.p, ci.order.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice, c->stream));
+  CHK(ensure(c->partial, sizeof(GeneAcc) * std::max<size_t>(n, 1) * 16));
+  return EQTL_OK;
+}
+
+// The cis autotuner: for each window-size class present in this block and not
+// yet timed for the current scan configuration, one launch per candidate item
+// size over that class's items alone (hit and polish output off, counters
+// cleared afterwards), event-timed; the fastest candidate is kept (ties: the
+// larger items). The tuned sizes change only the work split, not the tests.
+static int cis_autotune(eqtl_gpu_ctx* c, const WaldArgs& base, int64_t snp0, int64_t m, int n_groups) {
+  const std::vector<int> key = {c->scan_model, c->scan_prec, c->nq_pad, c->n_pad};
+  if (c->cis_tune_key != key) {
+    c->cis_tune_key = key;
+    for (int q = 0; q < kCisClasses; ++q) {
+      c->cis_tiles[q] = kCisItemTilesDefault;
+      c->cis_timed[q] = 0;
+    }
+  }
+  CisItems probe;
+  cis_items(c, snp0, m, n_groups, c->cis_tiles, -1, probe);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  for (int q = 0; q < kCisClasses; ++q) {
+    if (c->cis_timed[q] || !probe.classes_present[q]) continue;
+    if (!e0) {
+      HIPC(hipEventCreate(&e0));
+      HIPC(hipEventCreate(&e1));
+    }
+    float best_ms = 0;
+    int best = c->cis_tiles[q];
+    int rc = EQTL_OK;
+    for (int v = 0; v < 3 && rc == EQTL_OK; ++v) {
+      int tiles[kCisClasses];
+      for (int u = 0; u < kCisClasses; ++u) tiles[u] = c->cis_tiles[u];
+      tiles[q] = kCisCand[q][v];
+      CisItems ci;
+      cis_items(c, snp0, m, n_groups, tiles, q, ci);
+      rc = upload_items(c, ci);
+      if (rc != EQTL_OK) break;
+      WaldArgs t = base;
+      t.work = c->work.as<int32_t>();
+      t.order = c->item_order.as<int32_t>();
+      t.queue = c->queue.as<unsigned int>();
+      t.n_items = static_cast<int32_t>(ci.order.size());
+      t.partial = c->partial.as<GeneAcc>();
+      t.hit_cap = 0;
+      t.polish_cap = 0;
+      if (hipEventRecord(e0, c->stream) != hipSuccess) rc = fail(EQTL_E_HIP, "hipEventRecord failed");
+      if (rc == EQTL_OK) rc = launch_wald(c, t, dim3(static_cast<unsigned>(t.n_items)));
+      if (rc == EQTL_OK && hipEventRecord(e1, c->stream) != hipSuccess) rc = fail(EQTL_E_HIP, "hipEventRecord failed");
+      if (rc == EQTL_OK && hipEventSynchronize(e1) != hipSuccess) rc = fail(EQTL_E_HIP, "hipEventSynchronize failed");
+      float ms = 0;
+      if (rc == EQTL_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+      if (rc == EQTL_OK && (v == 0 || ms <= best_ms)) {
+        best_ms = ms;
+        best = kCisCand[q][v];
+      }
+    }
+    if (rc != EQTL_OK) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      return rc;
+    }
+    c->cis_tiles[q] = best;
+    c->cis_timed[q] = 1;
+  }
+  if (e0) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  return EQTL_OK;
+}
+
 int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, int64_t snp0, int64_t m,
                         int32_t bps, eqtl_hit* hits, int64_t hit_cap, int64_t* n_hits) {
   if (!c || !rows || m <= 0 || snp0 < 0) return fail(EQTL_E_ARG, "bad scan_block arguments");
@@ -1382,39 +1612,16 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   }
   const int n_groups = c->G_pad / 16;
   // cis: work items {group, tile_lo, tile_hi} over the tiles of this block
-  // that hold some window SNP of the group's genes
+  // that hold some window SNP of the group's genes (built after the tuner)
   const bool cis = c->scan_scope == EQTL_SCOPE_CIS;
   int n_items = 0;
   if (cis) {
-    std::vector<int32_t> items, first(n_groups + 1, 0);
-    for (int g = 0; g < n_groups; ++g) {
-      first[g] = static_cast<int32_t>(items.size() / 3);
-      int64_t lo = INT64_MAX, hi = -1;
-      for (int k = 16 * g; k < std::min(16 * g + 16, c->G); ++k)
-        if (c->win_lo[k] < c->win_hi[k]) {
-          lo = std::min(lo, c->win_lo[k]);
-          hi = std::max(hi, c->win_hi[k]);
-        }
-      const int64_t l = std::max(lo, snp0) - snp0, h = std::min(hi, snp0 + m) - snp0;
-      if (l >= h) continue;
-      const int t0 = static_cast<int>(l / kTileRows), t1 = static_cast<int>((h + kTileRows - 1) / kTileRows);
-      for (int t = t0; t < t1; t += kCisItemTiles) {
-        items.push_back(g);
-        items.push_back(t);
-        items.push_back(std::min(t + kCisItemTiles, t1));
-      }
-    }
-    first[n_groups] = static_cast<int32_t>(items.size() / 3);
-    n_items = first[n_groups];
-    if (n_items == 0) {
+    CisItems ci;
+    cis_items(c, snp0, m, n_groups, c->cis_tiles, -1, ci);
+    if (ci.order.empty()) {
       if (n_hits) *n_hits = 0;
       return EQTL_OK;
     }
-    CHK(ensure(c->work, sizeof(int32_t) * items.size()));
-    CHK(ensure(c->item_first, sizeof(int32_t) * first.size()));
-    HIPC(hipMemcpyAsync(c->work.p, items.data(), sizeof(int32_t) * items.size(), hipMemcpyHostToDevice, c->stream));
-    HIPC(hipMemcpyAsync(c->item_first.p, first.data(), sizeof(int32_t) * first.size(), hipMemcpyHostToDevice,
-                        c->stream));
   }
   // work split: enough workgroups to fill the chip, SNP tiles strided per split
   const int n_groups_pad = static_cast<int>(round_up(n_groups, 8));
@@ -1481,7 +1688,22 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   a.tau_hi = glm ? kGlmTauHi : kRefineTauHi;
   a.kappa = kPolishKappa;
   a.cis = cis ? 1 : 0;
-  a.work = cis ? c->work.as<int32_t>() : nullptr;
+  a.work = nullptr;
+  a.order = nullptr;
+  a.queue = nullptr;
+  a.n_items = 0;
+  if (cis) {
+    CHK(cis_autotune(c, a, snp0, m, n_groups));
+    CisItems ci;
+    cis_items(c, snp0, m, n_groups, c->cis_tiles, -1, ci);
+    CHK(upload_items(c, ci));
+    n_items = static_cast<int>(ci.order.size());
+    a.work = c->work.as<int32_t>();
+    a.order = c->item_order.as<int32_t>();
+    a.queue = c->queue.as<unsigned int>();
+    a.n_items = n_items;
+    a.partial = c->partial.as<GeneAcc>();
+  }
   int64_t got = 0, npol = 0;
   for (int pass = 0;; ++pass) {
     a.hits = c->hits.as<eqtl_hit>();
@@ -1613,6 +1835,15 @@ int eqtl_gpu_scan_hits(eqtl_gpu_ctx* c, int64_t first, eqtl_hit* hits, int64_t c
   const int64_t take = first >= total ? 0 : std::min(cap, total - first);
   if (take > 0) std::memcpy(hits, c->hit_stage.data() + first, sizeof(eqtl_hit) * take);
   *n_out = take;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_cis_tuning(eqtl_gpu_ctx* c, int32_t* tiles, int32_t* timed) {
+  if (!c || !tiles || !timed) return fail(EQTL_E_ARG, "bad cis_tuning arguments");
+  for (int q = 0; q < kCisClasses; ++q) {
+    tiles[q] = c->cis_tiles[q];
+    timed[q] = c->cis_timed[q];
+  }
   return EQTL_OK;
 }
 

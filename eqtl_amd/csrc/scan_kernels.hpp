@@ -372,10 +372,26 @@ struct WaldArgs {
   // scope: 0 trans (a gene's cis window is excluded), 1 cis (only the window
   // is tested). Cis launches run over work items {group, tile_lo, tile_hi}
   // (128-row tiles of the block) instead of (group, split) pairs; partial
-  // states are then written per item.
+  // states are then written per item. The workgroups of a cis launch are
+  // persistent: each takes the next queue position from *queue (zeroed before
+  // the launch) until n_items are taken; position q runs item order[q]
+  // (longest items first).
   int32_t cis;
   const int32_t* work;
+  const int32_t* order;
+  unsigned int* queue;
+  int32_t n_items;
 };
+
+// Next cis work item of a persistent workgroup (-1: the queue is drained).
+// Every thread of the workgroup must call it; slot is a workgroup-shared int.
+__device__ inline int next_item(const struct WaldArgs& a, int* slot) {
+  __syncthreads();  // the previous item is done with slot and with LDS
+  if (threadIdx.x == 0) *slot = static_cast<int>(atomicAdd(a.queue, 1u));
+  __syncthreads();
+  const int q = *slot;
+  return q < a.n_items ? a.order[q] : -1;
+}
 
 constexpr int kRB = 2;          // 16-row blocks per wave
 // s_waitcnt immediate (gfx9 encoding): vmcnt <= 2 * kRB (bits 3:0, 15:14),
@@ -645,35 +661,12 @@ __device__ __attribute__((always_inline)) inline void lm_kstep(
 // K8: fused trans/cis LM scan (gty = g^T y_s per test, FP64 Wald from it and
 // the per-SNP FP64 g^T g). One workgroup = one group of 16 genes x a strided
 // set of 128-SNP tiles, 4 waves x 32 rows; the group's one-block gene tile and
-// the SNP rows are staged in LDS as in the LMM kernel.
+// the SNP rows are staged in LDS as in the LMM kernel. lm_item runs one
+// (group, tiles) piece of work.
 template <int PREC>
-// Three waves per SIMD (registers capped at 168, two row stages so that three
-// workgroups fit in LDS): the FP64 epilogue bounds this kernel and needs the
-// extra waves to hide its latency (measured 10% faster than two waves with
-// three stages).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_wald_lm(WaldArgs a) {
+__device__ __attribute__((always_inline)) inline void lm_item(const WaldArgs& a, unsigned char* smem, int group,
+                                                              int split, int t_lo, int t_hi, int t_step) {
   constexpr int kAS = kLmStages;
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  // XCD-aware mapping: workgroups are dealt round-robin to the 8 XCDs; the
-  // n_splits consecutive workgroups of one XCD share a gene group, so the
-  // group's column blocks are served from that XCD's L2.
-  int group, split, t_lo, t_hi, t_step;
-  if (a.work) {  // cis: one work item per workgroup
-    group = a.work[3 * blockIdx.x];
-    t_lo = a.work[3 * blockIdx.x + 1];
-    t_hi = a.work[3 * blockIdx.x + 2];
-    split = blockIdx.x;
-    t_step = 1;
-  } else {
-    const int wg = blockIdx.x;
-    const int xcd = wg & 7;
-    const int jx = wg >> 3;
-    group = (jx / a.n_splits) * 8 + xcd;
-    split = jx % a.n_splits;
-    t_lo = split;
-    t_hi = (a.n_tiles + 1) >> 1;  // 128-row tiles
-    t_step = a.n_splits;
-  }
   if (group >= a.n_groups) return;
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -793,6 +786,29 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
     const size_t slot = a.work ? static_cast<size_t>(split) * 16 : static_cast<size_t>(split) * a.G_pad + group * 16;
     a.partial[slot + threadIdx.x] = t;
   }
+}
+
+template <int PREC>
+// Three waves per SIMD (registers capped at 168, two row stages so that three
+// workgroups fit in LDS): the FP64 epilogue bounds this kernel and needs the
+// extra waves to hide its latency (measured 10% faster than two waves with
+// three stages).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_wald_lm(WaldArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  if (a.work) {  // cis: persistent workgroups over the item queue
+    __shared__ int slot;
+    for (int it = next_item(a, &slot); it >= 0; it = next_item(a, &slot))
+      lm_item<PREC>(a, smem, a.work[3 * it], it, a.work[3 * it + 1], a.work[3 * it + 2], 1);
+    return;
+  }
+  // XCD-aware mapping: workgroups are dealt round-robin to the 8 XCDs; the
+  // n_splits consecutive workgroups of one XCD share a gene group, so the
+  // group's column blocks are served from that XCD's L2.
+  const int wg = blockIdx.x;
+  const int xcd = wg & 7;
+  const int jx = wg >> 3;
+  const int split = jx % a.n_splits;
+  lm_item<PREC>(a, smem, (jx / a.n_splits) * 8 + xcd, split, split, (a.n_tiles + 1) >> 1, a.n_splits);
 }
 
 // ---------------------------------------------------------------------------

@@ -299,30 +299,32 @@ __device__ inline GeneAcc shfl_xor_acc(const GeneAcc& x, int m) {
   return y;
 }
 
-// Work of one workgroup: gene group and SNP tiles (XCD-aware in trans scope:
+// Work of one workgroup: gene group and SNP tiles. Trans scope, XCD-aware:
 // the n_splits consecutive workgroups of one XCD share a gene group, so its
-// column blocks are served from that XCD's L2; cis scope: one work item).
+// column blocks are served from that XCD's L2. Cis scope: work item it of the
+// queue (its partial state goes to slot it).
 struct W8Work {
   int group, split, t_lo, t_hi, t_step;
 };
-__device__ inline W8Work w8_work(const WaldArgs& a) {
+__device__ inline W8Work w8_work_trans(const WaldArgs& a) {
   W8Work w;
-  if (a.work) {
-    w.group = a.work[3 * blockIdx.x];
-    w.t_lo = a.work[3 * blockIdx.x + 1];
-    w.t_hi = a.work[3 * blockIdx.x + 2];
-    w.split = blockIdx.x;
-    w.t_step = 1;
-  } else {
-    const int wg = blockIdx.x;
-    const int xcd = wg & 7;
-    const int jx = wg >> 3;
-    w.group = (jx / a.n_splits) * 8 + xcd;
-    w.split = jx % a.n_splits;
-    w.t_lo = w.split;
-    w.t_hi = (a.n_tiles + 1) >> 1;
-    w.t_step = a.n_splits;
-  }
+  const int wg = blockIdx.x;
+  const int xcd = wg & 7;
+  const int jx = wg >> 3;
+  w.group = (jx / a.n_splits) * 8 + xcd;
+  w.split = jx % a.n_splits;
+  w.t_lo = w.split;
+  w.t_hi = (a.n_tiles + 1) >> 1;
+  w.t_step = a.n_splits;
+  return w;
+}
+__device__ inline W8Work w8_work_item(const WaldArgs& a, int it) {
+  W8Work w;
+  w.group = a.work[3 * it];
+  w.t_lo = a.work[3 * it + 1];
+  w.t_hi = a.work[3 * it + 2];
+  w.split = it;
+  w.t_step = 1;
   return w;
 }
 
@@ -404,13 +406,13 @@ __device__ inline void w8_row_info(const WaldArgs& a, RowInfo& ri, int64_t row0)
   }
 }
 
-// MODEL: EQTL_MODEL_LMM (A = rotated dosage, Wald t test) or EQTL_MODEL_GLM
-// (A = centred dosage, NB score test); both contract the same column blocks.
+// One piece of work of the kernel below. MODEL: EQTL_MODEL_LMM (A = rotated
+// dosage, Wald t test) or EQTL_MODEL_GLM (A = centred dosage, NB score test);
+// both contract the same column blocks.
 template <int PREC, int NQ, int MODEL>
-__global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_wald_lmm(WaldArgs a) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+__device__ __attribute__((always_inline)) inline void w8_item(const WaldArgs& a, unsigned char* smem,
+                                                              const W8Work& wk) {
   constexpr int NQ0 = (NQ + 1) / 2, NQ1 = NQ - NQ0;
-  const W8Work wk = w8_work(a);
   if (wk.group >= a.n_groups) return;
   const int group = wk.group;
 
@@ -449,6 +451,18 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
         a.work ? static_cast<size_t>(wk.split) * 16 : static_cast<size_t>(wk.split) * a.G_pad + group * 16;
     a.partial[slot + threadIdx.x] = gstate[threadIdx.x];
   }
+}
+
+template <int PREC, int NQ, int MODEL>
+__global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_wald_lmm(WaldArgs a) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  if (a.work) {  // cis: persistent workgroups over the item queue
+    __shared__ int slot;
+    for (int it = next_item(a, &slot); it >= 0; it = next_item(a, &slot))
+      w8_item<PREC, NQ, MODEL>(a, smem, w8_work_item(a, it));
+    return;
+  }
+  w8_item<PREC, NQ, MODEL>(a, smem, w8_work_trans(a));
 }
 
 }  // namespace eqtl_amd

@@ -225,6 +225,14 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_
 /* Hits first .. first+cap-1 (same order) of the last scanned block; *n_out =
  * how many were copied (0 past the end). Valid until the next scan_block. */
 int eqtl_gpu_scan_hits(eqtl_gpu_ctx* ctx, int64_t first, eqtl_hit* hits, int64_t cap, int64_t* n_out);
+/* Work split chosen by the cis autotuner (no counterpart in the reference,
+ * which runs one gene window per thread, scan_cis.cpp:18-163): cis launches
+ * cut a gene group's tiles of 128 SNPs into work items that persistent
+ * workgroups take from a device queue, longest first. For the window-size
+ * classes (group spans of <= 4, <= 32 and more tiles) tiles[3] gets the tiles
+ * per item in use and timed[3] whether the class was timed (1) or still has
+ * the default of 4 (0) for the current scan configuration. */
+int eqtl_gpu_cis_tuning(eqtl_gpu_ctx* ctx, int32_t* tiles, int32_t* timed);
 /* Final per-gene summaries (G entries). */
 int eqtl_gpu_scan_end(eqtl_gpu_ctx* ctx, eqtl_gene_state* state);
 

@@ -150,7 +150,7 @@ struct RemlWork {
   // device-resident reference search: per-gene state, evaluation points and
   // values, and the captured launch sequence with the arguments it was
   // captured for (replayed while they match)
-  DBuf st, dek, ded, dF;
+  DBuf st, dek, ded, dF, cnt, list;
   hipGraphExec_t graph = nullptr;
   std::vector<const void*> graph_key, eager_key;
 };
@@ -723,11 +723,12 @@ constexpr int kEvalChunk = 16384;  // evaluations per GEMM batch
 
 // C (m x nn, ldc) = A^T B with A k x m (lda) and B k x nn (ldb): the FP64
 // matrix-core kernel k_dgemm_tn (no library call, so it can be captured).
+// nn_dev: when given, the column count is read on the device (nn bounds it).
 static int dgemm_tn(eqtl_gpu_ctx* c, int m, int nn, int k, const double* A, int lda, const double* B, int ldb, double* C,
-             int ldc) {
+             int ldc, const int* nn_dev = nullptr) {
   if (m <= 0 || nn <= 0) return EQTL_OK;
   const dim3 grid(static_cast<unsigned>((m + kDgT - 1) / kDgT), static_cast<unsigned>((nn + kDgT - 1) / kDgT));
-  hipLaunchKernelGGL(k_dgemm_tn, grid, dim3(256), 0, c->stream, m, nn, k, A, lda, B, ldb, C, ldc);
+  hipLaunchKernelGGL(k_dgemm_tn, grid, dim3(256), 0, c->stream, m, nn, k, A, lda, B, ldb, C, ldc, nn_dev);
   return check_launch("k_dgemm_tn");
 }
 
@@ -740,30 +741,51 @@ static int reml_setup(eqtl_gpu_ctx* c) {
   return check_launch("k_reml_outer");
 }
 
-// f-only evaluations at (gene ek[e], delta ed[e]), e < E, with device-side
-// arguments and result F[e]: the weights / b / c0 kernel, the X~^T D X~ GEMM
-// and the bordered LDLT. The workspace (W, CA, CB, S) must hold E evaluations.
-static int enqueue_eval_f(eqtl_gpu_ctx* c, int E, const int32_t* ek, const double* ed, double* F) {
+// f-only evaluations at (gene ek[e], delta ed[e]), e < E (or e < *E_dev,
+// counted on the device, E bounding it), with device-side arguments and result
+// F[e]. p <= 22: the fused weighted-Gram kernel k_reml_gram_f (weights formed
+// in registers, CA, CB, c0 and sum ln v in one launch); larger p: the weights
+// kernel, then the X~^T D X~ GEMM. Then the bordered LDLT. The workspace
+// (W for p > 22, CA, CB, S) must hold E evaluations.
+extern "C++" {
+template <int MB, int PB>
+static void launch_gram_f(eqtl_gpu_ctx* c, int E, const int* E_dev, const int32_t* ek, const double* ed) {
+  auto& r = c->rw;
+  const int n = c->n, p = c->p, TA = p * (p + 1) / 2;
+  hipLaunchKernelGGL((k_reml_gram_f<MB, PB>), dim3(static_cast<unsigned>((E + 63) / 64)), dim3(256), 0, c->stream,
+                     r.P.as<double>(), c->Xt64.as<double>(), c->Yt64.as<double>(), c->lam64.as<double>(), n, p, TA, E,
+                     E_dev, ek, ed, r.CA.as<double>(), r.CB.as<double>(), r.S.as<double>(), kNumScal);
+}
+}
+
+static int enqueue_eval_f(eqtl_gpu_ctx* c, int E, const int32_t* ek, const double* ed, double* F,
+                          const int* E_dev = nullptr) {
   const int n = c->n, p = c->p, TA = p * (p + 1) / 2, df = n - p;
   auto& r = c->rw;
-  // b, c0 and sum ln v come straight from the weights kernel; one GEMM for X~^T D X~
-  if (p <= 16)
-    hipLaunchKernelGGL((k_reml_eval_weights_f<16, 4>), dim3((E + 3) / 4), dim3(64), 0, c->stream,
-                       c->lam64.as<double>(), c->Yt64.as<double>(), c->Xt64.as<double>(), n, p, E, ek, ed,
-                       r.W.as<double>(), r.CB.as<double>(), r.S.as<double>());
-  else
+  const int mb = (TA + 15) / 16;
+  if (p <= 22) {
+    if (mb <= 2) launch_gram_f<2, 1>(c, E, E_dev, ek, ed);
+    else if (mb <= 4) launch_gram_f<4, 1>(c, E, E_dev, ek, ed);
+    else if (mb <= 6) launch_gram_f<6, 1>(c, E, E_dev, ek, ed);
+    else if (mb <= 9) launch_gram_f<9, 1>(c, E, E_dev, ek, ed);
+    else if (mb <= 12) launch_gram_f<12, 2>(c, E, E_dev, ek, ed);
+    else launch_gram_f<16, 2>(c, E, E_dev, ek, ed);
+    CHK(check_launch("k_reml_gram_f"));
+  } else {
+    // b, c0 and sum ln v from the weights kernel; one GEMM for X~^T D X~
     hipLaunchKernelGGL((k_reml_eval_weights_f<kRP, 2>), dim3((E + 1) / 2), dim3(64), 0, c->stream,
-                       c->lam64.as<double>(), c->Yt64.as<double>(), c->Xt64.as<double>(), n, p, E, ek, ed,
+                       c->lam64.as<double>(), c->Yt64.as<double>(), c->Xt64.as<double>(), n, p, E, E_dev, ek, ed,
                        r.W.as<double>(), r.CB.as<double>(), r.S.as<double>());
-  CHK(check_launch("k_reml_eval_weights_f"));
-  CHK(dgemm_tn(c, TA, E, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA));
+    CHK(check_launch("k_reml_eval_weights_f"));
+    CHK(dgemm_tn(c, TA, E, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA, E_dev));
+  }
   if (p < 32) {
     const size_t lds = 2 * reml_eval_f_bytes(p);
     hipLaunchKernelGGL(k_reml_eval_f<32>, dim3((E + 1) / 2), dim3(64), lds, c->stream, r.CA.as<double>(),
-                       r.CB.as<double>(), r.S.as<double>(), E, TA, p, df, F);
+                       r.CB.as<double>(), r.S.as<double>(), E, E_dev, TA, p, df, F);
   } else {
     hipLaunchKernelGGL(k_reml_eval_f<64>, dim3(E), dim3(64), reml_eval_f_bytes(p), c->stream, r.CA.as<double>(),
-                       r.CB.as<double>(), r.S.as<double>(), E, TA, p, df, F);
+                       r.CB.as<double>(), r.S.as<double>(), E, E_dev, TA, p, df, F);
   }
   return check_launch("k_reml_eval_f");
 }
@@ -911,12 +933,18 @@ static int reml_ref_enqueue(eqtl_gpu_ctx* c, int G, int ng, double* delta) {
   hipLaunchKernelGGL(k_ref_mid, gb, tb, 0, c->stream, st, G, ed);
   CHK(enqueue_eval_f(c, G, ek, ed, F));
   hipLaunchKernelGGL(k_ref_mid_done, gb, tb, 0, c->stream, st, G, F);
+  // Newton rounds over the genes still active, compacted on the device
+  // (count, then twice the count for the paired difference points)
+  int* cnt = r.cnt.as<int>();
+  int* list = r.list.as<int>();
   for (int it = 0; it < 8; ++it) {
-    hipLaunchKernelGGL(k_ref_newton_points, gb, tb, 0, c->stream, st, G, ed);
-    CHK(enqueue_eval_f(c, 2 * G, ek, ed, F));
-    hipLaunchKernelGGL(k_ref_newton_propose, gb, tb, 0, c->stream, st, G, F, ed);
-    CHK(enqueue_eval_f(c, G, ek, ed, F));
-    hipLaunchKernelGGL(k_ref_newton_accept, gb, tb, 0, c->stream, st, G, F);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->stream, reinterpret_cast<unsigned int*>(cnt), 2, 0u);
+    hipLaunchKernelGGL(k_ref_newton_points, gb, tb, 0, c->stream, st, G, cnt, list, ek, ed);
+    hipLaunchKernelGGL(k_ref_twice, dim3(1), dim3(64), 0, c->stream, cnt, cnt + 1);
+    CHK(enqueue_eval_f(c, 2 * G, ek, ed, F, cnt + 1));
+    hipLaunchKernelGGL(k_ref_newton_propose, gb, tb, 0, c->stream, st, cnt, list, F, ek, ed);
+    CHK(enqueue_eval_f(c, G, ek, ed, F, cnt));
+    hipLaunchKernelGGL(k_ref_newton_accept, gb, tb, 0, c->stream, st, cnt, list, F);
   }
   hipLaunchKernelGGL(k_ref_out, gb, tb, 0, c->stream, st, G, delta);
   return check_launch("k_ref");
@@ -931,7 +959,7 @@ static int reml_ref_device(eqtl_gpu_ctx* c, int G, double* delta) {
   const int n = c->n, p = c->p, TA = p * (p + 1) / 2;
   auto& r = c->rw;
   const size_t E2 = 2 * static_cast<size_t>(G);
-  CHK(ensure(r.W, sizeof(double) * n * E2));
+  if (p > 22) CHK(ensure(r.W, sizeof(double) * n * E2));  // only the unfused path writes weights
   CHK(ensure(r.CA, sizeof(double) * TA * E2));
   CHK(ensure(r.CB, sizeof(double) * p * E2));
   CHK(ensure(r.S, sizeof(double) * kNumScal * E2));
@@ -939,12 +967,14 @@ static int reml_ref_device(eqtl_gpu_ctx* c, int G, double* delta) {
   CHK(ensure(r.dek, sizeof(int32_t) * E2));
   CHK(ensure(r.ded, sizeof(double) * E2));
   CHK(ensure(r.dF, sizeof(double) * E2));
+  CHK(ensure(r.cnt, sizeof(int) * 2));
+  CHK(ensure(r.list, sizeof(int) * G));
   int ng = 0;
   CHK(reml_grid_setup(c, G, &ng));
   const std::vector<const void*> key = {
       reinterpret_cast<const void*>(static_cast<intptr_t>(G)), reinterpret_cast<const void*>(static_cast<intptr_t>(n)),
       reinterpret_cast<const void*>(static_cast<intptr_t>(p)), delta, c->lam64.p, c->Yt64.p, c->Xt64.p, r.P.p,
-      r.W.p, r.CA.p, r.CB.p, r.S.p, r.st.p, r.dek.p, r.ded.p, r.dF.p, r.dgrid.p, r.Dg.p, r.logvg.p, r.XD.p, r.Y2.p,
+      r.W.p, r.CA.p, r.CB.p, r.S.p, r.st.p, r.dek.p, r.ded.p, r.dF.p, r.cnt.p, r.list.p, r.dgrid.p, r.Dg.p, r.logvg.p, r.XD.p, r.Y2.p,
       r.CAg.p, r.Ainv.p, r.logdet.p, r.okg.p, r.Bg.p, r.C0.p, r.F.p, r.best.p};
   if (!r.graph || r.graph_key != key) {
     if (r.graph) {

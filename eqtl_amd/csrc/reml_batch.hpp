@@ -145,10 +145,13 @@ __global__ __launch_bounds__(256) void k_reml_eval_weights_fgh(const double* lam
 // reductions to a single wave (EV (p + 2) butterfly sums).
 template <int PB, int EV>
 __global__ __launch_bounds__(64) void k_reml_eval_weights_f(const double* lam, const double* Yt, const double* Xt,
-                                                             int n, int p, int E, const int32_t* ek,
-                                                             const double* ed, double* W, double* CB, double* S) {
+                                                             int n, int p, int E_arg, const int* E_dev,
+                                                             const int32_t* ek, const double* ed, double* W,
+                                                             double* CB, double* S) {
   constexpr int NV = PB + 2;  // per evaluation: c0, sum ln v, b[0..p)
+  const int E = E_dev ? *E_dev : E_arg;  // evaluations counted on the device (E_arg: the grid's bound)
   const int e0 = blockIdx.x * EV;
+  if (e0 >= E) return;
   double dl[EV];
   const double* y[EV];
 #pragma unroll
@@ -405,10 +408,12 @@ __host__ __device__ constexpr int reml_eval_f_doubles(int p) { return (p + 1) * 
 __host__ __device__ constexpr size_t reml_eval_f_bytes(int p) { return sizeof(double) * reml_eval_f_doubles(p) + 8; }
 
 template <int LANES>
-__global__ __launch_bounds__(64) void k_reml_eval_f(const double* CA, const double* CB, const double* S, int E, int TA,
-                                                    int p, int df, double* Fo) {
+__global__ __launch_bounds__(64) void k_reml_eval_f(const double* CA, const double* CB, const double* S, int E_arg,
+                                                    const int* E_dev, int TA, int p, int df, double* Fo) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_f[];
+  const int E = E_dev ? *E_dev : E_arg;  // evaluations counted on the device (E_arg: the grid's bound)
   constexpr int kPer = 64 / LANES;
+  if (static_cast<int>(blockIdx.x) * kPer >= E) return;
   const int sub = threadIdx.x / LANES, lane = threadIdx.x % LANES;
   const int e_raw = blockIdx.x * kPer + sub;
   const int e = e_raw < E ? e_raw : E - 1;  // a spare half repeats the last evaluation and writes nothing

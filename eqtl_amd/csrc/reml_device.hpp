@@ -34,9 +34,11 @@ __device__ inline f64x8 dg_frag(const double* col, bool in, int k0, int K) {
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_dgemm_tn(int M, int N, int K, const double* __restrict__ A, int lda,
+__global__ __launch_bounds__(256) void k_dgemm_tn(int M, int N_arg, int K, const double* __restrict__ A, int lda,
                                                   const double* __restrict__ B, int ldb, double* __restrict__ C,
-                                                  int ldc) {
+                                                  int ldc, const int* N_dev) {
+  const int N = N_dev ? *N_dev : N_arg;  // columns counted on the device (N_arg: the grid's bound)
+  if (static_cast<int>(blockIdx.y) * kDgT >= N) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int m0 = blockIdx.x * kDgT + wr * 32, n0 = blockIdx.y * kDgT + wc * 32;
@@ -83,10 +85,97 @@ __global__ __launch_bounds__(256) void k_dgemm_tn(int M, int N, int K, const dou
     }
 }
 
+// ---- fused f-only evaluation inputs on matrix cores ----------------------------
+// For evaluation e (gene ek[e], delta ed[e]) with d_i = 1 / max(delta lam_i + 1,
+// 1e-12): CA[e][t] = sum_i P[t][i] d_i (the packed X~^T D X~), CB[e][a] =
+// sum_i X~[i][a] d_i y~_i (b), S[e][0] = sum_i y~_i^2 d_i (c0) and S[e][3] =
+// sum_i ln v_i. The weights are formed in registers as the B operand of
+// v_mfma_f64_16x16x4f64 (lane: evaluation l & 15 of its wave's 16, samples
+// 8 (l >> 4) + [0..7] of the 32-sample step), against P (MB blocks of 16
+// packed entries) and X~ (PB blocks of 16 covariates) as A, so no weight
+// matrix is written. Four waves x 16 evaluations per workgroup. The count of
+// evaluations is E, or *E_dev when given (then read on the device, so that a
+// captured launch sequence can evaluate a varying number).
+template <int MB, int PB>
+__global__ __launch_bounds__(256) void k_reml_gram_f(const double* __restrict__ P, const double* __restrict__ Xt,
+                                                     const double* __restrict__ Yt, const double* __restrict__ lam,
+                                                     int n, int p, int TA, int E_arg, const int* E_dev,
+                                                     const int32_t* __restrict__ ek, const double* __restrict__ ed,
+                                                     double* __restrict__ CA, double* __restrict__ CB,
+                                                     double* __restrict__ S, int nscal) {
+  const int E = E_dev ? *E_dev : E_arg;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int e_blk = blockIdx.x * 64 + wave * 16;
+  if (e_blk >= E) return;
+  const int e = min(e_blk + (lane & 15), E - 1);
+  const double dl = ed[e];
+  const double* y = Yt + static_cast<size_t>(ek[e]) * n;
+  const int kk = 8 * (lane >> 4);
+  f64x4 a1[MB], a2[PB];
+#pragma unroll
+  for (int b = 0; b < MB; ++b) a1[b] = f64x4{0, 0, 0, 0};
+#pragma unroll
+  for (int b = 0; b < PB; ++b) a2[b] = f64x4{0, 0, 0, 0};
+  double c0 = 0, lv = 0;
+  for (int k0 = 0; k0 < n; k0 += 32) {
+    f64x8 wd, wy;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int i = k0 + kk + t;
+      if (i < n) {
+        double v = dl * lam[i] + 1.0;
+        if (v < 1e-12) v = 1e-12;
+        const double d = 1.0 / v, yi = y[i];
+        wd.v[t] = d;
+        wy.v[t] = d * yi;
+        c0 += yi * (d * yi);
+        lv += log(v);
+      } else {
+        wd.v[t] = 0.0;
+        wy.v[t] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < MB; ++b) {
+      const int t = 16 * b + (lane & 15);
+      a1[b] = mfma_f64x8(dg_frag(P + static_cast<size_t>(t < TA ? t : 0) * n, t < TA, k0 + kk, n), wd, a1[b]);
+    }
+#pragma unroll
+    for (int b = 0; b < PB; ++b) {
+      const int c = 16 * b + (lane & 15);
+      a2[b] = mfma_f64x8(dg_frag(Xt + static_cast<size_t>(c < p ? c : 0) * n, c < p, k0 + kk, n), wy, a2[b]);
+    }
+  }
+  // lanes l, l ^ 16, l ^ 32, l ^ 48 hold the same evaluation
+  c0 += __shfl_xor(c0, 16);
+  c0 += __shfl_xor(c0, 32);
+  lv += __shfl_xor(lv, 16);
+  lv += __shfl_xor(lv, 32);
+  const int eo = e_blk + (lane & 15);
+  if (eo >= E) return;
+  if (lane < 16) {
+    S[static_cast<size_t>(eo) * nscal + 0] = c0;
+    S[static_cast<size_t>(eo) * nscal + 3] = lv;
+  }
+#pragma unroll
+  for (int b = 0; b < MB; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int t = 16 * b + (lane >> 4) + 4 * r;
+      if (t < TA) CA[static_cast<size_t>(eo) * TA + t] = a1[b][r];
+    }
+#pragma unroll
+  for (int b = 0; b < PB; ++b)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int c = 16 * b + (lane >> 4) + 4 * r;
+      if (c < p) CB[static_cast<size_t>(eo) * p + c] = a2[b][r];
+    }
+}
+
 // ---- reference search state, one entry per gene ------------------------------
-// The arithmetic below restates the host form of the search step for step
-// (same expressions, no contraction into FMAs), so the device search gives
-// the same delta bit for bit.
+// The kernels below restate the search's steps expression for expression,
+// with no contraction into FMAs.
 struct RemlRefState {
   double lo, hi, d, cur, h, dn, step;
   int32_t active, pending;
@@ -140,65 +229,75 @@ __global__ void k_ref_mid_done(RemlRefState* st, int G, const double* F) {
   st[k].pending = 0;
 }
 
-// central-difference points d -+ h, h = max(d 1e-4, 1e-9) (inactive genes: d)
-__global__ void k_ref_newton_points(RemlRefState* st, int G, double* ed) {
+// Central-difference points of the active genes, compacted: list[q] = gene,
+// evaluations 2q (d - h) and 2q + 1 (d + h), h = max(d 1e-4, 1e-9); *count =
+// active genes (zeroed before the launch). Positions follow atomic order; each
+// evaluation depends on its gene only, so the results do not.
+__global__ void k_ref_newton_points(RemlRefState* st, int G, int* count, int* list, int32_t* ek, double* ed) {
 #pragma clang fp contract(off)
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= G) return;
+  if (k >= G || !st[k].active) return;
   const double d = st[k].d;
-  if (st[k].active) {
-    const double h = fmax(d * 1e-4, 1e-9);
-    st[k].h = h;
-    ed[k] = d - h;
-    ed[G + k] = d + h;
-  } else {
-    ed[k] = d;
-    ed[G + k] = d;
-  }
+  const double h = fmax(d * 1e-4, 1e-9);
+  st[k].h = h;
+  const int q = atomicAdd(count, 1);
+  list[q] = k;
+  ek[2 * q] = k;
+  ek[2 * q + 1] = k;
+  ed[2 * q] = d - h;
+  ed[2 * q + 1] = d + h;
 }
 
-// Newton proposal from the differences; a gene whose curvature is not
-// positive or whose step leaves (1e-8, 1e6) stops. ed[k] <- the trial point.
-__global__ void k_ref_newton_propose(RemlRefState* st, int G, const double* F, double* ed) {
+// count2 = 2 * count (the evaluations of the points above)
+__global__ void k_ref_twice(const int* count, int* count2) {
+  if (threadIdx.x == 0) *count2 = 2 * *count;
+}
+
+// Newton proposal of active gene list[q] from its differences; a gene whose
+// curvature is not positive or whose step leaves (1e-8, 1e6) stops.
+// Evaluation q (gene list[q]) is its trial point (d itself when none).
+__global__ void k_ref_newton_propose(RemlRefState* st, const int* count, const int* list, const double* F,
+                                     int32_t* ek, double* ed) {
 #pragma clang fp contract(off)
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= G) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= *count) return;
+  const int k = list[q];
   RemlRefState s = st[k];
   s.pending = 0;
-  if (s.active) {
-    const double h = s.h, fm = F[k], fp = F[G + k];
-    const double g1 = (fp - fm) / (2.0 * h);
-    const double g2 = (fp - 2.0 * s.cur + fm) / (h * h);
-    if (!(g2 > 1e-12)) {
+  const double h = s.h, fm = F[2 * q], fp = F[2 * q + 1];
+  const double g1 = (fp - fm) / (2.0 * h);
+  const double g2 = (fp - 2.0 * s.cur + fm) / (h * h);
+  if (!(g2 > 1e-12)) {
+    s.active = 0;
+  } else {
+    const double step = g1 / g2;
+    const double dn = s.d - step;
+    if (!(dn > 1e-8 && dn < 1e6)) {
       s.active = 0;
     } else {
-      const double step = g1 / g2;
-      const double dn = s.d - step;
-      if (!(dn > 1e-8 && dn < 1e6)) {
-        s.active = 0;
-      } else {
-        s.pending = 1;
-        s.dn = dn;
-        s.step = step;
-      }
+      s.pending = 1;
+      s.dn = dn;
+      s.step = step;
     }
   }
-  ed[k] = s.pending ? s.dn : s.d;
+  ek[q] = k;
+  ed[q] = s.pending ? s.dn : s.d;
   st[k] = s;
 }
 
 // accept a trial point that lowers f; stop when the step is below d 1e-8
-__global__ void k_ref_newton_accept(RemlRefState* st, int G, const double* F) {
+__global__ void k_ref_newton_accept(RemlRefState* st, const int* count, const int* list, const double* F) {
 #pragma clang fp contract(off)
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= G) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= *count) return;
+  const int k = list[q];
   RemlRefState s = st[k];
   if (s.pending) {
-    if (!(F[k] < s.cur)) {
+    if (!(F[q] < s.cur)) {
       s.active = 0;
     } else {
       s.d = s.dn;
-      s.cur = F[k];
+      s.cur = F[q];
       if (fabs(s.step) < s.d * 1e-8) s.active = 0;
     }
     s.pending = 0;

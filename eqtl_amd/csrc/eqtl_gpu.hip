@@ -1158,7 +1158,7 @@ static int covariate_basis(eqtl_gpu_ctx* c) {
   int ok = 0;
   HIPC(hipMemcpyAsync(&ok, okb.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
-  if (!ok) return fail(EQTL_E_ARG, "covariate matrix X^T X is not positive definite");
+  if (!ok) return fail(EQTL_E_ARG, "covariate matrix X^T X is not positive semidefinite");
   c->have_wx = true;
   return EQTL_OK;
 }
@@ -1227,7 +1227,7 @@ static int lmm_prep_core(eqtl_gpu_ctx* c, int G, int mode, const double* delta_i
   if (rss_out) HIPC(hipMemcpyAsync(rss_out, c->gscal.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   for (int k = 0; k < G; ++k)
-    if (!valid[k]) return fail(EQTL_E_ARG, "whitened covariate matrix is not positive definite (gene " + std::to_string(k) + ")");
+    if (!valid[k]) return fail(EQTL_E_ARG, "whitened covariate matrix is not positive semidefinite (gene " + std::to_string(k) + ")");
   c->prepped_model = EQTL_MODEL_LMM;
   return EQTL_OK;
 }

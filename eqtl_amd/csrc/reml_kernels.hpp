@@ -109,18 +109,39 @@ __device__ __attribute__((always_inline)) inline void reml_pass1(const double* _
   __syncthreads();
 }
 
-// Plain Cholesky (no pivoting) of sm.A into lower Lc (in sm.A1). Thread 0.
+// Rank-deficient covariates. A column whose remaining (Schur) variance after
+// the earlier columns is at most kDependentTol of its own is a linear
+// combination of them: its pivot and factor column are set to zero, so the
+// basis spans the covariates' column space without it (with an exactly
+// dependent column the reference's LDLT meets a zero pivot and its solves
+// take the pseudo-inverse, model_lmm.cpp:141-151 / model_lm.cpp:25, which is
+// the same projection). A pivot below -kDependentTol of the column's own
+// variance means the matrix is not positive semidefinite: failure.
+constexpr double kDependentTol = 1e-10;
+
+// 1: pivot sqrt(s); 0: dependent column (pivot 0); -1: not positive semidefinite
+__device__ inline int semidef_pivot(double s, double ajj, double* l) {
+  if (s > kDependentTol * ajj) {
+    *l = sqrt(s);
+    return 1;
+  }
+  *l = 0.0;
+  return s >= -kDependentTol * ajj ? 0 : -1;
+}
+
+// Cholesky (no pivoting) of sm.A into lower Lc (in sm.A1), dependent columns
+// zeroed as above. Thread 0.
 __device__ inline bool chol_lower(RemlSmem& sm, int p) {
   for (int j = 0; j < p; ++j) {
     double s = sm.A[j * kPMax + j];
     for (int k = 0; k < j; ++k) s -= sm.A1[k * kPMax + j] * sm.A1[k * kPMax + j];
-    if (!(s > 0)) return false;
-    const double ljj = sqrt(s);
+    double ljj;
+    if (semidef_pivot(s, sm.A[j * kPMax + j], &ljj) < 0) return false;
     sm.A1[j * kPMax + j] = ljj;
     for (int i = j + 1; i < p; ++i) {
       double t = sm.A[j * kPMax + i];
       for (int k = 0; k < j; ++k) t -= sm.A1[k * kPMax + i] * sm.A1[k * kPMax + j];
-      sm.A1[j * kPMax + i] = t / ljj;
+      sm.A1[j * kPMax + i] = ljj > 0 ? t / ljj : 0.0;
     }
   }
   return true;
@@ -130,7 +151,8 @@ __device__ inline bool chol_lower(RemlSmem& sm, int p) {
 // column: thread 0 forms the pivot, threads j < i < p the column below it, each
 // element with the serial form's operation order (so the factor is identical).
 // Thread p also carries the border row b^T through the same steps, leaving
-// u = Lc^-1 b in sm.u. Returns false when a pivot is not positive.
+// u = Lc^-1 b in sm.u (0 for a dependent column). Returns false when the
+// matrix is not positive semidefinite.
 __device__ inline bool chol_lower_block(RemlSmem& sm, int p) {
   const int t = threadIdx.x;
   if (t == 0) sm.flag = 1;
@@ -139,8 +161,9 @@ __device__ inline bool chol_lower_block(RemlSmem& sm, int p) {
     if (t == 0) {
       double s = sm.A[j * kPMax + j];
       for (int k = 0; k < j; ++k) s -= sm.A1[k * kPMax + j] * sm.A1[k * kPMax + j];
-      if (!(s > 0)) sm.flag = 0;
-      else sm.A1[j * kPMax + j] = sqrt(s);
+      double ljj;
+      if (semidef_pivot(s, sm.A[j * kPMax + j], &ljj) < 0) sm.flag = 0;
+      sm.A1[j * kPMax + j] = ljj;
     }
     __syncthreads();
     if (!sm.flag) break;
@@ -148,11 +171,11 @@ __device__ inline bool chol_lower_block(RemlSmem& sm, int p) {
     if (t > j && t < p) {
       double x = sm.A[j * kPMax + t];
       for (int k = 0; k < j; ++k) x -= sm.A1[k * kPMax + t] * sm.A1[k * kPMax + j];
-      sm.A1[j * kPMax + t] = x / ljj;
+      sm.A1[j * kPMax + t] = ljj > 0 ? x / ljj : 0.0;
     } else if (t == p) {
       double x = sm.b[j];
       for (int k = 0; k < j; ++k) x -= sm.A1[k * kPMax + j] * sm.u[k];
-      sm.u[j] = x / ljj;
+      sm.u[j] = ljj > 0 ? x / ljj : 0.0;
     }
     __syncthreads();
   }
@@ -203,7 +226,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
       if (c < p) {
         double t = s * Xt[static_cast<size_t>(c) * n + i];
         for (int q = 0; q < c; ++q) t -= w[q] * sm.A1[q * kPMax + c];
-        w[c] = t / sm.A1[c * kPMax + c];
+        const double lcc = sm.A1[c * kPMax + c];
+        w[c] = lcc > 0 ? t / lcc : 0.0;  // a dependent covariate adds no direction
         r -= w[c] * sm.u[c];
         put_val(gc, k, 1 + c, i, s * w[c]);
       }
@@ -235,7 +259,8 @@ __global__ __launch_bounds__(256) void k_lm_basis(const double* X, const double*
       if (c < p) {
         double t = X[static_cast<size_t>(c) * n + i];
         for (int q = 0; q < c; ++q) t -= w[q] * sm.A1[q * kPMax + c];
-        w[c] = t / sm.A1[c * kPMax + c];
+        const double lcc = sm.A1[c * kPMax + c];
+        w[c] = lcc > 0 ? t / lcc : 0.0;  // a dependent covariate adds no direction
         Wx[static_cast<size_t>(c) * n + i] = w[c];
       }
     }

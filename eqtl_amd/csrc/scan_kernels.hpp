@@ -935,7 +935,7 @@ __global__ __launch_bounds__(256) void k_polish(PolishArgs a) {
   for (int i = 0; i < p; ++i) {
     double t = va[2 + i];
     for (int c = 0; c < i; ++c) t -= L[c * p + i] * w[c];
-    w[i] = t / L[i * p + i];
+    w[i] = L[i * p + i] > 0 ? t / L[i * p + i] : 0.0;  // dependent covariate: no direction
     ss += w[i] * w[i];
   }
   const double gg = va[0], num = va[1], S = gg - ss, gscal = a.gene_scal[k], df = a.df;
@@ -1131,7 +1131,7 @@ __global__ __launch_bounds__(256) void k_exact_minp(ExactArgs a) {
 #pragma unroll
           for (int c = 0; c < kP; ++c)
             if (c < i) t -= Ls[c * p + i] * av[c];
-          av[i] = t / Ls[i * p + i];
+          av[i] = Ls[i * p + i] > 0 ? t / Ls[i * p + i] : 0.0;  // dependent covariate: no direction
           ss += av[i] * av[i];
         }
       }

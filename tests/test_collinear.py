@@ -13,6 +13,20 @@ Bars (both contraction forms, LM and LMM): tested sets, n_tested, n_sig and
 best SNP exact; p relative 1e-3 for p >= 1e-300; se relative 1e-4 (FP32) /
 5e-4 (BF16X3); beta relative the same where |t| > 0.1.
 
+Rank-deficient covariates (an exact copy of a column, an all-zero column, the
+sum of two columns appended): where the reference's LDLT meets a zero pivot
+its solves take the pseudo-inverse (model_lmm.cpp:141-151, model_lm.cpp:25)
+and its REML objective is 1e300 at every delta (model_lmm.cpp:23-33). Bars:
+the search's delta equal to the oracle's (1e-12 relative) for the copy and the
+zero column; the scan against the oracle at the oracle's delta with the
+collinear bars above, LM and LMM. The engine drops a covariate whose remaining
+variance is below 1e-10 of its own. For the sum column the reference's LM
+pivot is rounding noise (1e-13 of the column's scale) instead of zero, and its
+(X^T X)^-1 then moves beta by orders of magnitude from the model without the
+column (measured on the oracle: 1.4e4 relative), so LM is checked there
+against the model without the column: beta and df-adjusted se to the bars
+above (df keeps the full covariate count, as the reference's).
+
 Overflow cases: a scan with pthr = 1 and a 100-hit caller buffer returns every
 pair through eqtl_gpu_scan_hits, equal to the one-buffer run; a block with
 more than 65,536 tests at |t| > 8 (the polish list's first capacity) is
@@ -167,6 +181,62 @@ def test_collinear_parity(gpu_ctx, model, prec):
     np.add.at(sig, hits["gene"][hits["p"] <= 1e-3], 1)
     st2["n_sig"] = sig
     _check(gd, st2, ref, prec, f"collinear {model} prec={prec}")
+
+
+def _rank_deficient(X, kind):
+    if kind == "copy":
+        return np.column_stack([X, X[:, 1]])
+    if kind == "zero":
+        return np.column_stack([X, np.zeros(X.shape[0])])
+    return np.column_stack([X, X[:, 1] + X[:, 2]])
+
+
+@gpu
+@pytest.mark.parametrize("kind", ["copy", "zero", "sum"])
+@pytest.mark.parametrize("model", ["lm", "lmm"])
+def test_rank_deficient_covariates(gpu_ctx, model, kind):
+    c = collinear_case()
+    X = _rank_deficient(c["X"], kind)
+    kw = dict(gene_chrom=c["win"][0], gene_cstart=c["win"][1], gene_cend=c["win"][2], pthr=1e-3, dense=True,
+              hit_cap=c["m"] * c["G"] + 1)
+    if model == "lmm":
+        kw.update(Q=c["Q"], lam=c["lam"])
+    lm_sum = model == "lm" and kind == "sum"
+    ref = O.trans_scan(model, c["dos"], c["keep"], c["chrom"], c["pos"], c["Y"], c["X"] if lm_sum else X, **kw)
+    gpu_ctx.set_samples(c["n"])
+    gpu_ctx.set_policy(True, 0.8, 0.0)
+    gpu_ctx.set_snp_annot(c["chrom"], c["pos"])
+    if model == "lmm":
+        gpu_ctx.set_basis(c["Q"], c["lam"])
+        if kind != "sum":
+            delta, _ = gpu_ctx.lmm_prep(c["Y"], X, delta_mode=eqtl_amd.DELTA_REF)
+            print(f"rank-deficient ({kind}): delta {delta[:3]} oracle {ref['state']['delta'][:3]}", flush=True)
+            assert rel_err(delta, ref["state"]["delta"]).max() < 1e-12
+        gpu_ctx.lmm_prep(c["Y"], X, delta_mode=eqtl_amd.DELTA_GIVEN, delta_in=ref["state"]["delta"])
+        mc = eqtl_amd.MODEL_LMM
+    else:
+        gpu_ctx.lm_prep(c["Y"], X)
+        mc = eqtl_amd.MODEL_LM
+    gpu_ctx.set_gene_windows(*c["win"])
+    prec = eqtl_amd.PREC_BF16X3
+    hits, st = gpu_ctx.scan(mc, prec, c["rows"], pthr=1.0, block=512, hit_cap=1 << 16)
+    gd = dense_from_hits(hits, c["m"], c["G"])
+    if lm_sum:
+        tested = np.isfinite(ref["p"])
+        assert np.array_equal(tested, np.isfinite(gd["p"]))
+        assert np.array_equal(st["n_tested"], ref["state"]["n_tested"])
+        df_full, df_red = c["n"] - X.shape[1] - 1, c["n"] - c["X"].shape[1] - 1
+        big = tested & (np.abs(ref["stat"]) > 0.1)
+        brel = rel_err(gd["beta"][big], ref["beta"][big])
+        srel = rel_err(gd["se"][tested], ref["se"][tested] * np.sqrt(df_red / df_full))
+        print(f"rank-deficient sum lm vs the model without the column: beta rel {brel.max():.2e}, se rel {srel.max():.2e}")
+        assert brel.max() < TOL[prec] and srel.max() < TOL[prec]
+        return
+    st2 = st.copy()
+    sig = np.zeros(c["G"], np.int32)
+    np.add.at(sig, hits["gene"][hits["p"] <= 1e-3], 1)
+    st2["n_sig"] = sig
+    _check(gd, st2, ref, prec, f"rank-deficient {kind} {model}")
 
 
 @gpu

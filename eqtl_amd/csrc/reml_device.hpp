@@ -59,17 +59,29 @@ __global__ __launch_bounds__(256) void k_dgemm_tn(int M, int N_arg, int K, const
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) acc[a][b] = f64x4{0, 0, 0, 0};
+  // operands of the next 32-sample step are loaded before this step's MFMAs
+  f64x8 af[2], bf[2];
+#pragma unroll
+  for (int b = 0; b < 2; ++b) {
+    af[b] = dg_frag(acol[b], ain[b], kk, K);
+    bf[b] = dg_frag(bcol[b], bin[b], kk, K);
+  }
   for (int k0 = 0; k0 < K; k0 += 32) {
-    f64x8 af[2], bf[2];
+    f64x8 an[2], bn[2];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
-      af[b] = dg_frag(acol[b], ain[b], k0 + kk, K);
-      bf[b] = dg_frag(bcol[b], bin[b], k0 + kk, K);
+      an[b] = dg_frag(acol[b], ain[b], k0 + 32 + kk, K);
+      bn[b] = dg_frag(bcol[b], bin[b], k0 + 32 + kk, K);
     }
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) acc[a][b] = mfma_f64x8(af[a], bf[b], acc[a][b]);
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      af[b] = an[b];
+      bf[b] = bn[b];
+    }
   }
 #pragma unroll
   for (int a = 0; a < 2; ++a)

@@ -151,6 +151,8 @@ struct RemlWork {
   // values, and the captured launch sequence with the arguments it was
   // captured for (replayed while they match)
   DBuf st, dek, ded, dF, cnt, list;
+  // fragment-major A operands of k_reml_gram_f (P blocks, then X~ blocks)
+  DBuf Af;
   hipGraphExec_t graph = nullptr;
   std::vector<const void*> graph_key, eager_key;
 };
@@ -732,13 +734,32 @@ static int dgemm_tn(eqtl_gpu_ctx* c, int m, int nn, int k, const double* A, int 
   return check_launch("k_dgemm_tn");
 }
 
-// Per-sample outer products of X~ (once per lmm_prep).
+// Blocks of 16 packed entries (MB) and of 16 covariates (PB) that
+// k_reml_gram_f is compiled for at p (p <= 22).
+static void gram_blocks(int p, int* mb, int* pb) {
+  const int need = (p * (p + 1) / 2 + 15) / 16;
+  *mb = need <= 2 ? 2 : need <= 4 ? 4 : need <= 6 ? 6 : need <= 9 ? 9 : need <= 12 ? 12 : 16;
+  *pb = p <= 16 ? 1 : 2;
+}
+
+// Per-sample outer products of X~ (once per lmm_prep), and for p <= 22 the
+// same P and X~ packed fragment-major for k_reml_gram_f.
 static int reml_setup(eqtl_gpu_ctx* c) {
   const int n = c->n, p = c->p, TA = p * (p + 1) / 2;
   CHK(ensure(c->rw.P, sizeof(double) * static_cast<size_t>(n) * TA));
   hipLaunchKernelGGL(k_reml_outer, dim3((n + 255) / 256, TA), dim3(256), 0, c->stream, c->Xt64.as<double>(), n, p,
                      c->rw.P.as<double>());
-  return check_launch("k_reml_outer");
+  CHK(check_launch("k_reml_outer"));
+  if (p > 22) return EQTL_OK;
+  int mb = 0, pb = 0;
+  gram_blocks(p, &mb, &pb);
+  const size_t nks = (n + 31) / 32;
+  CHK(ensure(c->rw.Af, nks * (mb + pb) * 4096));
+  hipLaunchKernelGGL(k_frag_pack, dim3(256), dim3(256), 0, c->stream, c->rw.P.as<double>(), TA, n, mb, 0, mb + pb,
+                     c->rw.Af.as<double>());
+  hipLaunchKernelGGL(k_frag_pack, dim3(64), dim3(256), 0, c->stream, c->Xt64.as<double>(), p, n, pb, mb, mb + pb,
+                     c->rw.Af.as<double>());
+  return check_launch("k_frag_pack");
 }
 
 // f-only evaluations at (gene ek[e], delta ed[e]), e < E (or e < *E_dev,
@@ -749,12 +770,20 @@ static int reml_setup(eqtl_gpu_ctx* c) {
 // (W for p > 22, CA, CB, S) must hold E evaluations.
 extern "C++" {
 template <int MB, int PB>
-static void launch_gram_f(eqtl_gpu_ctx* c, int E, const int* E_dev, const int32_t* ek, const double* ed) {
+static int launch_gram_f(eqtl_gpu_ctx* c, int E, const int* E_dev, const int32_t* ek, const double* ed) {
   auto& r = c->rw;
   const int n = c->n, p = c->p, TA = p * (p + 1) / 2;
-  hipLaunchKernelGGL((k_reml_gram_f<MB, PB>), dim3(static_cast<unsigned>((E + 63) / 64)), dim3(256), 0, c->stream,
-                     r.P.as<double>(), c->Xt64.as<double>(), c->Yt64.as<double>(), c->lam64.as<double>(), n, p, TA, E,
-                     E_dev, ek, ed, r.CA.as<double>(), r.CB.as<double>(), r.S.as<double>(), kNumScal);
+  const size_t lds = 2 * gram_stage_bytes<MB, PB>();
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reml_gram_f<MB, PB>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_reml_gram_f<MB, PB>), dim3(static_cast<unsigned>((E + 63) / 64)), dim3(256), lds, c->stream,
+                     r.Af.as<double>(), c->Yt64.as<double>(), c->lam64.as<double>(), n, p, TA, E, E_dev, ek, ed,
+                     r.CA.as<double>(), r.CB.as<double>(), r.S.as<double>(), kNumScal);
+  return check_launch("k_reml_gram_f");
 }
 }
 
@@ -762,15 +791,19 @@ static int enqueue_eval_f(eqtl_gpu_ctx* c, int E, const int32_t* ek, const doubl
                           const int* E_dev = nullptr) {
   const int n = c->n, p = c->p, TA = p * (p + 1) / 2, df = n - p;
   auto& r = c->rw;
-  const int mb = (TA + 15) / 16;
   if (p <= 22) {
-    if (mb <= 2) launch_gram_f<2, 1>(c, E, E_dev, ek, ed);
-    else if (mb <= 4) launch_gram_f<4, 1>(c, E, E_dev, ek, ed);
-    else if (mb <= 6) launch_gram_f<6, 1>(c, E, E_dev, ek, ed);
-    else if (mb <= 9) launch_gram_f<9, 1>(c, E, E_dev, ek, ed);
-    else if (mb <= 12) launch_gram_f<12, 2>(c, E, E_dev, ek, ed);
-    else launch_gram_f<16, 2>(c, E, E_dev, ek, ed);
-    CHK(check_launch("k_reml_gram_f"));
+    int mb = 0, pb = 0;
+    gram_blocks(p, &mb, &pb);
+    int rc;
+    switch (mb) {
+      case 2: rc = launch_gram_f<2, 1>(c, E, E_dev, ek, ed); break;
+      case 4: rc = launch_gram_f<4, 1>(c, E, E_dev, ek, ed); break;
+      case 6: rc = launch_gram_f<6, 1>(c, E, E_dev, ek, ed); break;
+      case 9: rc = launch_gram_f<9, 1>(c, E, E_dev, ek, ed); break;
+      case 12: rc = launch_gram_f<12, 2>(c, E, E_dev, ek, ed); break;
+      default: rc = launch_gram_f<16, 2>(c, E, E_dev, ek, ed); break;
+    }
+    CHK(rc);
   } else {
     // b, c0 and sum ln v from the weights kernel; one GEMM for X~^T D X~
     hipLaunchKernelGGL((k_reml_eval_weights_f<kRP, 2>), dim3((E + 1) / 2), dim3(64), 0, c->stream,
@@ -974,7 +1007,7 @@ static int reml_ref_device(eqtl_gpu_ctx* c, int G, double* delta) {
   const std::vector<const void*> key = {
       reinterpret_cast<const void*>(static_cast<intptr_t>(G)), reinterpret_cast<const void*>(static_cast<intptr_t>(n)),
       reinterpret_cast<const void*>(static_cast<intptr_t>(p)), delta, c->lam64.p, c->Yt64.p, c->Xt64.p, r.P.p,
-      r.W.p, r.CA.p, r.CB.p, r.S.p, r.st.p, r.dek.p, r.ded.p, r.dF.p, r.cnt.p, r.list.p, r.dgrid.p, r.Dg.p, r.logvg.p, r.XD.p, r.Y2.p,
+      r.W.p, r.CA.p, r.CB.p, r.S.p, r.st.p, r.dek.p, r.ded.p, r.dF.p, r.cnt.p, r.list.p, r.Af.p, r.dgrid.p, r.Dg.p, r.logvg.p, r.XD.p, r.Y2.p,
       r.CAg.p, r.Ainv.p, r.logdet.p, r.okg.p, r.Bg.p, r.C0.p, r.F.p, r.best.p};
   if (!r.graph || r.graph_key != key) {
     if (r.graph) {

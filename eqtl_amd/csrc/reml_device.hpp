@@ -103,41 +103,94 @@ __global__ __launch_bounds__(256) void k_dgemm_tn(int M, int N_arg, int K, const
 // sum_i X~[i][a] d_i y~_i (b), S[e][0] = sum_i y~_i^2 d_i (c0) and S[e][3] =
 // sum_i ln v_i. The weights are formed in registers as the B operand of
 // v_mfma_f64_16x16x4f64 (lane: evaluation l & 15 of its wave's 16, samples
-// 8 (l >> 4) + [0..7] of the 32-sample step), against P (MB blocks of 16
-// packed entries) and X~ (PB blocks of 16 covariates) as A, so no weight
-// matrix is written. Four waves x 16 evaluations per workgroup. The count of
-// evaluations is E, or *E_dev when given (then read on the device, so that a
-// captured launch sequence can evaluate a varying number).
+// 8 (l >> 4) + [0..7] of the 32-sample step), against MB blocks of 16 packed
+// entries of P and PB blocks of 16 covariates of X~ as A, so no weight matrix
+// is written. The A operands come fragment-major (Af, k_frag_pack): per
+// 32-sample step, MB + PB blocks of 4 KiB, each 4 x 1 KiB (lane l's values
+// 2h, 2h + 1 of its 8 in chunk h), staged in LDS by direct 16-byte copies one
+// step ahead (double-buffered) and shared by the four waves of the
+// workgroup (4 x 16 evaluations). The count of evaluations is E, or *E_dev
+// when given (read on the device, so that a captured launch sequence can
+// evaluate a varying number).
 template <int MB, int PB>
-__global__ __launch_bounds__(256) void k_reml_gram_f(const double* __restrict__ P, const double* __restrict__ Xt,
-                                                     const double* __restrict__ Yt, const double* __restrict__ lam,
-                                                     int n, int p, int TA, int E_arg, const int* E_dev,
-                                                     const int32_t* __restrict__ ek, const double* __restrict__ ed,
-                                                     double* __restrict__ CA, double* __restrict__ CB,
-                                                     double* __restrict__ S, int nscal) {
+__host__ __device__ constexpr size_t gram_stage_bytes() {
+  return static_cast<size_t>(MB + PB) * 4096;
+}
+
+// Af[((s * nb + b) * 4 + h) * 128 + 2 lane + j] = M[16 b + (lane & 15)][32 s + 8 (lane >> 4) + 2 h + j]
+// for block b < nb (rows of M from row0: M = src rows, column-major with
+// leading dimension n), zero past the rows or the samples.
+__global__ void k_frag_pack(const double* __restrict__ src, int rows, int n, int nb, int b0, int nb_total,
+                            double* __restrict__ Af) {
+  const int nks = (n + 31) / 32;
+  const size_t total = static_cast<size_t>(nks) * nb * 4 * 128;
+  for (size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+       idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int j = static_cast<int>(idx & 1), lane = static_cast<int>((idx >> 1) & 63);
+    const int h = static_cast<int>((idx >> 7) & 3);
+    const size_t q = idx >> 9;
+    const int b = static_cast<int>(q % nb), st = static_cast<int>(q / nb);
+    const int r = 16 * b + (lane & 15), k = 32 * st + 8 * (lane >> 4) + 2 * h + j;
+    const double v = (r < rows && k < n) ? src[static_cast<size_t>(r) * n + k] : 0.0;
+    Af[((static_cast<size_t>(st) * nb_total + b0 + b) * 4 + h) * 128 + 2 * lane + j] = v;
+  }
+}
+
+template <int MB, int PB>
+__global__ __launch_bounds__(256) void k_reml_gram_f(const double* __restrict__ Af, const double* __restrict__ Yt,
+                                                     const double* __restrict__ lam, int n, int p, int TA, int E_arg,
+                                                     const int* E_dev, const int32_t* __restrict__ ek,
+                                                     const double* __restrict__ ed, double* __restrict__ CA,
+                                                     double* __restrict__ CB, double* __restrict__ S, int nscal) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char gsm[];
+  constexpr int NB = MB + PB;
+  constexpr size_t kStage = gram_stage_bytes<MB, PB>();
   const int E = E_dev ? *E_dev : E_arg;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (static_cast<int>(blockIdx.x) * 64 >= E) return;  // whole workgroup: no barrier is left waiting
   const int e_blk = blockIdx.x * 64 + wave * 16;
-  if (e_blk >= E) return;
   const int e = min(e_blk + (lane & 15), E - 1);
   const double dl = ed[e];
   const double* y = Yt + static_cast<size_t>(ek[e]) * n;
   const int kk = 8 * (lane >> 4);
+  const int nks = (n + 31) / 32;
+  // this wave's copies of a stage: chunks q = wave + 4 i (NB per wave)
+  auto issue = [&](int st, unsigned char* buf) {
+    const unsigned char* src = reinterpret_cast<const unsigned char*>(Af) + static_cast<size_t>(st) * kStage;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      const int q = wave + 4 * i;
+      __builtin_amdgcn_global_load_lds(src + q * 1024 + lane * 16,
+                                       (__attribute__((address_space(3))) void*)(buf + q * 1024), 16, 0, 0);
+    }
+  };
   f64x4 a1[MB], a2[PB];
 #pragma unroll
   for (int b = 0; b < MB; ++b) a1[b] = f64x4{0, 0, 0, 0};
 #pragma unroll
   for (int b = 0; b < PB; ++b) a2[b] = f64x4{0, 0, 0, 0};
   double c0 = 0, lv = 0;
-  for (int k0 = 0; k0 < n; k0 += 32) {
+  issue(0, gsm);
+  for (int st = 0; st < nks; ++st) {
+    unsigned char* cur = gsm + (st & 1) * kStage;
+    // this step's sample values first (older than the next stage's copies,
+    // so waiting for them leaves the copies in flight)
+    double yv[8], lv8[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int i = 32 * st + kk + t;
+      yv[t] = i < n ? y[i] : 0.0;
+      lv8[t] = i < n ? lam[i] : 0.0;
+    }
+    if (st + 1 < nks) issue(st + 1, gsm + ((st + 1) & 1) * kStage);
     f64x8 wd, wy;
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const int i = k0 + kk + t;
+      const int i = 32 * st + kk + t;
       if (i < n) {
-        double v = dl * lam[i] + 1.0;
+        double v = dl * lv8[t] + 1.0;
         if (v < 1e-12) v = 1e-12;
-        const double d = 1.0 / v, yi = y[i];
+        const double d = 1.0 / v, yi = yv[t];
         wd.v[t] = d;
         wy.v[t] = d * yi;
         c0 += yi * (d * yi);
@@ -147,16 +200,23 @@ __global__ __launch_bounds__(256) void k_reml_gram_f(const double* __restrict__ 
         wy.v[t] = 0.0;
       }
     }
+    // the stage of this step (copied one step ago by all four waves) has landed
+    if (st + 1 < nks) __builtin_amdgcn_s_waitcnt((NB & 0xF) | ((NB >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+    else __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
 #pragma unroll
-    for (int b = 0; b < MB; ++b) {
-      const int t = 16 * b + (lane & 15);
-      a1[b] = mfma_f64x8(dg_frag(P + static_cast<size_t>(t < TA ? t : 0) * n, t < TA, k0 + kk, n), wd, a1[b]);
-    }
+    for (int b = 0; b < NB; ++b) {
+      f64x8 af;
 #pragma unroll
-    for (int b = 0; b < PB; ++b) {
-      const int c = 16 * b + (lane & 15);
-      a2[b] = mfma_f64x8(dg_frag(Xt + static_cast<size_t>(c < p ? c : 0) * n, c < p, k0 + kk, n), wy, a2[b]);
+      for (int h = 0; h < 4; ++h) {
+        const double2 v = *reinterpret_cast<const double2*>(cur + (b * 4 + h) * 1024 + lane * 16);
+        af.v[2 * h] = v.x;
+        af.v[2 * h + 1] = v.y;
+      }
+      if (b < MB) a1[b] = mfma_f64x8(af, wd, a1[b]);
+      else a2[b - MB] = mfma_f64x8(af, wy, a2[b - MB]);
     }
+    __syncthreads();  // every wave is done with cur before the copies of step st + 2 overwrite it
   }
   // lanes l, l ^ 16, l ^ 32, l ^ 48 hold the same evaluation
   c0 += __shfl_xor(c0, 16);

@@ -754,14 +754,18 @@ def bench_cis_lmm(args):
 
 GRM_N, GRM_M, GRM_G = 5000, 1_000_000, 20_000
 FP64_PEAK_TFLOPS = 78.6
+INT8_PEAK_TOPS = 5000.0
 
 
 def bench_grm(args):
     """GRM (GCTA form) over n = 5000 samples x 1M SNPs (MAF ~ U(0.01, 0.5), no
-    missing calls); one step = decode + standardise + FP64 Gram accumulation
-    + the GCTA divisor, K copied to the host. Then (untimed by value) the
-    eigendecomposition and the REML adjoint (f, df/dlog delta,
-    d2f/dlog delta^2) of 20k intercept-only genes at their delta."""
+    missing calls); one step = decode + classify + the grouped int8 Gram of
+    the complete SNPs (held 2-bit rows grouped by minor allele count, expanded
+    to int8, exact int32 group Grams added into FP64 with the group weight,
+    rank-1 centring in FP64) + the GCTA divisor, K copied to the host. Then
+    (untimed by value) the eigendecomposition and the REML adjoint (f,
+    df/dlog delta, d2f/dlog delta^2) of 20k intercept-only genes at their
+    delta."""
     import ctypes
     import eqtl_amd
     import oracle as O
@@ -794,12 +798,17 @@ def bench_grm(args):
         log(f"step {s}: {used} SNPs used")
     ctx.synchronize()
     dt = time.perf_counter() - t0
-    g_ms, g_n = ctx.timing("grm")
+    g_ms, g_n = ctx.timing("grm_i8")
+    e_ms, _ = ctx.timing("grm_expand")
+    f_ms, f_n = ctx.timing("grm")
     ctx.set_timing(False)
     snps = m * args.steps
     achieved = float(n) * (n + 1) * snps / (g_ms * 1e-3) / 1e12
-    log(f"GRM step {dt / args.steps:.2f}s, gram kernel {g_ms / args.steps:.0f} ms ({achieved:.1f} TFLOP/s)")
-    extra = {"gram_ms_per_launch": g_ms / max(g_n, 1), "snps_per_launch": BLOCK}
+    log(f"GRM step {dt / args.steps:.3f}s, int8 gram {g_ms / args.steps:.1f} ms in {g_n} launches "
+        f"({achieved:.0f} TOP/s algorithmic), expand {e_ms / args.steps:.1f} ms, FP64 gram {f_ms:.1f} ms")
+    extra = {"gram_i8_ms_per_step": g_ms / args.steps, "gram_i8_launches_per_step": g_n / args.steps,
+             "expand_ms_per_step": e_ms / args.steps, "fp64_gram_ms_per_step": f_ms / args.steps,
+             "snps_per_block": BLOCK}
     # eigendecomposition and the REML adjoint at this n
     t1 = time.perf_counter()
     Q, lam = ctx.eigh(K)
@@ -846,16 +855,16 @@ def bench_grm(args):
                          f"the GCTA divisor)"}
         log("cpu baseline", cpu)
     out = {
-        "metric": "GRM SNPs/s (GCTA form, FP64)",
+        "metric": "GRM SNPs/s (GCTA form, exact int8 group Grams + FP64 weights)",
         "value": snps / dt, "unit": "GRM SNPs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * dt / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic (generated in HBM, seeded)",
+        "dtype": "int8 (exact int32 group sums), f64 weights and centring", "data": "synthetic (generated in HBM, seeded)",
         "config": {"workload": f"GRM n={n} samples x {m} SNPs, no missing calls; REML adjoint for {GRM_G} "
                                f"intercept-only genes", "samples": n, "snps_per_step": m, "snps_per_block": BLOCK},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "note": "k_grm_syrk: algorithmic n(n+1) FLOP per SNP (unique lower triangle incl. the "
-                             "diagonal) over its event-timed launches, FP64 matrix peak"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                     "frac": achieved / INT8_PEAK_TOPS, "traffic": None,
+                     "note": "k_grm_i8: algorithmic n(n+1) int8 ops per SNP (unique lower triangle incl. the "
+                             "diagonal) over its event-timed launches, dense int8 matrix peak"},
         "cpu_baseline": cpu,
         **extra,
     }

@@ -41,6 +41,7 @@ EXPORTS = [
     "eqtl_gpu_synchronize", "eqtl_gpu_selftest", "eqtl_gpu_upload_bed", "eqtl_gpu_cis_perm",
     "eqtl_gpu_beta_approx", "eqtl_gpu_set_geno_format", "eqtl_gpu_vcf_decode", "eqtl_gpu_glm_prep",
     "eqtl_gpu_set_test_subset", "eqtl_gpu_scan_hits", "eqtl_gpu_grm_accumulate_noalloc", "eqtl_gpu_cis_tuning",
+    "eqtl_gpu_grm_set_limits",
 ]
 
 _lib = None
@@ -200,6 +201,11 @@ class Context:
 
     def grm_begin(self):
         _check(lib().eqtl_gpu_grm_begin(self._c))
+
+    def grm_set_limits(self, exact_buffer_bytes=0, chunk_bytes=0):
+        """Device memory of the exact-SNP GRM path (0 keeps a value)."""
+        _check(lib().eqtl_gpu_grm_set_limits(self._c, ctypes.c_int64(exact_buffer_bytes),
+                                             ctypes.c_int64(chunk_bytes)))
 
     def grm_finish(self):
         n = self.n

@@ -168,6 +168,7 @@ struct eqtl_gpu_ctx {
   int n_file = 0, n = 0, n_pad = 0;
   DBuf sample_col;
   std::vector<int32_t> h_sample_col;
+  bool col_ident = false;  // sample_col[i] == i for every analysis sample
   // test subset: n (below) test samples out of n_src analysis samples;
   // test_col = file columns of the test samples (subset active only)
   int n_src = 0;
@@ -234,8 +235,14 @@ struct eqtl_gpu_ctx {
   // VCF decode
   DBuf vcf_text, vcf_recs, vcf_rows, vcf_bad;
   // GRM
-  DBuf grm_acc, grm_miss, grm_cnt, grm_z;
+  DBuf grm_acc, grm_miss, grm_cnt, grm_z, grm_snp;
   int64_t grm_m = 0;
+  // grouped int8 Gram of the exact SNPs: buffered 2-bit rows and allele counts,
+  // u terms (n_pad) then C, and the flush's work arrays
+  DBuf grm_ex_rows, grm_ex_key, grm_u, grm_hist, grm_fill, grm_gstart, grm_wstep, grm_order, grm_X, grm_upart,
+      grm_tiles;
+  int64_t grm_ex_n = 0;
+  size_t grm_buf_bytes = size_t(2) << 30, grm_chunk_bytes = size_t(2) << 30;
   // synthetic bed buffers
   std::vector<DBuf*> synth;
   // timing
@@ -529,6 +536,8 @@ int eqtl_gpu_set_samples(eqtl_gpu_ctx* c, int32_t n_file, const int32_t* sample_
   c->n_src = n;
   c->subset = false;
   c->h_sample_col.assign(sample_col, sample_col + n);
+  c->col_ident = true;
+  for (int i = 0; i < n; ++i) c->col_ident = c->col_ident && sample_col[i] == i;
   c->n_pad = static_cast<int>(round_up(n, 64));
   std::vector<uint8_t> used(static_cast<size_t>(n_file), 0);
   for (int i = 0; i < n; ++i) {
@@ -2299,10 +2308,13 @@ int eqtl_gpu_grm_begin(eqtl_gpu_ctx* c) {
   CHK(ensure(c->grm_acc, nn * sizeof(double)));
   CHK(ensure(c->grm_miss, nn * sizeof(int32_t)));
   CHK(ensure(c->grm_cnt, static_cast<size_t>(c->n_pad) * sizeof(int32_t)));
+  CHK(ensure(c->grm_u, (static_cast<size_t>(c->n_pad) + 1) * sizeof(double)));
   HIPC(hipMemsetAsync(c->grm_acc.p, 0, nn * sizeof(double), c->stream));
   HIPC(hipMemsetAsync(c->grm_miss.p, 0, nn * sizeof(int32_t), c->stream));
   HIPC(hipMemsetAsync(c->grm_cnt.p, 0, static_cast<size_t>(c->n_pad) * sizeof(int32_t), c->stream));
+  HIPC(hipMemsetAsync(c->grm_u.p, 0, (static_cast<size_t>(c->n_pad) + 1) * sizeof(double), c->stream));
   c->grm_m = 0;
+  c->grm_ex_n = 0;
   return EQTL_OK;
 }
 
@@ -2310,23 +2322,182 @@ int eqtl_gpu_grm_begin(eqtl_gpu_ctx* c) {
 
 namespace {
 
-// One block of the GRM accumulation over rows already on the device: the
-// block's standardised dosages z64 and missing flags mis (m_pad x n_pad each,
-// sample-major) and the two block counters `used` (SNPs used, SNPs with a
-// missing call; zeroed by the caller) are caller-provided scratch.
-int grm_accumulate_core(eqtl_gpu_ctx* c, const uint8_t* rd, int64_t m, int32_t bps, double* z64, __bf16* mis,
-                        unsigned long long* used) {
-  const int64_t m_pad = round_up(m, 128);
-  const int np = c->n_pad;
-  auto* kstd = c->geno_fmt == EQTL_GENO_DOSE32 ? &k_grm_standardize<EQTL_GENO_DOSE32> : &k_grm_standardize<EQTL_GENO_BED>;
-  hipLaunchKernelGGL(kstd, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream, rd, m, m_pad,
-                     bps, c->n, np, c->n_file, c->sample_col.as<int32_t>(), z64, mis, c->grm_cnt.as<int32_t>(),
-                     used, c->impute, c->max_miss, c->maf_min);
-  CHK(check_launch("k_grm_standardize"));
-  unsigned long long uu[2] = {0, 0};
-  HIPC(hipMemcpyAsync(uu, used, sizeof(uu), hipMemcpyDeviceToHost, c->stream));
+// Packed rows of the exact buffer: 2-bit dosage codes, 16 samples per dword.
+int grm_rowq(const eqtl_gpu_ctx* c) { return c->n_pad / 16; }
+
+// The grouped int8 Gram over the buffered exact SNPs (then the buffer is
+// empty): allele-count histogram, group starts and per-step weights, the
+// grouped order, then per chunk of steps the int8 operand, its u terms and
+// the Gram launch.
+int grm_exact_flush(eqtl_gpu_ctx* c) {
+  const int64_t cnt = c->grm_ex_n;
+  if (cnt == 0) return EQTL_OK;
+  const int n = c->n, np = c->n_pad, rowq = grm_rowq(c);
+  const int nkeys = n + 1;  // minor allele counts 1 .. n
+  CHK(ensure(c->grm_hist, sizeof(int32_t) * nkeys));
+  CHK(ensure(c->grm_fill, sizeof(int32_t) * nkeys));
+  CHK(ensure(c->grm_gstart, sizeof(int64_t) * (nkeys + 1)));
+  const int64_t max_steps = cnt / 64 + nkeys + 1;
+  CHK(ensure(c->grm_wstep, sizeof(double) * 2 * max_steps));
+  CHK(ensure(c->grm_order, sizeof(int32_t) * max_steps * 64));
+  int32_t* hist = c->grm_hist.as<int32_t>();
+  int64_t* gstart = c->grm_gstart.as<int64_t>();
+  double* wstep = c->grm_wstep.as<double>();
+  double* wmu = wstep + max_steps;
+  int64_t* nsteps_d = gstart + nkeys;
+  const unsigned fill_blocks = static_cast<unsigned>(std::min<int64_t>(4096, (max_steps * 64 + 255) / 256));
+  hipLaunchKernelGGL(k_fill_i32_gs, dim3(256), dim3(256), 0, c->stream, hist, static_cast<int64_t>(nkeys), 0);
+  hipLaunchKernelGGL(k_fill_i32_gs, dim3(fill_blocks), dim3(256), 0, c->stream, c->grm_order.as<int32_t>(),
+                     max_steps * 64, -1);
+  CHK(check_launch("k_fill_i32_gs"));
+  const unsigned rb = static_cast<unsigned>((cnt + 255) / 256);
+  hipLaunchKernelGGL(k_grm_hist, dim3(rb), dim3(256), 0, c->stream, c->grm_ex_key.as<int32_t>(), cnt, hist);
+  CHK(check_launch("k_grm_hist"));
+  hipLaunchKernelGGL(k_grm_groups, dim3(1), dim3(1024), 0, c->stream, hist, nkeys, n, gstart,
+                     c->grm_fill.as<int32_t>(), wstep, wmu, nsteps_d, c->grm_u.as<double>() + np);
+  CHK(check_launch("k_grm_groups"));
+  hipLaunchKernelGGL(k_grm_scatter, dim3(rb), dim3(256), 0, c->stream, c->grm_ex_key.as<int32_t>(), cnt, gstart,
+                     c->grm_fill.as<int32_t>(), c->grm_order.as<int32_t>());
+  CHK(check_launch("k_grm_scatter"));
+  int64_t nsteps = 0;
+  HIPC(hipMemcpyAsync(&nsteps, nsteps_d, sizeof(int64_t), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
-  {
+  if (nsteps <= 0 || nsteps > max_steps) return fail(EQTL_E_HIP, "grm: bad step count of the grouped Gram");
+  const int64_t chunk = std::max<int64_t>(
+      16, std::min<int64_t>(round_up(nsteps, 16), static_cast<int64_t>(c->grm_chunk_bytes / (static_cast<size_t>(np) * 64)) / 16 * 16));
+  CHK(ensure(c->grm_X, static_cast<size_t>(np) * chunk * 64));
+  CHK(ensure(c->grm_upart, sizeof(double) * static_cast<size_t>(chunk / 16) * np));
+  const int nt = (np + kI8T - 1) / kI8T;
+  // tile list: lower-triangle tiles (by << 16 | bx) in blocks of 4 tile rows x
+  // 8 tile columns, row blocks outer
+  std::vector<int32_t> tl;
+  for (int R = 0; R < nt; R += 4)
+    for (int C = 0; C <= R + 3 && C < nt; C += 8)
+      for (int by = R; by < std::min(nt, R + 4); ++by)
+        for (int bx = C; bx < std::min(C + 8, by + 1); ++bx) tl.push_back((by << 16) | bx);
+  const int ntiles = static_cast<int>(tl.size());
+  CHK(ensure(c->grm_tiles, sizeof(int32_t) * tl.size()));
+  HIPC(hipMemcpy(c->grm_tiles.p, tl.data(), sizeof(int32_t) * tl.size(), hipMemcpyHostToDevice));
+  const unsigned i8_grid = static_cast<unsigned>(256 * ((ntiles + 255) / 256));
+  static bool i8_attr = false;
+  if (!i8_attr) {
+    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_grm_i8), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             static_cast<int>(kI8LdsBytes)));
+    i8_attr = true;
+  }
+  for (int64_t s0 = 0; s0 < nsteps; s0 += chunk) {
+    const int64_t s1 = std::min(nsteps, s0 + chunk);
+    const unsigned nb = static_cast<unsigned>((s1 - s0 + 15) / 16);
+    {
+      Timed t(c, "grm_expand");
+      hipLaunchKernelGGL(k_grm_expand, dim3(nb, static_cast<unsigned>((np + 255) / 256)), dim3(256), 0, c->stream,
+                         c->grm_ex_rows.as<uint32_t>(), rowq, c->grm_order.as<int32_t>(), s0, s1, wmu, np,
+                         c->grm_X.as<int8_t>(), c->grm_upart.as<double>());
+      CHK(check_launch("k_grm_expand"));
+      hipLaunchKernelGGL(k_grm_reduce_u, dim3(static_cast<unsigned>((np + 255) / 256)), dim3(256), 0, c->stream,
+                         c->grm_upart.as<double>(), static_cast<int>(nb), np, c->grm_u.as<double>());
+      CHK(check_launch("k_grm_reduce_u"));
+      CHK(t.done());
+    }
+    Timed t(c, "grm_i8");
+    hipLaunchKernelGGL(k_grm_i8, dim3(i8_grid), dim3(64 * kI8Waves), kI8LdsBytes, c->stream, c->grm_X.as<int8_t>(),
+                       static_cast<int>(s1 - s0), np, c->grm_tiles.as<int32_t>(), ntiles, wstep + s0,
+                       c->grm_acc.as<double>());
+    CHK(check_launch("k_grm_i8"));
+    CHK(t.done());
+  }
+  c->grm_ex_n = 0;
+  return EQTL_OK;
+}
+
+// Room for `more` exact rows in the exact buffer: grows it (keeping its rows)
+// up to grm_buf_bytes, else runs the grouped Gram over what it holds.
+int grm_exact_reserve(eqtl_gpu_ctx* c, int64_t more) {
+  const size_t rowb = static_cast<size_t>(grm_rowq(c)) * 4;
+  const int64_t need = c->grm_ex_n + more;
+  if (static_cast<int64_t>(c->grm_ex_rows.bytes / rowb) >= need &&
+      static_cast<int64_t>(c->grm_ex_key.bytes / sizeof(int32_t)) >= need)
+    return EQTL_OK;
+  const int64_t cap = std::max<int64_t>(more, static_cast<int64_t>(c->grm_buf_bytes / rowb));
+  if (need > cap) {
+    CHK(grm_exact_flush(c));
+    return grm_exact_reserve(c, more);
+  }
+  const int64_t rows_new = std::min(cap, std::max(need, 2 * static_cast<int64_t>(c->grm_ex_rows.bytes / rowb)));
+  DBuf r2, k2;
+  CHK(ensure(r2, static_cast<size_t>(rows_new) * rowb));
+  CHK(ensure(k2, static_cast<size_t>(rows_new) * sizeof(int32_t)));
+  if (c->grm_ex_n > 0) {
+    HIPC(hipMemcpyAsync(r2.p, c->grm_ex_rows.p, static_cast<size_t>(c->grm_ex_n) * rowb, hipMemcpyDeviceToDevice,
+                        c->stream));
+    HIPC(hipMemcpyAsync(k2.p, c->grm_ex_key.p, static_cast<size_t>(c->grm_ex_n) * sizeof(int32_t),
+                        hipMemcpyDeviceToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+  }
+  std::swap(c->grm_ex_rows.p, r2.p);
+  std::swap(c->grm_ex_rows.bytes, r2.bytes);
+  std::swap(c->grm_ex_key.p, k2.p);
+  std::swap(c->grm_ex_key.bytes, k2.bytes);
+  return EQTL_OK;
+}
+
+// Per-SNP arrays of one block (class, allele count, mean, sd, position) and
+// the block counters, carved from `base`.
+struct GrmSnpArrays {
+  int64_t* counts;
+  double* mean;
+  double* sd;
+  int32_t* ac;
+  int32_t* pos;
+  int8_t* cls;
+};
+size_t grm_snp_bytes(int64_t m) { return 256 + static_cast<size_t>(round_up(m, 64)) * (8 + 8 + 4 + 4 + 1); }
+GrmSnpArrays grm_snp_arrays(void* base, int64_t m) {
+  const size_t mr = static_cast<size_t>(round_up(m, 64));
+  unsigned char* p = static_cast<unsigned char*>(base);
+  GrmSnpArrays a;
+  a.counts = reinterpret_cast<int64_t*>(p);
+  a.mean = reinterpret_cast<double*>(p + 256);
+  a.sd = a.mean + mr;
+  a.ac = reinterpret_cast<int32_t*>(a.sd + mr);
+  a.pos = a.ac + mr;
+  a.cls = reinterpret_cast<int8_t*>(a.pos + mr);
+  return a;
+}
+
+// One block of the GRM accumulation over rows already on the device. z64 and
+// mis (m_pad x n_pad each, sample-major) and the per-SNP arrays are
+// caller-provided scratch. With exact_ok, SNPs whose calls are all present and
+// integral go to the exact buffer (the grouped int8 Gram, run when the buffer
+// fills and at grm_finish); the others, and every SNP without exact_ok, go
+// through the FP64 Gram of this block.
+int grm_accumulate_core(eqtl_gpu_ctx* c, const uint8_t* rd, int64_t m, int32_t bps, double* z64, __bf16* mis,
+                        const GrmSnpArrays& sa, bool exact_ok) {
+  const int np = c->n_pad;
+  const bool dose = c->geno_fmt == EQTL_GENO_DOSE32;
+  const bool ident = !dose && c->col_ident;
+  auto* kst = dose ? &k_grm_stats<EQTL_GENO_DOSE32, false>
+                   : (ident ? &k_grm_stats<EQTL_GENO_BED, true> : &k_grm_stats<EQTL_GENO_BED, false>);
+  hipLaunchKernelGGL(kst, dim3(static_cast<unsigned>((m + 3) / 4)), dim3(256), 0, c->stream, rd, m, bps, c->n,
+                     c->sample_col.as<int32_t>(), c->impute, c->max_miss, c->maf_min, exact_ok ? 1 : 0, sa.cls,
+                     sa.ac, sa.mean, sa.sd);
+  CHK(check_launch("k_grm_stats"));
+  hipLaunchKernelGGL(k_grm_scan, dim3(1), dim3(1024), 0, c->stream, sa.cls, m, sa.pos, sa.counts);
+  CHK(check_launch("k_grm_scan"));
+  int64_t cn[4] = {0, 0, 0, 0};
+  HIPC(hipMemcpyAsync(cn, sa.counts, sizeof(cn), hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  if (cn[3] > 0) CHK(grm_exact_reserve(c, cn[3]));
+  const int64_t zstride = cn[2] > 0 ? std::max<int64_t>(128, round_up(cn[2], 128)) : 0;
+  auto* kem = dose ? &k_grm_emit<EQTL_GENO_DOSE32, false>
+                   : (ident ? &k_grm_emit<EQTL_GENO_BED, true> : &k_grm_emit<EQTL_GENO_BED, false>);
+  hipLaunchKernelGGL(kem, dim3(static_cast<unsigned>((m + 128 + 3) / 4)), dim3(256), 0, c->stream, rd, m, bps, c->n,
+                     np, c->sample_col.as<int32_t>(), sa.cls, sa.pos, sa.ac, sa.counts, sa.mean, sa.sd, z64, mis,
+                     zstride, c->grm_cnt.as<int32_t>(), c->grm_ex_rows.as<uint32_t>(), c->grm_ex_key.as<int32_t>(),
+                     c->grm_ex_n, grm_rowq(c));
+  CHK(check_launch("k_grm_emit"));
+  c->grm_ex_n += cn[3];
+  if (cn[2] > 0) {
     Timed t(c, "grm");
     static bool attr_set = false;
     if (!attr_set) {
@@ -2334,27 +2505,26 @@ int grm_accumulate_core(eqtl_gpu_ctx* c, const uint8_t* rd, int64_t m, int32_t b
                                static_cast<int>(kGramLdsBytes)));
       attr_set = true;
     }
+    const int zs = static_cast<int>(zstride);
     if (np >= kGramLargeN) {
       const unsigned nt = static_cast<unsigned>((np + kGramT - 1) / kGramT);
-      hipLaunchKernelGGL(k_grm_syrk, dim3(nt, nt), dim3(256), kGramLdsBytes, c->stream, z64, static_cast<int>(m_pad),
-                         np, c->grm_acc.as<double>());
+      hipLaunchKernelGGL(k_grm_syrk, dim3(nt, nt), dim3(256), kGramLdsBytes, c->stream, z64, zs, np,
+                         c->grm_acc.as<double>());
       CHK(check_launch("k_grm_syrk"));
-      if (uu[1]) {
+      if (cn[1]) {
         dim3 grid(static_cast<unsigned>(np / 64), static_cast<unsigned>(np / 64));
-        hipLaunchKernelGGL(k_grm_joint, grid, dim3(256), 0, c->stream, mis, static_cast<int>(m_pad), np,
-                           c->grm_miss.as<int32_t>());
+        hipLaunchKernelGGL(k_grm_joint, grid, dim3(256), 0, c->stream, mis, zs, np, c->grm_miss.as<int32_t>());
         CHK(check_launch("k_grm_joint"));
       }
     } else {
       dim3 grid(static_cast<unsigned>(np / 64), static_cast<unsigned>(np / 64));
-      hipLaunchKernelGGL(k_grm_gram, grid, dim3(256), 0, c->stream, z64, mis, static_cast<int>(m_pad), np,
-                         c->grm_acc.as<double>(), c->grm_miss.as<int32_t>(), uu[1] ? 1 : 0);
+      hipLaunchKernelGGL(k_grm_gram, grid, dim3(256), 0, c->stream, z64, mis, zs, np, c->grm_acc.as<double>(),
+                         c->grm_miss.as<int32_t>(), cn[1] ? 1 : 0);
       CHK(check_launch("k_grm_gram"));
     }
     CHK(t.done());
   }
-  HIPC(hipStreamSynchronize(c->stream));
-  c->grm_m += static_cast<int64_t>(uu[0]);
+  c->grm_m += cn[0];
   return EQTL_OK;
 }
 
@@ -2371,15 +2541,13 @@ int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev
   const int64_t m_pad = round_up(m, 128);
   const int np = c->n_pad;
   CHK(ensure(c->grm_z, static_cast<size_t>(m_pad) * np * (sizeof(double) + sizeof(__bf16))));
+  CHK(ensure(c->grm_snp, grm_snp_bytes(m)));
   int rc;
   const uint8_t* rd = stage_rows(c, rows, on_dev, m, bps, &rc);
   if (rc) return rc;
   double* z64 = c->grm_z.as<double>();
   __bf16* mis = reinterpret_cast<__bf16*>(z64 + static_cast<size_t>(m_pad) * np);
-  DBuf used;
-  CHK(ensure(used, 2 * sizeof(unsigned long long)));
-  HIPC(hipMemsetAsync(used.p, 0, 2 * sizeof(unsigned long long), c->stream));
-  return grm_accumulate_core(c, rd, m, bps, z64, mis, used.as<unsigned long long>());
+  return grm_accumulate_core(c, rd, m, bps, z64, mis, grm_snp_arrays(c->grm_snp.p, m), true);
 }
 
 int eqtl_gpu_grm_accumulate_noalloc(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, int64_t m, int32_t bps) {
@@ -2390,7 +2558,8 @@ int eqtl_gpu_grm_accumulate_noalloc(eqtl_gpu_ctx* c, const uint8_t* rows, int32_
   const int64_t m_pad = round_up(m, 128);
   const size_t np = static_cast<size_t>(c->n_pad);
   const size_t stage = on_dev ? 0 : static_cast<size_t>(round_up(m * static_cast<int64_t>(bps), 256));
-  const size_t need = 256 + stage + static_cast<size_t>(m_pad) * np * (sizeof(double) + sizeof(__bf16));
+  const size_t snp = round_up(static_cast<int64_t>(grm_snp_bytes(m)), 256);
+  const size_t need = snp + stage + static_cast<size_t>(m_pad) * np * (sizeof(double) + sizeof(__bf16));
   if (need > kScratchBytes)
     return fail(EQTL_E_CAPACITY, "grm_accumulate_noalloc: " + std::to_string(m) + " SNPs x " +
                                      std::to_string(c->n) + " samples need " + std::to_string(need) +
@@ -2398,18 +2567,25 @@ int eqtl_gpu_grm_accumulate_noalloc(eqtl_gpu_ctx* c, const uint8_t* rows, int32_
                                      " (pass fewer SNPs per call)");
   CHK(set_device(c));
   unsigned char* sp = static_cast<unsigned char*>(c->scratch);
-  auto* used = reinterpret_cast<unsigned long long*>(sp);
   const uint8_t* rd = rows;
   if (!on_dev) {
-    HIPC(hipMemcpyAsync(sp + 256, rows, static_cast<size_t>(m) * bps, hipMemcpyHostToDevice, c->stream));
-    rd = sp + 256;
+    HIPC(hipMemcpyAsync(sp + snp, rows, static_cast<size_t>(m) * bps, hipMemcpyHostToDevice, c->stream));
+    rd = sp + snp;
   }
-  double* z64 = reinterpret_cast<double*>(sp + 256 + stage);
+  double* z64 = reinterpret_cast<double*>(sp + snp + stage);
   __bf16* mis = reinterpret_cast<__bf16*>(z64 + static_cast<size_t>(m_pad) * np);
-  // the counters keep the previous call's values: zero them (a kernel, not a memset node)
-  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream, used, 2, 0ull);
-  CHK(check_launch("k_fill_u64"));
-  return grm_accumulate_core(c, rd, m, bps, z64, mis, used);
+  // every array the kernels read is written by this call first (class and
+  // position of each SNP, the counters by the scan), so the previous call's
+  // contents do not leak in; the exact buffer is not used (it allocates)
+  return grm_accumulate_core(c, rd, m, bps, z64, mis, grm_snp_arrays(sp, m), false);
+}
+
+int eqtl_gpu_grm_set_limits(eqtl_gpu_ctx* c, int64_t exact_buffer_bytes, int64_t chunk_bytes) {
+  if (!c || exact_buffer_bytes < 0 || chunk_bytes < 0) return fail(EQTL_E_ARG, "bad grm_set_limits arguments");
+  if (c->grm_ex_n > 0) return fail(EQTL_E_STATE, "grm_set_limits: exact SNPs are held; set the limits before accumulating");
+  if (exact_buffer_bytes) c->grm_buf_bytes = static_cast<size_t>(exact_buffer_bytes);
+  if (chunk_bytes) c->grm_chunk_bytes = static_cast<size_t>(chunk_bytes);
+  return EQTL_OK;
 }
 
 int eqtl_gpu_grm_finish(eqtl_gpu_ctx* c, double* K, int32_t* N_packed, int64_t* m_used) {
@@ -2417,13 +2593,15 @@ int eqtl_gpu_grm_finish(eqtl_gpu_ctx* c, double* K, int32_t* N_packed, int64_t* 
   if (!c->grm_acc.p) return fail(EQTL_E_STATE, "grm_begin first");
   if (c->grm_m == 0) return fail(EQTL_E_ARG, "no SNPs for GRM");
   CHK(set_device(c));
+  CHK(grm_exact_flush(c));
   const int n = c->n, np = c->n_pad;
   DBuf Kd, Nd;
   CHK(ensure(Kd, sizeof(double) * static_cast<size_t>(n) * n));
   CHK(ensure(Nd, sizeof(int32_t) * static_cast<size_t>(n) * (n + 1) / 2));
   hipLaunchKernelGGL(k_grm_finish, dim3((n + 15) / 16, (n + 15) / 16), dim3(16, 16), 0, c->stream,
-                     c->grm_acc.as<double>(), c->grm_miss.as<int32_t>(), c->grm_cnt.as<int32_t>(), n, np,
-                     static_cast<int>(c->grm_m), Kd.as<double>(), Nd.as<int32_t>());
+                     c->grm_acc.as<double>(), c->grm_miss.as<int32_t>(), c->grm_cnt.as<int32_t>(),
+                     c->grm_u.as<double>(), c->grm_u.as<double>() + np, n, np, static_cast<int>(c->grm_m),
+                     Kd.as<double>(), Nd.as<int32_t>());
   CHK(check_launch("k_grm_finish"));
   if (K) HIPC(hipMemcpyAsync(K, Kd.p, sizeof(double) * static_cast<size_t>(n) * n, hipMemcpyDeviceToHost, c->stream));
   if (N_packed)

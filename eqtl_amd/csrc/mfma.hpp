@@ -50,6 +50,11 @@ __device__ inline f64x4 mfma_f64x8(const f64x8& a, const f64x8& b, f64x4 c) {
   return c;
 }
 
+// Workgroup barrier without the memory-model fences of __syncthreads(): the
+// caller has already waited for the counters it needs. The "memory" clobber
+// keeps the compiler from moving memory operations across it.
+__device__ inline void lds_barrier() { asm volatile("s_barrier" ::: "memory"); }
+
 // Round-to-nearest-even fp32 -> bf16 and the exact residual split x = hi + lo.
 __device__ inline __bf16 to_bf16(float x) { return static_cast<__bf16>(x); }
 

@@ -401,10 +401,6 @@ constexpr int kWaitVm4Lgkm0 = (2 * kRB) | (0x7 << 4);  // same, and lgkmcnt 0
 constexpr int kWaitLgkm0 = 0xF | (0x7 << 4) | (0x3 << 14);  // lgkmcnt 0 only
 static_assert(2 * kRB < 16, "vmcnt field");
 
-// Workgroup barrier without the memory-model fences of __syncthreads(): the
-// caller has already waited for the counters it needs. The "memory" clobber
-// keeps the compiler from moving memory operations across it.
-__device__ inline void lds_barrier() { asm volatile("s_barrier" ::: "memory"); }
 constexpr int kTileRows = 128;  // SNP rows per workgroup tile
 
 // Per-tile SNP data staged in LDS.

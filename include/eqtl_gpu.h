@@ -146,10 +146,20 @@ int eqtl_gpu_vcf_decode(eqtl_gpu_ctx* ctx, const char* text, int32_t text_on_dev
                         const eqtl_vcf_record* recs, int64_t m, const uint8_t** rows_dev);
 
 /* ---- GRM (compute_grm_from, grm.cpp:112-224) ----------------------------- */
-/* Accumulate m bed rows into the GRM; call repeatedly, then eqtl_gpu_grm_finish. */
+/* Accumulate m bed rows into the GRM; call repeatedly, then eqtl_gpu_grm_finish.
+ * SNPs whose calls are all present and integral are held (2 bits per sample)
+ * until the buffer limit or eqtl_gpu_grm_finish and then summed exactly by
+ * allele-count group on the int8 matrix cores; the others are summed per call
+ * in FP64. Same K either way to FP64 rounding. */
 int eqtl_gpu_grm_begin(eqtl_gpu_ctx* ctx);
 int eqtl_gpu_grm_accumulate(eqtl_gpu_ctx* ctx, const uint8_t* rows, int32_t rows_on_device,
                             int64_t m, int32_t bytes_per_snp);
+/* Device memory of the exact-SNP path (no counterpart in the reference):
+ * bytes of held 2-bit rows before the grouped Gram runs (default 2 GiB) and
+ * bytes of the int8 operand expanded per Gram launch (default 2 GiB; at least
+ * 16 steps of 64 SNPs). 0 keeps a value. Smaller limits cost speed, not
+ * accuracy. */
+int eqtl_gpu_grm_set_limits(eqtl_gpu_ctx* ctx, int64_t exact_buffer_bytes, int64_t chunk_bytes);
 /* eqtl_gpu_grm_accumulate without any device allocation, for callers that
  * cannot allocate (tight serving loops, code under HIP graph capture of the
  * caller's own work, harnesses that forbid hipMalloc). It has no counterpart

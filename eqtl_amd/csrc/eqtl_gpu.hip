@@ -23,6 +23,7 @@
 #include "vcf_kernels.hpp"
 #include "glm_kernels.hpp"
 #include "wald_lmm.hpp"
+#include "wald_lr.hpp"
 #include "grm_kernels.hpp"
 #include "reml_batch.hpp"
 #include "perm_rng.hpp"
@@ -243,6 +244,12 @@ struct eqtl_gpu_ctx {
       grm_tiles;
   int64_t grm_ex_n = 0;
   size_t grm_buf_bytes = size_t(2) << 30, grm_chunk_bytes = size_t(2) << 30;
+  // trans LMM, split form, covariate projections interpolated in ln delta
+  // (k_wald_lr): built by scan_begin; slots hold the genes in ln-delta order
+  bool lr_on = false;
+  int lr_n_sg = 0, lr_ncols = 0;
+  int64_t lr_ldT = 0;
+  DBuf lr_slot_gene, lr_sg_col, lr_sg_R, lr_coef, lr_gb, lr_QZhi, lr_QZlo, lr_Tp;
   // synthetic bed buffers
   std::vector<DBuf*> synth;
   // timing
@@ -1440,6 +1447,7 @@ static int cis_windows(eqtl_gpu_ctx* c) {
   return EQTL_OK;
 }
 
+static int lr_build(eqtl_gpu_ctx* c);
 static int scan_begin_impl(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t scope, double pthr,
                            bool windows_ready);
 
@@ -1491,8 +1499,211 @@ static int scan_begin_impl(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t
   CHK(check_launch("k_state_init"));
   CHK(ensure(c->hit_count, 2 * sizeof(unsigned long long)));
   CHK(ensure_tail(c));
+  c->lr_on = false;
+  if (model == EQTL_MODEL_LMM && prec == EQTL_PREC_BF16X3 && scope == EQTL_SCOPE_TRANS) CHK(lr_build(c));
   c->scanning = true;
   return EQTL_OK;
+}
+
+// ---- low-rank trans LMM scan (k_wald_lr) -------------------------------------
+// Widths of ln-delta segments that 32, 64, 96 and 128 Chebyshev nodes
+// interpolate to ~1e-12 (the whitened covariate projection is analytic in
+// ln delta within pi of the real axis; measured on synthetic spectra).
+constexpr double kLrWidth[4] = {6.5, 13.0, 23.0, 33.5};
+
+// Builds the plan and operands of k_wald_lr for the prepared LMM genes: genes
+// sorted by ln delta into super-groups of 64, segments of ln delta (a
+// super-group's genes inside one segment), per segment R nodes, per node the
+// whitened projection Z (k_lr_node_basis) mapped to sample space (Q Z) and
+// split for the rotation kernel, the genes' interpolation weights and their
+// (d, s o r) tiles. Leaves lr_on false (the eight-wave kernel then runs) when
+// a node basis fails or the deltas span more than 33.5 in ln delta.
+static int lr_build(eqtl_gpu_ctx* c) {
+  c->lr_on = false;
+  const int G = c->G, n = c->n, np = c->n_pad, p = c->p;
+  if (G <= 0 || n - p <= 0 || !c->have_basis) return EQTL_OK;
+  std::vector<double> dl(G);
+  std::vector<int> valid(G);
+  HIPC(hipMemcpyAsync(dl.data(), c->delta.p, sizeof(double) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipMemcpyAsync(valid.data(), c->gvalid.p, sizeof(int) * G, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  std::vector<double> u(G);
+  for (int k = 0; k < G; ++k) {
+    if (!(dl[k] > 0) || !std::isfinite(dl[k])) return EQTL_OK;
+    u[k] = std::log(dl[k]);
+  }
+  std::vector<int> order(G);
+  for (int k = 0; k < G; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    if (valid[x] != valid[y]) return valid[x] > valid[y];
+    return valid[x] ? u[x] < u[y] : false;
+  });
+  const int n_sg = (G + 63) / 64, nslot = n_sg * 64;
+  std::vector<int32_t> slot_gene(nslot, -1), slot_of(G);
+  for (int s2 = 0; s2 < G; ++s2) {
+    slot_gene[s2] = order[s2];
+    slot_of[order[s2]] = s2;
+  }
+  // segments, greedily over the super-groups in ln-delta order
+  struct Seg { double lo, hi; int R, base; };
+  std::vector<Seg> segs;
+  std::vector<int> sg_seg(n_sg, 0);
+  for (int g = 0; g < n_sg; ++g) {
+    double lo = 0, hi = 0;
+    bool any = false;
+    for (int s2 = 64 * g; s2 < std::min(G, 64 * g + 64); ++s2) {
+      const int k = slot_gene[s2];
+      if (!valid[k]) continue;
+      lo = any ? std::min(lo, u[k]) : u[k];
+      hi = any ? std::max(hi, u[k]) : u[k];
+      any = true;
+    }
+    if (!any) {
+      if (!segs.empty()) lo = hi = segs.back().lo;
+    }
+    if (!segs.empty() && std::max(hi, segs.back().hi) - std::min(lo, segs.back().lo) <= kLrWidth[0] &&
+        segs.back().hi - segs.back().lo <= kLrWidth[0]) {
+      segs.back().lo = std::min(lo, segs.back().lo);
+      segs.back().hi = std::max(hi, segs.back().hi);
+    } else {
+      segs.push_back(Seg{lo, hi, 0, 0});
+    }
+    sg_seg[g] = static_cast<int>(segs.size()) - 1;
+  }
+  int ncols = 0;
+  for (Seg& sgm : segs) {
+    if (sgm.hi - sgm.lo < 0.5) {
+      const double mid = 0.5 * (sgm.lo + sgm.hi);
+      sgm.lo = mid - 0.25;
+      sgm.hi = mid + 0.25;
+    }
+    int kx = 0;
+    while (kx < 4 && sgm.hi - sgm.lo > kLrWidth[kx]) ++kx;
+    if (kx == 4) return EQTL_OK;  // wider than 128 nodes interpolate: the eight-wave kernel runs
+    sgm.R = 32 * (kx + 1);
+    sgm.base = ncols;
+    ncols += sgm.R * p;
+  }
+  const int64_t ldT = round_up(ncols, 32);
+  // nodes (Chebyshev points of the first kind) and their columns
+  std::vector<double> nd;
+  std::vector<int32_t> col0, stride;
+  for (const Seg& sgm : segs)
+    for (int r = 0; r < sgm.R; ++r) {
+      const double x = std::cos((2.0 * r + 1.0) * M_PI / (2.0 * sgm.R));
+      nd.push_back(std::exp(0.5 * (sgm.lo + sgm.hi) + 0.5 * (sgm.hi - sgm.lo) * x));
+      col0.push_back(sgm.base + r);
+      stride.push_back(sgm.R);
+    }
+  const int nnodes = static_cast<int>(nd.size());
+  // interpolation weights per slot (barycentric form), split hi / mid / lo
+  const size_t ncoef = static_cast<size_t>(nslot / 16) * 4 * 1536;
+  std::vector<__bf16> coef(ncoef, static_cast<__bf16>(0.f));
+  std::vector<int32_t> sgc(n_sg), sgR(n_sg);
+  for (int g = 0; g < n_sg; ++g) {
+    sgc[g] = segs[sg_seg[g]].base;
+    sgR[g] = segs[sg_seg[g]].R;
+  }
+  std::vector<double> ur, wr, ell;
+  for (int s2 = 0; s2 < G; ++s2) {
+    const int k = slot_gene[s2];
+    if (!valid[k]) continue;
+    const Seg& sgm = segs[sg_seg[s2 / 64]];
+    const int R = sgm.R;
+    ur.resize(R);
+    wr.resize(R);
+    ell.assign(R, 0.0);
+    int hit = -1;
+    for (int r = 0; r < R; ++r) {
+      const double th = (2.0 * r + 1.0) * M_PI / (2.0 * R);
+      ur[r] = 0.5 * (sgm.lo + sgm.hi) + 0.5 * (sgm.hi - sgm.lo) * std::cos(th);
+      wr[r] = ((r & 1) ? -1.0 : 1.0) * std::sin(th);
+      if (u[k] == ur[r]) hit = r;
+    }
+    if (hit >= 0) {
+      ell[hit] = 1.0;
+    } else {
+      double den = 0.0;
+      for (int r = 0; r < R; ++r) den += wr[r] / (u[k] - ur[r]);
+      for (int r = 0; r < R; ++r) ell[r] = wr[r] / (u[k] - ur[r]) / den;
+    }
+    for (int r = 0; r < R; ++r) {
+      const __bf16 h = static_cast<__bf16>(static_cast<float>(ell[r]));
+      const double r1 = ell[r] - static_cast<double>(static_cast<float>(h));
+      const __bf16 m = static_cast<__bf16>(static_cast<float>(r1));
+      const double r2 = r1 - static_cast<double>(static_cast<float>(m));
+      coef[lr_coef_index(s2, r, 0)] = h;
+      coef[lr_coef_index(s2, r, 1)] = m;
+      coef[lr_coef_index(s2, r, 2)] = static_cast<__bf16>(static_cast<float>(r2));
+    }
+  }
+  // node bases Z (np x ncols), sample-space Q Z, split
+  DBuf Z, QZ, nodes, okb;
+  const size_t zc = static_cast<size_t>(np) * ncols;
+  CHK(ensure(Z, sizeof(double) * zc));
+  CHK(ensure(QZ, sizeof(double) * zc));
+  CHK(ensure(nodes, sizeof(double) * nnodes + 2 * sizeof(int32_t) * nnodes + 64));
+  CHK(ensure(okb, sizeof(int) * nnodes));
+  double* nd_d = nodes.as<double>();
+  int32_t* c0_d = reinterpret_cast<int32_t*>(nd_d + nnodes);
+  int32_t* st_d = c0_d + nnodes;
+  HIPC(hipMemcpyAsync(nd_d, nd.data(), sizeof(double) * nnodes, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c0_d, col0.data(), sizeof(int32_t) * nnodes, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(st_d, stride.data(), sizeof(int32_t) * nnodes, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_lr_node_basis, dim3(nnodes), dim3(256), 0, c->stream, c->Xt64.as<double>(),
+                     c->zeros.as<double>(), c->lam64.as<double>(), n, np, p, nd_d, c0_d, st_d, Z.as<double>(),
+                     okb.as<int>());
+  CHK(check_launch("k_lr_node_basis"));
+  HIPC(hipMemsetAsync(QZ.p, 0, sizeof(double) * zc, c->stream));
+  const double one = 1.0, zero = 0.0;
+  if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, ncols, n, &one, c->Q64.as<double>(), n,
+                    Z.as<double>(), np, &zero, QZ.as<double>(), np) != rocblas_status_success)
+    return fail(EQTL_E_HIP, "rocblas_dgemm failed");
+  CHK(ensure(c->lr_QZhi, sizeof(__bf16) * zc));
+  CHK(ensure(c->lr_QZlo, sizeof(__bf16) * zc));
+  hipLaunchKernelGGL(k_split_bf16_f64, dim3(1024), dim3(256), 0, c->stream, QZ.as<double>(), static_cast<int64_t>(zc),
+                     c->lr_QZhi.as<__bf16>(), c->lr_QZlo.as<__bf16>());
+  CHK(check_launch("k_split_bf16_f64"));
+  // gene tiles by slot
+  const int nks = np / 32;
+  const size_t gbn = static_cast<size_t>(n_sg) * nks * (kLrTileBytes / sizeof(__bf16));
+  CHK(ensure(c->lr_gb, sizeof(__bf16) * gbn));
+  HIPC(hipMemsetAsync(c->lr_gb.p, 0, sizeof(__bf16) * gbn, c->stream));
+  DBuf sof;
+  CHK(ensure(sof, sizeof(int32_t) * G));
+  HIPC(hipMemcpyAsync(sof.p, slot_of.data(), sizeof(int32_t) * G, hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_lr_gene_tiles, dim3(G), dim3(256), 0, c->stream, c->lam64.as<double>(), c->delta.as<double>(),
+                     c->U64.as<double>(), sof.as<int32_t>(), n, nks, G, c->lr_gb.as<__bf16>());
+  CHK(check_launch("k_lr_gene_tiles"));
+  CHK(ensure(c->lr_slot_gene, sizeof(int32_t) * nslot));
+  CHK(ensure(c->lr_sg_col, sizeof(int32_t) * n_sg));
+  CHK(ensure(c->lr_sg_R, sizeof(int32_t) * n_sg));
+  CHK(ensure(c->lr_coef, sizeof(__bf16) * ncoef));
+  HIPC(hipMemcpyAsync(c->lr_slot_gene.p, slot_gene.data(), sizeof(int32_t) * nslot, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->lr_sg_col.p, sgc.data(), sizeof(int32_t) * n_sg, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->lr_sg_R.p, sgR.data(), sizeof(int32_t) * n_sg, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->lr_coef.p, coef.data(), sizeof(__bf16) * ncoef, hipMemcpyHostToDevice, c->stream));
+  std::vector<int> ok(nnodes);
+  HIPC(hipMemcpyAsync(ok.data(), okb.p, sizeof(int) * nnodes, hipMemcpyDeviceToHost, c->stream));
+  HIPC(hipStreamSynchronize(c->stream));
+  for (int v : ok)
+    if (!v) return EQTL_OK;
+  c->lr_n_sg = n_sg;
+  c->lr_ncols = ncols;
+  c->lr_ldT = ldT;
+  c->lr_on = true;
+  return EQTL_OK;
+}
+
+int launch_wald_lr(eqtl_gpu_ctx* c, const LrArgs& la, dim3 grid) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald_lr), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             static_cast<int>(kLrLdsBytes)));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(k_wald_lr, grid, dim3(kW8Threads), kLrLdsBytes, c->stream, la);
+  return check_launch("k_wald_lr");
 }
 
 // Cis work items of one scan block over [snp0, snp0 + m): per gene group the
@@ -1676,7 +1887,15 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
       }
       const dim3 g2(static_cast<unsigned>((np + kRotT - 1) / kRotT), static_cast<unsigned>(m_pad / kRotT));
       hipLaunchKernelGGL(k_rotate_lds, g2, dim3(256), kRotLds, c->stream, c->A32.as<float>(), c->Qhi.as<__bf16>(),
-                         c->Qlo.as<__bf16>(), c->Gt.as<float>(), np);
+                         c->Qlo.as<__bf16>(), c->Gt.as<float>(), np, np, static_cast<int64_t>(np));
+      if (c->lr_on) {
+        // the SNPs' projections at the interpolation nodes: A x (Q Z)
+        CHK(ensure(c->lr_Tp, sizeof(float) * static_cast<size_t>(m_pad) * c->lr_ldT));
+        const dim3 g3(static_cast<unsigned>((c->lr_ncols + kRotT - 1) / kRotT), static_cast<unsigned>(m_pad / kRotT));
+        hipLaunchKernelGGL(k_rotate_lds, g3, dim3(256), kRotLds, c->stream, c->A32.as<float>(),
+                           c->lr_QZhi.as<__bf16>(), c->lr_QZlo.as<__bf16>(), c->lr_Tp.as<float>(), np, c->lr_ncols,
+                           c->lr_ldT);
+      }
     }
     CHK(check_launch("k_rotate"));
     CHK(t.done());
@@ -1704,7 +1923,9 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   // SNP tiles against one group at a time, so the group's column blocks are
   // read from HBM once and served from that XCD's L2 to all of its CUs.
   const int cus = std::max(c->n_cu, 8);
-  int n_splits = std::max(cus / 8, (2 * cus + n_groups - 1) / n_groups);
+  const bool lr = c->lr_on && !cis && c->scan_model == EQTL_MODEL_LMM;
+  const int n_units = lr ? c->lr_n_sg : n_groups;  // the low-rank kernel's unit is a super-group of 64 genes
+  int n_splits = std::max(cus / 8, (2 * cus + n_units - 1) / n_units);
   n_splits = std::max(1, std::min(n_tiles128, n_splits));
   CHK(ensure(c->partial, sizeof(GeneAcc) * (cis ? static_cast<size_t>(n_items) * 16
                                                 : static_cast<size_t>(n_splits) * c->G_pad)));
@@ -1785,7 +2006,22 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     HIPC(hipMemsetAsync(c->hit_count.p, 0, 2 * sizeof(unsigned long long), c->stream));
     {
       Timed t(c, "wald");
-      CHK(launch_wald(c, a, dim3(static_cast<unsigned>(cis ? n_items : n_groups_pad * n_splits))));
+      if (lr) {
+        LrArgs la;
+        la.w = a;
+        la.Tp = c->lr_Tp.as<float>();
+        la.ldT = c->lr_ldT;
+        la.sg_col = c->lr_sg_col.as<int32_t>();
+        la.sg_R = c->lr_sg_R.as<int32_t>();
+        la.slot_gene = c->lr_slot_gene.as<int32_t>();
+        la.lcoef = c->lr_coef.as<__bf16>();
+        la.gb = c->lr_gb.as<__bf16>();
+        la.n_sg = c->lr_n_sg;
+        la.p = c->p;
+        CHK(launch_wald_lr(c, la, dim3(static_cast<unsigned>(round_up(c->lr_n_sg, 8) * n_splits))));
+      } else {
+        CHK(launch_wald(c, a, dim3(static_cast<unsigned>(cis ? n_items : n_groups_pad * n_splits))));
+      }
       CHK(t.done());
     }
     unsigned long long cnt2[2] = {0, 0};

@@ -243,6 +243,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   }
 }
 
+// Interpolation nodes of the low-rank trans LMM scan: at node k (REML delta
+// node_delta[k]) the whitened covariate projection Z = D X~ Lc^-T (n x p, the
+// columns s o W of k_lmm_basis), written to column col0[k] + a * stride[k] of
+// Z (column-major, leading dimension n_pad, rows past n zero). One workgroup
+// per node; ok_out[k] = 0 when X~^T D X~ is not positive semidefinite.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_lr_node_basis(
+    const double* Xt, const double* zeros, const double* lam, int n, int n_pad, int p, const double* node_delta,
+    const int32_t* col0, const int32_t* stride, double* Z, int* ok_out) {
+  __shared__ RemlSmem sm;
+  const int k = blockIdx.x;
+  const double dk = node_delta[k];
+  reml_pass1(Xt, zeros, lam, n, p, dk, sm);
+  const bool ok = chol_lower_block(sm, p);
+  const size_t c0 = static_cast<size_t>(col0[k]), cs = static_cast<size_t>(stride[k]);
+  for (int i = threadIdx.x; i < n_pad; i += 256) {
+    if (i >= n || !ok) {
+      for (int c = 0; c < p; ++c) Z[(c0 + c * cs) * n_pad + i] = 0.0;
+      continue;
+    }
+    asm volatile("" ::: "memory");
+    double v = dk * lam[i] + 1.0;
+    if (v < 1e-12) v = 1e-12;
+    const double s = sqrt(1.0 / v);
+    double w[kPMax];
+#pragma unroll
+    for (int c = 0; c < kPMax; ++c) {
+      if (c < p) {
+        double t = s * Xt[static_cast<size_t>(c) * n + i];
+        for (int q = 0; q < c; ++q) t -= w[q] * sm.A1[q * kPMax + c];
+        const double lcc = sm.A1[c * kPMax + c];
+        w[c] = lcc > 0 ? t / lcc : 0.0;
+        Z[(c0 + c * cs) * n_pad + i] = s * w[c];
+      }
+    }
+  }
+  if (threadIdx.x == 0) ok_out[k] = ok ? 1 : 0;
+}
+
 // LM: orthonormal basis Wx (n x p, col-major) of X (single workgroup).
 __global__ __launch_bounds__(256) void k_lm_basis(const double* X, const double* zeros, int n, int p, double* Wx,
                                                   int* ok_out) {

@@ -228,9 +228,11 @@ constexpr size_t kRotA = 8 * kRotT * 16;       // FP32 rows: 16 KiB per step
 constexpr size_t kRotStage = kRotA + 8 * kRotT * 16;  // + Qhi and Qlo: 32 KiB
 constexpr size_t kRotLds = 2 * kRotStage;
 
+// Also used for any B of n_cols coordinate-major columns (output leading
+// dimension ld_out); the rotation is n_cols = ld_out = n_pad.
 __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A, const __bf16* __restrict__ Qhi,
                                                     const __bf16* __restrict__ Qlo, float* __restrict__ Gt,
-                                                    int n_pad) {
+                                                    int n_pad, int n_cols, int64_t ld_out) {
   extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = wave >> 1, wc = wave & 1;
@@ -251,7 +253,7 @@ __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A,
       dst_off[i] = (c * kRotT + h * 64) * 16;
     } else {
       const int c = (q & 7) >> 1;
-      const int coord = min(C0 + h * 64 + lane, n_pad - 1);
+      const int coord = min(C0 + h * 64 + lane, n_cols - 1);
       const __bf16* base = q < 24 ? Qhi : Qlo;
       src[i] = reinterpret_cast<const unsigned char*>(base + static_cast<size_t>(coord) * n_pad + 8 * c);
       step_bytes[i] = 32 * static_cast<int>(sizeof(__bf16));
@@ -309,11 +311,11 @@ __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A,
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) {
       const int col = C0 + wc * 64 + cb * 16 + (lane & 15);
-      if (col >= n_pad) continue;
+      if (col >= n_cols) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t row = R0 + wr * 64 + rb * 16 + 4 * (lane >> 4) + r;
-        Gt[row * n_pad + col] = acc[rb][cb][r];
+        Gt[row * ld_out + col] = acc[rb][cb][r];
       }
     }
 }

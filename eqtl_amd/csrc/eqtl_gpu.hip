@@ -247,9 +247,10 @@ struct eqtl_gpu_ctx {
   // trans LMM, split form, covariate projections interpolated in ln delta
   // (k_wald_lr): built by scan_begin; slots hold the genes in ln-delta order
   bool lr_on = false;
-  int lr_n_sg = 0, lr_ncols = 0;
+  int lr_n_sg = 0, lr_ncols = 0, lr_nnodes = 0;
   int64_t lr_ldT = 0;
-  DBuf lr_slot_gene, lr_sg_col, lr_sg_R, lr_coef, lr_gb, lr_QZhi, lr_QZlo, lr_Tp;
+  DBuf lr_slot_gene, lr_sg_col, lr_sg_R, lr_coef, lr_gscale, lr_gb, lr_QZhi, lr_QZlo, lr_Dhi, lr_Dlo, lr_Tp,
+      lr_G2;
   // synthetic bed buffers
   std::vector<DBuf*> synth;
   // timing
@@ -1584,7 +1585,9 @@ static int lr_build(eqtl_gpu_ctx* c) {
     sgm.base = ncols;
     ncols += sgm.R * p;
   }
-  const int64_t ldT = round_up(ncols, 32);
+  int nnodes_total = 0;
+  for (const Seg& sgm : segs) nnodes_total += sgm.R;
+  const int64_t ldT = round_up(ncols + nnodes_total, 32);
   // nodes (Chebyshev points of the first kind) and their columns
   std::vector<double> nd;
   std::vector<int32_t> col0, stride;
@@ -1599,9 +1602,14 @@ static int lr_build(eqtl_gpu_ctx* c) {
   // interpolation weights per slot (barycentric form), split hi / mid / lo
   const size_t ncoef = static_cast<size_t>(nslot / 16) * 4 * 1536;
   std::vector<__bf16> coef(ncoef, static_cast<__bf16>(0.f));
-  std::vector<int32_t> sgc(n_sg), sgR(n_sg);
+  std::vector<int32_t> sgc(n_sg), sgR(n_sg), sgg(n_sg), gbase(segs.size());
+  for (size_t q = 0, acc = 0; q < segs.size(); ++q) {
+    gbase[q] = static_cast<int32_t>(ncols + acc);  // the segment's node sums of squares follow all projections
+    acc += segs[q].R;
+  }
   for (int g = 0; g < n_sg; ++g) {
     sgc[g] = segs[sg_seg[g]].base;
+    sgg[g] = gbase[sg_seg[g]];
     sgR[g] = segs[sg_seg[g]].R;
   }
   std::vector<double> ur, wr, ell;
@@ -1664,6 +1672,35 @@ static int lr_build(eqtl_gpu_ctx* c) {
   hipLaunchKernelGGL(k_split_bf16_f64, dim3(1024), dim3(256), 0, c->stream, QZ.as<double>(), static_cast<int64_t>(zc),
                      c->lr_QZhi.as<__bf16>(), c->lr_QZlo.as<__bf16>());
   CHK(check_launch("k_split_bf16_f64"));
+  // node weights d(delta_r, lam_c) / d(delta_r, lbar) per rotated coordinate,
+  // coordinate-major per node (the B operand of the squared-row product),
+  // split; per slot the scale d(delta_k, lbar) of the interpolated gg
+  std::vector<double> gsc(nslot, 0.0);
+  {
+    std::vector<double> lam(n);
+    HIPC(hipMemcpyAsync(lam.data(), c->lam64.p, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+    double lbar = 0.0;
+    for (int i = 0; i < n; ++i) lbar += lam[i];
+    lbar = lbar > 0 ? lbar / n : 1.0;
+    for (int s2 = 0; s2 < G; ++s2) gsc[s2] = 1.0 / (dl[slot_gene[s2]] * lbar + 1.0);
+    const size_t dn = static_cast<size_t>(nnodes_total) * np;
+    std::vector<__bf16> dh(dn, static_cast<__bf16>(0.f)), dlo(dn, static_cast<__bf16>(0.f));
+    for (int k = 0; k < nnodes; ++k)
+      for (int i = 0; i < n; ++i) {
+        double v = nd[k] * lam[i] + 1.0;
+        if (v < 1e-12) v = 1e-12;
+        const float f = static_cast<float>((nd[k] * lbar + 1.0) / v);
+        const __bf16 h = static_cast<__bf16>(f);
+        dh[static_cast<size_t>(k) * np + i] = h;
+        dlo[static_cast<size_t>(k) * np + i] = static_cast<__bf16>(f - static_cast<float>(h));
+      }
+    CHK(ensure(c->lr_Dhi, sizeof(__bf16) * dn));
+    CHK(ensure(c->lr_Dlo, sizeof(__bf16) * dn));
+    HIPC(hipMemcpyAsync(c->lr_Dhi.p, dh.data(), sizeof(__bf16) * dn, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipMemcpyAsync(c->lr_Dlo.p, dlo.data(), sizeof(__bf16) * dn, hipMemcpyHostToDevice, c->stream));
+    HIPC(hipStreamSynchronize(c->stream));
+  }
   // gene tiles by slot
   const int nks = np / 32;
   const size_t gbn = static_cast<size_t>(n_sg) * nks * (kLrTileBytes / sizeof(__bf16));
@@ -1672,15 +1709,19 @@ static int lr_build(eqtl_gpu_ctx* c) {
   DBuf sof;
   CHK(ensure(sof, sizeof(int32_t) * G));
   HIPC(hipMemcpyAsync(sof.p, slot_of.data(), sizeof(int32_t) * G, hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_lr_gene_tiles, dim3(G), dim3(256), 0, c->stream, c->lam64.as<double>(), c->delta.as<double>(),
-                     c->U64.as<double>(), sof.as<int32_t>(), n, nks, G, c->lr_gb.as<__bf16>());
+  hipLaunchKernelGGL(k_lr_gene_tiles, dim3(G), dim3(256), 0, c->stream, c->U64.as<double>(), sof.as<int32_t>(), n, nks,
+                     G, c->lr_gb.as<__bf16>());
   CHK(check_launch("k_lr_gene_tiles"));
   CHK(ensure(c->lr_slot_gene, sizeof(int32_t) * nslot));
-  CHK(ensure(c->lr_sg_col, sizeof(int32_t) * n_sg));
+  CHK(ensure(c->lr_sg_col, sizeof(int32_t) * 2 * n_sg));
   CHK(ensure(c->lr_sg_R, sizeof(int32_t) * n_sg));
   CHK(ensure(c->lr_coef, sizeof(__bf16) * ncoef));
+  CHK(ensure(c->lr_gscale, sizeof(double) * nslot));
+  HIPC(hipMemcpyAsync(c->lr_gscale.p, gsc.data(), sizeof(double) * nslot, hipMemcpyHostToDevice, c->stream));
   HIPC(hipMemcpyAsync(c->lr_slot_gene.p, slot_gene.data(), sizeof(int32_t) * nslot, hipMemcpyHostToDevice, c->stream));
   HIPC(hipMemcpyAsync(c->lr_sg_col.p, sgc.data(), sizeof(int32_t) * n_sg, hipMemcpyHostToDevice, c->stream));
+  HIPC(hipMemcpyAsync(c->lr_sg_col.as<int32_t>() + n_sg, sgg.data(), sizeof(int32_t) * n_sg, hipMemcpyHostToDevice,
+                      c->stream));
   HIPC(hipMemcpyAsync(c->lr_sg_R.p, sgR.data(), sizeof(int32_t) * n_sg, hipMemcpyHostToDevice, c->stream));
   HIPC(hipMemcpyAsync(c->lr_coef.p, coef.data(), sizeof(__bf16) * ncoef, hipMemcpyHostToDevice, c->stream));
   std::vector<int> ok(nnodes);
@@ -1690,6 +1731,7 @@ static int lr_build(eqtl_gpu_ctx* c) {
     if (!v) return EQTL_OK;
   c->lr_n_sg = n_sg;
   c->lr_ncols = ncols;
+  c->lr_nnodes = nnodes_total;
   c->lr_ldT = ldT;
   c->lr_on = true;
   return EQTL_OK;
@@ -1881,20 +1923,35 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     else {
       static bool attr_set = false;
       if (!attr_set) {
-        HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rotate_lds),
+        HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rotate_lds<false>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kRotLds)));
+        HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rotate_lds<true>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kRotLds)));
         attr_set = true;
       }
       const dim3 g2(static_cast<unsigned>((np + kRotT - 1) / kRotT), static_cast<unsigned>(m_pad / kRotT));
-      hipLaunchKernelGGL(k_rotate_lds, g2, dim3(256), kRotLds, c->stream, c->A32.as<float>(), c->Qhi.as<__bf16>(),
-                         c->Qlo.as<__bf16>(), c->Gt.as<float>(), np, np, static_cast<int64_t>(np));
-      if (c->lr_on) {
-        // the SNPs' projections at the interpolation nodes: A x (Q Z)
-        CHK(ensure(c->lr_Tp, sizeof(float) * static_cast<size_t>(m_pad) * c->lr_ldT));
+      const bool lr_rows = c->lr_on && c->scan_scope == EQTL_SCOPE_TRANS;
+      if (lr_rows) CHK(ensure(c->lr_G2, sizeof(__bf16) * 2 * static_cast<size_t>(m_pad) * np));
+      __bf16* ghi = lr_rows ? c->lr_G2.as<__bf16>() : nullptr;
+      hipLaunchKernelGGL(k_rotate_lds<false>, g2, dim3(256), kRotLds, c->stream, c->A32.as<float>(),
+                         c->Qhi.as<__bf16>(), c->Qlo.as<__bf16>(), c->Gt.as<float>(), np, np,
+                         static_cast<int64_t>(np), ghi, ghi ? ghi + static_cast<size_t>(m_pad) * np : nullptr,
+                         nullptr);
+      if (lr_rows) {
+        // the SNPs' covariate projections at the interpolation nodes, A x (Q Z),
+        // then their weighted sums of squares, (Q^T g)^2 x D(nodes): written
+        // as the hi / mid / lo planes the interpolation reads
+        const size_t tpl = static_cast<size_t>(m_pad) * c->lr_ldT;
+        CHK(ensure(c->lr_Tp, sizeof(__bf16) * 3 * tpl));
+        __bf16* th = c->lr_Tp.as<__bf16>();
         const dim3 g3(static_cast<unsigned>((c->lr_ncols + kRotT - 1) / kRotT), static_cast<unsigned>(m_pad / kRotT));
-        hipLaunchKernelGGL(k_rotate_lds, g3, dim3(256), kRotLds, c->stream, c->A32.as<float>(),
-                           c->lr_QZhi.as<__bf16>(), c->lr_QZlo.as<__bf16>(), c->lr_Tp.as<float>(), np, c->lr_ncols,
-                           c->lr_ldT);
+        hipLaunchKernelGGL(k_rotate_lds<false>, g3, dim3(256), kRotLds, c->stream, c->A32.as<float>(),
+                           c->lr_QZhi.as<__bf16>(), c->lr_QZlo.as<__bf16>(), nullptr, np, c->lr_ncols, c->lr_ldT, th,
+                           th + 2 * tpl, th + tpl);
+        const dim3 g4(static_cast<unsigned>((c->lr_nnodes + kRotT - 1) / kRotT), static_cast<unsigned>(m_pad / kRotT));
+        hipLaunchKernelGGL(k_rotate_lds<true>, g4, dim3(256), kRotLds, c->stream, c->Gt.as<float>(),
+                           c->lr_Dhi.as<__bf16>(), c->lr_Dlo.as<__bf16>(), nullptr, np, c->lr_nnodes, c->lr_ldT,
+                           th + c->lr_ncols, th + 2 * tpl + c->lr_ncols, th + tpl + c->lr_ncols);
       }
     }
     CHK(check_launch("k_rotate"));
@@ -2009,12 +2066,17 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
       if (lr) {
         LrArgs la;
         la.w = a;
-        la.Tp = c->lr_Tp.as<float>();
+        la.Ghi = c->lr_G2.as<__bf16>();
+        la.lo_off = m_pad * np;
+        la.Tp = c->lr_Tp.as<__bf16>();
+        la.tpl = static_cast<int64_t>(m_pad) * c->lr_ldT;
         la.ldT = c->lr_ldT;
         la.sg_col = c->lr_sg_col.as<int32_t>();
+        la.sg_gcol = c->lr_sg_col.as<int32_t>() + c->lr_n_sg;
         la.sg_R = c->lr_sg_R.as<int32_t>();
         la.slot_gene = c->lr_slot_gene.as<int32_t>();
         la.lcoef = c->lr_coef.as<__bf16>();
+        la.gscale = c->lr_gscale.as<double>();
         la.gb = c->lr_gb.as<__bf16>();
         la.n_sg = c->lr_n_sg;
         la.p = c->p;

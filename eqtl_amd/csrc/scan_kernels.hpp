@@ -229,10 +229,15 @@ constexpr size_t kRotStage = kRotA + 8 * kRotT * 16;  // + Qhi and Qlo: 32 KiB
 constexpr size_t kRotLds = 2 * kRotStage;
 
 // Also used for any B of n_cols coordinate-major columns (output leading
-// dimension ld_out); the rotation is n_cols = ld_out = n_pad.
+// dimension ld_out); the rotation is n_cols = ld_out = n_pad. SQUARE: the A
+// values are squared before the split (sum_i A_ji^2 B_ic). Outputs (each
+// nullable, leading dimension ld_out): Gt fp32; Ohi / Olo the bf16 hi / lo
+// split of the fp32 value; with Omid, the exact three-way split hi / mid / lo.
+template <bool SQUARE>
 __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A, const __bf16* __restrict__ Qhi,
                                                     const __bf16* __restrict__ Qlo, float* __restrict__ Gt,
-                                                    int n_pad, int n_cols, int64_t ld_out) {
+                                                    int n_pad, int n_cols, int64_t ld_out, __bf16* __restrict__ Ohi,
+                                                    __bf16* __restrict__ Olo, __bf16* __restrict__ Omid) {
   extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = wave >> 1, wc = wave & 1;
@@ -294,6 +299,10 @@ __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A,
       f32x8 av;
       av.v[0] = lo.x; av.v[1] = lo.y; av.v[2] = lo.z; av.v[3] = lo.w;
       av.v[4] = hi.x; av.v[5] = hi.y; av.v[6] = hi.z; av.v[7] = hi.w;
+      if constexpr (SQUARE) {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) av.v[t] *= av.v[t];
+      }
       bf16x8 ah, al;
       split_bf16(av, ah, al);
 #pragma unroll
@@ -315,7 +324,21 @@ __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A,
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t row = R0 + wr * 64 + rb * 16 + 4 * (lane >> 4) + r;
-        Gt[row * ld_out + col] = acc[rb][cb][r];
+        const float v = acc[rb][cb][r];
+        const int64_t o = row * ld_out + col;
+        if (Gt) Gt[o] = v;
+        if (Ohi) {
+          const __bf16 h = to_bf16(v);
+          const float r1 = v - static_cast<float>(h);
+          Ohi[o] = h;
+          if (Omid) {
+            const __bf16 m = to_bf16(r1);
+            Omid[o] = m;
+            Olo[o] = to_bf16(r1 - static_cast<float>(m));
+          } else {
+            Olo[o] = to_bf16(r1);
+          }
+        }
       }
     }
 }

@@ -1984,6 +1984,20 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
   const int n_units = lr ? c->lr_n_sg : n_groups;  // the low-rank kernel's unit is a super-group of 64 genes
   int n_splits = std::max(cus / 8, (2 * cus + n_units - 1) / n_units);
   n_splits = std::max(1, std::min(n_tiles128, n_splits));
+  // The low-rank kernel shares each XCD's CUs between lr_spx super-groups
+  // (row splits n_splits = CUs per XCD / lr_spx): with four, an SNP tile is
+  // read into an XCD's L2 once per four super-groups.
+  int lr_spx = 1;
+  if (lr) {
+    const int cpx = cus / 8;
+    for (int s : {4, 2}) {
+      if (cpx % s == 0 && c->lr_n_sg >= 8 * s) {
+        lr_spx = s;
+        break;
+      }
+    }
+    n_splits = cpx / lr_spx;
+  }
   CHK(ensure(c->partial, sizeof(GeneAcc) * (cis ? static_cast<size_t>(n_items) * 16
                                                 : static_cast<size_t>(n_splits) * c->G_pad)));
   // device lists: hits (every hit of the block) and polish items; both grow
@@ -2080,7 +2094,9 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
         la.gb = c->lr_gb.as<__bf16>();
         la.n_sg = c->lr_n_sg;
         la.p = c->p;
-        CHK(launch_wald_lr(c, la, dim3(static_cast<unsigned>(round_up(c->lr_n_sg, 8) * n_splits))));
+        la.sg_per_xcd = lr_spx;
+        const int64_t rounds = (c->lr_n_sg + 8 * lr_spx - 1) / (8 * lr_spx);
+        CHK(launch_wald_lr(c, la, dim3(static_cast<unsigned>(rounds * 8 * lr_spx * n_splits))));
       } else {
         CHK(launch_wald(c, a, dim3(static_cast<unsigned>(cis ? n_items : n_groups_pad * n_splits))));
       }

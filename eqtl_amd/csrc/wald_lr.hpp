@@ -61,6 +61,7 @@ struct LrArgs {
   const double* gscale;      // per slot d(delta_k, lbar): the scale of the interpolated gg
   const __bf16* gb;          // gene tiles, see lr_tile_index
   int32_t n_sg, p;
+  int32_t sg_per_xcd;        // super-groups per XCD and round (the launch has n_splits splits of each)
 };
 
 // Both copy rings (gene tile, row planes) hold kLrStages k-steps and are
@@ -69,6 +70,11 @@ struct LrArgs {
 constexpr int kLrStages = 4;
 constexpr size_t kLrTileBytes = 8 * 1024;  // one k-step of a super-group's gene tile
 constexpr size_t kLrStageRegion = kLrStages * (kLrTileBytes + kAStageBytes);
+// The interpolation's ring: kMixStages steps of 24 KiB (128 rows x 32 nodes x
+// three bf16 planes), inside the contraction's staging region.
+constexpr int kMixStages = 4;
+constexpr size_t kMixStageBytes = 24 * 1024;
+static_assert(kMixStages * kMixStageBytes <= kLrStageRegion, "interpolation ring");
 // after the contraction: per group and test gg, num (fp32, from fp32
 // accumulators) and the FP64 sum of squares, then the eight waves' gene partials
 constexpr size_t kLrFoldBytes = 4 * kW8Tests * (sizeof(float) * 2 + sizeof(double));
@@ -204,16 +210,6 @@ __device__ inline LrW lr_weights(const __bf16* lb) {
 struct LrX {
   bf16x8 h[kRB], m[kRB], l[kRB];
 };
-__device__ inline LrX lr_values(const __bf16* const (&trow)[kRB], int64_t col, int64_t tpl) {
-  LrX x;
-#pragma unroll
-  for (int rb = 0; rb < kRB; ++rb) {
-    x.h[rb] = load8_bf16(trow[rb] + col);
-    x.m[rb] = load8_bf16(trow[rb] + col + tpl);
-    x.l[rb] = load8_bf16(trow[rb] + col + 2 * tpl);
-  }
-  return x;
-}
 
 // Accumulates one quantity's interpolation for both groups of a wave:
 // pj[gq][rb] += sum over the chunk's 32 nodes of l_r x value_r, six products.
@@ -243,12 +239,16 @@ __device__ __attribute__((always_inline)) inline void lr_mix(f32x4 (&pj)[2][kRB]
 __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_wald_lr(LrArgs la) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const WaldArgs& a = la.w;
-  // XCD-aware: the n_splits consecutive workgroups of one XCD share a
-  // super-group, whose gene tiles are then served from that XCD's L2
-  const int wg = blockIdx.x;
-  const int jx = wg >> 3;
-  const int sg = (jx / a.n_splits) * 8 + (wg & 7);
-  const int split = jx % a.n_splits;
+  // XCD-aware: workgroup wg runs on XCD wg % 8. Per round each XCD takes
+  // sg_per_xcd consecutive super-groups x n_splits row splits; the workgroups
+  // of one split sweep the same SNP tiles in the same order (and those of one
+  // super-group read the same gene tiles), so both are served from the XCD's
+  // L2 instead of each super-group streaming the whole block.
+  const int wg = blockIdx.x, xcd = wg & 7, jx = wg >> 3;
+  const int per_xcd = la.sg_per_xcd * a.n_splits;
+  const int l = jx % per_xcd;
+  const int sg = ((jx / per_xcd) * 8 + xcd) * la.sg_per_xcd + l / a.n_splits;
+  const int split = l % a.n_splits;
   if (sg >= la.n_sg) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = wave >> 2, r = wave & 3;
@@ -342,9 +342,6 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
           ssq[gq][rb][e] = 0.0;
           ggv[gq][rb][e] = 0.f;
         }
-    const __bf16* trow[kRB];
-#pragma unroll
-    for (int rb = 0; rb < kRB; ++rb) trow[rb] = la.Tp + static_cast<size_t>(rowP + 16 * rb + gl) * la.ldT + kk;
     // column of chunk qr of quantity ca (ca == p: the node sums of squares)
     auto qcol = [&](int ca, int qr) { return static_cast<int64_t>((ca < la.p ? segc + ca * segR : segg) + 32 * qr); };
     f32x4 pj[2][kRB];
@@ -366,32 +363,87 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
           pj[gq][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
     };
+    // Step st = (quantity ca, chunk qr) stages the tile's 128 rows x 32 node
+    // values in LDS (ring slot st % kMixStages): chunk ch = (rr * 2 + rb) * 3
+    // + plane of 1 KiB holds rows 32 rr + 16 rb + (l & 15), nodes 8 (l >> 4)
+    // .. + 7 of that plane in lane l's 16 bytes. Wave w copies chunks 3w ..
+    // 3w + 2; the two waves of a row quarter read the same chunks.
+    const int nsteps = (la.p + 1) * nqr;
+    const int ch0 = 3 * wave;
+    auto issue = [&](int st) {
+      const int ca = st / nqr, qr = st - ca * nqr;
+      const int64_t col = qcol(ca, qr) + kk;
+      unsigned char* dst = smem + (st % kMixStages) * kMixStageBytes;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const int ch = ch0 + k, rr = ch / 6, rb = (ch / 3) & 1, plane = ch % 3;
+        const __bf16* src = la.Tp + plane * la.tpl + (row0 + 32 * rr + 16 * rb + gl) * la.ldT + col;
+        lds_direct16(src, dst + ch * 1024);
+      }
+    };
+    auto values = [&](int st) {
+      const unsigned char* slot = smem + (st % kMixStages) * kMixStageBytes + lane * 16;
+      LrX x;
+#pragma unroll
+      for (int rb = 0; rb < kRB; ++rb) {
+        const int ch = (r * 2 + rb) * 3;
+        x.h[rb] = *reinterpret_cast<const bf16x8*>(slot + (ch + 0) * 1024);
+        x.m[rb] = *reinterpret_cast<const bf16x8*>(slot + (ch + 1) * 1024);
+        x.l[rb] = *reinterpret_cast<const bf16x8*>(slot + (ch + 2) * 1024);
+      }
+      return x;
+    };
     if (nqr == 1) {
-      // one chunk: the weights are the same for every quantity
+      // one chunk: the weights are the same for every quantity (loaded before
+      // the copies, so the waits below cover them too)
       const LrW w = lr_weights(lbase);
-      LrX xv = lr_values(trow, qcol(0, 0), la.tpl);
+      __builtin_amdgcn_sched_barrier(0);
+      for (int st = 0; st < kMixStages - 1 && st < nsteps; ++st) issue(st);
 #pragma unroll 1
-      for (int ca = 0; ca <= la.p; ++ca) {
-        LrX xn;
-        if (ca < la.p) xn = lr_values(trow, qcol(ca + 1, 0), la.tpl);
-        lr_mix(pj, xv, w);
-        finish(ca == la.p);
-        xv = xn;
+      for (int st = 0; st < nsteps; ++st) {
+        // step st's chunks have landed: the copies of the up to two later
+        // steps already issued may stay in flight
+        const int ahead = min(nsteps - 1, st + kMixStages - 2) - st;
+        if (ahead >= 2) __builtin_amdgcn_s_waitcnt(lr_wait_vm(6));
+        else if (ahead == 1) __builtin_amdgcn_s_waitcnt(lr_wait_vm(3));
+        else __builtin_amdgcn_s_waitcnt(0);
+        lds_barrier();  // every wave's chunks of step st are in; step st - 1's slot is free
+        if (st + kMixStages - 1 < nsteps) issue(st + kMixStages - 1);
+        __builtin_amdgcn_sched_barrier(0);
+        lr_mix(pj, values(st), w);
+        finish(st == la.p);
       }
     } else {
-      const int nsteps = (la.p + 1) * nqr;
-      LrX xv = lr_values(trow, qcol(0, 0), la.tpl);
+      // per chunk weights, loaded one step ahead and issued before that
+      // step's copies, so waiting for them also retires the older copies
+      LrW w = lr_weights(lbase);
+      __builtin_amdgcn_sched_barrier(0);
+      for (int st = 0; st < kMixStages - 1 && st < nsteps; ++st) issue(st);
 #pragma unroll 1
       for (int st = 0; st < nsteps; ++st) {
         const int ca = st / nqr, qr = st - ca * nqr;
-        LrX xn;
-        if (st + 1 < nsteps) {
-          const int ca1 = (st + 1) / nqr, qr1 = st + 1 - ca1 * nqr;
-          xn = lr_values(trow, qcol(ca1, qr1), la.tpl);
+        if (st == 0) {
+          const int ahead = min(nsteps - 1, kMixStages - 2);
+          if (ahead >= 2) __builtin_amdgcn_s_waitcnt(lr_wait_vm(6));
+          else if (ahead == 1) __builtin_amdgcn_s_waitcnt(lr_wait_vm(3));
+          else __builtin_amdgcn_s_waitcnt(0);
+        } else if (st + kMixStages - 2 < nsteps) {
+          __builtin_amdgcn_s_waitcnt(lr_wait_vm(3));
+        } else {
+          __builtin_amdgcn_s_waitcnt(0);
         }
-        lr_mix(pj, xv, lr_weights(lbase + static_cast<size_t>(qr) * 1536));
+        lds_barrier();
+        LrW wn;
+        if (st + 1 < nsteps) {
+          const int qr1 = (st + 1) % nqr;
+          wn = lr_weights(lbase + static_cast<size_t>(qr1) * 1536);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+        if (st + kMixStages - 1 < nsteps) issue(st + kMixStages - 1);
+        __builtin_amdgcn_sched_barrier(0);
+        lr_mix(pj, values(st), w);
         if (qr == nqr - 1) finish(ca == la.p);
-        xv = xn;
+        w = wn;
       }
     }
     // fold: each wave's two groups (gg before its per-gene scale, sum of squares, num) into LDS

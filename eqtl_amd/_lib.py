@@ -37,7 +37,7 @@ EXPORTS = [
     "eqtl_gpu_grm_begin", "eqtl_gpu_grm_accumulate", "eqtl_gpu_grm_finish", "eqtl_gpu_eigh",
     "eqtl_gpu_set_basis", "eqtl_gpu_lmm_prep", "eqtl_gpu_reml_grad_hess", "eqtl_gpu_lm_prep",
     "eqtl_gpu_set_gene_windows", "eqtl_gpu_scan_begin", "eqtl_gpu_scan_block", "eqtl_gpu_scan_end",
-    "eqtl_gpu_set_timing", "eqtl_gpu_get_timing", "eqtl_gpu_synth_bed", "eqtl_gpu_copy_rows",
+    "eqtl_gpu_set_timing", "eqtl_gpu_get_timing", "eqtl_gpu_scan_lr_plan", "eqtl_gpu_synth_bed", "eqtl_gpu_copy_rows",
     "eqtl_gpu_synchronize", "eqtl_gpu_selftest", "eqtl_gpu_upload_bed", "eqtl_gpu_cis_perm",
     "eqtl_gpu_beta_approx", "eqtl_gpu_set_geno_format", "eqtl_gpu_vcf_decode", "eqtl_gpu_glm_prep",
     "eqtl_gpu_set_test_subset", "eqtl_gpu_scan_hits", "eqtl_gpu_grm_accumulate_noalloc", "eqtl_gpu_cis_tuning",
@@ -369,6 +369,14 @@ class Context:
         n = ctypes.c_int64(0)
         _check(lib().eqtl_gpu_get_timing(self._c, family.encode(), ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def lr_plan(self):
+        """Plan of the prepared trans LMM scan: (low_rank, super_groups, segments, mean_nodes)."""
+        on, nsg, nseg = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+        mn = ctypes.c_double(0)
+        _check(lib().eqtl_gpu_scan_lr_plan(self._c, ctypes.byref(on), ctypes.byref(nsg), ctypes.byref(nseg),
+                                           ctypes.byref(mn)))
+        return bool(on.value), nsg.value, nseg.value, mn.value
 
     def synth_bed(self, seed, m, n, maf_lo=0.05, maf_hi=0.5, miss=0.01):
         ptr = ctypes.c_void_p()

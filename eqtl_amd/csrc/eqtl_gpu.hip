@@ -247,7 +247,8 @@ struct eqtl_gpu_ctx {
   // trans LMM, split form, covariate projections interpolated in ln delta
   // (k_wald_lr): built by scan_begin; slots hold the genes in ln-delta order
   bool lr_on = false;
-  int lr_n_sg = 0, lr_ncols = 0, lr_nnodes = 0;
+  int lr_n_sg = 0, lr_ncols = 0, lr_nnodes = 0, lr_nseg = 0;
+  double lr_mean_nodes = 0.0;  // interpolation nodes per valid gene, averaged
   int64_t lr_ldT = 0;
   DBuf lr_slot_gene, lr_sg_col, lr_sg_R, lr_coef, lr_gscale, lr_gb, lr_QZhi, lr_QZlo, lr_Dhi, lr_Dlo, lr_Tp,
       lr_G2;
@@ -1732,6 +1733,17 @@ static int lr_build(eqtl_gpu_ctx* c) {
   c->lr_n_sg = n_sg;
   c->lr_ncols = ncols;
   c->lr_nnodes = nnodes_total;
+  c->lr_nseg = static_cast<int>(segs.size());
+  {
+    double sum = 0.0;
+    int cnt = 0;
+    for (int s2 = 0; s2 < G; ++s2)
+      if (valid[slot_gene[s2]]) {
+        sum += segs[sg_seg[s2 / 64]].R;
+        ++cnt;
+      }
+    c->lr_mean_nodes = cnt ? sum / cnt : 0.0;
+  }
   c->lr_ldT = ldT;
   c->lr_on = true;
   return EQTL_OK;
@@ -2939,6 +2951,17 @@ int eqtl_gpu_get_timing(eqtl_gpu_ctx* c, const char* fam, double* ms, int64_t* l
   auto it = c->tslots.find(fam);
   if (ms) *ms = it == c->tslots.end() ? 0.0 : it->second.ms;
   if (launches) *launches = it == c->tslots.end() ? 0 : it->second.launches;
+  return EQTL_OK;
+}
+
+int eqtl_gpu_scan_lr_plan(eqtl_gpu_ctx* c, int32_t* low_rank, int32_t* n_super_groups, int32_t* n_segments,
+                          double* mean_nodes) {
+  if (!c) return fail(EQTL_E_ARG, "bad scan_lr_plan arguments");
+  const bool on = c->lr_on;
+  if (low_rank) *low_rank = on ? 1 : 0;
+  if (n_super_groups) *n_super_groups = on ? c->lr_n_sg : 0;
+  if (n_segments) *n_segments = on ? c->lr_nseg : 0;
+  if (mean_nodes) *mean_nodes = on ? c->lr_mean_nodes : 0.0;
   return EQTL_OK;
 }
 

@@ -247,11 +247,15 @@ namespace eqtl_amd {
 // fitted at Chebyshev nodes of the exact expression, and evaluated at the same
 // rounded x the reference forms (so its rounding of x carries over):
 //   A: r(u) = s/u with u = sqrt(y), 64 uniform intervals on [0, sqrt(1-x_sw)]
+//   M: ln p as a function of u, 64 uniform intervals from sqrt(1-x_sw) to
+//      u_m (|t| = 8, at most u = 0.9): one exp and no log for the |t| most
+//      tests have, which would otherwise take piece B's two logs
 //   B: ln p as a function of w = ln(-a ln x), uniform intervals up to
 //      ln p = -760 (p = 0 beyond: the reference's exp() underflows there).
 // ---------------------------------------------------------------------------
 constexpr int kTailDeg = 8;      // coefficients per interval
 constexpr int kTailA = 64;       // intervals in piece A
+constexpr int kTailM = 64;       // intervals in piece M
 constexpr int kTailB = 512;      // intervals in piece B
 
 struct TTail {
@@ -261,7 +265,10 @@ struct TTail {
   double x_zero;                 // x at/below which p == 0
   double coef_a[kTailA * kTailDeg];
   double coef_b[kTailB * kTailDeg];
+  double u_m, inv_h_m;           // piece M: u range end, 1 / interval width (0: no piece M)
+  double coef_m[kTailM * kTailDeg];
 };
+static_assert(offsetof(TTail, coef_m) == offsetof(TTail, u_m) + 2 * sizeof(double), "TTail piece M layout");
 static_assert(sizeof(double) * 8 == offsetof(TTail, coef_a), "TTail header layout");
 
 EQ_HD double tail_poly(const double* c, double z) {
@@ -275,12 +282,10 @@ EQ_HD double tail_poly(const double* c, double z) {
 // piece B (larger |t|) from global memory.
 constexpr int kTailHead = 8 + kTailA * kTailDeg;
 
-// head: df, half_df, u_sw, inv_h_a, w_lo, w_hi, inv_hw_b, x_zero, coef_a[]
-EQ_HD double p_from_t_tab2(const double* head, const double* coef_b, double t) {
-  const double df = head[0];
-  if (!isfinite(t) || df <= 0) return 1.0;
-  const double x = df / (df + t * t);
-  const double y = 1.0 - x;
+// p at x = df / (df + t^2) and y = 1 - x (y >= 0).
+// head: df, half_df, u_sw, inv_h_a, w_lo, w_hi, inv_hw_b, x_zero, coef_a[];
+// mid: u_m, inv_h_m, coef_m[]; coef_b: piece B.
+EQ_HD double p_tail_xy(const double* head, const double* mid, const double* coef_b, double x, double y) {
   const double u = sqrt(y);
   if (u <= head[2]) {
     const double q = u * head[3];
@@ -288,6 +293,13 @@ EQ_HD double p_from_t_tab2(const double* head, const double* coef_b, double t) {
     if (i >= kTailA) i = kTailA - 1;
     const double z = 2.0 * (q - i) - 1.0;
     return fmin(1.0, fmax(0.0, 1.0 - u * tail_poly(head + 8 + i * kTailDeg, z)));
+  }
+  if (u <= mid[0]) {
+    const double q = (u - head[2]) * mid[1];
+    int i = static_cast<int>(q);
+    if (i >= kTailM) i = kTailM - 1;
+    const double z = 2.0 * (q - i) - 1.0;
+    return fmin(1.0, fmax(0.0, exp(tail_poly(mid + 2 + i * kTailDeg, z))));
   }
   if (x <= head[7]) return 0.0;
   const double w = log(-head[1] * log(x));
@@ -299,7 +311,15 @@ EQ_HD double p_from_t_tab2(const double* head, const double* coef_b, double t) {
   return fmin(1.0, fmax(0.0, exp(tail_poly(coef_b + i * kTailDeg, z))));
 }
 
-EQ_HD double p_from_t_tab(const TTail& T, double t) { return p_from_t_tab2(&T.df, T.coef_b, t); }
+// p from t, with x formed as the reference forms it (util.cpp:259-264).
+EQ_HD double p_from_t_tab2(const double* head, const double* mid, const double* coef_b, double t) {
+  const double df = head[0];
+  if (!isfinite(t) || df <= 0) return 1.0;
+  const double x = df / (df + t * t);
+  return p_tail_xy(head, mid, coef_b, x, 1.0 - x);
+}
+
+EQ_HD double p_from_t_tab(const TTail& T, double t) { return p_from_t_tab2(&T.df, &T.u_m, T.coef_b, t); }
 
 // Host: fit one interval (Chebyshev interpolation, then monomial in z).
 inline void tail_fit_interval(double (*f)(double, const void*), const void* ctx, double lo, double hi,
@@ -350,6 +370,14 @@ inline double tail_fb(double w, const void* vp) {
   return lfront + log(betacf_dev(c.a, c.b, x));
 }
 
+// piece M: ln p at u = sqrt(1 - x)
+inline double tail_fm(double u, const void* vp) {
+  const TailCtx& c = *static_cast<const TailCtx*>(vp);
+  const double y = u * u, x = 1.0 - y;
+  const double lfront = log1p(-y) * c.a + log(y) * c.b - c.lbeta - log(c.a);
+  return lfront + log(betacf_dev(c.a, c.b, x));
+}
+
 inline void tail_build(TTail& T, double df, double lbeta) {
   T.df = df;
   T.half_df = 0.5 * df;
@@ -370,6 +398,16 @@ inline void tail_build(TTail& T, double df, double lbeta) {
   for (int i = 0; i < kTailB; ++i)
     tail_fit_interval(tail_fb, &c, T.w_lo + i * hb, T.w_lo + (i + 1) * hb, T.coef_b + i * kTailDeg);
   T.x_zero = exp(-v_hi / a);
+  T.u_m = fmin(sqrt(64.0 / (df + 64.0)), 0.9);
+  if (T.u_m > T.u_sw) {
+    const double hm = (T.u_m - T.u_sw) / kTailM;
+    T.inv_h_m = 1.0 / hm;
+    for (int i = 0; i < kTailM; ++i)
+      tail_fit_interval(tail_fm, &c, T.u_sw + i * hm, T.u_sw + (i + 1) * hm, T.coef_m + i * kTailDeg);
+  } else {
+    T.u_m = T.u_sw;
+    T.inv_h_m = 0.0;
+  }
 }
 
 }  // namespace eqtl_amd

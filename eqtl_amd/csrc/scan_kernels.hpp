@@ -439,7 +439,8 @@ struct RowInfo {
 
 struct TailView {
   const double* head;    // LDS copy of the header + piece A
-  const double* coef_b;  // global
+  const double* mid;     // piece M (global)
+  const double* coef_b;  // piece B (global)
 };
 
 // FP64 numerator refinement at both ends of the t range.
@@ -466,10 +467,11 @@ __device__ inline bool refine_wanted(const WaldArgs& a, bool ok, double gg, doub
     return U * U < tau2 * S;  // z^2 = U^2 / S
   } else if constexpr (MODEL == EQTL_MODEL_LMM) {
     if (gg < 1e-12 || S <= 1e-15) return false;
-    const double beta = num / S;
-    const double qq = fmax(gscal - S * beta * beta, 0.0);
-    const double t2q = beta * beta * S * a.df;  // t^2 = t2q / qq
-    return t2q < tau2 * qq || t2q > hi2 * qq;
+    // t^2 = num^2 df / (S qq) with S qq = max(S gscal - num^2, 0)
+    const double n2 = num * num;
+    const double sq = fmax(S * gscal - n2, 0.0);
+    const double t2s = n2 * a.df;
+    return t2s < tau2 * sq || t2s > hi2 * sq;
   } else {
     if (gtg < 1e-12) return false;
     const double beta = num / gtg;
@@ -528,6 +530,8 @@ __device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, i
   const int64_t snp = a.snp0 + j;
   double beta = 0, se = 0, stat = 0, p = 1.0, r2 = 0;
   const double df = a.df;
+  bool big_t = false;  // |t| > tau_hi (LMM, GLM: polished after the kernel)
+  bool lazy = false;   // LMM: beta, se, t, r2 still to be formed
   if constexpr (MODEL == EQTL_MODEL_GLM) {
     // test_glm_nb (model_glm.cpp:117-131): U = g^T r with the uncentred dosage
     // = (g - mean)^T r + mean * sum(r); info = S; two-sided normal p
@@ -538,17 +542,25 @@ __device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, i
       se = 1.0 / rs;
       stat = U / rs;
       p = isfinite(stat) ? fmin(1.0, fmax(0.0, erfc(fabs(stat) * 0.70710678118654752440))) : 1.0;
+      big_t = fabs(stat) > a.tau_hi;
     }
   } else if constexpr (MODEL == EQTL_MODEL_LMM) {
     if (!(gg < 1e-12) && !(S <= 1e-15)) {
-      beta = num / S;
-      double qq = gscal - S * beta * beta;
-      if (qq < 0) qq = 0;
-      const double sigma2 = df > 0 ? qq / df : 1.0;
-      se = sqrt(fmax(sigma2 / S, 0.0));
-      stat = se > 0 ? beta / se : 0.0;
-      p = p_from_t_tab2(tv.head, tv.coef_b, stat);
-      r2 = gscal > 1e-15 ? fmax(0.0, 1.0 - qq / gscal) : 0.0;
+      // p depends on t only through y = t^2 / (df + t^2) = num^2 / (S gscal)
+      // (t^2 = num^2 df / (S qq), S qq = S gscal - num^2): one division per
+      // test. qq clamped to 0 gives t = 0 and df <= 0 gives p = 1, as in the
+      // reference; beta, se, t and r2 are formed its way below, only for the
+      // tests that report them (hits, best).
+      const double n2 = num * num, sgs = S * gscal, sq = sgs - n2;
+      const double th2 = a.tau_hi * a.tau_hi;
+      if (df > 0 && sq > 0) {
+        const double y = n2 / sgs;
+        p = isfinite(y) ? p_tail_xy(tv.head, tv.mid, tv.coef_b, 1.0 - y, y) : 1.0;
+        big_t = n2 * df > th2 * sq;
+      } else {
+        big_t = !(df > 0) && n2 > th2 * S;
+      }
+      lazy = true;
     }
   } else {
     const double gtg = ri.gtg[jl];
@@ -559,7 +571,7 @@ __device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, i
       const double s2 = fmax(rss / df, 0.0);
       se = sqrt(s2 / gtg);
       stat = se > 0 ? beta / se : 0.0;
-      p = p_from_t_tab2(tv.head, tv.coef_b, stat);
+      p = p_from_t_tab2(tv.head, tv.mid, tv.coef_b, stat);
       r2 = gscal > 0 ? 1.0 - rss / gscal : 0.0;
     } else if (!(gtg < 1e-12)) {
       beta = gty / gtg;  // df <= 0: p = 1 with beta set (model_lm.cpp:44-48)
@@ -570,7 +582,7 @@ __device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, i
   // LMM / GLM tests with |t| > tau_hi or a cancelling S are polished in FP64
   // after the kernel, which then does their ACAT term, n_sig, hit and best
   // (k_polish, k_pbest_*); here they only count as tested
-  const bool polish = MODEL != EQTL_MODEL_LM && (fabs(stat) > a.tau_hi || (S < a.kappa * gg && !(S <= 1e-15)));
+  const bool polish = MODEL != EQTL_MODEL_LM && (big_t || (S < a.kappa * gg && !(S <= 1e-15)));
   if (polish) {
     const unsigned long long pi = atomicAdd(a.hit_count + 1, 1ull);
     if (static_cast<int64_t>(pi) < a.polish_cap) a.polish[pi] = PolishItem{static_cast<int32_t>(j), gene, -1};
@@ -582,6 +594,17 @@ __device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, i
     ++st.used;
   }
   const double afj = ri.af[jl];
+  const bool report = p <= a.pthr || p < st.best_p || (p == st.best_p && snp < st.best_snp);
+  if (MODEL == EQTL_MODEL_LMM && lazy && report) {
+    // test_lmm_gtil's own arithmetic (scan_trans_lmm.cpp:28-37)
+    beta = num / S;
+    double qq = gscal - S * beta * beta;
+    if (qq < 0) qq = 0;
+    const double sigma2 = df > 0 ? qq / df : 1.0;
+    se = sqrt(fmax(sigma2 / S, 0.0));
+    stat = se > 0 ? beta / se : 0.0;
+    r2 = gscal > 1e-15 ? fmax(0.0, 1.0 - qq / gscal) : 0.0;
+  }
   if (p <= a.pthr) {
     ++st.n_sig;
     const unsigned long long idx = atomicAdd(a.hit_count, 1ull);
@@ -712,7 +735,7 @@ __device__ __attribute__((always_inline)) inline void lm_item(const WaldArgs& a,
   double* thead = reinterpret_cast<double*>(smem + r0b + sizeof(RowInfo));
   double* fold = reinterpret_cast<double*>(smem);
   for (int i = threadIdx.x; i < kTailHead; i += 256) thead[i] = (&a.tail->df)[i];
-  const TailView tv{thead, a.tail->coef_b};
+  const TailView tv{thead, &a.tail->u_m, a.tail->coef_b};
   const unsigned char* gsrc =
       PREC == EQTL_PREC_FP32 ? reinterpret_cast<const unsigned char*>(a.gc.f32 + a.gc.tile_f32(group, 0))
                              : reinterpret_cast<const unsigned char*>(a.gc.bf + a.gc.tile_bf(group, 0));
@@ -1119,7 +1142,7 @@ __global__ __launch_bounds__(256) void k_exact_minp(ExactArgs a) {
         const double s2 = fmax(rss / a.df, 0.0);
         const double se = sqrt(s2 / gtg);
         const double t = se > 0 ? beta / se : 0.0;
-        pv = p_from_t_tab2(&a.tail->df, a.tail->coef_b, t);
+        pv = p_from_t_tab2(&a.tail->df, &a.tail->u_m, a.tail->coef_b, t);
       }
     } else {
       double gg = 0.0, num = 0.0, av[kP];
@@ -1164,7 +1187,7 @@ __global__ __launch_bounds__(256) void k_exact_minp(ExactArgs a) {
         const double sigma2 = a.df > 0 ? qq / a.df : 1.0;
         const double se = sqrt(fmax(sigma2 / S, 0.0));
         const double t = se > 0 ? beta / se : 0.0;
-        pv = p_from_t_tab2(&a.tail->df, a.tail->coef_b, t);
+        pv = p_from_t_tab2(&a.tail->df, &a.tail->u_m, a.tail->coef_b, t);
       }
     }
     if (isfinite(pv)) best = fmin(best, pv);

@@ -428,7 +428,7 @@ __device__ __attribute__((always_inline)) inline void w8_item(const WaldArgs& a,
   GeneAcc* wpart = reinterpret_cast<GeneAcc*>(smem + kW8FoldBytes);
   for (int i = threadIdx.x; i < kTailHead; i += kW8Threads) thead[i] = (&a.tail->df)[i];
   if (threadIdx.x < 16) acc_init(gstate[threadIdx.x]);
-  const TailView tv{thead, a.tail->coef_b};
+  const TailView tv{thead, &a.tail->u_m, a.tail->coef_b};
   const unsigned char* gsrc =
       PREC == EQTL_PREC_FP32 ? reinterpret_cast<const unsigned char*>(a.gc.f32 + a.gc.tile_f32(group, 0))
                              : reinterpret_cast<const unsigned char*>(a.gc.bf + a.gc.tile_bf(group, 0));

@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
   GeneAcc* wpart = reinterpret_cast<GeneAcc*>(smem + kLrFoldBytes);
   for (int i = threadIdx.x; i < kTailHead; i += kW8Threads) thead[i] = (&a.tail->df)[i];
   if (threadIdx.x < 64) acc_init(gstate[threadIdx.x]);
-  const TailView tv{thead, a.tail->coef_b};
+  const TailView tv{thead, &a.tail->u_m, a.tail->coef_b};
   const unsigned char* gsrc = reinterpret_cast<const unsigned char*>(la.gb) + static_cast<size_t>(sg) * nks * kLrTileBytes;
   const int pair_off = r * 4096;
   const int segR = la.sg_R[sg], segc = la.sg_col[sg], segg = la.sg_gcol[sg], nqr = segR >> 5;

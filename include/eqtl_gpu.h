@@ -280,6 +280,12 @@ int eqtl_gpu_beta_approx(eqtl_gpu_ctx* ctx, const double* perm_min_p, const doub
  * ("wald", "rotate", "decode", "reml", "vcf") with HIP events on its stream. */
 int eqtl_gpu_set_timing(eqtl_gpu_ctx* ctx, int32_t enable);
 int eqtl_gpu_get_timing(eqtl_gpu_ctx* ctx, const char* family, double* total_ms, int64_t* launches);
+/* Plan of the trans LMM scan prepared by eqtl_gpu_scan_begin (no counterpart
+ * in the reference): *low_rank 1 when the interpolated form runs, the number
+ * of 64-gene super-groups and ln-delta segments, and the mean number of
+ * interpolation nodes per gene (32 to 128; 0 when the form is off). */
+int eqtl_gpu_scan_lr_plan(eqtl_gpu_ctx* ctx, int32_t* low_rank, int32_t* n_super_groups, int32_t* n_segments,
+                          double* mean_nodes);
 /* Synthetic PLINK rows generated on the device (benchmark inputs resident in
  * HBM): m SNPs x n samples, MAF ~ U(maf_lo, maf_hi), A1 flipped by a fair
  * coin, HWE genotypes, miss rate; counter-based hash, so row j depends only

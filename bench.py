@@ -54,9 +54,12 @@ def make_inputs(ctx, seed=1003):
     return w.rows_dev, w.bps, w.chrom, w.pos, w.X, w.Y, w.win, w.extra["planted"]
 
 
-def trans_pass(ctx, rows_dev, bps, X, Y, m, prec, delta_mode, delta_in=None, pthr=1e-5):
+def trans_pass(ctx, rows_dev, bps, X, Y, m, prec, delta_mode, delta_in=None, pthr=1e-5, Y_dev=None):
     import eqtl_amd
-    delta, _ = ctx.lmm_prep(Y, X, delta_mode=delta_mode, delta_in=delta_in)
+    if Y_dev is not None:
+        delta, _ = ctx.lmm_prep(None, X, delta_mode=delta_mode, delta_in=delta_in, Y_dev=Y_dev)
+    else:
+        delta, _ = ctx.lmm_prep(Y, X, delta_mode=delta_mode, delta_in=delta_in)
     ctx.scan_begin(eqtl_amd.MODEL_LMM, prec, pthr)
     nh = 0
     for j0 in range(0, m, BLOCK):
@@ -877,7 +880,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--snps", type=int, default=None,
@@ -939,15 +942,24 @@ def main():
     ctx.set_snp_annot(chrom, pos)
     ctx.set_gene_windows(*win)
 
+    # the phenotypes stay in device memory like the genotype rows (column-major
+    # n x G), allocated through the HIP runtime the library runs on
+    hip = ctypes.CDLL("libamdhip64.so")
+    y_host = np.asfortranarray(Y, dtype=np.float64)
+    y_ptr = ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(y_ptr), ctypes.c_size_t(y_host.nbytes)) != 0 or \
+            hip.hipMemcpy(y_ptr, y_host.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(y_host.nbytes), 1) != 0:
+        raise RuntimeError("device copy of the phenotypes failed")
+    Y_dev = (y_ptr.value, Y.shape[1])
     prec = eqtl_amd.PREC_BF16X3
     for w in range(args.warmup):
-        trans_pass(ctx, rows_dev, bps, X, Y, min(m, 4 * BLOCK), prec, eqtl_amd.DELTA_NEWTON)
+        trans_pass(ctx, rows_dev, bps, X, Y, min(m, 4 * BLOCK), prec, eqtl_amd.DELTA_NEWTON, Y_dev=Y_dev)
     ctx.set_timing(True)
     ctx.synchronize()
     t0 = time.perf_counter()
     tests = 0
     for s in range(args.steps):
-        delta, st, nh = trans_pass(ctx, rows_dev, bps, X, Y, m, prec, eqtl_amd.DELTA_NEWTON)
+        delta, st, nh = trans_pass(ctx, rows_dev, bps, X, Y, m, prec, eqtl_amd.DELTA_NEWTON, Y_dev=Y_dev)
         tests += int(st["n_tested"].sum())
         log(f"step {s}: {int(st['n_tested'].sum())} tests, {nh} hits")
     ctx.synchronize()
@@ -982,7 +994,7 @@ def main():
     ctx.set_timing(True)
     ctx.synchronize()
     t1 = time.perf_counter()
-    d32, st32, _ = trans_pass(ctx, rows_dev, bps, X, Y, m32, eqtl_amd.PREC_FP32, eqtl_amd.DELTA_REF)
+    d32, st32, _ = trans_pass(ctx, rows_dev, bps, X, Y, m32, eqtl_amd.PREC_FP32, eqtl_amd.DELTA_REF, Y_dev=Y_dev)
     ctx.synchronize()
     dt32 = time.perf_counter() - t1
     w32, n32 = ctx.timing("wald")
@@ -1022,6 +1034,7 @@ def main():
     }
     print(json.dumps(out))
     ctx.close()
+    hip.hipFree(y_ptr)
 
 
 if __name__ == "__main__":

@@ -230,14 +230,20 @@ class Context:
         _check(lib().eqtl_gpu_set_basis(self._c, _p(Q), _p(lam), ctypes.c_int32(lam.size)))
 
     # ---- genes ----
-    def lmm_prep(self, Y, X, delta_mode=DELTA_REF, delta_in=None):
-        Y = np.asfortranarray(Y, dtype=np.float64)
+    def lmm_prep(self, Y, X, delta_mode=DELTA_REF, delta_in=None, Y_dev=None):
+        """Y: n x G host array, or None with Y_dev = (device pointer, G) of an
+        n x G column-major FP64 matrix already in device memory."""
         X = np.asfortranarray(X, dtype=np.float64)
-        G, p = Y.shape[1], X.shape[1]
+        if Y_dev is not None:
+            yp, G = ctypes.c_void_p(Y_dev[0]), int(Y_dev[1])
+        else:
+            Y = np.asfortranarray(Y, dtype=np.float64)
+            yp, G = _p(Y), Y.shape[1]
+        p = X.shape[1]
         din = np.ascontiguousarray(delta_in, dtype=np.float64) if delta_in is not None else None
         dout = np.empty(G)
         rss = np.empty(G)
-        _check(lib().eqtl_gpu_lmm_prep(self._c, _p(Y), ctypes.c_int32(G), _p(X), ctypes.c_int32(p),
+        _check(lib().eqtl_gpu_lmm_prep(self._c, yp, ctypes.c_int32(G), _p(X), ctypes.c_int32(p),
                                        ctypes.c_int32(delta_mode), _p(din), _p(dout), _p(rss)))
         self.G = G
         return dout, rss

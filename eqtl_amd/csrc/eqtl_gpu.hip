@@ -250,6 +250,7 @@ struct eqtl_gpu_ctx {
   int lr_n_sg = 0, lr_ncols = 0, lr_nnodes = 0, lr_nseg = 0;
   double lr_mean_nodes = 0.0;  // interpolation nodes per valid gene, averaged
   int64_t lr_ldT = 0;
+  DBuf lr_fold;  // k_wald_lr -> k_lr_tests: gg, S, num per test of a block (lr_fold_base)
   DBuf lr_slot_gene, lr_sg_col, lr_sg_R, lr_coef, lr_gscale, lr_gb, lr_QZhi, lr_QZlo, lr_Dhi, lr_Dlo, lr_Tp,
       lr_G2;
   // synthetic bed buffers
@@ -1099,28 +1100,55 @@ static int reml_newton(eqtl_gpu_ctx* c, std::vector<double>& delta) {
     state[k] = 2;
   };
   for (int k = 0; k < G; ++k) propose(k);
+  // Once a trial has been rejected, or when few genes are left (the rounds
+  // then cost more than their evaluations), a pending trial is evaluated
+  // together with its next kLadder - 1 halvings (formed by the same
+  // recurrence) and the first one that does not raise f is taken: the
+  // accepted points are those of one trial per round, in fewer rounds.
+  constexpr int kLadder = 8, kFewGenes = 2048;
+  std::vector<int32_t> first(G + 1), cnt(G);
+  std::vector<double> xs;
   for (int round = 0; round < 60 * 32; ++round) {
     ek.clear();
     ed.clear();
-    for (int k = 0; k < G; ++k)
-      if (state[k] == 2) { ek.push_back(k); ed.push_back(std::exp(tn[k])); }
+    xs.clear();
+    int active = 0;
+    for (int k = 0; k < G; ++k) active += state[k] == 2;
+    for (int k = 0; k < G; ++k) {
+      first[k] = static_cast<int32_t>(ek.size());
+      cnt[k] = 0;
+      if (state[k] != 2) continue;
+      const int len = (back[k] > 0 || active < kFewGenes) ? kLadder : 1;
+      double x = tn[k];
+      for (int b = back[k]; b <= 30 && cnt[k] < len; ++b) {
+        ek.push_back(k);
+        ed.push_back(std::exp(x));
+        xs.push_back(x);
+        ++cnt[k];
+        x = th[k] + 0.5 * (x - th[k]);
+      }
+    }
     if (ek.empty()) break;
     const size_t A = ek.size();
     F.assign(A, 0.0); Gd.assign(A, 0.0); H.assign(A, 0.0);
     CHK(reml_eval(c, ek, ed, true, F.data(), Gd.data(), H.data()));
-    for (size_t q = 0; q < A; ++q) {
-      const int k = ek[q];
-      if (F[q] > f[k] + 1e-13 * std::fabs(f[k])) {
-        if (back[k] < 30) {
-          tn[k] = th[k] + 0.5 * (tn[k] - th[k]);
-          ++back[k];
-        } else {
-          state[k] = 0;
+    for (int k = 0; k < G; ++k) {
+      if (!cnt[k]) continue;
+      int acc = -1;
+      for (int i = 0; i < cnt[k]; ++i)
+        if (!(F[first[k] + i] > f[k] + 1e-13 * std::fabs(f[k]))) {
+          acc = i;
+          break;
         }
+      if (acc < 0) {
+        back[k] += cnt[k];
+        if (back[k] > 30) state[k] = 0;
+        else tn[k] = th[k] + 0.5 * (xs[first[k] + cnt[k] - 1] - th[k]);
         continue;
       }
-      const double moved = std::fabs(tn[k] - th[k]);
-      th[k] = tn[k]; f[k] = F[q]; g[k] = Gd[q]; h[k] = H[q];
+      const size_t q = static_cast<size_t>(first[k] + acc);
+      const double moved = std::fabs(xs[q] - th[k]);
+      th[k] = xs[q]; f[k] = F[q]; g[k] = Gd[q]; h[k] = H[q];
       ++iters[k];
       if (moved < 1e-10) { state[k] = 0; continue; }
       propose(k);
@@ -1283,6 +1311,14 @@ static int lmm_prep_core(eqtl_gpu_ctx* c, int G, int mode, const double* delta_i
   return EQTL_OK;
 }
 
+// Copy kind for a caller buffer that may live in host or device memory.
+static hipMemcpyKind copy_in_kind(const void* ptr) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, ptr) == hipSuccess && at.type == hipMemoryTypeDevice) return hipMemcpyDeviceToDevice;
+  (void)hipGetLastError();  // a host pointer may leave an error code behind
+  return hipMemcpyHostToDevice;
+}
+
 int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double* X, int32_t p,
                       int32_t mode, const double* delta_in, double* delta_out, double* rss_out) {
   if (!c || !Y || !X || G <= 0 || p <= 0) return fail(EQTL_E_ARG, "bad lmm_prep arguments");
@@ -1301,8 +1337,9 @@ int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* c, const double* Y, int32_t G, const double*
   CHK(ensure(c->U64, sizeof(double) * static_cast<size_t>(n) * G));
   CHK(ensure(c->V64, sizeof(double) * static_cast<size_t>(n) * G));
   CHK(ensure(c->Lc, sizeof(double) * static_cast<size_t>(p) * p * G));
-  HIPC(hipMemcpyAsync(c->Y64.p, Y, sizeof(double) * static_cast<size_t>(n) * G, hipMemcpyHostToDevice, c->stream));
-  HIPC(hipMemcpyAsync(c->X64.p, X, sizeof(double) * static_cast<size_t>(n) * p, hipMemcpyHostToDevice, c->stream));
+  // host or device memory
+  HIPC(hipMemcpyAsync(c->Y64.p, Y, sizeof(double) * static_cast<size_t>(n) * G, copy_in_kind(Y), c->stream));
+  HIPC(hipMemcpyAsync(c->X64.p, X, sizeof(double) * static_cast<size_t>(n) * p, copy_in_kind(X), c->stream));
   CHK(covariate_basis(c));
   // K4: Y~ = Q^T Y, X~ = Q^T X once for all genes (FP64 library GEMM)
   const double one = 1.0, zero = 0.0;
@@ -1613,22 +1650,30 @@ static int lr_build(eqtl_gpu_ctx* c) {
     sgg[g] = gbase[sg_seg[g]];
     sgR[g] = segs[sg_seg[g]].R;
   }
-  std::vector<double> ur, wr, ell;
+  // Chebyshev nodes of the first kind and their barycentric weights, per segment
+  std::vector<std::vector<double>> seg_ur(segs.size()), seg_wr(segs.size());
+  for (size_t q = 0; q < segs.size(); ++q) {
+    const Seg& sgm = segs[q];
+    seg_ur[q].resize(sgm.R);
+    seg_wr[q].resize(sgm.R);
+    for (int r = 0; r < sgm.R; ++r) {
+      const double th = (2.0 * r + 1.0) * M_PI / (2.0 * sgm.R);
+      seg_ur[q][r] = 0.5 * (sgm.lo + sgm.hi) + 0.5 * (sgm.hi - sgm.lo) * std::cos(th);
+      seg_wr[q][r] = ((r & 1) ? -1.0 : 1.0) * std::sin(th);
+    }
+  }
+  std::vector<double> ell;
   for (int s2 = 0; s2 < G; ++s2) {
     const int k = slot_gene[s2];
     if (!valid[k]) continue;
-    const Seg& sgm = segs[sg_seg[s2 / 64]];
-    const int R = sgm.R;
-    ur.resize(R);
-    wr.resize(R);
+    const int q = sg_seg[s2 / 64];
+    const int R = segs[q].R;
+    const std::vector<double>& ur = seg_ur[q];
+    const std::vector<double>& wr = seg_wr[q];
     ell.assign(R, 0.0);
     int hit = -1;
-    for (int r = 0; r < R; ++r) {
-      const double th = (2.0 * r + 1.0) * M_PI / (2.0 * R);
-      ur[r] = 0.5 * (sgm.lo + sgm.hi) + 0.5 * (sgm.hi - sgm.lo) * std::cos(th);
-      wr[r] = ((r & 1) ? -1.0 : 1.0) * std::sin(th);
+    for (int r = 0; r < R; ++r)
       if (u[k] == ur[r]) hit = r;
-    }
     if (hit >= 0) {
       ell[hit] = 1.0;
     } else {
@@ -1749,15 +1794,22 @@ static int lr_build(eqtl_gpu_ctx* c) {
   return EQTL_OK;
 }
 
+// The interpolated form: k_wald_lr (contractions, interpolation, fold out)
+// over grid, then k_lr_tests over n_sg x n_splits workgroups.
 int launch_wald_lr(eqtl_gpu_ctx* c, const LrArgs& la, dim3 grid) {
   static bool attr_set = false;
   if (!attr_set) {
     HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald_lr), hipFuncAttributeMaxDynamicSharedMemorySize,
                              static_cast<int>(kLrLdsBytes)));
+    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lr_tests),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLrTestsLdsBytes)));
     attr_set = true;
   }
   hipLaunchKernelGGL(k_wald_lr, grid, dim3(kW8Threads), kLrLdsBytes, c->stream, la);
-  return check_launch("k_wald_lr");
+  CHK(check_launch("k_wald_lr"));
+  hipLaunchKernelGGL(k_lr_tests, dim3(static_cast<unsigned>(la.n_sg * la.w.n_splits)), dim3(kW8Threads),
+                     kLrTestsLdsBytes, c->stream, la);
+  return check_launch("k_lr_tests");
 }
 
 // Cis work items of one scan block over [snp0, snp0 + m): per gene group the
@@ -2107,6 +2159,9 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
         la.n_sg = c->lr_n_sg;
         la.p = c->p;
         la.sg_per_xcd = lr_spx;
+        const int n_t128 = static_cast<int>((a.n_tiles + 1) / 2);
+        CHK(ensure(c->lr_fold, sizeof(float) * lr_fold_base(n_t128, 0, c->lr_n_sg)));
+        la.fold = c->lr_fold.as<float>();
         const int64_t rounds = (c->lr_n_sg + 8 * lr_spx - 1) / (8 * lr_spx);
         CHK(launch_wald_lr(c, la, dim3(static_cast<unsigned>(rounds * 8 * lr_spx * n_splits))));
       } else {

@@ -62,7 +62,15 @@ struct LrArgs {
   const __bf16* gb;          // gene tiles, see lr_tile_index
   int32_t n_sg, p;
   int32_t sg_per_xcd;        // super-groups per XCD and round (the launch has n_splits splits of each)
+  float* fold;               // per (tile, super-group): gg, S, num planes of the 4 x 2048 tests, see lr_fold_base
 };
+
+// First float of the fold block of (tile, super-group): three planes (gg, S,
+// num), each 4 groups x 2048 tests in the eight-wave epilogue's test order
+// (t = row * 16 + gene).
+__host__ __device__ inline size_t lr_fold_base(int tile, int sg, int n_sg) {
+  return (static_cast<size_t>(tile) * n_sg + sg) * 3 * 4 * kW8Tests;
+}
 
 // Both copy rings (gene tile, row planes) hold kLrStages k-steps and are
 // filled kLrStages - 1 k-steps ahead: a k-step's twelve MFMAs per wave are
@@ -75,15 +83,19 @@ constexpr size_t kLrStageRegion = kLrStages * (kLrTileBytes + kAStageBytes);
 constexpr int kMixStages = 4;
 constexpr size_t kMixStageBytes = 24 * 1024;
 static_assert(kMixStages * kMixStageBytes <= kLrStageRegion, "interpolation ring");
-// after the contraction: per group and test gg, num (fp32, from fp32
-// accumulators) and the FP64 sum of squares, then the eight waves' gene partials
-constexpr size_t kLrFoldBytes = 4 * kW8Tests * (sizeof(float) * 2 + sizeof(double));
-constexpr size_t kLrRegion0 = kLrStageRegion > kLrFoldBytes + kW8MergeBytes ? kLrStageRegion
-                                                                            : kLrFoldBytes + kW8MergeBytes;
+// after the interpolation: per group and test gg, S and num (fp32), copied
+// out to the fold buffer
+constexpr size_t kLrFoldBytes = 3 * 4 * kW8Tests * sizeof(float);
+constexpr size_t kLrRegion0 = kLrStageRegion > kLrFoldBytes ? kLrStageRegion : kLrFoldBytes;
 // s_waitcnt immediates (gfx9 encoding): vmcnt <= v, expcnt at its maximum, lgkmcnt 0
 constexpr int lr_wait_vm(int v) { return (v & 0xF) | ((v >> 4) << 14) | (0x7 << 4); }
-constexpr size_t kLrLdsBytes = kLrRegion0 + sizeof(RowInfo) + sizeof(double) * kTailHead + sizeof(GeneAcc) * 64;
+constexpr size_t kLrLdsBytes = kLrRegion0;
 static_assert(kLrLdsBytes + 64 <= kLdsCap, "LR Wald LDS layout");
+// the tests kernel: RowInfo, the tail head, the 64 genes' states, the eight
+// waves' partials
+constexpr size_t kLrTestsLdsBytes =
+    sizeof(RowInfo) + sizeof(double) * kTailHead + sizeof(GeneAcc) * 64 + kW8MergeBytes;
+static_assert(kLrTestsLdsBytes + 64 <= kLdsCap, "LR tests LDS layout");
 
 // Element (slot, sample i) of the gene tiles (s o r): per super-group sg and
 // k-step s one 8 KiB tile, the hi plane (4 groups x 64 lanes x 8 values) then
@@ -257,13 +269,9 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
 
   unsigned char* const bbuf = smem;
   unsigned char* const abuf = smem + kLrStages * kLrTileBytes;
-  RowInfo& ri = *reinterpret_cast<RowInfo*>(smem + kLrRegion0);
-  double* thead = reinterpret_cast<double*>(smem + kLrRegion0 + sizeof(RowInfo));
-  GeneAcc* gstate = reinterpret_cast<GeneAcc*>(smem + kLrRegion0 + sizeof(RowInfo) + sizeof(double) * kTailHead);
-  GeneAcc* wpart = reinterpret_cast<GeneAcc*>(smem + kLrFoldBytes);
-  for (int i = threadIdx.x; i < kTailHead; i += kW8Threads) thead[i] = (&a.tail->df)[i];
-  if (threadIdx.x < 64) acc_init(gstate[threadIdx.x]);
-  const TailView tv{thead, &a.tail->u_m, a.tail->coef_b};
+  // the per-gene scale of gg for this wave's two groups
+  const double gsc0 = la.gscale[(sg * 4 + 2 * c) * 16 + gl];
+  const double gsc1 = la.gscale[(sg * 4 + 2 * c + 1) * 16 + gl];
   const unsigned char* gsrc = reinterpret_cast<const unsigned char*>(la.gb) + static_cast<size_t>(sg) * nks * kLrTileBytes;
   const int pair_off = r * 4096;
   const int segR = la.sg_R[sg], segc = la.sg_col[sg], segg = la.sg_gcol[sg], nqr = segR >> 5;
@@ -272,8 +280,7 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
 
   for (int tile = split; tile < n_tiles; tile += a.n_splits) {
     const int64_t row0 = static_cast<int64_t>(tile) * kTileRows;
-    __syncthreads();  // the previous tile's epilogue is done with ri, fold and the gene partials
-    w8_row_info<EQTL_MODEL_LMM>(a, ri, row0);
+    __syncthreads();  // the previous tile's fold has been copied out
     const int64_t rowP = row0 + 32 * r;
     const __bf16* a_row[kRB];
 #pragma unroll
@@ -446,36 +453,69 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
         w = wn;
       }
     }
-    // fold: each wave's two groups (gg before its per-gene scale, sum of squares, num) into LDS
+    // fold: gg = d(delta, lbar) x the interpolated sum, S = gg - ||W^T h||^2
+    // (FP64), num; into LDS planes in the tests' order, then copied out whole
     float* fgg = reinterpret_cast<float*>(smem);
-    float* fnum = fgg + 4 * kW8Tests;
-    double* fss = reinterpret_cast<double*>(fnum + 4 * kW8Tests);
-    __syncthreads();  // the staging buffers are free (every wave is past the last k-step)
+    float* fS = fgg + 4 * kW8Tests;
+    float* fnum = fS + 4 * kW8Tests;
+    __syncthreads();  // the staging buffers are free (every wave is past the last step)
 #pragma unroll
     for (int gq = 0; gq < 2; ++gq) {
       const int q = 2 * c + gq;
+      const double gsc = gq ? gsc1 : gsc0;
 #pragma unroll
       for (int rb = 0; rb < kRB; ++rb)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int t = q * kW8Tests + (32 * r + 16 * rb + 4 * (lane >> 4) + e) * 16 + gl;
-          fgg[t] = ggv[gq][rb][e];
+          const double gg = gsc * ggv[gq][rb][e];
+          fgg[t] = static_cast<float>(gg);
+          fS[t] = static_cast<float>(gg - ssq[gq][rb][e]);
           fnum[t] = an[gq][rb][e];
-          fss[t] = ssq[gq][rb][e];
         }
     }
-    // the FP64 tests, one gene group at a time
+    __syncthreads();
+    {
+      const float4* src = reinterpret_cast<const float4*>(smem);
+      float4* dst = reinterpret_cast<float4*>(la.fold + lr_fold_base(tile, sg, la.n_sg));
+#pragma unroll 4
+      for (int i = threadIdx.x; i < static_cast<int>(kLrFoldBytes / 16); i += kW8Threads) dst[i] = src[i];
+    }
+  }
+}
+
+// The FP64 tests of the interpolated form over the fold buffer: one
+// workgroup per (super-group, split) takes the split's tiles; per tile and
+// gene group the eight-wave epilogue (refinement, polish hand-off, hits, ACAT,
+// best) and, at the end, the 64 genes' partial states.
+__global__ __launch_bounds__(kW8Threads) void k_lr_tests(LrArgs la) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  const WaldArgs& a = la.w;
+  const int sg = blockIdx.x / a.n_splits, split = blockIdx.x % a.n_splits;
+  if (sg >= la.n_sg) return;
+  RowInfo& ri = *reinterpret_cast<RowInfo*>(smem);
+  double* thead = reinterpret_cast<double*>(smem + sizeof(RowInfo));
+  GeneAcc* gstate = reinterpret_cast<GeneAcc*>(smem + sizeof(RowInfo) + sizeof(double) * kTailHead);
+  GeneAcc* wpart = gstate + 64;
+  for (int i = threadIdx.x; i < kTailHead; i += kW8Threads) thead[i] = (&a.tail->df)[i];
+  if (threadIdx.x < 64) acc_init(gstate[threadIdx.x]);
+  const TailView tv{thead, &a.tail->u_m, a.tail->coef_b};
+  const int n_tiles = (a.n_tiles + 1) >> 1;
+  for (int tile = split; tile < n_tiles; tile += a.n_splits) {
+    const int64_t row0 = static_cast<int64_t>(tile) * kTileRows;
+    __syncthreads();  // the previous tile's tests are done with ri
+    w8_row_info<EQTL_MODEL_LMM>(a, ri, row0);
+    const float* f = la.fold + lr_fold_base(tile, sg, la.n_sg);
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
-      __syncthreads();  // the fold is complete / the previous group's partials are merged
+      __syncthreads();  // ri is in / the previous group's partials are merged
       const W8Gene gi = lr_gene(la, sg * 4 + q);
       const int qo = q * kW8Tests;
-      const double gsc = la.gscale[(sg * 4 + q) * 16 + gi.gl];  // tests t of this thread: gene t & 15 == gl
       w8_epilogue<EQTL_MODEL_LMM>(a, gi, ri, tv, gstate + 16 * q, wpart, row0,
                                   [&](int t, double& gg, double& S, double& num) {
-                                    gg = gsc * fgg[qo + t];
-                                    S = gg - fss[qo + t];
-                                    num = fnum[qo + t];
+                                    gg = f[qo + t];
+                                    S = f[4 * kW8Tests + qo + t];
+                                    num = f[8 * kW8Tests + qo + t];
                                   });
     }
   }

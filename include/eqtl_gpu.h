@@ -189,7 +189,8 @@ int eqtl_gpu_set_basis(eqtl_gpu_ctx* ctx, const double* Q, const double* lambda,
  * Y~ = Q^T Y, X~ = Q^T X (once, FP64), per-gene REML delta (delta_mode), then
  * the per-gene whitened orthonormal covariate basis and null residual that the
  * fused Wald kernel contracts against. delta_in used when mode is GIVEN.
- * delta_out / rss_null_out (G each) nullable. */
+ * delta_out / rss_null_out (G each) nullable. Y (n x G) and X (n x p),
+ * column-major, may be host or device memory. */
 int eqtl_gpu_lmm_prep(eqtl_gpu_ctx* ctx, const double* Y, int32_t G, const double* X, int32_t p,
                       int32_t delta_mode, const double* delta_in, double* delta_out,
                       double* rss_null_out);

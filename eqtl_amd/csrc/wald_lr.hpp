@@ -94,7 +94,7 @@ static_assert(kLrLdsBytes + 64 <= kLdsCap, "LR Wald LDS layout");
 // the tests kernel: RowInfo, the tail head, the 64 genes' states, the eight
 // waves' partials
 constexpr size_t kLrTestsLdsBytes =
-    sizeof(RowInfo) + sizeof(double) * kTailHead + sizeof(GeneAcc) * 64 + kW8MergeBytes;
+    sizeof(RowInfo) + sizeof(double) * kTailHead + sizeof(GeneAcc) * 64 + kW8MergeBytes + sizeof(W8Gene) * 64;
 static_assert(kLrTestsLdsBytes + 64 <= kLdsCap, "LR tests LDS layout");
 
 // Element (slot, sample i) of the gene tiles (s o r): per super-group sg and
@@ -497,26 +497,82 @@ __global__ __launch_bounds__(kW8Threads) void k_lr_tests(LrArgs la) {
   double* thead = reinterpret_cast<double*>(smem + sizeof(RowInfo));
   GeneAcc* gstate = reinterpret_cast<GeneAcc*>(smem + sizeof(RowInfo) + sizeof(double) * kTailHead);
   GeneAcc* wpart = gstate + 64;
+  W8Gene* ginfo = reinterpret_cast<W8Gene*>(wpart + kW8Waves * 16);  // the super-group's 64 genes
   for (int i = threadIdx.x; i < kTailHead; i += kW8Threads) thead[i] = (&a.tail->df)[i];
-  if (threadIdx.x < 64) acc_init(gstate[threadIdx.x]);
+  if (threadIdx.x < 64) {
+    acc_init(gstate[threadIdx.x]);
+    ginfo[threadIdx.x] = lr_gene(la, sg * 4 + (threadIdx.x >> 4));
+  }
   const TailView tv{thead, &a.tail->u_m, a.tail->coef_b};
   const int n_tiles = (a.n_tiles + 1) >> 1;
+  // the SNP data of the next tile, loaded while this tile's tests run
+  uint8_t r_keep = 0;
+  double r_af = 0.0;
+  int32_t r_chrom = -1;
+  int64_t r_pos = 0;
+  auto load_rows = [&](int tl) {
+    if (threadIdx.x < kTileRows) {
+      const int64_t j = static_cast<int64_t>(tl) * kTileRows + threadIdx.x;
+      const bool in = j < a.m, ann = in && a.snp_chrom;
+      r_keep = in ? a.keep[j] : 0;
+      r_af = in ? a.af[j] : 0.0;
+      r_chrom = ann ? a.snp_chrom[a.snp0 + j] : -1;
+      r_pos = ann ? a.snp_pos[a.snp0 + j] : 0;
+    }
+  };
+  if (split < n_tiles) load_rows(split);
+  // Thread x takes tests x + 512 i (i = 0..3) of each group, in that order.
+  // Their gg, S, num are loaded one group ahead (the next group's loads are
+  // in flight while this group's tests run).
+  static_assert(kW8Tests == 4 * kW8Threads, "four tests per thread and group");
+  float c0[4], c1[4], c2[4], n0[4], n1[4], n2[4];
+  auto load = [&](int tl, int q, float (&g0)[4], float (&g1)[4], float (&g2)[4]) {
+    const float* fb = la.fold + lr_fold_base(tl, sg, la.n_sg) + q * kW8Tests + threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      g0[i] = fb[kW8Threads * i];
+      g1[i] = fb[4 * kW8Tests + kW8Threads * i];
+      g2[i] = fb[8 * kW8Tests + kW8Threads * i];
+    }
+  };
+  if (split < n_tiles) load(split, 0, c0, c1, c2);
   for (int tile = split; tile < n_tiles; tile += a.n_splits) {
     const int64_t row0 = static_cast<int64_t>(tile) * kTileRows;
     __syncthreads();  // the previous tile's tests are done with ri
-    w8_row_info<EQTL_MODEL_LMM>(a, ri, row0);
-    const float* f = la.fold + lr_fold_base(tile, sg, la.n_sg);
+    if (threadIdx.x < kTileRows) {
+      const int t = threadIdx.x;
+      ri.keep[t] = r_keep;
+      ri.af[t] = r_af;
+      ri.gtg[t] = 0.0;
+      ri.chrom[t] = r_chrom;
+      ri.pos[t] = r_pos;
+    }
+    if (tile + a.n_splits < n_tiles) load_rows(tile + a.n_splits);
 #pragma unroll 1
     for (int q = 0; q < 4; ++q) {
+      const int nt = q < 3 ? tile : tile + a.n_splits, nq = q < 3 ? q + 1 : 0;
+      if (nt < n_tiles) load(nt, nq, n0, n1, n2);
       __syncthreads();  // ri is in / the previous group's partials are merged
-      const W8Gene gi = lr_gene(la, sg * 4 + q);
-      const int qo = q * kW8Tests;
+      W8Gene gi = ginfo[q * 16 + (threadIdx.x & 15)];
+      gi.gl = threadIdx.x & 15;
       w8_epilogue<EQTL_MODEL_LMM>(a, gi, ri, tv, gstate + 16 * q, wpart, row0,
-                                  [&](int t, double& gg, double& S, double& num) {
-                                    gg = f[qo + t];
-                                    S = f[4 * kW8Tests + qo + t];
-                                    num = f[8 * kW8Tests + qo + t];
+                                  [&](int, double& gg, double& S, double& num) {
+                                    gg = c0[0];
+                                    S = c1[0];
+                                    num = c2[0];
+#pragma unroll
+                                    for (int i = 0; i < 3; ++i) {
+                                      c0[i] = c0[i + 1];
+                                      c1[i] = c1[i + 1];
+                                      c2[i] = c2[i + 1];
+                                    }
                                   });
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        c0[i] = n0[i];
+        c1[i] = n1[i];
+        c2[i] = n2[i];
+      }
     }
   }
   __syncthreads();

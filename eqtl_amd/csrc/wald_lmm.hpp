@@ -351,14 +351,13 @@ __device__ inline W8Gene w8_gene(const WaldArgs& a, int group) {
   return g;
 }
 
-// FP64 epilogue of one tile (2048 tests, 4 per thread), after the fold is in
-// LDS: fold(t, gg, S, num) reads test t = row * 16 + gene. The per-thread gene
-// states are merged over the four lanes holding the same gene, then over the
-// eight waves (wpart) into the running state in gstate.
+// FP64 tests of one tile and gene group (2048 tests, 4 per thread, in the
+// order i = 0..3): fold(t, gg, S, num) gives test t = row * 16 + gene. Returns
+// the thread's gene state merged over the four lanes holding its gene (lanes
+// l, l ^ 16, l ^ 32, l ^ 48 then hold the wave's total for gene l & 15).
 template <int MODEL, class Fold>
-__device__ inline void w8_epilogue(const WaldArgs& a, const W8Gene& gi, const RowInfo& ri, const TailView& tv,
-                                   GeneAcc* gstate, GeneAcc* wpart, int64_t row0, Fold fold) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ inline GeneAcc w8_tests(const WaldArgs& a, const W8Gene& gi, const RowInfo& ri, const TailView& tv,
+                                   int64_t row0, Fold fold) {
   GeneAcc st;
   acc_init(st);
   dd asum{0.0, 0.0};
@@ -381,6 +380,16 @@ __device__ inline void w8_epilogue(const WaldArgs& a, const W8Gene& gi, const Ro
   // lanes l, l ^ 16, l ^ 32, l ^ 48 hold the same gene
   acc_merge(st, shfl_xor_acc(st, 16));
   acc_merge(st, shfl_xor_acc(st, 32));
+  return st;
+}
+
+// w8_tests, then the eight waves' totals (wpart) merged into the running
+// states of the group's 16 genes in gstate.
+template <int MODEL, class Fold>
+__device__ inline void w8_epilogue(const WaldArgs& a, const W8Gene& gi, const RowInfo& ri, const TailView& tv,
+                                   GeneAcc* gstate, GeneAcc* wpart, int64_t row0, Fold fold) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const GeneAcc st = w8_tests<MODEL>(a, gi, ri, tv, row0, fold);
   if (lane < 16) wpart[wave * 16 + lane] = st;
   __syncthreads();
   if (threadIdx.x < 16) {

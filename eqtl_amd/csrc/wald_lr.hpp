@@ -94,7 +94,7 @@ static_assert(kLrLdsBytes + 64 <= kLdsCap, "LR Wald LDS layout");
 // the tests kernel: RowInfo, the tail head, the 64 genes' states, the eight
 // waves' partials
 constexpr size_t kLrTestsLdsBytes =
-    sizeof(RowInfo) + sizeof(double) * kTailHead + sizeof(GeneAcc) * 64 + kW8MergeBytes + sizeof(W8Gene) * 64;
+    sizeof(RowInfo) + sizeof(double) * kTailHead + sizeof(GeneAcc) * 64 * kW8Waves + sizeof(W8Gene) * 64;
 static_assert(kLrTestsLdsBytes + 64 <= kLdsCap, "LR tests LDS layout");
 
 // Element (slot, sample i) of the gene tiles (s o r): per super-group sg and
@@ -495,14 +495,14 @@ __global__ __launch_bounds__(kW8Threads) void k_lr_tests(LrArgs la) {
   if (sg >= la.n_sg) return;
   RowInfo& ri = *reinterpret_cast<RowInfo*>(smem);
   double* thead = reinterpret_cast<double*>(smem + sizeof(RowInfo));
-  GeneAcc* gstate = reinterpret_cast<GeneAcc*>(smem + sizeof(RowInfo) + sizeof(double) * kTailHead);
-  GeneAcc* wpart = gstate + 64;
-  W8Gene* ginfo = reinterpret_cast<W8Gene*>(wpart + kW8Waves * 16);  // the super-group's 64 genes
+  // per wave the running states of the super-group's 64 genes: a wave adds
+  // its own totals without waiting for the others; merged at the end
+  GeneAcc* wstate = reinterpret_cast<GeneAcc*>(smem + sizeof(RowInfo) + sizeof(double) * kTailHead);
+  W8Gene* ginfo = reinterpret_cast<W8Gene*>(wstate + 64 * kW8Waves);  // the super-group's 64 genes
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < kTailHead; i += kW8Threads) thead[i] = (&a.tail->df)[i];
-  if (threadIdx.x < 64) {
-    acc_init(gstate[threadIdx.x]);
-    ginfo[threadIdx.x] = lr_gene(la, sg * 4 + (threadIdx.x >> 4));
-  }
+  acc_init(wstate[threadIdx.x]);
+  if (threadIdx.x < 64) ginfo[threadIdx.x] = lr_gene(la, sg * 4 + (threadIdx.x >> 4));
   const TailView tv{thead, &a.tail->u_m, a.tail->coef_b};
   const int n_tiles = (a.n_tiles + 1) >> 1;
   // the SNP data of the next tile, loaded while this tile's tests run
@@ -552,10 +552,10 @@ __global__ __launch_bounds__(kW8Threads) void k_lr_tests(LrArgs la) {
     for (int q = 0; q < 4; ++q) {
       const int nt = q < 3 ? tile : tile + a.n_splits, nq = q < 3 ? q + 1 : 0;
       if (nt < n_tiles) load(nt, nq, n0, n1, n2);
-      __syncthreads();  // ri is in / the previous group's partials are merged
+      if (q == 0) __syncthreads();  // ri is in
       W8Gene gi = ginfo[q * 16 + (threadIdx.x & 15)];
       gi.gl = threadIdx.x & 15;
-      w8_epilogue<EQTL_MODEL_LMM>(a, gi, ri, tv, gstate + 16 * q, wpart, row0,
+      const GeneAcc st = w8_tests<EQTL_MODEL_LMM>(a, gi, ri, tv, row0,
                                   [&](int, double& gg, double& S, double& num) {
                                     gg = c0[0];
                                     S = c1[0];
@@ -567,6 +567,7 @@ __global__ __launch_bounds__(kW8Threads) void k_lr_tests(LrArgs la) {
                                       c2[i] = c2[i + 1];
                                     }
                                   });
+      if (lane < 16) acc_merge(wstate[wave * 64 + q * 16 + lane], st);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         c0[i] = n0[i];
@@ -577,8 +578,10 @@ __global__ __launch_bounds__(kW8Threads) void k_lr_tests(LrArgs la) {
   }
   __syncthreads();
   if (threadIdx.x < 64) {
+    GeneAcc g = wstate[threadIdx.x];
+    for (int w = 1; w < kW8Waves; ++w) acc_merge(g, wstate[w * 64 + threadIdx.x]);
     const int gene = la.slot_gene[sg * 64 + threadIdx.x];
-    if (gene >= 0) a.partial[static_cast<size_t>(split) * a.G_pad + gene] = gstate[threadIdx.x];
+    if (gene >= 0) a.partial[static_cast<size_t>(split) * a.G_pad + gene] = g;
   }
 }
 

@@ -1795,7 +1795,7 @@ static int lr_build(eqtl_gpu_ctx* c) {
 }
 
 // The interpolated form: k_wald_lr (contractions, interpolation, fold out)
-// over grid, then k_lr_tests over n_sg x n_splits workgroups.
+// over grid; then launch_lr_tests: k_lr_tests over the same grid.
 int launch_wald_lr(eqtl_gpu_ctx* c, const LrArgs& la, dim3 grid) {
   static bool attr_set = false;
   if (!attr_set) {
@@ -1806,9 +1806,11 @@ int launch_wald_lr(eqtl_gpu_ctx* c, const LrArgs& la, dim3 grid) {
     attr_set = true;
   }
   hipLaunchKernelGGL(k_wald_lr, grid, dim3(kW8Threads), kLrLdsBytes, c->stream, la);
-  CHK(check_launch("k_wald_lr"));
-  hipLaunchKernelGGL(k_lr_tests, dim3(static_cast<unsigned>(la.n_sg * la.w.n_splits)), dim3(kW8Threads),
-                     kLrTestsLdsBytes, c->stream, la);
+  return check_launch("k_wald_lr");
+}
+
+int launch_lr_tests(eqtl_gpu_ctx* c, const LrArgs& la, dim3 grid) {
+  hipLaunchKernelGGL(k_lr_tests, grid, dim3(kW8Threads), kLrTestsLdsBytes, c->stream, la);
   return check_launch("k_lr_tests");
 }
 
@@ -2139,10 +2141,11 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     a.polish = polish_on ? c->polish.as<PolishItem>() : nullptr;
     a.polish_cap = polish_on ? c->polish_cap_dev : 0;
     HIPC(hipMemsetAsync(c->hit_count.p, 0, 2 * sizeof(unsigned long long), c->stream));
+    LrArgs la;
+    dim3 lr_grid;
     {
-      Timed t(c, "wald");
+      Timed t(c, "wald");  // k_wald_lr (its FP64 tests are timed as "tests"), or the eight-wave kernel
       if (lr) {
-        LrArgs la;
         la.w = a;
         la.Ghi = c->lr_G2.as<__bf16>();
         la.lo_off = m_pad * np;
@@ -2163,10 +2166,16 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
         CHK(ensure(c->lr_fold, sizeof(float) * lr_fold_base(n_t128, 0, c->lr_n_sg)));
         la.fold = c->lr_fold.as<float>();
         const int64_t rounds = (c->lr_n_sg + 8 * lr_spx - 1) / (8 * lr_spx);
-        CHK(launch_wald_lr(c, la, dim3(static_cast<unsigned>(rounds * 8 * lr_spx * n_splits))));
+        lr_grid = dim3(static_cast<unsigned>(rounds * 8 * lr_spx * n_splits));
+        CHK(launch_wald_lr(c, la, lr_grid));
       } else {
         CHK(launch_wald(c, a, dim3(static_cast<unsigned>(cis ? n_items : n_groups_pad * n_splits))));
       }
+      CHK(t.done());
+    }
+    if (lr) {
+      Timed t(c, "tests");
+      CHK(launch_lr_tests(c, la, lr_grid));
       CHK(t.done());
     }
     unsigned long long cnt2[2] = {0, 0};

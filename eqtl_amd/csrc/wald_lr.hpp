@@ -485,13 +485,21 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
 }
 
 // The FP64 tests of the interpolated form over the fold buffer: one
-// workgroup per (super-group, split) takes the split's tiles; per tile and
-// gene group the eight-wave epilogue (refinement, polish hand-off, hits, ACAT,
-// best) and, at the end, the 64 genes' partial states.
+// workgroup per (super-group, split), laid out as in k_wald_lr, takes the
+// split's tiles; per tile and gene group the tests of the eight-wave epilogue
+// (refinement, polish hand-off, hits, ACAT, best) and, at the end, the 64
+// genes' partial states.
 __global__ __launch_bounds__(kW8Threads) void k_lr_tests(LrArgs la) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const WaldArgs& a = la.w;
-  const int sg = blockIdx.x / a.n_splits, split = blockIdx.x % a.n_splits;
+  // the workgroup layout of k_wald_lr: the workgroups of one XCD that share a
+  // split test the same SNP rows (whose FP64 rows the refinement reads), those
+  // that share a super-group the same genes (their FP64 residual vectors)
+  const int wg = blockIdx.x, xcd = wg & 7, jx = wg >> 3;
+  const int per_xcd = la.sg_per_xcd * a.n_splits;
+  const int l = jx % per_xcd;
+  const int sg = ((jx / per_xcd) * 8 + xcd) * la.sg_per_xcd + l / a.n_splits;
+  const int split = l % a.n_splits;
   if (sg >= la.n_sg) return;
   RowInfo& ri = *reinterpret_cast<RowInfo*>(smem);
   double* thead = reinterpret_cast<double*>(smem + sizeof(RowInfo));

@@ -244,12 +244,12 @@ def cpu_baseline(Q, lam, X, Y, rows, delta):
                       f"prep untimed ({tests} tests in {dt:.1f} s)"}
 
 
-def measured_traffic(block):
-    """HBM bytes per launch of the fused Wald kernel (split form, this workload)
-    from the committed rocprofv3 PMC summary: FETCH_SIZE and WRITE_SIZE runs,
-    each scaled by the same counter on a 1 GiB device copy. None when the
-    summary was taken at another block size or is absent."""
-    path = os.path.join(ROOT, "profiles", "wald_bf16x3_trans_lmm_n2000_g20000_pmc.md")
+def measured_traffic(block, name="wald_lr_trans_lmm_n2000_g20000_pmc.md"):
+    """HBM bytes per launch of the dominant Wald kernel (this workload) from the
+    committed rocprofv3 PMC summary: FETCH_SIZE and WRITE_SIZE runs, each
+    scaled by the same counter on a 1 GiB device copy. None when the summary
+    was taken at another block size or is absent."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     vals = {}
@@ -1024,7 +1024,8 @@ def main():
                    "snps_per_step": m, "genes": G, "samples": N, "covariates": p, "delta": "batched Newton",
                    "snps_per_block": BLOCK},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / BF16_PEAK_TFLOPS, "traffic": measured_traffic(BLOCK),
+                     "frac": achieved / BF16_PEAK_TFLOPS,
+                     "traffic": measured_traffic(BLOCK) if lr_on else None,
                      "note": ("k_wald_lr, the dominant kernel: algorithmic 2n + 2R(p+1) FLOP/test (num over the "
                               "samples, p projections and gg interpolated from R nodes) over its event-timed "
                               "launches; it issues 3 bf16 MFMA products per num MAC and 6 per interpolation MAC; "

@@ -237,11 +237,59 @@ def cpu_baseline(Q, lam, X, Y, rows, delta):
                        pthr=1e-5, hit_cap=1 << 16)
     dt = res["loop_seconds"]
     tests = int(res["state"]["n_tested"].sum())
-    return {"value": tests / dt, "unit": "Wald tests/s", "cores": threads, "kind": "port",
-            "sample": f"{n_snp} SNPs x {Y.shape[1]} genes of the same data (n={N}, p={X.shape[1]}, the genes' "
-                      f"own delta), FP64 restatement of the reference SNP-outer loop (Q^T g per SNP, "
-                      f"test_lmm_gtil per gene) on {threads} threads, timed over the SNP loop only, per-gene "
-                      f"prep untimed ({tests} tests in {dt:.1f} s)"}
+    out = {"value": tests / dt, "unit": "Wald tests/s", "cores": threads, "kind": "port",
+           "sample": f"{n_snp} SNPs x {Y.shape[1]} genes of the same data (n={N}, p={X.shape[1]}, the genes' "
+                     f"own delta), FP64 restatement of the reference SNP-outer loop (Q^T g per SNP, "
+                     f"test_lmm_gtil per gene) on {threads} threads, timed over the SNP loop only, per-gene "
+                     f"prep untimed ({tests} tests in {dt:.1f} s)"}
+    cal = reference_calibration(Q, lam, X, Y[:, 0], dos[keep.astype(bool)][:CAL_SNPS])
+    if cal is not None:
+        out["calibration"] = cal
+    return out
+
+
+CAL_SNPS = 150
+
+
+def reference_calibration(Q, lam, X, y, dos):
+    """The port against the reference's own compiled code on identical work, one
+    thread each: prep_lmm (REML search) + test_lmm (Q^T g and the bordered-Schur
+    test per SNP, model_lmm.cpp:124-223) for one gene over the given SNP rows,
+    as the reference's cis scan calls them. None where the reference build
+    (oracle/_ref) is absent."""
+    import ctypes
+    import oracle as O
+    R = O.ref_lib()
+    if R is None:
+        return None
+    dos = np.ascontiguousarray(dos, dtype=np.float64)
+    c, n = dos.shape
+    p = X.shape[1]
+    Xf = np.asfortranarray(X, dtype=np.float64)
+    Qf = np.asfortranarray(Q, dtype=np.float64)
+    yv = np.ascontiguousarray(y, dtype=np.float64)
+    lv = np.ascontiguousarray(lam, dtype=np.float64)
+    t0 = time.perf_counter()
+    port = O.cis_gene("lmm", yv, Xf, dos, B=0, gene="cal", Q=Qf, lam=lv)
+    t_port = time.perf_counter() - t0
+    res = np.empty(c * 5)
+    prep = np.empty(4)
+    vp = ctypes.c_void_p
+    t0 = time.perf_counter()
+    rc = R.ref_lmm(yv.ctypes.data_as(vp), Xf.ctypes.data_as(vp), ctypes.c_int(n), ctypes.c_int(p),
+                   Qf.ctypes.data_as(vp), lv.ctypes.data_as(vp), dos.ctypes.data_as(vp), ctypes.c_int(c),
+                   res.ctypes.data_as(vp), prep.ctypes.data_as(vp))
+    t_ref = time.perf_counter() - t0
+    if rc != 0:
+        return None
+    pr = res.reshape(c, 5)[:, 3]
+    ok = np.isfinite(port["p"]) & np.isfinite(pr) & (pr > 0)
+    return {"port_tests_per_s": c / t_port, "reference_tests_per_s": c / t_ref,
+            "port_over_reference": t_ref / t_port,
+            "max_p_rel_diff": float(np.max(np.abs(port["p"][ok] - pr[ok]) / pr[ok])) if ok.any() else None,
+            "sample": f"one gene x {c} SNPs (n={n}, p={p}): REML delta search + Q^T g and the test per SNP, "
+                      f"the oracle's restatement vs the reference's model_lmm.cpp compiled from its own "
+                      f"sources (oracle/_ref), one thread each"}
 
 
 def measured_traffic(block, name="wald_lr_trans_lmm_n2000_g20000_pmc.md"):

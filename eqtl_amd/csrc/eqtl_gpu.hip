@@ -1972,7 +1972,7 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     // g' = g_c - Wx Wx^T g_c (FP64) and gtg = ||g'||^2 before the rotation / contraction
     if (!c->have_wx) return fail(EQTL_E_STATE, "covariate basis missing: prepare genes first");
     Timed t(c, "residualize");
-    hipLaunchKernelGGL(k_residualize, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_residualize, dim3(static_cast<unsigned>(m_pad / (4 * kResRows))), dim3(256), 0, c->stream,
                        c->A64.as<double>(), c->A32.as<float>(), m_pad, c->n, np, c->Wx.as<double>(), c->p,
                        c->gtg.as<double>());
     CHK(check_launch("k_residualize"));
@@ -2367,7 +2367,7 @@ static int exact_rows(eqtl_gpu_ctx* c, int64_t lo, int64_t hi, bool lmm) {
     const int64_t m_pad = round_up(mb, 128);
     CHK(ensure_block(c, mb, bps, true));
     CHK(run_decode(c, c->bed_res.as<uint8_t>() + j0 * bps, mb, bps, true, nullptr));
-    hipLaunchKernelGGL(k_residualize, dim3(static_cast<unsigned>(m_pad / 4)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(k_residualize, dim3(static_cast<unsigned>(m_pad / (4 * kResRows))), dim3(256), 0, c->stream,
                        c->A64.as<double>(), c->A32.as<float>(), m_pad, n, np, c->Wx.as<double>(), c->p,
                        c->gtg.as<double>());
     CHK(check_launch("k_residualize"));

@@ -1301,49 +1301,80 @@ __global__ void k_finalize(const GeneAcc* state, eqtl_gene_state* out, int G) {
 // ||g'|| instead of ||g||, so a SNP that the covariates explain almost fully
 // (a conditioning SNP in LD, genotype PCs) keeps its relative accuracy in S
 // and in the numerator.
+// kResRows SNP rows per wave: each covariate value loaded serves all of them
+// (one row per wave re-read the n x p basis from L2 for every SNP). Per row the
+// sums and FMAs run in the same order as for one row alone.
+constexpr int kResRows = 4;
 __global__ __launch_bounds__(256) void k_residualize(double* A64, float* A32, int64_t m_pad, int n, int n_pad,
                                                      const double* Wx, int p, double* gtg) {
   constexpr int kPCov = 32;  // covariate columns at most (intercept included)
   constexpr int kCh = 8;     // columns per pass (independent reductions in flight)
-  const int lane = threadIdx.x & 63;
-  const int64_t j = static_cast<int64_t>(blockIdx.x) * 4 + (threadIdx.x >> 6);
-  if (j >= m_pad || p > kPCov) return;
-  double* g = A64 + j * n_pad;
-  float* g32 = A32 + j * n_pad;
-  double cf[kPCov];  // indexed with compile-time indices only (kept in registers)
+  __shared__ double cfs[4][kResRows][kPCov];  // per wave: the rows' coefficients Wx^T g_c
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t j0 = (static_cast<int64_t>(blockIdx.x) * 4 + w) * kResRows;
+  if (j0 >= m_pad || p > kPCov) return;  // m_pad is a multiple of 4 * kResRows
+  double* g[kResRows];
+  float* g32[kResRows];
+#pragma unroll
+  for (int r = 0; r < kResRows; ++r) {
+    g[r] = A64 + (j0 + r) * n_pad;
+    g32[r] = A32 + (j0 + r) * n_pad;
+  }
 #pragma unroll
   for (int c0 = 0; c0 < kPCov; c0 += kCh) {
-    double s[kCh];
+    if (c0 >= p) break;
+    double sm[kResRows][kCh];
 #pragma unroll
-    for (int u = 0; u < kCh; ++u) s[u] = 0.0;
-    if (c0 < p) {
-      for (int i = lane; i < n; i += 64) {
-        const double gi = g[i];
+    for (int r = 0; r < kResRows; ++r)
 #pragma unroll
-        for (int u = 0; u < kCh; ++u)
-          if (c0 + u < p) s[u] = fma(Wx[static_cast<size_t>(c0 + u) * n + i], gi, s[u]);
-      }
+      for (int u = 0; u < kCh; ++u) sm[r][u] = 0.0;
+    for (int i = lane; i < n; i += 64) {
+      double gi[kResRows];
+#pragma unroll
+      for (int r = 0; r < kResRows; ++r) gi[r] = g[r][i];
 #pragma unroll
       for (int u = 0; u < kCh; ++u)
+        if (c0 + u < p) {
+          const double wv = Wx[static_cast<size_t>(c0 + u) * n + i];
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) s[u] += __shfl_xor(s[u], o);
+          for (int r = 0; r < kResRows; ++r) sm[r][u] = fma(wv, gi[r], sm[r][u]);
+        }
     }
 #pragma unroll
-    for (int u = 0; u < kCh; ++u) cf[c0 + u] = s[u];
+    for (int r = 0; r < kResRows; ++r)
+#pragma unroll
+      for (int u = 0; u < kCh; ++u) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sm[r][u] += __shfl_xor(sm[r][u], o);
+        if (lane == 0 && c0 + u < kPCov) cfs[w][r][c0 + u] = sm[r][u];
+      }
   }
-  double ss = 0.0;
+  __builtin_amdgcn_wave_barrier();
+  double ss[kResRows];
+#pragma unroll
+  for (int r = 0; r < kResRows; ++r) ss[r] = 0.0;
   for (int i = lane; i < n; i += 64) {
-    double v = g[i];
+    double v[kResRows];
 #pragma unroll
-    for (int c = 0; c < kPCov; ++c)
-      if (c < p) v = fma(-Wx[static_cast<size_t>(c) * n + i], cf[c], v);
-    g[i] = v;
-    g32[i] = static_cast<float>(v);
-    ss = fma(v, v, ss);
+    for (int r = 0; r < kResRows; ++r) v[r] = g[r][i];
+    for (int c = 0; c < p; ++c) {
+      const double wv = Wx[static_cast<size_t>(c) * n + i];
+#pragma unroll
+      for (int r = 0; r < kResRows; ++r) v[r] = fma(-wv, cfs[w][r][c], v[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < kResRows; ++r) {
+      g[r][i] = v[r];
+      g32[r][i] = static_cast<float>(v[r]);
+      ss[r] = fma(v[r], v[r], ss[r]);
+    }
   }
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
-  if (lane == 0 && gtg) gtg[j] = ss;
+  for (int r = 0; r < kResRows; ++r) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ss[r] += __shfl_xor(ss[r], o);
+    if (lane == 0 && gtg) gtg[j0 + r] = ss[r];
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -13,6 +13,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/eqtl_gpu.h"
@@ -251,6 +252,7 @@ struct eqtl_gpu_ctx {
   double lr_mean_nodes = 0.0;  // interpolation nodes per valid gene, averaged
   int64_t lr_ldT = 0;
   DBuf lr_fold;  // k_wald_lr -> k_lr_tests: gg, S, num per test of a block (lr_fold_base)
+  DBuf lr_Z, lr_QZ, lr_nodes, lr_okb;  // plan workspace (node bases, nodes, their status)
   DBuf lr_slot_gene, lr_sg_col, lr_sg_R, lr_coef, lr_gscale, lr_gb, lr_QZhi, lr_QZlo, lr_Dhi, lr_Dlo, lr_Tp,
       lr_G2;
   // synthetic bed buffers
@@ -464,6 +466,25 @@ int launch_wald(eqtl_gpu_ctx* c, const WaldArgs& a, dim3 grid) {
 }  // namespace
 
 // ===========================================================================
+// Runs body(lo, hi) over [0, count) split into contiguous ranges of at least
+// grain items on up to eight host threads (the calling thread takes the first).
+template <class Body>
+static void host_parallel(int count, Body body, int grain = 256) {
+  const int nt = std::max(1, std::min(8, count / grain));
+  if (nt == 1) {
+    body(0, count);
+    return;
+  }
+  std::vector<std::thread> th;
+  const int per = (count + nt - 1) / nt;
+  for (int t = 1; t < nt; ++t) {
+    const int lo = t * per, hi = std::min(count, lo + per);
+    if (lo < hi) th.emplace_back(body, lo, hi);
+  }
+  body(0, std::min(count, per));
+  for (auto& x : th) x.join();
+}
+
 extern "C" {
 
 const char* eqtl_gpu_last_error(void) { return g_err.c_str(); }
@@ -1662,37 +1683,44 @@ static int lr_build(eqtl_gpu_ctx* c) {
       seg_wr[q][r] = ((r & 1) ? -1.0 : 1.0) * std::sin(th);
     }
   }
-  std::vector<double> ell;
-  for (int s2 = 0; s2 < G; ++s2) {
-    const int k = slot_gene[s2];
-    if (!valid[k]) continue;
-    const int q = sg_seg[s2 / 64];
-    const int R = segs[q].R;
-    const std::vector<double>& ur = seg_ur[q];
-    const std::vector<double>& wr = seg_wr[q];
-    ell.assign(R, 0.0);
-    int hit = -1;
-    for (int r = 0; r < R; ++r)
-      if (u[k] == ur[r]) hit = r;
-    if (hit >= 0) {
-      ell[hit] = 1.0;
-    } else {
-      double den = 0.0;
-      for (int r = 0; r < R; ++r) den += wr[r] / (u[k] - ur[r]);
-      for (int r = 0; r < R; ++r) ell[r] = wr[r] / (u[k] - ur[r]) / den;
+  // the genes' weights, slots split over host threads (disjoint outputs)
+  auto weights = [&](int s_lo, int s_hi) {
+    std::vector<double> ell;
+    for (int s2 = s_lo; s2 < s_hi; ++s2) {
+      const int k = slot_gene[s2];
+      if (!valid[k]) continue;
+      const int q = sg_seg[s2 / 64];
+      const int R = segs[q].R;
+      const std::vector<double>& ur = seg_ur[q];
+      const std::vector<double>& wr = seg_wr[q];
+      ell.assign(R, 0.0);
+      int hit = -1;
+      for (int r = 0; r < R; ++r)
+        if (u[k] == ur[r]) hit = r;
+      if (hit >= 0) {
+        ell[hit] = 1.0;
+      } else {
+        double den = 0.0;
+        for (int r = 0; r < R; ++r) den += wr[r] / (u[k] - ur[r]);
+        for (int r = 0; r < R; ++r) ell[r] = wr[r] / (u[k] - ur[r]) / den;
+      }
+      for (int r = 0; r < R; ++r) {
+        const __bf16 h = static_cast<__bf16>(static_cast<float>(ell[r]));
+        const double r1 = ell[r] - static_cast<double>(static_cast<float>(h));
+        const __bf16 m = static_cast<__bf16>(static_cast<float>(r1));
+        const double r2 = r1 - static_cast<double>(static_cast<float>(m));
+        coef[lr_coef_index(s2, r, 0)] = h;
+        coef[lr_coef_index(s2, r, 1)] = m;
+        coef[lr_coef_index(s2, r, 2)] = static_cast<__bf16>(static_cast<float>(r2));
+      }
     }
-    for (int r = 0; r < R; ++r) {
-      const __bf16 h = static_cast<__bf16>(static_cast<float>(ell[r]));
-      const double r1 = ell[r] - static_cast<double>(static_cast<float>(h));
-      const __bf16 m = static_cast<__bf16>(static_cast<float>(r1));
-      const double r2 = r1 - static_cast<double>(static_cast<float>(m));
-      coef[lr_coef_index(s2, r, 0)] = h;
-      coef[lr_coef_index(s2, r, 1)] = m;
-      coef[lr_coef_index(s2, r, 2)] = static_cast<__bf16>(static_cast<float>(r2));
-    }
-  }
+  };
+  host_parallel(G, weights);
   // node bases Z (np x ncols), sample-space Q Z, split
-  DBuf Z, QZ, nodes, okb;
+  DBuf& Z = c->lr_Z;
+  DBuf& QZ = c->lr_QZ;
+  DBuf& nodes = c->lr_nodes;
+  DBuf& okb = c->lr_okb;
   const size_t zc = static_cast<size_t>(np) * ncols;
   CHK(ensure(Z, sizeof(double) * zc));
   CHK(ensure(QZ, sizeof(double) * zc));
@@ -1732,15 +1760,17 @@ static int lr_build(eqtl_gpu_ctx* c) {
     for (int s2 = 0; s2 < G; ++s2) gsc[s2] = 1.0 / (dl[slot_gene[s2]] * lbar + 1.0);
     const size_t dn = static_cast<size_t>(nnodes_total) * np;
     std::vector<__bf16> dh(dn, static_cast<__bf16>(0.f)), dlo(dn, static_cast<__bf16>(0.f));
-    for (int k = 0; k < nnodes; ++k)
-      for (int i = 0; i < n; ++i) {
-        double v = nd[k] * lam[i] + 1.0;
-        if (v < 1e-12) v = 1e-12;
-        const float f = static_cast<float>((nd[k] * lbar + 1.0) / v);
-        const __bf16 h = static_cast<__bf16>(f);
-        dh[static_cast<size_t>(k) * np + i] = h;
-        dlo[static_cast<size_t>(k) * np + i] = static_cast<__bf16>(f - static_cast<float>(h));
-      }
+    host_parallel(nnodes, [&](int k_lo, int k_hi) {
+      for (int k = k_lo; k < k_hi; ++k)
+        for (int i = 0; i < n; ++i) {
+          double v = nd[k] * lam[i] + 1.0;
+          if (v < 1e-12) v = 1e-12;
+          const float f = static_cast<float>((nd[k] * lbar + 1.0) / v);
+          const __bf16 h = static_cast<__bf16>(f);
+          dh[static_cast<size_t>(k) * np + i] = h;
+          dlo[static_cast<size_t>(k) * np + i] = static_cast<__bf16>(f - static_cast<float>(h));
+        }
+    }, 16);
     CHK(ensure(c->lr_Dhi, sizeof(__bf16) * dn));
     CHK(ensure(c->lr_Dlo, sizeof(__bf16) * dn));
     HIPC(hipMemcpyAsync(c->lr_Dhi.p, dh.data(), sizeof(__bf16) * dn, hipMemcpyHostToDevice, c->stream));

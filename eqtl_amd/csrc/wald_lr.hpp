@@ -479,7 +479,15 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
       const float4* src = reinterpret_cast<const float4*>(smem);
       float4* dst = reinterpret_cast<float4*>(la.fold + lr_fold_base(tile, sg, la.n_sg));
 #pragma unroll 4
-      for (int i = threadIdx.x; i < static_cast<int>(kLrFoldBytes / 16); i += kW8Threads) dst[i] = src[i];
+      // streaming stores: the fold is read back only by k_lr_tests, so it
+      // should not displace the rows and node values this kernel re-reads
+      for (int i = threadIdx.x; i < static_cast<int>(kLrFoldBytes / 16); i += kW8Threads) {
+        const float4 v = src[i];
+        __builtin_nontemporal_store(v.x, &dst[i].x);
+        __builtin_nontemporal_store(v.y, &dst[i].y);
+        __builtin_nontemporal_store(v.z, &dst[i].z);
+        __builtin_nontemporal_store(v.w, &dst[i].w);
+      }
     }
   }
 }

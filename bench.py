@@ -1013,13 +1013,15 @@ def main():
     ctx.synchronize()
     dt = time.perf_counter() - t0
     value = tests / dt
+    num_ms, num_n = ctx.timing("num")
     wald_ms, wald_n = ctx.timing("wald")
     tests_ms, tests_n = ctx.timing("tests")
     rot_ms, _ = ctx.timing("rotate")
     dec_ms, _ = ctx.timing("decode")
     reml_ms, _ = ctx.timing("reml")
     basis_ms, _ = ctx.timing("basis")
-    log(f"step time {dt / args.steps:.2f}s: wald {wald_ms / args.steps:.0f} ms + tests {tests_ms / args.steps:.0f} ms, "
+    log(f"step time {dt / args.steps:.2f}s: num {num_ms / args.steps:.0f} ms + wald {wald_ms / args.steps:.0f} ms + "
+        f"tests {tests_ms / args.steps:.0f} ms, "
         f"rotate {rot_ms / args.steps:.0f} ms, decode {dec_ms / args.steps:.0f} ms, reml {reml_ms / args.steps:.0f} ms, "
         f"basis {basis_ms / args.steps:.0f} ms")
     ctx.set_timing(False)

@@ -251,7 +251,7 @@ struct eqtl_gpu_ctx {
   int lr_n_sg = 0, lr_ncols = 0, lr_nnodes = 0, lr_nseg = 0;
   double lr_mean_nodes = 0.0;  // interpolation nodes per valid gene, averaged
   int64_t lr_ldT = 0;
-  DBuf lr_fold;  // k_wald_lr -> k_lr_tests: gg, S, num per test of a block (lr_fold_base)
+  DBuf lr_fold;  // k_wald_lr (gg, S), k_lr_num (num) -> k_lr_tests, per test of a block (lr_fold_base)
   DBuf lr_Z, lr_QZ, lr_nodes, lr_okb;  // plan workspace (node bases, nodes, their status)
   DBuf lr_slot_gene, lr_sg_col, lr_sg_R, lr_coef, lr_gscale, lr_gb, lr_QZhi, lr_QZlo, lr_Dhi, lr_Dlo, lr_Tp,
       lr_G2;
@@ -1565,13 +1565,13 @@ static int scan_begin_impl(eqtl_gpu_ctx* c, int32_t model, int32_t prec, int32_t
   return EQTL_OK;
 }
 
-// ---- low-rank trans LMM scan (k_wald_lr) -------------------------------------
+// ---- low-rank trans LMM scan (k_lr_num, k_wald_lr) -----------------------------
 // Widths of ln-delta segments that 32, 64, 96 and 128 Chebyshev nodes
 // interpolate to ~1e-12 (the whitened covariate projection is analytic in
 // ln delta within pi of the real axis; measured on synthetic spectra).
 constexpr double kLrWidth[4] = {6.5, 13.0, 23.0, 33.5};
 
-// Builds the plan and operands of k_wald_lr for the prepared LMM genes: genes
+// Builds the plan and operands of k_lr_num / k_wald_lr for the prepared LMM genes: genes
 // sorted by ln delta into super-groups of 64, segments of ln delta (a
 // super-group's genes inside one segment), per segment R nodes, per node the
 // whitened projection Z (k_lr_node_basis) mapped to sample space (Q Z) and
@@ -1824,22 +1824,36 @@ static int lr_build(eqtl_gpu_ctx* c) {
   return EQTL_OK;
 }
 
-// The interpolated form: k_wald_lr (contractions, interpolation, fold out)
-// over grid; then launch_lr_tests: k_lr_tests over the same grid.
-int launch_wald_lr(eqtl_gpu_ctx* c, const LrArgs& la, dim3 grid) {
+// The interpolated form: k_lr_num (num, over the super-group pairs' grid)
+// and k_wald_lr (the interpolation: gg and S, over the super-groups' grid)
+// fill the fold; then k_lr_tests over the super-groups' grid.
+static int lr_set_attrs() {
   static bool attr_set = false;
-  if (!attr_set) {
-    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald_lr), hipFuncAttributeMaxDynamicSharedMemorySize,
-                             static_cast<int>(kLrLdsBytes)));
-    HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lr_tests),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLrTestsLdsBytes)));
-    attr_set = true;
-  }
+  if (attr_set) return EQTL_OK;
+  HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lr_num), hipFuncAttributeMaxDynamicSharedMemorySize,
+                           static_cast<int>(kLrNumLdsBytes)));
+  HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_wald_lr), hipFuncAttributeMaxDynamicSharedMemorySize,
+                           static_cast<int>(kLrLdsBytes)));
+  HIPC(hipFuncSetAttribute(reinterpret_cast<const void*>(&k_lr_tests), hipFuncAttributeMaxDynamicSharedMemorySize,
+                           static_cast<int>(kLrTestsLdsBytes)));
+  attr_set = true;
+  return EQTL_OK;
+}
+
+int launch_lr_num(eqtl_gpu_ctx* c, const LrArgs& la, dim3 grid) {
+  CHK(lr_set_attrs());
+  hipLaunchKernelGGL(k_lr_num, grid, dim3(kW8Threads), kLrNumLdsBytes, c->stream, la);
+  return check_launch("k_lr_num");
+}
+
+int launch_wald_lr(eqtl_gpu_ctx* c, const LrArgs& la, dim3 grid) {
+  CHK(lr_set_attrs());
   hipLaunchKernelGGL(k_wald_lr, grid, dim3(kW8Threads), kLrLdsBytes, c->stream, la);
   return check_launch("k_wald_lr");
 }
 
 int launch_lr_tests(eqtl_gpu_ctx* c, const LrArgs& la, dim3 grid) {
+  CHK(lr_set_attrs());
   hipLaunchKernelGGL(k_lr_tests, grid, dim3(kW8Threads), kLrTestsLdsBytes, c->stream, la);
   return check_launch("k_lr_tests");
 }
@@ -2172,35 +2186,46 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
     a.polish_cap = polish_on ? c->polish_cap_dev : 0;
     HIPC(hipMemsetAsync(c->hit_count.p, 0, 2 * sizeof(unsigned long long), c->stream));
     LrArgs la;
-    dim3 lr_grid;
-    {
-      Timed t(c, "wald");  // k_wald_lr (its FP64 tests are timed as "tests"), or the eight-wave kernel
-      if (lr) {
-        la.w = a;
-        la.Ghi = c->lr_G2.as<__bf16>();
-        la.lo_off = m_pad * np;
-        la.Tp = c->lr_Tp.as<__bf16>();
-        la.tpl = static_cast<int64_t>(m_pad) * c->lr_ldT;
-        la.ldT = c->lr_ldT;
-        la.sg_col = c->lr_sg_col.as<int32_t>();
-        la.sg_gcol = c->lr_sg_col.as<int32_t>() + c->lr_n_sg;
-        la.sg_R = c->lr_sg_R.as<int32_t>();
-        la.slot_gene = c->lr_slot_gene.as<int32_t>();
-        la.lcoef = c->lr_coef.as<__bf16>();
-        la.gscale = c->lr_gscale.as<double>();
-        la.gb = c->lr_gb.as<__bf16>();
-        la.n_sg = c->lr_n_sg;
-        la.p = c->p;
-        la.sg_per_xcd = lr_spx;
-        const int n_t128 = static_cast<int>((a.n_tiles + 1) / 2);
-        CHK(ensure(c->lr_fold, sizeof(float) * lr_fold_base(n_t128, 0, c->lr_n_sg)));
-        la.fold = c->lr_fold.as<float>();
-        const int64_t rounds = (c->lr_n_sg + 8 * lr_spx - 1) / (8 * lr_spx);
-        lr_grid = dim3(static_cast<unsigned>(rounds * 8 * lr_spx * n_splits));
-        CHK(launch_wald_lr(c, la, lr_grid));
-      } else {
-        CHK(launch_wald(c, a, dim3(static_cast<unsigned>(cis ? n_items : n_groups_pad * n_splits))));
+    dim3 lr_grid, num_grid;
+    if (lr) {
+      la.w = a;
+      la.Ghi = c->lr_G2.as<__bf16>();
+      la.lo_off = m_pad * np;
+      la.Tp = c->lr_Tp.as<__bf16>();
+      la.tpl = static_cast<int64_t>(m_pad) * c->lr_ldT;
+      la.ldT = c->lr_ldT;
+      la.sg_col = c->lr_sg_col.as<int32_t>();
+      la.sg_gcol = c->lr_sg_col.as<int32_t>() + c->lr_n_sg;
+      la.sg_R = c->lr_sg_R.as<int32_t>();
+      la.slot_gene = c->lr_slot_gene.as<int32_t>();
+      la.lcoef = c->lr_coef.as<__bf16>();
+      la.gscale = c->lr_gscale.as<double>();
+      la.gb = c->lr_gb.as<__bf16>();
+      la.n_sg = c->lr_n_sg;
+      la.p = c->p;
+      la.sg_per_xcd = lr_spx;
+      la.pair_per_xcd = lr_spx;
+      const int n_t128 = static_cast<int>((a.n_tiles + 1) / 2);
+      CHK(ensure(c->lr_fold, sizeof(float) * lr_fold_base(n_t128, 0, c->lr_n_sg)));
+      la.fold = c->lr_fold.as<float>();
+      const int64_t rounds = (c->lr_n_sg + 8 * lr_spx - 1) / (8 * lr_spx);
+      lr_grid = dim3(static_cast<unsigned>(rounds * 8 * lr_spx * n_splits));
+      const int64_t n_pairs = (c->lr_n_sg + 1) / 2;
+      const int64_t prounds = (n_pairs + 8 * lr_spx - 1) / (8 * lr_spx);
+      num_grid = dim3(static_cast<unsigned>(prounds * 8 * lr_spx * n_splits));
+      {
+        Timed t(c, "num");  // k_lr_num
+        CHK(launch_lr_num(c, la, num_grid));
+        CHK(t.done());
       }
+      {
+        Timed t(c, "wald");  // k_wald_lr (the interpolation)
+        CHK(launch_wald_lr(c, la, lr_grid));
+        CHK(t.done());
+      }
+    } else {
+      Timed t(c, "wald");  // the eight-wave kernel
+      CHK(launch_wald(c, a, dim3(static_cast<unsigned>(cis ? n_items : n_groups_pad * n_splits))));
       CHK(t.done());
     }
     if (lr) {

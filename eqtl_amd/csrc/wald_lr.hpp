@@ -21,26 +21,27 @@
 // block over the samples (num) and R node weights per quantity instead of
 // p + 1 blocks over the samples.
 //
-// One workgroup = one super-group (64 genes) x a strided set of 128-row SNP
-// tiles, eight waves: wave w = 4 c + r covers rows 32 r .. 32 r + 31 of a tile
-// against groups 2c, 2c + 1 of the super-group.
-//  * Main loop over n (32-sample k-steps): per group the block s_k o r_k
-//    against the rotated rows, both split bf16 (the rotation writes the rows'
-//    hi / lo planes), three products. The super-group's gene tile (4 groups x
-//    hi/lo, 8 KiB per k-step) is copied global -> LDS by the half-0 waves
-//    (double-buffered), the row pair's planes by the half-1 waves (a ring of
-//    three stages).
-//  * Interpolation: per quantity (p projections, then gg) and 32-node chunk,
-//    the row block's T or U values (written by the rotation as the exact
-//    three-way bf16 split hi + mid + lo of the fp32 value, read from global
-//    memory one quantity ahead) against the genes' weights l_r(u_k) split the
-//    same way: six bf16 products (errors ~2^-24 relative), FP32 accumulation
-//    over the nodes; a projection's square is added to the test's FP64 sum of
-//    squares, gg is kept. Segments of 32 nodes load the weights once.
-//  * Epilogue: all four groups' gg, sum of squares and num are folded into LDS
-//    (the staging buffers are free by then), then per group the FP64 test of
-//    the eight-wave LMM kernel (refinement, polish hand-off, hits, ACAT, best)
-//    through w8_epilogue.
+// Three kernels per block:
+//  * k_lr_num: num = (s_k o r_k)^T g~ over the samples. One workgroup = a
+//    pair of super-groups (128 genes) x a strided set of 128-row SNP tiles,
+//    eight waves: wave w = 4 c + r covers rows 32 r .. 32 r + 31 of a tile
+//    against the four groups of super-group c of the pair. Per 32-sample
+//    k-step the pair's gene tiles (2 x 8 KiB, s_k o r_k split bf16 hi / lo)
+//    and the tile's rows (hi / lo planes written by the rotation, 16 KiB) are
+//    copied global -> LDS into four-stage rings (four direct copies per wave
+//    and k-step), three bf16 products per MAC; num is stored to the fold.
+//    With 128 genes per workgroup a k-step moves 32 KiB for 192 MFMAs (one
+//    super-group: 24 KiB for 96).
+//  * k_wald_lr: the interpolation. One workgroup = one super-group x the
+//    same tiles; per quantity (p projections, then gg) and 32-node chunk the
+//    tile's T or U values (written by the rotation as the exact three-way
+//    bf16 split hi + mid + lo of the fp32 value) are staged through an LDS
+//    ring and contracted against the genes' weights l_r(u_k) split the same
+//    way: six bf16 products (errors ~2^-24 relative), FP32 accumulation over
+//    the nodes; a projection's square is added to the test's FP64 sum of
+//    squares, gg is kept. gg and S = gg - sum of squares go to the fold.
+//  * k_lr_tests: the FP64 tests of the eight-wave LMM kernel (refinement,
+//    polish hand-off, hits, ACAT, best) over the fold.
 #pragma once
 #include "wald_lmm.hpp"
 
@@ -62,6 +63,7 @@ struct LrArgs {
   const __bf16* gb;          // gene tiles, see lr_tile_index
   int32_t n_sg, p;
   int32_t sg_per_xcd;        // super-groups per XCD and round (the launch has n_splits splits of each)
+  int32_t pair_per_xcd;      // k_lr_num: super-group pairs per XCD and round (n_splits splits of each)
   float* fold;               // per (tile, super-group): gg, S, num planes of the 4 x 2048 tests, see lr_fold_base
 };
 
@@ -72,25 +74,24 @@ __host__ __device__ inline size_t lr_fold_base(int tile, int sg, int n_sg) {
   return (static_cast<size_t>(tile) * n_sg + sg) * 3 * 4 * kW8Tests;
 }
 
-// Both copy rings (gene tile, row planes) hold kLrStages k-steps and are
-// filled kLrStages - 1 k-steps ahead: a k-step's twelve MFMAs per wave are
-// short against the copy latency.
+// k_lr_num: both copy rings (the pair's gene tiles, the row planes) hold
+// kLrStages k-steps and are filled kLrStages - 1 k-steps ahead.
 constexpr int kLrStages = 4;
 constexpr size_t kLrTileBytes = 8 * 1024;  // one k-step of a super-group's gene tile
-constexpr size_t kLrStageRegion = kLrStages * (kLrTileBytes + kAStageBytes);
-// The interpolation's ring: kMixStages steps of 24 KiB (128 rows x 32 nodes x
-// three bf16 planes), inside the contraction's staging region.
+constexpr size_t kLrPairTileBytes = 2 * kLrTileBytes;
+constexpr size_t kLrNumLdsBytes = kLrStages * (kLrPairTileBytes + kAStageBytes);
+// k_wald_lr: the interpolation's ring of kMixStages steps of 24 KiB (128 rows
+// x 32 nodes x three bf16 planes); after it, per group and test gg and S
+// (fp32), copied out to the fold.
 constexpr int kMixStages = 4;
 constexpr size_t kMixStageBytes = 24 * 1024;
-static_assert(kMixStages * kMixStageBytes <= kLrStageRegion, "interpolation ring");
-// after the interpolation: per group and test gg, S and num (fp32), copied
-// out to the fold buffer
-constexpr size_t kLrFoldBytes = 3 * 4 * kW8Tests * sizeof(float);
-constexpr size_t kLrRegion0 = kLrStageRegion > kLrFoldBytes ? kLrStageRegion : kLrFoldBytes;
+constexpr size_t kLrMixFoldBytes = 2 * 4 * kW8Tests * sizeof(float);
+constexpr size_t kLrLdsBytes =
+    kMixStages * kMixStageBytes > kLrMixFoldBytes ? kMixStages * kMixStageBytes : kLrMixFoldBytes;
 // s_waitcnt immediates (gfx9 encoding): vmcnt <= v, expcnt at its maximum, lgkmcnt 0
 constexpr int lr_wait_vm(int v) { return (v & 0xF) | ((v >> 4) << 14) | (0x7 << 4); }
-constexpr size_t kLrLdsBytes = kLrRegion0;
-static_assert(kLrLdsBytes + 64 <= kLdsCap, "LR Wald LDS layout");
+static_assert(kLrLdsBytes + 64 <= kLdsCap, "LR interpolation LDS layout");
+static_assert(kLrNumLdsBytes + 64 <= kLdsCap, "LR num LDS layout");
 // the tests kernel: RowInfo, the tail head, the 64 genes' states, the eight
 // waves' partials
 constexpr size_t kLrTestsLdsBytes =
@@ -115,13 +116,15 @@ __host__ __device__ inline size_t lr_coef_index(int slot, int r, int plane) {
   return (static_cast<size_t>(slot >> 4) * 4 + (r >> 5)) * 1536 + plane * 512 + lane * 8 + (r & 7);
 }
 
-// One 32-sample k-step of wave (c, r): its row pair's hi / lo fragments from
-// the current stage (a_rd), the gene tile (b_cur); half 0 copies its share of
-// the gene tile kLrStages - 1 k-steps on (b_nxt), half 1 the row pair's planes
-// of that k-step (a_wr). The LDS pointers are __restrict__ so the compiler
-// does not drain the copies in flight before the reads.
+// One 32-sample k-step of k_lr_num's wave (c, r): its row pair's hi / lo
+// fragments from the current stage (a_rd) against the four gene groups of
+// super-group c of the pair (b_cur), 24 MFMAs; it issues its four copies of
+// the k-step kLrStages - 1 on (c = 0: gene-tile chunks 4 r .. 4 r + 3 of the
+// pair from b_src into b_nxt; c = 1: the row pair's planes into a_wr). The LDS
+// pointers are __restrict__ so the compiler does not drain the copies in
+// flight before the reads.
 __device__ __attribute__((always_inline)) inline void lr_kstep(
-    f32x4 (&an)[2][kRB], const unsigned char* __restrict__ a_rd, unsigned char* __restrict__ a_wr,
+    f32x4 (&an)[4][kRB], const unsigned char* __restrict__ a_rd, unsigned char* __restrict__ a_wr,
     const __bf16* const (&a_src)[kRB], int64_t lo_off, const unsigned char* __restrict__ b_cur,
     unsigned char* __restrict__ b_nxt, const unsigned char* b_src, int c, int r, int lane) {
   bf16x8 ah[kRB], al[kRB];
@@ -133,9 +136,9 @@ __device__ __attribute__((always_inline)) inline void lr_kstep(
   if (c == 0) {
     if (b_nxt) {
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        const int ch = r + 4 * k;
-        lds_direct16(b_src + ch * 1024 + lane * 16, b_nxt + ch * 1024);
+      for (int k = 0; k < 4; ++k) {
+        const int ch = 4 * r + k;
+        lds_direct16(b_src + (ch & 7) * 1024 + lane * 16, b_nxt + ch * 1024);
       }
     }
   } else if (a_wr) {
@@ -146,17 +149,17 @@ __device__ __attribute__((always_inline)) inline void lr_kstep(
     }
   }
   __builtin_amdgcn_sched_barrier(0);
+  const unsigned char* bt = b_cur + c * kLrTileBytes;
 #pragma unroll
-  for (int gq = 0; gq < 2; ++gq) {
-    const int q = 2 * c + gq;
-    const bf16x8 vh = *reinterpret_cast<const bf16x8*>(b_cur + q * 1024 + lane * 16);
-    const bf16x8 vl = *reinterpret_cast<const bf16x8*>(b_cur + 4096 + q * 1024 + lane * 16);
+  for (int q = 0; q < 4; ++q) {
+    const bf16x8 vh = *reinterpret_cast<const bf16x8*>(bt + q * 1024 + lane * 16);
+    const bf16x8 vl = *reinterpret_cast<const bf16x8*>(bt + 4096 + q * 1024 + lane * 16);
 #pragma unroll
-    for (int rb = 0; rb < kRB; ++rb) an[gq][rb] = mfma_bf16(ah[rb], vh, an[gq][rb]);
+    for (int rb = 0; rb < kRB; ++rb) an[q][rb] = mfma_bf16(ah[rb], vh, an[q][rb]);
 #pragma unroll
-    for (int rb = 0; rb < kRB; ++rb) an[gq][rb] = mfma_bf16(ah[rb], vl, an[gq][rb]);
+    for (int rb = 0; rb < kRB; ++rb) an[q][rb] = mfma_bf16(ah[rb], vl, an[q][rb]);
 #pragma unroll
-    for (int rb = 0; rb < kRB; ++rb) an[gq][rb] = mfma_bf16(al[rb], vh, an[gq][rb]);
+    for (int rb = 0; rb < kRB; ++rb) an[q][rb] = mfma_bf16(al[rb], vh, an[q][rb]);
   }
 }
 
@@ -248,56 +251,60 @@ __device__ __attribute__((always_inline)) inline void lr_mix(f32x4 (&pj)[2][kRB]
   }
 }
 
-__global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_wald_lr(LrArgs la) {
+// num of every test of a pair of super-groups: the contraction over the
+// samples, stored to the fold's num plane (fp32).
+__global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_lr_num(LrArgs la) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const WaldArgs& a = la.w;
   // XCD-aware: workgroup wg runs on XCD wg % 8. Per round each XCD takes
-  // sg_per_xcd consecutive super-groups x n_splits row splits; the workgroups
-  // of one split sweep the same SNP tiles in the same order (and those of one
-  // super-group read the same gene tiles), so both are served from the XCD's
-  // L2 instead of each super-group streaming the whole block.
+  // pair_per_xcd consecutive super-group pairs x n_splits row splits; the
+  // workgroups of one split sweep the same SNP tiles in the same order (and
+  // those of one pair read the same gene tiles), so both are served from the
+  // XCD's L2 instead of each pair streaming the whole block.
   const int wg = blockIdx.x, xcd = wg & 7, jx = wg >> 3;
-  const int per_xcd = la.sg_per_xcd * a.n_splits;
+  const int per_xcd = la.pair_per_xcd * a.n_splits;
   const int l = jx % per_xcd;
-  const int sg = ((jx / per_xcd) * 8 + xcd) * la.sg_per_xcd + l / a.n_splits;
+  const int pr = ((jx / per_xcd) * 8 + xcd) * la.pair_per_xcd + l / a.n_splits;
   const int split = l % a.n_splits;
-  if (sg >= la.n_sg) return;
+  if (2 * pr >= la.n_sg) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int c = wave >> 2, r = wave & 3;
+  const int c = wave >> 2, r = wave & 3;  // super-group 2 pr + c of the pair, row quarter r
+  const int sg = 2 * pr + c;
+  const bool sg_ok = sg < la.n_sg;  // an odd count leaves the last pair's second half empty
   const int gl = lane & 15, kk = 8 * (lane >> 4);
   const int nks = a.n_pad >> 5;
 
   unsigned char* const bbuf = smem;
-  unsigned char* const abuf = smem + kLrStages * kLrTileBytes;
-  // the per-gene scale of gg for this wave's two groups
-  const double gsc0 = la.gscale[(sg * 4 + 2 * c) * 16 + gl];
-  const double gsc1 = la.gscale[(sg * 4 + 2 * c + 1) * 16 + gl];
-  const unsigned char* gsrc = reinterpret_cast<const unsigned char*>(la.gb) + static_cast<size_t>(sg) * nks * kLrTileBytes;
+  unsigned char* const abuf = smem + kLrStages * kLrPairTileBytes;
+  // gene-tile chunks this wave copies (c = 0): 4 r .. 4 r + 3 of the pair's
+  // 16, all of super-group 2 pr + (r >> 1) (the first one again for an empty
+  // second half: its products are not stored)
+  const int sg_src = min(2 * pr + (r >> 1), la.n_sg - 1);
+  const unsigned char* gsrc =
+      reinterpret_cast<const unsigned char*>(la.gb) + static_cast<size_t>(sg_src) * nks * kLrTileBytes;
   const int pair_off = r * 4096;
-  const int segR = la.sg_R[sg], segc = la.sg_col[sg], segg = la.sg_gcol[sg], nqr = segR >> 5;
   const int n_tiles = (a.n_tiles + 1) >> 1;
-  const __bf16* lbase = la.lcoef + static_cast<size_t>((sg * 4 + 2 * c) * 4) * 1536 + lane * 8;
 
   for (int tile = split; tile < n_tiles; tile += a.n_splits) {
     const int64_t row0 = static_cast<int64_t>(tile) * kTileRows;
-    __syncthreads();  // the previous tile's fold has been copied out
+    __syncthreads();  // every wave is done with the previous tile's stages
     const int64_t rowP = row0 + 32 * r;
     const __bf16* a_row[kRB];
 #pragma unroll
     for (int rb = 0; rb < kRB; ++rb) a_row[rb] = la.Ghi + (rowP + 16 * rb + gl) * a.n_pad + kk;
-    f32x4 an[2][kRB];
+    f32x4 an[4][kRB];
 #pragma unroll
-    for (int gq = 0; gq < 2; ++gq)
+    for (int q = 0; q < 4; ++q)
 #pragma unroll
-      for (int rb = 0; rb < kRB; ++rb) an[gq][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
-    // prologue: the first kLrStages - 1 gene tiles (half 0) and row stages (half 1)
+      for (int rb = 0; rb < kRB; ++rb) an[q][rb] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // prologue: the first kLrStages - 1 k-steps, four copies per wave each
     for (int s = 0; s < kLrStages - 1 && s < nks; ++s) {
       if (c == 0) {
 #pragma unroll
-        for (int k = 0; k < 2; ++k) {
-          const int ch = r + 4 * k;
-          lds_direct16(gsrc + static_cast<size_t>(s) * kLrTileBytes + ch * 1024 + lane * 16,
-                       bbuf + s * kLrTileBytes + ch * 1024);
+        for (int k = 0; k < 4; ++k) {
+          const int ch = 4 * r + k;
+          lds_direct16(gsrc + static_cast<size_t>(s) * kLrTileBytes + (ch & 7) * 1024 + lane * 16,
+                       bbuf + s * kLrPairTileBytes + ch * 1024);
         }
       } else {
         unsigned char* dst = abuf + s * kAStageBytes + pair_off;
@@ -318,24 +325,63 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
       const bool fetch = sf < nks;
       unsigned char* const a_wr = fetch ? abuf + (sf % kLrStages) * kAStageBytes + pair_off : nullptr;
       const unsigned char* const a_rd = abuf + (s % kLrStages) * kAStageBytes + pair_off;
-      unsigned char* const b_nxt = fetch ? bbuf + (sf % kLrStages) * kLrTileBytes : nullptr;
-      lr_kstep(an, a_rd, a_wr, src, la.lo_off, bbuf + (s % kLrStages) * kLrTileBytes, b_nxt,
+      unsigned char* const b_nxt = fetch ? bbuf + (sf % kLrStages) * kLrPairTileBytes : nullptr;
+      lr_kstep(an, a_rd, a_wr, src, la.lo_off, bbuf + (s % kLrStages) * kLrPairTileBytes, b_nxt,
                gsrc + static_cast<size_t>(sf) * kLrTileBytes, c, r, lane);
-      // k-step s + 1's operands must have landed: of this wave's copies the
-      // younger k-steps' (two per k-step in half 0, four in half 1) may stay
-      // in flight
+      // k-step s + 1's operands must have landed: of this wave's copies (four
+      // per k-step) those of the younger k-steps may stay in flight
       const int ahead = min(nks - 1, s + kLrStages - 1) - (s + 1);
-      if (ahead >= 2) {
-        if (c == 0) __builtin_amdgcn_s_waitcnt(lr_wait_vm(4));
-        else __builtin_amdgcn_s_waitcnt(lr_wait_vm(8));
-      } else if (ahead == 1) {
-        if (c == 0) __builtin_amdgcn_s_waitcnt(lr_wait_vm(2));
-        else __builtin_amdgcn_s_waitcnt(lr_wait_vm(4));
-      } else {
-        __builtin_amdgcn_s_waitcnt(0);
-      }
+      if (ahead >= 2) __builtin_amdgcn_s_waitcnt(lr_wait_vm(8));
+      else if (ahead == 1) __builtin_amdgcn_s_waitcnt(lr_wait_vm(4));
+      else __builtin_amdgcn_s_waitcnt(0);
       lds_barrier();
     }
+    // num plane of (tile, sg): test (row, gene) of group q at q * 2048 + row * 16 + gene;
+    // streaming stores (read back only by k_lr_tests)
+    if (sg_ok) {
+      float* fnum = la.fold + lr_fold_base(tile, sg, la.n_sg) + 2 * 4 * kW8Tests;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int rb = 0; rb < kRB; ++rb)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            __builtin_nontemporal_store(an[q][rb][e],
+                                        fnum + q * kW8Tests + (32 * r + 16 * rb + 4 * (lane >> 4) + e) * 16 + gl);
+    }
+  }
+}
+
+// gg and S of every test of one super-group: the interpolation over the
+// segment's nodes, stored to the fold's gg and S planes.
+__global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_wald_lr(LrArgs la) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  const WaldArgs& a = la.w;
+  // XCD-aware: workgroup wg runs on XCD wg % 8. Per round each XCD takes
+  // sg_per_xcd consecutive super-groups x n_splits row splits; the workgroups
+  // of one split sweep the same SNP tiles in the same order (and those of one
+  // super-group read the same gene tiles), so both are served from the XCD's
+  // L2 instead of each super-group streaming the whole block.
+  const int wg = blockIdx.x, xcd = wg & 7, jx = wg >> 3;
+  const int per_xcd = la.sg_per_xcd * a.n_splits;
+  const int l = jx % per_xcd;
+  const int sg = ((jx / per_xcd) * 8 + xcd) * la.sg_per_xcd + l / a.n_splits;
+  const int split = l % a.n_splits;
+  if (sg >= la.n_sg) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int c = wave >> 2, r = wave & 3;
+  const int gl = lane & 15, kk = 8 * (lane >> 4);
+
+  // the per-gene scale of gg for this wave's two groups
+  const double gsc0 = la.gscale[(sg * 4 + 2 * c) * 16 + gl];
+  const double gsc1 = la.gscale[(sg * 4 + 2 * c + 1) * 16 + gl];
+  const int segR = la.sg_R[sg], segc = la.sg_col[sg], segg = la.sg_gcol[sg], nqr = segR >> 5;
+  const int n_tiles = (a.n_tiles + 1) >> 1;
+  const __bf16* lbase = la.lcoef + static_cast<size_t>((sg * 4 + 2 * c) * 4) * 1536 + lane * 8;
+
+  for (int tile = split; tile < n_tiles; tile += a.n_splits) {
+    const int64_t row0 = static_cast<int64_t>(tile) * kTileRows;
+    __syncthreads();  // the previous tile's fold has been copied out
     // interpolation: quantities 0 .. p-1 (projections, squared and summed),
     // then p (gg); each quantity's values are read one quantity ahead
     double ssq[2][kRB][4];
@@ -454,10 +500,9 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
       }
     }
     // fold: gg = d(delta, lbar) x the interpolated sum, S = gg - ||W^T h||^2
-    // (FP64), num; into LDS planes in the tests' order, then copied out whole
+    // (FP64); into LDS planes in the tests' order, then copied out whole
     float* fgg = reinterpret_cast<float*>(smem);
     float* fS = fgg + 4 * kW8Tests;
-    float* fnum = fS + 4 * kW8Tests;
     __syncthreads();  // the staging buffers are free (every wave is past the last step)
 #pragma unroll
     for (int gq = 0; gq < 2; ++gq) {
@@ -471,7 +516,6 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
           const double gg = gsc * ggv[gq][rb][e];
           fgg[t] = static_cast<float>(gg);
           fS[t] = static_cast<float>(gg - ssq[gq][rb][e]);
-          fnum[t] = an[gq][rb][e];
         }
     }
     __syncthreads();
@@ -481,7 +525,7 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
 #pragma unroll 4
       // streaming stores: the fold is read back only by k_lr_tests, so it
       // should not displace the rows and node values this kernel re-reads
-      for (int i = threadIdx.x; i < static_cast<int>(kLrFoldBytes / 16); i += kW8Threads) {
+      for (int i = threadIdx.x; i < static_cast<int>(kLrMixFoldBytes / 16); i += kW8Threads) {
         const float4 v = src[i];
         __builtin_nontemporal_store(v.x, &dst[i].x);
         __builtin_nontemporal_store(v.y, &dst[i].y);

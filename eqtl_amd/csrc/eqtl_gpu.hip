@@ -2046,7 +2046,7 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
       hipLaunchKernelGGL(k_rotate_lds<false>, g2, dim3(256), kRotLds, c->stream, c->A32.as<float>(),
                          c->Qhi.as<__bf16>(), c->Qlo.as<__bf16>(), c->Gt.as<float>(), np, np,
                          static_cast<int64_t>(np), ghi, ghi ? ghi + static_cast<size_t>(m_pad) * np : nullptr,
-                         nullptr);
+                         nullptr, 0);
       if (lr_rows) {
         // the SNPs' covariate projections at the interpolation nodes, A x (Q Z),
         // then their weighted sums of squares, (Q^T g)^2 x D(nodes): written
@@ -2057,11 +2057,11 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
         const dim3 g3(static_cast<unsigned>((c->lr_ncols + kRotT - 1) / kRotT), static_cast<unsigned>(m_pad / kRotT));
         hipLaunchKernelGGL(k_rotate_lds<false>, g3, dim3(256), kRotLds, c->stream, c->A32.as<float>(),
                            c->lr_QZhi.as<__bf16>(), c->lr_QZlo.as<__bf16>(), nullptr, np, c->lr_ncols, c->lr_ldT, th,
-                           th + 2 * tpl, th + tpl);
+                           th + 2 * tpl, th + tpl, 0);
         const dim3 g4(static_cast<unsigned>((c->lr_nnodes + kRotT - 1) / kRotT), static_cast<unsigned>(m_pad / kRotT));
         hipLaunchKernelGGL(k_rotate_lds<true>, g4, dim3(256), kRotLds, c->stream, c->Gt.as<float>(),
-                           c->lr_Dhi.as<__bf16>(), c->lr_Dlo.as<__bf16>(), nullptr, np, c->lr_nnodes, c->lr_ldT,
-                           th + c->lr_ncols, th + 2 * tpl + c->lr_ncols, th + tpl + c->lr_ncols);
+                           c->lr_Dhi.as<__bf16>(), c->lr_Dlo.as<__bf16>(), nullptr, np, c->lr_nnodes, c->lr_ldT, th,
+                           th + 2 * tpl, th + tpl, c->lr_ncols);
       }
     }
     CHK(check_launch("k_rotate"));
@@ -2204,15 +2204,16 @@ int eqtl_gpu_scan_block(eqtl_gpu_ctx* c, const uint8_t* rows, int32_t on_dev, in
       la.n_sg = c->lr_n_sg;
       la.p = c->p;
       la.sg_per_xcd = lr_spx;
-      la.pair_per_xcd = lr_spx;
+      const int lr_ppx = lr_spx;  // k_lr_num: pairs per XCD and round, with the same n_splits
+      la.pair_per_xcd = lr_ppx;
       const int n_t128 = static_cast<int>((a.n_tiles + 1) / 2);
       CHK(ensure(c->lr_fold, sizeof(float) * lr_fold_base(n_t128, 0, c->lr_n_sg)));
       la.fold = c->lr_fold.as<float>();
       const int64_t rounds = (c->lr_n_sg + 8 * lr_spx - 1) / (8 * lr_spx);
       lr_grid = dim3(static_cast<unsigned>(rounds * 8 * lr_spx * n_splits));
       const int64_t n_pairs = (c->lr_n_sg + 1) / 2;
-      const int64_t prounds = (n_pairs + 8 * lr_spx - 1) / (8 * lr_spx);
-      num_grid = dim3(static_cast<unsigned>(prounds * 8 * lr_spx * n_splits));
+      const int64_t prounds = (n_pairs + 8 * lr_ppx - 1) / (8 * lr_ppx);
+      num_grid = dim3(static_cast<unsigned>(prounds * 8 * lr_ppx * n_splits));
       {
         Timed t(c, "num");  // k_lr_num
         CHK(launch_lr_num(c, la, num_grid));

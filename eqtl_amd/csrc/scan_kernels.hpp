@@ -228,16 +228,28 @@ constexpr size_t kRotA = 8 * kRotT * 16;       // FP32 rows: 16 KiB per step
 constexpr size_t kRotStage = kRotA + 8 * kRotT * 16;  // + Qhi and Qlo: 32 KiB
 constexpr size_t kRotLds = 2 * kRotStage;
 
+// Element (row, col) of a bf16 plane in fragment-major order: per 16-row block
+// and 32-column chunk one contiguous 1 KiB block of 64 x 16 bytes, lane l
+// holding row (l & 15), columns 8 (l >> 4) .. + 7 of the chunk — the operand
+// fragment of v_mfma_f32_16x16x32_bf16, so a 64-lane direct copy of 16 bytes
+// per lane moves one whole block. nch = chunks per row block.
+__host__ __device__ inline int64_t frag_index(int64_t row, int64_t col, int64_t nch) {
+  return ((row >> 4) * nch + (col >> 5)) * 512 + ((row & 15) + 16 * ((col & 31) >> 3)) * 8 + (col & 7);
+}
+
 // Also used for any B of n_cols coordinate-major columns (output leading
 // dimension ld_out); the rotation is n_cols = ld_out = n_pad. SQUARE: the A
 // values are squared before the split (sum_i A_ji^2 B_ic). Outputs (each
-// nullable, leading dimension ld_out): Gt fp32; Ohi / Olo the bf16 hi / lo
-// split of the fp32 value; with Omid, the exact three-way split hi / mid / lo.
+// nullable): Gt fp32 (row-major, leading dimension ld_out); Ohi / Olo the
+// bf16 hi / lo split of the fp32 value, with Omid the exact three-way split
+// hi / mid / lo, in fragment-major order (frag_index, ld_out / 32 chunks per
+// row block) at columns frag_col0 + col.
 template <bool SQUARE>
 __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A, const __bf16* __restrict__ Qhi,
                                                     const __bf16* __restrict__ Qlo, float* __restrict__ Gt,
                                                     int n_pad, int n_cols, int64_t ld_out, __bf16* __restrict__ Ohi,
-                                                    __bf16* __restrict__ Olo, __bf16* __restrict__ Omid) {
+                                                    __bf16* __restrict__ Olo, __bf16* __restrict__ Omid,
+                                                    int frag_col0) {
   extern __shared__ __attribute__((aligned(16))) unsigned char rsm[];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = wave >> 1, wc = wave & 1;
@@ -325,9 +337,9 @@ __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A,
       for (int r = 0; r < 4; ++r) {
         const int64_t row = R0 + wr * 64 + rb * 16 + 4 * (lane >> 4) + r;
         const float v = acc[rb][cb][r];
-        const int64_t o = row * ld_out + col;
-        if (Gt) Gt[o] = v;
+        if (Gt) Gt[row * ld_out + col] = v;
         if (Ohi) {
+          const int64_t o = frag_index(row, frag_col0 + col, ld_out >> 5);
           const __bf16 h = to_bf16(v);
           const float r1 = v - static_cast<float>(h);
           Ohi[o] = h;

@@ -49,10 +49,10 @@ namespace eqtl_amd {
 
 struct LrArgs {
   WaldArgs w;
-  const __bf16* Ghi;         // rotated rows, bf16 hi plane (m_pad x n_pad); lo plane at Ghi + lo_off
-  int64_t lo_off;
-  const __bf16* Tp;          // m_pad x ldT node projections (T) and node sums of squares (U): hi plane,
-  int64_t tpl;               //   then the mid plane at Tp + tpl and the lo plane at Tp + 2 tpl
+  const __bf16* Ghi;         // rotated rows, bf16 hi plane (m_pad x n_pad, fragment-major: frag_index);
+  int64_t lo_off;            //   lo plane at Ghi + lo_off
+  const __bf16* Tp;          // m_pad x ldT node projections (T) and node sums of squares (U), fragment-major:
+  int64_t tpl;               //   hi plane, then the mid plane at Tp + tpl and the lo plane at Tp + 2 tpl
   int64_t ldT;
   const int32_t* sg_col;     // per super-group: first T column of its segment
   const int32_t* sg_gcol;    // per super-group: first U column of its segment
@@ -90,6 +90,16 @@ constexpr size_t kLrLdsBytes =
     kMixStages * kMixStageBytes > kLrMixFoldBytes ? kMixStages * kMixStageBytes : kLrMixFoldBytes;
 // s_waitcnt immediates (gfx9 encoding): vmcnt <= v, expcnt at its maximum, lgkmcnt 0
 constexpr int lr_wait_vm(int v) { return (v & 0xF) | ((v >> 4) << 14) | (0x7 << 4); }
+// k_wald_lr's ring: waits until at most the three copies of each of the
+// `ahead` younger steps (0 <= ahead <= kMixStages - 2) are in flight
+__device__ __attribute__((always_inline)) inline void lr_wait3(int ahead) {
+  static_assert(kMixStages <= 6, "lr_wait3 covers up to four younger steps");
+  if (ahead >= 4) __builtin_amdgcn_s_waitcnt(lr_wait_vm(12));
+  else if (ahead == 3) __builtin_amdgcn_s_waitcnt(lr_wait_vm(9));
+  else if (ahead == 2) __builtin_amdgcn_s_waitcnt(lr_wait_vm(6));
+  else if (ahead == 1) __builtin_amdgcn_s_waitcnt(lr_wait_vm(3));
+  else __builtin_amdgcn_s_waitcnt(0);
+}
 static_assert(kLrLdsBytes + 64 <= kLdsCap, "LR interpolation LDS layout");
 static_assert(kLrNumLdsBytes + 64 <= kLdsCap, "LR num LDS layout");
 // the tests kernel: RowInfo, the tail head, the 64 genes' states, the eight
@@ -271,7 +281,7 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
   const int c = wave >> 2, r = wave & 3;  // super-group 2 pr + c of the pair, row quarter r
   const int sg = 2 * pr + c;
   const bool sg_ok = sg < la.n_sg;  // an odd count leaves the last pair's second half empty
-  const int gl = lane & 15, kk = 8 * (lane >> 4);
+  const int gl = lane & 15;
   const int nks = a.n_pad >> 5;
 
   unsigned char* const bbuf = smem;
@@ -288,10 +298,12 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
   for (int tile = split; tile < n_tiles; tile += a.n_splits) {
     const int64_t row0 = static_cast<int64_t>(tile) * kTileRows;
     __syncthreads();  // every wave is done with the previous tile's stages
-    const int64_t rowP = row0 + 32 * r;
+    // this wave's two 16-row blocks in the fragment-major planes (one 1 KiB
+    // block per k-step; the lo plane lo_off further on)
     const __bf16* a_row[kRB];
 #pragma unroll
-    for (int rb = 0; rb < kRB; ++rb) a_row[rb] = la.Ghi + (rowP + 16 * rb + gl) * a.n_pad + kk;
+    for (int rb = 0; rb < kRB; ++rb)
+      a_row[rb] = la.Ghi + ((row0 >> 4) + 2 * r + rb) * static_cast<int64_t>(nks) * 512 + lane * 8;
     f32x4 an[4][kRB];
 #pragma unroll
     for (int q = 0; q < 4; ++q)
@@ -310,8 +322,8 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
         unsigned char* dst = abuf + s * kAStageBytes + pair_off;
 #pragma unroll
         for (int rb = 0; rb < kRB; ++rb) {
-          lds_direct16(a_row[rb] + s * 32, dst + rb * 2048);
-          lds_direct16(a_row[rb] + s * 32 + la.lo_off, dst + rb * 2048 + 1024);
+          lds_direct16(a_row[rb] + s * 512, dst + rb * 2048);
+          lds_direct16(a_row[rb] + s * 512 + la.lo_off, dst + rb * 2048 + 1024);
         }
       }
     }
@@ -321,7 +333,7 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
       const int sf = s + kLrStages - 1;  // k-step whose operands are fetched now
       const __bf16* src[kRB];
 #pragma unroll
-      for (int rb = 0; rb < kRB; ++rb) src[rb] = a_row[rb] + sf * 32;
+      for (int rb = 0; rb < kRB; ++rb) src[rb] = a_row[rb] + sf * 512;
       const bool fetch = sf < nks;
       unsigned char* const a_wr = fetch ? abuf + (sf % kLrStages) * kAStageBytes + pair_off : nullptr;
       const unsigned char* const a_rd = abuf + (s % kLrStages) * kAStageBytes + pair_off;
@@ -370,7 +382,7 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
   if (sg >= la.n_sg) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int c = wave >> 2, r = wave & 3;
-  const int gl = lane & 15, kk = 8 * (lane >> 4);
+  const int gl = lane & 15;
 
   // the per-gene scale of gg for this wave's two groups
   const double gsc0 = la.gscale[(sg * 4 + 2 * c) * 16 + gl];
@@ -425,12 +437,13 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
     const int ch0 = 3 * wave;
     auto issue = [&](int st) {
       const int ca = st / nqr, qr = st - ca * nqr;
-      const int64_t col = qcol(ca, qr) + kk;
+      const int64_t chunk = qcol(ca, qr) >> 5;
       unsigned char* dst = smem + (st % kMixStages) * kMixStageBytes;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const int ch = ch0 + k, rr = ch / 6, rb = (ch / 3) & 1, plane = ch % 3;
-        const __bf16* src = la.Tp + plane * la.tpl + (row0 + 32 * rr + 16 * rb + gl) * la.ldT + col;
+        const __bf16* src =
+            la.Tp + plane * la.tpl + (((row0 >> 4) + 2 * rr + rb) * (la.ldT >> 5) + chunk) * 512 + lane * 8;
         lds_direct16(src, dst + ch * 1024);
       }
     };
@@ -456,10 +469,7 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
       for (int st = 0; st < nsteps; ++st) {
         // step st's chunks have landed: the copies of the up to two later
         // steps already issued may stay in flight
-        const int ahead = min(nsteps - 1, st + kMixStages - 2) - st;
-        if (ahead >= 2) __builtin_amdgcn_s_waitcnt(lr_wait_vm(6));
-        else if (ahead == 1) __builtin_amdgcn_s_waitcnt(lr_wait_vm(3));
-        else __builtin_amdgcn_s_waitcnt(0);
+        lr_wait3(min(nsteps - 1, st + kMixStages - 2) - st);
         lds_barrier();  // every wave's chunks of step st are in; step st - 1's slot is free
         if (st + kMixStages - 1 < nsteps) issue(st + kMixStages - 1);
         __builtin_amdgcn_sched_barrier(0);
@@ -476,10 +486,7 @@ __global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2
       for (int st = 0; st < nsteps; ++st) {
         const int ca = st / nqr, qr = st - ca * nqr;
         if (st == 0) {
-          const int ahead = min(nsteps - 1, kMixStages - 2);
-          if (ahead >= 2) __builtin_amdgcn_s_waitcnt(lr_wait_vm(6));
-          else if (ahead == 1) __builtin_amdgcn_s_waitcnt(lr_wait_vm(3));
-          else __builtin_amdgcn_s_waitcnt(0);
+          lr_wait3(min(nsteps - 1, kMixStages - 2));
         } else if (st + kMixStages - 2 < nsteps) {
           __builtin_amdgcn_s_waitcnt(lr_wait_vm(3));
         } else {

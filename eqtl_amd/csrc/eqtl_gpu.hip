@@ -732,14 +732,20 @@ static int upload_basis(eqtl_gpu_ctx* c, int n) {
   HIPC(hipMemcpyAsync(Q.data(), c->Q64.p, sizeof(double) * Q.size(), hipMemcpyDeviceToHost, c->stream));
   HIPC(hipStreamSynchronize(c->stream));
   std::vector<float> q32(static_cast<size_t>(np) * np, 0.f);
-  std::vector<__bf16> qh(static_cast<size_t>(np) * np), ql(static_cast<size_t>(np) * np);
   for (int cc = 0; cc < n; ++cc)
     for (int i = 0; i < n; ++i) q32[static_cast<size_t>(cc) * np + i] = static_cast<float>(Q[static_cast<size_t>(cc) * n + i]);
-  for (size_t t = 0; t < q32.size(); ++t) {
-    const __bf16 h = static_cast<__bf16>(q32[t]);
-    qh[t] = h;
-    ql[t] = static_cast<__bf16>(q32[t] - static_cast<float>(h));
-  }
+  // the split planes in the rotation kernel's fragment-major B order
+  const size_t qb = static_cast<size_t>(rot_b_coords(np)) * np;
+  const __bf16 z16 = static_cast<__bf16>(0.f);
+  std::vector<__bf16> qh(qb, z16), ql(qb, z16);
+  for (int cc = 0; cc < np; ++cc)
+    for (int i = 0; i < np; ++i) {
+      const float f = q32[static_cast<size_t>(cc) * np + i];
+      const __bf16 h = static_cast<__bf16>(f);
+      const int64_t o = rot_b_index(cc, i, np / 32);
+      qh[o] = h;
+      ql[o] = static_cast<__bf16>(f - static_cast<float>(h));
+    }
   CHK(ensure(c->Q32, sizeof(float) * q32.size()));
   CHK(ensure(c->Qhi, sizeof(__bf16) * qh.size()));
   CHK(ensure(c->Qlo, sizeof(__bf16) * ql.size()));
@@ -1741,11 +1747,12 @@ static int lr_build(eqtl_gpu_ctx* c) {
   if (rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, n, ncols, n, &one, c->Q64.as<double>(), n,
                     Z.as<double>(), np, &zero, QZ.as<double>(), np) != rocblas_status_success)
     return fail(EQTL_E_HIP, "rocblas_dgemm failed");
-  CHK(ensure(c->lr_QZhi, sizeof(__bf16) * zc));
-  CHK(ensure(c->lr_QZlo, sizeof(__bf16) * zc));
-  hipLaunchKernelGGL(k_split_bf16_f64, dim3(1024), dim3(256), 0, c->stream, QZ.as<double>(), static_cast<int64_t>(zc),
+  const size_t zb = static_cast<size_t>(rot_b_coords(ncols)) * np;
+  CHK(ensure(c->lr_QZhi, sizeof(__bf16) * zb));
+  CHK(ensure(c->lr_QZlo, sizeof(__bf16) * zb));
+  hipLaunchKernelGGL(k_split_rot_b, dim3(1024), dim3(256), 0, c->stream, QZ.as<double>(), np, ncols,
                      c->lr_QZhi.as<__bf16>(), c->lr_QZlo.as<__bf16>());
-  CHK(check_launch("k_split_bf16_f64"));
+  CHK(check_launch("k_split_rot_b"));
   // node weights d(delta_r, lam_c) / d(delta_r, lbar) per rotated coordinate,
   // coordinate-major per node (the B operand of the squared-row product),
   // split; per slot the scale d(delta_k, lbar) of the interpolated gg
@@ -1758,7 +1765,8 @@ static int lr_build(eqtl_gpu_ctx* c) {
     for (int i = 0; i < n; ++i) lbar += lam[i];
     lbar = lbar > 0 ? lbar / n : 1.0;
     for (int s2 = 0; s2 < G; ++s2) gsc[s2] = 1.0 / (dl[slot_gene[s2]] * lbar + 1.0);
-    const size_t dn = static_cast<size_t>(nnodes_total) * np;
+    // fragment-major B order of the rotation kernel (zero past the nodes and past n)
+    const size_t dn = static_cast<size_t>(rot_b_coords(nnodes_total)) * np;
     std::vector<__bf16> dh(dn, static_cast<__bf16>(0.f)), dlo(dn, static_cast<__bf16>(0.f));
     host_parallel(nnodes, [&](int k_lo, int k_hi) {
       for (int k = k_lo; k < k_hi; ++k)
@@ -1767,8 +1775,9 @@ static int lr_build(eqtl_gpu_ctx* c) {
           if (v < 1e-12) v = 1e-12;
           const float f = static_cast<float>((nd[k] * lbar + 1.0) / v);
           const __bf16 h = static_cast<__bf16>(f);
-          dh[static_cast<size_t>(k) * np + i] = h;
-          dlo[static_cast<size_t>(k) * np + i] = static_cast<__bf16>(f - static_cast<float>(h));
+          const int64_t o = rot_b_index(k, i, np / 32);
+          dh[o] = h;
+          dlo[o] = static_cast<__bf16>(f - static_cast<float>(h));
         }
     }, 16);
     CHK(ensure(c->lr_Dhi, sizeof(__bf16) * dn));

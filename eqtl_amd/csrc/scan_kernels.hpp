@@ -216,16 +216,28 @@ __global__ __launch_bounds__(256) void k_rotate_f32(const float* __restrict__ A,
 
 // K6, split form, staged through LDS: 128 SNP rows x 128 coordinates per
 // workgroup. Per 32-sample step the rows' FP32 values (8 chunks of 16 B per
-// row) and the coordinates' Qhi / Qlo (4 chunks each) are copied global -> LDS
-// with direct 16-byte loads, double-buffered and stored chunk-major (slot
-// (c, r) at (c * 128 + r) * 16), so the 16 lanes of an MFMA fragment read 16
-// consecutive slots. Wave (wr, wc) owns the 64 x 64 quadrant (rows wr*64..,
-// coordinates wc*64..) as 4 x 4 blocks, three bf16 products per block
-// (hi*hi, hi*lo, lo*hi, in that order). Coordinates past n_pad
-// are read clamped and not stored.
+// row) and the coordinates' Bhi / Blo (4 chunks each) are copied global -> LDS
+// with direct 16-byte loads, double-buffered. Each copy instruction reads
+// whole cache lines: a row copy takes 8 rows x 128 B (lane l: row l >> 3,
+// slot l & 7 holding chunk (l & 7) ^ ((row >> 1) & 7), so the 16 rows an MFMA
+// fragment reads lie in distinct LDS banks), a coordinate copy one 1 KiB block
+// of the fragment-major B planes (rot_b_index; stored chunk-major in LDS, slot
+// (c, r) at (c * 128 + r) * 16). Wave (wr, wc) owns the 64 x 64 quadrant
+// (rows wr*64.., coordinates wc*64..) as 4 x 4 blocks, three bf16 products
+// per block (hi*hi, hi*lo, lo*hi, in that order). The B planes hold whole
+// 128-coordinate blocks (zero past n_cols); coordinates past n_cols are not
+// stored.
 constexpr int kRotT = 128;
+// Element (coordinate, sample k) of the rotation's B planes: per 64-coordinate
+// block and 32-sample step one contiguous 4 KiB block, chunk (k & 31) >> 3
+// then coordinate then k & 7 (nks = samples / 32).
+__host__ __device__ inline int64_t rot_b_index(int64_t coord, int64_t k, int64_t nks) {
+  return ((coord >> 6) * nks + (k >> 5)) * 2048 + ((k & 31) >> 3) * 512 + (coord & 63) * 8 + (k & 7);
+}
+// coordinates the B planes hold for n_cols columns
+__host__ __device__ inline int64_t rot_b_coords(int64_t n_cols) { return (n_cols + kRotT - 1) / kRotT * kRotT; }
 constexpr size_t kRotA = 8 * kRotT * 16;       // FP32 rows: 16 KiB per step
-constexpr size_t kRotStage = kRotA + 8 * kRotT * 16;  // + Qhi and Qlo: 32 KiB
+constexpr size_t kRotStage = kRotA + 8 * kRotT * 16;  // + Bhi and Blo: 32 KiB
 constexpr size_t kRotLds = 2 * kRotStage;
 
 // Element (row, col) of a bf16 plane in fragment-major order: per 16-row block
@@ -245,8 +257,8 @@ __host__ __device__ inline int64_t frag_index(int64_t row, int64_t col, int64_t 
 // hi / mid / lo, in fragment-major order (frag_index, ld_out / 32 chunks per
 // row block) at columns frag_col0 + col.
 template <bool SQUARE>
-__global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A, const __bf16* __restrict__ Qhi,
-                                                    const __bf16* __restrict__ Qlo, float* __restrict__ Gt,
+__global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A, const __bf16* __restrict__ Bhi,
+                                                    const __bf16* __restrict__ Blo, float* __restrict__ Gt,
                                                     int n_pad, int n_cols, int64_t ld_out, __bf16* __restrict__ Ohi,
                                                     __bf16* __restrict__ Olo, __bf16* __restrict__ Omid,
                                                     int frag_col0) {
@@ -256,7 +268,9 @@ __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A,
   const int64_t R0 = static_cast<int64_t>(blockIdx.y) * kRotT;
   const int C0 = blockIdx.x * kRotT;
   // this wave's 8 of the step's 32 copy instructions (q = wave + 4 i):
-  // q < 16 rows of A (chunk q >> 1), then Qhi, then Qlo (chunks (q & 7) >> 1), half h = q & 1
+  // q < 16 rows 8q .. 8q + 7 of A, then Bhi, then Blo (chunk (q & 7) >> 1 of
+  // the coordinates' half h = q & 1)
+  const int nks = n_pad / 32;
   const unsigned char* src[8];
   int step_bytes[8], dst_off[8];
 #pragma unroll
@@ -264,16 +278,16 @@ __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A,
     const int q = wave + 4 * i;
     const int h = q & 1;
     if (q < 16) {
-      const int c = q >> 1;
-      src[i] = reinterpret_cast<const unsigned char*>(A + (R0 + h * 64 + lane) * n_pad + 4 * c);
+      const int row = 8 * q + (lane >> 3);
+      const int c = (lane & 7) ^ ((row >> 1) & 7);
+      src[i] = reinterpret_cast<const unsigned char*>(A + (R0 + row) * n_pad + 4 * c);
       step_bytes[i] = 32 * static_cast<int>(sizeof(float));
-      dst_off[i] = (c * kRotT + h * 64) * 16;
+      dst_off[i] = q * 1024;
     } else {
       const int c = (q & 7) >> 1;
-      const int coord = min(C0 + h * 64 + lane, n_cols - 1);
-      const __bf16* base = q < 24 ? Qhi : Qlo;
-      src[i] = reinterpret_cast<const unsigned char*>(base + static_cast<size_t>(coord) * n_pad + 8 * c);
-      step_bytes[i] = 32 * static_cast<int>(sizeof(__bf16));
+      const __bf16* base = q < 24 ? Bhi : Blo;
+      src[i] = reinterpret_cast<const unsigned char*>(base + rot_b_index(C0 + h * 64, 0, nks) + c * 512 + lane * 8);
+      step_bytes[i] = 2048 * static_cast<int>(sizeof(__bf16));
       dst_off[i] = static_cast<int>(kRotA) + (q < 24 ? 0 : 4 * kRotT * 16) + (c * kRotT + h * 64) * 16;
     }
   }
@@ -288,7 +302,6 @@ __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A,
   for (int rb = 0; rb < 4; ++rb)
 #pragma unroll
     for (int cb = 0; cb < 4; ++cb) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const int nks = n_pad / 32;
   issue(0, rsm);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
@@ -306,8 +319,9 @@ __global__ __launch_bounds__(256) void k_rotate_lds(const float* __restrict__ A,
 #pragma unroll
     for (int rb = 0; rb < 4; ++rb) {
       const int ra = wr * 64 + rb * 16 + (lane & 15);
-      const float4 lo = *reinterpret_cast<const float4*>(cur + ((2 * g) * kRotT + ra) * 16);
-      const float4 hi = *reinterpret_cast<const float4*>(cur + ((2 * g + 1) * kRotT + ra) * 16);
+      const int sw = (ra >> 1) & 7;
+      const float4 lo = *reinterpret_cast<const float4*>(cur + (ra * 8 + ((2 * g) ^ sw)) * 16);
+      const float4 hi = *reinterpret_cast<const float4*>(cur + (ra * 8 + ((2 * g + 1) ^ sw)) * 16);
       f32x8 av;
       av.v[0] = lo.x; av.v[1] = lo.y; av.v[2] = lo.z; av.v[3] = lo.w;
       av.v[4] = hi.x; av.v[5] = hi.y; av.v[6] = hi.z; av.v[7] = hi.w;

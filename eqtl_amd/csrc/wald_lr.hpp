@@ -189,14 +189,19 @@ __global__ __launch_bounds__(256) void k_lr_gene_tiles(const double* u64, const 
   }
 }
 
-// fp64 column-major (leading dimension ld) -> fp32 -> bf16 hi / lo, same layout
-__global__ void k_split_bf16_f64(const double* x, int64_t count, __bf16* hi, __bf16* lo) {
+// fp64 columns (leading dimension np, ncols of them) -> fp32 -> bf16 hi / lo
+// in the rotation kernel's fragment-major B order (rot_b_index), zero for the
+// padding columns up to rot_b_coords(ncols)
+__global__ void k_split_rot_b(const double* x, int np, int ncols, __bf16* hi, __bf16* lo) {
+  const int64_t count = rot_b_coords(ncols) * np;
   for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < count;
        t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
-    const float f = static_cast<float>(x[t]);
+    const int64_t col = t / np, i = t - col * np;
+    const float f = col < ncols ? static_cast<float>(x[t]) : 0.f;
     const __bf16 h = to_bf16(f);
-    hi[t] = h;
-    lo[t] = to_bf16(f - static_cast<float>(h));
+    const int64_t o = rot_b_index(col, i, np / 32);
+    hi[o] = h;
+    lo[o] = to_bf16(f - static_cast<float>(h));
   }
 }
 

@@ -292,22 +292,24 @@ def reference_calibration(Q, lam, X, y, dos):
                       f"sources (oracle/_ref), one thread each"}
 
 
-def measured_traffic(block, name="wald_lr_trans_lmm_n2000_g20000_pmc.md"):
-    """HBM bytes per launch of the dominant Wald kernel (this workload) from the
-    committed rocprofv3 PMC summary: FETCH_SIZE and WRITE_SIZE runs, each
-    scaled by the same counter on a 1 GiB device copy. None when the summary
-    was taken at another block size or is absent."""
+def measured_traffic(block, kernel, name="lr_kernels_trans_lmm_n2000_g20000_pmc.md"):
+    """HBM bytes per launch of one kernel of this workload from the committed
+    rocprofv3 PMC summary (line 'traffic_bytes_per_launch.<kernel>: B'):
+    FETCH_SIZE and WRITE_SIZE runs, each scaled by the same counter on a 1 GiB
+    device copy. None when the summary was taken at another block size or
+    does not cover the kernel."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     vals = {}
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:") or line.startswith("snps_per_launch:"):
+        if line.startswith("traffic_bytes_per_launch.") or line.startswith("snps_per_launch:"):
             k, v = line.split(":", 1)
             vals[k.strip()] = float(v.strip())
-    if vals.get("snps_per_launch") != block or "traffic_bytes_per_launch" not in vals:
+    key = "traffic_bytes_per_launch." + kernel
+    if vals.get("snps_per_launch") != block or key not in vals:
         return None
-    return vals["traffic_bytes_per_launch"]
+    return vals[key]
 
 
 VCF_N, VCF_M = 2000, 32768
@@ -1025,24 +1027,37 @@ def main():
         f"rotate {rot_ms / args.steps:.0f} ms, decode {dec_ms / args.steps:.0f} ms, reml {reml_ms / args.steps:.0f} ms, "
         f"basis {basis_ms / args.steps:.0f} ms")
     ctx.set_timing(False)
-    # Algorithmic work of the contraction kernel per test. Interpolated form
-    # (k_wald_lr; its FP64 tests run in k_lr_tests, timed apart): 2n FLOP for
-    # num = (s o r)^T g~ and 2 R (p + 1) for the p covariate projections and gg
-    # interpolated from R nodes (R averaged over the genes). Eight-wave form
-    # (k_wald_lmm, tests included): 2n(p+2) (num, gg and the p projections
-    # contracted over the samples).
+    # Algorithmic work per test of each kernel, over its event-timed launches.
+    # Interpolated form, three kernels per block:
+    #   k_lr_num   num = (s o r)^T g~ over the samples: 2n FLOP (matrix cores)
+    #   k_wald_lr  the p covariate projections and gg interpolated from R
+    #              nodes (R averaged over the genes): 2R(p+1) FLOP (matrix cores)
+    #   k_lr_tests the FP64 tests over the fold: reads gg, S and num, 12 bytes
+    # Eight-wave form (k_wald_lmm, tests included): 2n(p+2) FLOP (num, gg and
+    # the p projections contracted over the samples).
     p = X.shape[1]
     lr_on, lr_sg, lr_seg, lr_nodes = ctx.lr_plan()
     dense_flop = 2.0 * N * (p + 2)
-    flop_per_test = 2.0 * N + 2.0 * lr_nodes * (p + 1) if lr_on else dense_flop
-    wald_launch_ms = wald_ms / max(wald_n, 1)
-    tests_per_launch = tests / max(wald_n, 1)
-    achieved = flop_per_test * tests_per_launch / (wald_launch_ms * 1e-3) / 1e12
-    extra = {"wald_plan": {"kernel": "k_wald_lr" if lr_on else "k_wald_lmm", "super_groups": lr_sg,
-                           "segments": lr_seg, "mean_nodes": lr_nodes, "flop_per_test": flop_per_test,
+    n_launch = max(wald_n, 1)
+    tests_per_launch = tests / n_launch
+    if lr_on:
+        kern = {"k_lr_num": {"ms": num_ms / max(num_n, 1), "bound": "mfma", "work_per_test": 2.0 * N},
+                "k_wald_lr": {"ms": wald_ms / n_launch, "bound": "mfma", "work_per_test": 2.0 * lr_nodes * (p + 1)},
+                "k_lr_tests": {"ms": tests_ms / max(tests_n, 1), "bound": "hbm", "work_per_test": 12.0}}
+    else:
+        kern = {"k_wald_lmm": {"ms": wald_ms / n_launch, "bound": "mfma", "work_per_test": dense_flop}}
+    for v in kern.values():
+        mfma = v["bound"] == "mfma"
+        v["unit"] = "TFLOP/s" if mfma else "GB/s"
+        v["achieved"] = v["work_per_test"] * tests_per_launch / (v["ms"] * 1e-3) / (1e12 if mfma else 1e9)
+        v["peak"] = BF16_PEAK_TFLOPS if mfma else HBM_PEAK_GBS
+        v["frac"] = v["achieved"] / v["peak"]
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    all_ms = sum(v["ms"] for v in kern.values())
+    dense_tflops = dense_flop * tests_per_launch / (all_ms * 1e-3) / 1e12
+    extra = {"wald_plan": {"super_groups": lr_sg, "segments": lr_seg, "mean_nodes": lr_nodes,
                            "dense_flop_per_test": dense_flop, "tests_per_launch": tests_per_launch,
-                           "launch_ms": wald_launch_ms,
-                           "tests_kernel_ms": tests_ms / max(tests_n, 1) if lr_on else None}}
+                           "kernels": kern}}
     # plain FP32 form on a slice of the same data (REF delta search, as the reference)
     m32 = min(m, 131072)
     ctx.set_timing(True)
@@ -1073,17 +1088,17 @@ def main():
         "config": {"workload": "trans-LMM n=2000 samples x 500k SNPs x 20k genes, p=21, 1% missing",
                    "snps_per_step": m, "genes": G, "samples": N, "covariates": p, "delta": "batched Newton",
                    "snps_per_block": BLOCK},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / BF16_PEAK_TFLOPS,
-                     "traffic": measured_traffic(BLOCK) if lr_on else None,
-                     "note": ("k_wald_lr, the dominant kernel: algorithmic 2n + 2R(p+1) FLOP/test (num over the "
-                              "samples, p projections and gg interpolated from R nodes) over its event-timed "
-                              "launches; it issues 3 bf16 MFMA products per num MAC and 6 per interpolation MAC; "
-                              "the FP64 tests (t, tail, ACAT, hits) run after it in k_lr_tests "
-                              f"({tests_ms / max(tests_n, 1):.1f} ms per launch); dense-equivalent 2n(p+2) "
-                              "FLOP/test over both kernels: "
-                              f"{dense_flop * tests_per_launch / ((wald_ms + tests_ms) / max(wald_n, 1) * 1e-3) / 1e12:.0f}"
-                              " TFLOP/s")
+        "roofline": {"kernel": dom, "bound": kern[dom]["bound"], "achieved": kern[dom]["achieved"],
+                     "peak": kern[dom]["peak"], "unit": kern[dom]["unit"], "frac": kern[dom]["frac"],
+                     "traffic": measured_traffic(BLOCK, dom),
+                     "note": (f"{dom}, the longest kernel per block; per-test algorithmic work: k_lr_num 2n FLOP "
+                              f"({kern['k_lr_num']['frac']:.3f} of bf16 dense peak), k_wald_lr 2R(p+1) FLOP "
+                              f"({kern['k_wald_lr']['frac']:.3f}), k_lr_tests 12 B read (gg, S, num; "
+                              f"{kern['k_lr_tests']['frac']:.3f} of HBM peak: its FP64 tail, ACAT term and "
+                              "near-null refinement bound it by FP64 instruction issue, not by HBM); the "
+                              "contraction kernels issue 3 bf16 MFMA products per num MAC and 6 per interpolation "
+                              f"MAC; dense-equivalent 2n(p+2) FLOP/test over the three kernels: {dense_tflops:.0f} "
+                              "TFLOP/s")
                      if lr_on else "algorithmic 2n(p+2) FLOP/test over the fused Wald kernel's event-timed launches; "
                                    "the split form issues 3 bf16 MFMA products per algorithmic MAC"},
         "cpu_baseline": cpu,

@@ -278,9 +278,10 @@ int eqtl_gpu_beta_approx(eqtl_gpu_ctx* ctx, const double* perm_min_p, const doub
 
 /* ---- instrumentation ------------------------------------------------------- */
 /* When enabled, the library brackets each launch of the named kernel family
- * ("wald", "tests", "rotate", "decode", "reml", "vcf") with HIP events on its
- * stream; "tests" is the FP64 test kernel of the interpolated trans LMM form,
- * whose contraction kernel counts as "wald". */
+ * ("wald", "num", "tests", "rotate", "decode", "reml", "vcf") with HIP events
+ * on its stream. In the interpolated trans LMM form "num" is the contraction
+ * over the samples, "wald" the interpolation and "tests" the FP64 test kernel;
+ * otherwise "wald" is the fused Wald kernel. */
 int eqtl_gpu_set_timing(eqtl_gpu_ctx* ctx, int32_t enable);
 int eqtl_gpu_get_timing(eqtl_gpu_ctx* ctx, const char* family, double* total_ms, int64_t* launches);
 /* Plan of the trans LMM scan prepared by eqtl_gpu_scan_begin (no counterpart

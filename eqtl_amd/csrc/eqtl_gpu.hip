@@ -149,6 +149,7 @@ struct TimingSlot {
 struct RemlWork {
   DBuf P, dgrid, Dg, logvg, XD, Y2, CAg, Ainv, logdet, okg, Bg, C0, F, best;
   DBuf W, Z, CA, CB, S, ek, ed, Fo;
+  DBuf Csplit;  // dgemm_tn_split's partial products
   // device-resident reference search: per-gene state, evaluation points and
   // values, and the captured launch sequence with the arguments it was
   // captured for (replayed while they match)
@@ -776,8 +777,35 @@ static int dgemm_tn(eqtl_gpu_ctx* c, int m, int nn, int k, const double* A, int 
              int ldc, const int* nn_dev = nullptr) {
   if (m <= 0 || nn <= 0) return EQTL_OK;
   const dim3 grid(static_cast<unsigned>((m + kDgT - 1) / kDgT), static_cast<unsigned>((nn + kDgT - 1) / kDgT));
-  hipLaunchKernelGGL(k_dgemm_tn, grid, dim3(256), 0, c->stream, m, nn, k, A, lda, B, ldb, C, ldc, nn_dev);
+  hipLaunchKernelGGL(k_dgemm_tn, grid, dim3(256), 0, c->stream, m, nn, k, A, lda, B, ldb, C, ldc, nn_dev, k,
+                     static_cast<int64_t>(0));
   return check_launch("k_dgemm_tn");
+}
+
+// The same product for shapes with few output tiles (the late Newton rounds
+// of the REML search, a few hundred genes): the samples are split over up to
+// 16 workgroups per tile (at most 512 workgroups in all), each writing its
+// own partial product to the workspace, then summed in split order (the
+// result does not depend on scheduling). Not for captured sequences (the
+// workspace may grow).
+static int dgemm_tn_split(eqtl_gpu_ctx* c, int m, int nn, int k, const double* A, int lda, const double* B, int ldb,
+                          double* C, int ldc) {
+  if (m <= 0 || nn <= 0) return EQTL_OK;
+  const int gx = (m + kDgT - 1) / kDgT, gy = (nn + kDgT - 1) / kDgT;
+  const int tiles = gx * gy;
+  int S = std::min(16, std::min(512 / std::max(tiles, 1), (k + 127) / 128));
+  if (tiles >= 256 || S < 2) return dgemm_tn(c, m, nn, k, A, lda, B, ldb, C, ldc);
+  const int kchunk = (((k + S - 1) / S) + 31) / 32 * 32;
+  S = (k + kchunk - 1) / kchunk;
+  const int64_t mn = static_cast<int64_t>(m) * nn;
+  CHK(ensure(c->rw.Csplit, sizeof(double) * static_cast<size_t>(S) * mn));
+  double* P = c->rw.Csplit.as<double>();
+  hipLaunchKernelGGL(k_dgemm_tn, dim3(gx, gy, S), dim3(256), 0, c->stream, m, nn, k, A, lda, B, ldb, P, m,
+                     static_cast<const int*>(nullptr), kchunk, mn);
+  CHK(check_launch("k_dgemm_tn"));
+  const int blocks = static_cast<int>(std::min<int64_t>((mn + 255) / 256, 1024));
+  hipLaunchKernelGGL(k_dgemm_sum_splits, dim3(blocks), dim3(256), 0, c->stream, P, S, m, nn, C, ldc);
+  return check_launch("k_dgemm_sum_splits");
 }
 
 // Blocks of 16 packed entries (MB) and of 16 covariates (PB) that
@@ -900,8 +928,8 @@ static int reml_eval(eqtl_gpu_ctx* c, const std::vector<int32_t>& ek, const std:
                          c->Yt64.as<double>(), n, Ec, r.ek.as<int32_t>(), r.ed.as<double>(), r.W.as<double>(),
                          r.Z.as<double>(), r.S.as<double>());
       CHK(check_launch("k_reml_eval_weights_fgh"));
-      CHK(dgemm_tn(c, TA, nw * Ec, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA));
-      CHK(dgemm_tn(c, p, nw * Ec, n, c->Xt64.as<double>(), n, r.Z.as<double>(), n, r.CB.as<double>(), p));
+      CHK(dgemm_tn_split(c, TA, nw * Ec, n, r.P.as<double>(), n, r.W.as<double>(), n, r.CA.as<double>(), TA));
+      CHK(dgemm_tn_split(c, p, nw * Ec, n, c->Xt64.as<double>(), n, r.Z.as<double>(), n, r.CB.as<double>(), p));
       hipLaunchKernelGGL(k_reml_eval_fgh, dim3(Ec), dim3(64), 0, c->stream, r.CA.as<double>(), r.CB.as<double>(),
                          r.S.as<double>(), Ec, TA, p, df, Fo, Go, Ho);
       CHK(check_launch("k_reml_eval_fgh"));

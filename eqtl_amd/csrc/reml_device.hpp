@@ -34,11 +34,21 @@ __device__ inline f64x8 dg_frag(const double* col, bool in, int k0, int K) {
   return v;
 }
 
+// Split over k when gridDim.z > 1: workgroup z sums samples [z kchunk,
+// (z + 1) kchunk) into its own output C + z cstride (k_dgemm_sum_splits adds
+// them up).
 __global__ __launch_bounds__(256) void k_dgemm_tn(int M, int N_arg, int K, const double* __restrict__ A, int lda,
                                                   const double* __restrict__ B, int ldb, double* __restrict__ C,
-                                                  int ldc, const int* N_dev) {
+                                                  int ldc, const int* N_dev, int kchunk, int64_t cstride) {
   const int N = N_dev ? *N_dev : N_arg;  // columns counted on the device (N_arg: the grid's bound)
   if (static_cast<int>(blockIdx.y) * kDgT >= N) return;
+  {
+    const int kb = static_cast<int>(blockIdx.z) * kchunk;
+    A += kb;
+    B += kb;
+    K = min(K - kb, kchunk);
+    C += static_cast<int64_t>(blockIdx.z) * cstride;
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int m0 = blockIdx.x * kDgT + wr * 32, n0 = blockIdx.y * kDgT + wc * 32;
@@ -95,6 +105,19 @@ __global__ __launch_bounds__(256) void k_dgemm_tn(int M, int N_arg, int K, const
         if (m < M) C[static_cast<size_t>(j) * ldc + m] = acc[a][b][r];
       }
     }
+}
+
+// C (M x N, ldc) = sum over the S split outputs P + z M N (leading dimension M).
+__global__ void k_dgemm_sum_splits(const double* __restrict__ P, int S, int M, int N, double* __restrict__ C,
+                                   int ldc) {
+  const int64_t MN = static_cast<int64_t>(M) * N;
+  for (int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; t < MN;
+       t += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    double v = 0.0;
+    for (int z = 0; z < S; ++z) v += P[z * MN + t];
+    const int64_t j = t / M, m = t - j * M;
+    C[j * ldc + m] = v;
+  }
 }
 
 // ---- fused f-only evaluation inputs on matrix cores ----------------------------

@@ -284,22 +284,25 @@ constexpr int kTailHead = 8 + kTailA * kTailDeg;
 
 // p at x = df / (df + t^2) and y = 1 - x (y >= 0).
 // head: df, half_df, u_sw, inv_h_a, w_lo, w_hi, inv_hw_b, x_zero, coef_a[];
-// mid: u_m, inv_h_m, coef_m[]; coef_b: piece B.
-EQ_HD double p_tail_xy(const double* head, const double* mid, const double* coef_b, double x, double y) {
+// mid: u_m, inv_h_m, coef_m[]; coef_b: piece B. sa / sm: doubles from one
+// interval's coefficients of piece A / M to the next (kTailDeg in TTail; a
+// padded copy in LDS spreads the intervals over the banks).
+EQ_HD double p_tail_xy(const double* head, const double* mid, const double* coef_b, double x, double y,
+                       int sa = kTailDeg, int sm = kTailDeg) {
   const double u = sqrt(y);
   if (u <= head[2]) {
     const double q = u * head[3];
     int i = static_cast<int>(q);
     if (i >= kTailA) i = kTailA - 1;
     const double z = 2.0 * (q - i) - 1.0;
-    return fmin(1.0, fmax(0.0, 1.0 - u * tail_poly(head + 8 + i * kTailDeg, z)));
+    return fmin(1.0, fmax(0.0, 1.0 - u * tail_poly(head + 8 + i * sa, z)));
   }
   if (u <= mid[0]) {
     const double q = (u - head[2]) * mid[1];
     int i = static_cast<int>(q);
     if (i >= kTailM) i = kTailM - 1;
     const double z = 2.0 * (q - i) - 1.0;
-    return fmin(1.0, fmax(0.0, exp(tail_poly(mid + 2 + i * kTailDeg, z))));
+    return fmin(1.0, fmax(0.0, exp(tail_poly(mid + 2 + i * sm, z))));
   }
   if (x <= head[7]) return 0.0;
   const double w = log(-head[1] * log(x));
@@ -312,11 +315,12 @@ EQ_HD double p_tail_xy(const double* head, const double* mid, const double* coef
 }
 
 // p from t, with x formed as the reference forms it (util.cpp:259-264).
-EQ_HD double p_from_t_tab2(const double* head, const double* mid, const double* coef_b, double t) {
+EQ_HD double p_from_t_tab2(const double* head, const double* mid, const double* coef_b, double t,
+                           int sa = kTailDeg, int sm = kTailDeg) {
   const double df = head[0];
   if (!isfinite(t) || df <= 0) return 1.0;
   const double x = df / (df + t * t);
-  return p_tail_xy(head, mid, coef_b, x, 1.0 - x);
+  return p_tail_xy(head, mid, coef_b, x, 1.0 - x, sa, sm);
 }
 
 EQ_HD double p_from_t_tab(const TTail& T, double t) { return p_from_t_tab2(&T.df, &T.u_m, T.coef_b, t); }

@@ -465,8 +465,10 @@ struct RowInfo {
 
 struct TailView {
   const double* head;    // LDS copy of the header + piece A
-  const double* mid;     // piece M (global)
+  const double* mid;     // piece M (global, or an LDS copy)
   const double* coef_b;  // piece B (global)
+  int sa = kTailDeg;     // coefficient row strides of pieces A and M (p_tail_xy)
+  int sm = kTailDeg;
 };
 
 // FP64 numerator refinement at both ends of the t range.
@@ -581,7 +583,7 @@ __device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, i
       const double th2 = a.tau_hi * a.tau_hi;
       if (df > 0 && sq > 0) {
         const double y = n2 / sgs;
-        p = isfinite(y) ? p_tail_xy(tv.head, tv.mid, tv.coef_b, 1.0 - y, y) : 1.0;
+        p = isfinite(y) ? p_tail_xy(tv.head, tv.mid, tv.coef_b, 1.0 - y, y, tv.sa, tv.sm) : 1.0;
         big_t = n2 * df > th2 * sq;
       } else {
         big_t = !(df > 0) && n2 > th2 * S;
@@ -597,7 +599,7 @@ __device__ inline void wald_test(const WaldArgs& a, const RowInfo& ri, int jl, i
       const double s2 = fmax(rss / df, 0.0);
       se = sqrt(s2 / gtg);
       stat = se > 0 ? beta / se : 0.0;
-      p = p_from_t_tab2(tv.head, tv.mid, tv.coef_b, stat);
+      p = p_from_t_tab2(tv.head, tv.mid, tv.coef_b, stat, tv.sa, tv.sm);
       r2 = gscal > 0 ? 1.0 - rss / gscal : 0.0;
     } else if (!(gtg < 1e-12)) {
       beta = gty / gtg;  // df <= 0: p = 1 with beta set (model_lm.cpp:44-48)

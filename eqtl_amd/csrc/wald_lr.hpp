@@ -28,8 +28,9 @@
 //    against the four groups of super-group c of the pair. Per 32-sample
 //    k-step the pair's gene tiles (2 x 8 KiB, s_k o r_k split bf16 hi / lo)
 //    and the tile's rows (hi / lo planes written by the rotation, 16 KiB) are
-//    copied global -> LDS into four-stage rings (four direct copies per wave
-//    and k-step), three bf16 products per MAC; num is stored to the fold.
+//    copied global -> LDS into two-stage rings (four direct copies per wave
+//    and k-step; two workgroups per CU), three bf16 products per MAC; num is
+//    stored to the fold.
 //    With 128 genes per workgroup a k-step moves 32 KiB for 192 MFMAs (one
 //    super-group: 24 KiB for 96).
 //  * k_wald_lr: the interpolation. One workgroup = one super-group x the
@@ -75,8 +76,11 @@ __host__ __device__ inline size_t lr_fold_base(int tile, int sg, int n_sg) {
 }
 
 // k_lr_num: both copy rings (the pair's gene tiles, the row planes) hold
-// kLrStages k-steps and are filled kLrStages - 1 k-steps ahead.
-constexpr int kLrStages = 4;
+// kLrStages k-steps and are filled kLrStages - 1 k-steps ahead. Two stages
+// (64 KiB) let two workgroups share a CU (at most 128 registers per lane),
+// so one workgroup's k-step barrier is covered by the other's MFMAs (four
+// stages and one workgroup per CU: 13% slower).
+constexpr int kLrStages = 2;
 constexpr size_t kLrTileBytes = 8 * 1024;  // one k-step of a super-group's gene tile
 constexpr size_t kLrPairTileBytes = 2 * kLrTileBytes;
 constexpr size_t kLrNumLdsBytes = kLrStages * (kLrPairTileBytes + kAStageBytes);
@@ -273,7 +277,7 @@ __device__ __attribute__((always_inline)) inline void lr_mix(f32x4 (&pj)[2][kRB]
 
 // num of every test of a pair of super-groups: the contraction over the
 // samples, stored to the fold's num plane (fp32).
-__global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_lr_num(LrArgs la) {
+__global__ __launch_bounds__(kW8Threads) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_lr_num(LrArgs la) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   const WaldArgs& a = la.w;
   // XCD-aware: workgroup wg runs on XCD wg % 8. Per round each XCD takes

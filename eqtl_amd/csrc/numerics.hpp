@@ -282,6 +282,14 @@ EQ_HD double tail_poly(const double* c, double z) {
 // piece B (larger |t|) from global memory.
 constexpr int kTailHead = 8 + kTailA * kTailDeg;
 
+// LDS copy of the header and pieces A and M for kernels that evaluate many
+// tails: each interval's coefficients kTailLdsStride doubles apart (one more
+// than the coefficients, so the lanes' data-dependent intervals fall on
+// different LDS banks). head: kTailLdsHead doubles, mid: kTailLdsMid.
+constexpr int kTailLdsStride = kTailDeg + 1;
+constexpr int kTailLdsHead = 8 + kTailA * kTailLdsStride;
+constexpr int kTailLdsMid = 2 + kTailM * kTailLdsStride;
+
 // p at x = df / (df + t^2) and y = 1 - x (y >= 0).
 // head: df, half_df, u_sw, inv_h_a, w_lo, w_hi, inv_hw_b, x_zero, coef_a[];
 // mid: u_m, inv_h_m, coef_m[]; coef_b: piece B. sa / sm: doubles from one
@@ -324,6 +332,17 @@ EQ_HD double p_from_t_tab2(const double* head, const double* mid, const double* 
 }
 
 EQ_HD double p_from_t_tab(const TTail& T, double t) { return p_from_t_tab2(&T.df, &T.u_m, T.coef_b, t); }
+
+// Fills the padded LDS copy (head, mid) with the nthreads threads of a
+// workgroup (thread tid); the caller synchronises before use.
+EQ_HD void tail_to_lds(const TTail& T, double* head, double* mid, int tid, int nthreads) {
+  for (int i = tid; i < 8; i += nthreads) head[i] = (&T.df)[i];
+  for (int i = tid; i < kTailA * kTailDeg; i += nthreads)
+    head[8 + (i / kTailDeg) * kTailLdsStride + i % kTailDeg] = T.coef_a[i];
+  for (int i = tid; i < 2; i += nthreads) mid[i] = (&T.u_m)[i];
+  for (int i = tid; i < kTailM * kTailDeg; i += nthreads)
+    mid[2 + (i / kTailDeg) * kTailLdsStride + i % kTailDeg] = T.coef_m[i];
+}
 
 // Host: fit one interval (Chebyshev interpolation, then monomial in z).
 inline void tail_fit_interval(double (*f)(double, const void*), const void* ctx, double lo, double hi,

@@ -656,22 +656,19 @@ __device__ inline void lds_direct16(const void* gsrc, unsigned char* lds_wave_ba
 // LDS layout of the LM Wald kernel. Region 0 holds, during the contraction,
 // the two k-step buffers of the gene tile (2 KiB each) and a ring of
 // kLmStages SNP-row stages; then gty of the tile's tests during the epilogue; then the
-// per-gene partials of the final reduction. RowInfo and the t-tail head
-// follow it.
+// per-gene partials of the final reduction. RowInfo and the t tail's padded
+// LDS copy (tail_to_lds) follow it.
 constexpr int kFoldDoubles = kRB * 4 * 256;  // one gty per test of the tile
 constexpr int kAStageBytes = kTileRows * 32 * 4;  // 128 rows x 32 samples fp32
 constexpr int kLmStages = 2;
 constexpr long kLdsCap = 160 * 1024;
-__host__ __device__ constexpr long wald_tail_bytes() {
-  return static_cast<long>(sizeof(RowInfo)) + static_cast<long>(sizeof(double)) * kTailHead;
-}
 __host__ __device__ constexpr size_t lm_region0_bytes() {
   const size_t b = 2 * 2048 + static_cast<size_t>(kLmStages) * kAStageBytes;
   const size_t f = sizeof(double) * kFoldDoubles, r = 256 * sizeof(GeneAcc);
   return b > f ? (b > r ? b : r) : (f > r ? f : r);
 }
 __host__ __device__ constexpr size_t lm_lds_bytes() {
-  return lm_region0_bytes() + static_cast<size_t>(wald_tail_bytes());
+  return lm_region0_bytes() + sizeof(RowInfo) + sizeof(double) * (kTailLdsHead + kTailLdsMid);
 }
 static_assert(lm_lds_bytes() <= kLdsCap, "LM Wald LDS layout");
 
@@ -761,9 +758,10 @@ __device__ __attribute__((always_inline)) inline void lm_item(const WaldArgs& a,
   const size_t r0b = lm_region0_bytes();
   RowInfo& ri = *reinterpret_cast<RowInfo*>(smem + r0b);
   double* thead = reinterpret_cast<double*>(smem + r0b + sizeof(RowInfo));
+  double* tmid = thead + kTailLdsHead;
   double* fold = reinterpret_cast<double*>(smem);
-  for (int i = threadIdx.x; i < kTailHead; i += 256) thead[i] = (&a.tail->df)[i];
-  const TailView tv{thead, &a.tail->u_m, a.tail->coef_b};
+  tail_to_lds(*a.tail, thead, tmid, threadIdx.x, 256);
+  const TailView tv{thead, tmid, a.tail->coef_b, kTailLdsStride, kTailLdsStride};
   const unsigned char* gsrc =
       PREC == EQTL_PREC_FP32 ? reinterpret_cast<const unsigned char*>(a.gc.f32 + a.gc.tile_f32(group, 0))
                              : reinterpret_cast<const unsigned char*>(a.gc.bf + a.gc.tile_bf(group, 0));

@@ -106,14 +106,9 @@ __device__ __attribute__((always_inline)) inline void lr_wait3(int ahead) {
 }
 static_assert(kLrLdsBytes + 64 <= kLdsCap, "LR interpolation LDS layout");
 static_assert(kLrNumLdsBytes + 64 <= kLdsCap, "LR num LDS layout");
-// the tests kernel: RowInfo, the t tail's header and pieces A and M with
-// their intervals kLrTailStride doubles apart (one more than the coefficients:
-// the lanes' data-dependent intervals then fall on different LDS banks), the
+// the tests kernel: RowInfo, the t tail's padded LDS copy (tail_to_lds), the
 // 64 genes' states per wave, the genes' data
-constexpr int kLrTailStride = kTailDeg + 1;
-constexpr int kLrTailHead = 8 + kTailA * kLrTailStride;
-constexpr int kLrTailMid = 2 + kTailM * kLrTailStride;
-constexpr size_t kLrTestsLdsBytes = sizeof(RowInfo) + sizeof(double) * (kLrTailHead + kLrTailMid) +
+constexpr size_t kLrTestsLdsBytes = sizeof(RowInfo) + sizeof(double) * (kTailLdsHead + kTailLdsMid) +
                                     sizeof(GeneAcc) * 64 * kW8Waves + sizeof(W8Gene) * 64;
 static_assert(kLrTestsLdsBytes + 64 <= kLdsCap, "LR tests LDS layout");
 
@@ -576,24 +571,16 @@ __global__ __launch_bounds__(kW8Threads) void k_lr_tests(LrArgs la) {
   if (sg >= la.n_sg) return;
   RowInfo& ri = *reinterpret_cast<RowInfo*>(smem);
   double* thead = reinterpret_cast<double*>(smem + sizeof(RowInfo));
-  double* tmid = thead + kLrTailHead;
+  double* tmid = thead + kTailLdsHead;
   // per wave the running states of the super-group's 64 genes: a wave adds
   // its own totals without waiting for the others; merged at the end
-  GeneAcc* wstate = reinterpret_cast<GeneAcc*>(tmid + kLrTailMid);
+  GeneAcc* wstate = reinterpret_cast<GeneAcc*>(tmid + kTailLdsMid);
   W8Gene* ginfo = reinterpret_cast<W8Gene*>(wstate + 64 * kW8Waves);  // the super-group's 64 genes
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  {
-    const TTail& T = *a.tail;
-    for (int i = threadIdx.x; i < 8; i += kW8Threads) thead[i] = (&T.df)[i];
-    for (int i = threadIdx.x; i < kTailA * kTailDeg; i += kW8Threads)
-      thead[8 + (i / kTailDeg) * kLrTailStride + i % kTailDeg] = T.coef_a[i];
-    if (threadIdx.x < 2) tmid[threadIdx.x] = (&T.u_m)[threadIdx.x];
-    for (int i = threadIdx.x; i < kTailM * kTailDeg; i += kW8Threads)
-      tmid[2 + (i / kTailDeg) * kLrTailStride + i % kTailDeg] = T.coef_m[i];
-  }
+  tail_to_lds(*a.tail, thead, tmid, threadIdx.x, kW8Threads);
   acc_init(wstate[threadIdx.x]);
   if (threadIdx.x < 64) ginfo[threadIdx.x] = lr_gene(la, sg * 4 + (threadIdx.x >> 4));
-  const TailView tv{thead, tmid, a.tail->coef_b, kLrTailStride, kLrTailStride};
+  const TailView tv{thead, tmid, a.tail->coef_b, kTailLdsStride, kTailLdsStride};
   const int n_tiles = (a.n_tiles + 1) >> 1;
   // the SNP data of the next tile, loaded while this tile's tests run
   uint8_t r_keep = 0;

@@ -247,7 +247,7 @@ struct eqtl_gpu_ctx {
   int64_t grm_ex_n = 0;
   size_t grm_buf_bytes = size_t(2) << 30, grm_chunk_bytes = size_t(2) << 30;
   // trans LMM, split form, covariate projections interpolated in ln delta
-  // (k_wald_lr): built by scan_begin; slots hold the genes in ln-delta order
+  // (k_lr_num, k_wald_lr, k_lr_tests): built by scan_begin; slots hold the genes in ln-delta order
   bool lr_on = false;
   int lr_n_sg = 0, lr_ncols = 0, lr_nnodes = 0, lr_nseg = 0;
   double lr_mean_nodes = 0.0;  // interpolation nodes per valid gene, averaged

@@ -1,5 +1,5 @@
 """Trans LMM scan in its interpolated form (covariate projections and gg
-interpolated in ln delta over Chebyshev nodes, k_wald_lr + k_lr_tests)
+interpolated in ln delta over Chebyshev nodes, k_lr_num + k_wald_lr + k_lr_tests)
 against the FP64 oracle (test_lmm_gtil, scan_trans_lmm.cpp:11-40) at given
 deltas chosen so that every node count of the plan runs: segments of 32, 64,
 96 and 128 nodes, several 64-gene super-groups with padding slots, and a
@@ -72,8 +72,9 @@ def _check(ds, hits, st, ref, label):
 
 
 def test_lr_segments_32_64_and_padding(gpu_ctx):
-    """150 genes: three super-groups (the last with 42 padding slots) over
-    segments of 32, 64 and 32 nodes."""
+    """150 genes: three super-groups (the last with 42 padding slots; an odd
+    count, so the num kernel's second pair has an empty half) over segments
+    of 32, 64 and 32 nodes."""
     ds, rows, dos, keep, Q, lam = _case(61, n=160, m=1500, G=150)
     rng = np.random.default_rng(5)
     delta = _deltas(rng, 150, [(64, -9.0, -8.6), (64, -8.5, 3.0), (22, 3.5, 4.0)])

@@ -1328,17 +1328,23 @@ __global__ void k_finalize(const GeneAcc* state, eqtl_gene_state* out, int G) {
 // (a conditioning SNP in LD, genotype PCs) keeps its relative accuracy in S
 // and in the numerator.
 // kResRows SNP rows per wave: each covariate value loaded serves all of them
-// (one row per wave re-read the n x p basis from L2 for every SNP). Per row the
-// sums and FMAs run in the same order as for one row alone.
+// (one row per wave re-read the n x p basis from L2 for every SNP), and the
+// basis is staged in LDS 64 samples at a time for the workgroup's four waves
+// (16 rows per load of the basis from L2). Per row the sums and FMAs run in
+// the same order as for one row alone.
 constexpr int kResRows = 4;
 __global__ __launch_bounds__(256) void k_residualize(double* A64, float* A32, int64_t m_pad, int n, int n_pad,
                                                      const double* Wx, int p, double* gtg) {
   constexpr int kPCov = 32;  // covariate columns at most (intercept included)
   constexpr int kCh = 8;     // columns per pass (independent reductions in flight)
   __shared__ double cfs[4][kResRows][kPCov];  // per wave: the rows' coefficients Wx^T g_c
+  __shared__ double wsh[kPCov][64];           // the basis at 64 samples
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // every wave of the workgroup has rows (m_pad is a multiple of 4 * kResRows
+  // and the grid covers it exactly), so the workgroup barriers below are
+  // reached by all of its waves
   const int64_t j0 = (static_cast<int64_t>(blockIdx.x) * 4 + w) * kResRows;
-  if (j0 >= m_pad || p > kPCov) return;  // m_pad is a multiple of 4 * kResRows
+  if (p > kPCov) return;
   double* g[kResRows];
   float* g32[kResRows];
 #pragma unroll
@@ -1346,25 +1352,39 @@ __global__ __launch_bounds__(256) void k_residualize(double* A64, float* A32, in
     g[r] = A64 + (j0 + r) * n_pad;
     g32[r] = A32 + (j0 + r) * n_pad;
   }
+  // wsh[c][t] = Wx[c][i0 + t] for columns c0 .. c0 + nc - 1 (zero past n)
+  auto stage = [&](int c0, int nc, int i0) {
+    __syncthreads();  // every wave is done with the previous chunk
+    for (int q = threadIdx.x; q < nc * 64; q += 256) {
+      const int c = c0 + (q >> 6), i = i0 + (q & 63);
+      wsh[c][q & 63] = i < n ? Wx[static_cast<size_t>(c) * n + i] : 0.0;
+    }
+    __syncthreads();
+  };
 #pragma unroll
   for (int c0 = 0; c0 < kPCov; c0 += kCh) {
     if (c0 >= p) break;
+    const int nc = min(kCh, p - c0);
     double sm[kResRows][kCh];
 #pragma unroll
     for (int r = 0; r < kResRows; ++r)
 #pragma unroll
       for (int u = 0; u < kCh; ++u) sm[r][u] = 0.0;
-    for (int i = lane; i < n; i += 64) {
-      double gi[kResRows];
+    for (int i0 = 0; i0 < n; i0 += 64) {
+      stage(c0, nc, i0);
+      const int i = i0 + lane;
+      if (i < n) {
+        double gi[kResRows];
 #pragma unroll
-      for (int r = 0; r < kResRows; ++r) gi[r] = g[r][i];
+        for (int r = 0; r < kResRows; ++r) gi[r] = g[r][i];
 #pragma unroll
-      for (int u = 0; u < kCh; ++u)
-        if (c0 + u < p) {
-          const double wv = Wx[static_cast<size_t>(c0 + u) * n + i];
+        for (int u = 0; u < kCh; ++u)
+          if (c0 + u < p) {
+            const double wv = wsh[c0 + u][lane];
 #pragma unroll
-          for (int r = 0; r < kResRows; ++r) sm[r][u] = fma(wv, gi[r], sm[r][u]);
-        }
+            for (int r = 0; r < kResRows; ++r) sm[r][u] = fma(wv, gi[r], sm[r][u]);
+          }
+      }
     }
 #pragma unroll
     for (int r = 0; r < kResRows; ++r)
@@ -1375,24 +1395,27 @@ __global__ __launch_bounds__(256) void k_residualize(double* A64, float* A32, in
         if (lane == 0 && c0 + u < kPCov) cfs[w][r][c0 + u] = sm[r][u];
       }
   }
-  __builtin_amdgcn_wave_barrier();
   double ss[kResRows];
 #pragma unroll
   for (int r = 0; r < kResRows; ++r) ss[r] = 0.0;
-  for (int i = lane; i < n; i += 64) {
-    double v[kResRows];
+  for (int i0 = 0; i0 < n; i0 += 64) {
+    stage(0, p, i0);  // its first barrier also publishes cfs
+    const int i = i0 + lane;
+    if (i < n) {
+      double v[kResRows];
 #pragma unroll
-    for (int r = 0; r < kResRows; ++r) v[r] = g[r][i];
-    for (int c = 0; c < p; ++c) {
-      const double wv = Wx[static_cast<size_t>(c) * n + i];
+      for (int r = 0; r < kResRows; ++r) v[r] = g[r][i];
+      for (int c = 0; c < p; ++c) {
+        const double wv = wsh[c][lane];
 #pragma unroll
-      for (int r = 0; r < kResRows; ++r) v[r] = fma(-wv, cfs[w][r][c], v[r]);
-    }
+        for (int r = 0; r < kResRows; ++r) v[r] = fma(-wv, cfs[w][r][c], v[r]);
+      }
 #pragma unroll
-    for (int r = 0; r < kResRows; ++r) {
-      g[r][i] = v[r];
-      g32[r][i] = static_cast<float>(v[r]);
-      ss[r] = fma(v[r], v[r], ss[r]);
+      for (int r = 0; r < kResRows; ++r) {
+        g[r][i] = v[r];
+        g32[r][i] = static_cast<float>(v[r]);
+        ss[r] = fma(v[r], v[r], ss[r]);
+      }
     }
   }
 #pragma unroll
